@@ -1,0 +1,159 @@
+/*
+ * mprgp_amd.h -- C ABI of the MI355X-native MPRGP contact-QP solver.
+ *
+ * Solves  min_x 1/2 x'Ax - b'x  s.t.  n_i . x_k(i) >= c_i  (one half-space row
+ * per node at most), the decoupled contact QP of the reference simulator, on
+ * one AMD Instinct MI355X (gfx950).  Plain pointers and sizes only; every
+ * function returns an mprgp status code (0 = success) unless noted.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   MPRGPDecoupledCon<T>::solve          src/MPRGPSolver/MPRGPSolver.h:538-590
+ *   MPRGPMonotonic / MPRGP solve loops   src/MPRGPSolver/MPRGPSolver.h:345-456
+ *   FixedSparseMatrix                    src/MPRGPSolver/MPRGPSolver.h:459-500
+ *   DecoupledConProjector                src/MPRGPSolver/MPRGPProjection.h:320-397
+ *   DiagonalDecouplePreconSolver         src/MPRGPSolver/MPRGPPrecondition.h:72-110
+ *   MPRGPBase::specRad (power_ev)        src/MPRGPSolver/MPRGPSolver.h:42-79
+ *   MprgpFemSolver::buildLinearSystem    src/CollisionHandle/MprgpFemSolver.cpp:315-356
+ *   FEMSystem::buildSystem               src/ElasticSolver/FEMSystem.cpp:167-192
+ *   MaterialEnergy::eval (LINEAR/COROT)  src/ElasticSolver/FEMEnergy.cpp:263-304, 657-1057
+ *
+ * Threading: a context owns one HIP stream; calls on one context must not
+ * overlap.  Different contexts are independent.  No host-global mutable state
+ * besides a thread-local last-error string.
+ */
+#ifndef MPRGP_AMD_H
+#define MPRGP_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (the reference returns only 0 / -1) */
+#define MPRGP_OK 0
+#define MPRGP_NOT_CONVERGED (-1) /* reference's -1: ||phi+beta|| > tol after max_it */
+#define MPRGP_EBADINPUT (-2)     /* invalid sizes/indices, coupled rows, zero normal   */
+#define MPRGP_ENOMEM (-3)        /* device allocation failed                           */
+#define MPRGP_EHIP (-4)          /* HIP runtime error                                  */
+#define MPRGP_ENAN (-5)          /* non-finite residual reached                        */
+#define MPRGP_ENODEV (-6)        /* no usable GPU                                      */
+#define MPRGP_EBUDGET (-7)       /* HBM budget of the context exceeded                 */
+#define MPRGP_ESTATE (-8)        /* object not ready (e.g. rhs not set)                */
+
+/* solver variants */
+#define MPRGP_MONOTONIC 0 /* MPRGPMonotonic (MPRGPSolver.h:400-456), the simulator's */
+#define MPRGP_CLASSIC 1   /* MPRGP with max-feasible-step test (MPRGPSolver.h:344-398) */
+
+/* precision forms */
+#define MPRGP_FP64 0  /* every vector and reduction in FP64                        */
+#define MPRGP_MIXED 1 /* FP32 SpMV/updates, FP64 step lengths, KKT norm, refinement */
+
+typedef struct mprgp_ctx mprgp_ctx;
+typedef struct mprgp_matrix mprgp_matrix;
+typedef struct mprgp_qp mprgp_qp;
+
+typedef struct {
+  double tol;          /* bound on ||phi + beta||_2, absolute (reference tol)        */
+  int max_it;          /* reference max_it                                           */
+  int variant;         /* MPRGP_MONOTONIC (default) or MPRGP_CLASSIC                 */
+  int precision;       /* MPRGP_FP64 (default) or MPRGP_MIXED                        */
+  int preconditioned;  /* 1: tangent-Jacobi (reference `true`), 0: none              */
+  int project_first;   /* 1: x = P(x) before solving (the J,c overload, :561-574)    */
+  int use_ev;          /* 1: warm-start the power method from the QP's stored vector */
+  double alpha_bar;    /* > 0: used as is; <= 0: 2/specRad(A, ev, power_tol)         */
+  double power_tol;    /* power-method tolerance; <= 0 means tol, as the reference   */
+  int tiles;           /* SELL slices per workgroup (1,2,4,8); 0 = autotuned         */
+  int graph_steps;     /* launches per captured hipGraph; 0 = autotuned              */
+  int max_refine;      /* MIXED: outer refinement sweeps (0 = default 8)             */
+  int reserved_i[5];
+  double inner_tol;    /* MIXED: inner tolerance factor (0 = default 1e-3 relative)  */
+  double reserved_d[3];
+} mprgp_params;
+
+typedef struct {
+  int code;           /* 0 converged, -1 not converged (reference result_code)      */
+  int iterations;     /* reference `iteration` ("total iter")                       */
+  int num_cg, num_exp, num_prop;
+  int active;         /* COUNT_CONSTRAINTS(face) at exit                             */
+  int power_iters;    /* power-method iterations run in this solve                  */
+  int ev_valid;       /* power vector stored for the next solve                     */
+  int refinements;    /* MIXED: outer refinement sweeps                             */
+  int steps;          /* device launches this QP took part in                       */
+  int reserved_i[2];
+  double residual;    /* final ||phi + beta||_2 (FP64)                              */
+  double alpha_bar;
+  double lambda_max;  /* spectral-radius estimate behind alpha_bar                  */
+  double elapsed_ms;  /* device time of the solve call (whole batch)                */
+  double reserved_d[4];
+} mprgp_stats;
+
+/* ---- context ------------------------------------------------------------ */
+void mprgp_params_default(mprgp_params *p);
+const char *mprgp_last_error(void);
+int mprgp_device_count(int *count);
+/* hbm_budget_bytes = 0: no budget beyond the device's memory. */
+int mprgp_ctx_create(int device, size_t hbm_budget_bytes, mprgp_ctx **out);
+void mprgp_ctx_destroy(mprgp_ctx *ctx);
+int mprgp_ctx_memory(mprgp_ctx *ctx, size_t *in_use, size_t *budget);
+
+/* ---- system matrix: FixedSparseMatrix (MPRGPSolver.h:459-500) ------------- */
+/* 3x3 block CSR in node order: rowptr[n_nodes+1], colidx[nb], blocks[9*nb]
+ * row-major.  Every row needs its diagonal block.  The matrix is copied into
+ * device memory (SELL-64 layout in a row-length order) and owned by ctx. */
+int mprgp_matrix_create_bsr3(mprgp_ctx *ctx, int n_nodes, const int *rowptr, const int *colidx,
+                             const double *blocks, mprgp_matrix **out);
+/* Scalar CSR of a symmetric matrix with n = 3*nodes rows.  For a symmetric
+ * Eigen::SparseMatrix<double> (column-major) pass outerIndexPtr(),
+ * innerIndexPtr(), valuePtr(): its CSC arrays are the CSR arrays. */
+int mprgp_matrix_create_csr(mprgp_ctx *ctx, int n, const int *ptr, const int *idx,
+                            const double *val, mprgp_matrix **out);
+/* new values, same pattern and block order as at creation */
+int mprgp_matrix_update_bsr3(mprgp_matrix *A, const double *blocks);
+int mprgp_matrix_info(const mprgp_matrix *A, int *n_nodes, long long *n_blocks, long long *bytes);
+/* y = A x on host vectors (node order), FixedSparseMatrix::multiply (:469-473) */
+int mprgp_matrix_spmv(mprgp_matrix *A, const double *x, double *y);
+void mprgp_matrix_destroy(mprgp_matrix *A);
+
+/* ---- QP: matrix + rows + right-hand side + iterate, device-resident ------- */
+int mprgp_qp_create(mprgp_matrix *A, mprgp_qp **out);
+void mprgp_qp_destroy(mprgp_qp *qp);
+/* DecoupledConProjector(J, JJt, c) (MPRGPProjection.h:330-331) in per-node
+ * form: row i constrains node[i] with normal[3i..3i+2] and offset c[i].
+ * Duplicate nodes, zero normals or bad indices give MPRGP_EBADINPUT. */
+int mprgp_qp_set_rows(mprgp_qp *qp, int m, const int *node, const double *normal, const double *c);
+/* Same from the sparse J the reference builds (LinearConCollider.cpp:206-230):
+ * J is m x 3*nodes in CSR (col_major = 0) or CSC (col_major = 1, Eigen's
+ * default).  Each row must lie inside one node's three columns. */
+int mprgp_qp_set_rows_sparse(mprgp_qp *qp, int m, int col_major, const int *outer, const int *inner,
+                             const double *val, const double *c);
+int mprgp_qp_set_rhs(mprgp_qp *qp, const double *b);
+int mprgp_qp_set_x(mprgp_qp *qp, const double *x);   /* also kept as the reset point */
+int mprgp_qp_reset_x(mprgp_qp *qp);                  /* x = last set_x, on device   */
+int mprgp_qp_get_x(mprgp_qp *qp, double *x);
+int mprgp_qp_set_ev(mprgp_qp *qp, const double *ev); /* power_ev warm start (:47-48)*/
+int mprgp_qp_get_ev(mprgp_qp *qp, double *ev, int *valid);
+int mprgp_qp_get_face(mprgp_qp *qp, unsigned char *face_per_node);
+/* MPRGPDecoupledCon<double>::solve<FixedSparseMatrix, preconditioned>
+ * (MPRGPSolver.h:544-559): x is read and overwritten on the device. */
+int mprgp_qp_solve(mprgp_qp *qp, const mprgp_params *params, mprgp_stats *stats);
+/* Many QPs advanced together (params: one per QP, or NULL-terminated use of
+ * params[0] for all when params_stride == 0).  Returns MPRGP_OK when every QP
+ * converged, MPRGP_NOT_CONVERGED if any did not, an error otherwise. */
+int mprgp_solve_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprgp_params *params,
+                      int params_stride, mprgp_stats *stats);
+
+/* ---- one-shot drop-in ------------------------------------------------------
+ * MPRGPDecoupledCon<double>::solve<FixedSparseMatrix<double>,true>(A, B, J, c,
+ * x, tol, max_it, name, ev) (MPRGPSolver.h:561-574): A as symmetric scalar
+ * CSR/CSC arrays (n x n, n = 3*nodes), J in CSC (Eigen column-major, m x n),
+ * x projected first, ev/ev_valid the caller's power vector (may be NULL). */
+int mprgp_decoupled_solve(int device, int n, const int *a_outer, const int *a_inner,
+                          const double *a_val, const double *B, int m, const int *j_outer,
+                          const int *j_inner, const double *j_val, const double *c, double *x,
+                          double tol, int max_it, double *ev, int *ev_valid, mprgp_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPRGP_AMD_H */

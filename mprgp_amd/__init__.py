@@ -1,0 +1,265 @@
+"""MI355X-native MPRGP contact-QP solver (host side over the C ABI).
+
+Object API (device-resident, mirrors include/mprgp_amd.h)::
+
+    ctx = Context(0)
+    A = ctx.matrix_csr(A_scipy)            # or ctx.matrix_bsr3(n, rowptr, colidx, blocks)
+    qp = A.qp()
+    qp.set_rows(node, normal, c); qp.set_rhs(b); qp.set_x(x0)
+    code, stats = qp.solve(tol=1e-8, max_it=500)
+    x = qp.get_x()
+
+Reference-shaped facade (``MPRGPDecoupledCon<double>::solve``,
+src/MPRGPSolver/MPRGPSolver.h:538-590)::
+
+    code = MPRGPDecoupledCon.solve(A, B, J, c, x, tol, max_it, "MPRGP", ev)
+
+Every call runs the HIP kernels of ``libmprgp_amd.so``; without the built
+library (or without a GPU) the calls raise instead of falling back.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (CLASSIC, COROTATIONAL, FP64, LINEAR, MIXED, MONOTONIC, MprgpError, Params,
+                   Stats, check, last_error, load)
+
+__all__ = ["Context", "Matrix", "QP", "MPRGPDecoupledCon", "solve_batch", "params",
+           "MprgpError", "MONOTONIC", "CLASSIC", "FP64", "MIXED", "LINEAR", "COROTATIONAL",
+           "device_count"]
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = load().mprgp_device_count(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+def params(**kw) -> Params:
+    p = Params()
+    load().mprgp_params_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+class Context:
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self.h = C.c_void_p()
+        check(load().mprgp_ctx_create(int(device), C.c_size_t(hbm_budget), C.byref(self.h)))
+        self._objs = []
+
+    def close(self):
+        if self.h:
+            for o in reversed(self._objs):
+                o.close()
+            self._objs = []
+            load().mprgp_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def memory(self):
+        u, b = C.c_size_t(), C.c_size_t()
+        check(load().mprgp_ctx_memory(self.h, C.byref(u), C.byref(b)))
+        return u.value, b.value
+
+    def matrix_bsr3(self, n_nodes, rowptr, colidx, blocks) -> "Matrix":
+        rowptr, colidx = _i32(rowptr), _i32(colidx)
+        blocks = _f64(blocks)
+        h = C.c_void_p()
+        check(load().mprgp_matrix_create_bsr3(self.h, int(n_nodes), _p(rowptr), _p(colidx),
+                                              _p(blocks), C.byref(h)))
+        m = Matrix(self, h, int(n_nodes))
+        self._objs.append(m)
+        return m
+
+    def matrix_csr(self, A) -> "Matrix":
+        import scipy.sparse as sp
+        A = sp.csr_matrix(A)
+        A.sort_indices()
+        ptr, idx, val = _i32(A.indptr), _i32(A.indices), _f64(A.data)
+        h = C.c_void_p()
+        check(load().mprgp_matrix_create_csr(self.h, int(A.shape[0]), _p(ptr), _p(idx), _p(val),
+                                             C.byref(h)))
+        m = Matrix(self, h, A.shape[0] // 3)
+        self._objs.append(m)
+        return m
+
+
+class Matrix:
+    def __init__(self, ctx, h, n_nodes):
+        self.ctx, self.h, self.n_nodes = ctx, h, n_nodes
+
+    def close(self):
+        if self.h:
+            load().mprgp_matrix_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def info(self):
+        n, nb, by = C.c_int(), C.c_longlong(), C.c_longlong()
+        check(load().mprgp_matrix_info(self.h, C.byref(n), C.byref(nb), C.byref(by)))
+        return {"n_nodes": n.value, "n_blocks": nb.value, "bytes": by.value}
+
+    def update_bsr3(self, blocks):
+        b = _f64(blocks)
+        check(load().mprgp_matrix_update_bsr3(self.h, _p(b)))
+
+    def spmv(self, x):
+        x = _f64(x, 3 * self.n_nodes)
+        y = np.empty(3 * self.n_nodes)
+        check(load().mprgp_matrix_spmv(self.h, _p(x), _p(y)))
+        return y
+
+    def qp(self) -> "QP":
+        h = C.c_void_p()
+        check(load().mprgp_qp_create(self.h, C.byref(h)))
+        q = QP(self, h)
+        self.ctx._objs.append(q)
+        return q
+
+
+class QP:
+    """Device-resident QP: matrix + half-space rows + b + x (+ power vector)."""
+
+    def __init__(self, A: Matrix, h):
+        self.A, self.h = A, h
+        self.n = 3 * A.n_nodes
+
+    def close(self):
+        if self.h:
+            load().mprgp_qp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def set_rows(self, node, normal, c):
+        node = _i32(node)
+        m = node.size
+        normal = _f64(normal, 3 * m)
+        c = _f64(c, m)
+        check(load().mprgp_qp_set_rows(self.h, int(m), _p(node), _p(normal), _p(c)))
+
+    def set_rows_sparse(self, J, c):
+        """J: scipy sparse m x n (rows inside one node each)."""
+        import scipy.sparse as sp
+        J = sp.csc_matrix(J)
+        J.sort_indices()
+        c = _f64(c, J.shape[0])
+        outer, inner, val = _i32(J.indptr), _i32(J.indices), _f64(J.data)
+        check(load().mprgp_qp_set_rows_sparse(self.h, int(J.shape[0]), 1, _p(outer), _p(inner),
+                                              _p(val), _p(c)))
+
+    def set_rhs(self, b):
+        b = _f64(b, self.n)
+        check(load().mprgp_qp_set_rhs(self.h, _p(b)))
+
+    def set_x(self, x):
+        x = _f64(x, self.n)
+        check(load().mprgp_qp_set_x(self.h, _p(x)))
+
+    def reset_x(self):
+        check(load().mprgp_qp_reset_x(self.h))
+
+    def get_x(self):
+        x = np.empty(self.n)
+        check(load().mprgp_qp_get_x(self.h, _p(x)))
+        return x
+
+    def set_ev(self, ev):
+        if ev is None:
+            check(load().mprgp_qp_set_ev(self.h, None))
+        else:
+            ev = _f64(ev, self.n)
+            check(load().mprgp_qp_set_ev(self.h, _p(ev)))
+
+    def get_ev(self):
+        ev = np.empty(self.n)
+        v = C.c_int()
+        check(load().mprgp_qp_get_ev(self.h, _p(ev), C.byref(v)))
+        return ev if v.value else None
+
+    def get_face(self):
+        f = np.empty(self.n // 3, dtype=np.uint8)
+        check(load().mprgp_qp_get_face(self.h, _p(f)))
+        return f
+
+    def solve(self, p: Params | None = None, **kw):
+        p = params(**kw) if p is None else p
+        st = Stats()
+        rc = load().mprgp_qp_solve(self.h, C.byref(p), C.byref(st))
+        check(rc, allow=(_lib.OK, _lib.NOT_CONVERGED, _lib.ENAN))
+        return rc, st.as_dict()
+
+
+def solve_batch(ctx: Context, qps, p=None, per_qp=None, **kw):
+    """Advances all QPs together; returns (rc, [stats dicts])."""
+    n = len(qps)
+    arr = (C.c_void_p * n)(*[q.h.value for q in qps])
+    if per_qp is not None:
+        ps = (Params * n)(*per_qp)
+        stride = 1
+    else:
+        ps = (Params * 1)(p if p is not None else params(**kw))
+        stride = 0
+    st = (Stats * n)()
+    rc = load().mprgp_solve_batch(ctx.h, n, C.cast(arr, C.POINTER(C.c_void_p)), ps, stride, st)
+    check(rc, allow=(_lib.OK, _lib.NOT_CONVERGED, _lib.ENAN))
+    return rc, [s.as_dict() for s in st]
+
+
+class MPRGPDecoupledCon:
+    """Mirror of ``MPRGPDecoupledCon<double>`` (MPRGPSolver.h:538-590)."""
+
+    @staticmethod
+    def solve(A, B, J, c, x, tol=1e-3, max_it=1000, solver_name="MPRGP", ev=None,
+              device=0, stats_out=None):
+        """J/c overload (MPRGPSolver.h:561-574): projects x first, solves with the
+        tangent-Jacobi preconditioner, writes the solution into ``x`` (in place) and
+        returns the reference's code (0 converged, -1 not).  ``ev`` (a numpy array of
+        length n, or None) is the power-method vector carried across solves."""
+        import scipy.sparse as sp
+        A = sp.csr_matrix(A)
+        A.sort_indices()
+        Jc = sp.csc_matrix(J, shape=(J.shape[0], A.shape[0]))
+        Jc.sort_indices()
+        n = A.shape[0]
+        if not (isinstance(x, np.ndarray) and x.dtype == np.float64 and x.flags.c_contiguous):
+            raise TypeError("x must be a contiguous float64 numpy array (it is overwritten)")
+        B = _f64(B, n)
+        c = _f64(c, Jc.shape[0])
+        ap, ai, av = _i32(A.indptr), _i32(A.indices), _f64(A.data)
+        jp, ji, jv = _i32(Jc.indptr), _i32(Jc.indices), _f64(Jc.data)
+        evb = np.zeros(n) if ev is None else ev
+        valid = C.c_int(0 if ev is None or not np.any(ev) else 1)
+        st = Stats()
+        rc = load().mprgp_decoupled_solve(int(device), n, _p(ap), _p(ai), _p(av), _p(B),
+                                          int(Jc.shape[0]), _p(jp), _p(ji), _p(jv), _p(c), _p(x),
+                                          float(tol), int(max_it), _p(evb), C.byref(valid),
+                                          C.byref(st))
+        check(rc, allow=(_lib.OK, _lib.NOT_CONVERGED, _lib.ENAN))
+        if stats_out is not None:
+            stats_out.update(st.as_dict())
+        return 0 if rc == _lib.OK else -1

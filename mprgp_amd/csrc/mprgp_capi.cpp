@@ -1,0 +1,882 @@
+// mprgp_capi.cpp -- C ABI implementation: contexts, matrices, QPs and the
+// batched device-resident solve loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+
+#include "mprgp_host.h"
+
+namespace mprgp {
+
+static thread_local std::string g_error;
+
+void set_error(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_error = buf;
+}
+
+int fail(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return code;
+}
+
+// splitmix64: deterministic start vector of the power method (replaces the
+// reference's unseeded setRandom()).
+static uint64_t splitmix64(uint64_t &s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+}  // namespace mprgp
+
+using namespace mprgp;
+
+int mprgp_ctx::alloc(void **p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  if (budget && in_use + bytes > budget)
+    return fail(MPRGP_EBUDGET, "HBM budget exceeded: %zu in use + %zu requested > %zu", in_use, bytes,
+                budget);
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(MPRGP_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  in_use += bytes;
+  return MPRGP_OK;
+}
+
+void mprgp_ctx::release(void *p, size_t bytes) {
+  if (!p) return;
+  if (bytes == 0) bytes = 256;
+  (void)hipFree(p);
+  in_use -= std::min(in_use, bytes);
+}
+
+// ------------------------------------------------------------------ context
+
+extern "C" void mprgp_params_default(mprgp_params *p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->tol = 1e-3;  // MPRGPDecoupledCon::solve default
+  p->max_it = 1000;
+  p->variant = MPRGP_MONOTONIC;
+  p->precision = MPRGP_FP64;
+  p->preconditioned = 1;
+  p->project_first = 0;
+  p->use_ev = 1;
+  p->alpha_bar = 0.0;
+  p->power_tol = 0.0;
+}
+
+extern "C" const char *mprgp_last_error(void) { return g_error.c_str(); }
+
+extern "C" int mprgp_device_count(int *count) {
+  if (!count) return fail(MPRGP_EBADINPUT, "count is NULL");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *count = 0;
+    return fail(MPRGP_ENODEV, "hipGetDeviceCount: %s", hipGetErrorString(e));
+  }
+  *count = n;
+  return MPRGP_OK;
+}
+
+extern "C" int mprgp_ctx_create(int device, size_t hbm_budget_bytes, mprgp_ctx **out) {
+  if (!out) return fail(MPRGP_EBADINPUT, "out is NULL");
+  *out = nullptr;
+  int n = 0;
+  if (mprgp_device_count(&n) != MPRGP_OK || n <= 0) return fail(MPRGP_ENODEV, "no HIP device available");
+  if (device < 0 || device >= n) return fail(MPRGP_ENODEV, "device %d out of range (%d devices)", device, n);
+  MPRGP_HIP(hipSetDevice(device));
+  mprgp_ctx *c = new mprgp_ctx();
+  c->device = device;
+  c->budget = hbm_budget_bytes;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_running, 2 * sizeof(int), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipEventCreate(&c->t0);
+  if (e == hipSuccess) e = hipEventCreate(&c->t1);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->q[0], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->q[1], hipEventDisableTiming);
+  if (e != hipSuccess) {
+    mprgp_ctx_destroy(c);
+    return fail(MPRGP_EHIP, "context setup failed: %s", hipGetErrorString(e));
+  }
+  int rc = c->alloc((void **)&c->d_ctrl, 4 * sizeof(int));
+  if (rc) {
+    mprgp_ctx_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return MPRGP_OK;
+}
+
+extern "C" void mprgp_ctx_destroy(mprgp_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto &kv : c->graph_defs) (void)hipGraphDestroy(kv.second);
+  c->release(c->d_qps, sizeof(QpDev) * (size_t)c->qps_cap);
+  c->release(c->d_tiles, sizeof(Tile) * (size_t)c->tiles_cap);
+  c->release(c->d_ctrl, 4 * sizeof(int));
+  if (c->h_running) (void)hipHostFree(c->h_running);
+  if (c->t0) (void)hipEventDestroy(c->t0);
+  if (c->t1) (void)hipEventDestroy(c->t1);
+  if (c->q[0]) (void)hipEventDestroy(c->q[0]);
+  if (c->q[1]) (void)hipEventDestroy(c->q[1]);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+extern "C" int mprgp_ctx_memory(mprgp_ctx *c, size_t *in_use, size_t *budget) {
+  if (!c) return fail(MPRGP_EBADINPUT, "ctx is NULL");
+  if (in_use) *in_use = c->in_use;
+  if (budget) *budget = c->budget;
+  return MPRGP_OK;
+}
+
+// ------------------------------------------------------------------ matrix
+
+// Builds the SELL-64 layout from a node-order block CSR and uploads it.
+static int build_matrix(mprgp_ctx *ctx, int n_nodes, const int *rowptr, const int *colidx,
+                        const double *blocks, mprgp_matrix **out) {
+  if (!ctx || !out || n_nodes <= 0 || !rowptr || !colidx || !blocks)
+    return fail(MPRGP_EBADINPUT, "bad matrix arguments");
+  *out = nullptr;
+  const long long nb = rowptr[n_nodes];
+  if (rowptr[0] != 0 || nb <= 0) return fail(MPRGP_EBADINPUT, "rowptr must start at 0 and be non-empty");
+  std::vector<int> len(n_nodes);
+  for (int i = 0; i < n_nodes; ++i) {
+    if (rowptr[i + 1] < rowptr[i]) return fail(MPRGP_EBADINPUT, "rowptr not monotone at row %d", i);
+    len[i] = rowptr[i + 1] - rowptr[i];
+    bool diag = false;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+      if (colidx[k] < 0 || colidx[k] >= n_nodes)
+        return fail(MPRGP_EBADINPUT, "column index %d out of range in row %d", colidx[k], i);
+      diag |= colidx[k] == i;
+    }
+    if (!diag) return fail(MPRGP_EBADINPUT, "row %d has no diagonal block", i);
+  }
+  for (long long k = 0; k < 9 * nb; ++k)
+    if (!std::isfinite(blocks[k])) return fail(MPRGP_EBADINPUT, "non-finite matrix value at %lld", k);
+  mprgp_matrix *A = new mprgp_matrix();
+  A->ctx = ctx;
+  A->n_nodes = n_nodes;
+  A->nb = nb;
+  A->perm.resize(n_nodes);
+  std::iota(A->perm.begin(), A->perm.end(), 0);
+  // internal order: descending block-row length, stable (keeps mesh locality
+  // within each length class)
+  std::stable_sort(A->perm.begin(), A->perm.end(), [&](int a, int b) { return len[a] > len[b]; });
+  A->iperm.resize(n_nodes);
+  for (int i = 0; i < n_nodes; ++i) A->iperm[A->perm[i]] = i;
+  A->n_slices = (n_nodes + kSlice - 1) / kSlice;
+  A->sl_off_h.resize(A->n_slices + 1);
+  A->sl_off_h[0] = 0;
+  long long slots = 0;
+  for (int s = 0; s < A->n_slices; ++s) {
+    int w = 0;
+    for (int l = 0; l < kSlice; ++l) {
+      int i = s * kSlice + l;
+      if (i < n_nodes) w = std::max(w, len[A->perm[i]]);
+    }
+    slots += (long long)w * kSlice;
+    if (slots > 0x7fffffffLL / 9) {
+      delete A;
+      return fail(MPRGP_EBADINPUT, "matrix too large for 32-bit SELL offsets");
+    }
+    A->sl_off_h[s + 1] = (int)slots;
+  }
+  A->slots = slots;
+  std::vector<int> col(slots, -1);
+  std::vector<double> val((size_t)slots * 9, 0.0);
+  A->blk_slot.resize(nb);
+  for (int s = 0; s < A->n_slices; ++s) {
+    const long long b0 = A->sl_off_h[s];
+    for (int l = 0; l < kSlice; ++l) {
+      int i = s * kSlice + l;
+      if (i >= n_nodes) continue;
+      int o = A->perm[i];
+      for (int k = 0; k < len[o]; ++k) {
+        long long src = rowptr[o] + k;
+        long long slot = b0 + (long long)k * kSlice + l;
+        col[slot] = A->iperm[colidx[src]];
+        A->blk_slot[src] = slot;
+        long long vbase = b0 * 9 + (long long)k * 9 * kSlice + l;
+        for (int e = 0; e < 9; ++e) val[vbase + (long long)e * kSlice] = blocks[9 * src + e];
+      }
+    }
+  }
+  size_t b_off = sizeof(int) * (A->n_slices + 1), b_col = sizeof(int) * slots,
+         b_val = sizeof(double) * slots * 9, b_d = sizeof(double) * 3 * n_nodes, b_p = sizeof(int) * n_nodes;
+  int rc = ctx->alloc((void **)&A->d_sl_off, b_off);
+  if (!rc) rc = ctx->alloc((void **)&A->d_col, b_col);
+  if (!rc) rc = ctx->alloc((void **)&A->d_val, b_val);
+  if (!rc) rc = ctx->alloc((void **)&A->d_dinv, b_d);
+  if (!rc) rc = ctx->alloc((void **)&A->d_perm, b_p);
+  A->bytes = b_off + b_col + b_val + b_d + b_p;
+  if (rc) {
+    mprgp_matrix_destroy(A);
+    return rc;
+  }
+  hipStream_t s = ctx->stream;
+  hipError_t e = hipMemcpyAsync(A->d_sl_off, A->sl_off_h.data(), b_off, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(A->d_col, col.data(), b_col, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(A->d_val, val.data(), b_val, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(A->d_perm, A->perm.data(), b_p, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = launch_setup_diag(s, A->d_sl_off, A->d_col, A->d_val, A->d_dinv, n_nodes);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    mprgp_matrix_destroy(A);
+    return fail(MPRGP_EHIP, "matrix upload failed: %s", hipGetErrorString(e));
+  }
+  *out = A;
+  return MPRGP_OK;
+}
+
+extern "C" int mprgp_matrix_create_bsr3(mprgp_ctx *ctx, int n_nodes, const int *rowptr, const int *colidx,
+                                        const double *blocks, mprgp_matrix **out) {
+  if (!ctx) return fail(MPRGP_EBADINPUT, "ctx is NULL");
+  MPRGP_HIP(hipSetDevice(ctx->device));
+  return build_matrix(ctx, n_nodes, rowptr, colidx, blocks, out);
+}
+
+extern "C" int mprgp_matrix_create_csr(mprgp_ctx *ctx, int n, const int *ptr, const int *idx,
+                                       const double *val, mprgp_matrix **out) {
+  if (!ctx || !ptr || !idx || !val || !out) return fail(MPRGP_EBADINPUT, "NULL argument");
+  if (n <= 0 || n % 3) return fail(MPRGP_EBADINPUT, "n = %d is not a positive multiple of 3", n);
+  const int N = n / 3;
+  std::vector<int> rowptr(N + 1, 0);
+  std::vector<int> colidx;
+  std::vector<double> blocks;
+  std::vector<int> mark(N, -1), slot(N, -1);
+  for (int a = 0; a < N; ++a) {
+    std::vector<int> cols;
+    for (int r = 0; r < 3; ++r) {
+      int row = 3 * a + r;
+      if (ptr[row + 1] < ptr[row]) return fail(MPRGP_EBADINPUT, "ptr not monotone at %d", row);
+      for (int k = ptr[row]; k < ptr[row + 1]; ++k) {
+        if (idx[k] < 0 || idx[k] >= n) return fail(MPRGP_EBADINPUT, "index %d out of range", idx[k]);
+        int bcol = idx[k] / 3;
+        if (mark[bcol] != a) {
+          mark[bcol] = a;
+          cols.push_back(bcol);
+        }
+      }
+    }
+    if (mark[a] != a) cols.push_back(a);  // keep a (zero) diagonal block
+    std::sort(cols.begin(), cols.end());
+    size_t base = colidx.size();
+    for (size_t k = 0; k < cols.size(); ++k) slot[cols[k]] = (int)(base + k);
+    colidx.insert(colidx.end(), cols.begin(), cols.end());
+    blocks.resize(colidx.size() * 9, 0.0);
+    for (int r = 0; r < 3; ++r) {
+      int row = 3 * a + r;
+      for (int k = ptr[row]; k < ptr[row + 1]; ++k)
+        blocks[(size_t)slot[idx[k] / 3] * 9 + r * 3 + idx[k] % 3] += val[k];
+    }
+    rowptr[a + 1] = (int)colidx.size();
+  }
+  MPRGP_HIP(hipSetDevice(ctx->device));
+  return build_matrix(ctx, N, rowptr.data(), colidx.data(), blocks.data(), out);
+}
+
+extern "C" int mprgp_matrix_update_bsr3(mprgp_matrix *A, const double *blocks) {
+  if (!A || !blocks) return fail(MPRGP_EBADINPUT, "NULL argument");
+  mprgp_ctx *ctx = A->ctx;
+  MPRGP_HIP(hipSetDevice(ctx->device));
+  std::vector<double> val((size_t)A->slots * 9, 0.0);
+  for (long long b = 0; b < A->nb; ++b) {
+    long long slot = A->blk_slot[b];
+    // slot = b0 + k*64 + l with b0 a multiple of 64: value base = b0*9 + k*576 + l
+    long long l = slot % kSlice;
+    long long b0k = slot - l;  // b0 + 64k
+    // find slice start b0 <= slot
+    int s = (int)(std::upper_bound(A->sl_off_h.begin(), A->sl_off_h.end(), (int)slot) - A->sl_off_h.begin()) - 1;
+    long long b0 = A->sl_off_h[s];
+    long long k = (b0k - b0) / kSlice;
+    long long vbase = b0 * 9 + k * 9 * kSlice + l;
+    for (int e = 0; e < 9; ++e) {
+      double v = blocks[9 * b + e];
+      if (!std::isfinite(v)) return fail(MPRGP_EBADINPUT, "non-finite matrix value");
+      val[vbase + (long long)e * kSlice] = v;
+    }
+  }
+  hipStream_t s = ctx->stream;
+  MPRGP_HIP(hipMemcpyAsync(A->d_val, val.data(), val.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  MPRGP_HIP(launch_setup_diag(s, A->d_sl_off, A->d_col, A->d_val, A->d_dinv, A->n_nodes));
+  MPRGP_HIP(hipStreamSynchronize(s));
+  A->version++;
+  return MPRGP_OK;
+}
+
+extern "C" int mprgp_matrix_info(const mprgp_matrix *A, int *n_nodes, long long *n_blocks, long long *bytes) {
+  if (!A) return fail(MPRGP_EBADINPUT, "NULL matrix");
+  if (n_nodes) *n_nodes = A->n_nodes;
+  if (n_blocks) *n_blocks = A->nb;
+  if (bytes) *bytes = (long long)A->bytes;
+  return MPRGP_OK;
+}
+
+extern "C" int mprgp_matrix_spmv(mprgp_matrix *A, const double *x, double *y) {
+  if (!A || !x || !y) return fail(MPRGP_EBADINPUT, "NULL argument");
+  mprgp_ctx *ctx = A->ctx;
+  MPRGP_HIP(hipSetDevice(ctx->device));
+  const int n = A->n_nodes;
+  const size_t vb = sizeof(double) * 3 * n;
+  double *buf = nullptr;
+  int rc = ctx->alloc((void **)&buf, 3 * vb);
+  if (rc) return rc;
+  double *du = buf, *dx = buf + 3 * n, *dy = buf + 6 * n;
+  hipStream_t s = ctx->stream;
+  hipError_t e = hipMemcpyAsync(du, x, vb, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = launch_gather_nodes(s, du, dx, A->d_perm, n);
+  if (e == hipSuccess) e = launch_spmv_plain(s, A->d_sl_off, A->d_col, A->d_val, dx, dy, n, 0);
+  if (e == hipSuccess) e = launch_scatter_nodes(s, dy, du, A->d_perm, n);
+  if (e == hipSuccess) e = hipMemcpyAsync(y, du, vb, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  ctx->release(buf, 3 * vb);
+  if (e != hipSuccess) return fail(MPRGP_EHIP, "spmv failed: %s", hipGetErrorString(e));
+  return MPRGP_OK;
+}
+
+extern "C" void mprgp_matrix_destroy(mprgp_matrix *A) {
+  if (!A) return;
+  mprgp_ctx *ctx = A->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->release(A->d_sl_off, sizeof(int) * (A->n_slices + 1));
+  ctx->release(A->d_col, sizeof(int) * A->slots);
+  ctx->release(A->d_val, sizeof(double) * A->slots * 9);
+  ctx->release(A->d_dinv, sizeof(double) * 3 * A->n_nodes);
+  ctx->release(A->d_perm, sizeof(int) * A->n_nodes);
+  delete A;
+}
+
+// ---------------------------------------------------------------------- QP
+
+static size_t qp_tiles_max(int n_nodes) { return (size_t)((n_nodes + kSlice - 1) / kSlice); }
+
+extern "C" int mprgp_qp_create(mprgp_matrix *A, mprgp_qp **out) {
+  if (!A || !out) return fail(MPRGP_EBADINPUT, "NULL argument");
+  *out = nullptr;
+  mprgp_ctx *ctx = A->ctx;
+  MPRGP_HIP(hipSetDevice(ctx->device));
+  mprgp_qp *q = new mprgp_qp();
+  q->ctx = ctx;
+  q->A = A;
+  q->n = A->n_nodes;
+  const size_t n3 = 3 * (size_t)q->n;
+  const int n_vec = 15;      // x x0 b g p z phi beta y ap py ev pv pw stage
+  const int n_row = 3 + 3;   // cn(3) cc cj cs
+  q->part_cap = (int)qp_tiles_max(q->n);
+  size_t doubles = n_vec * n3 + n_row * (size_t)q->n + (size_t)q->part_cap * kNumPart;
+  size_t bytes = doubles * sizeof(double) + sizeof(QpState) + 2 * (size_t)q->n + 512;
+  int rc = ctx->alloc((void **)&q->base, bytes);
+  if (rc) {
+    delete q;
+    return rc;
+  }
+  q->bytes = bytes;
+  double *p = q->base;
+  double **vecs[] = {&q->x, &q->x0, &q->b, &q->g, &q->p, &q->z, &q->phi, &q->beta,
+                     &q->y, &q->ap, &q->py, &q->ev, &q->pv, &q->pw, &q->stage};
+  for (double **v : vecs) {
+    *v = p;
+    p += n3;
+  }
+  q->cn = p; p += n3;
+  q->cc = p; p += q->n;
+  q->cj = p; p += q->n;
+  q->cs = p; p += q->n;
+  q->part = p; p += (size_t)q->part_cap * kNumPart;
+  uintptr_t u = (uintptr_t)p;
+  u = (u + 127) & ~(uintptr_t)127;
+  q->st = (QpState *)u;
+  q->cf = (unsigned char *)(q->st + 1);
+  q->face = q->cf + q->n;
+  hipStream_t s = ctx->stream;
+  hipError_t e = hipMemsetAsync(q->base, 0, bytes, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    mprgp_qp_destroy(q);
+    return fail(MPRGP_EHIP, "qp init failed: %s", hipGetErrorString(e));
+  }
+  *out = q;
+  return MPRGP_OK;
+}
+
+extern "C" void mprgp_qp_destroy(mprgp_qp *q) {
+  if (!q) return;
+  (void)hipSetDevice(q->ctx->device);
+  (void)hipStreamSynchronize(q->ctx->stream);
+  q->ctx->release(q->base, q->bytes);
+  delete q;
+}
+
+// host node-order vector -> device internal order
+static int upload_vec(mprgp_qp *q, const double *h, double *dst) {
+  const size_t vb = sizeof(double) * 3 * q->n;
+  hipStream_t s = q->ctx->stream;
+  MPRGP_HIP(hipMemcpyAsync(q->stage, h, vb, hipMemcpyHostToDevice, s));
+  MPRGP_HIP(launch_gather_nodes(s, q->stage, dst, q->A->d_perm, q->n));
+  MPRGP_HIP(hipStreamSynchronize(s));
+  return MPRGP_OK;
+}
+static int download_vec(mprgp_qp *q, const double *src, double *h) {
+  const size_t vb = sizeof(double) * 3 * q->n;
+  hipStream_t s = q->ctx->stream;
+  MPRGP_HIP(launch_scatter_nodes(s, src, q->stage, q->A->d_perm, q->n));
+  MPRGP_HIP(hipMemcpyAsync(h, q->stage, vb, hipMemcpyDeviceToHost, s));
+  MPRGP_HIP(hipStreamSynchronize(s));
+  return MPRGP_OK;
+}
+
+static int check_finite(const double *v, size_t n, const char *what) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(v[i])) return fail(MPRGP_EBADINPUT, "%s[%zu] is not finite", what, i);
+  return MPRGP_OK;
+}
+
+extern "C" int mprgp_qp_set_rhs(mprgp_qp *q, const double *b) {
+  if (!q || !b) return fail(MPRGP_EBADINPUT, "NULL argument");
+  if (int rc = check_finite(b, 3 * (size_t)q->n, "b")) return rc;
+  MPRGP_HIP(hipSetDevice(q->ctx->device));
+  int rc = upload_vec(q, b, q->b);
+  if (!rc) q->has_rhs = true;
+  return rc;
+}
+
+extern "C" int mprgp_qp_set_x(mprgp_qp *q, const double *x) {
+  if (!q || !x) return fail(MPRGP_EBADINPUT, "NULL argument");
+  if (int rc = check_finite(x, 3 * (size_t)q->n, "x")) return rc;
+  MPRGP_HIP(hipSetDevice(q->ctx->device));
+  int rc = upload_vec(q, x, q->x0);
+  if (rc) return rc;
+  MPRGP_HIP(hipMemcpyAsync(q->x, q->x0, sizeof(double) * 3 * q->n, hipMemcpyDeviceToDevice, q->ctx->stream));
+  MPRGP_HIP(hipStreamSynchronize(q->ctx->stream));
+  q->has_x = true;
+  return MPRGP_OK;
+}
+
+extern "C" int mprgp_qp_reset_x(mprgp_qp *q) {
+  if (!q) return fail(MPRGP_EBADINPUT, "NULL argument");
+  if (!q->has_x) return fail(MPRGP_ESTATE, "x was never set");
+  MPRGP_HIP(hipSetDevice(q->ctx->device));
+  MPRGP_HIP(hipMemcpyAsync(q->x, q->x0, sizeof(double) * 3 * q->n, hipMemcpyDeviceToDevice, q->ctx->stream));
+  return MPRGP_OK;
+}
+
+extern "C" int mprgp_qp_get_x(mprgp_qp *q, double *x) {
+  if (!q || !x) return fail(MPRGP_EBADINPUT, "NULL argument");
+  MPRGP_HIP(hipSetDevice(q->ctx->device));
+  return download_vec(q, q->x, x);
+}
+
+extern "C" int mprgp_qp_set_ev(mprgp_qp *q, const double *ev) {
+  if (!q) return fail(MPRGP_EBADINPUT, "NULL argument");
+  if (!ev) {
+    q->ev_valid = 0;
+    return MPRGP_OK;
+  }
+  if (int rc = check_finite(ev, 3 * (size_t)q->n, "ev")) return rc;
+  MPRGP_HIP(hipSetDevice(q->ctx->device));
+  int rc = upload_vec(q, ev, q->ev);
+  if (!rc) q->ev_valid = 1;
+  return rc;
+}
+
+extern "C" int mprgp_qp_get_ev(mprgp_qp *q, double *ev, int *valid) {
+  if (!q) return fail(MPRGP_EBADINPUT, "NULL argument");
+  if (valid) *valid = q->ev_valid;
+  if (!ev || !q->ev_valid) return MPRGP_OK;
+  MPRGP_HIP(hipSetDevice(q->ctx->device));
+  return download_vec(q, q->ev, ev);
+}
+
+extern "C" int mprgp_qp_get_face(mprgp_qp *q, unsigned char *face) {
+  if (!q || !face) return fail(MPRGP_EBADINPUT, "NULL argument");
+  MPRGP_HIP(hipSetDevice(q->ctx->device));
+  std::vector<unsigned char> f(q->n);
+  MPRGP_HIP(hipMemcpyAsync(f.data(), q->face, q->n, hipMemcpyDeviceToHost, q->ctx->stream));
+  MPRGP_HIP(hipStreamSynchronize(q->ctx->stream));
+  for (int i = 0; i < q->n; ++i) face[q->A->perm[i]] = f[i];
+  return MPRGP_OK;
+}
+
+static int upload_rows(mprgp_qp *q, int m, const std::vector<int> &node, const std::vector<double> &nrm,
+                       const std::vector<double> &c) {
+  const int N = q->n;
+  std::vector<double> cn(3 * (size_t)N, 0.0), cc(N, 0.0), cj(N, 0.0);
+  std::vector<unsigned char> cf(N, 0);
+  for (int i = 0; i < m; ++i) {
+    int k = node[i];
+    if (k < 0 || k >= N) return fail(MPRGP_EBADINPUT, "row %d: node %d out of range", i, k);
+    int r = q->A->iperm[k];
+    if (cf[r]) return fail(MPRGP_EBADINPUT, "row %d: node %d already has a row (J J^T not diagonal)", i, k);
+    double n0 = nrm[3 * i], n1 = nrm[3 * i + 1], n2 = nrm[3 * i + 2];
+    if (!std::isfinite(n0) || !std::isfinite(n1) || !std::isfinite(n2) || !std::isfinite(c[i]))
+      return fail(MPRGP_EBADINPUT, "row %d: non-finite normal or offset", i);
+    double jj = n0 * n0 + n1 * n1 + n2 * n2;
+    if (!(jj >= kEps)) return fail(MPRGP_EBADINPUT, "row %d: |n|^2 = %g below 1e-9", i, jj);
+    cf[r] = 1;
+    cn[3 * r] = n0;
+    cn[3 * r + 1] = n1;
+    cn[3 * r + 2] = n2;
+    cc[r] = c[i];
+    cj[r] = jj;
+  }
+  hipStream_t s = q->ctx->stream;
+  MPRGP_HIP(hipMemcpyAsync(q->cn, cn.data(), cn.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  MPRGP_HIP(hipMemcpyAsync(q->cc, cc.data(), cc.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  MPRGP_HIP(hipMemcpyAsync(q->cj, cj.data(), cj.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  MPRGP_HIP(hipMemcpyAsync(q->cf, cf.data(), cf.size(), hipMemcpyHostToDevice, s));
+  MPRGP_HIP(hipStreamSynchronize(s));
+  q->m = m;
+  return MPRGP_OK;
+}
+
+extern "C" int mprgp_qp_set_rows(mprgp_qp *q, int m, const int *node, const double *normal, const double *c) {
+  if (!q || m < 0 || (m > 0 && (!node || !normal || !c))) return fail(MPRGP_EBADINPUT, "bad row arguments");
+  if (m > q->n) return fail(MPRGP_EBADINPUT, "m = %d exceeds node count %d", m, q->n);
+  MPRGP_HIP(hipSetDevice(q->ctx->device));
+  std::vector<int> nd(node, node + m);
+  std::vector<double> nr(normal, normal + 3 * (size_t)m), cv(c, c + m);
+  return upload_rows(q, m, nd, nr, cv);
+}
+
+extern "C" int mprgp_qp_set_rows_sparse(mprgp_qp *q, int m, int col_major, const int *outer, const int *inner,
+                                        const double *val, const double *c) {
+  if (!q || m < 0 || !outer || (m > 0 && (!inner || !val || !c)))
+    return fail(MPRGP_EBADINPUT, "bad sparse row arguments");
+  const int ncol = 3 * q->n;
+  std::vector<int> node(m, -1);
+  std::vector<double> nrm(3 * (size_t)m, 0.0);
+  std::vector<double> cv(c, c + m);
+  auto put = [&](int row, int colj, double v) -> int {
+    if (row < 0 || row >= m || colj < 0 || colj >= ncol)
+      return fail(MPRGP_EBADINPUT, "J entry (%d,%d) out of range", row, colj);
+    int k = colj / 3;
+    if (node[row] >= 0 && node[row] != k)
+      return fail(MPRGP_EBADINPUT, "J row %d spans nodes %d and %d (only per-node rows are decoupled here)",
+                  row, node[row], k);
+    node[row] = k;
+    nrm[3 * (size_t)row + colj % 3] += v;
+    return MPRGP_OK;
+  };
+  if (col_major) {
+    for (int j = 0; j < ncol; ++j)
+      for (int k = outer[j]; k < outer[j + 1]; ++k)
+        if (int rc = put(inner[k], j, val[k])) return rc;
+  } else {
+    for (int i = 0; i < m; ++i)
+      for (int k = outer[i]; k < outer[i + 1]; ++k)
+        if (int rc = put(i, inner[k], val[k])) return rc;
+  }
+  for (int i = 0; i < m; ++i)
+    if (node[i] < 0) return fail(MPRGP_EBADINPUT, "J row %d is empty", i);
+  MPRGP_HIP(hipSetDevice(q->ctx->device));
+  return upload_rows(q, m, node, nrm, cv);
+}
+
+// ----------------------------------------------------------- batch solve
+
+static int ensure_tables(mprgp_ctx *ctx, int n_qps, int n_tiles) {
+  if (n_qps > ctx->qps_cap) {
+    int cap = std::max(n_qps, 2 * ctx->qps_cap);
+    ctx->release(ctx->d_qps, sizeof(QpDev) * (size_t)ctx->qps_cap);
+    ctx->d_qps = nullptr;
+    ctx->qps_cap = 0;
+    if (int rc = ctx->alloc((void **)&ctx->d_qps, sizeof(QpDev) * (size_t)cap)) return rc;
+    ctx->qps_cap = cap;
+    // graphs captured the old table pointers
+    for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto &kv : ctx->graph_defs) (void)hipGraphDestroy(kv.second);
+    ctx->graphs.clear();
+    ctx->graph_defs.clear();
+  }
+  if (n_tiles > ctx->tiles_cap) {
+    int cap = std::max(n_tiles, 2 * ctx->tiles_cap);
+    ctx->release(ctx->d_tiles, sizeof(Tile) * (size_t)ctx->tiles_cap);
+    ctx->d_tiles = nullptr;
+    ctx->tiles_cap = 0;
+    if (int rc = ctx->alloc((void **)&ctx->d_tiles, sizeof(Tile) * (size_t)cap)) return rc;
+    ctx->tiles_cap = cap;
+    for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (auto &kv : ctx->graph_defs) (void)hipGraphDestroy(kv.second);
+    ctx->graphs.clear();
+    ctx->graph_defs.clear();
+  }
+  return MPRGP_OK;
+}
+
+static int get_graph(mprgp_ctx *ctx, int ts, int steps, int grid, hipGraphExec_t *out) {
+  auto key = std::make_tuple(ts, steps, grid);
+  auto it = ctx->graphs.find(key);
+  if (it != ctx->graphs.end()) {
+    *out = it->second;
+    return MPRGP_OK;
+  }
+  hipStream_t s = ctx->stream;
+  MPRGP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < steps; ++i) {
+    hipError_t e = launch_step(ts, grid, s, ctx->d_qps, ctx->d_tiles, ctx->d_ctrl, ctx->d_ctrl + 1);
+    if (e != hipSuccess) {
+      hipGraph_t g;
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      return fail(MPRGP_EHIP, "capture failed: %s", hipGetErrorString(e));
+    }
+  }
+  hipGraph_t g = nullptr;
+  MPRGP_HIP(hipStreamEndCapture(s, &g));
+  hipGraphExec_t ex = nullptr;
+  hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail(MPRGP_EHIP, "graph instantiate failed: %s", hipGetErrorString(e));
+  }
+  ctx->graph_defs[key] = g;
+  ctx->graphs[key] = ex;
+  *out = ex;
+  return MPRGP_OK;
+}
+
+static int pick_tiles(const mprgp_params &p, int total_nodes) {
+  if (p.tiles == 1 || p.tiles == 2 || p.tiles == 4 || p.tiles == 8) return p.tiles;
+  // enough workgroups to cover 256 CUs several times over
+  long slices = (total_nodes + kSlice - 1) / kSlice;
+  if (slices >= 4L * 2048) return 4;
+  if (slices >= 2L * 1024) return 2;
+  return 1;
+}
+
+static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprgp_params *params,
+                     int stride, mprgp_stats *stats) {
+  if (!ctx || count <= 0 || !qps || !params) return fail(MPRGP_EBADINPUT, "bad batch arguments");
+  MPRGP_HIP(hipSetDevice(ctx->device));
+  long total_nodes = 0;
+  for (int i = 0; i < count; ++i) {
+    const mprgp_qp *q = qps[i];
+    const mprgp_params &p = params[stride ? i : 0];
+    if (!q) return fail(MPRGP_EBADINPUT, "qp %d is NULL", i);
+    if (q->ctx != ctx) return fail(MPRGP_EBADINPUT, "qp %d belongs to another context", i);
+    if (!q->has_rhs || !q->has_x) return fail(MPRGP_ESTATE, "qp %d: rhs and x must be set", i);
+    if (!(p.tol == p.tol) || p.max_it < 0) return fail(MPRGP_EBADINPUT, "qp %d: bad tol/max_it", i);
+    if (p.precision != MPRGP_FP64)
+      return fail(MPRGP_EBADINPUT, "qp %d: precision %d not available in this entry", i, p.precision);
+    if (p.variant != MPRGP_MONOTONIC && p.variant != MPRGP_CLASSIC)
+      return fail(MPRGP_EBADINPUT, "qp %d: unknown variant %d", i, p.variant);
+    for (int j = 0; j < i; ++j)
+      if (qps[j] == q) return fail(MPRGP_EBADINPUT, "qp %d appears twice in the batch", i);
+    total_nodes += q->n;
+  }
+  const int ts = pick_tiles(params[0], (int)std::min<long>(total_nodes, 0x7fffffff));
+  // tile table and device descriptors
+  std::vector<Tile> tiles;
+  std::vector<QpDev> devs(count);
+  std::vector<QpState> states(count);
+  long max_steps = 0;
+  hipStream_t s = ctx->stream;
+  for (int i = 0; i < count; ++i) {
+    mprgp_qp *q = qps[i];
+    const mprgp_params &p = params[stride ? i : 0];
+    const int n_slices = q->A->n_slices;
+    QpDev &d = devs[i];
+    d.n_nodes = q->n;
+    d.n_slices = n_slices;
+    d.tile0 = (int)tiles.size();
+    d.sl_off = q->A->d_sl_off;
+    d.col = q->A->d_col;
+    d.val = q->A->d_val;
+    d.dinv = q->A->d_dinv;
+    d.x = q->x; d.b = q->b; d.g = q->g; d.p = q->p; d.z = q->z; d.phi = q->phi; d.beta = q->beta;
+    d.y = q->y; d.ap = q->ap; d.py = q->py; d.ev = q->ev; d.pv = q->pv; d.pw = q->pw;
+    d.cn = q->cn; d.cc = q->cc; d.cj = q->cj; d.cs = q->cs; d.cf = q->cf; d.face = q->face;
+    d.st = q->st;
+    d.part = q->part;
+    int local = 0;
+    for (int s0 = 0; s0 < n_slices; s0 += ts, ++local)
+      tiles.push_back(Tile{i, s0, std::min(ts, n_slices - s0), local});
+    QpState &st = states[i];
+    std::memset(&st, 0, sizeof(st));
+    st.n_tiles = local;
+    st.max_it = p.max_it;
+    st.tol = p.tol < 1e-30 ? 1e-30 : p.tol;
+    st.gamma = 1.0;
+    st.code = -1;
+    st.has_rows = q->m > 0;
+    st.precond = p.preconditioned ? 1 : 0;
+    st.variant = p.variant;
+    const bool power = !(p.alpha_bar > 0.0);
+    st.alpha_bar = power ? 0.0 : p.alpha_bar;
+    st.lambda_max = power ? 0.0 : 2.0 / p.alpha_bar;
+    st.power_eps = p.power_tol > 0.0 ? p.power_tol : p.tol;
+    st.pow_next = power ? OP_POW_MV : OP_INIT;
+    st.op = p.project_first ? OP_PROJX : st.pow_next;
+    max_steps = std::max<long>(max_steps, 8L * ((long)p.max_it + 4) + (power ? 2L * 1002 + 4 : 0) + 16);
+    if (power) {
+      if (p.use_ev && q->ev_valid) {
+        MPRGP_HIP(hipMemcpyAsync(q->pv, q->ev, sizeof(double) * 3 * q->n, hipMemcpyDeviceToDevice, s));
+      } else {
+        // deterministic start vector in user node order, normalised
+        std::vector<double> v(3 * (size_t)q->n);
+        uint64_t seed = 0x5EEDULL;
+        for (auto &e : v) e = 2.0 * ((double)(splitmix64(seed) >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+        double nr = 0.0;
+        for (double e : v) nr += e * e;
+        nr = std::sqrt(nr);
+        if (nr > 0.0)
+          for (auto &e : v) e /= nr;
+        MPRGP_HIP(hipMemcpyAsync(q->stage, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        MPRGP_HIP(launch_gather_nodes(s, q->stage, q->pv, q->A->d_perm, q->n));
+        MPRGP_HIP(hipStreamSynchronize(s));
+      }
+    }
+    // tangent-Jacobi factors depend on the current matrix values
+    MPRGP_HIP(launch_setup_rows(s, q->A->d_dinv, q->cn, q->cf, q->cs, q->n));
+    MPRGP_HIP(hipMemcpyAsync(q->st, &st, sizeof(QpState), hipMemcpyHostToDevice, s));
+  }
+  const int n_tiles = (int)tiles.size();
+  if (int rc = ensure_tables(ctx, count, n_tiles)) return rc;
+  int ctrl[2] = {n_tiles, count};
+  MPRGP_HIP(hipMemcpyAsync(ctx->d_qps, devs.data(), sizeof(QpDev) * count, hipMemcpyHostToDevice, s));
+  MPRGP_HIP(hipMemcpyAsync(ctx->d_tiles, tiles.data(), sizeof(Tile) * n_tiles, hipMemcpyHostToDevice, s));
+  MPRGP_HIP(hipMemcpyAsync(ctx->d_ctrl, ctrl, sizeof(ctrl), hipMemcpyHostToDevice, s));
+  MPRGP_HIP(hipStreamSynchronize(s));
+
+  const int grid = std::min(n_tiles, 8192);
+  int steps = params[0].graph_steps > 0 ? params[0].graph_steps : 16;
+  hipGraphExec_t ex = nullptr;
+  if (int rc = get_graph(ctx, ts, steps, grid, &ex)) return rc;
+
+  MPRGP_HIP(hipEventRecord(ctx->t0, s));
+  long launched = 0;
+  int k = 0;
+  bool done = false;
+  auto enqueue = [&](int slot) -> int {
+    MPRGP_HIP(hipGraphLaunch(ex, s));
+    launched += steps;
+    MPRGP_HIP(hipMemcpyAsync(ctx->h_running + slot, ctx->d_ctrl + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    MPRGP_HIP(hipEventRecord(ctx->q[slot], s));
+    return MPRGP_OK;
+  };
+  if (int rc = enqueue(0)) return rc;
+  while (!done) {
+    if (int rc = enqueue((k + 1) & 1)) return rc;
+    MPRGP_HIP(hipEventSynchronize(ctx->q[k & 1]));
+    if (ctx->h_running[k & 1] == 0) done = true;
+    ++k;
+    if (!done && launched > max_steps + 2L * steps) {
+      (void)hipStreamSynchronize(s);
+      return fail(MPRGP_EHIP, "solve did not finish within %ld device steps", launched);
+    }
+  }
+  MPRGP_HIP(hipEventRecord(ctx->t1, s));
+  MPRGP_HIP(hipStreamSynchronize(s));
+  float ms = 0.f;
+  MPRGP_HIP(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
+
+  int worst = MPRGP_OK;
+  for (int i = 0; i < count; ++i) {
+    QpState st;
+    MPRGP_HIP(hipMemcpy(&st, qps[i]->st, sizeof(QpState), hipMemcpyDeviceToHost));
+    if (st.ev_valid) qps[i]->ev_valid = 1;
+    if (stats) {
+      mprgp_stats &o = stats[i];
+      std::memset(&o, 0, sizeof(o));
+      o.code = st.code;
+      o.iterations = st.iteration;
+      o.num_cg = st.num_cg;
+      o.num_exp = st.num_exp;
+      o.num_prop = st.num_prop;
+      o.active = st.active;
+      o.power_iters = st.power_it;
+      o.ev_valid = qps[i]->ev_valid;
+      o.steps = st.steps;
+      o.residual = st.residual;
+      o.alpha_bar = st.alpha_bar;
+      o.lambda_max = st.lambda_max;
+      o.elapsed_ms = ms;
+    }
+    if (!std::isfinite(st.residual)) worst = MPRGP_ENAN;
+    else if (st.code != 0 && worst == MPRGP_OK) worst = MPRGP_NOT_CONVERGED;
+  }
+  return worst;
+}
+
+extern "C" int mprgp_qp_solve(mprgp_qp *q, const mprgp_params *params, mprgp_stats *stats) {
+  if (!q) return fail(MPRGP_EBADINPUT, "NULL qp");
+  mprgp_params def;
+  if (!params) {
+    mprgp_params_default(&def);
+    params = &def;
+  }
+  return run_batch(q->ctx, 1, &q, params, 0, stats);
+}
+
+extern "C" int mprgp_solve_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprgp_params *params,
+                                 int params_stride, mprgp_stats *stats) {
+  mprgp_params def;
+  if (!params) {
+    mprgp_params_default(&def);
+    params = &def;
+    params_stride = 0;
+  }
+  return run_batch(ctx, count, qps, params, params_stride ? 1 : 0, stats);
+}
+
+extern "C" int mprgp_decoupled_solve(int device, int n, const int *a_outer, const int *a_inner,
+                                     const double *a_val, const double *B, int m, const int *j_outer,
+                                     const int *j_inner, const double *j_val, const double *c, double *x,
+                                     double tol, int max_it, double *ev, int *ev_valid, mprgp_stats *stats) {
+  if (!B || !x) return fail(MPRGP_EBADINPUT, "NULL B or x");
+  mprgp_ctx *ctx = nullptr;
+  int rc = mprgp_ctx_create(device, 0, &ctx);
+  if (rc) return rc;
+  mprgp_matrix *A = nullptr;
+  mprgp_qp *q = nullptr;
+  rc = mprgp_matrix_create_csr(ctx, n, a_outer, a_inner, a_val, &A);
+  if (!rc) rc = mprgp_qp_create(A, &q);
+  if (!rc) rc = mprgp_qp_set_rows_sparse(q, m, 1, j_outer, j_inner, j_val, c);
+  if (!rc) rc = mprgp_qp_set_rhs(q, B);
+  if (!rc) rc = mprgp_qp_set_x(q, x);
+  if (!rc && ev && ev_valid && *ev_valid) rc = mprgp_qp_set_ev(q, ev);
+  int code = rc;
+  if (!rc) {
+    mprgp_params p;
+    mprgp_params_default(&p);
+    p.tol = tol;
+    p.max_it = max_it;
+    p.project_first = 1;
+    code = mprgp_qp_solve(q, &p, stats);
+    if (code == MPRGP_OK || code == MPRGP_NOT_CONVERGED) {
+      int rc2 = mprgp_qp_get_x(q, x);
+      if (!rc2 && ev && ev_valid) rc2 = mprgp_qp_get_ev(q, ev, ev_valid);
+      if (rc2) code = rc2;
+    }
+  }
+  mprgp_qp_destroy(q);
+  mprgp_matrix_destroy(A);
+  mprgp_ctx_destroy(ctx);
+  return code;
+}

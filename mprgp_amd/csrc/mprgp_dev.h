@@ -1,0 +1,108 @@
+// mprgp_dev.h -- device-side data layout shared by the HIP kernels and the
+// host driver of the MI355X MPRGP contact-QP solver.
+//
+// Layout in HBM (per QP, all in the QP's internal node order, which sorts
+// nodes by descending block-row length so that SELL slices carry no padding):
+//   matrix  SELL-64 of 3x3 blocks: slice s covers internal nodes 64s..64s+63;
+//           its blocks occupy [sl_off[s], sl_off[s+1]) (a multiple of 64);
+//           block k of lane l sits at sl_off[s] + 64k + l; column index col[]
+//           (internal node, -1 for padding); values val[9*blk0 + 64*(9k+e) + l]
+//           for entry e = 3*row + col of the block, so every load of one entry
+//           across a wavefront is 64 consecutive values (512 B for FP64).
+//   vectors 3 values per node, node-major (x0 y0 z0 x1 ...).
+//   rows    at most one half-space row per node: normal cn[3r..3r+2], offset
+//           cc[r], |n|^2 in cj[r], tangent-Jacobi factor cs[r], flag cf[r]
+//           (1 = node has a row), active set face[r].
+#pragma once
+#include <cstdint>
+
+namespace mprgp {
+
+constexpr int kSlice = 64;       // rows per SELL slice = lanes per wavefront
+constexpr int kNumPart = 8;      // partial sums per tile
+constexpr double kEps = 1e-9;    // ScalarUtil<double>::scalar_eps of the simulator
+
+enum Op : int {
+  OP_DONE = 0,
+  OP_PROJX,      // x = P(x) before the solve (J/c facade overload)
+  OP_POW_MV,     // power method: w = A v, |w|^2
+  OP_POW_NRM,    // w /= nu, |w - v|^2
+  OP_POW_SAVE,   // ev = v
+  OP_INIT,       // g = Ax - b, face(x), phi, z, p = z
+  OP_TOP,        // phi, beta, |phi+beta|^2, beta.beta, phitphi
+  OP_CG_AP,      // AP = A p, p.AP, z.g
+  OP_CG_PY,      // y = x - alpha p, py = P(y), feasibility of y
+  OP_CG_Y,       // A py (and A x at CG entry): objective values
+  OP_CG_A,       // x = y, g -= alpha AP, phi, z, z.AP
+  OP_CG_B,       // p = z - beta p, beta, |phi+beta|^2, beta.beta, phitphi
+  OP_EXP1,       // y = P(x)
+  OP_EXP2,       // g = A y - b, face(y), phi, y' = y - abar phi, x = P(y')
+  OP_EXP3,       // g = A x - b, face(x), phi, z, p = z, y = y'
+  OP_PROP1,      // g = A x - b, AD = A beta, beta.AD, g.beta
+  OP_PROP2,      // x = P(x - a beta), g -= a AD, face(x), phi, z, p = z
+  OP_FINAL,      // x = P(x) (monotone only), active count
+  OP_CL_AP,      // classic: AP = A p, p.AP, z.g, max feasible step
+  OP_CL_CG,      // classic CG step: x -= a p, g -= a AP, phi, z, z.AP
+  OP_CL_EXP,     // classic expansion (first half, elementwise)
+  OP_CL_EXP2,    // classic expansion (second half, SpMV)
+  OP_CL_PROP1,   // classic proportioning: AD = A beta
+  OP_NUM
+};
+
+enum Variant : int { VAR_MONOTONIC = 0, VAR_CLASSIC = 1 };
+
+// Per-QP control block, written by the last tile of each launch.
+struct alignas(128) QpState {
+  int op;
+  int arrive;        // tiles that finished the current launch
+  int n_tiles;
+  int iteration, num_cg, num_exp, num_prop;
+  int max_it;
+  int prop;          // result_is_prop
+  int cg_entry;      // next CG_Y also evaluates f(x)
+  int y_valid, y_feas;
+  int code;
+  int active;
+  int power_it;
+  int ev_valid;      // ev holds a power vector on exit
+  int has_rows;      // QP has at least one constraint row
+  int precond;       // tangent-Jacobi preconditioning on
+  int variant;
+  int p_pending;     // classic: p = z - beta p still to apply
+  int pow_next;      // op after the power method
+  int steps;         // launches this QP took part in
+  int pad0;
+  double tol, alpha_bar, gamma;
+  double alpha, beta_cg, pd, zg, residual, fx, fy;
+  double alpha_f, nu, power_eps, lambda_max;
+  double pad1[3];
+};
+
+// Device pointers of one QP (FP64 form).
+struct QpDev {
+  int n_nodes;
+  int n_slices;
+  int tile0;         // first entry of this QP in the batch tile table
+  int pad;
+  const int *sl_off;
+  const int *col;
+  const double *val;
+  const double *dinv;   // 1/diag(A), 3 per node
+  double *x, *b, *g, *p, *z, *phi, *beta, *y, *ap, *py;
+  double *ev, *pv, *pw; // power method vectors
+  const double *cn, *cc, *cj, *cs;
+  const unsigned char *cf;
+  unsigned char *face;
+  QpState *st;
+  double *part;         // n_tiles * kNumPart partial sums
+};
+
+// One work item of a launch: up to `slices` consecutive slices of one QP.
+struct Tile {
+  int qp;        // index into the batch's QpDev table
+  int slice0;
+  int slices;
+  int local;     // tile index inside its QP (partials slot)
+};
+
+}  // namespace mprgp

@@ -1,0 +1,101 @@
+// mprgp_host.h -- host-side objects behind the C ABI (internal header).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/mprgp_amd.h"
+#include "mprgp_dev.h"
+
+namespace mprgp {
+
+// last-error plumbing
+void set_error(const char *fmt, ...);
+int fail(int code, const char *fmt, ...);
+
+#define MPRGP_HIP(call)                                                                   \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return ::mprgp::fail(MPRGP_EHIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                           __FILE__, __LINE__);                                           \
+  } while (0)
+
+// kernel launch wrappers (mprgp_kernels.hip)
+hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
+                       const int *n_tiles, int *running);
+hipError_t launch_gather_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n);
+hipError_t launch_scatter_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n);
+hipError_t launch_setup_diag(hipStream_t s, const int *sl_off, const int *col, const double *val,
+                             double *dinv, int n);
+hipError_t launch_setup_rows(hipStream_t s, const double *dinv, const double *cn,
+                             const unsigned char *cf, double *cs, int n);
+hipError_t launch_spmv_plain(hipStream_t s, const int *sl_off, const int *col, const double *val,
+                             const double *x, double *y, int n, int accumulate);
+
+}  // namespace mprgp
+
+struct mprgp_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t budget = 0;
+  size_t in_use = 0;
+  // batch launch tables (device), grown on demand
+  mprgp::QpDev *d_qps = nullptr;
+  int qps_cap = 0;
+  mprgp::Tile *d_tiles = nullptr;
+  int tiles_cap = 0;
+  int *d_ctrl = nullptr;      // [0] = tile count, [1] = running QPs
+  int *h_running = nullptr;   // pinned, 2 slots
+  hipEvent_t t0 = nullptr, t1 = nullptr, q[2] = {nullptr, nullptr};
+  std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
+  std::map<std::tuple<int, int, int>, hipGraph_t> graph_defs;
+  // autotuner cache: node-count bucket -> (tiles per workgroup)
+  std::map<int, int> tuned_ts;
+
+  int alloc(void **p, size_t bytes);
+  void release(void *p, size_t bytes);
+};
+
+struct mprgp_matrix {
+  mprgp_ctx *ctx = nullptr;
+  int n_nodes = 0;
+  int n_slices = 0;
+  long long nb = 0;      // real blocks
+  long long slots = 0;   // SELL slots including padding
+  std::vector<int> perm;   // internal -> user node
+  std::vector<int> iperm;  // user -> internal node
+  std::vector<long long> blk_slot;  // input block -> SELL slot (for value updates)
+  std::vector<int> sl_off_h;
+  int *d_sl_off = nullptr;
+  int *d_col = nullptr;
+  double *d_val = nullptr;
+  double *d_dinv = nullptr;
+  int *d_perm = nullptr;
+  size_t bytes = 0;
+  int version = 0;  // bumped on value updates
+};
+
+struct mprgp_qp {
+  mprgp_ctx *ctx = nullptr;
+  mprgp_matrix *A = nullptr;
+  int n = 0;  // nodes
+  double *base = nullptr;
+  size_t bytes = 0;
+  double *x = nullptr, *x0 = nullptr, *b = nullptr, *g = nullptr, *p = nullptr, *z = nullptr,
+         *phi = nullptr, *beta = nullptr, *y = nullptr, *ap = nullptr, *py = nullptr, *ev = nullptr,
+         *pv = nullptr, *pw = nullptr, *stage = nullptr;
+  double *cn = nullptr, *cc = nullptr, *cj = nullptr, *cs = nullptr;
+  unsigned char *cf = nullptr, *face = nullptr;
+  mprgp::QpState *st = nullptr;
+  double *part = nullptr;
+  int part_cap = 0;  // tiles
+  int m = 0;
+  bool has_rhs = false, has_x = false;
+  int ev_valid = 0;
+};

@@ -1,0 +1,848 @@
+// mprgp_kernels.hip -- CDNA4 (gfx950) kernels of the MPRGP contact-QP solve.
+//
+// One kernel, `mprgp_step`, advances every QP of a batch by one operation of
+// the reference's MPRGP control flow (MPRGPSolver.h:148-456).  Each QP keeps
+// its own state machine on the device (QpState::op); a launch executes, for
+// every tile of every QP, the op that QP is in, and the last tile of a QP to
+// finish reduces that QP's partial sums in a fixed order and chooses the
+// QP's next op.  The host never reads a scalar of the iteration: it replays
+// launches (captured in a hipGraph) and polls a done-counter.
+//
+// Work mapping: one wavefront per SELL slice, one lane per block row
+// (= node).  SpMV ops stream the slice's 3x3 blocks with fully coalesced
+// loads (the slice layout puts one block entry of 64 consecutive rows in
+// 512 contiguous bytes) and gather the input vector per neighbour node.
+// Elementwise ops (projection, gradient split, preconditioner, vector
+// updates) run in the same lane-per-node mapping and are fused into the SpMV
+// ops wherever the reference's data flow allows.
+#include <hip/hip_runtime.h>
+
+#include "mprgp_dev.h"
+
+namespace mprgp {
+
+struct V3 {
+  double x, y, z;
+};
+
+__device__ __forceinline__ V3 ld3(const double *v, int r) {
+  const double *p = v + 3 * (size_t)r;
+  return V3{p[0], p[1], p[2]};
+}
+__device__ __forceinline__ void st3(double *v, int r, V3 a) {
+  double *p = v + 3 * (size_t)r;
+  p[0] = a.x;
+  p[1] = a.y;
+  p[2] = a.z;
+}
+__device__ __forceinline__ V3 add(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+// a - s*b
+__device__ __forceinline__ V3 axmy(V3 a, double s, V3 b) {
+  return V3{a.x - s * b.x, a.y - s * b.y, a.z - s * b.z};
+}
+__device__ __forceinline__ double dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// The half-space row of one node (at most one per node).
+struct Row {
+  V3 n;
+  double c, jj, s;
+  bool has;
+};
+
+__device__ __forceinline__ Row ldrow(const QpDev &q, int r) {
+  Row R;
+  R.has = q.cf[r] != 0;
+  if (R.has) {
+    R.n = ld3(q.cn, r);
+    R.c = q.cc[r];
+    R.jj = q.cj[r];
+    R.s = q.cs[r];
+  } else {
+    R.n = V3{0.0, 0.0, 0.0};
+    R.c = R.jj = R.s = 0.0;
+  }
+  return R;
+}
+
+// DecoupledConProjector::project (MPRGPProjection.h:333-348), one node.
+__device__ __forceinline__ V3 proj(const Row &R, V3 x) {
+  if (!R.has) return x;
+  double v = (R.c - dot3(R.n, x)) / R.jj;
+  double lam = (v < 0.0) ? 0.0 : v;
+  return V3{x.x + R.n.x * lam, x.y + R.n.y * lam, x.z + R.n.z * lam};
+}
+
+// DECIDE_FACE (MPRGPProjection.h:283-297): |n.x - c| < eps.
+__device__ __forceinline__ bool face_of(const Row &R, V3 x) {
+  return R.has && fabs(dot3(R.n, x) - R.c) < kEps;
+}
+
+// isFeasible (MPRGPProjection.h:298-310): violated when n.x < c - eps.
+__device__ __forceinline__ double infeasible(const Row &R, V3 x) {
+  return (R.has && dot3(R.n, x) < R.c - kEps) ? 1.0 : 0.0;
+}
+
+// PHI (MPRGPProjection.h:349-367): free gradient.
+__device__ __forceinline__ V3 phi_of(const Row &R, bool face, V3 g) {
+  if (!face) return g;
+  double lam = -dot3(R.n, g) / R.jj;
+  return V3{g.x + R.n.x * lam, g.y + R.n.y * lam, g.z + R.n.z * lam};
+}
+
+// BETA (MPRGPProjection.h:368-383): chopped gradient.  Rows outside the
+// face subtract lambda = n.beta unscaled, exactly as the reference does.
+__device__ __forceinline__ V3 beta_of(const Row &R, bool face, V3 g, V3 phi) {
+  V3 b = sub(g, phi);
+  if (!R.has) return b;
+  double lam = dot3(R.n, b);
+  if (face) {
+    double v = lam / R.jj;
+    lam = (0.0 < v) ? v : 0.0;
+  }
+  return V3{b.x - R.n.x * lam, b.y - R.n.y * lam, b.z - R.n.z * lam};
+}
+
+// DiagonalDecouplePreconSolver::solve (MPRGPPrecondition.h:91-105):
+// z = D^-1 g - (n o D^-1) lambda with lambda = s (n o D^-1).g on active rows;
+// with preconditioning off or no rows at all, z = phi.
+__device__ __forceinline__ V3 precond_of(const QpDev &q, bool on, const Row &R, bool face, int r,
+                                         V3 g, V3 phi) {
+  if (!on) return phi;
+  V3 d = ld3(q.dinv, r);
+  V3 z{d.x * g.x, d.y * g.y, d.z * g.z};
+  if (face) {
+    V3 jm{R.n.x * d.x, R.n.y * d.y, R.n.z * d.z};
+    double lam = R.s * dot3(jm, g);
+    z = V3{z.x - jm.x * lam, z.y - jm.y * lam, z.z - jm.z * lam};
+  }
+  return z;
+}
+
+// ------------------------------------------------------------------ SpMV
+
+// Accumulates A v for the lane's block row over the slice's blocks; `gather`
+// returns NV input vectors at a neighbour node.
+template <int NV, class G>
+__device__ __forceinline__ void spmv_row(const QpDev &q, int s, int lane, G gather, V3 (&acc)[NV]) {
+#pragma unroll
+  for (int t = 0; t < NV; ++t) acc[t] = V3{0.0, 0.0, 0.0};
+  const int b0 = q.sl_off[s];
+  const int W = (q.sl_off[s + 1] - b0) / kSlice;
+  const int *cp = q.col + b0 + lane;
+  const double *vp = q.val + (size_t)b0 * 9 + lane;
+#pragma unroll 2
+  for (int k = 0; k < W; ++k) {
+    const int j = cp[k * kSlice];
+    if (j >= 0) {
+      const double *v = vp + (size_t)k * (9 * kSlice);
+      const double a00 = v[0 * kSlice], a01 = v[1 * kSlice], a02 = v[2 * kSlice];
+      const double a10 = v[3 * kSlice], a11 = v[4 * kSlice], a12 = v[5 * kSlice];
+      const double a20 = v[6 * kSlice], a21 = v[7 * kSlice], a22 = v[8 * kSlice];
+      V3 xs[NV];
+      gather(j, xs);
+#pragma unroll
+      for (int t = 0; t < NV; ++t) {
+        acc[t].x += a00 * xs[t].x;
+        acc[t].x += a01 * xs[t].y;
+        acc[t].x += a02 * xs[t].z;
+        acc[t].y += a10 * xs[t].x;
+        acc[t].y += a11 * xs[t].y;
+        acc[t].y += a12 * xs[t].z;
+        acc[t].z += a20 * xs[t].x;
+        acc[t].z += a21 * xs[t].y;
+        acc[t].z += a22 * xs[t].z;
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------- reductions
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmin(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
+// Partial slot 7 reduces with min (max feasible step); slots 0..6 with sum.
+constexpr int kMinSlot = 7;
+
+// ------------------------------------------------------------- control
+
+struct Sums {
+  double v[kNumPart];
+};
+
+__device__ __forceinline__ void to_final(QpState &st) { st.op = OP_FINAL; }
+
+// Op chosen after the outer loop's `iteration < max_iterations` test.
+__device__ __forceinline__ void next_outer(QpState &st) {
+  st.op = (st.iteration < st.max_it) ? OP_TOP : OP_FINAL;
+}
+
+// proportional (MPRGPSolver.h:174-181) with PHITPHI's scaling
+// (MPRGPProjection.h:384-393).
+__device__ __forceinline__ int proportional(const QpState &st, double bb, double xpx_phi) {
+  double r = xpx_phi * (1.0 / st.alpha_bar);
+  double phitphi = r > 0.0 ? r : 0.0;
+  return bb <= st.gamma * st.gamma * phitphi;
+}
+
+// Runs on one thread of the last tile of a QP: consumes the QP's reduced
+// partial sums and moves its state machine (MPRGPSolver.h:42-79, 148-456).
+__device__ void control(QpState &st, const Sums &S, int *running) {
+  switch (st.op) {
+    case OP_PROJX:
+      st.op = st.pow_next;
+      break;
+    case OP_POW_MV: {
+      double nu = sqrt(S.v[0]);
+      st.nu = nu;
+      if (nu < kEps) {  // specRad: tiny norm ends the iteration, ev = v
+        st.nu = kEps;
+        st.op = OP_POW_SAVE;
+      } else {
+        st.op = OP_POW_NRM;
+      }
+      break;
+    }
+    case OP_POW_NRM: {
+      double delta = sqrt(S.v[0]);
+      if (delta <= st.power_eps) {
+        st.op = OP_POW_SAVE;
+      } else {
+        st.power_it++;
+        if (st.power_it > 1000) {  // cap: ev untouched
+          st.lambda_max = st.nu;
+          st.alpha_bar = 2.0 / st.nu;
+          st.op = OP_INIT;
+        } else {
+          st.op = OP_POW_MV;  // v and w swap roles (power_it parity)
+        }
+      }
+      break;
+    }
+    case OP_POW_SAVE:
+      st.ev_valid = 1;
+      st.lambda_max = st.nu;
+      st.alpha_bar = 2.0 / st.nu;
+      st.op = OP_INIT;
+      break;
+    case OP_INIT:
+      st.iteration = st.num_cg = st.num_exp = st.num_prop = 0;
+      st.residual = 0.0;
+      st.op = (st.max_it > 0) ? OP_TOP : OP_FINAL;
+      break;
+    case OP_TOP: {
+      st.p_pending = 0;
+      st.residual = sqrt(S.v[0]);
+      if (st.residual <= st.tol) {
+        to_final(st);
+        break;
+      }
+      st.prop = proportional(st, S.v[1], S.v[2]);
+      if (st.variant == VAR_CLASSIC) {
+        st.op = st.prop ? OP_CL_AP : OP_CL_PROP1;
+      } else if (st.prop) {
+        st.cg_entry = 1;
+        st.op = OP_CG_AP;
+      } else {
+        st.op = (!st.y_valid || !st.y_feas) ? OP_EXP1 : OP_PROP1;
+      }
+      break;
+    }
+    case OP_CG_AP:
+      st.pd = S.v[0];
+      st.zg = S.v[1];
+      st.alpha = st.zg / st.pd;
+      st.op = OP_CG_PY;
+      break;
+    case OP_CG_PY:
+      st.y_valid = 1;
+      st.y_feas = S.v[0] == 0.0;
+      st.op = OP_CG_Y;
+      break;
+    case OP_CG_Y: {
+      double fnew = 0.5 * S.v[0] - S.v[1];
+      if (st.cg_entry) {
+        st.fx = 0.5 * S.v[2] - S.v[3];
+        st.fy = fnew;
+        st.prop = 1;
+        st.cg_entry = 0;
+      } else {
+        st.fx = st.fy;
+        st.fy = fnew;
+        st.num_cg++;
+        st.iteration++;
+      }
+      if (st.prop && st.residual > st.tol && st.fy <= st.fx && st.iteration < st.max_it) {
+        st.op = OP_CG_A;
+      } else if (st.residual <= st.tol) {
+        to_final(st);
+      } else {
+        st.op = (st.prop || !st.y_valid || !st.y_feas) ? OP_EXP1 : OP_PROP1;
+      }
+      break;
+    }
+    case OP_CG_A:
+      st.beta_cg = S.v[0] / st.pd;
+      st.op = OP_CG_B;
+      break;
+    case OP_CG_B:
+      st.residual = sqrt(S.v[0]);
+      st.prop = proportional(st, S.v[1], S.v[2]);
+      st.op = OP_CG_AP;
+      break;
+    case OP_EXP1:
+      st.op = OP_EXP2;
+      break;
+    case OP_EXP2:
+      st.y_valid = 1;
+      st.y_feas = S.v[0] == 0.0;
+      st.op = OP_EXP3;
+      break;
+    case OP_EXP3:
+      st.num_exp++;
+      st.iteration++;
+      next_outer(st);
+      break;
+    case OP_PROP1:
+      st.alpha = S.v[1] / S.v[0];
+      st.op = OP_PROP2;
+      break;
+    case OP_PROP2:
+      st.num_prop++;
+      st.iteration++;
+      if (st.variant == VAR_CLASSIC) {
+        st.op = (st.iteration < st.max_it) ? OP_TOP : OP_FINAL;
+      } else {
+        next_outer(st);
+      }
+      break;
+    case OP_CL_AP: {
+      st.pd = S.v[0];
+      st.zg = S.v[1];
+      double acg = st.zg / st.pd;
+      double af = (acg == 1.7976931348623157e308) ? acg : (acg + kEps);
+      double lim = S.v[kMinSlot];
+      if (lim < af) af = lim;
+      st.alpha = acg;
+      st.alpha_f = af;
+      st.op = (acg <= af) ? OP_CL_CG : OP_CL_EXP;
+      break;
+    }
+    case OP_CL_CG:
+      st.beta_cg = S.v[0] / st.pd;
+      st.p_pending = 1;
+      st.num_cg++;
+      st.iteration++;
+      st.op = (st.iteration < st.max_it) ? OP_TOP : OP_FINAL;
+      break;
+    case OP_CL_EXP:
+      st.op = OP_CL_EXP2;
+      break;
+    case OP_CL_EXP2:
+      st.num_exp++;
+      st.iteration++;
+      st.op = (st.iteration < st.max_it) ? OP_TOP : OP_FINAL;
+      break;
+    case OP_CL_PROP1:
+      st.alpha = S.v[1] / S.v[0];
+      st.op = OP_PROP2;
+      break;
+    case OP_FINAL:
+      st.active = (int)S.v[0];
+      st.code = st.residual <= st.tol ? 0 : -1;
+      st.op = OP_DONE;
+      atomicSub(running, 1);
+      break;
+    default:
+      break;
+  }
+}
+
+// --------------------------------------------------------------- kernel
+
+template <int TS>
+__device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &st, double (&P)[kNumPart]) {
+  const int wave = threadIdx.x / kSlice;
+  const int lane = threadIdx.x % kSlice;
+  if (wave >= t.slices) return;
+  const int s = t.slice0 + wave;
+  const int r = s * kSlice + lane;
+  if (r >= q.n_nodes) return;
+  const bool pre = st.precond && st.has_rows;
+
+  switch (op) {
+    case OP_PROJX: {
+      Row R = ldrow(q, r);
+      st3(q.x, r, proj(R, ld3(q.x, r)));
+      break;
+    }
+    case OP_POW_MV: {
+      // v alternates between pv and pw with the iteration parity
+      const double *v = (st.power_it & 1) ? q.pw : q.pv;
+      double *w = (st.power_it & 1) ? q.pv : q.pw;
+      V3 acc[1];
+      spmv_row<1>(q, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(v, j); }, acc);
+      st3(w, r, acc[0]);
+      P[0] = dot3(acc[0], acc[0]);
+      break;
+    }
+    case OP_POW_NRM: {
+      const double *v = (st.power_it & 1) ? q.pw : q.pv;
+      double *w = (st.power_it & 1) ? q.pv : q.pw;
+      V3 a = ld3(w, r), b = ld3(v, r);
+      a = V3{a.x / st.nu, a.y / st.nu, a.z / st.nu};
+      st3(w, r, a);
+      V3 dlt = sub(a, b);
+      P[0] = dot3(dlt, dlt);
+      break;
+    }
+    case OP_POW_SAVE: {
+      const double *v = (st.power_it & 1) ? q.pw : q.pv;
+      st3(q.ev, r, ld3(v, r));
+      break;
+    }
+    case OP_INIT:
+    case OP_EXP3:
+    case OP_CL_EXP2: {
+      // g = A x - b ; DECIDE_FACE(x) ; phi ; z ; p = z   (MPRGPSolver.h:148-156, 263-268, 210-215)
+      V3 acc[1];
+      const double *x = q.x;
+      spmv_row<1>(q, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
+      V3 g = sub(acc[0], ld3(q.b, r));
+      Row R = ldrow(q, r);
+      bool f = face_of(R, ld3(q.x, r));
+      V3 ph = phi_of(R, f, g);
+      V3 z = precond_of(q, pre, R, f, r, g, ph);
+      st3(q.g, r, g);
+      st3(q.phi, r, ph);
+      st3(q.z, r, z);
+      st3(q.p, r, z);
+      q.face[r] = f;
+      if (op == OP_EXP3) st3(q.y, r, ld3(q.py, r));
+      break;
+    }
+    case OP_TOP: {
+      // computeGradients (MPRGPSolver.h:163-173) + proportional (:174-181)
+      Row R = ldrow(q, r);
+      bool f = q.face[r];
+      if (st.p_pending) {  // classic CGStep tail: p = z - beta p (MPRGPSolver.h:192-193)
+        st3(q.p, r, axmy(ld3(q.z, r), st.beta_cg, ld3(q.p, r)));
+      }
+      V3 g = ld3(q.g, r);
+      V3 ph = phi_of(R, f, g);
+      V3 be = beta_of(R, f, g, ph);
+      st3(q.phi, r, ph);
+      st3(q.beta, r, be);
+      V3 gp = add(ph, be);
+      P[0] = dot3(gp, gp);
+      P[1] = dot3(be, be);
+      V3 x = ld3(q.x, r);
+      V3 px = proj(R, axmy(x, st.alpha_bar, ph));
+      P[2] = dot3(sub(x, px), ph);
+      break;
+    }
+    case OP_CG_AP:
+    case OP_CL_AP: {
+      V3 acc[1];
+      const double *p = q.p;
+      spmv_row<1>(q, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(p, j); }, acc);
+      st3(q.ap, r, acc[0]);
+      V3 pr = ld3(q.p, r);
+      P[0] = dot3(pr, acc[0]);
+      P[1] = dot3(ld3(q.z, r), ld3(q.g, r));
+      if (op == OP_CL_AP) {
+        // stepLimit (MPRGPProjection.h:263-282)
+        Row R = ldrow(q, r);
+        double lim = 1.7976931348623157e308;
+        if (R.has) {
+          double jx = dot3(R.n, ld3(q.x, r)), jd = dot3(R.n, pr);
+          if (jd > kEps && jx > R.c) lim = (jx - R.c) / jd;
+        }
+        P[kMinSlot] = lim;
+      }
+      break;
+    }
+    case OP_CG_PY: {
+      Row R = ldrow(q, r);
+      V3 y = axmy(ld3(q.x, r), st.alpha, ld3(q.p, r));
+      st3(q.y, r, y);
+      st3(q.py, r, proj(R, y));
+      P[0] = infeasible(R, y);
+      break;
+    }
+    case OP_CG_Y: {
+      const double *py = q.py;
+      V3 pyr = ld3(q.py, r);
+      V3 br = ld3(q.b, r);
+      if (st.cg_entry) {
+        const double *x = q.x;
+        V3 acc[2];
+        spmv_row<2>(q, s, lane, [&](int j, V3 (&xs)[2]) {
+          xs[0] = ld3(py, j);
+          xs[1] = ld3(x, j);
+        }, acc);
+        V3 xr = ld3(q.x, r);
+        P[0] = dot3(pyr, acc[0]);
+        P[1] = dot3(pyr, br);
+        P[2] = dot3(xr, acc[1]);
+        P[3] = dot3(xr, br);
+      } else {
+        V3 acc[1];
+        spmv_row<1>(q, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(py, j); }, acc);
+        P[0] = dot3(pyr, acc[0]);
+        P[1] = dot3(pyr, br);
+      }
+      break;
+    }
+    case OP_CG_A: {
+      // CGMonotonicStep body, first half (MPRGPSolver.h:293-297)
+      Row R = ldrow(q, r);
+      bool f = q.face[r];
+      V3 ap = ld3(q.ap, r);
+      st3(q.x, r, ld3(q.y, r));
+      V3 g = axmy(ld3(q.g, r), st.alpha, ap);
+      V3 ph = phi_of(R, f, g);
+      V3 z = precond_of(q, pre, R, f, r, g, ph);
+      st3(q.g, r, g);
+      st3(q.phi, r, ph);
+      st3(q.z, r, z);
+      P[0] = dot3(z, ap);
+      break;
+    }
+    case OP_CG_B: {
+      // p = z - beta p ; computeGradients(g, false) ; proportional (MPRGPSolver.h:298-302)
+      Row R = ldrow(q, r);
+      bool f = q.face[r];
+      st3(q.p, r, axmy(ld3(q.z, r), st.beta_cg, ld3(q.p, r)));
+      V3 g = ld3(q.g, r);
+      V3 ph = ld3(q.phi, r);
+      V3 be = beta_of(R, f, g, ph);
+      st3(q.beta, r, be);
+      V3 gp = add(ph, be);
+      P[0] = dot3(gp, gp);
+      P[1] = dot3(be, be);
+      V3 x = ld3(q.x, r);
+      V3 px = proj(R, axmy(x, st.alpha_bar, ph));
+      P[2] = dot3(sub(x, px), ph);
+      break;
+    }
+    case OP_EXP1: {
+      Row R = ldrow(q, r);
+      st3(q.y, r, proj(R, ld3(q.x, r)));
+      break;
+    }
+    case OP_EXP2: {
+      // ExpMonotonicStep, first half (MPRGPSolver.h:255-261): the new y goes
+      // to py because neighbours still gather y in this launch.
+      V3 acc[1];
+      const double *y = q.y;
+      spmv_row<1>(q, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(y, j); }, acc);
+      V3 g = sub(acc[0], ld3(q.b, r));
+      Row R = ldrow(q, r);
+      V3 yr = ld3(q.y, r);
+      bool f = face_of(R, yr);
+      V3 ph = phi_of(R, f, g);
+      V3 yn = axmy(yr, st.alpha_bar, ph);
+      st3(q.g, r, g);
+      st3(q.py, r, yn);
+      st3(q.x, r, proj(R, yn));
+      q.face[r] = f;
+      P[0] = infeasible(R, yn);
+      break;
+    }
+    case OP_PROP1: {
+      // g = A x - b ; AD = A beta ; beta.AD ; g.beta  (MPRGPSolver.h:435-437, 224-231)
+      V3 acc[2];
+      const double *x = q.x, *be = q.beta;
+      spmv_row<2>(q, s, lane, [&](int j, V3 (&xs)[2]) {
+        xs[0] = ld3(x, j);
+        xs[1] = ld3(be, j);
+      }, acc);
+      V3 g = sub(acc[0], ld3(q.b, r));
+      st3(q.g, r, g);
+      st3(q.ap, r, acc[1]);
+      V3 d = ld3(q.beta, r);
+      P[0] = dot3(d, acc[1]);
+      P[1] = dot3(g, d);
+      break;
+    }
+    case OP_CL_PROP1: {
+      V3 acc[1];
+      const double *be = q.beta;
+      spmv_row<1>(q, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(be, j); }, acc);
+      st3(q.ap, r, acc[0]);
+      V3 d = ld3(q.beta, r);
+      P[0] = dot3(d, acc[0]);
+      P[1] = dot3(ld3(q.g, r), d);
+      break;
+    }
+    case OP_PROP2: {
+      // PropStep tail (MPRGPSolver.h:232-241)
+      Row R = ldrow(q, r);
+      V3 xn = proj(R, axmy(ld3(q.x, r), st.alpha, ld3(q.beta, r)));
+      V3 g = axmy(ld3(q.g, r), st.alpha, ld3(q.ap, r));
+      bool f = face_of(R, xn);
+      V3 ph = phi_of(R, f, g);
+      V3 z = precond_of(q, pre, R, f, r, g, ph);
+      st3(q.x, r, xn);
+      st3(q.g, r, g);
+      st3(q.phi, r, ph);
+      st3(q.z, r, z);
+      st3(q.p, r, z);
+      q.face[r] = f;
+      break;
+    }
+    case OP_CL_CG: {
+      // CGStep (MPRGPSolver.h:188-191); p is updated at the next TOP
+      Row R = ldrow(q, r);
+      bool f = q.face[r];
+      V3 ap = ld3(q.ap, r);
+      st3(q.x, r, axmy(ld3(q.x, r), st.alpha, ld3(q.p, r)));
+      V3 g = axmy(ld3(q.g, r), st.alpha, ap);
+      V3 ph = phi_of(R, f, g);
+      V3 z = precond_of(q, pre, R, f, r, g, ph);
+      st3(q.g, r, g);
+      st3(q.phi, r, ph);
+      st3(q.z, r, z);
+      P[0] = dot3(z, ap);
+      break;
+    }
+    case OP_CL_EXP: {
+      // ExpStep first half (MPRGPSolver.h:202-208)
+      Row R = ldrow(q, r);
+      V3 xt = axmy(ld3(q.x, r), st.alpha_f, ld3(q.p, r));
+      V3 g = axmy(ld3(q.g, r), st.alpha_f, ld3(q.ap, r));
+      bool f = face_of(R, xt);
+      V3 ph = phi_of(R, f, g);
+      xt = axmy(xt, st.alpha_bar, ph);
+      st3(q.x, r, proj(R, xt));
+      st3(q.g, r, g);
+      q.face[r] = f;
+      break;
+    }
+    case OP_FINAL: {
+      if (st.variant != VAR_CLASSIC) {
+        Row R = ldrow(q, r);
+        st3(q.x, r, proj(R, ld3(q.x, r)));
+      }
+      P[0] = q.face[r] ? 1.0 : 0.0;
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+template <int TS>
+__global__ void __launch_bounds__(TS * kSlice) mprgp_step(const QpDev *__restrict__ qps,
+                                                           const Tile *__restrict__ tiles,
+                                                           const int *__restrict__ n_tiles_dev,
+                                                           int *running) {
+  __shared__ double red[TS][kNumPart];
+  __shared__ int last;
+  __shared__ int sop;
+  const int n_tiles = *n_tiles_dev;
+  const int wave = threadIdx.x / kSlice;
+  const int lane = threadIdx.x % kSlice;
+  for (int ti = blockIdx.x; ti < n_tiles; ti += gridDim.x) {
+    const Tile t = tiles[ti];
+    const QpDev &q = qps[t.qp];
+    if (threadIdx.x == 0) sop = q.st->op;
+    __syncthreads();
+    const int op = sop;
+    if (op == OP_DONE) {
+      __syncthreads();
+      continue;
+    }
+    const QpState st = *q.st;
+    double P[kNumPart];
+#pragma unroll
+    for (int j = 0; j < kNumPart; ++j) P[j] = 0.0;
+    P[kMinSlot] = 1.7976931348623157e308;
+    tile_work<TS>(q, t, op, st, P);
+    // wave then block reduction in a fixed order
+#pragma unroll
+    for (int j = 0; j < kMinSlot; ++j) P[j] = wave_sum(P[j]);
+    P[kMinSlot] = wave_min(P[kMinSlot]);
+    if (lane == 0) {
+#pragma unroll
+      for (int j = 0; j < kNumPart; ++j) red[wave][j] = P[j];
+    }
+    __syncthreads();
+    if (wave == 0) {
+      if (lane < kNumPart) {
+        const int j = lane;
+        double acc = red[0][j];
+        for (int w = 1; w < TS; ++w) acc = (j == kMinSlot) ? fmin(acc, red[w][j]) : acc + red[w][j];
+        q.part[(size_t)t.local * kNumPart + j] = acc;
+      }
+      __threadfence();  // release the tile's partials before arriving
+      if (lane == 0) {
+        int old = atomicAdd(&q.st->arrive, 1);
+        last = (old == st.n_tiles - 1);
+      }
+    }
+    __syncthreads();
+    if (last && wave == 0) {
+      __threadfence();
+      // lane j-th partial: each lane sums tiles lane, lane+64, ... in order
+      Sums S;
+#pragma unroll
+      for (int j = 0; j < kNumPart; ++j) {
+        double acc = (j == kMinSlot) ? 1.7976931348623157e308 : 0.0;
+        for (int k = lane; k < st.n_tiles; k += kSlice) {
+          double v = __hip_atomic_load(q.part + (size_t)k * kNumPart + j, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+          acc = (j == kMinSlot) ? fmin(acc, v) : acc + v;
+        }
+        S.v[j] = (j == kMinSlot) ? wave_min(acc) : wave_sum(acc);
+      }
+      if (lane == 0) {
+        QpState &ws = *q.st;
+        ws.arrive = 0;
+        ws.steps++;
+        control(ws, S, running);
+        __threadfence();
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------- helper kernels
+
+// out[perm_i] = in[i] style permutations between user and internal order.
+__global__ void gather_nodes(const double *__restrict__ in, double *__restrict__ out,
+                             const int *__restrict__ perm, int n_nodes) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_nodes) return;
+  int o = perm[i];
+  out[3 * (size_t)i + 0] = in[3 * (size_t)o + 0];
+  out[3 * (size_t)i + 1] = in[3 * (size_t)o + 1];
+  out[3 * (size_t)i + 2] = in[3 * (size_t)o + 2];
+}
+__global__ void scatter_nodes(const double *__restrict__ in, double *__restrict__ out,
+                              const int *__restrict__ perm, int n_nodes) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_nodes) return;
+  int o = perm[i];
+  out[3 * (size_t)o + 0] = in[3 * (size_t)i + 0];
+  out[3 * (size_t)o + 1] = in[3 * (size_t)i + 1];
+  out[3 * (size_t)o + 2] = in[3 * (size_t)i + 2];
+}
+
+// dinv = 1/diag(A) from the diagonal block of each row; cs = 1/((n o dinv).n)
+// (MPRGPPrecondition.h:44-49, 81-88).
+__global__ void setup_diag(const int *__restrict__ sl_off, const int *__restrict__ col,
+                           const double *__restrict__ val, double *__restrict__ dinv, int n_nodes) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_nodes) return;
+  int s = r / kSlice, lane = r % kSlice;
+  int b0 = sl_off[s];
+  int W = (sl_off[s + 1] - b0) / kSlice;
+  double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+  for (int k = 0; k < W; ++k) {
+    if (col[b0 + k * kSlice + lane] == r) {
+      const double *v = val + (size_t)b0 * 9 + lane + (size_t)k * 9 * kSlice;
+      d0 = v[0 * kSlice];
+      d1 = v[4 * kSlice];
+      d2 = v[8 * kSlice];
+      break;
+    }
+  }
+  dinv[3 * (size_t)r + 0] = 1.0 / d0;
+  dinv[3 * (size_t)r + 1] = 1.0 / d1;
+  dinv[3 * (size_t)r + 2] = 1.0 / d2;
+}
+__global__ void setup_rows(const double *__restrict__ dinv, const double *__restrict__ cn,
+                           const unsigned char *__restrict__ cf, double *__restrict__ cs, int n_nodes) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_nodes) return;
+  if (!cf[r]) {
+    cs[r] = 0.0;
+    return;
+  }
+  double s = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    double n = cn[3 * (size_t)r + d];
+    s += (n * dinv[3 * (size_t)r + d]) * n;
+  }
+  cs[r] = 1.0 / s;
+}
+
+// Plain SELL SpMV y = A x (used by tests and by the assembly's A x1 term).
+__global__ void __launch_bounds__(256) spmv_plain(const int *__restrict__ sl_off,
+                                                  const int *__restrict__ col,
+                                                  const double *__restrict__ val,
+                                                  const double *__restrict__ x, double *__restrict__ y,
+                                                  int n_nodes, int accumulate) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = r / kSlice, lane = r % kSlice;
+  if (s * kSlice >= n_nodes) return;
+  QpDev q{};
+  q.sl_off = sl_off;
+  q.col = col;
+  q.val = val;
+  V3 acc[1];
+  spmv_row<1>(q, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
+  if (r < n_nodes) {
+    if (accumulate) acc[0] = add(acc[0], ld3(y, r));
+    st3(y, r, acc[0]);
+  }
+}
+
+// Host-callable launch wrappers ------------------------------------------
+
+hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
+                       const int *n_tiles, int *running) {
+  switch (ts) {
+    case 1: mprgp_step<1><<<grid, 1 * kSlice, 0, s>>>(qps, tiles, n_tiles, running); break;
+    case 2: mprgp_step<2><<<grid, 2 * kSlice, 0, s>>>(qps, tiles, n_tiles, running); break;
+    case 4: mprgp_step<4><<<grid, 4 * kSlice, 0, s>>>(qps, tiles, n_tiles, running); break;
+    case 8: mprgp_step<8><<<grid, 8 * kSlice, 0, s>>>(qps, tiles, n_tiles, running); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n) {
+  if (n <= 0) return hipSuccess;
+  gather_nodes<<<(n + 255) / 256, 256, 0, s>>>(in, out, perm, n);
+  return hipGetLastError();
+}
+hipError_t launch_scatter_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n) {
+  if (n <= 0) return hipSuccess;
+  scatter_nodes<<<(n + 255) / 256, 256, 0, s>>>(in, out, perm, n);
+  return hipGetLastError();
+}
+hipError_t launch_setup_diag(hipStream_t s, const int *sl_off, const int *col, const double *val,
+                             double *dinv, int n) {
+  if (n <= 0) return hipSuccess;
+  setup_diag<<<(n + 255) / 256, 256, 0, s>>>(sl_off, col, val, dinv, n);
+  return hipGetLastError();
+}
+hipError_t launch_setup_rows(hipStream_t s, const double *dinv, const double *cn,
+                             const unsigned char *cf, double *cs, int n) {
+  if (n <= 0) return hipSuccess;
+  setup_rows<<<(n + 255) / 256, 256, 0, s>>>(dinv, cn, cf, cs, n);
+  return hipGetLastError();
+}
+hipError_t launch_spmv_plain(hipStream_t s, const int *sl_off, const int *col, const double *val,
+                             const double *x, double *y, int n, int accumulate) {
+  if (n <= 0) return hipSuccess;
+  int slices = (n + kSlice - 1) / kSlice;
+  int threads = slices * kSlice;
+  spmv_plain<<<(threads + 255) / 256, 256, 0, s>>>(sl_off, col, val, x, y, n, accumulate);
+  return hipGetLastError();
+}
+
+}  // namespace mprgp

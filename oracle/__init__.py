@@ -1,0 +1,315 @@
+"""CPU oracle for the MPRGP contact-QP path -- TEST INFRASTRUCTURE ONLY.
+
+ctypes bindings to ``oracle/liboracle.so`` (built from ``mprgp_oracle.c`` and
+``fem_oracle.c`` by ``oracle/Makefile``).  Only the tests, the smoke check and
+the CPU-baseline leg of the benchmark import this package; the product library
+never does.  The C sources state, function by function, which reference
+file:line they restate and how each function is pinned.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import scipy.sparse as sp
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+MONOTONIC = 0
+CLASSIC = 1
+LINEAR = 1
+COROTATIONAL = 2
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    cmd = ["make", "-s", "-C", _HERE]
+    if force:
+        cmd.insert(2, "-B")
+    subprocess.run(cmd, check=True)
+    return _LIB_PATH
+
+
+class _CSR(C.Structure):
+    _fields_ = [("rows", C.c_int), ("cols", C.c_int), ("ptr", C.c_void_p),
+                ("idx", C.c_void_p), ("val", C.c_void_p)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("code", C.c_int), ("iterations", C.c_int), ("num_cg", C.c_int),
+                ("num_exp", C.c_int), ("num_prop", C.c_int), ("active", C.c_int),
+                ("power_iters", C.c_int), ("residual", C.c_double),
+                ("alpha_bar", C.c_double), ("lambda_max", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        P = C.c_void_p
+        D = C.c_double
+        I = C.c_int
+        L.orc_spmv.argtypes = [C.POINTER(_CSR), P, P]
+        L.orc_func_value.argtypes = [C.POINTER(_CSR), P, P]
+        L.orc_func_value.restype = D
+        L.orc_projector_new.argtypes = [C.POINTER(_CSR), P]
+        L.orc_projector_new.restype = P
+        L.orc_projector_free.argtypes = [P]
+        L.orc_projector_is_decoupled.argtypes = [P]
+        L.orc_project.argtypes = [P, P, P]
+        L.orc_decide_face.argtypes = [P, P]
+        L.orc_is_feasible.argtypes = [P, P]
+        L.orc_phi.argtypes = [P, P, P]
+        L.orc_beta.argtypes = [P, P, P, P]
+        L.orc_phitphi.argtypes = [P, P, D, P]
+        L.orc_phitphi.restype = D
+        L.orc_step_limit.argtypes = [P, P, P, D]
+        L.orc_step_limit.restype = D
+        L.orc_face.argtypes = [P]
+        L.orc_face.restype = C.POINTER(C.c_char)
+        L.orc_num_active.argtypes = [P]
+        L.orc_set_face.argtypes = [P, P]
+        L.orc_precond_new.argtypes = [C.POINTER(_CSR), P, I]
+        L.orc_precond_new.restype = P
+        L.orc_precond_free.argtypes = [P]
+        L.orc_precond_solve.argtypes = [P, P, P, P, P]
+        L.orc_spec_rad.argtypes = [C.POINTER(_CSR), P, P, D, C.c_ulonglong, P]
+        L.orc_spec_rad.restype = D
+        L.orc_decoupled_solve.argtypes = [C.POINTER(_CSR), P, C.POINTER(_CSR), P, P, D, I, I, I,
+                                          D, P, P, I, C.POINTER(Stats)]
+        L.orc_fem_setup.argtypes = [I, I, P, P, P, P, P]
+        L.orc_polar_rotation.argtypes = [P, P]
+        L.orc_fem_element.argtypes = [I, P, P, D, D, D, P, P]
+        L.orc_fem_element.restype = D
+        L.orc_fem_build_system.argtypes = [I, I, P, P, P, P, P, P, D, P, P, P, D, I,
+                                           C.POINTER(C.POINTER(C.c_int)),
+                                           C.POINTER(C.POINTER(C.c_int)),
+                                           C.POINTER(C.POINTER(C.c_double)), P]
+        L.orc_free.argtypes = [P]
+        _lib = L
+    return _lib
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class _Csr:
+    """Keeps the numpy buffers of a CSR alive while C holds pointers."""
+
+    def __init__(self, M, cols=None):
+        M = sp.csr_matrix(M)
+        M.sort_indices()
+        self.ptr = _i32(M.indptr)
+        self.idx = _i32(M.indices)
+        self.val = _f64(M.data)
+        self.s = _CSR(M.shape[0], M.shape[1] if cols is None else cols,
+                      _ptr(self.ptr), _ptr(self.idx), _ptr(self.val))
+
+
+def spmv(A, x):
+    a = _Csr(A)
+    x = _f64(x)
+    y = np.empty(A.shape[0])
+    lib().orc_spmv(C.byref(a.s), _ptr(x), _ptr(y))
+    return y
+
+
+def func_value(A, x, b):
+    a = _Csr(A)
+    x, b = _f64(x), _f64(b)
+    return lib().orc_func_value(C.byref(a.s), _ptr(x), _ptr(b))
+
+
+class Projector:
+    """DecoupledConProjector restated (MPRGPProjection.h:320-397)."""
+
+    def __init__(self, J, c):
+        self._J = _Csr(J)
+        self.c = _f64(c)
+        self.n = J.shape[1]
+        self.m = J.shape[0]
+        self.h = lib().orc_projector_new(C.byref(self._J.s), _ptr(self.c))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_projector_free(self.h)
+            self.h = None
+
+    def is_decoupled(self):
+        return bool(lib().orc_projector_is_decoupled(self.h))
+
+    def project(self, x):
+        x = _f64(x)
+        y = np.empty(self.n)
+        lib().orc_project(self.h, _ptr(x), _ptr(y))
+        return y
+
+    def decide_face(self, x):
+        x = _f64(x)
+        lib().orc_decide_face(self.h, _ptr(x))
+        return self.face()
+
+    def face(self):
+        f = lib().orc_face(self.h)
+        return np.array([f[i][0] if isinstance(f[i], bytes) else f[i] for i in range(self.m)],
+                        dtype=np.int8) if self.m else np.zeros(0, np.int8)
+
+    def set_face(self, face):
+        f = np.ascontiguousarray(face, dtype=np.int8)
+        lib().orc_set_face(self.h, _ptr(f))
+
+    def num_active(self):
+        return lib().orc_num_active(self.h)
+
+    def is_feasible(self, x):
+        x = _f64(x)
+        return bool(lib().orc_is_feasible(self.h, _ptr(x)))
+
+    def phi(self, g):
+        g = _f64(g)
+        out = np.empty(self.n)
+        lib().orc_phi(self.h, _ptr(g), _ptr(out))
+        return out
+
+    def beta(self, g, phi):
+        g, phi = _f64(g), _f64(phi)
+        out = np.empty(self.n)
+        lib().orc_beta(self.h, _ptr(g), _ptr(phi), _ptr(out))
+        return out
+
+    def phitphi(self, x, alpha_bar, phi):
+        x, phi = _f64(x), _f64(phi)
+        return lib().orc_phitphi(self.h, _ptr(x), float(alpha_bar), _ptr(phi))
+
+    def step_limit(self, x, d, alpha_cg=np.finfo(np.float64).max):
+        x, d = _f64(x), _f64(d)
+        return lib().orc_step_limit(self.h, _ptr(x), _ptr(d), float(alpha_cg))
+
+
+class Precond:
+    """DiagonalDecouplePreconSolver restated (MPRGPPrecondition.h:72-110)."""
+
+    def __init__(self, A, projector: Projector, preconditioned=True):
+        self._A = _Csr(A)
+        self.P = projector
+        self.n = A.shape[0]
+        self.h = lib().orc_precond_new(C.byref(self._A.s), projector.h, int(preconditioned))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_precond_free(self.h)
+            self.h = None
+
+    def solve(self, g, phi):
+        g, phi = _f64(g), _f64(phi)
+        z = np.empty(self.n)
+        lib().orc_precond_solve(self.h, self.P.h, _ptr(g), _ptr(z), _ptr(phi))
+        return z
+
+
+def spec_rad(A, eps, ev=None, seed=0x5EED):
+    """MPRGPBase::specRad restated; returns (lambda, iterations, ev_out or None)."""
+    a = _Csr(A)
+    n = A.shape[0]
+    evb = np.zeros(n) if ev is None else _f64(ev).copy()
+    valid = C.c_int(0 if ev is None else 1)
+    it = C.c_int(0)
+    lam = lib().orc_spec_rad(C.byref(a.s), _ptr(evb), C.byref(valid), float(eps),
+                             C.c_ulonglong(seed), C.byref(it))
+    return lam, it.value, (evb if valid.value else None)
+
+
+def decoupled_solve(A, B, J, c, x, tol=1e-3, max_it=1000, variant=MONOTONIC,
+                    preconditioned=True, alpha_bar=0.0, ev=None, project_first=True):
+    """MPRGPDecoupledCon<double>::solve restated.  Returns (code, x, stats-dict, ev)."""
+    a = _Csr(A)
+    j = _Csr(J, cols=A.shape[0])
+    B = _f64(B)
+    c = _f64(c)
+    x = _f64(x).copy()
+    n = A.shape[0]
+    evb = np.zeros(n) if ev is None else _f64(ev).copy()
+    valid = C.c_int(0 if ev is None else 1)
+    st = Stats()
+    code = lib().orc_decoupled_solve(C.byref(a.s), _ptr(B), C.byref(j.s), _ptr(c), _ptr(x),
+                                     float(tol), int(max_it), int(variant), int(preconditioned),
+                                     float(alpha_bar), _ptr(evb), C.byref(valid),
+                                     int(project_first), C.byref(st))
+    return code, x, st.as_dict(), (evb if valid.value else None)
+
+
+# ------------------------------------------------------------------ FEM
+
+def fem_setup(n_nodes, tets, X0):
+    tets = _i32(tets).copy()
+    X0 = _f64(X0)
+    T = tets.shape[0]
+    d = np.empty((T, 9))
+    vol = np.empty(T)
+    mass = np.empty(n_nodes)
+    lib().orc_fem_setup(int(n_nodes), int(T), _ptr(tets), _ptr(X0), _ptr(d), _ptr(vol), _ptr(mass))
+    return tets, d, vol, mass
+
+
+def polar_rotation(F):
+    F = _f64(F).reshape(9)
+    R = np.empty(9)
+    lib().orc_polar_rotation(_ptr(F), _ptr(R))
+    return R.reshape(3, 3)
+
+
+def fem_element(mtype, xe, d, vol, lam, mu):
+    xe = _f64(xe).reshape(12)
+    d = _f64(d).reshape(9)
+    fB = np.empty(12)
+    HB = np.empty(144)
+    E = lib().orc_fem_element(int(mtype), _ptr(xe), _ptr(d), float(vol), float(lam), float(mu),
+                              _ptr(fB), _ptr(HB))
+    return E, fB, HB.reshape(12, 12)
+
+
+def fem_build_system(n_nodes, tets_oriented, d, vol, mass, young, poisson, density,
+                     x_n, v_n, gravity, h, mtype):
+    """Returns (A as scipy CSR, b)."""
+    tets = _i32(tets_oriented)
+    T = tets.shape[0]
+    d, vol, mass = _f64(d), _f64(vol), _f64(mass)
+    young = _f64(np.broadcast_to(young, (T,)))
+    poisson = _f64(np.broadcast_to(poisson, (T,)))
+    x_n, v_n, gravity = _f64(x_n), _f64(v_n), _f64(gravity)
+    n = 3 * n_nodes
+    b = np.empty(n)
+    pp = C.POINTER(C.c_int)()
+    ip = C.POINTER(C.c_int)()
+    vp = C.POINTER(C.c_double)()
+    L = lib()
+    L.orc_fem_build_system(int(n_nodes), int(T), _ptr(tets), _ptr(d), _ptr(vol), _ptr(mass),
+                           _ptr(young), _ptr(poisson), float(density), _ptr(x_n), _ptr(v_n),
+                           _ptr(gravity), float(h), int(mtype), C.byref(pp), C.byref(ip),
+                           C.byref(vp), _ptr(b))
+    ptr = np.ctypeslib.as_array(pp, shape=(n + 1,)).copy()
+    nnz = int(ptr[-1])
+    idx = np.ctypeslib.as_array(ip, shape=(nnz,)).copy()
+    val = np.ctypeslib.as_array(vp, shape=(nnz,)).copy()
+    L.orc_free(C.cast(pp, C.c_void_p))
+    L.orc_free(C.cast(ip, C.c_void_p))
+    L.orc_free(C.cast(vp, C.c_void_p))
+    return sp.csr_matrix((val, idx, ptr), shape=(n, n)), b

@@ -1,0 +1,129 @@
+/*
+ * mprgp_oracle.h -- TEST INFRASTRUCTURE, NOT PRODUCT CODE.
+ *
+ * CPU FP64 restatement of the reference's decoupled-constraint MPRGP QP solve
+ * and of the FEM system assembly that produces its matrix.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and only
+ * as the checker or as the timed CPU baseline.  The product library never links
+ * or calls anything here.
+ *
+ * Reference files followed (paths relative to the reference root):
+ *   src/MPRGPSolver/MPRGPSolver.h       MPRGPBase / MPRGP / MPRGPMonotonic / FixedSparseMatrix / MPRGPDecoupledCon
+ *   src/MPRGPSolver/MPRGPProjection.h   BaseGeneralConProjector / DecoupledConProjector
+ *   src/MPRGPSolver/MPRGPPrecondition.h DiagonalInFacePreconSolver / DiagonalDecouplePreconSolver
+ *   src/MPRGPSolver/MPRGPUtility.h      COUNT_CONSTRAINTS, getDiagonal
+ *   src/ElasticSolver/FEMEnergy.cpp, FEMUtils.h, FEMMesh.cpp, FEMRotationUtil.h, FEMSystem.cpp
+ *   src/CollisionHandle/MprgpFemSolver.cpp
+ *
+ * Pin status of every function is stated in the header comment of
+ * mprgp_oracle.c / fem_oracle.c and checked by tests/test_oracle_*.py.
+ */
+#ifndef MPRGP_ORACLE_H
+#define MPRGP_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Compressed sparse rows.  For the system matrix A (square, symmetric) the
+ * column indices of each row are sorted ascending, which makes the row-wise
+ * dot product add terms in the same order as the reference's column-major
+ * scatter SpMV over a symmetric matrix. */
+typedef struct {
+  int rows, cols;
+  const int *ptr;
+  const int *idx;
+  const double *val;
+} orc_csr;
+
+typedef struct {
+  int code;          /* 0 converged, -1 not converged (reference return code) */
+  int iterations;    /* the reference's `iteration` counter ("total iter")      */
+  int num_cg, num_exp, num_prop;
+  int active;        /* COUNT_CONSTRAINTS(face) at exit                        */
+  int power_iters;   /* power-method iterations (0 when alpha_bar was given)   */
+  double residual;   /* residual_out = ||phi + beta||_2                        */
+  double alpha_bar;
+  double lambda_max; /* spectral-radius estimate used for alpha_bar            */
+} orc_stats;
+
+/* ---- matrix helpers -------------------------------------------------- */
+void orc_spmv(const orc_csr *A, const double *x, double *y);
+double orc_func_value(const orc_csr *A, const double *x, const double *b);
+void orc_get_diagonal(const orc_csr *A, double *diag);
+
+/* ---- projector (decoupled J: rows with disjoint column support) ------- */
+typedef struct orc_projector orc_projector;
+orc_projector *orc_projector_new(const orc_csr *J, const double *c);
+void orc_projector_free(orc_projector *P);
+int orc_projector_is_decoupled(const orc_projector *P);
+void orc_project(const orc_projector *P, const double *x, double *y);
+void orc_decide_face(orc_projector *P, const double *x);
+int orc_is_feasible(const orc_projector *P, const double *x);
+void orc_phi(const orc_projector *P, const double *g, double *phi);
+void orc_beta(const orc_projector *P, const double *g, const double *phi, double *beta);
+double orc_phitphi(const orc_projector *P, const double *x, double alpha_bar, const double *phi);
+double orc_step_limit(const orc_projector *P, const double *x, const double *d, double alpha_cg);
+const char *orc_face(const orc_projector *P);
+int orc_num_active(const orc_projector *P);
+void orc_set_face(orc_projector *P, const char *face);
+
+/* ---- preconditioner ---------------------------------------------------- */
+typedef struct orc_precond orc_precond;
+orc_precond *orc_precond_new(const orc_csr *A, const orc_projector *P, int preconditioned);
+void orc_precond_free(orc_precond *pc);
+void orc_precond_solve(const orc_precond *pc, const orc_projector *P, const double *g,
+                       double *z, const double *phi);
+
+/* ---- spectral radius (power method) ----------------------------------- */
+/* ev: in/out start vector of length n; *ev_valid says whether it holds one.
+ * When no start vector is given, a deterministic pseudo-random one from
+ * `seed` replaces the reference's setRandom(). */
+double orc_spec_rad(const orc_csr *A, double *ev, int *ev_valid, double eps,
+                    unsigned long long seed, int *iters_out);
+
+/* ---- whole solves ------------------------------------------------------ */
+enum { ORC_MONOTONIC = 0, ORC_CLASSIC = 1 };
+
+/* MPRGPDecoupledCon<double>::solve<FixedSparseMatrix,preconditioned>(A,B,J,c,x,tol,max_it,name,ev)
+ * project_first=1 reproduces the J/c overload (x is projected before the
+ * solve); 0 reproduces the projector overload.  alpha_bar > 0 is used as is,
+ * otherwise 2/specRad(A, ev, tol) as in the reference. */
+int orc_decoupled_solve(const orc_csr *A, const double *B, const orc_csr *J, const double *c,
+                        double *x, double tol, int max_it, int variant, int preconditioned,
+                        double alpha_bar, double *ev, int *ev_valid, int project_first,
+                        orc_stats *st);
+
+/* ---- FEM assembly (fem_oracle.c) -------------------------------------- */
+enum { ORC_LINEAR = 1, ORC_COROTATIONAL = 2 };
+
+/* FEMBody::assemble + makePositive + buildD: orients tets positively
+ * (tets is modified in place), fills d (9 per tet, row-major Dm^-1),
+ * vol (per tet) and node_mass (per node, sum V/4, not yet times density). */
+void orc_fem_setup(int n_nodes, int n_tets, int *tets, const double *X0,
+                   double *d, double *vol, double *node_mass);
+
+/* Polar rotation R = U V^T following adjustF3D (eigen-decomposition of F^T F). */
+void orc_polar_rotation(const double *F, double *R);
+
+/* MaterialEnergy::eval: element force fB (12) and stiffness HB (12x12,
+ * row-major) of one tet at current node positions xe (4x3, node-major). */
+double orc_fem_element(int type, const double *xe, const double *d, double vol,
+                       double lambda, double mu, double *fB, double *HB);
+
+/* MprgpFemSolver::buildLinearSystem for one body, first Newton iteration
+ * (x1 = x_n): A = M + h^2 K, b = h^2 f(x1) + h^2 M g + M (x_n + h v_n - x1) + A x1.
+ * young/poisson are per tet.  Returns the CSR in caller-freed arrays
+ * (orc_free).  Returns 0 on success. */
+int orc_fem_build_system(int n_nodes, int n_tets, const int *tets_oriented,
+                         const double *d, const double *vol, const double *node_mass,
+                         const double *young, const double *poisson, double density,
+                         const double *x_n, const double *v_n, const double *gravity,
+                         double h, int type,
+                         int **ptr_out, int **idx_out, double **val_out, double *b_out);
+void orc_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,251 @@
+"""GPU parity: the HIP solve through the C ABI against the CPU oracle.
+
+Bars (FP64 form): solution relative error <= 1e-8, the same active set except
+nodes within 1e-10 of their plane, objective relative error <= 1e-10; SpMV
+relative error <= 1e-13 per entry (different summation order and FMA)."""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import mprgp_amd as M
+import oracle
+from mprgp_amd import problems
+from tests.common import brute_force_qp, kkt_report, random_spd, scene_qp
+
+pytestmark = pytest.mark.gpu
+
+
+def make_qp(ctx, A, b, node, normal, c, x0, ev=None):
+    Ad = ctx.matrix_csr(A)
+    q = Ad.qp()
+    q.set_rows(node, normal, c)
+    q.set_rhs(b)
+    q.set_x(x0)
+    if ev is not None:
+        q.set_ev(ev)
+    return q
+
+
+def slack(x, node, normal, c):
+    return np.einsum("ij,ij->i", normal, x.reshape(-1, 3)[node]) - c
+
+
+def assert_parity(A, b, node, normal, c, x_gpu, x_ref, st_gpu, st_ref):
+    rel = np.linalg.norm(x_gpu - x_ref) / np.linalg.norm(x_ref)
+    assert rel <= 1e-8, rel
+    f = lambda x: 0.5 * x @ (A @ x) - b @ x
+    fg, fr = f(x_gpu), f(x_ref)
+    assert abs(fg - fr) <= 1e-10 * abs(fr), (fg, fr)
+    sg, sr = slack(x_gpu, node, normal, c), slack(x_ref, node, normal, c)
+    act_g, act_r = np.abs(sg) < 1e-9, np.abs(sr) < 1e-9
+    differ = act_g != act_r
+    near = (np.abs(sg) < 1e-10) | (np.abs(sr) < 1e-10)
+    assert not np.any(differ & ~near), np.nonzero(differ & ~near)
+    assert st_gpu["code"] == st_ref["code"] == 0
+
+
+def test_spmv_matches_oracle(ctx):
+    A, b, node, normal, c, x0 = scene_qp(problems.config1())
+    Ad = ctx.matrix_csr(A)
+    rng = np.random.default_rng(0)
+    for _ in range(3):
+        x = rng.normal(size=A.shape[0])
+        y = Ad.spmv(x)
+        yr = oracle.spmv(A, x)
+        scale = np.abs(A) @ np.abs(x)
+        assert np.all(np.abs(y - yr) <= 1e-13 * scale + 1e-300)
+    info = Ad.info()
+    assert info["n_nodes"] == 525 and info["n_blocks"] == 525 + 2 * 2476
+
+
+def test_bsr3_and_csr_uploads_agree(ctx):
+    A, *_ = scene_qp(problems.config1())
+    N = A.shape[0] // 3
+    Ab = sp.bsr_matrix(A, blocksize=(3, 3))
+    Ab.sort_indices()
+    Mb = ctx.matrix_bsr3(N, Ab.indptr, Ab.indices, Ab.data.reshape(-1))
+    Mc = ctx.matrix_csr(A)
+    x = np.random.default_rng(1).normal(size=3 * N)
+    np.testing.assert_array_equal(Mb.spmv(x), Mc.spmv(x))
+
+
+@pytest.mark.parametrize("which", ["config1", "config1_flat"])
+def test_monotone_parity_given_alpha_bar(ctx, which):
+    sc = getattr(problems, which)()
+    A, b, node, normal, c, x0 = scene_qp(sc)
+    J = problems.rows_to_J(node, normal, sc.n_nodes)
+    lam, _, _ = oracle.spec_rad(A, 1e-10)
+    ab = 2.0 / lam
+    code, xr, sr, _ = oracle.decoupled_solve(A, b, J, c, x0, tol=1e-10, max_it=5000, alpha_bar=ab)
+    q = make_qp(ctx, A, b, node, normal, c, x0)
+    rc, sg = q.solve(tol=1e-10, max_it=5000, alpha_bar=ab)
+    xg = q.get_x()
+    print(which, "oracle", sr, "gpu", sg)
+    assert_parity(A, b, node, normal, c, xg, xr, sg, sr)
+    assert sg["residual"] <= 1e-10 and sg["active"] == sr["active"]
+
+
+@pytest.mark.parametrize("which", ["config1", "config1_flat"])
+def test_monotone_same_step_sequence(ctx, which):
+    # at tol 1e-6 every branch (fy <= fx, proportionality, feasibility of y)
+    # is decided far from rounding, so GPU and oracle take the same steps
+    sc = getattr(problems, which)()
+    A, b, node, normal, c, x0 = scene_qp(sc)
+    J = problems.rows_to_J(node, normal, sc.n_nodes)
+    ab = 2.0 / oracle.spec_rad(A, 1e-10)[0]
+    code, xr, sr, _ = oracle.decoupled_solve(A, b, J, c, x0, tol=1e-6, max_it=5000, alpha_bar=ab)
+    q = make_qp(ctx, A, b, node, normal, c, x0)
+    rc, sg = q.solve(tol=1e-6, max_it=5000, alpha_bar=ab)
+    print(which, "oracle", sr, "gpu", sg)
+    for k in ("code", "iterations", "num_cg", "num_exp", "num_prop", "active"):
+        assert sg[k] == sr[k], (k, sg, sr)
+    rel = np.linalg.norm(q.get_x() - xr) / np.linalg.norm(xr)
+    assert rel <= 1e-10, rel
+
+
+def test_ill_conditioned_parity(ctx):
+    # config 3: per-tet E log-uniform over 4 decades (kappa ~ 1e6); branch
+    # decisions are rounding-sensitive, so only the solution is compared
+    sc = problems.config3()
+    A, b, node, normal, c, x0 = scene_qp(sc)
+    J = problems.rows_to_J(node, normal, sc.n_nodes)
+    ab = 2.0 / oracle.spec_rad(A, 1e-10)[0]
+    code, xr, sr, _ = oracle.decoupled_solve(A, b, J, c, x0, tol=1e-8, max_it=20000, alpha_bar=ab)
+    q = make_qp(ctx, A, b, node, normal, c, x0)
+    rc, sg = q.solve(tol=1e-8, max_it=20000, alpha_bar=ab)
+    print("config3 oracle", sr, "gpu", sg)
+    assert_parity(A, b, node, normal, c, q.get_x(), xr, sg, sr)
+
+
+def test_power_method_matches_oracle(ctx):
+    sc = problems.config1()
+    A, b, node, normal, c, x0 = scene_qp(sc)
+    J = problems.rows_to_J(node, normal, sc.n_nodes)
+    code, xr, sr, evr = oracle.decoupled_solve(A, b, J, c, x0, tol=1e-8, max_it=5000)
+    q = make_qp(ctx, A, b, node, normal, c, x0)
+    rc, sg = q.solve(tol=1e-8, max_it=5000)
+    assert sg["power_iters"] == sr["power_iters"]
+    assert sg["lambda_max"] == pytest.approx(sr["lambda_max"], rel=1e-10)
+    assert sg["ev_valid"] == (evr is not None)
+    assert_parity(A, b, node, normal, c, q.get_x(), xr, sg, sr)
+    # reference: power_ev carried into the next solve (MprgpFemSolver.cpp:533)
+    if evr is not None:
+        q.reset_x()
+        rc2, sg2 = q.solve(tol=1e-8, max_it=5000)
+        assert sg2["power_iters"] <= 2
+
+
+def test_classic_variant_parity(ctx):
+    sc = problems.config1_flat()
+    A, b, node, normal, c, x0 = scene_qp(sc)
+    J = problems.rows_to_J(node, normal, sc.n_nodes)
+    ab = 2.0 / oracle.spec_rad(A, 1e-10)[0]
+    code, xr, sr, _ = oracle.decoupled_solve(A, b, J, c, x0, tol=1e-8, max_it=5000, alpha_bar=ab,
+                                             variant=oracle.CLASSIC)
+    q = make_qp(ctx, A, b, node, normal, c, x0)
+    rc, sg = q.solve(tol=1e-8, max_it=5000, alpha_bar=ab, variant=M.CLASSIC)
+    assert_parity(A, b, node, normal, c, q.get_x(), xr, sg, sr)
+    assert (sg["num_cg"], sg["num_exp"], sg["num_prop"]) == (sr["num_cg"], sr["num_exp"], sr["num_prop"])
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_tiny_qp_matches_brute_force(ctx, seed):
+    rng = np.random.default_rng(200 + seed)
+    N, m = 6, 4
+    A = random_spd(rng, 3 * N, 30.0) + 0.5 * np.eye(3 * N)
+    b = rng.normal(size=3 * N) * 3
+    node = rng.choice(N, m, replace=False).astype(np.int32)
+    normal = rng.normal(size=(m, 3))
+    normal /= np.linalg.norm(normal, axis=1, keepdims=True)
+    c = rng.normal(size=m)
+    x_bf, f_bf, _ = brute_force_qp(A, b, node, normal, c)
+    q = make_qp(ctx, sp.csr_matrix(A), b, node, normal, c, rng.normal(size=3 * N))
+    rc, st = q.solve(tol=1e-8, max_it=10000, project_first=1)
+    assert rc == 0, st
+    lmin = np.linalg.eigvalsh(A)[0]
+    np.testing.assert_allclose(q.get_x(), x_bf, atol=4e-8 / lmin)
+
+
+def test_unconstrained_matches_cholesky(ctx):
+    rng = np.random.default_rng(6)
+    A = random_spd(rng, 45, 200.0) + np.eye(45)
+    b = rng.normal(size=45)
+    q = make_qp(ctx, sp.csr_matrix(A), b, np.zeros(0, np.int32), np.zeros((0, 3)), np.zeros(0),
+                np.zeros(45))
+    rc, st = q.solve(tol=1e-10, max_it=1000)
+    assert rc == 0
+    np.testing.assert_allclose(q.get_x(), np.linalg.solve(A, b), rtol=1e-8, atol=1e-10)
+
+
+def test_reference_facade_on_test_solver_case():
+    # test_case/MPRGPTest.cpp:70-98 with per-node rows (5 nodes, axis normals)
+    rng = np.random.default_rng(7)
+    N = 5
+    Mx = rng.uniform(-1, 1, (3 * N, 3 * N)) + 3 * np.eye(3 * N)
+    A = sp.csr_matrix(Mx.T @ Mx)
+    node = np.arange(N, dtype=np.int32)
+    normal = np.eye(3)[rng.integers(0, 3, N)]
+    J = problems.rows_to_J(node, normal, N)
+    B, c, x = rng.uniform(-1, 1, 3 * N), rng.uniform(-1, 1, N), rng.uniform(-1, 1, 3 * N)
+    xr = x.copy()
+    code_r, yr, _, _ = oracle.decoupled_solve(A, B, J, c, xr, tol=1e-8, max_it=1000)
+    st = {}
+    code = M.MPRGPDecoupledCon.solve(A, B, J, c, x, 1e-8, 1000, "decoupled MPRGP", None, stats_out=st)
+    assert code == 0 == code_r
+    assert np.all(slack(x, node, normal, c) >= -1e-9)
+    np.testing.assert_allclose(x, yr, atol=1e-7)
+
+
+def test_edge_cases(ctx):
+    sc = problems.config1()
+    A, b, node, normal, c, x0 = scene_qp(sc)
+    q = make_qp(ctx, A, b, node, normal, c, x0)
+    rc, st = q.solve(tol=1e-8, max_it=0, alpha_bar=1e-3)
+    assert rc == 0 and st["iterations"] == 0            # residual_out starts at 0
+    q.reset_x()
+    rc, st = q.solve(tol=1e-8, max_it=3, alpha_bar=1e-3)
+    assert rc == M._lib.NOT_CONVERGED and st["code"] == -1
+    with pytest.raises(M.MprgpError) as e:
+        q.set_rows(np.array([0, 0], np.int32), np.array([[0, 1.0, 0], [0, 1.0, 0]]), np.zeros(2))
+    assert e.value.code == M._lib.EBADINPUT
+    with pytest.raises(M.MprgpError):
+        q.set_rows(np.array([1], np.int32), np.zeros((1, 3)), np.zeros(1))
+    Jbad = sp.csr_matrix(([1.0, 1.0], ([0, 0], [0, 3])), shape=(1, A.shape[0]))  # spans 2 nodes
+    with pytest.raises(M.MprgpError):
+        q.set_rows_sparse(Jbad, np.zeros(1))
+
+
+def test_batch_equals_alone_bitwise(ctx):
+    scenes = [problems.config1(seed=s) for s in (11, 12)] + [problems.config1_flat(seed=13)]
+    rng = np.random.default_rng(3)
+    qs, alone = [], []
+    for sc in scenes:
+        A, b, node, normal, c, x0 = scene_qp(sc)
+        q = make_qp(ctx, A, b, node, normal, c, x0)
+        rc, st = q.solve(tol=1e-9, max_it=3000, alpha_bar=2.0 / oracle.spec_rad(A, 1e-10)[0])
+        alone.append((q.get_x(), st))
+        qs.append((q, st["alpha_bar"]))
+    for q, _ in qs:
+        q.reset_x()
+    per = [M.params(tol=1e-9, max_it=3000, alpha_bar=ab) for _, ab in qs]
+    rc, stats = M.solve_batch(ctx, [q for q, _ in qs], per_qp=per)
+    for (q, _), (xa, sa), sb in zip(qs, alone, stats):
+        np.testing.assert_array_equal(q.get_x(), xa)
+        for k in ("iterations", "num_cg", "num_exp", "num_prop", "active", "residual"):
+            assert sb[k] == sa[k]
+
+
+def test_config2_full_size_parity(ctx):
+    sc = problems.config2()
+    A, b, node, normal, c, x0 = scene_qp(sc)
+    J = problems.rows_to_J(node, normal, sc.n_nodes)
+    ab = 2.0 / oracle.spec_rad(A, 1e-6)[0]
+    code, xr, sr, _ = oracle.decoupled_solve(A, b, J, c, x0, tol=sc.tol, max_it=sc.max_it, alpha_bar=ab)
+    q = make_qp(ctx, A, b, node, normal, c, x0)
+    rc, sg = q.solve(tol=sc.tol, max_it=sc.max_it, alpha_bar=ab)
+    print("config2 oracle", sr, "gpu", sg)
+    assert_parity(A, b, node, normal, c, q.get_x(), xr, sg, sr)
+    rep = kkt_report(A, b, node, normal, c, q.get_x())
+    assert rep["infeas"] <= 1e-12 and rep["neg_mult"] <= 1e-6
