@@ -45,6 +45,10 @@ extern "C" {
 #define MPRGP_MONOTONIC 0 /* MPRGPMonotonic (MPRGPSolver.h:400-456), the simulator's */
 #define MPRGP_CLASSIC 1   /* MPRGP with max-feasible-step test (MPRGPSolver.h:344-398) */
 
+/* FEM materials (MaterialEnergy TYPE values used on the path) */
+#define MPRGP_LINEAR 1
+#define MPRGP_COROTATIONAL 2
+
 /* precision forms */
 #define MPRGP_FP64 0  /* every vector and reduction in FP64                        */
 #define MPRGP_MIXED 1 /* FP32 SpMV/updates, FP64 step lengths, KKT norm, refinement */
@@ -52,6 +56,7 @@ extern "C" {
 typedef struct mprgp_ctx mprgp_ctx;
 typedef struct mprgp_matrix mprgp_matrix;
 typedef struct mprgp_qp mprgp_qp;
+typedef struct mprgp_fem mprgp_fem;
 
 typedef struct {
   double tol;          /* bound on ||phi + beta||_2, absolute (reference tol)        */
@@ -128,12 +133,15 @@ int mprgp_qp_set_rows(mprgp_qp *qp, int m, const int *node, const double *normal
 int mprgp_qp_set_rows_sparse(mprgp_qp *qp, int m, int col_major, const int *outer, const int *inner,
                              const double *val, const double *c);
 int mprgp_qp_set_rhs(mprgp_qp *qp, const double *b);
+int mprgp_qp_get_rhs(mprgp_qp *qp, double *b);       /* node order               */
 int mprgp_qp_set_x(mprgp_qp *qp, const double *x);   /* also kept as the reset point */
 int mprgp_qp_reset_x(mprgp_qp *qp);                  /* x = last set_x, on device   */
 int mprgp_qp_get_x(mprgp_qp *qp, double *x);
 int mprgp_qp_set_ev(mprgp_qp *qp, const double *ev); /* power_ev warm start (:47-48)*/
 int mprgp_qp_get_ev(mprgp_qp *qp, double *ev, int *valid);
 int mprgp_qp_get_face(mprgp_qp *qp, unsigned char *face_per_node);
+/* device launches the last solve spent in each internal operation (counts[0..n)) */
+int mprgp_qp_op_counts(mprgp_qp *qp, int *counts, int n);
 /* MPRGPDecoupledCon<double>::solve<FixedSparseMatrix, preconditioned>
  * (MPRGPSolver.h:544-559): x is read and overwritten on the device. */
 int mprgp_qp_solve(mprgp_qp *qp, const mprgp_params *params, mprgp_stats *stats);
@@ -152,6 +160,24 @@ int mprgp_decoupled_solve(int device, int n, const int *a_outer, const int *a_in
                           const double *a_val, const double *B, int m, const int *j_outer,
                           const int *j_inner, const double *j_val, const double *c, double *x,
                           double tol, int max_it, double *ev, int *ev_valid, mprgp_stats *stats);
+
+/* ---- FEM assembly on the device ------------------------------------------
+ * FEMSystem::buildSystem (FEMSystem.cpp:167-192) + MprgpFemSolver::
+ * buildLinearSystem (MprgpFemSolver.cpp:315-356) for one body at the first
+ * Newton iterate x1 = x_n:  A = M + h^2 K(x_n),
+ * b = h^2 f(x_n) + h^2 M g + M (x_n + h v_n - x_n) + A x_n,
+ * K from LINEAR or COROTATIONAL tets (MaterialEnergy::eval, FEMEnergy.cpp:263-304,
+ * 657-1057), lumped mass rho*sum(V/4) (FEMMesh.cpp:488-510).  The mesh is
+ * fixed at creation; tets are re-oriented to positive volume (makePositive,
+ * FEMMesh.cpp:158-165).  young/poisson per tet. */
+int mprgp_fem_create(mprgp_ctx *ctx, int n_nodes, int n_tets, const int *tets, const double *rest_pos,
+                     const double *young, const double *poisson, double density, mprgp_fem **out);
+/* Writes A into the FEM's matrix and b into qp (created on mprgp_fem_matrix).
+ * x_n, v_n: node positions and velocities (node order, host); gravity[3]. */
+int mprgp_fem_assemble(mprgp_fem *fem, int material, const double *x_n, const double *v_n,
+                       const double *gravity, double h, mprgp_qp *qp);
+mprgp_matrix *mprgp_fem_matrix(mprgp_fem *fem);
+void mprgp_fem_destroy(mprgp_fem *fem);
 
 #ifdef __cplusplus
 }

@@ -27,7 +27,7 @@ from . import _lib
 from ._lib import (CLASSIC, COROTATIONAL, FP64, LINEAR, MIXED, MONOTONIC, MprgpError, Params,
                    Stats, check, last_error, load)
 
-__all__ = ["Context", "Matrix", "QP", "MPRGPDecoupledCon", "solve_batch", "params",
+__all__ = ["Context", "Matrix", "QP", "Fem", "MPRGPDecoupledCon", "solve_batch", "params",
            "MprgpError", "MONOTONIC", "CLASSIC", "FP64", "MIXED", "LINEAR", "COROTATIONAL",
            "device_count"]
 
@@ -111,14 +111,47 @@ class Context:
         return m
 
 
+class Fem:
+    """Device FEM assembly of one body (buildLinearSystem at x1 = x_n)."""
+
+    def __init__(self, ctx: Context, rest_pos, tets, young, poisson, density):
+        X = _f64(rest_pos)
+        T = _i32(tets)
+        nt = T.size // 4
+        E = _f64(np.broadcast_to(np.asarray(young, np.float64), (nt,)))
+        nu = _f64(np.broadcast_to(np.asarray(poisson, np.float64), (nt,)))
+        self.h = C.c_void_p()
+        check(load().mprgp_fem_create(ctx.h, X.size // 3, nt, _p(T), _p(X), _p(E), _p(nu),
+                                      float(density), C.byref(self.h)))
+        self.ctx = ctx
+        self.n_nodes = X.size // 3
+        self.matrix = Matrix(ctx, C.c_void_p(load().mprgp_fem_matrix(self.h)), self.n_nodes)
+        self.matrix.owned_by_fem = True
+        ctx._objs.append(self)
+
+    def close(self):
+        if self.h:
+            load().mprgp_fem_destroy(self.h)
+            self.h = C.c_void_p()
+            self.matrix.h = C.c_void_p()
+
+    def assemble(self, qp: "QP", x_n, v_n, gravity=(0.0, -9.8, 0.0), h=0.01, material=LINEAR):
+        x = _f64(x_n, 3 * self.n_nodes)
+        v = _f64(v_n, 3 * self.n_nodes)
+        g = _f64(gravity, 3)
+        check(load().mprgp_fem_assemble(self.h, int(material), _p(x), _p(v), _p(g), float(h), qp.h))
+
+
 class Matrix:
     def __init__(self, ctx, h, n_nodes):
         self.ctx, self.h, self.n_nodes = ctx, h, n_nodes
 
+    owned_by_fem = False
+
     def close(self):
-        if self.h:
+        if self.h and not self.owned_by_fem:
             load().mprgp_matrix_destroy(self.h)
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
 
     def info(self):
         n, nb, by = C.c_int(), C.c_longlong(), C.c_longlong()
@@ -176,6 +209,11 @@ class QP:
         b = _f64(b, self.n)
         check(load().mprgp_qp_set_rhs(self.h, _p(b)))
 
+    def get_rhs(self):
+        b = np.empty(self.n)
+        check(load().mprgp_qp_get_rhs(self.h, _p(b)))
+        return b
+
     def set_x(self, x):
         x = _f64(x, self.n)
         check(load().mprgp_qp_set_x(self.h, _p(x)))
@@ -200,6 +238,11 @@ class QP:
         v = C.c_int()
         check(load().mprgp_qp_get_ev(self.h, _p(ev), C.byref(v)))
         return ev if v.value else None
+
+    def op_counts(self):
+        c = np.zeros(32, dtype=np.int32)
+        check(load().mprgp_qp_op_counts(self.h, _p(c), 32))
+        return c
 
     def get_face(self):
         f = np.empty(self.n // 3, dtype=np.uint8)

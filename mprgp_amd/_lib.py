@@ -9,7 +9,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmprgp_amd.so")
+LIB_PATH = os.environ.get("MPRGP_LIB") or os.path.join(HERE, "libmprgp_amd.so")
 
 OK = 0
 NOT_CONVERGED = -1
@@ -82,16 +82,22 @@ SIGNATURES = {
     "mprgp_qp_set_rows": (I, [P, I, P, P, P]),
     "mprgp_qp_set_rows_sparse": (I, [P, I, I, P, P, P, P]),
     "mprgp_qp_set_rhs": (I, [P, P]),
+    "mprgp_qp_get_rhs": (I, [P, P]),
     "mprgp_qp_set_x": (I, [P, P]),
     "mprgp_qp_reset_x": (I, [P]),
     "mprgp_qp_get_x": (I, [P, P]),
     "mprgp_qp_set_ev": (I, [P, P]),
     "mprgp_qp_get_ev": (I, [P, P, C.POINTER(I)]),
     "mprgp_qp_get_face": (I, [P, P]),
+    "mprgp_qp_op_counts": (I, [P, P, I]),
     "mprgp_qp_solve": (I, [P, C.POINTER(Params), C.POINTER(Stats)]),
     "mprgp_solve_batch": (I, [P, I, PP, C.POINTER(Params), I, C.POINTER(Stats)]),
     "mprgp_decoupled_solve": (I, [I, I, P, P, P, P, I, P, P, P, P, P, D, I, P, C.POINTER(I),
                                   C.POINTER(Stats)]),
+    "mprgp_fem_create": (I, [P, I, I, P, P, P, P, D, PP]),
+    "mprgp_fem_assemble": (I, [P, I, P, P, P, D, P]),
+    "mprgp_fem_matrix": (P, [P]),
+    "mprgp_fem_destroy": (None, [P]),
 }
 
 

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -67,6 +68,16 @@ void mprgp_ctx::release(void *p, size_t bytes) {
   if (bytes == 0) bytes = 256;
   (void)hipFree(p);
   in_use -= std::min(in_use, bytes);
+}
+
+int mprgp_ctx::scratch_reserve(size_t bytes) {
+  if (bytes <= scratch_bytes) return MPRGP_OK;
+  release(scratch, scratch_bytes);
+  scratch = nullptr;
+  scratch_bytes = 0;
+  if (int rc = alloc(&scratch, bytes)) return rc;
+  scratch_bytes = bytes;
+  return MPRGP_OK;
 }
 
 // ------------------------------------------------------------------ context
@@ -138,6 +149,7 @@ extern "C" void mprgp_ctx_destroy(mprgp_ctx *c) {
   c->release(c->d_qps, sizeof(QpDev) * (size_t)c->qps_cap);
   c->release(c->d_tiles, sizeof(Tile) * (size_t)c->tiles_cap);
   c->release(c->d_ctrl, 4 * sizeof(int));
+  c->release(c->scratch, c->scratch_bytes);
   if (c->h_running) (void)hipHostFree(c->h_running);
   if (c->t0) (void)hipEventDestroy(c->t0);
   if (c->t1) (void)hipEventDestroy(c->t1);
@@ -157,9 +169,9 @@ extern "C" int mprgp_ctx_memory(mprgp_ctx *c, size_t *in_use, size_t *budget) {
 // ------------------------------------------------------------------ matrix
 
 // Builds the SELL-64 layout from a node-order block CSR and uploads it.
-static int build_matrix(mprgp_ctx *ctx, int n_nodes, const int *rowptr, const int *colidx,
+int mprgp::build_matrix(mprgp_ctx *ctx, int n_nodes, const int *rowptr, const int *colidx,
                         const double *blocks, mprgp_matrix **out) {
-  if (!ctx || !out || n_nodes <= 0 || !rowptr || !colidx || !blocks)
+  if (!ctx || !out || n_nodes <= 0 || !rowptr || !colidx)
     return fail(MPRGP_EBADINPUT, "bad matrix arguments");
   *out = nullptr;
   const long long nb = rowptr[n_nodes];
@@ -176,8 +188,9 @@ static int build_matrix(mprgp_ctx *ctx, int n_nodes, const int *rowptr, const in
     }
     if (!diag) return fail(MPRGP_EBADINPUT, "row %d has no diagonal block", i);
   }
-  for (long long k = 0; k < 9 * nb; ++k)
-    if (!std::isfinite(blocks[k])) return fail(MPRGP_EBADINPUT, "non-finite matrix value at %lld", k);
+  if (blocks)
+    for (long long k = 0; k < 9 * nb; ++k)
+      if (!std::isfinite(blocks[k])) return fail(MPRGP_EBADINPUT, "non-finite matrix value at %lld", k);
   mprgp_matrix *A = new mprgp_matrix();
   A->ctx = ctx;
   A->n_nodes = n_nodes;
@@ -207,7 +220,14 @@ static int build_matrix(mprgp_ctx *ctx, int n_nodes, const int *rowptr, const in
     A->sl_off_h[s + 1] = (int)slots;
   }
   A->slots = slots;
-  std::vector<int> col(slots, -1);
+  // padding slots point at a valid node (the row itself, or the last node for
+  // lanes past the end) with zero values, so the SpMV needs no branch
+  std::vector<int> col(slots);
+  for (int s = 0; s < A->n_slices; ++s)
+    for (long long k = A->sl_off_h[s]; k < A->sl_off_h[s + 1]; ++k) {
+      int i = s * kSlice + (int)((k - A->sl_off_h[s]) % kSlice);
+      col[k] = i < n_nodes ? i : n_nodes - 1;
+    }
   std::vector<double> val((size_t)slots * 9, 0.0);
   A->blk_slot.resize(nb);
   for (int s = 0; s < A->n_slices; ++s) {
@@ -222,7 +242,8 @@ static int build_matrix(mprgp_ctx *ctx, int n_nodes, const int *rowptr, const in
         col[slot] = A->iperm[colidx[src]];
         A->blk_slot[src] = slot;
         long long vbase = b0 * 9 + (long long)k * 9 * kSlice + l;
-        for (int e = 0; e < 9; ++e) val[vbase + (long long)e * kSlice] = blocks[9 * src + e];
+        if (blocks)
+          for (int e = 0; e < 9; ++e) val[vbase + (long long)e * kSlice] = blocks[9 * src + e];
       }
     }
   }
@@ -255,7 +276,7 @@ static int build_matrix(mprgp_ctx *ctx, int n_nodes, const int *rowptr, const in
 
 extern "C" int mprgp_matrix_create_bsr3(mprgp_ctx *ctx, int n_nodes, const int *rowptr, const int *colidx,
                                         const double *blocks, mprgp_matrix **out) {
-  if (!ctx) return fail(MPRGP_EBADINPUT, "ctx is NULL");
+  if (!ctx || !blocks) return fail(MPRGP_EBADINPUT, "ctx or blocks is NULL");
   MPRGP_HIP(hipSetDevice(ctx->device));
   return build_matrix(ctx, n_nodes, rowptr, colidx, blocks, out);
 }
@@ -389,8 +410,8 @@ extern "C" int mprgp_qp_create(mprgp_matrix *A, mprgp_qp **out) {
   const int n_vec = 15;      // x x0 b g p z phi beta y ap py ev pv pw stage
   const int n_row = 3 + 3;   // cn(3) cc cj cs
   q->part_cap = (int)qp_tiles_max(q->n);
-  size_t doubles = n_vec * n3 + n_row * (size_t)q->n + (size_t)q->part_cap * kNumPart;
-  size_t bytes = doubles * sizeof(double) + sizeof(QpState) + 2 * (size_t)q->n + 512;
+  size_t doubles = n_vec * n3 + n_row * (size_t)q->n + 2 * (size_t)q->part_cap * kNumPart;
+  size_t bytes = doubles * sizeof(double) + 2 * sizeof(QpState) + 2 * (size_t)q->n + 512;
   int rc = ctx->alloc((void **)&q->base, bytes);
   if (rc) {
     delete q;
@@ -408,11 +429,11 @@ extern "C" int mprgp_qp_create(mprgp_matrix *A, mprgp_qp **out) {
   q->cc = p; p += q->n;
   q->cj = p; p += q->n;
   q->cs = p; p += q->n;
-  q->part = p; p += (size_t)q->part_cap * kNumPart;
+  q->part = p; p += 2 * (size_t)q->part_cap * kNumPart;
   uintptr_t u = (uintptr_t)p;
   u = (u + 127) & ~(uintptr_t)127;
   q->st = (QpState *)u;
-  q->cf = (unsigned char *)(q->st + 1);
+  q->cf = (unsigned char *)(q->st + 2);
   q->face = q->cf + q->n;
   hipStream_t s = ctx->stream;
   hipError_t e = hipMemsetAsync(q->base, 0, bytes, s);
@@ -492,6 +513,13 @@ extern "C" int mprgp_qp_get_x(mprgp_qp *q, double *x) {
   return download_vec(q, q->x, x);
 }
 
+extern "C" int mprgp_qp_get_rhs(mprgp_qp *q, double *b) {
+  if (!q || !b) return fail(MPRGP_EBADINPUT, "NULL argument");
+  if (!q->has_rhs) return fail(MPRGP_ESTATE, "rhs was never set");
+  MPRGP_HIP(hipSetDevice(q->ctx->device));
+  return download_vec(q, q->b, b);
+}
+
 extern "C" int mprgp_qp_set_ev(mprgp_qp *q, const double *ev) {
   if (!q) return fail(MPRGP_EBADINPUT, "NULL argument");
   if (!ev) {
@@ -511,6 +539,12 @@ extern "C" int mprgp_qp_get_ev(mprgp_qp *q, double *ev, int *valid) {
   if (!ev || !q->ev_valid) return MPRGP_OK;
   MPRGP_HIP(hipSetDevice(q->ctx->device));
   return download_vec(q, q->ev, ev);
+}
+
+extern "C" int mprgp_qp_op_counts(mprgp_qp *q, int *counts, int n) {
+  if (!q || !counts || n <= 0) return fail(MPRGP_EBADINPUT, "bad arguments");
+  for (int i = 0; i < n; ++i) counts[i] = i < OP_NUM ? q->opcnt[i] : 0;
+  return MPRGP_OK;
 }
 
 extern "C" int mprgp_qp_get_face(mprgp_qp *q, unsigned char *face) {
@@ -639,7 +673,7 @@ static int get_graph(mprgp_ctx *ctx, int ts, int steps, int grid, hipGraphExec_t
   hipStream_t s = ctx->stream;
   MPRGP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   for (int i = 0; i < steps; ++i) {
-    hipError_t e = launch_step(ts, grid, s, ctx->d_qps, ctx->d_tiles, ctx->d_ctrl, ctx->d_ctrl + 1);
+    hipError_t e = launch_step(ts, grid, s, ctx->d_qps, ctx->d_tiles, ctx->d_ctrl, ctx->d_ctrl + 1, i & 1);
     if (e != hipSuccess) {
       hipGraph_t g;
       (void)hipStreamEndCapture(s, &g);
@@ -733,6 +767,7 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
     st.power_eps = p.power_tol > 0.0 ? p.power_tol : p.tol;
     st.pow_next = power ? OP_POW_MV : OP_INIT;
     st.op = p.project_first ? OP_PROJX : st.pow_next;
+    st.fresh = 1;
     max_steps = std::max<long>(max_steps, 8L * ((long)p.max_it + 4) + (power ? 2L * 1002 + 4 : 0) + 16);
     if (power) {
       if (p.use_ev && q->ev_valid) {
@@ -755,6 +790,7 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
     // tangent-Jacobi factors depend on the current matrix values
     MPRGP_HIP(launch_setup_rows(s, q->A->d_dinv, q->cn, q->cf, q->cs, q->n));
     MPRGP_HIP(hipMemcpyAsync(q->st, &st, sizeof(QpState), hipMemcpyHostToDevice, s));
+    MPRGP_HIP(hipMemcpyAsync(q->st + 1, &st, sizeof(QpState), hipMemcpyHostToDevice, s));
   }
   const int n_tiles = (int)tiles.size();
   if (int rc = ensure_tables(ctx, count, n_tiles)) return rc;
@@ -766,15 +802,27 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
 
   const int grid = std::min(n_tiles, 8192);
   int steps = params[0].graph_steps > 0 ? params[0].graph_steps : 16;
+  steps += steps & 1;  // even, so every replay starts at parity 0
+  // graph_steps < 0 (or MPRGP_NO_GRAPH set): plain stream launches, e.g. for
+  // profilers that do not follow graph replays
+  const char *ng = getenv("MPRGP_NO_GRAPH");
+  const bool use_graph = params[0].graph_steps >= 0 && !(ng && ng[0] && ng[0] != '0');
+  if (params[0].graph_steps < 0) steps = -params[0].graph_steps + (params[0].graph_steps & 1);
   hipGraphExec_t ex = nullptr;
-  if (int rc = get_graph(ctx, ts, steps, grid, &ex)) return rc;
+  if (use_graph)
+    if (int rc = get_graph(ctx, ts, steps, grid, &ex)) return rc;
 
   MPRGP_HIP(hipEventRecord(ctx->t0, s));
   long launched = 0;
   int k = 0;
   bool done = false;
   auto enqueue = [&](int slot) -> int {
-    MPRGP_HIP(hipGraphLaunch(ex, s));
+    if (use_graph) {
+      MPRGP_HIP(hipGraphLaunch(ex, s));
+    } else {
+      for (int i = 0; i < steps; ++i)
+        MPRGP_HIP(launch_step(ts, grid, s, ctx->d_qps, ctx->d_tiles, ctx->d_ctrl, ctx->d_ctrl + 1, i & 1));
+    }
     launched += steps;
     MPRGP_HIP(hipMemcpyAsync(ctx->h_running + slot, ctx->d_ctrl + 1, sizeof(int), hipMemcpyDeviceToHost, s));
     MPRGP_HIP(hipEventRecord(ctx->q[slot], s));
@@ -798,8 +846,11 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
 
   int worst = MPRGP_OK;
   for (int i = 0; i < count; ++i) {
-    QpState st;
-    MPRGP_HIP(hipMemcpy(&st, qps[i]->st, sizeof(QpState), hipMemcpyDeviceToHost));
+    QpState both[2];
+    MPRGP_HIP(hipMemcpy(both, qps[i]->st, 2 * sizeof(QpState), hipMemcpyDeviceToHost));
+    // the copy that reached DONE (both do after one more launch)
+    const QpState &st = (both[1].op == OP_DONE && (both[0].op != OP_DONE || both[1].steps > both[0].steps))
+                            ? both[1] : both[0];
     if (st.ev_valid) qps[i]->ev_valid = 1;
     if (stats) {
       mprgp_stats &o = stats[i];
@@ -817,6 +868,7 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
       o.alpha_bar = st.alpha_bar;
       o.lambda_max = st.lambda_max;
       o.elapsed_ms = ms;
+      std::memcpy(qps[i]->opcnt, st.opcnt, sizeof(st.opcnt));
     }
     if (!std::isfinite(st.residual)) worst = MPRGP_ENAN;
     else if (st.code != 0 && worst == MPRGP_OK) worst = MPRGP_NOT_CONVERGED;

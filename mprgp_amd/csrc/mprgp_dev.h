@@ -6,7 +6,8 @@
 //   matrix  SELL-64 of 3x3 blocks: slice s covers internal nodes 64s..64s+63;
 //           its blocks occupy [sl_off[s], sl_off[s+1]) (a multiple of 64);
 //           block k of lane l sits at sl_off[s] + 64k + l; column index col[]
-//           (internal node, -1 for padding); values val[9*blk0 + 64*(9k+e) + l]
+//           (internal node; padding slots name the row itself, or the last
+//           node past the end, with zero values); values val[9*blk0 + 64*(9k+e) + l]
 //           for entry e = 3*row + col of the block, so every load of one entry
 //           across a wavefront is 64 consecutive values (512 B for FP64).
 //   vectors 3 values per node, node-major (x0 y0 z0 x1 ...).
@@ -51,10 +52,15 @@ enum Op : int {
 
 enum Variant : int { VAR_MONOTONIC = 0, VAR_CLASSIC = 1 };
 
-// Per-QP control block, written by the last tile of each launch.
+// Per-QP control block.  Two copies per QP alternate with the launch parity:
+// launch k reads copy (k+1)%2 (the state the previous launch executed) and
+// the previous launch's per-tile partial sums, applies the control
+// transition in every tile (identically), executes the resulting op, and its
+// first tile stores the new state into copy k%2.  No device-scope fence is
+// needed: launch boundaries order everything.
 struct alignas(128) QpState {
   int op;
-  int arrive;        // tiles that finished the current launch
+  int fresh;         // set by the host: no previous launch to reduce
   int n_tiles;
   int iteration, num_cg, num_exp, num_prop;
   int max_it;
@@ -76,6 +82,7 @@ struct alignas(128) QpState {
   double alpha, beta_cg, pd, zg, residual, fx, fy;
   double alpha_f, nu, power_eps, lambda_max;
   double pad1[3];
+  int opcnt[OP_NUM];  // launches this QP spent in each op
 };
 
 // Device pointers of one QP (FP64 form).
@@ -93,8 +100,8 @@ struct QpDev {
   const double *cn, *cc, *cj, *cs;
   const unsigned char *cf;
   unsigned char *face;
-  QpState *st;
-  double *part;         // n_tiles * kNumPart partial sums
+  QpState *st;          // st[0], st[1] (launch parity)
+  double *part;         // [2][n_tiles][kNumPart] partial sums (launch parity)
 };
 
 // One work item of a launch: up to `slices` consecutive slices of one QP.

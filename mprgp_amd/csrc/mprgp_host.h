@@ -12,6 +12,9 @@
 #include "../../include/mprgp_amd.h"
 #include "mprgp_dev.h"
 
+struct mprgp_ctx;
+struct mprgp_matrix;
+
 namespace mprgp {
 
 // last-error plumbing
@@ -28,7 +31,7 @@ int fail(int code, const char *fmt, ...);
 
 // kernel launch wrappers (mprgp_kernels.hip)
 hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
-                       const int *n_tiles, int *running);
+                       const int *n_tiles, int *running, int parity);
 hipError_t launch_gather_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n);
 hipError_t launch_scatter_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n);
 hipError_t launch_setup_diag(hipStream_t s, const int *sl_off, const int *col, const double *val,
@@ -37,6 +40,11 @@ hipError_t launch_setup_rows(hipStream_t s, const double *dinv, const double *cn
                              const unsigned char *cf, double *cs, int n);
 hipError_t launch_spmv_plain(hipStream_t s, const int *sl_off, const int *col, const double *val,
                              const double *x, double *y, int n, int accumulate);
+
+// Builds the SELL-64 device matrix from a node-order block CSR (blocks may be
+// NULL for a zero matrix of that pattern).
+int build_matrix(mprgp_ctx *ctx, int n_nodes, const int *rowptr, const int *colidx,
+                 const double *blocks, mprgp_matrix **out);
 
 }  // namespace mprgp
 
@@ -57,6 +65,10 @@ struct mprgp_ctx {
   std::map<std::tuple<int, int, int>, hipGraph_t> graph_defs;
   // autotuner cache: node-count bucket -> (tiles per workgroup)
   std::map<int, int> tuned_ts;
+  // scratch shared by calls on this context (calls on one context never overlap)
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+  int scratch_reserve(size_t bytes);
 
   int alloc(void **p, size_t bytes);
   void release(void *p, size_t bytes);
@@ -98,4 +110,5 @@ struct mprgp_qp {
   int m = 0;
   bool has_rhs = false, has_x = false;
   int ev_valid = 0;
+  int opcnt[mprgp::OP_NUM] = {};  // device launches per op in the last solve
 };

@@ -122,37 +122,64 @@ __device__ __forceinline__ V3 precond_of(const QpDev &q, bool on, const Row &R, 
 // ------------------------------------------------------------------ SpMV
 
 // Accumulates A v for the lane's block row over the slice's blocks; `gather`
-// returns NV input vectors at a neighbour node.
+// returns NV input vectors at a neighbour node.  Padding slots hold a valid
+// column and zero values, so the loop has no branch and the column indices,
+// matrix values and gathers of U blocks are all in flight together.
 template <int NV, class G>
 __device__ __forceinline__ void spmv_row(const QpDev &q, int s, int lane, G gather, V3 (&acc)[NV]) {
+  constexpr int U = NV == 1 ? 4 : 2;
 #pragma unroll
   for (int t = 0; t < NV; ++t) acc[t] = V3{0.0, 0.0, 0.0};
   const int b0 = q.sl_off[s];
   const int W = (q.sl_off[s + 1] - b0) / kSlice;
   const int *cp = q.col + b0 + lane;
   const double *vp = q.val + (size_t)b0 * 9 + lane;
-#pragma unroll 2
-  for (int k = 0; k < W; ++k) {
-    const int j = cp[k * kSlice];
-    if (j >= 0) {
-      const double *v = vp + (size_t)k * (9 * kSlice);
-      const double a00 = v[0 * kSlice], a01 = v[1 * kSlice], a02 = v[2 * kSlice];
-      const double a10 = v[3 * kSlice], a11 = v[4 * kSlice], a12 = v[5 * kSlice];
-      const double a20 = v[6 * kSlice], a21 = v[7 * kSlice], a22 = v[8 * kSlice];
-      V3 xs[NV];
-      gather(j, xs);
+  int k = 0;
+  for (; k + U <= W; k += U) {
+    int j[U];
+    double a[U][9];
+    V3 xs[U][NV];
+#pragma unroll
+    for (int u = 0; u < U; ++u) j[u] = cp[(k + u) * kSlice];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int e = 0; e < 9; ++e) a[u][e] = vp[(size_t)(k + u) * (9 * kSlice) + e * kSlice];
+#pragma unroll
+    for (int u = 0; u < U; ++u) gather(j[u], xs[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
 #pragma unroll
       for (int t = 0; t < NV; ++t) {
-        acc[t].x += a00 * xs[t].x;
-        acc[t].x += a01 * xs[t].y;
-        acc[t].x += a02 * xs[t].z;
-        acc[t].y += a10 * xs[t].x;
-        acc[t].y += a11 * xs[t].y;
-        acc[t].y += a12 * xs[t].z;
-        acc[t].z += a20 * xs[t].x;
-        acc[t].z += a21 * xs[t].y;
-        acc[t].z += a22 * xs[t].z;
+        acc[t].x += a[u][0] * xs[u][t].x;
+        acc[t].x += a[u][1] * xs[u][t].y;
+        acc[t].x += a[u][2] * xs[u][t].z;
+        acc[t].y += a[u][3] * xs[u][t].x;
+        acc[t].y += a[u][4] * xs[u][t].y;
+        acc[t].y += a[u][5] * xs[u][t].z;
+        acc[t].z += a[u][6] * xs[u][t].x;
+        acc[t].z += a[u][7] * xs[u][t].y;
+        acc[t].z += a[u][8] * xs[u][t].z;
       }
+  }
+  for (; k < W; ++k) {
+    const int j = cp[k * kSlice];
+    double a[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) a[e] = vp[(size_t)k * (9 * kSlice) + e * kSlice];
+    V3 xs[NV];
+    gather(j, xs);
+#pragma unroll
+    for (int t = 0; t < NV; ++t) {
+      acc[t].x += a[0] * xs[t].x;
+      acc[t].x += a[1] * xs[t].y;
+      acc[t].x += a[2] * xs[t].z;
+      acc[t].y += a[3] * xs[t].x;
+      acc[t].y += a[4] * xs[t].y;
+      acc[t].y += a[5] * xs[t].z;
+      acc[t].z += a[6] * xs[t].x;
+      acc[t].z += a[7] * xs[t].y;
+      acc[t].z += a[8] * xs[t].z;
     }
   }
 }
@@ -194,9 +221,10 @@ __device__ __forceinline__ int proportional(const QpState &st, double bb, double
   return bb <= st.gamma * st.gamma * phitphi;
 }
 
-// Runs on one thread of the last tile of a QP: consumes the QP's reduced
-// partial sums and moves its state machine (MPRGPSolver.h:42-79, 148-456).
-__device__ void control(QpState &st, const Sums &S, int *running) {
+// Consumes the reduced partial sums of the op a QP executed in the previous
+// launch and moves its state machine to the next op (MPRGPSolver.h:42-79,
+// 148-456).  Every tile of the QP runs it on identical inputs.
+__device__ void control(QpState &st, const Sums &S) {
   switch (st.op) {
     case OP_PROJX:
       st.op = st.pow_next;
@@ -360,7 +388,6 @@ __device__ void control(QpState &st, const Sums &S, int *running) {
       st.active = (int)S.v[0];
       st.code = st.residual <= st.tol ? 0 : -1;
       st.op = OP_DONE;
-      atomicSub(running, 1);
       break;
     default:
       break;
@@ -642,76 +669,95 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
   }
 }
 
+#ifndef MPRGP_STEP_MIN_WAVES
+#define MPRGP_STEP_MIN_WAVES 3
+#endif
+
 template <int TS>
-__global__ void __launch_bounds__(TS * kSlice) mprgp_step(const QpDev *__restrict__ qps,
+__global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(const QpDev *__restrict__ qps,
                                                            const Tile *__restrict__ tiles,
                                                            const int *__restrict__ n_tiles_dev,
-                                                           int *running) {
+                                                           int *running, int parity) {
   __shared__ double red[TS][kNumPart];
-  __shared__ int last;
-  __shared__ int sop;
+  __shared__ QpState sst;
+  __shared__ int prev_done;
   const int n_tiles = *n_tiles_dev;
   const int wave = threadIdx.x / kSlice;
   const int lane = threadIdx.x % kSlice;
   for (int ti = blockIdx.x; ti < n_tiles; ti += gridDim.x) {
     const Tile t = tiles[ti];
     const QpDev &q = qps[t.qp];
-    if (threadIdx.x == 0) sop = q.st->op;
-    __syncthreads();
-    const int op = sop;
-    if (op == OP_DONE) {
-      __syncthreads();
-      continue;
-    }
-    const QpState st = *q.st;
-    double P[kNumPart];
-#pragma unroll
-    for (int j = 0; j < kNumPart; ++j) P[j] = 0.0;
-    P[kMinSlot] = 1.7976931348623157e308;
-    tile_work<TS>(q, t, op, st, P);
-    // wave then block reduction in a fixed order
-#pragma unroll
-    for (int j = 0; j < kMinSlot; ++j) P[j] = wave_sum(P[j]);
-    P[kMinSlot] = wave_min(P[kMinSlot]);
-    if (lane == 0) {
-#pragma unroll
-      for (int j = 0; j < kNumPart; ++j) red[wave][j] = P[j];
+    // 1. state the previous launch executed + its partials -> this launch's op
+    constexpr int kWords = sizeof(QpState) / sizeof(int);
+    if (wave == 0) {
+      const int *src = reinterpret_cast<const int *>(q.st + (parity ^ 1));
+      int *dst = reinterpret_cast<int *>(&sst);
+      for (int i = lane; i < kWords; i += kSlice) dst[i] = src[i];
     }
     __syncthreads();
     if (wave == 0) {
-      if (lane < kNumPart) {
-        const int j = lane;
-        double acc = red[0][j];
-        for (int w = 1; w < TS; ++w) acc = (j == kMinSlot) ? fmin(acc, red[w][j]) : acc + red[w][j];
-        q.part[(size_t)t.local * kNumPart + j] = acc;
+      const bool skip = sst.fresh || sst.op == OP_DONE;
+      const int nt = sst.n_tiles;
+      Sums S;
+      if (!skip) {
+        const double *pp = q.part + (size_t)(parity ^ 1) * nt * kNumPart;
+#pragma unroll
+        for (int j = 0; j < kNumPart; ++j) {
+          double acc = (j == kMinSlot) ? 1.7976931348623157e308 : 0.0;
+          for (int k = lane; k < nt; k += kSlice) {
+            const double v = pp[(size_t)k * kNumPart + j];
+            acc = (j == kMinSlot) ? fmin(acc, v) : acc + v;
+          }
+          S.v[j] = (j == kMinSlot) ? wave_min(acc) : wave_sum(acc);
+        }
       }
-      __threadfence();  // release the tile's partials before arriving
       if (lane == 0) {
-        int old = atomicAdd(&q.st->arrive, 1);
-        last = (old == st.n_tiles - 1);
+        prev_done = sst.op == OP_DONE;
+        if (!skip) control(sst, S);
+        sst.fresh = 0;
       }
     }
     __syncthreads();
-    if (last && wave == 0) {
-      __threadfence();
-      // lane j-th partial: each lane sums tiles lane, lane+64, ... in order
-      Sums S;
+    const QpState &st = sst;
+    const int op = st.op;
+    if (op != OP_DONE) {
+      double P[kNumPart];
 #pragma unroll
-      for (int j = 0; j < kNumPart; ++j) {
-        double acc = (j == kMinSlot) ? 1.7976931348623157e308 : 0.0;
-        for (int k = lane; k < st.n_tiles; k += kSlice) {
-          double v = __hip_atomic_load(q.part + (size_t)k * kNumPart + j, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-          acc = (j == kMinSlot) ? fmin(acc, v) : acc + v;
-        }
-        S.v[j] = (j == kMinSlot) ? wave_min(acc) : wave_sum(acc);
-      }
+      for (int j = 0; j < kNumPart; ++j) P[j] = 0.0;
+      P[kMinSlot] = 1.7976931348623157e308;
+      tile_work<TS>(q, t, op, st, P);
+      // wave then block reduction in a fixed order
+#pragma unroll
+      for (int j = 0; j < kMinSlot; ++j) P[j] = wave_sum(P[j]);
+      P[kMinSlot] = wave_min(P[kMinSlot]);
       if (lane == 0) {
-        QpState &ws = *q.st;
-        ws.arrive = 0;
-        ws.steps++;
-        control(ws, S, running);
-        __threadfence();
+#pragma unroll
+        for (int j = 0; j < kNumPart; ++j) red[wave][j] = P[j];
+      }
+      __syncthreads();
+      if (wave == 0 && lane < kNumPart) {
+        const int j = lane;
+        double acc = red[0][j];
+        for (int w = 1; w < TS; ++w) acc = (j == kMinSlot) ? fmin(acc, red[w][j]) : acc + red[w][j];
+        q.part[((size_t)parity * st.n_tiles + t.local) * kNumPart + j] = acc;
+      }
+    }
+    // 2. the QP's first tile persists the state this launch executes
+    __syncthreads();
+    if (t.local == 0) {
+      if (threadIdx.x == 0) {
+        if (op != OP_DONE) {
+          sst.steps++;
+          sst.opcnt[op]++;
+        } else if (!prev_done) {
+          atomicSub(running, 1);
+        }
+      }
+      __syncthreads();
+      if (wave == 0) {
+        const int *src = reinterpret_cast<const int *>(&sst);
+        int *dst = reinterpret_cast<int *>(q.st + parity);
+        for (int i = lane; i < kWords; i += kSlice) dst[i] = src[i];
       }
     }
     __syncthreads();
@@ -803,12 +849,12 @@ __global__ void __launch_bounds__(256) spmv_plain(const int *__restrict__ sl_off
 // Host-callable launch wrappers ------------------------------------------
 
 hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
-                       const int *n_tiles, int *running) {
+                       const int *n_tiles, int *running, int parity) {
   switch (ts) {
-    case 1: mprgp_step<1><<<grid, 1 * kSlice, 0, s>>>(qps, tiles, n_tiles, running); break;
-    case 2: mprgp_step<2><<<grid, 2 * kSlice, 0, s>>>(qps, tiles, n_tiles, running); break;
-    case 4: mprgp_step<4><<<grid, 4 * kSlice, 0, s>>>(qps, tiles, n_tiles, running); break;
-    case 8: mprgp_step<8><<<grid, 8 * kSlice, 0, s>>>(qps, tiles, n_tiles, running); break;
+    case 1: mprgp_step<1><<<grid, 1 * kSlice, 0, s>>>(qps, tiles, n_tiles, running, parity); break;
+    case 2: mprgp_step<2><<<grid, 2 * kSlice, 0, s>>>(qps, tiles, n_tiles, running, parity); break;
+    case 4: mprgp_step<4><<<grid, 4 * kSlice, 0, s>>>(qps, tiles, n_tiles, running, parity); break;
+    case 8: mprgp_step<8><<<grid, 8 * kSlice, 0, s>>>(qps, tiles, n_tiles, running, parity); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
