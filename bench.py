@@ -45,6 +45,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ts", type=int, default=0, help="wavefronts per workgroup (0: library choice)")
+    ap.add_argument("--spw", type=int, default=0, help="slices per wavefront per tile (0: library choice)")
+    ap.add_argument("--graph-steps", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the batched QPs shard across "
@@ -73,7 +76,9 @@ def main():
 
     prec = M.MIXED if args.precision == "mixed" else M.FP64
     per = [M.params(tol=args.tol, max_it=4000, alpha_bar=q.alpha_bar, project_first=1,
-                    precision=prec) for q in qps]
+                    precision=prec, tiles=args.ts, graph_steps=args.graph_steps) for q in qps]
+    for p in per:
+        p.reserved_i[0] = args.spw
 
     def batch_step():
         for q in qps:

@@ -149,6 +149,7 @@ extern "C" void mprgp_ctx_destroy(mprgp_ctx *c) {
   c->release(c->d_qps, sizeof(QpDev) * (size_t)c->qps_cap);
   c->release(c->d_tiles, sizeof(Tile) * (size_t)c->tiles_cap);
   c->release(c->d_ctrl, 4 * sizeof(int));
+  c->release(c->d_flags, sizeof(int) * (size_t)c->flags_cap);
   c->release(c->scratch, c->scratch_bytes);
   if (c->h_running) (void)hipHostFree(c->h_running);
   if (c->t0) (void)hipEventDestroy(c->t0);
@@ -635,6 +636,14 @@ extern "C" int mprgp_qp_set_rows_sparse(mprgp_qp *q, int m, int col_major, const
 // ----------------------------------------------------------- batch solve
 
 static int ensure_tables(mprgp_ctx *ctx, int n_qps, int n_tiles) {
+  if (n_qps > ctx->flags_cap) {
+    int cap = std::max(n_qps, 2 * ctx->flags_cap);
+    ctx->release(ctx->d_flags, sizeof(int) * (size_t)ctx->flags_cap);
+    ctx->d_flags = nullptr;
+    ctx->flags_cap = 0;
+    if (int rc = ctx->alloc((void **)&ctx->d_flags, sizeof(int) * (size_t)cap)) return rc;
+    ctx->flags_cap = cap;
+  }
   if (n_qps > ctx->qps_cap) {
     int cap = std::max(n_qps, 2 * ctx->qps_cap);
     ctx->release(ctx->d_qps, sizeof(QpDev) * (size_t)ctx->qps_cap);
@@ -695,13 +704,43 @@ static int get_graph(mprgp_ctx *ctx, int ts, int steps, int grid, hipGraphExec_t
   return MPRGP_OK;
 }
 
-static int pick_tiles(const mprgp_params &p, int total_nodes) {
-  if (p.tiles == 1 || p.tiles == 2 || p.tiles == 4 || p.tiles == 8) return p.tiles;
-  // enough workgroups to cover 256 CUs several times over
-  long slices = (total_nodes + kSlice - 1) / kSlice;
-  if (slices >= 4L * 2048) return 4;
-  if (slices >= 2L * 1024) return 2;
-  return 1;
+// Launch shape: TS wavefronts per workgroup, each taking `spw` slices of
+// 64 rows, so a tile covers TS*spw slices.  Small problems use narrow
+// workgroups so enough of them exist to fill the chip.
+struct Shape {
+  int ts, spw;
+};
+static Shape pick_shape(const mprgp_params &p, long total_slices) {
+  Shape sh{4, 1};
+  if (p.tiles == 1 || p.tiles == 2 || p.tiles == 4 || p.tiles == 8) {
+    sh.ts = p.tiles;
+  } else if (total_slices < 2048) {
+    sh.ts = 1;
+  } else if (total_slices < 8192) {
+    sh.ts = 2;
+  }
+  // one slice per wavefront measured fastest on the 256-QP batch (more
+  // slices per wave serialise the SpMV latency chains); callers may override
+  sh.spw = p.reserved_i[0] > 0 ? p.reserved_i[0] : 1;
+  return sh;
+}
+
+static void build_tiles(const std::vector<int> &n_slices, const std::vector<char> &keep, Shape sh,
+                        std::vector<Tile> &tiles) {
+  tiles.clear();
+  const int span = sh.ts * sh.spw;
+  for (size_t i = 0; i < n_slices.size(); ++i) {
+    if (!keep[i]) continue;
+    int local = 0;
+    for (int s0 = 0; s0 < n_slices[i]; s0 += span, ++local)
+      tiles.push_back(Tile{(int)i, s0, std::min(span, n_slices[i] - s0), local});
+  }
+}
+
+static int round_grid(int n_tiles) {
+  int g = 64;
+  while (g < n_tiles && g < 8192) g *= 2;
+  return std::min(g, 8192);
 }
 
 static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprgp_params *params,
@@ -724,21 +763,30 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
       if (qps[j] == q) return fail(MPRGP_EBADINPUT, "qp %d appears twice in the batch", i);
     total_nodes += q->n;
   }
-  const int ts = pick_tiles(params[0], (int)std::min<long>(total_nodes, 0x7fffffff));
+  long total_slices = 0;
+  std::vector<int> n_slices(count);
+  for (int i = 0; i < count; ++i) total_slices += (n_slices[i] = qps[i]->A->n_slices);
+  const Shape sh = pick_shape(params[0], total_slices);
+  const int ts = sh.ts;
   // tile table and device descriptors
   std::vector<Tile> tiles;
+  std::vector<char> keep(count, 1);
+  build_tiles(n_slices, keep, sh, tiles);
+  std::vector<int> tiles_of(count, 0);
+  for (const Tile &t : tiles) tiles_of[t.qp]++;
   std::vector<QpDev> devs(count);
   std::vector<QpState> states(count);
   long max_steps = 0;
   hipStream_t s = ctx->stream;
+  if (int rc = ensure_tables(ctx, count, (int)tiles.size())) return rc;
   for (int i = 0; i < count; ++i) {
     mprgp_qp *q = qps[i];
     const mprgp_params &p = params[stride ? i : 0];
-    const int n_slices = q->A->n_slices;
     QpDev &d = devs[i];
     d.n_nodes = q->n;
-    d.n_slices = n_slices;
-    d.tile0 = (int)tiles.size();
+    d.n_slices = n_slices[i];
+    d.n_tiles = tiles_of[i];
+    d.done_flag = ctx->d_flags + i;
     d.sl_off = q->A->d_sl_off;
     d.col = q->A->d_col;
     d.val = q->A->d_val;
@@ -748,12 +796,9 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
     d.cn = q->cn; d.cc = q->cc; d.cj = q->cj; d.cs = q->cs; d.cf = q->cf; d.face = q->face;
     d.st = q->st;
     d.part = q->part;
-    int local = 0;
-    for (int s0 = 0; s0 < n_slices; s0 += ts, ++local)
-      tiles.push_back(Tile{i, s0, std::min(ts, n_slices - s0), local});
     QpState &st = states[i];
     std::memset(&st, 0, sizeof(st));
-    st.n_tiles = local;
+    st.n_tiles = tiles_of[i];
     st.max_it = p.max_it;
     st.tol = p.tol < 1e-30 ? 1e-30 : p.tol;
     st.gamma = 1.0;
@@ -793,14 +838,14 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
     MPRGP_HIP(hipMemcpyAsync(q->st + 1, &st, sizeof(QpState), hipMemcpyHostToDevice, s));
   }
   const int n_tiles = (int)tiles.size();
-  if (int rc = ensure_tables(ctx, count, n_tiles)) return rc;
   int ctrl[2] = {n_tiles, count};
   MPRGP_HIP(hipMemcpyAsync(ctx->d_qps, devs.data(), sizeof(QpDev) * count, hipMemcpyHostToDevice, s));
   MPRGP_HIP(hipMemcpyAsync(ctx->d_tiles, tiles.data(), sizeof(Tile) * n_tiles, hipMemcpyHostToDevice, s));
   MPRGP_HIP(hipMemcpyAsync(ctx->d_ctrl, ctrl, sizeof(ctrl), hipMemcpyHostToDevice, s));
+  MPRGP_HIP(hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * count, s));
   MPRGP_HIP(hipStreamSynchronize(s));
 
-  const int grid = std::min(n_tiles, 8192);
+  int grid = round_grid(n_tiles);
   int steps = params[0].graph_steps > 0 ? params[0].graph_steps : 16;
   steps += steps & 1;  // even, so every replay starts at parity 0
   // graph_steps < 0 (or MPRGP_NO_GRAPH set): plain stream launches, e.g. for
@@ -811,6 +856,9 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
   hipGraphExec_t ex = nullptr;
   if (use_graph)
     if (int rc = get_graph(ctx, ts, steps, grid, &ex)) return rc;
+  int in_table = count;  // QPs whose tiles are in the table
+  std::vector<int> flags(count);
+  std::vector<std::vector<Tile>> uploads;  // host buffers of pending table uploads
 
   MPRGP_HIP(hipEventRecord(ctx->t0, s));
   long launched = 0;
@@ -832,8 +880,35 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
   while (!done) {
     if (int rc = enqueue((k + 1) & 1)) return rc;
     MPRGP_HIP(hipEventSynchronize(ctx->q[k & 1]));
-    if (ctx->h_running[k & 1] == 0) done = true;
+    const int running = ctx->h_running[k & 1];
+    if (running == 0) done = true;
     ++k;
+    // drop the tiles of finished QPs once a quarter of the table is idle
+    if (!done && count > 1 && 4 * running < 3 * in_table) {
+      MPRGP_HIP(hipMemcpyAsync(flags.data(), ctx->d_flags, sizeof(int) * count, hipMemcpyDeviceToHost, s));
+      MPRGP_HIP(hipStreamSynchronize(s));
+      int live = 0;
+      for (int i = 0; i < count; ++i) {
+        keep[i] = flags[i] == 0;
+        live += keep[i];
+      }
+      if (live > 0 && live < in_table) {
+        uploads.emplace_back();
+        build_tiles(n_slices, keep, sh, uploads.back());
+        const int nt = (int)uploads.back().size();
+        MPRGP_HIP(hipMemcpyAsync(ctx->d_tiles, uploads.back().data(), sizeof(Tile) * nt,
+                                 hipMemcpyHostToDevice, s));
+        MPRGP_HIP(hipMemcpyAsync(ctx->d_ctrl, &nt, sizeof(int), hipMemcpyHostToDevice, s));
+        MPRGP_HIP(hipStreamSynchronize(s));
+        in_table = live;
+        grid = round_grid(nt);
+        if (use_graph)
+          if (int rc = get_graph(ctx, ts, steps, grid, &ex)) return rc;
+      }
+      // restart the two-deep polling pipeline
+      if (int rc = enqueue(k & 1)) return rc;
+      continue;
+    }
     if (!done && launched > max_steps + 2L * steps) {
       (void)hipStreamSynchronize(s);
       return fail(MPRGP_EHIP, "solve did not finish within %ld device steps", launched);

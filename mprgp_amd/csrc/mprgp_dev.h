@@ -90,7 +90,7 @@ struct QpDev {
   int n_nodes;
   int n_slices;
   int tile0;         // first entry of this QP in the batch tile table
-  int pad;
+  int n_tiles;       // tiles of this QP (partial-sum slots per launch)
   const int *sl_off;
   const int *col;
   const double *val;
@@ -101,6 +101,7 @@ struct QpDev {
   const unsigned char *cf;
   unsigned char *face;
   QpState *st;          // st[0], st[1] (launch parity)
+  int *done_flag;       // set once when the QP reaches DONE (tile compaction)
   double *part;         // [2][n_tiles][kNumPart] partial sums (launch parity)
 };
 

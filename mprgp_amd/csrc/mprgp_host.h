@@ -59,6 +59,8 @@ struct mprgp_ctx {
   mprgp::Tile *d_tiles = nullptr;
   int tiles_cap = 0;
   int *d_ctrl = nullptr;      // [0] = tile count, [1] = running QPs
+  int *d_flags = nullptr;     // per batch QP: reached DONE
+  int flags_cap = 0;
   int *h_running = nullptr;   // pinned, 2 slots
   hipEvent_t t0 = nullptr, t1 = nullptr, q[2] = {nullptr, nullptr};
   std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;

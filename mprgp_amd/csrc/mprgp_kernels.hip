@@ -396,14 +396,9 @@ __device__ void control(QpState &st, const Sums &S) {
 
 // --------------------------------------------------------------- kernel
 
-template <int TS>
-__device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &st, double (&P)[kNumPart]) {
-  const int wave = threadIdx.x / kSlice;
-  const int lane = threadIdx.x % kSlice;
-  if (wave >= t.slices) return;
-  const int s = t.slice0 + wave;
-  const int r = s * kSlice + lane;
-  if (r >= q.n_nodes) return;
+// One row (= node r of slice s) of op `op`; partial sums accumulate into P.
+__device__ __forceinline__ void row_work(const QpDev &q, int s, int lane, int r, int op, const QpState &st,
+                                         double (&P)[kNumPart]) {
   const bool pre = st.precond && st.has_rows;
 
   switch (op) {
@@ -419,7 +414,7 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
       V3 acc[1];
       spmv_row<1>(q, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(v, j); }, acc);
       st3(w, r, acc[0]);
-      P[0] = dot3(acc[0], acc[0]);
+      P[0] += dot3(acc[0], acc[0]);
       break;
     }
     case OP_POW_NRM: {
@@ -429,7 +424,7 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
       a = V3{a.x / st.nu, a.y / st.nu, a.z / st.nu};
       st3(w, r, a);
       V3 dlt = sub(a, b);
-      P[0] = dot3(dlt, dlt);
+      P[0] += dot3(dlt, dlt);
       break;
     }
     case OP_POW_SAVE: {
@@ -470,11 +465,11 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
       st3(q.phi, r, ph);
       st3(q.beta, r, be);
       V3 gp = add(ph, be);
-      P[0] = dot3(gp, gp);
-      P[1] = dot3(be, be);
+      P[0] += dot3(gp, gp);
+      P[1] += dot3(be, be);
       V3 x = ld3(q.x, r);
       V3 px = proj(R, axmy(x, st.alpha_bar, ph));
-      P[2] = dot3(sub(x, px), ph);
+      P[2] += dot3(sub(x, px), ph);
       break;
     }
     case OP_CG_AP:
@@ -484,8 +479,8 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
       spmv_row<1>(q, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(p, j); }, acc);
       st3(q.ap, r, acc[0]);
       V3 pr = ld3(q.p, r);
-      P[0] = dot3(pr, acc[0]);
-      P[1] = dot3(ld3(q.z, r), ld3(q.g, r));
+      P[0] += dot3(pr, acc[0]);
+      P[1] += dot3(ld3(q.z, r), ld3(q.g, r));
       if (op == OP_CL_AP) {
         // stepLimit (MPRGPProjection.h:263-282)
         Row R = ldrow(q, r);
@@ -494,7 +489,7 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
           double jx = dot3(R.n, ld3(q.x, r)), jd = dot3(R.n, pr);
           if (jd > kEps && jx > R.c) lim = (jx - R.c) / jd;
         }
-        P[kMinSlot] = lim;
+        P[kMinSlot] = fmin(P[kMinSlot], lim);
       }
       break;
     }
@@ -503,7 +498,7 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
       V3 y = axmy(ld3(q.x, r), st.alpha, ld3(q.p, r));
       st3(q.y, r, y);
       st3(q.py, r, proj(R, y));
-      P[0] = infeasible(R, y);
+      P[0] += infeasible(R, y);
       break;
     }
     case OP_CG_Y: {
@@ -518,15 +513,15 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
           xs[1] = ld3(x, j);
         }, acc);
         V3 xr = ld3(q.x, r);
-        P[0] = dot3(pyr, acc[0]);
-        P[1] = dot3(pyr, br);
-        P[2] = dot3(xr, acc[1]);
-        P[3] = dot3(xr, br);
+        P[0] += dot3(pyr, acc[0]);
+        P[1] += dot3(pyr, br);
+        P[2] += dot3(xr, acc[1]);
+        P[3] += dot3(xr, br);
       } else {
         V3 acc[1];
         spmv_row<1>(q, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(py, j); }, acc);
-        P[0] = dot3(pyr, acc[0]);
-        P[1] = dot3(pyr, br);
+        P[0] += dot3(pyr, acc[0]);
+        P[1] += dot3(pyr, br);
       }
       break;
     }
@@ -542,7 +537,7 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
       st3(q.g, r, g);
       st3(q.phi, r, ph);
       st3(q.z, r, z);
-      P[0] = dot3(z, ap);
+      P[0] += dot3(z, ap);
       break;
     }
     case OP_CG_B: {
@@ -555,11 +550,11 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
       V3 be = beta_of(R, f, g, ph);
       st3(q.beta, r, be);
       V3 gp = add(ph, be);
-      P[0] = dot3(gp, gp);
-      P[1] = dot3(be, be);
+      P[0] += dot3(gp, gp);
+      P[1] += dot3(be, be);
       V3 x = ld3(q.x, r);
       V3 px = proj(R, axmy(x, st.alpha_bar, ph));
-      P[2] = dot3(sub(x, px), ph);
+      P[2] += dot3(sub(x, px), ph);
       break;
     }
     case OP_EXP1: {
@@ -583,7 +578,7 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
       st3(q.py, r, yn);
       st3(q.x, r, proj(R, yn));
       q.face[r] = f;
-      P[0] = infeasible(R, yn);
+      P[0] += infeasible(R, yn);
       break;
     }
     case OP_PROP1: {
@@ -598,8 +593,8 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
       st3(q.g, r, g);
       st3(q.ap, r, acc[1]);
       V3 d = ld3(q.beta, r);
-      P[0] = dot3(d, acc[1]);
-      P[1] = dot3(g, d);
+      P[0] += dot3(d, acc[1]);
+      P[1] += dot3(g, d);
       break;
     }
     case OP_CL_PROP1: {
@@ -608,8 +603,8 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
       spmv_row<1>(q, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(be, j); }, acc);
       st3(q.ap, r, acc[0]);
       V3 d = ld3(q.beta, r);
-      P[0] = dot3(d, acc[0]);
-      P[1] = dot3(ld3(q.g, r), d);
+      P[0] += dot3(d, acc[0]);
+      P[1] += dot3(ld3(q.g, r), d);
       break;
     }
     case OP_PROP2: {
@@ -640,7 +635,7 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
       st3(q.g, r, g);
       st3(q.phi, r, ph);
       st3(q.z, r, z);
-      P[0] = dot3(z, ap);
+      P[0] += dot3(z, ap);
       break;
     }
     case OP_CL_EXP: {
@@ -661,11 +656,24 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
         Row R = ldrow(q, r);
         st3(q.x, r, proj(R, ld3(q.x, r)));
       }
-      P[0] = q.face[r] ? 1.0 : 0.0;
+      P[0] += q.face[r] ? 1.0 : 0.0;
       break;
     }
     default:
       break;
+  }
+}
+
+// A tile's rows: wave w takes slices w, w+TS, w+2TS, ... of the tile, so the
+// waves of a workgroup stream neighbouring slices at the same time.
+template <int TS>
+__device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &st, double (&P)[kNumPart]) {
+  const int wave = threadIdx.x / kSlice;
+  const int lane = threadIdx.x % kSlice;
+  for (int i = wave; i < t.slices; i += TS) {
+    const int s = t.slice0 + i;
+    const int r = s * kSlice + lane;
+    if (r < q.n_nodes) row_work(q, s, lane, r, op, st, P);
   }
 }
 
@@ -688,30 +696,31 @@ __global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(
     const Tile t = tiles[ti];
     const QpDev &q = qps[t.qp];
     // 1. state the previous launch executed + its partials -> this launch's op
+    //    (state words and partials are loaded together by wave 0)
     constexpr int kWords = sizeof(QpState) / sizeof(int);
     if (wave == 0) {
+      const int nt = q.n_tiles;
+      const double *pp = q.part + (size_t)(parity ^ 1) * nt * kNumPart;
+      double acc[kNumPart];
+#pragma unroll
+      for (int j = 0; j < kNumPart; ++j) acc[j] = (j == kMinSlot) ? 1.7976931348623157e308 : 0.0;
+      for (int k = lane; k < nt; k += kSlice) {
+#pragma unroll
+        for (int j = 0; j < kNumPart; ++j) {
+          const double v = pp[(size_t)k * kNumPart + j];
+          acc[j] = (j == kMinSlot) ? fmin(acc[j], v) : acc[j] + v;
+        }
+      }
       const int *src = reinterpret_cast<const int *>(q.st + (parity ^ 1));
       int *dst = reinterpret_cast<int *>(&sst);
       for (int i = lane; i < kWords; i += kSlice) dst[i] = src[i];
-    }
-    __syncthreads();
-    if (wave == 0) {
-      const bool skip = sst.fresh || sst.op == OP_DONE;
-      const int nt = sst.n_tiles;
       Sums S;
-      if (!skip) {
-        const double *pp = q.part + (size_t)(parity ^ 1) * nt * kNumPart;
 #pragma unroll
-        for (int j = 0; j < kNumPart; ++j) {
-          double acc = (j == kMinSlot) ? 1.7976931348623157e308 : 0.0;
-          for (int k = lane; k < nt; k += kSlice) {
-            const double v = pp[(size_t)k * kNumPart + j];
-            acc = (j == kMinSlot) ? fmin(acc, v) : acc + v;
-          }
-          S.v[j] = (j == kMinSlot) ? wave_min(acc) : wave_sum(acc);
-        }
-      }
+      for (int j = 0; j < kNumPart; ++j) S.v[j] = (j == kMinSlot) ? wave_min(acc[j]) : wave_sum(acc[j]);
+      __builtin_amdgcn_s_waitcnt(0);
+      __builtin_amdgcn_wave_barrier();
       if (lane == 0) {
+        const bool skip = sst.fresh || sst.op == OP_DONE;
         prev_done = sst.op == OP_DONE;
         if (!skip) control(sst, S);
         sst.fresh = 0;
@@ -751,6 +760,7 @@ __global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(
           sst.opcnt[op]++;
         } else if (!prev_done) {
           atomicSub(running, 1);
+          *q.done_flag = 1;
         }
       }
       __syncthreads();
