@@ -100,7 +100,7 @@ def main():
     log(f"batched: {value:.1f} solves/s, {1e3 * total / args.steps:.2f} ms/step, converged {conv}/{args.qps}")
 
     # roofline of mprgp_step over one batched solve: algorithmic bytes / device time
-    launches = max(s["steps"] for s in stats)
+    launches = stats[0]["launches"]
     alg = 0
     s_bytes = 8
     for q, s in zip(qps, stats):

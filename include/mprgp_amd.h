@@ -85,7 +85,7 @@ typedef struct {
   int ev_valid;       /* power vector stored for the next solve                     */
   int refinements;    /* MIXED: outer refinement sweeps                             */
   int steps;          /* device launches this QP took part in                       */
-  int reserved_i[2];
+  int reserved_i[2];  /* [0]: step-kernel launches issued for the whole call        */
   double residual;    /* final ||phi + beta||_2 (FP64)                              */
   double alpha_bar;
   double lambda_max;  /* spectral-radius estimate behind alpha_bar                  */

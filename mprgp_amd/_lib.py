@@ -47,7 +47,9 @@ class Stats(C.Structure):
                 ("elapsed_ms", C.c_double), ("reserved_d", C.c_double * 4)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
+        d["launches"] = self.reserved_i[0]
+        return d
 
 
 class MprgpError(RuntimeError):

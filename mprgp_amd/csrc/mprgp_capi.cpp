@@ -943,6 +943,7 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
       o.alpha_bar = st.alpha_bar;
       o.lambda_max = st.lambda_max;
       o.elapsed_ms = ms;
+      o.reserved_i[0] = (int)launched;  // step-kernel launches issued for the whole batch
       std::memcpy(qps[i]->opcnt, st.opcnt, sizeof(st.opcnt));
     }
     if (!std::isfinite(st.residual)) worst = MPRGP_ENAN;
