@@ -7,9 +7,13 @@ obstacle planes) advanced together from their warm start to KKT residual
 assembled on the device before timing and stay resident; alpha_bar = 2/|A| is
 estimated once per matrix (power method) before timing and passed in.
 
-Also measured: the plain FP64 form solving the same QPs one at a time
-(p50/p99 latency, solves/s), the CPU oracle on a bounded sample of them
-(single core), and the roofline of the solve kernel from algorithmic bytes.
+The headline ``value`` is the batched mixed FP32/FP64 form (``--precision``
+switches it to the plain FP64 form).  Also measured on the same QPs: the
+batched FP64 form, the plain FP64 form one QP at a time (p50/p99 latency,
+solves/s), the mixed form's error against the FP64 results (every QP against
+the GPU FP64 solve, the CPU sample against the oracle), the CPU oracle on a
+bounded sample (single core), and the roofline of the solve kernel from
+algorithmic bytes and device time.
 
 Prints one JSON line.
 """
@@ -41,10 +45,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--qps", type=int, default=256)
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64-batch", action="store_true")
     ap.add_argument("--ts", type=int, default=0, help="wavefronts per workgroup (0: library choice)")
     ap.add_argument("--spw", type=int, default=0, help="slices per wavefront per tile (0: library choice)")
     ap.add_argument("--graph-steps", type=int, default=0)
@@ -75,64 +80,90 @@ def main():
         f"iterations max {max(s['power_iters'] for s in st)}")
 
     prec = M.MIXED if args.precision == "mixed" else M.FP64
-    per = [M.params(tol=args.tol, max_it=4000, alpha_bar=q.alpha_bar, project_first=1,
-                    precision=prec, tiles=args.ts, graph_steps=args.graph_steps) for q in qps]
-    for p in per:
-        p.reserved_i[0] = args.spw
+    mixed = prec == M.MIXED
 
-    def batch_step():
+    def batch_params(precision):
+        per = [M.params(tol=args.tol, max_it=4000, alpha_bar=q.alpha_bar, project_first=1,
+                        precision=precision, tiles=args.ts, graph_steps=args.graph_steps) for q in qps]
+        for p in per:
+            p.reserved_i[0] = args.spw
+        return per
+
+    per = batch_params(prec)
+
+    def batch_step(params):
         for q in qps:
             q.qp.reset_x()
-        return M.solve_batch(ctx, [q.qp for q in qps], per_qp=per)
+        return M.solve_batch(ctx, [q.qp for q in qps], per_qp=params)
 
     for _ in range(args.warmup):
-        batch_step()
+        batch_step(per)
     times, stats = [], None
     tt0 = time.perf_counter()
     for _ in range(args.steps):
         s0 = time.perf_counter()
-        rc, stats = batch_step()
+        rc, stats = batch_step(per)
         times.append(time.perf_counter() - s0)
     total = time.perf_counter() - tt0
     conv = sum(1 for s in stats if s["code"] == 0)
-    dev_ms = [s["elapsed_ms"] for s in stats][0]
     value = args.qps * args.steps / total
-    log(f"batched: {value:.1f} solves/s, {1e3 * total / args.steps:.2f} ms/step, converged {conv}/{args.qps}")
+    log(f"batched {args.precision}: {value:.1f} solves/s, {1e3 * total / args.steps:.2f} ms/step, "
+        f"converged {conv}/{args.qps}")
+    x_head = [q.qp.get_x() for q in qps]
 
-    # roofline of mprgp_step over one batched solve: algorithmic bytes / device time
+    # roofline of mprgp_step over the last batched solve: algorithmic bytes /
+    # device time (HIP events on the solve stream around the launches)
     launches = stats[0]["launches"]
-    alg = 0
-    s_bytes = 8
-    for q, s in zip(qps, stats):
-        cnt = q.qp.op_counts()
-        alg += workloads.solve_bytes(cnt, q.n_nodes, q.n_blocks, q.m, s_bytes)
-    dev_s = dev_ms / 1e3
+    dev_s = stats[0]["elapsed_ms"] / 1e3
+    alg = sum(workloads.solve_bytes(q.qp.op_counts(), q.n_nodes, q.n_blocks, q.m, mixed) for q in qps)
     achieved = alg / dev_s / 1e9
     roof = {"kernel": "mprgp_step", "bound": "hbm", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None, "bytes_per_launch": int(alg / max(launches, 1)),
-            "avg_launch_us": round(1e6 * dev_s / max(launches, 1), 2), "launches": launches}
+            "avg_launch_us": round(1e6 * dev_s / max(launches, 1), 2), "launches": launches,
+            "iterations_total": int(sum(s["iterations"] for s in stats))}
+    log(f"roofline: {roof}")
 
-    plain = None
+    other = None
+    if mixed and not args.no_fp64_batch:
+        per64 = batch_params(M.FP64)
+        batch_step(per64)
+        k = max(1, args.steps // 2)
+        s0 = time.perf_counter()
+        for _ in range(k):
+            rc, st64 = batch_step(per64)
+        t64 = time.perf_counter() - s0
+        other = {"solves_per_s": round(args.qps * k / t64, 2), "ms_per_step": round(1e3 * t64 / k, 3),
+                 "converged": sum(1 for s in st64 if s["code"] == 0),
+                 "iterations_total": int(sum(s["iterations"] for s in st64))}
+        log(f"batched fp64: {other}")
+
+    plain, x_plain = None, None
     if not args.no_plain:
-        lat = []
-        for q, p in zip(qps, per):
+        lat, x_plain = [], []
+        for q in qps:
             q.qp.reset_x()
-            pp = M.params(tol=p.tol, max_it=p.max_it, alpha_bar=p.alpha_bar, project_first=1)
+            pp = M.params(tol=args.tol, max_it=4000, alpha_bar=q.alpha_bar, project_first=1)
             s0 = time.perf_counter()
             q.qp.solve(pp)
             lat.append(time.perf_counter() - s0)
+            x_plain.append(q.qp.get_x())
         lat = np.array(lat) * 1e3
         plain = {"solves_per_s": round(len(lat) / (lat.sum() / 1e3), 2),
                  "p50_ms": round(float(np.percentile(lat, 50)), 3),
                  "p99_ms": round(float(np.percentile(lat, 99)), 3)}
         log(f"plain fp64 one at a time: {plain}")
 
+    err = None
+    if mixed and x_plain is not None:
+        rel = np.array([np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300) for a, b in zip(x_head, x_plain)])
+        err = {"vs_gpu_fp64_max": float(rel.max()), "vs_gpu_fp64_median": float(np.median(rel))}
+
     cpu = None
     if not args.no_cpu:
         import oracle
         order = np.argsort([q.n_nodes for q in qps])
-        done, spent, dof = 0, 0.0, 0
+        done, spent, dof, rel_o = 0, 0.0, 0, []
         for i in order:
             sc = scenes[i]
             tets, d, vol, mass = oracle.fem_setup(sc.n_nodes, sc.tets, sc.X0)
@@ -142,9 +173,10 @@ def main():
             node, normal, c = sc.rows()
             J = problems.rows_to_J(node, normal, sc.n_nodes)
             s0 = time.perf_counter()
-            oracle.decoupled_solve(A, b, J, c, sc.x_n.ravel(), tol=args.tol, max_it=4000,
-                                   alpha_bar=qps[i].alpha_bar, project_first=True)
+            _, xo, _, _ = oracle.decoupled_solve(A, b, J, c, sc.x_n.ravel(), tol=args.tol, max_it=4000,
+                                                 alpha_bar=qps[i].alpha_bar, project_first=True)
             spent += time.perf_counter() - s0
+            rel_o.append(np.linalg.norm(x_head[i] - xo) / np.linalg.norm(xo))
             done += 1
             dof += 3 * sc.n_nodes
             if spent > args.cpu_seconds:
@@ -152,10 +184,14 @@ def main():
         cpu = {"value": round(done / spent, 3), "unit": "solves/s", "cores": 1, "kind": "port",
                "sample": f"{done} smallest of the {args.qps} QPs ({dof} DoF total), oracle FP64, "
                          f"same alpha_bar, {spent:.1f}s; host {platform.processor() or platform.machine()}"}
+        if err is None:
+            err = {}
+        err["vs_oracle_max"] = float(max(rel_o))
+        err["vs_oracle_sample"] = done
         log(f"cpu oracle: {cpu}")
 
     out = {
-        "metric": "MPRGP solves/s at KKT res <=1e-6 (batched, 256 QPs)",
+        "metric": "MPRGP solves/s at KKT res \u22641e-6, batched FP32/FP64 vs plain FP64; p50/p99 ms",
         "value": round(value, 2),
         "unit": "solves/s",
         "n_gpus": 1,
@@ -165,16 +201,18 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == M.FP64 else "f32+f64",
+        "dtype": "f64" if not mixed else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": "batched beams (configs[3])", "qps": args.qps,
+        "config": {"workload": "256 beam contact QPs batched, DoF 5k-60k each", "qps": args.qps,
                    "dof_total": 3 * total_nodes, "blocks_total": total_blocks,
                    "dof_range": [5000, 60000], "tol": args.tol, "precision": args.precision,
                    "converged": conv},
         "batch_p50_ms": round(1e3 * float(np.percentile(times, 50)), 3),
         "batch_p99_ms": round(1e3 * float(np.percentile(times, 99)), 3),
+        "batched_fp64": other,
         "plain_fp64_one_at_a_time": plain,
         "speedup_vs_plain": round(value / plain["solves_per_s"], 2) if plain else None,
+        "solution_rel_err": err,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

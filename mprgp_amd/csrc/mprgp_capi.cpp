@@ -391,7 +391,46 @@ extern "C" void mprgp_matrix_destroy(mprgp_matrix *A) {
   ctx->release(A->d_val, sizeof(double) * A->slots * 9);
   ctx->release(A->d_dinv, sizeof(double) * 3 * A->n_nodes);
   ctx->release(A->d_perm, sizeof(int) * A->n_nodes);
+  ctx->release(A->d_val32, sizeof(float) * A->slots * 9);
   delete A;
+}
+
+// FP32 copy of the values for the mixed form.
+static int ensure_matrix32(mprgp_matrix *A) {
+  mprgp_ctx *ctx = A->ctx;
+  if (!A->d_val32) {
+    if (int rc = ctx->alloc((void **)&A->d_val32, sizeof(float) * A->slots * 9)) return rc;
+  }
+  if (A->version32 != A->version) {
+    hipStream_t s = ctx->stream;
+    MPRGP_HIP(launch_to_f32(s, A->d_val, A->d_val32, (size_t)A->slots * 9));
+    A->version32 = A->version;
+  }
+  return MPRGP_OK;
+}
+
+static int ensure_qp32(mprgp_qp *q) {
+  if (q->base32) return MPRGP_OK;
+  const size_t n3 = 3 * (size_t)q->n;
+  const size_t bytes = (7 * n3 + (size_t)q->n) * sizeof(double) + (3 * n3 + 64) * sizeof(float);
+  if (int rc = q->ctx->alloc(&q->base32, bytes)) return rc;
+  q->bytes32 = bytes;
+  double *dp = (double *)q->base32;
+  double **dvecs[] = {&q->fx, &q->fy, &q->fpy, &q->fb, &q->fg, &q->fphi, &q->fbeta};
+  for (double **v : dvecs) {
+    *v = dp;
+    dp += n3;
+  }
+  q->fcc = dp;
+  dp += q->n;
+  float *fp32 = (float *)dp;
+  float **fvecs[] = {&q->fp, &q->fz, &q->fap};
+  for (float **v : fvecs) {
+    *v = fp32;
+    fp32 += n3;
+  }
+  MPRGP_HIP(hipMemsetAsync(q->base32, 0, q->bytes32, q->ctx->stream));
+  return MPRGP_OK;
 }
 
 // ---------------------------------------------------------------------- QP
@@ -452,6 +491,7 @@ extern "C" void mprgp_qp_destroy(mprgp_qp *q) {
   (void)hipSetDevice(q->ctx->device);
   (void)hipStreamSynchronize(q->ctx->stream);
   q->ctx->release(q->base, q->bytes);
+  q->ctx->release(q->base32, q->bytes32);
   delete q;
 }
 
@@ -672,8 +712,8 @@ static int ensure_tables(mprgp_ctx *ctx, int n_qps, int n_tiles) {
   return MPRGP_OK;
 }
 
-static int get_graph(mprgp_ctx *ctx, int ts, int steps, int grid, hipGraphExec_t *out) {
-  auto key = std::make_tuple(ts, steps, grid);
+static int get_graph(mprgp_ctx *ctx, int ts, int steps, int grid, bool mixed, hipGraphExec_t *out) {
+  auto key = std::make_tuple(ts, steps, grid, (int)mixed);
   auto it = ctx->graphs.find(key);
   if (it != ctx->graphs.end()) {
     *out = it->second;
@@ -682,7 +722,7 @@ static int get_graph(mprgp_ctx *ctx, int ts, int steps, int grid, hipGraphExec_t
   hipStream_t s = ctx->stream;
   MPRGP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   for (int i = 0; i < steps; ++i) {
-    hipError_t e = launch_step(ts, grid, s, ctx->d_qps, ctx->d_tiles, ctx->d_ctrl, ctx->d_ctrl + 1, i & 1);
+    hipError_t e = launch_step(ts, grid, s, ctx->d_qps, ctx->d_tiles, ctx->d_ctrl, ctx->d_ctrl + 1, i & 1, mixed);
     if (e != hipSuccess) {
       hipGraph_t g;
       (void)hipStreamEndCapture(s, &g);
@@ -755,14 +795,18 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
     if (q->ctx != ctx) return fail(MPRGP_EBADINPUT, "qp %d belongs to another context", i);
     if (!q->has_rhs || !q->has_x) return fail(MPRGP_ESTATE, "qp %d: rhs and x must be set", i);
     if (!(p.tol == p.tol) || p.max_it < 0) return fail(MPRGP_EBADINPUT, "qp %d: bad tol/max_it", i);
-    if (p.precision != MPRGP_FP64)
-      return fail(MPRGP_EBADINPUT, "qp %d: precision %d not available in this entry", i, p.precision);
+    if (p.precision != MPRGP_FP64 && p.precision != MPRGP_MIXED)
+      return fail(MPRGP_EBADINPUT, "qp %d: unknown precision %d", i, p.precision);
+    if (p.precision == MPRGP_MIXED && p.variant != MPRGP_MONOTONIC)
+      return fail(MPRGP_EBADINPUT, "qp %d: the mixed form runs the monotone variant", i);
     if (p.variant != MPRGP_MONOTONIC && p.variant != MPRGP_CLASSIC)
       return fail(MPRGP_EBADINPUT, "qp %d: unknown variant %d", i, p.variant);
     for (int j = 0; j < i; ++j)
       if (qps[j] == q) return fail(MPRGP_EBADINPUT, "qp %d appears twice in the batch", i);
     total_nodes += q->n;
   }
+  bool any_mixed = false;
+  for (int i = 0; i < count; ++i) any_mixed |= params[stride ? i : 0].precision == MPRGP_MIXED;
   long total_slices = 0;
   std::vector<int> n_slices(count);
   for (int i = 0; i < count; ++i) total_slices += (n_slices[i] = qps[i]->A->n_slices);
@@ -789,13 +833,25 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
     d.done_flag = ctx->d_flags + i;
     d.sl_off = q->A->d_sl_off;
     d.col = q->A->d_col;
-    d.val = q->A->d_val;
-    d.dinv = q->A->d_dinv;
-    d.x = q->x; d.b = q->b; d.g = q->g; d.p = q->p; d.z = q->z; d.phi = q->phi; d.beta = q->beta;
-    d.y = q->y; d.ap = q->ap; d.py = q->py; d.ev = q->ev; d.pv = q->pv; d.pw = q->pw;
-    d.cn = q->cn; d.cc = q->cc; d.cj = q->cj; d.cs = q->cs; d.cf = q->cf; d.face = q->face;
+    d.d.val = q->A->d_val;
+    d.d.dinv = q->A->d_dinv;
+    d.d.x = q->x; d.d.b = q->b; d.d.g = q->g; d.d.p = q->p; d.d.z = q->z; d.d.phi = q->phi;
+    d.d.beta = q->beta; d.d.y = q->y; d.d.ap = q->ap; d.d.py = q->py;
+    d.d.cn = q->cn; d.d.cc = q->cc; d.d.cj = q->cj; d.d.cs = q->cs;
+    d.ev = q->ev; d.pv = q->pv; d.pw = q->pw;
+    d.cf = q->cf; d.face = q->face;
     d.st = q->st;
     d.part = q->part;
+    const bool mixed = p.precision == MPRGP_MIXED;
+    if (mixed) {
+      if (int rc = ensure_matrix32(q->A)) return rc;
+      if (int rc = ensure_qp32(q)) return rc;
+      d.f.val = q->A->d_val32;
+      d.f.dinv = q->A->d_dinv;
+      d.f.x = q->fx; d.f.b = q->fb; d.f.g = q->fg; d.f.p = q->fp; d.f.z = q->fz; d.f.phi = q->fphi;
+      d.f.beta = q->fbeta; d.f.y = q->fy; d.f.ap = q->fap; d.f.py = q->fpy;
+      d.f.cn = q->cn; d.f.cj = q->cj; d.f.cs = q->cs; d.f.cc = q->fcc;
+    }
     QpState &st = states[i];
     std::memset(&st, 0, sizeof(st));
     st.n_tiles = tiles_of[i];
@@ -810,10 +866,18 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
     st.alpha_bar = power ? 0.0 : p.alpha_bar;
     st.lambda_max = power ? 0.0 : 2.0 / p.alpha_bar;
     st.power_eps = p.power_tol > 0.0 ? p.power_tol : p.tol;
-    st.pow_next = power ? OP_POW_MV : OP_INIT;
+    st.pow_next = power ? OP_POW_MV : (mixed ? OP_R_START : OP_INIT);
     st.op = p.project_first ? OP_PROJX : st.pow_next;
     st.fresh = 1;
-    max_steps = std::max<long>(max_steps, 8L * ((long)p.max_it + 4) + (power ? 2L * 1002 + 4 : 0) + 16);
+    if (mixed) {
+      st.mixed = 1;
+      st.tol_outer = st.tol;
+      st.max_refine = p.max_refine > 0 ? p.max_refine : 8;
+      st.inner_rel = p.inner_tol > 0.0 ? p.inner_tol : 1e-3;
+    }
+    const long sweeps = mixed ? (p.max_refine > 0 ? p.max_refine : 8) + 1 : 1;
+    max_steps = std::max<long>(max_steps, sweeps * (8L * ((long)p.max_it + 4) + 8) +
+                                              (power ? 2L * 1002 + 4 : 0) + 16);
     if (power) {
       if (p.use_ev && q->ev_valid) {
         MPRGP_HIP(hipMemcpyAsync(q->pv, q->ev, sizeof(double) * 3 * q->n, hipMemcpyDeviceToDevice, s));
@@ -855,7 +919,7 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
   if (params[0].graph_steps < 0) steps = -params[0].graph_steps + (params[0].graph_steps & 1);
   hipGraphExec_t ex = nullptr;
   if (use_graph)
-    if (int rc = get_graph(ctx, ts, steps, grid, &ex)) return rc;
+    if (int rc = get_graph(ctx, ts, steps, grid, any_mixed, &ex)) return rc;
   int in_table = count;  // QPs whose tiles are in the table
   std::vector<int> flags(count);
   std::vector<std::vector<Tile>> uploads;  // host buffers of pending table uploads
@@ -869,7 +933,8 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
       MPRGP_HIP(hipGraphLaunch(ex, s));
     } else {
       for (int i = 0; i < steps; ++i)
-        MPRGP_HIP(launch_step(ts, grid, s, ctx->d_qps, ctx->d_tiles, ctx->d_ctrl, ctx->d_ctrl + 1, i & 1));
+        MPRGP_HIP(launch_step(ts, grid, s, ctx->d_qps, ctx->d_tiles, ctx->d_ctrl, ctx->d_ctrl + 1, i & 1,
+                              any_mixed));
     }
     launched += steps;
     MPRGP_HIP(hipMemcpyAsync(ctx->h_running + slot, ctx->d_ctrl + 1, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -903,7 +968,7 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
         in_table = live;
         grid = round_grid(nt);
         if (use_graph)
-          if (int rc = get_graph(ctx, ts, steps, grid, &ex)) return rc;
+          if (int rc = get_graph(ctx, ts, steps, grid, any_mixed, &ex)) return rc;
       }
       // restart the two-deep polling pipeline
       if (int rc = enqueue(k & 1)) return rc;
@@ -931,6 +996,7 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
       mprgp_stats &o = stats[i];
       std::memset(&o, 0, sizeof(o));
       o.code = st.code;
+      o.refinements = st.mixed ? st.refine : 0;
       o.iterations = st.iteration;
       o.num_cg = st.num_cg;
       o.num_exp = st.num_exp;

@@ -10,6 +10,7 @@
 //           node past the end, with zero values); values val[9*blk0 + 64*(9k+e) + l]
 //           for entry e = 3*row + col of the block, so every load of one entry
 //           across a wavefront is 64 consecutive values (512 B for FP64).
+//           The mixed form keeps an FP32 copy of the values in the same layout.
 //   vectors 3 values per node, node-major (x0 y0 z0 x1 ...).
 //   rows    at most one half-space row per node: normal cn[3r..3r+2], offset
 //           cc[r], |n|^2 in cj[r], tangent-Jacobi factor cs[r], flag cf[r]
@@ -47,6 +48,9 @@ enum Op : int {
   OP_CL_EXP,     // classic expansion (first half, elementwise)
   OP_CL_EXP2,    // classic expansion (second half, SpMV)
   OP_CL_PROP1,   // classic proportioning: AD = A beta
+  OP_R_START,    // mixed: FP64 g = A x - b, FP64 KKT norm, correction QP set-up
+  OP_R_UPD,      // mixed: x += d (FP64)
+  OP_R_FINAL,    // mixed: x = P(x) in FP64, active count
   OP_NUM
 };
 
@@ -77,15 +81,38 @@ struct alignas(128) QpState {
   int p_pending;     // classic: p = z - beta p still to apply
   int pow_next;      // op after the power method
   int steps;         // launches this QP took part in
-  int pad0;
+  int mixed;         // inner ops run on the FP32 copies (mixed form)
+  int refine, max_refine;   // mixed: outer sweeps done / allowed
+  int iter_total;           // mixed: inner iterations over all sweeps
+  int num_cg_t, num_exp_t, num_prop_t;
+  int since_best;           // mixed inner: iterations without halving the residual
   double tol, alpha_bar, gamma;
   double alpha, beta_cg, pd, zg, residual, fx, fy;
   double alpha_f, nu, power_eps, lambda_max;
-  double pad1[3];
+  double tol_outer, res64, inner_rel, best_res;
   int opcnt[OP_NUM];  // launches this QP spent in each op
 };
 
-// Device pointers of one QP (FP64 form).
+// Storage of one precision form: FP64 for the plain solve and the outer
+// refinement; for the inner solve of the mixed form, the matrix values and
+// the direction vectors p, z, A p are FP32 (the matrix is ~85% of the
+// traffic).  Everything that carries the contact forces stays FP64: on an
+// active node the gradient's normal component is the multiplier, orders of
+// magnitude above the tangential residual, and any FP32 rounding of g, b, or
+// of the tangent-Jacobi operands (1/diag against the FP64 factor s) leaks
+// eps32*|lambda| into the tangential part, which either floors the KKT norm
+// or pushes CG steps through the planes.
+template <class T>
+struct Vecs {
+  const T *val;               // SELL values
+  const double *dinv;         // 1/diag(A)
+  double *x, *y, *py;         // iterate and trial points
+  double *b, *g, *phi, *beta; // gradient side
+  T *p, *z, *ap;              // search direction side
+  const double *cn, *cj, *cs; // normal, |n|^2, tangent-Jacobi factor
+  double *cc;                 // offset (mixed inner: shifted c - n.x)
+};
+
 struct QpDev {
   int n_nodes;
   int n_slices;
@@ -93,11 +120,9 @@ struct QpDev {
   int n_tiles;       // tiles of this QP (partial-sum slots per launch)
   const int *sl_off;
   const int *col;
-  const double *val;
-  const double *dinv;   // 1/diag(A), 3 per node
-  double *x, *b, *g, *p, *z, *phi, *beta, *y, *ap, *py;
-  double *ev, *pv, *pw; // power method vectors
-  const double *cn, *cc, *cj, *cs;
+  Vecs<double> d;
+  Vecs<float> f;
+  double *ev, *pv, *pw; // power method vectors (FP64)
   const unsigned char *cf;
   unsigned char *face;
   QpState *st;          // st[0], st[1] (launch parity)

@@ -31,7 +31,7 @@ int fail(int code, const char *fmt, ...);
 
 // kernel launch wrappers (mprgp_kernels.hip)
 hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
-                       const int *n_tiles, int *running, int parity);
+                       const int *n_tiles, int *running, int parity, bool mixed);
 hipError_t launch_gather_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n);
 hipError_t launch_scatter_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n);
 hipError_t launch_setup_diag(hipStream_t s, const int *sl_off, const int *col, const double *val,
@@ -40,6 +40,7 @@ hipError_t launch_setup_rows(hipStream_t s, const double *dinv, const double *cn
                              const unsigned char *cf, double *cs, int n);
 hipError_t launch_spmv_plain(hipStream_t s, const int *sl_off, const int *col, const double *val,
                              const double *x, double *y, int n, int accumulate);
+hipError_t launch_to_f32(hipStream_t s, const double *in, float *out, size_t n);
 
 // Builds the SELL-64 device matrix from a node-order block CSR (blocks may be
 // NULL for a zero matrix of that pattern).
@@ -63,8 +64,8 @@ struct mprgp_ctx {
   int flags_cap = 0;
   int *h_running = nullptr;   // pinned, 2 slots
   hipEvent_t t0 = nullptr, t1 = nullptr, q[2] = {nullptr, nullptr};
-  std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
-  std::map<std::tuple<int, int, int>, hipGraph_t> graph_defs;
+  std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
+  std::map<std::tuple<int, int, int, int>, hipGraph_t> graph_defs;
   // autotuner cache: node-count bucket -> (tiles per workgroup)
   std::map<int, int> tuned_ts;
   // scratch shared by calls on this context (calls on one context never overlap)
@@ -93,6 +94,9 @@ struct mprgp_matrix {
   int *d_perm = nullptr;
   size_t bytes = 0;
   int version = 0;  // bumped on value updates
+  // FP32 copy of the values for the mixed form (made on first use, refreshed on updates)
+  float *d_val32 = nullptr;
+  int version32 = -1;
 };
 
 struct mprgp_qp {
@@ -113,4 +117,11 @@ struct mprgp_qp {
   bool has_rhs = false, has_x = false;
   int ev_valid = 0;
   int opcnt[mprgp::OP_NUM] = {};  // device launches per op in the last solve
+  // FP32 storage of the mixed form's inner solve (allocated on first use)
+  void *base32 = nullptr;
+  size_t bytes32 = 0;
+  double *fx = nullptr, *fy = nullptr, *fpy = nullptr;  // FP64 iterate/trial points of the inner solve
+  double *fb = nullptr, *fg = nullptr, *fphi = nullptr, *fbeta = nullptr;  // FP64 gradient side
+  float *fp = nullptr, *fz = nullptr, *fap = nullptr;   // FP32 direction side
+  double *fcc = nullptr;  // shifted offsets c - n.x of the correction QP
 };

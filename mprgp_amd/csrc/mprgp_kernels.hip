@@ -2,38 +2,50 @@
 //
 // One kernel, `mprgp_step`, advances every QP of a batch by one operation of
 // the reference's MPRGP control flow (MPRGPSolver.h:148-456).  Each QP keeps
-// its own state machine on the device (QpState::op); a launch executes, for
-// every tile of every QP, the op that QP is in, and the last tile of a QP to
-// finish reduces that QP's partial sums in a fixed order and chooses the
-// QP's next op.  The host never reads a scalar of the iteration: it replays
-// launches (captured in a hipGraph) and polls a done-counter.
+// its own state machine on the device (QpState::op).  A launch first applies,
+// in every tile, the control transition of the op the QP executed in the
+// previous launch (from that launch's per-tile partial sums, reduced in a
+// fixed order), then executes the resulting op on the tile's rows.  The host
+// never reads a scalar of the iteration: it replays launches (captured in a
+// hipGraph) and polls a done-counter.
 //
 // Work mapping: one wavefront per SELL slice, one lane per block row
 // (= node).  SpMV ops stream the slice's 3x3 blocks with fully coalesced
-// loads (the slice layout puts one block entry of 64 consecutive rows in
-// 512 contiguous bytes) and gather the input vector per neighbour node.
-// Elementwise ops (projection, gradient split, preconditioner, vector
-// updates) run in the same lane-per-node mapping and are fused into the SpMV
-// ops wherever the reference's data flow allows.
+// loads (one block entry of 64 consecutive rows is contiguous) and gather
+// the input vector per neighbour node.  Elementwise ops (projection, gradient
+// split, preconditioner, vector updates) run in the same lane-per-node
+// mapping and are fused into the SpMV ops wherever the data flow allows.
+//
+// Precision: arithmetic and every reduction run in FP64.  The plain form
+// stores matrix and vectors in FP64; the mixed form runs the inner solve on
+// an FP32 copy of the matrix (and FP32 search directions) inside an FP64
+// iterative refinement: the residual b - A x, the KKT norm and the update
+// x += d use the FP64 data.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "mprgp_dev.h"
 
 namespace mprgp {
 
+constexpr double kDblMax = 1.7976931348623157e308;
+
 struct V3 {
   double x, y, z;
 };
 
-__device__ __forceinline__ V3 ld3(const double *v, int r) {
-  const double *p = v + 3 * (size_t)r;
-  return V3{p[0], p[1], p[2]};
+template <class T>
+__device__ __forceinline__ V3 ld3(const T *v, int r) {
+  const T *p = v + 3 * (size_t)r;
+  return V3{(double)p[0], (double)p[1], (double)p[2]};
 }
-__device__ __forceinline__ void st3(double *v, int r, V3 a) {
-  double *p = v + 3 * (size_t)r;
-  p[0] = a.x;
-  p[1] = a.y;
-  p[2] = a.z;
+template <class T>
+__device__ __forceinline__ void st3(T *v, int r, V3 a) {
+  T *p = v + 3 * (size_t)r;
+  p[0] = (T)a.x;
+  p[1] = (T)a.y;
+  p[2] = (T)a.z;
 }
 __device__ __forceinline__ V3 add(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
 __device__ __forceinline__ V3 sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
@@ -50,14 +62,15 @@ struct Row {
   bool has;
 };
 
-__device__ __forceinline__ Row ldrow(const QpDev &q, int r) {
+template <class T>
+__device__ __forceinline__ Row ldrow(const QpDev &q, const Vecs<T> &v, int r) {
   Row R;
   R.has = q.cf[r] != 0;
   if (R.has) {
-    R.n = ld3(q.cn, r);
-    R.c = q.cc[r];
-    R.jj = q.cj[r];
-    R.s = q.cs[r];
+    R.n = ld3(v.cn, r);
+    R.c = v.cc[r];
+    R.jj = v.cj[r];
+    R.s = v.cs[r];
   } else {
     R.n = V3{0.0, 0.0, 0.0};
     R.c = R.jj = R.s = 0.0;
@@ -106,10 +119,11 @@ __device__ __forceinline__ V3 beta_of(const Row &R, bool face, V3 g, V3 phi) {
 // DiagonalDecouplePreconSolver::solve (MPRGPPrecondition.h:91-105):
 // z = D^-1 g - (n o D^-1) lambda with lambda = s (n o D^-1).g on active rows;
 // with preconditioning off or no rows at all, z = phi.
-__device__ __forceinline__ V3 precond_of(const QpDev &q, bool on, const Row &R, bool face, int r,
-                                         V3 g, V3 phi) {
+template <class T>
+__device__ __forceinline__ V3 precond_of(const Vecs<T> &v, bool on, const Row &R, bool face, int r, V3 g,
+                                         V3 phi) {
   if (!on) return phi;
-  V3 d = ld3(q.dinv, r);
+  V3 d = ld3(v.dinv, r);
   V3 z{d.x * g.x, d.y * g.y, d.z * g.z};
   if (face) {
     V3 jm{R.n.x * d.x, R.n.y * d.y, R.n.z * d.z};
@@ -125,15 +139,16 @@ __device__ __forceinline__ V3 precond_of(const QpDev &q, bool on, const Row &R, 
 // returns NV input vectors at a neighbour node.  Padding slots hold a valid
 // column and zero values, so the loop has no branch and the column indices,
 // matrix values and gathers of U blocks are all in flight together.
-template <int NV, class G>
-__device__ __forceinline__ void spmv_row(const QpDev &q, int s, int lane, G gather, V3 (&acc)[NV]) {
+template <int NV, class T, class G>
+__device__ __forceinline__ void spmv_row(const int *sl_off, const int *col, const T *val, int s, int lane,
+                                         G gather, V3 (&acc)[NV]) {
   constexpr int U = NV == 1 ? 4 : 2;
 #pragma unroll
   for (int t = 0; t < NV; ++t) acc[t] = V3{0.0, 0.0, 0.0};
-  const int b0 = q.sl_off[s];
-  const int W = (q.sl_off[s + 1] - b0) / kSlice;
-  const int *cp = q.col + b0 + lane;
-  const double *vp = q.val + (size_t)b0 * 9 + lane;
+  const int b0 = sl_off[s];
+  const int W = (sl_off[s + 1] - b0) / kSlice;
+  const int *cp = col + b0 + lane;
+  const T *vp = val + (size_t)b0 * 9 + lane;
   int k = 0;
   for (; k + U <= W; k += U) {
     int j[U];
@@ -144,7 +159,7 @@ __device__ __forceinline__ void spmv_row(const QpDev &q, int s, int lane, G gath
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
-      for (int e = 0; e < 9; ++e) a[u][e] = vp[(size_t)(k + u) * (9 * kSlice) + e * kSlice];
+      for (int e = 0; e < 9; ++e) a[u][e] = (double)vp[(size_t)(k + u) * (9 * kSlice) + e * kSlice];
 #pragma unroll
     for (int u = 0; u < U; ++u) gather(j[u], xs[u]);
 #pragma unroll
@@ -166,7 +181,7 @@ __device__ __forceinline__ void spmv_row(const QpDev &q, int s, int lane, G gath
     const int j = cp[k * kSlice];
     double a[9];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) a[e] = vp[(size_t)k * (9 * kSlice) + e * kSlice];
+    for (int e = 0; e < 9; ++e) a[e] = (double)vp[(size_t)k * (9 * kSlice) + e * kSlice];
     V3 xs[NV];
     gather(j, xs);
 #pragma unroll
@@ -206,8 +221,6 @@ struct Sums {
   double v[kNumPart];
 };
 
-__device__ __forceinline__ void to_final(QpState &st) { st.op = OP_FINAL; }
-
 // Op chosen after the outer loop's `iteration < max_iterations` test.
 __device__ __forceinline__ void next_outer(QpState &st) {
   st.op = (st.iteration < st.max_it) ? OP_TOP : OP_FINAL;
@@ -219,6 +232,25 @@ __device__ __forceinline__ int proportional(const QpState &st, double bb, double
   double r = xpx_phi * (1.0 / st.alpha_bar);
   double phitphi = r > 0.0 ? r : 0.0;
   return bb <= st.gamma * st.gamma * phitphi;
+}
+
+// Mixed form only: the inner solve on FP32 data has a precision floor; when
+// 128 iterations pass without halving the best residual the inner solve
+// ends and the FP64 refinement takes over.  The FP64 form never stops early.
+__device__ __forceinline__ bool inner_stalled(QpState &st) {
+  if (!st.mixed) return false;
+  if (st.residual < 0.5 * st.best_res) {
+    st.best_res = st.residual;
+    st.since_best = st.iteration;
+    return false;
+  }
+  return st.iteration - st.since_best > 128;
+}
+
+__device__ __forceinline__ void finish_power(QpState &st) {
+  st.lambda_max = st.nu;
+  st.alpha_bar = 2.0 / st.nu;
+  st.op = st.mixed ? OP_R_START : OP_INIT;
 }
 
 // Consumes the reduced partial sums of the op a QP executed in the previous
@@ -247,9 +279,7 @@ __device__ void control(QpState &st, const Sums &S) {
       } else {
         st.power_it++;
         if (st.power_it > 1000) {  // cap: ev untouched
-          st.lambda_max = st.nu;
-          st.alpha_bar = 2.0 / st.nu;
-          st.op = OP_INIT;
+          finish_power(st);
         } else {
           st.op = OP_POW_MV;  // v and w swap roles (power_it parity)
         }
@@ -258,20 +288,21 @@ __device__ void control(QpState &st, const Sums &S) {
     }
     case OP_POW_SAVE:
       st.ev_valid = 1;
-      st.lambda_max = st.nu;
-      st.alpha_bar = 2.0 / st.nu;
-      st.op = OP_INIT;
+      finish_power(st);
       break;
     case OP_INIT:
       st.iteration = st.num_cg = st.num_exp = st.num_prop = 0;
       st.residual = 0.0;
+      st.y_valid = 0;
+      st.best_res = kDblMax;
+      st.since_best = 0;
       st.op = (st.max_it > 0) ? OP_TOP : OP_FINAL;
       break;
     case OP_TOP: {
       st.p_pending = 0;
       st.residual = sqrt(S.v[0]);
-      if (st.residual <= st.tol) {
-        to_final(st);
+      if (st.residual <= st.tol || inner_stalled(st)) {
+        st.op = OP_FINAL;
         break;
       }
       st.prop = proportional(st, S.v[1], S.v[2]);
@@ -312,7 +343,7 @@ __device__ void control(QpState &st, const Sums &S) {
       if (st.prop && st.residual > st.tol && st.fy <= st.fx && st.iteration < st.max_it) {
         st.op = OP_CG_A;
       } else if (st.residual <= st.tol) {
-        to_final(st);
+        st.op = OP_FINAL;
       } else {
         st.op = (st.prop || !st.y_valid || !st.y_feas) ? OP_EXP1 : OP_PROP1;
       }
@@ -325,7 +356,8 @@ __device__ void control(QpState &st, const Sums &S) {
     case OP_CG_B:
       st.residual = sqrt(S.v[0]);
       st.prop = proportional(st, S.v[1], S.v[2]);
-      st.op = OP_CG_AP;
+      // a stalled inner solve (mixed form) ends through the loop's exit path
+      st.op = inner_stalled(st) ? OP_FINAL : OP_CG_AP;
       break;
     case OP_EXP1:
       st.op = OP_EXP2;
@@ -347,17 +379,13 @@ __device__ void control(QpState &st, const Sums &S) {
     case OP_PROP2:
       st.num_prop++;
       st.iteration++;
-      if (st.variant == VAR_CLASSIC) {
-        st.op = (st.iteration < st.max_it) ? OP_TOP : OP_FINAL;
-      } else {
-        next_outer(st);
-      }
+      next_outer(st);
       break;
     case OP_CL_AP: {
       st.pd = S.v[0];
       st.zg = S.v[1];
       double acg = st.zg / st.pd;
-      double af = (acg == 1.7976931348623157e308) ? acg : (acg + kEps);
+      double af = (acg == kDblMax) ? acg : (acg + kEps);
       double lim = S.v[kMinSlot];
       if (lim < af) af = lim;
       st.alpha = acg;
@@ -370,7 +398,7 @@ __device__ void control(QpState &st, const Sums &S) {
       st.p_pending = 1;
       st.num_cg++;
       st.iteration++;
-      st.op = (st.iteration < st.max_it) ? OP_TOP : OP_FINAL;
+      next_outer(st);
       break;
     case OP_CL_EXP:
       st.op = OP_CL_EXP2;
@@ -378,15 +406,50 @@ __device__ void control(QpState &st, const Sums &S) {
     case OP_CL_EXP2:
       st.num_exp++;
       st.iteration++;
-      st.op = (st.iteration < st.max_it) ? OP_TOP : OP_FINAL;
+      next_outer(st);
       break;
     case OP_CL_PROP1:
       st.alpha = S.v[1] / S.v[0];
       st.op = OP_PROP2;
       break;
     case OP_FINAL:
+      if (st.mixed) {  // inner solve done: fold its counts, apply x += d
+        st.iter_total += st.iteration;
+        st.num_cg_t += st.num_cg;
+        st.num_exp_t += st.num_exp;
+        st.num_prop_t += st.num_prop;
+        st.op = OP_R_UPD;
+        break;
+      }
       st.active = (int)S.v[0];
       st.code = st.residual <= st.tol ? 0 : -1;
+      st.op = OP_DONE;
+      break;
+    case OP_R_START: {
+      st.res64 = sqrt(S.v[0]);
+      if (!(st.res64 == st.res64)) {  // NaN: stop
+        st.op = OP_R_FINAL;
+      } else if (st.res64 <= st.tol_outer || st.refine >= st.max_refine) {
+        st.op = OP_R_FINAL;
+      } else {
+        double t = st.inner_rel * st.res64;
+        st.tol = t > 0.5 * st.tol_outer ? t : 0.5 * st.tol_outer;
+        st.op = OP_INIT;
+      }
+      break;
+    }
+    case OP_R_UPD:
+      st.refine++;
+      st.op = OP_R_START;
+      break;
+    case OP_R_FINAL:
+      st.active = (int)S.v[0];
+      st.residual = st.res64;
+      st.code = st.res64 <= st.tol_outer ? 0 : -1;
+      st.iteration = st.iter_total;
+      st.num_cg = st.num_cg_t;
+      st.num_exp = st.num_exp_t;
+      st.num_prop = st.num_prop_t;
       st.op = OP_DONE;
       break;
     default:
@@ -396,78 +459,49 @@ __device__ void control(QpState &st, const Sums &S) {
 
 // --------------------------------------------------------------- kernel
 
-// One row (= node r of slice s) of op `op`; partial sums accumulate into P.
-__device__ __forceinline__ void row_work(const QpDev &q, int s, int lane, int r, int op, const QpState &st,
-                                         double (&P)[kNumPart]) {
+// One row (= node r of slice s) of a solve op on storage T; partial sums
+// accumulate into P (FP64).
+template <class T>
+__device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s, int lane, int r, int op,
+                                         const QpState &st, double (&P)[kNumPart]) {
   const bool pre = st.precond && st.has_rows;
-
   switch (op) {
-    case OP_PROJX: {
-      Row R = ldrow(q, r);
-      st3(q.x, r, proj(R, ld3(q.x, r)));
-      break;
-    }
-    case OP_POW_MV: {
-      // v alternates between pv and pw with the iteration parity
-      const double *v = (st.power_it & 1) ? q.pw : q.pv;
-      double *w = (st.power_it & 1) ? q.pv : q.pw;
-      V3 acc[1];
-      spmv_row<1>(q, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(v, j); }, acc);
-      st3(w, r, acc[0]);
-      P[0] += dot3(acc[0], acc[0]);
-      break;
-    }
-    case OP_POW_NRM: {
-      const double *v = (st.power_it & 1) ? q.pw : q.pv;
-      double *w = (st.power_it & 1) ? q.pv : q.pw;
-      V3 a = ld3(w, r), b = ld3(v, r);
-      a = V3{a.x / st.nu, a.y / st.nu, a.z / st.nu};
-      st3(w, r, a);
-      V3 dlt = sub(a, b);
-      P[0] += dot3(dlt, dlt);
-      break;
-    }
-    case OP_POW_SAVE: {
-      const double *v = (st.power_it & 1) ? q.pw : q.pv;
-      st3(q.ev, r, ld3(v, r));
-      break;
-    }
     case OP_INIT:
     case OP_EXP3:
     case OP_CL_EXP2: {
       // g = A x - b ; DECIDE_FACE(x) ; phi ; z ; p = z   (MPRGPSolver.h:148-156, 263-268, 210-215)
       V3 acc[1];
-      const double *x = q.x;
-      spmv_row<1>(q, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
-      V3 g = sub(acc[0], ld3(q.b, r));
-      Row R = ldrow(q, r);
-      bool f = face_of(R, ld3(q.x, r));
+      const double *x = v.x;
+      spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
+      V3 g = sub(acc[0], ld3(v.b, r));
+      Row R = ldrow(q, v, r);
+      bool f = face_of(R, ld3(v.x, r));
       V3 ph = phi_of(R, f, g);
-      V3 z = precond_of(q, pre, R, f, r, g, ph);
-      st3(q.g, r, g);
-      st3(q.phi, r, ph);
-      st3(q.z, r, z);
-      st3(q.p, r, z);
+      V3 z = precond_of(v, pre, R, f, r, g, ph);
+      st3(v.g, r, g);
+      st3(v.phi, r, ph);
+      st3(v.z, r, z);
+      st3(v.p, r, z);
       q.face[r] = f;
-      if (op == OP_EXP3) st3(q.y, r, ld3(q.py, r));
+      if (op == OP_EXP3) st3(v.y, r, ld3(v.py, r));
       break;
     }
     case OP_TOP: {
       // computeGradients (MPRGPSolver.h:163-173) + proportional (:174-181)
-      Row R = ldrow(q, r);
+      Row R = ldrow(q, v, r);
       bool f = q.face[r];
       if (st.p_pending) {  // classic CGStep tail: p = z - beta p (MPRGPSolver.h:192-193)
-        st3(q.p, r, axmy(ld3(q.z, r), st.beta_cg, ld3(q.p, r)));
+        st3(v.p, r, axmy(ld3(v.z, r), st.beta_cg, ld3(v.p, r)));
       }
-      V3 g = ld3(q.g, r);
+      V3 g = ld3(v.g, r);
       V3 ph = phi_of(R, f, g);
       V3 be = beta_of(R, f, g, ph);
-      st3(q.phi, r, ph);
-      st3(q.beta, r, be);
+      st3(v.phi, r, ph);
+      st3(v.beta, r, be);
       V3 gp = add(ph, be);
       P[0] += dot3(gp, gp);
       P[1] += dot3(be, be);
-      V3 x = ld3(q.x, r);
+      V3 x = ld3(v.x, r);
       V3 px = proj(R, axmy(x, st.alpha_bar, ph));
       P[2] += dot3(sub(x, px), ph);
       break;
@@ -475,18 +509,19 @@ __device__ __forceinline__ void row_work(const QpDev &q, int s, int lane, int r,
     case OP_CG_AP:
     case OP_CL_AP: {
       V3 acc[1];
-      const double *p = q.p;
-      spmv_row<1>(q, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(p, j); }, acc);
-      st3(q.ap, r, acc[0]);
-      V3 pr = ld3(q.p, r);
+      const T *p = v.p;
+      spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(p, j); }, acc);
+      st3(v.ap, r, acc[0]);
+      V3 pr = ld3(v.p, r);
       P[0] += dot3(pr, acc[0]);
-      P[1] += dot3(ld3(q.z, r), ld3(q.g, r));
+      P[1] += dot3(ld3(v.z, r), ld3(v.g, r));
       if (op == OP_CL_AP) {
         // stepLimit (MPRGPProjection.h:263-282)
-        Row R = ldrow(q, r);
-        double lim = 1.7976931348623157e308;
+        Row R = ldrow(q, v, r);
+        double lim = kDblMax;
         if (R.has) {
-          double jx = dot3(R.n, ld3(q.x, r)), jd = dot3(R.n, pr);
+          V3 xr = ld3(v.x, r);
+          double jx = dot3(R.n, xr), jd = dot3(R.n, pr);
           if (jd > kEps && jx > R.c) lim = (jx - R.c) / jd;
         }
         P[kMinSlot] = fmin(P[kMinSlot], lim);
@@ -494,32 +529,32 @@ __device__ __forceinline__ void row_work(const QpDev &q, int s, int lane, int r,
       break;
     }
     case OP_CG_PY: {
-      Row R = ldrow(q, r);
-      V3 y = axmy(ld3(q.x, r), st.alpha, ld3(q.p, r));
-      st3(q.y, r, y);
-      st3(q.py, r, proj(R, y));
+      Row R = ldrow(q, v, r);
+      V3 y = axmy(ld3(v.x, r), st.alpha, ld3(v.p, r));
+      st3(v.y, r, y);
+      st3(v.py, r, proj(R, y));
       P[0] += infeasible(R, y);
       break;
     }
     case OP_CG_Y: {
-      const double *py = q.py;
-      V3 pyr = ld3(q.py, r);
-      V3 br = ld3(q.b, r);
+      const double *py = v.py;
+      V3 pyr = ld3(v.py, r);
+      V3 br = ld3(v.b, r);
       if (st.cg_entry) {
-        const double *x = q.x;
+        const double *x = v.x;
         V3 acc[2];
-        spmv_row<2>(q, s, lane, [&](int j, V3 (&xs)[2]) {
+        spmv_row<2>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[2]) {
           xs[0] = ld3(py, j);
           xs[1] = ld3(x, j);
         }, acc);
-        V3 xr = ld3(q.x, r);
+        V3 xr = ld3(v.x, r);
         P[0] += dot3(pyr, acc[0]);
         P[1] += dot3(pyr, br);
         P[2] += dot3(xr, acc[1]);
         P[3] += dot3(xr, br);
       } else {
         V3 acc[1];
-        spmv_row<1>(q, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(py, j); }, acc);
+        spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(py, j); }, acc);
         P[0] += dot3(pyr, acc[0]);
         P[1] += dot3(pyr, br);
       }
@@ -527,56 +562,56 @@ __device__ __forceinline__ void row_work(const QpDev &q, int s, int lane, int r,
     }
     case OP_CG_A: {
       // CGMonotonicStep body, first half (MPRGPSolver.h:293-297)
-      Row R = ldrow(q, r);
+      Row R = ldrow(q, v, r);
       bool f = q.face[r];
-      V3 ap = ld3(q.ap, r);
-      st3(q.x, r, ld3(q.y, r));
-      V3 g = axmy(ld3(q.g, r), st.alpha, ap);
+      V3 ap = ld3(v.ap, r);
+      st3(v.x, r, ld3(v.y, r));
+      V3 g = axmy(ld3(v.g, r), st.alpha, ap);
       V3 ph = phi_of(R, f, g);
-      V3 z = precond_of(q, pre, R, f, r, g, ph);
-      st3(q.g, r, g);
-      st3(q.phi, r, ph);
-      st3(q.z, r, z);
+      V3 z = precond_of(v, pre, R, f, r, g, ph);
+      st3(v.g, r, g);
+      st3(v.phi, r, ph);
+      st3(v.z, r, z);
       P[0] += dot3(z, ap);
       break;
     }
     case OP_CG_B: {
       // p = z - beta p ; computeGradients(g, false) ; proportional (MPRGPSolver.h:298-302)
-      Row R = ldrow(q, r);
+      Row R = ldrow(q, v, r);
       bool f = q.face[r];
-      st3(q.p, r, axmy(ld3(q.z, r), st.beta_cg, ld3(q.p, r)));
-      V3 g = ld3(q.g, r);
-      V3 ph = ld3(q.phi, r);
+      st3(v.p, r, axmy(ld3(v.z, r), st.beta_cg, ld3(v.p, r)));
+      V3 g = ld3(v.g, r);
+      V3 ph = ld3(v.phi, r);
       V3 be = beta_of(R, f, g, ph);
-      st3(q.beta, r, be);
+      st3(v.beta, r, be);
       V3 gp = add(ph, be);
       P[0] += dot3(gp, gp);
       P[1] += dot3(be, be);
-      V3 x = ld3(q.x, r);
+      V3 x = ld3(v.x, r);
       V3 px = proj(R, axmy(x, st.alpha_bar, ph));
       P[2] += dot3(sub(x, px), ph);
       break;
     }
     case OP_EXP1: {
-      Row R = ldrow(q, r);
-      st3(q.y, r, proj(R, ld3(q.x, r)));
+      Row R = ldrow(q, v, r);
+      st3(v.y, r, proj(R, ld3(v.x, r)));
       break;
     }
     case OP_EXP2: {
       // ExpMonotonicStep, first half (MPRGPSolver.h:255-261): the new y goes
       // to py because neighbours still gather y in this launch.
       V3 acc[1];
-      const double *y = q.y;
-      spmv_row<1>(q, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(y, j); }, acc);
-      V3 g = sub(acc[0], ld3(q.b, r));
-      Row R = ldrow(q, r);
-      V3 yr = ld3(q.y, r);
+      const double *y = v.y;
+      spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(y, j); }, acc);
+      V3 g = sub(acc[0], ld3(v.b, r));
+      Row R = ldrow(q, v, r);
+      V3 yr = ld3(v.y, r);
       bool f = face_of(R, yr);
       V3 ph = phi_of(R, f, g);
       V3 yn = axmy(yr, st.alpha_bar, ph);
-      st3(q.g, r, g);
-      st3(q.py, r, yn);
-      st3(q.x, r, proj(R, yn));
+      st3(v.g, r, g);
+      st3(v.py, r, yn);
+      st3(v.x, r, proj(R, yn));
       q.face[r] = f;
       P[0] += infeasible(R, yn);
       break;
@@ -584,77 +619,78 @@ __device__ __forceinline__ void row_work(const QpDev &q, int s, int lane, int r,
     case OP_PROP1: {
       // g = A x - b ; AD = A beta ; beta.AD ; g.beta  (MPRGPSolver.h:435-437, 224-231)
       V3 acc[2];
-      const double *x = q.x, *be = q.beta;
-      spmv_row<2>(q, s, lane, [&](int j, V3 (&xs)[2]) {
+      const double *x = v.x;
+      const double *be = v.beta;
+      spmv_row<2>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[2]) {
         xs[0] = ld3(x, j);
         xs[1] = ld3(be, j);
       }, acc);
-      V3 g = sub(acc[0], ld3(q.b, r));
-      st3(q.g, r, g);
-      st3(q.ap, r, acc[1]);
-      V3 d = ld3(q.beta, r);
+      V3 g = sub(acc[0], ld3(v.b, r));
+      st3(v.g, r, g);
+      st3(v.ap, r, acc[1]);
+      V3 d = ld3(v.beta, r);
       P[0] += dot3(d, acc[1]);
       P[1] += dot3(g, d);
       break;
     }
     case OP_CL_PROP1: {
       V3 acc[1];
-      const double *be = q.beta;
-      spmv_row<1>(q, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(be, j); }, acc);
-      st3(q.ap, r, acc[0]);
-      V3 d = ld3(q.beta, r);
+      const double *be = v.beta;
+      spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(be, j); }, acc);
+      st3(v.ap, r, acc[0]);
+      V3 d = ld3(v.beta, r);
       P[0] += dot3(d, acc[0]);
-      P[1] += dot3(ld3(q.g, r), d);
+      P[1] += dot3(ld3(v.g, r), d);
       break;
     }
     case OP_PROP2: {
       // PropStep tail (MPRGPSolver.h:232-241)
-      Row R = ldrow(q, r);
-      V3 xn = proj(R, axmy(ld3(q.x, r), st.alpha, ld3(q.beta, r)));
-      V3 g = axmy(ld3(q.g, r), st.alpha, ld3(q.ap, r));
+      Row R = ldrow(q, v, r);
+      V3 xn = proj(R, axmy(ld3(v.x, r), st.alpha, ld3(v.beta, r)));
+      V3 g = axmy(ld3(v.g, r), st.alpha, ld3(v.ap, r));
       bool f = face_of(R, xn);
       V3 ph = phi_of(R, f, g);
-      V3 z = precond_of(q, pre, R, f, r, g, ph);
-      st3(q.x, r, xn);
-      st3(q.g, r, g);
-      st3(q.phi, r, ph);
-      st3(q.z, r, z);
-      st3(q.p, r, z);
+      V3 z = precond_of(v, pre, R, f, r, g, ph);
+      st3(v.x, r, xn);
+      st3(v.g, r, g);
+      st3(v.phi, r, ph);
+      st3(v.z, r, z);
+      st3(v.p, r, z);
       q.face[r] = f;
       break;
     }
     case OP_CL_CG: {
       // CGStep (MPRGPSolver.h:188-191); p is updated at the next TOP
-      Row R = ldrow(q, r);
+      Row R = ldrow(q, v, r);
       bool f = q.face[r];
-      V3 ap = ld3(q.ap, r);
-      st3(q.x, r, axmy(ld3(q.x, r), st.alpha, ld3(q.p, r)));
-      V3 g = axmy(ld3(q.g, r), st.alpha, ap);
+      V3 ap = ld3(v.ap, r);
+      st3(v.x, r, axmy(ld3(v.x, r), st.alpha, ld3(v.p, r)));
+      V3 g = axmy(ld3(v.g, r), st.alpha, ap);
       V3 ph = phi_of(R, f, g);
-      V3 z = precond_of(q, pre, R, f, r, g, ph);
-      st3(q.g, r, g);
-      st3(q.phi, r, ph);
-      st3(q.z, r, z);
+      V3 z = precond_of(v, pre, R, f, r, g, ph);
+      st3(v.g, r, g);
+      st3(v.phi, r, ph);
+      st3(v.z, r, z);
       P[0] += dot3(z, ap);
       break;
     }
     case OP_CL_EXP: {
       // ExpStep first half (MPRGPSolver.h:202-208)
-      Row R = ldrow(q, r);
-      V3 xt = axmy(ld3(q.x, r), st.alpha_f, ld3(q.p, r));
-      V3 g = axmy(ld3(q.g, r), st.alpha_f, ld3(q.ap, r));
+      Row R = ldrow(q, v, r);
+      V3 xt = axmy(ld3(v.x, r), st.alpha_f, ld3(v.p, r));
+      V3 g = axmy(ld3(v.g, r), st.alpha_f, ld3(v.ap, r));
       bool f = face_of(R, xt);
       V3 ph = phi_of(R, f, g);
       xt = axmy(xt, st.alpha_bar, ph);
-      st3(q.x, r, proj(R, xt));
-      st3(q.g, r, g);
+      st3(v.x, r, proj(R, xt));
+      st3(v.g, r, g);
       q.face[r] = f;
       break;
     }
     case OP_FINAL: {
       if (st.variant != VAR_CLASSIC) {
-        Row R = ldrow(q, r);
-        st3(q.x, r, proj(R, ld3(q.x, r)));
+        Row R = ldrow(q, v, r);
+        st3(v.x, r, proj(R, ld3(v.x, r)));
       }
       P[0] += q.face[r] ? 1.0 : 0.0;
       break;
@@ -664,16 +700,102 @@ __device__ __forceinline__ void row_work(const QpDev &q, int s, int lane, int r,
   }
 }
 
+// Ops on the FP64 data only: projection of the warm start, the power
+// method, and the outer refinement of the mixed form.
+__device__ __forceinline__ void row_work_fp64(const QpDev &q, int s, int lane, int r, int op,
+                                              const QpState &st, double (&P)[kNumPart]) {
+  const Vecs<double> &v = q.d;
+  switch (op) {
+    case OP_PROJX: {
+      Row R = ldrow(q, v, r);
+      st3(v.x, r, proj(R, ld3(v.x, r)));
+      break;
+    }
+    case OP_POW_MV: {
+      // v and w alternate between pv and pw with the iteration parity
+      const double *pv = (st.power_it & 1) ? q.pw : q.pv;
+      double *w = (st.power_it & 1) ? q.pv : q.pw;
+      V3 acc[1];
+      spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(pv, j); }, acc);
+      st3(w, r, acc[0]);
+      P[0] += dot3(acc[0], acc[0]);
+      break;
+    }
+    case OP_POW_NRM: {
+      const double *pv = (st.power_it & 1) ? q.pw : q.pv;
+      double *w = (st.power_it & 1) ? q.pv : q.pw;
+      V3 a = ld3(w, r), b = ld3(pv, r);
+      a = V3{a.x / st.nu, a.y / st.nu, a.z / st.nu};
+      st3(w, r, a);
+      V3 dlt = sub(a, b);
+      P[0] += dot3(dlt, dlt);
+      break;
+    }
+    case OP_POW_SAVE: {
+      const double *pv = (st.power_it & 1) ? q.pw : q.pv;
+      st3(q.ev, r, ld3(pv, r));
+      break;
+    }
+    case OP_R_START: {
+      // FP64 residual and KKT norm at x; the correction QP
+      //   min 1/2 d'Ad - r'd  s.t.  n.d >= c - n.x,  r = b - A x
+      // is set up in the FP32 storage with d = 0.
+      V3 acc[1];
+      const double *x = v.x;
+      spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
+      V3 g = sub(acc[0], ld3(v.b, r));
+      Row R = ldrow(q, v, r);
+      V3 xr = ld3(v.x, r);
+      bool f = face_of(R, xr);
+      V3 ph = phi_of(R, f, g);
+      V3 be = beta_of(R, f, g, ph);
+      V3 gp = add(ph, be);
+      P[0] += dot3(gp, gp);
+      q.face[r] = f;
+      st3(q.f.b, r, V3{-g.x, -g.y, -g.z});
+      st3(q.f.x, r, V3{0.0, 0.0, 0.0});
+      if (R.has) q.f.cc[r] = R.c - dot3(R.n, xr);
+      break;
+    }
+    case OP_R_UPD: {
+      st3(v.x, r, add(ld3(v.x, r), ld3(q.f.x, r)));
+      break;
+    }
+    case OP_R_FINAL: {
+      Row R = ldrow(q, v, r);
+      st3(v.x, r, proj(R, ld3(v.x, r)));
+      P[0] += q.face[r] ? 1.0 : 0.0;
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+__device__ __forceinline__ bool fp64_op(int op) {
+  return op == OP_PROJX || op == OP_POW_MV || op == OP_POW_NRM || op == OP_POW_SAVE ||
+         op == OP_R_START || op == OP_R_UPD || op == OP_R_FINAL;
+}
+
 // A tile's rows: wave w takes slices w, w+TS, w+2TS, ... of the tile, so the
 // waves of a workgroup stream neighbouring slices at the same time.
-template <int TS>
+template <int TS, bool MX>
 __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &st, double (&P)[kNumPart]) {
   const int wave = threadIdx.x / kSlice;
   const int lane = threadIdx.x % kSlice;
+  const bool f64 = fp64_op(op);
   for (int i = wave; i < t.slices; i += TS) {
     const int s = t.slice0 + i;
     const int r = s * kSlice + lane;
-    if (r < q.n_nodes) row_work(q, s, lane, r, op, st, P);
+    if (r >= q.n_nodes) continue;
+    if (f64) {
+      row_work_fp64(q, s, lane, r, op, st, P);
+    } else {
+      if constexpr (MX)
+        row_work<float>(q, q.f, s, lane, r, op, st, P);
+      else
+        row_work<double>(q, q.d, s, lane, r, op, st, P);
+    }
   }
 }
 
@@ -681,7 +803,8 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
 #define MPRGP_STEP_MIN_WAVES 3
 #endif
 
-template <int TS>
+// MX: the batch runs the mixed form (inner ops on the FP32 copies).
+template <int TS, bool MX>
 __global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(const QpDev *__restrict__ qps,
                                                            const Tile *__restrict__ tiles,
                                                            const int *__restrict__ n_tiles_dev,
@@ -703,7 +826,7 @@ __global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(
       const double *pp = q.part + (size_t)(parity ^ 1) * nt * kNumPart;
       double acc[kNumPart];
 #pragma unroll
-      for (int j = 0; j < kNumPart; ++j) acc[j] = (j == kMinSlot) ? 1.7976931348623157e308 : 0.0;
+      for (int j = 0; j < kNumPart; ++j) acc[j] = (j == kMinSlot) ? kDblMax : 0.0;
       for (int k = lane; k < nt; k += kSlice) {
 #pragma unroll
         for (int j = 0; j < kNumPart; ++j) {
@@ -733,8 +856,8 @@ __global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(
       double P[kNumPart];
 #pragma unroll
       for (int j = 0; j < kNumPart; ++j) P[j] = 0.0;
-      P[kMinSlot] = 1.7976931348623157e308;
-      tile_work<TS>(q, t, op, st, P);
+      P[kMinSlot] = kDblMax;
+      tile_work<TS, MX>(q, t, op, st, P);
       // wave then block reduction in a fixed order
 #pragma unroll
       for (int j = 0; j < kMinSlot; ++j) P[j] = wave_sum(P[j]);
@@ -776,7 +899,7 @@ __global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(
 
 // ------------------------------------------------------- helper kernels
 
-// out[perm_i] = in[i] style permutations between user and internal order.
+// out[i] = in[perm[i]]: user node order -> internal order.
 __global__ void gather_nodes(const double *__restrict__ in, double *__restrict__ out,
                              const int *__restrict__ perm, int n_nodes) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -786,6 +909,7 @@ __global__ void gather_nodes(const double *__restrict__ in, double *__restrict__
   out[3 * (size_t)i + 1] = in[3 * (size_t)o + 1];
   out[3 * (size_t)i + 2] = in[3 * (size_t)o + 2];
 }
+// out[perm[i]] = in[i]: internal order -> user node order.
 __global__ void scatter_nodes(const double *__restrict__ in, double *__restrict__ out,
                               const int *__restrict__ perm, int n_nodes) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -796,8 +920,7 @@ __global__ void scatter_nodes(const double *__restrict__ in, double *__restrict_
   out[3 * (size_t)o + 2] = in[3 * (size_t)i + 2];
 }
 
-// dinv = 1/diag(A) from the diagonal block of each row; cs = 1/((n o dinv).n)
-// (MPRGPPrecondition.h:44-49, 81-88).
+// dinv = 1/diag(A) from the diagonal block of each row (MPRGPPrecondition.h:44-49).
 __global__ void setup_diag(const int *__restrict__ sl_off, const int *__restrict__ col,
                            const double *__restrict__ val, double *__restrict__ dinv, int n_nodes) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -819,6 +942,8 @@ __global__ void setup_diag(const int *__restrict__ sl_off, const int *__restrict
   dinv[3 * (size_t)r + 1] = 1.0 / d1;
   dinv[3 * (size_t)r + 2] = 1.0 / d2;
 }
+
+// cs = 1/((n o dinv).n) per row (MPRGPPrecondition.h:81-88).
 __global__ void setup_rows(const double *__restrict__ dinv, const double *__restrict__ cn,
                            const unsigned char *__restrict__ cf, double *__restrict__ cs, int n_nodes) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -835,6 +960,13 @@ __global__ void setup_rows(const double *__restrict__ dinv, const double *__rest
   cs[r] = 1.0 / s;
 }
 
+// FP32 copies for the mixed form.
+__global__ void to_f32(const double *__restrict__ in, float *__restrict__ out, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) out[i] = (float)in[i];
+}
+
 // Plain SELL SpMV y = A x (used by tests and by the assembly's A x1 term).
 __global__ void __launch_bounds__(256) spmv_plain(const int *__restrict__ sl_off,
                                                   const int *__restrict__ col,
@@ -844,12 +976,8 @@ __global__ void __launch_bounds__(256) spmv_plain(const int *__restrict__ sl_off
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   int s = r / kSlice, lane = r % kSlice;
   if (s * kSlice >= n_nodes) return;
-  QpDev q{};
-  q.sl_off = sl_off;
-  q.col = col;
-  q.val = val;
   V3 acc[1];
-  spmv_row<1>(q, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
+  spmv_row<1>(sl_off, col, val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
   if (r < n_nodes) {
     if (accumulate) acc[0] = add(acc[0], ld3(y, r));
     st3(y, r, acc[0]);
@@ -858,16 +986,23 @@ __global__ void __launch_bounds__(256) spmv_plain(const int *__restrict__ sl_off
 
 // Host-callable launch wrappers ------------------------------------------
 
-hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
-                       const int *n_tiles, int *running, int parity) {
+template <bool MX>
+static hipError_t launch_step_t(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
+                                const int *n_tiles, int *running, int parity) {
   switch (ts) {
-    case 1: mprgp_step<1><<<grid, 1 * kSlice, 0, s>>>(qps, tiles, n_tiles, running, parity); break;
-    case 2: mprgp_step<2><<<grid, 2 * kSlice, 0, s>>>(qps, tiles, n_tiles, running, parity); break;
-    case 4: mprgp_step<4><<<grid, 4 * kSlice, 0, s>>>(qps, tiles, n_tiles, running, parity); break;
-    case 8: mprgp_step<8><<<grid, 8 * kSlice, 0, s>>>(qps, tiles, n_tiles, running, parity); break;
+    case 1: mprgp_step<1, MX><<<grid, 1 * kSlice, 0, s>>>(qps, tiles, n_tiles, running, parity); break;
+    case 2: mprgp_step<2, MX><<<grid, 2 * kSlice, 0, s>>>(qps, tiles, n_tiles, running, parity); break;
+    case 4: mprgp_step<4, MX><<<grid, 4 * kSlice, 0, s>>>(qps, tiles, n_tiles, running, parity); break;
+    case 8: mprgp_step<8, MX><<<grid, 8 * kSlice, 0, s>>>(qps, tiles, n_tiles, running, parity); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
+                       const int *n_tiles, int *running, int parity, bool mixed) {
+  return mixed ? launch_step_t<true>(ts, grid, s, qps, tiles, n_tiles, running, parity)
+               : launch_step_t<false>(ts, grid, s, qps, tiles, n_tiles, running, parity);
 }
 
 hipError_t launch_gather_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n) {
@@ -890,6 +1025,12 @@ hipError_t launch_setup_rows(hipStream_t s, const double *dinv, const double *cn
                              const unsigned char *cf, double *cs, int n) {
   if (n <= 0) return hipSuccess;
   setup_rows<<<(n + 255) / 256, 256, 0, s>>>(dinv, cn, cf, cs, n);
+  return hipGetLastError();
+}
+hipError_t launch_to_f32(hipStream_t s, const double *in, float *out, size_t n) {
+  if (n == 0) return hipSuccess;
+  size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+  to_f32<<<(unsigned)blocks, 256, 0, s>>>(in, out, n);
   return hipGetLastError();
 }
 hipError_t launch_spmv_plain(hipStream_t s, const int *sl_off, const int *col, const double *val,

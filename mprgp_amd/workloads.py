@@ -16,7 +16,7 @@ from mprgp_amd import problems
 # op indices of the device state machine (mprgp_dev.h)
 OPS = ["DONE", "PROJX", "POW_MV", "POW_NRM", "POW_SAVE", "INIT", "TOP", "CG_AP", "CG_PY", "CG_Y",
        "CG_A", "CG_B", "EXP1", "EXP2", "EXP3", "PROP1", "PROP2", "FINAL", "CL_AP", "CL_CG",
-       "CL_EXP", "CL_EXP2", "CL_PROP1"]
+       "CL_EXP", "CL_EXP2", "CL_PROP1", "R_START", "R_UPD", "R_FINAL"]
 
 
 class DeviceQP:
@@ -36,54 +36,78 @@ class DeviceQP:
         self.alpha_bar = 0.0
 
 
+# Streams each op touches, every stream counted once (ideal traffic): "M" the
+# matrix of the op's precision form and "M64" the FP64 matrix (values, int32
+# column indices, slice offsets; a fused two-vector SpMV still reads it once);
+# vector names stand for 3 values per node (gathered SpMV inputs count once);
+# "rows" is the per-node flag byte plus the 6 FP64 words of each constraint
+# row (normal, offset, |n|^2, tangent-Jacobi factor); "cc" one FP64 offset per
+# row; "face" one byte per node.
+_STREAMS = {
+    "PROJX": ("rows", "x", "x"),
+    "POW_MV": ("M64", "pv", "pw"),
+    "POW_NRM": ("pw", "pv", "pw"),
+    "POW_SAVE": ("pv", "ev"),
+    "INIT": ("M", "x", "b", "rows", "dinv", "g", "phi", "z", "p", "face"),
+    "EXP3": ("M", "x", "b", "rows", "dinv", "g", "phi", "z", "p", "face", "py", "y"),
+    "CL_EXP2": ("M", "x", "b", "rows", "dinv", "g", "phi", "z", "p", "face"),
+    "TOP": ("rows", "face", "g", "x", "phi", "beta"),
+    "CG_AP": ("M", "p", "ap", "z", "g"),
+    "CL_AP": ("M", "p", "ap", "z", "g", "rows", "x"),
+    "CG_PY": ("rows", "x", "p", "y", "py"),
+    "CG_Y": ("M", "py", "b"),
+    "CG_A": ("rows", "face", "ap", "y", "g", "dinv", "x", "g", "phi", "z"),
+    "CG_B": ("rows", "face", "z", "p", "g", "phi", "x", "p", "beta"),
+    "EXP1": ("rows", "x", "y"),
+    "EXP2": ("M", "y", "b", "rows", "g", "py", "x", "face"),
+    "PROP1": ("M", "x", "beta", "b", "g", "ap"),
+    "PROP2": ("rows", "x", "beta", "g", "ap", "dinv", "x", "g", "phi", "z", "p", "face"),
+    "FINAL": ("rows", "x", "face", "x"),
+    "CL_CG": ("rows", "face", "ap", "x", "p", "g", "dinv", "x", "g", "phi", "z"),
+    "CL_EXP": ("rows", "x", "p", "g", "ap", "x", "g", "face"),
+    "CL_PROP1": ("M", "beta", "g", "ap"),
+    "R_START": ("M64", "x64", "b64", "rows", "face", "b", "x", "cc"),
+    "R_UPD": ("x64", "x", "x64"),
+    "R_FINAL": ("rows", "x64", "face", "x64"),
+}
+
+# FP32 streams of the mixed form's inner solve; everything else is FP64.
+MIXED_F32 = {"M", "p", "z", "ap"}
+
+
 def spmv_bytes(N, nb, s=8):
     """Matrix once, input vector once, output vector once (index bytes int32)."""
     return nb * (9 * s + 4) + 4 * (N + 1) + 2 * 3 * N * s
 
 
-def op_bytes(op, N, nb, m, s=8, precond=True):
+def op_bytes(op, N, nb, m, mixed=False, precond=True):
     """Algorithmic HBM bytes of one device op on one QP (ideal traffic: every
-    stream read or written once)."""
-    v = 3 * N * s
-    rows = N + m * 6 * s          # flag per node, normal/c/|n|^2/factor per row
-    face = N
-    dinv = v if precond else 0
-    mat = nb * (9 * s + 4) + 4 * (N + 1)
-    spmv = spmv_bytes(N, nb, s)
+    stream read or written once).  ``mixed``: the inner-solve ops read the
+    FP32 matrix and FP32 direction vectors."""
     name = OPS[op]
-    table = {
-        "PROJX": 2 * v + rows,
-        "POW_MV": spmv,
-        "POW_NRM": 3 * v,
-        "POW_SAVE": 2 * v,
-        "INIT": spmv + v + 3 * v + rows + face + dinv,
-        "EXP3": spmv + v + 3 * v + rows + face + dinv + 2 * v,
-        "CL_EXP2": spmv + v + 3 * v + rows + face + dinv,
-        "TOP": 5 * v + face + rows,
-        "CG_AP": spmv + 2 * v,
-        "CL_AP": spmv + 3 * v + rows,
-        "CG_PY": 4 * v + rows,
-        "CG_Y": mat + v + v,
-        "CG_A": 7 * v + dinv + face + rows,
-        "CG_B": 7 * v + face + rows,
-        "EXP1": 2 * v + rows,
-        "EXP2": spmv + v + 2 * v + face + rows,
-        "PROP1": mat + 2 * v + 2 * v + v,
-        "PROP2": 9 * v + face + rows + dinv,
-        "FINAL": 2 * v + rows + face,
-        "CL_CG": 8 * v + dinv + face + rows,
-        "CL_EXP": 6 * v + face + rows,
-        "CL_PROP1": spmv + v,
-    }
-    return table.get(name, 0)
+    total = 0
+    for st in _STREAMS.get(name, ()):
+        if st == "rows":
+            total += N + 48 * m
+        elif st == "cc":
+            total += 8 * m
+        elif st == "face":
+            total += N
+        elif st in ("M", "M64"):
+            s = 4 if (mixed and st == "M") else 8
+            total += nb * (9 * s + 4) + 4 * (N + 1)
+        elif st == "dinv" and not precond:
+            continue
+        else:
+            s = 4 if (mixed and st in MIXED_F32) else 8
+            total += 3 * N * s
+    return total
 
 
-def solve_bytes(counts, N, nb, m, s=8, precond=True, cg_entries=0):
+def solve_bytes(counts, N, nb, m, mixed=False, precond=True):
     """Algorithmic bytes of a whole solve from its per-op launch counts."""
     total = 0
     for op, k in enumerate(counts[:len(OPS)]):
         if k:
-            total += int(k) * op_bytes(op, N, nb, m, s, precond)
-    # a CG entry evaluates f(x) as well: a second input vector on CG_Y
-    total += cg_entries * 3 * N * s
+            total += int(k) * op_bytes(op, N, nb, m, mixed, precond)
     return total

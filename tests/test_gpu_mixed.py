@@ -1,0 +1,79 @@
+"""Mixed-precision form: FP32 inner solves inside FP64 iterative refinement.
+
+Every solve reports its error against the FP64 oracle (printed), and is held
+to: KKT residual (FP64, at the returned x) <= tol, solution relative error
+<= 1e-6 and objective relative gap <= 1e-9 at tol 1e-8, and the same active
+set except nodes within 1e-8 of their plane."""
+import numpy as np
+import pytest
+
+import mprgp_amd as M
+import oracle
+from mprgp_amd import problems
+from tests.common import kkt_report, scene_qp
+
+pytestmark = pytest.mark.gpu
+
+
+def slack(x, node, normal, c):
+    return np.einsum("ij,ij->i", normal, x.reshape(-1, 3)[node]) - c
+
+
+def run_pair(ctx, sc, tol):
+    A, b, node, normal, c, x0 = scene_qp(sc)
+    J = problems.rows_to_J(node, normal, sc.n_nodes)
+    ab = 2.0 / oracle.spec_rad(A, 1e-10)[0]
+    code, xr, sr, _ = oracle.decoupled_solve(A, b, J, c, x0, tol=tol, max_it=50000, alpha_bar=ab)
+    Ad = ctx.matrix_csr(A)
+    q = Ad.qp()
+    q.set_rows(node, normal, c)
+    q.set_rhs(b)
+    q.set_x(x0)
+    rc, sm = q.solve(tol=tol, max_it=50000, alpha_bar=ab, precision=M.MIXED)
+    xm = q.get_x()
+    q.reset_x()
+    rc64, s64 = q.solve(tol=tol, max_it=50000, alpha_bar=ab)
+    f = lambda x: 0.5 * x @ (A @ x) - b @ x
+    err = np.linalg.norm(xm - xr) / np.linalg.norm(xr)
+    gap = abs(f(xm) - f(xr)) / abs(f(xr))
+    print(f"{sc.name}: mixed rel err {err:.2e}, objective gap {gap:.2e}, refinements {sm['refinements']}, "
+          f"iterations mixed {sm['iterations']} / fp64 gpu {s64['iterations']} / oracle {sr['iterations']}")
+    return A, b, node, normal, c, xm, xr, sm, sr, err, gap
+
+
+@pytest.mark.parametrize("which", ["config1", "config1_flat", "config3"])
+def test_mixed_against_fp64_oracle(ctx, which):
+    sc = getattr(problems, which)()
+    A, b, node, normal, c, xm, xr, sm, sr, err, gap = run_pair(ctx, sc, 1e-8)
+    assert sm["code"] == 0, sm
+    assert sm["residual"] <= 1e-8
+    rep = kkt_report(A, b, node, normal, c, xm)
+    assert rep["infeas"] <= 1e-12
+    assert err <= 1e-6 and gap <= 1e-9
+    sg, srr = slack(xm, node, normal, c), slack(xr, node, normal, c)
+    differ = (np.abs(sg) < 1e-9) != (np.abs(srr) < 1e-9)
+    near = (np.abs(sg) < 1e-8) | (np.abs(srr) < 1e-8)
+    assert not np.any(differ & ~near)
+
+
+def test_mixed_batch_equals_alone(ctx):
+    scenes = [problems.config1(seed=s) for s in (21, 22, 23)]
+    qs, alone = [], []
+    for sc in scenes:
+        A, b, node, normal, c, x0 = scene_qp(sc)
+        q = ctx.matrix_csr(A).qp()
+        q.set_rows(node, normal, c)
+        q.set_rhs(b)
+        q.set_x(x0)
+        ab = 2.0 / oracle.spec_rad(A, 1e-10)[0]
+        rc, st = q.solve(tol=1e-7, max_it=5000, alpha_bar=ab, precision=M.MIXED)
+        alone.append((q.get_x(), st))
+        qs.append((q, ab))
+    for q, _ in qs:
+        q.reset_x()
+    rc, stats = M.solve_batch(ctx, [q for q, _ in qs],
+                              per_qp=[M.params(tol=1e-7, max_it=5000, alpha_bar=ab, precision=M.MIXED)
+                                      for _, ab in qs])
+    for (q, _), (xa, sa), sb in zip(qs, alone, stats):
+        np.testing.assert_array_equal(q.get_x(), xa)
+        assert sb["iterations"] == sa["iterations"] and sb["refinements"] == sa["refinements"]
