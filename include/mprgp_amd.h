@@ -17,9 +17,11 @@
  *   FEMSystem::buildSystem               src/ElasticSolver/FEMSystem.cpp:167-192
  *   MaterialEnergy::eval (LINEAR/COROT)  src/ElasticSolver/FEMEnergy.cpp:263-304, 657-1057
  *
- * Threading: a context owns one HIP stream; calls on one context must not
- * overlap.  Different contexts are independent.  No host-global mutable state
- * besides a thread-local last-error string.
+ * Threading: a context owns one HIP stream; calls that use it are serialised
+ * by the context (several threads may share a context).  An engine runs its
+ * own worker thread and stream on a context; a QP handed to an engine belongs
+ * to it until its ticket completes.  No host-global mutable state besides a
+ * thread-local last-error string.
  */
 #ifndef MPRGP_AMD_H
 #define MPRGP_AMD_H
@@ -39,7 +41,8 @@ extern "C" {
 #define MPRGP_ENAN (-5)          /* non-finite residual reached                        */
 #define MPRGP_ENODEV (-6)        /* no usable GPU                                      */
 #define MPRGP_EBUDGET (-7)       /* HBM budget of the context exceeded                 */
-#define MPRGP_ESTATE (-8)        /* object not ready (e.g. rhs not set)                */
+#define MPRGP_ESTATE (-8)        /* object not ready (e.g. rhs not set) or busy        */
+#define MPRGP_EPENDING (-9)      /* engine ticket still running (wait timed out)       */
 
 /* solver variants */
 #define MPRGP_MONOTONIC 0 /* MPRGPMonotonic (MPRGPSolver.h:400-456), the simulator's */
@@ -48,6 +51,9 @@ extern "C" {
 /* FEM materials (MaterialEnergy TYPE values used on the path) */
 #define MPRGP_LINEAR 1
 #define MPRGP_COROTATIONAL 2
+/* flag or-ed into the material of mprgp_fem_assemble: write b only and keep
+ * the matrix of the last full assembly (lagged stiffness between re-assemblies) */
+#define MPRGP_RHS_ONLY 0x100
 
 /* precision forms */
 #define MPRGP_FP64 0  /* every vector and reduction in FP64                        */
@@ -178,6 +184,51 @@ int mprgp_fem_assemble(mprgp_fem *fem, int material, const double *x_n, const do
                        const double *gravity, double h, mprgp_qp *qp);
 mprgp_matrix *mprgp_fem_matrix(mprgp_fem *fem);
 void mprgp_fem_destroy(mprgp_fem *fem);
+
+/* ---- serving engine (continuous batching) --------------------------------
+ * No counterpart in the reference, which calls MPRGPDecoupledCon::solve once
+ * per frame and per body (MprgpFemSolver.cpp:532-533); the engine is the
+ * request loop around that call for many simulation sessions: a worker
+ * thread keeps one batch of QPs advancing on the device and, between graph
+ * replays, retires finished QPs and admits queued ones as slots and the HBM
+ * budget allow.  Sessions keep their matrices and warm starts resident in
+ * their QPs; a submitted QP is solved from its current x (the previous
+ * solution unless set_x was called) exactly as mprgp_qp_solve would solve
+ * it with the same tile shape: results are bit-identical alone or in any mix.
+ */
+typedef struct mprgp_engine mprgp_engine;
+typedef struct mprgp_engine_config {
+  int max_active;      /* QPs in the batch at once (0 = 256)                          */
+  int tiles;           /* wavefronts per workgroup, 1/2/4/8 (0 = 4)                   */
+  int graph_steps;     /* launches per captured graph replay (0 = 8)                  */
+  int reserved_i[5];
+  size_t hbm_budget;   /* bytes of matrices + QP vectors of admitted solves (0 = none) */
+  double reserved_d[4];
+} mprgp_engine_config;
+
+typedef struct mprgp_engine_info {
+  long long submitted, completed, rounds, launches;
+  int active, queued, peak_active, reserved_i[5];
+  size_t active_bytes, peak_bytes;
+  double reserved_d[4];
+} mprgp_engine_info;
+
+void mprgp_engine_config_default(mprgp_engine_config *cfg);
+int mprgp_engine_create(mprgp_ctx *ctx, const mprgp_engine_config *cfg, mprgp_engine **out);
+/* Queues a solve of qp (rhs, rows and x set).  The QP is busy until its ticket
+ * is waited for.  MPRGP_EBUDGET if this QP alone exceeds the budget. */
+int mprgp_engine_submit(mprgp_engine *eng, mprgp_qp *qp, const mprgp_params *params, long long *ticket);
+/* Waits up to timeout_ms (< 0: forever) for a ticket; returns its solve code
+ * (0, -1, or an error) and fills stats (elapsed_ms = submit-to-completion
+ * latency, reserved_d[0] = queue wait in ms, reserved_i[0] = launches while
+ * admitted), or MPRGP_EPENDING on time-out.  A ticket is waited for once. */
+int mprgp_engine_wait(mprgp_engine *eng, long long ticket, int timeout_ms, mprgp_stats *stats);
+/* As mprgp_engine_wait for whichever ticket completed first (in completion
+ * order); *ticket receives it.  MPRGP_ESTATE if no ticket is outstanding. */
+int mprgp_engine_wait_any(mprgp_engine *eng, int timeout_ms, long long *ticket, mprgp_stats *stats);
+int mprgp_engine_get_info(mprgp_engine *eng, mprgp_engine_info *info);
+/* Finishes every submitted solve, then stops the worker. */
+void mprgp_engine_destroy(mprgp_engine *eng);
 
 #ifdef __cplusplus
 }

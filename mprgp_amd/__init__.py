@@ -24,12 +24,12 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (CLASSIC, COROTATIONAL, FP64, LINEAR, MIXED, MONOTONIC, MprgpError, Params,
-                   Stats, check, last_error, load)
+from ._lib import (CLASSIC, COROTATIONAL, FP64, LINEAR, MIXED, MONOTONIC, RHS_ONLY, MprgpError,
+                   Params, Stats, check, last_error, load)
 
-__all__ = ["Context", "Matrix", "QP", "Fem", "MPRGPDecoupledCon", "solve_batch", "params",
+__all__ = ["Context", "Matrix", "QP", "Fem", "Engine", "MPRGPDecoupledCon", "solve_batch", "params",
            "MprgpError", "MONOTONIC", "CLASSIC", "FP64", "MIXED", "LINEAR", "COROTATIONAL",
-           "device_count"]
+           "RHS_ONLY", "device_count"]
 
 
 def _f64(a, n=None):
@@ -71,7 +71,9 @@ class Context:
 
     def close(self):
         if self.h:
-            for o in reversed(self._objs):
+            # engines first: they may still hold QPs
+            for o in [o for o in self._objs if isinstance(o, Engine)] + \
+                     [o for o in reversed(self._objs) if not isinstance(o, Engine)]:
                 o.close()
             self._objs = []
             load().mprgp_ctx_destroy(self.h)
@@ -271,6 +273,63 @@ def solve_batch(ctx: Context, qps, p=None, per_qp=None, **kw):
     rc = load().mprgp_solve_batch(ctx.h, n, C.cast(arr, C.POINTER(C.c_void_p)), ps, stride, st)
     check(rc, allow=(_lib.OK, _lib.NOT_CONVERGED, _lib.ENAN))
     return rc, [s.as_dict() for s in st]
+
+
+class Engine:
+    """Continuous-batching solve engine (``mprgp_engine_*``): ``submit`` queues a
+    device-resident QP and returns a ticket at once; a native worker thread
+    admits it into the running batch between graph replays (slots and the HBM
+    budget permitting); ``wait`` returns its code and stats (``elapsed_ms`` is
+    the submit-to-completion latency, ``queue_ms`` the wait for admission)."""
+
+    def __init__(self, ctx: Context, max_active=256, tiles=4, graph_steps=8, hbm_budget=0):
+        cfg = _lib.EngineConfig()
+        load().mprgp_engine_config_default(C.byref(cfg))
+        cfg.max_active, cfg.tiles, cfg.graph_steps = int(max_active), int(tiles), int(graph_steps)
+        cfg.hbm_budget = int(hbm_budget)
+        self.h = C.c_void_p()
+        check(load().mprgp_engine_create(ctx.h, C.byref(cfg), C.byref(self.h)))
+        self.ctx = ctx
+        ctx._objs.append(self)
+
+    def close(self):
+        if self.h:
+            load().mprgp_engine_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def submit(self, qp: "QP", p: Params | None = None, **kw) -> int:
+        p = params(**kw) if p is None else p
+        t = C.c_longlong()
+        check(load().mprgp_engine_submit(self.h, qp.h, C.byref(p), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1):
+        """Returns (rc, stats) or None when the ticket is still running after timeout_ms."""
+        st = Stats()
+        rc = load().mprgp_engine_wait(self.h, C.c_longlong(ticket), int(timeout_ms), C.byref(st))
+        if rc == _lib.EPENDING:
+            return None
+        check(rc, allow=(_lib.OK, _lib.NOT_CONVERGED, _lib.ENAN))
+        d = st.as_dict()
+        d["queue_ms"] = st.reserved_d[0]
+        return rc, d
+
+    def wait_any(self, timeout_ms: int = -1):
+        """Returns (ticket, rc, stats) of the first completed ticket, or None on time-out."""
+        st = Stats()
+        t = C.c_longlong()
+        rc = load().mprgp_engine_wait_any(self.h, int(timeout_ms), C.byref(t), C.byref(st))
+        if rc == _lib.EPENDING:
+            return None
+        check(rc, allow=(_lib.OK, _lib.NOT_CONVERGED, _lib.ENAN))
+        d = st.as_dict()
+        d["queue_ms"] = st.reserved_d[0]
+        return t.value, rc, d
+
+    def info(self):
+        i = _lib.EngineInfo()
+        check(load().mprgp_engine_get_info(self.h, C.byref(i)))
+        return i.as_dict()
 
 
 class MPRGPDecoupledCon:

@@ -20,6 +20,7 @@ ENAN = -5
 ENODEV = -6
 EBUDGET = -7
 ESTATE = -8
+EPENDING = -9
 
 MONOTONIC = 0
 CLASSIC = 1
@@ -27,6 +28,7 @@ FP64 = 0
 MIXED = 1
 LINEAR = 1
 COROTATIONAL = 2
+RHS_ONLY = 0x100
 
 
 class Params(C.Structure):
@@ -50,6 +52,21 @@ class Stats(C.Structure):
         d = {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
         d["launches"] = self.reserved_i[0]
         return d
+
+
+class EngineConfig(C.Structure):
+    _fields_ = [("max_active", C.c_int), ("tiles", C.c_int), ("graph_steps", C.c_int),
+                ("reserved_i", C.c_int * 5), ("hbm_budget", C.c_size_t), ("reserved_d", C.c_double * 4)]
+
+
+class EngineInfo(C.Structure):
+    _fields_ = [("submitted", C.c_longlong), ("completed", C.c_longlong), ("rounds", C.c_longlong),
+                ("launches", C.c_longlong), ("active", C.c_int), ("queued", C.c_int),
+                ("peak_active", C.c_int), ("reserved_i", C.c_int * 5), ("active_bytes", C.c_size_t),
+                ("peak_bytes", C.c_size_t), ("reserved_d", C.c_double * 4)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
 
 
 class MprgpError(RuntimeError):
@@ -100,6 +117,13 @@ SIGNATURES = {
     "mprgp_fem_assemble": (I, [P, I, P, P, P, D, P]),
     "mprgp_fem_matrix": (P, [P]),
     "mprgp_fem_destroy": (None, [P]),
+    "mprgp_engine_config_default": (None, [C.POINTER(EngineConfig)]),
+    "mprgp_engine_create": (I, [P, C.POINTER(EngineConfig), PP]),
+    "mprgp_engine_submit": (I, [P, P, C.POINTER(Params), C.POINTER(C.c_longlong)]),
+    "mprgp_engine_wait": (I, [P, C.c_longlong, I, C.POINTER(Stats)]),
+    "mprgp_engine_wait_any": (I, [P, I, C.POINTER(C.c_longlong), C.POINTER(Stats)]),
+    "mprgp_engine_get_info": (I, [P, C.POINTER(EngineInfo)]),
+    "mprgp_engine_destroy": (None, [P]),
 }
 
 

@@ -50,6 +50,7 @@ using namespace mprgp;
 int mprgp_ctx::alloc(void **p, size_t bytes) {
   *p = nullptr;
   if (bytes == 0) bytes = 256;
+  std::lock_guard<std::mutex> lock(amu);
   if (budget && in_use + bytes > budget)
     return fail(MPRGP_EBUDGET, "HBM budget exceeded: %zu in use + %zu requested > %zu", in_use, bytes,
                 budget);
@@ -66,6 +67,7 @@ int mprgp_ctx::alloc(void **p, size_t bytes) {
 void mprgp_ctx::release(void *p, size_t bytes) {
   if (!p) return;
   if (bytes == 0) bytes = 256;
+  std::lock_guard<std::mutex> lock(amu);
   (void)hipFree(p);
   in_use -= std::min(in_use, bytes);
 }
@@ -131,7 +133,7 @@ extern "C" int mprgp_ctx_create(int device, size_t hbm_budget_bytes, mprgp_ctx *
     mprgp_ctx_destroy(c);
     return fail(MPRGP_EHIP, "context setup failed: %s", hipGetErrorString(e));
   }
-  int rc = c->alloc((void **)&c->d_ctrl, 4 * sizeof(int));
+  int rc = c->tab.init(c);
   if (rc) {
     mprgp_ctx_destroy(c);
     return rc;
@@ -144,12 +146,7 @@ extern "C" void mprgp_ctx_destroy(mprgp_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (auto &kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
-  for (auto &kv : c->graph_defs) (void)hipGraphDestroy(kv.second);
-  c->release(c->d_qps, sizeof(QpDev) * (size_t)c->qps_cap);
-  c->release(c->d_tiles, sizeof(Tile) * (size_t)c->tiles_cap);
-  c->release(c->d_ctrl, 4 * sizeof(int));
-  c->release(c->d_flags, sizeof(int) * (size_t)c->flags_cap);
+  c->tab.destroy();
   c->release(c->scratch, c->scratch_bytes);
   if (c->h_running) (void)hipHostFree(c->h_running);
   if (c->t0) (void)hipEventDestroy(c->t0);
@@ -279,6 +276,7 @@ extern "C" int mprgp_matrix_create_bsr3(mprgp_ctx *ctx, int n_nodes, const int *
                                         const double *blocks, mprgp_matrix **out) {
   if (!ctx || !blocks) return fail(MPRGP_EBADINPUT, "ctx or blocks is NULL");
   MPRGP_HIP(hipSetDevice(ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
   return build_matrix(ctx, n_nodes, rowptr, colidx, blocks, out);
 }
 
@@ -319,6 +317,7 @@ extern "C" int mprgp_matrix_create_csr(mprgp_ctx *ctx, int n, const int *ptr, co
     rowptr[a + 1] = (int)colidx.size();
   }
   MPRGP_HIP(hipSetDevice(ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
   return build_matrix(ctx, N, rowptr.data(), colidx.data(), blocks.data(), out);
 }
 
@@ -326,6 +325,7 @@ extern "C" int mprgp_matrix_update_bsr3(mprgp_matrix *A, const double *blocks) {
   if (!A || !blocks) return fail(MPRGP_EBADINPUT, "NULL argument");
   mprgp_ctx *ctx = A->ctx;
   MPRGP_HIP(hipSetDevice(ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
   std::vector<double> val((size_t)A->slots * 9, 0.0);
   for (long long b = 0; b < A->nb; ++b) {
     long long slot = A->blk_slot[b];
@@ -363,6 +363,7 @@ extern "C" int mprgp_matrix_spmv(mprgp_matrix *A, const double *x, double *y) {
   if (!A || !x || !y) return fail(MPRGP_EBADINPUT, "NULL argument");
   mprgp_ctx *ctx = A->ctx;
   MPRGP_HIP(hipSetDevice(ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
   const int n = A->n_nodes;
   const size_t vb = sizeof(double) * 3 * n;
   double *buf = nullptr;
@@ -385,6 +386,7 @@ extern "C" void mprgp_matrix_destroy(mprgp_matrix *A) {
   if (!A) return;
   mprgp_ctx *ctx = A->ctx;
   (void)hipSetDevice(ctx->device);
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
   (void)hipStreamSynchronize(ctx->stream);
   ctx->release(A->d_sl_off, sizeof(int) * (A->n_slices + 1));
   ctx->release(A->d_col, sizeof(int) * A->slots);
@@ -396,13 +398,12 @@ extern "C" void mprgp_matrix_destroy(mprgp_matrix *A) {
 }
 
 // FP32 copy of the values for the mixed form.
-static int ensure_matrix32(mprgp_matrix *A) {
+static int ensure_matrix32(mprgp_matrix *A, hipStream_t s) {
   mprgp_ctx *ctx = A->ctx;
   if (!A->d_val32) {
     if (int rc = ctx->alloc((void **)&A->d_val32, sizeof(float) * A->slots * 9)) return rc;
   }
   if (A->version32 != A->version) {
-    hipStream_t s = ctx->stream;
     MPRGP_HIP(launch_to_f32(s, A->d_val, A->d_val32, (size_t)A->slots * 9));
     A->version32 = A->version;
   }
@@ -442,6 +443,7 @@ extern "C" int mprgp_qp_create(mprgp_matrix *A, mprgp_qp **out) {
   *out = nullptr;
   mprgp_ctx *ctx = A->ctx;
   MPRGP_HIP(hipSetDevice(ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
   mprgp_qp *q = new mprgp_qp();
   q->ctx = ctx;
   q->A = A;
@@ -489,6 +491,7 @@ extern "C" int mprgp_qp_create(mprgp_matrix *A, mprgp_qp **out) {
 extern "C" void mprgp_qp_destroy(mprgp_qp *q) {
   if (!q) return;
   (void)hipSetDevice(q->ctx->device);
+  std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
   (void)hipStreamSynchronize(q->ctx->stream);
   q->ctx->release(q->base, q->bytes);
   q->ctx->release(q->base32, q->bytes32);
@@ -522,7 +525,9 @@ static int check_finite(const double *v, size_t n, const char *what) {
 extern "C" int mprgp_qp_set_rhs(mprgp_qp *q, const double *b) {
   if (!q || !b) return fail(MPRGP_EBADINPUT, "NULL argument");
   if (int rc = check_finite(b, 3 * (size_t)q->n, "b")) return rc;
+  MPRGP_NOT_BUSY(q);
   MPRGP_HIP(hipSetDevice(q->ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
   int rc = upload_vec(q, b, q->b);
   if (!rc) q->has_rhs = true;
   return rc;
@@ -531,7 +536,9 @@ extern "C" int mprgp_qp_set_rhs(mprgp_qp *q, const double *b) {
 extern "C" int mprgp_qp_set_x(mprgp_qp *q, const double *x) {
   if (!q || !x) return fail(MPRGP_EBADINPUT, "NULL argument");
   if (int rc = check_finite(x, 3 * (size_t)q->n, "x")) return rc;
+  MPRGP_NOT_BUSY(q);
   MPRGP_HIP(hipSetDevice(q->ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
   int rc = upload_vec(q, x, q->x0);
   if (rc) return rc;
   MPRGP_HIP(hipMemcpyAsync(q->x, q->x0, sizeof(double) * 3 * q->n, hipMemcpyDeviceToDevice, q->ctx->stream));
@@ -543,21 +550,27 @@ extern "C" int mprgp_qp_set_x(mprgp_qp *q, const double *x) {
 extern "C" int mprgp_qp_reset_x(mprgp_qp *q) {
   if (!q) return fail(MPRGP_EBADINPUT, "NULL argument");
   if (!q->has_x) return fail(MPRGP_ESTATE, "x was never set");
+  MPRGP_NOT_BUSY(q);
   MPRGP_HIP(hipSetDevice(q->ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
   MPRGP_HIP(hipMemcpyAsync(q->x, q->x0, sizeof(double) * 3 * q->n, hipMemcpyDeviceToDevice, q->ctx->stream));
   return MPRGP_OK;
 }
 
 extern "C" int mprgp_qp_get_x(mprgp_qp *q, double *x) {
   if (!q || !x) return fail(MPRGP_EBADINPUT, "NULL argument");
+  MPRGP_NOT_BUSY(q);
   MPRGP_HIP(hipSetDevice(q->ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
   return download_vec(q, q->x, x);
 }
 
 extern "C" int mprgp_qp_get_rhs(mprgp_qp *q, double *b) {
   if (!q || !b) return fail(MPRGP_EBADINPUT, "NULL argument");
   if (!q->has_rhs) return fail(MPRGP_ESTATE, "rhs was never set");
+  MPRGP_NOT_BUSY(q);
   MPRGP_HIP(hipSetDevice(q->ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
   return download_vec(q, q->b, b);
 }
 
@@ -568,7 +581,9 @@ extern "C" int mprgp_qp_set_ev(mprgp_qp *q, const double *ev) {
     return MPRGP_OK;
   }
   if (int rc = check_finite(ev, 3 * (size_t)q->n, "ev")) return rc;
+  MPRGP_NOT_BUSY(q);
   MPRGP_HIP(hipSetDevice(q->ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
   int rc = upload_vec(q, ev, q->ev);
   if (!rc) q->ev_valid = 1;
   return rc;
@@ -578,7 +593,9 @@ extern "C" int mprgp_qp_get_ev(mprgp_qp *q, double *ev, int *valid) {
   if (!q) return fail(MPRGP_EBADINPUT, "NULL argument");
   if (valid) *valid = q->ev_valid;
   if (!ev || !q->ev_valid) return MPRGP_OK;
+  MPRGP_NOT_BUSY(q);
   MPRGP_HIP(hipSetDevice(q->ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
   return download_vec(q, q->ev, ev);
 }
 
@@ -590,7 +607,9 @@ extern "C" int mprgp_qp_op_counts(mprgp_qp *q, int *counts, int n) {
 
 extern "C" int mprgp_qp_get_face(mprgp_qp *q, unsigned char *face) {
   if (!q || !face) return fail(MPRGP_EBADINPUT, "NULL argument");
+  MPRGP_NOT_BUSY(q);
   MPRGP_HIP(hipSetDevice(q->ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
   std::vector<unsigned char> f(q->n);
   MPRGP_HIP(hipMemcpyAsync(f.data(), q->face, q->n, hipMemcpyDeviceToHost, q->ctx->stream));
   MPRGP_HIP(hipStreamSynchronize(q->ctx->stream));
@@ -633,7 +652,9 @@ static int upload_rows(mprgp_qp *q, int m, const std::vector<int> &node, const s
 extern "C" int mprgp_qp_set_rows(mprgp_qp *q, int m, const int *node, const double *normal, const double *c) {
   if (!q || m < 0 || (m > 0 && (!node || !normal || !c))) return fail(MPRGP_EBADINPUT, "bad row arguments");
   if (m > q->n) return fail(MPRGP_EBADINPUT, "m = %d exceeds node count %d", m, q->n);
+  MPRGP_NOT_BUSY(q);
   MPRGP_HIP(hipSetDevice(q->ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
   std::vector<int> nd(node, node + m);
   std::vector<double> nr(normal, normal + 3 * (size_t)m), cv(c, c + m);
   return upload_rows(q, m, nd, nr, cv);
@@ -669,62 +690,87 @@ extern "C" int mprgp_qp_set_rows_sparse(mprgp_qp *q, int m, int col_major, const
   }
   for (int i = 0; i < m; ++i)
     if (node[i] < 0) return fail(MPRGP_EBADINPUT, "J row %d is empty", i);
+  MPRGP_NOT_BUSY(q);
   MPRGP_HIP(hipSetDevice(q->ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
   return upload_rows(q, m, node, nrm, cv);
 }
 
 // ----------------------------------------------------------- batch solve
 
-static int ensure_tables(mprgp_ctx *ctx, int n_qps, int n_tiles) {
-  if (n_qps > ctx->flags_cap) {
-    int cap = std::max(n_qps, 2 * ctx->flags_cap);
-    ctx->release(ctx->d_flags, sizeof(int) * (size_t)ctx->flags_cap);
-    ctx->d_flags = nullptr;
-    ctx->flags_cap = 0;
-    if (int rc = ctx->alloc((void **)&ctx->d_flags, sizeof(int) * (size_t)cap)) return rc;
-    ctx->flags_cap = cap;
+namespace mprgp {
+
+int BatchTables::init(mprgp_ctx *ctx) {
+  this->ctx = ctx;
+  if (int rc = ctx->alloc((void **)&d_ctrl, 4 * sizeof(int))) return rc;
+  return MPRGP_OK;
+}
+
+void BatchTables::drop_graphs() {
+  for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto &kv : graph_defs) (void)hipGraphDestroy(kv.second);
+  graphs.clear();
+  graph_defs.clear();
+}
+
+void BatchTables::destroy() {
+  if (!ctx) return;
+  drop_graphs();
+  ctx->release(d_qps, sizeof(QpDev) * (size_t)qps_cap);
+  ctx->release(d_tiles, sizeof(Tile) * (size_t)tiles_cap);
+  ctx->release(d_ctrl, 4 * sizeof(int));
+  ctx->release(d_flags, sizeof(int) * (size_t)flags_cap);
+  d_qps = nullptr;
+  d_tiles = nullptr;
+  d_ctrl = nullptr;
+  d_flags = nullptr;
+  qps_cap = tiles_cap = flags_cap = 0;
+}
+
+// Grows the tables; graphs captured the old table pointers, so a regrowth
+// drops them.
+int BatchTables::ensure(int n_qps, int n_tiles) {
+  if (n_qps > flags_cap) {
+    int cap = std::max(n_qps, 2 * flags_cap);
+    ctx->release(d_flags, sizeof(int) * (size_t)flags_cap);
+    d_flags = nullptr;
+    flags_cap = 0;
+    if (int rc = ctx->alloc((void **)&d_flags, sizeof(int) * (size_t)cap)) return rc;
+    flags_cap = cap;
   }
-  if (n_qps > ctx->qps_cap) {
-    int cap = std::max(n_qps, 2 * ctx->qps_cap);
-    ctx->release(ctx->d_qps, sizeof(QpDev) * (size_t)ctx->qps_cap);
-    ctx->d_qps = nullptr;
-    ctx->qps_cap = 0;
-    if (int rc = ctx->alloc((void **)&ctx->d_qps, sizeof(QpDev) * (size_t)cap)) return rc;
-    ctx->qps_cap = cap;
-    // graphs captured the old table pointers
-    for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
-    for (auto &kv : ctx->graph_defs) (void)hipGraphDestroy(kv.second);
-    ctx->graphs.clear();
-    ctx->graph_defs.clear();
+  if (n_qps > qps_cap) {
+    int cap = std::max(n_qps, 2 * qps_cap);
+    ctx->release(d_qps, sizeof(QpDev) * (size_t)qps_cap);
+    d_qps = nullptr;
+    qps_cap = 0;
+    if (int rc = ctx->alloc((void **)&d_qps, sizeof(QpDev) * (size_t)cap)) return rc;
+    qps_cap = cap;
+    drop_graphs();
   }
-  if (n_tiles > ctx->tiles_cap) {
-    int cap = std::max(n_tiles, 2 * ctx->tiles_cap);
-    ctx->release(ctx->d_tiles, sizeof(Tile) * (size_t)ctx->tiles_cap);
-    ctx->d_tiles = nullptr;
-    ctx->tiles_cap = 0;
-    if (int rc = ctx->alloc((void **)&ctx->d_tiles, sizeof(Tile) * (size_t)cap)) return rc;
-    ctx->tiles_cap = cap;
-    for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
-    for (auto &kv : ctx->graph_defs) (void)hipGraphDestroy(kv.second);
-    ctx->graphs.clear();
-    ctx->graph_defs.clear();
+  if (n_tiles > tiles_cap) {
+    int cap = std::max(n_tiles, 2 * tiles_cap);
+    ctx->release(d_tiles, sizeof(Tile) * (size_t)tiles_cap);
+    d_tiles = nullptr;
+    tiles_cap = 0;
+    if (int rc = ctx->alloc((void **)&d_tiles, sizeof(Tile) * (size_t)cap)) return rc;
+    tiles_cap = cap;
+    drop_graphs();
   }
   return MPRGP_OK;
 }
 
-static int get_graph(mprgp_ctx *ctx, int ts, int steps, int grid, bool mixed, hipGraphExec_t *out) {
+int BatchTables::graph(hipStream_t s, int ts, int steps, int grid, bool mixed, hipGraphExec_t *out) {
   auto key = std::make_tuple(ts, steps, grid, (int)mixed);
-  auto it = ctx->graphs.find(key);
-  if (it != ctx->graphs.end()) {
+  auto it = graphs.find(key);
+  if (it != graphs.end()) {
     *out = it->second;
     return MPRGP_OK;
   }
-  hipStream_t s = ctx->stream;
   MPRGP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   for (int i = 0; i < steps; ++i) {
-    hipError_t e = launch_step(ts, grid, s, ctx->d_qps, ctx->d_tiles, ctx->d_ctrl, ctx->d_ctrl + 1, i & 1, mixed);
+    hipError_t e = launch_step(ts, grid, s, d_qps, d_tiles, d_ctrl, d_ctrl + 1, i & 1, mixed);
     if (e != hipSuccess) {
-      hipGraph_t g;
+      hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(s, &g);
       if (g) (void)hipGraphDestroy(g);
       return fail(MPRGP_EHIP, "capture failed: %s", hipGetErrorString(e));
@@ -738,175 +784,233 @@ static int get_graph(mprgp_ctx *ctx, int ts, int steps, int grid, bool mixed, hi
     (void)hipGraphDestroy(g);
     return fail(MPRGP_EHIP, "graph instantiate failed: %s", hipGetErrorString(e));
   }
-  ctx->graph_defs[key] = g;
-  ctx->graphs[key] = ex;
+  graph_defs[key] = g;
+  graphs[key] = ex;
   *out = ex;
   return MPRGP_OK;
 }
 
-// Launch shape: TS wavefronts per workgroup, each taking `spw` slices of
-// 64 rows, so a tile covers TS*spw slices.  Small problems use narrow
-// workgroups so enough of them exist to fill the chip.
-struct Shape {
-  int ts, spw;
-};
-static Shape pick_shape(const mprgp_params &p, long total_slices) {
-  Shape sh{4, 1};
-  if (p.tiles == 1 || p.tiles == 2 || p.tiles == 4 || p.tiles == 8) {
-    sh.ts = p.tiles;
-  } else if (total_slices < 2048) {
-    sh.ts = 1;
-  } else if (total_slices < 8192) {
-    sh.ts = 2;
-  }
-  // one slice per wavefront measured fastest on the 256-QP batch (more
-  // slices per wave serialise the SpMV latency chains); callers may override
-  sh.spw = p.reserved_i[0] > 0 ? p.reserved_i[0] : 1;
-  return sh;
+// Wavefronts per workgroup: small batches use narrow workgroups so enough of
+// them exist to fill the chip.  A QP's partial sums are grouped per tile, so
+// its result is bit-identical across batches that use the same value.
+int pick_ts(const mprgp_params &p, long total_slices) {
+  if (p.tiles == 1 || p.tiles == 2 || p.tiles == 4 || p.tiles == 8) return p.tiles;
+  if (total_slices < 2048) return 1;
+  if (total_slices < 8192) return 2;
+  return 4;
 }
 
-static void build_tiles(const std::vector<int> &n_slices, const std::vector<char> &keep, Shape sh,
-                        std::vector<Tile> &tiles) {
-  tiles.clear();
-  const int span = sh.ts * sh.spw;
-  for (size_t i = 0; i < n_slices.size(); ++i) {
-    if (!keep[i]) continue;
-    int local = 0;
-    for (int s0 = 0; s0 < n_slices[i]; s0 += span, ++local)
-      tiles.push_back(Tile{(int)i, s0, std::min(span, n_slices[i] - s0), local});
-  }
+// Tiles of one QP (slot `slot` of the batch table): `span` = TS*spw slices each.
+void append_tiles(int slot, int n_slices, int span, std::vector<Tile> &tiles) {
+  int local = 0;
+  for (int s0 = 0; s0 < n_slices; s0 += span, ++local)
+    tiles.push_back(Tile{slot, s0, std::min(span, n_slices - s0), local});
 }
 
-static int round_grid(int n_tiles) {
+int tiles_for(int n_slices, int span) { return (n_slices + span - 1) / span; }
+
+int round_grid(int n_tiles) {
   int g = 64;
   while (g < n_tiles && g < 8192) g *= 2;
   return std::min(g, 8192);
 }
 
+int check_batch_qp(const mprgp_ctx *ctx, const mprgp_qp *q, const mprgp_params &p, int i) {
+  if (!q) return fail(MPRGP_EBADINPUT, "qp %d is NULL", i);
+  if (q->busy) return fail(MPRGP_ESTATE, "qp %d is busy in an engine", i);
+  if (q->ctx != ctx) return fail(MPRGP_EBADINPUT, "qp %d belongs to another context", i);
+  if (!q->has_rhs || !q->has_x) return fail(MPRGP_ESTATE, "qp %d: rhs and x must be set", i);
+  if (!(p.tol == p.tol) || p.max_it < 0) return fail(MPRGP_EBADINPUT, "qp %d: bad tol/max_it", i);
+  if (p.precision != MPRGP_FP64 && p.precision != MPRGP_MIXED)
+    return fail(MPRGP_EBADINPUT, "qp %d: unknown precision %d", i, p.precision);
+  if (p.variant != MPRGP_MONOTONIC && p.variant != MPRGP_CLASSIC)
+    return fail(MPRGP_EBADINPUT, "qp %d: unknown variant %d", i, p.variant);
+  if (p.precision == MPRGP_MIXED && p.variant != MPRGP_MONOTONIC)
+    return fail(MPRGP_EBADINPUT, "qp %d: the mixed form runs the monotone variant", i);
+  return MPRGP_OK;
+}
+
+// Launch bound of one solve: every op sequence of an iteration is at most 8
+// launches, plus the power method and the refinement sweeps.
+long solve_step_bound(const mprgp_params &p) {
+  const bool mixed = p.precision == MPRGP_MIXED;
+  const bool power = !(p.alpha_bar > 0.0);
+  const long sweeps = mixed ? (p.max_refine > 0 ? p.max_refine : 8) + 1 : 1;
+  return sweeps * (8L * ((long)p.max_it + 4) + 8) + (power ? 2L * 1002 + 4 : 0) + 16;
+}
+
+// Bytes of device memory a solve of `q` works on (matrix, its FP32 copy for
+// the mixed form, the QP's vectors).
+size_t solve_bytes(const mprgp_qp *q, const mprgp_params &p) {
+  size_t b = q->A->bytes + q->bytes;
+  if (p.precision == MPRGP_MIXED) {
+    b += sizeof(float) * 9 * (size_t)q->A->slots;
+    b += (7 * 3 * (size_t)q->n + (size_t)q->n) * sizeof(double) + (3 * 3 * (size_t)q->n + 64) * sizeof(float);
+  }
+  return b;
+}
+
+// Fills the device descriptor and the initial state of a QP entering a batch
+// and enqueues its set-up on stream `s` (FP32 copies, power start vector,
+// tangent-Jacobi factors).  The caller uploads `st` into both state copies.
+// A power start vector needs a host upload: it goes through `q->stage` and
+// synchronises `s`.
+int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, int *done_flag, int tag,
+               QpDev &d, QpState &st) {
+  std::memset(&d, 0, sizeof(d));
+  d.n_nodes = q->n;
+  d.n_slices = q->A->n_slices;
+  d.n_tiles = n_tiles;
+  d.done_flag = done_flag;
+  d.tag = tag;
+  d.sl_off = q->A->d_sl_off;
+  d.col = q->A->d_col;
+  d.d.val = q->A->d_val;
+  d.d.dinv = q->A->d_dinv;
+  d.d.x = q->x; d.d.b = q->b; d.d.g = q->g; d.d.p = q->p; d.d.z = q->z; d.d.phi = q->phi;
+  d.d.beta = q->beta; d.d.y = q->y; d.d.ap = q->ap; d.d.py = q->py;
+  d.d.cn = q->cn; d.d.cc = q->cc; d.d.cj = q->cj; d.d.cs = q->cs;
+  d.ev = q->ev; d.pv = q->pv; d.pw = q->pw;
+  d.cf = q->cf; d.face = q->face;
+  d.st = q->st;
+  d.part = q->part;
+  const bool mixed = p.precision == MPRGP_MIXED;
+  if (mixed) {
+    if (int rc = ensure_matrix32(q->A, s)) return rc;
+    if (int rc = ensure_qp32(q)) return rc;
+    d.f.val = q->A->d_val32;
+    d.f.dinv = q->A->d_dinv;
+    d.f.x = q->fx; d.f.b = q->fb; d.f.g = q->fg; d.f.p = q->fp; d.f.z = q->fz; d.f.phi = q->fphi;
+    d.f.beta = q->fbeta; d.f.y = q->fy; d.f.ap = q->fap; d.f.py = q->fpy;
+    d.f.cn = q->cn; d.f.cj = q->cj; d.f.cs = q->cs; d.f.cc = q->fcc;
+  }
+  std::memset(&st, 0, sizeof(st));
+  st.n_tiles = n_tiles;
+  st.max_it = p.max_it;
+  st.tol = p.tol < 1e-30 ? 1e-30 : p.tol;
+  st.gamma = 1.0;
+  st.code = -1;
+  st.has_rows = q->m > 0;
+  st.precond = p.preconditioned ? 1 : 0;
+  st.variant = p.variant;
+  const bool power = !(p.alpha_bar > 0.0);
+  st.alpha_bar = power ? 0.0 : p.alpha_bar;
+  st.lambda_max = power ? 0.0 : 2.0 / p.alpha_bar;
+  st.power_eps = p.power_tol > 0.0 ? p.power_tol : p.tol;
+  st.pow_next = power ? OP_POW_MV : (mixed ? OP_R_START : OP_INIT);
+  st.op = p.project_first ? OP_PROJX : st.pow_next;
+  st.fresh = 1;
+  if (mixed) {
+    st.mixed = 1;
+    st.tol_outer = st.tol;
+    st.max_refine = p.max_refine > 0 ? p.max_refine : 8;
+    st.inner_rel = p.inner_tol > 0.0 ? p.inner_tol : 1e-3;
+  }
+  if (power) {
+    if (p.use_ev && q->ev_valid) {
+      MPRGP_HIP(hipMemcpyAsync(q->pv, q->ev, sizeof(double) * 3 * q->n, hipMemcpyDeviceToDevice, s));
+    } else {
+      // deterministic start vector in user node order, normalised
+      std::vector<double> v(3 * (size_t)q->n);
+      uint64_t seed = 0x5EEDULL;
+      for (auto &e : v) e = 2.0 * ((double)(splitmix64(seed) >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+      double nr = 0.0;
+      for (double e : v) nr += e * e;
+      nr = std::sqrt(nr);
+      if (nr > 0.0)
+        for (auto &e : v) e /= nr;
+      MPRGP_HIP(hipMemcpyAsync(q->stage, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, s));
+      MPRGP_HIP(launch_gather_nodes(s, q->stage, q->pv, q->A->d_perm, q->n));
+      MPRGP_HIP(hipStreamSynchronize(s));
+    }
+  }
+  // tangent-Jacobi factors depend on the current matrix values
+  MPRGP_HIP(launch_setup_rows(s, q->A->d_dinv, q->cn, q->cf, q->cs, q->n));
+  return MPRGP_OK;
+}
+
+// The state copy that reached DONE (both do one launch later).
+const QpState &final_state(const QpState both[2]) {
+  return (both[1].op == OP_DONE && (both[0].op != OP_DONE || both[1].steps > both[0].steps)) ? both[1]
+                                                                                              : both[0];
+}
+
+int fill_stats(mprgp_qp *q, const QpState both[2], float ms, long launched, mprgp_stats *out) {
+  const QpState &st = final_state(both);
+  if (st.ev_valid) q->ev_valid = 1;
+  std::memcpy(q->opcnt, st.opcnt, sizeof(st.opcnt));
+  if (out) {
+    mprgp_stats &o = *out;
+    std::memset(&o, 0, sizeof(o));
+    o.code = st.code;
+    o.refinements = st.mixed ? st.refine : 0;
+    o.iterations = st.iteration;
+    o.num_cg = st.num_cg;
+    o.num_exp = st.num_exp;
+    o.num_prop = st.num_prop;
+    o.active = st.active;
+    o.power_iters = st.power_it;
+    o.ev_valid = q->ev_valid;
+    o.steps = st.steps;
+    o.residual = st.residual;
+    o.alpha_bar = st.alpha_bar;
+    o.lambda_max = st.lambda_max;
+    o.elapsed_ms = ms;
+    o.reserved_i[0] = (int)launched;  // step-kernel launches issued for the whole batch
+  }
+  if (!std::isfinite(st.residual)) return MPRGP_ENAN;
+  if (st.op != OP_DONE) return MPRGP_EHIP;
+  return st.code == 0 ? MPRGP_OK : MPRGP_NOT_CONVERGED;
+}
+
+}  // namespace mprgp
+
 static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprgp_params *params,
                      int stride, mprgp_stats *stats) {
   if (!ctx || count <= 0 || !qps || !params) return fail(MPRGP_EBADINPUT, "bad batch arguments");
   MPRGP_HIP(hipSetDevice(ctx->device));
-  long total_nodes = 0;
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
+  std::lock_guard<std::recursive_mutex> lock(ctx->mu);
   for (int i = 0; i < count; ++i) {
-    const mprgp_qp *q = qps[i];
-    const mprgp_params &p = params[stride ? i : 0];
-    if (!q) return fail(MPRGP_EBADINPUT, "qp %d is NULL", i);
-    if (q->ctx != ctx) return fail(MPRGP_EBADINPUT, "qp %d belongs to another context", i);
-    if (!q->has_rhs || !q->has_x) return fail(MPRGP_ESTATE, "qp %d: rhs and x must be set", i);
-    if (!(p.tol == p.tol) || p.max_it < 0) return fail(MPRGP_EBADINPUT, "qp %d: bad tol/max_it", i);
-    if (p.precision != MPRGP_FP64 && p.precision != MPRGP_MIXED)
-      return fail(MPRGP_EBADINPUT, "qp %d: unknown precision %d", i, p.precision);
-    if (p.precision == MPRGP_MIXED && p.variant != MPRGP_MONOTONIC)
-      return fail(MPRGP_EBADINPUT, "qp %d: the mixed form runs the monotone variant", i);
-    if (p.variant != MPRGP_MONOTONIC && p.variant != MPRGP_CLASSIC)
-      return fail(MPRGP_EBADINPUT, "qp %d: unknown variant %d", i, p.variant);
+    if (int rc = check_batch_qp(ctx, qps[i], params[stride ? i : 0], i)) return rc;
     for (int j = 0; j < i; ++j)
-      if (qps[j] == q) return fail(MPRGP_EBADINPUT, "qp %d appears twice in the batch", i);
-    total_nodes += q->n;
+      if (qps[j] == qps[i]) return fail(MPRGP_EBADINPUT, "qp %d appears twice in the batch", i);
   }
   bool any_mixed = false;
-  for (int i = 0; i < count; ++i) any_mixed |= params[stride ? i : 0].precision == MPRGP_MIXED;
-  long total_slices = 0;
+  long total_slices = 0, max_steps = 0;
   std::vector<int> n_slices(count);
-  for (int i = 0; i < count; ++i) total_slices += (n_slices[i] = qps[i]->A->n_slices);
-  const Shape sh = pick_shape(params[0], total_slices);
-  const int ts = sh.ts;
-  // tile table and device descriptors
-  std::vector<Tile> tiles;
-  std::vector<char> keep(count, 1);
-  build_tiles(n_slices, keep, sh, tiles);
-  std::vector<int> tiles_of(count, 0);
-  for (const Tile &t : tiles) tiles_of[t.qp]++;
-  std::vector<QpDev> devs(count);
-  std::vector<QpState> states(count);
-  long max_steps = 0;
-  hipStream_t s = ctx->stream;
-  if (int rc = ensure_tables(ctx, count, (int)tiles.size())) return rc;
   for (int i = 0; i < count; ++i) {
-    mprgp_qp *q = qps[i];
     const mprgp_params &p = params[stride ? i : 0];
-    QpDev &d = devs[i];
-    d.n_nodes = q->n;
-    d.n_slices = n_slices[i];
-    d.n_tiles = tiles_of[i];
-    d.done_flag = ctx->d_flags + i;
-    d.sl_off = q->A->d_sl_off;
-    d.col = q->A->d_col;
-    d.d.val = q->A->d_val;
-    d.d.dinv = q->A->d_dinv;
-    d.d.x = q->x; d.d.b = q->b; d.d.g = q->g; d.d.p = q->p; d.d.z = q->z; d.d.phi = q->phi;
-    d.d.beta = q->beta; d.d.y = q->y; d.d.ap = q->ap; d.d.py = q->py;
-    d.d.cn = q->cn; d.d.cc = q->cc; d.d.cj = q->cj; d.d.cs = q->cs;
-    d.ev = q->ev; d.pv = q->pv; d.pw = q->pw;
-    d.cf = q->cf; d.face = q->face;
-    d.st = q->st;
-    d.part = q->part;
-    const bool mixed = p.precision == MPRGP_MIXED;
-    if (mixed) {
-      if (int rc = ensure_matrix32(q->A)) return rc;
-      if (int rc = ensure_qp32(q)) return rc;
-      d.f.val = q->A->d_val32;
-      d.f.dinv = q->A->d_dinv;
-      d.f.x = q->fx; d.f.b = q->fb; d.f.g = q->fg; d.f.p = q->fp; d.f.z = q->fz; d.f.phi = q->fphi;
-      d.f.beta = q->fbeta; d.f.y = q->fy; d.f.ap = q->fap; d.f.py = q->fpy;
-      d.f.cn = q->cn; d.f.cj = q->cj; d.f.cs = q->cs; d.f.cc = q->fcc;
-    }
-    QpState &st = states[i];
-    std::memset(&st, 0, sizeof(st));
-    st.n_tiles = tiles_of[i];
-    st.max_it = p.max_it;
-    st.tol = p.tol < 1e-30 ? 1e-30 : p.tol;
-    st.gamma = 1.0;
-    st.code = -1;
-    st.has_rows = q->m > 0;
-    st.precond = p.preconditioned ? 1 : 0;
-    st.variant = p.variant;
-    const bool power = !(p.alpha_bar > 0.0);
-    st.alpha_bar = power ? 0.0 : p.alpha_bar;
-    st.lambda_max = power ? 0.0 : 2.0 / p.alpha_bar;
-    st.power_eps = p.power_tol > 0.0 ? p.power_tol : p.tol;
-    st.pow_next = power ? OP_POW_MV : (mixed ? OP_R_START : OP_INIT);
-    st.op = p.project_first ? OP_PROJX : st.pow_next;
-    st.fresh = 1;
-    if (mixed) {
-      st.mixed = 1;
-      st.tol_outer = st.tol;
-      st.max_refine = p.max_refine > 0 ? p.max_refine : 8;
-      st.inner_rel = p.inner_tol > 0.0 ? p.inner_tol : 1e-3;
-    }
-    const long sweeps = mixed ? (p.max_refine > 0 ? p.max_refine : 8) + 1 : 1;
-    max_steps = std::max<long>(max_steps, sweeps * (8L * ((long)p.max_it + 4) + 8) +
-                                              (power ? 2L * 1002 + 4 : 0) + 16);
-    if (power) {
-      if (p.use_ev && q->ev_valid) {
-        MPRGP_HIP(hipMemcpyAsync(q->pv, q->ev, sizeof(double) * 3 * q->n, hipMemcpyDeviceToDevice, s));
-      } else {
-        // deterministic start vector in user node order, normalised
-        std::vector<double> v(3 * (size_t)q->n);
-        uint64_t seed = 0x5EEDULL;
-        for (auto &e : v) e = 2.0 * ((double)(splitmix64(seed) >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
-        double nr = 0.0;
-        for (double e : v) nr += e * e;
-        nr = std::sqrt(nr);
-        if (nr > 0.0)
-          for (auto &e : v) e /= nr;
-        MPRGP_HIP(hipMemcpyAsync(q->stage, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, s));
-        MPRGP_HIP(launch_gather_nodes(s, q->stage, q->pv, q->A->d_perm, q->n));
-        MPRGP_HIP(hipStreamSynchronize(s));
-      }
-    }
-    // tangent-Jacobi factors depend on the current matrix values
-    MPRGP_HIP(launch_setup_rows(s, q->A->d_dinv, q->cn, q->cf, q->cs, q->n));
-    MPRGP_HIP(hipMemcpyAsync(q->st, &st, sizeof(QpState), hipMemcpyHostToDevice, s));
-    MPRGP_HIP(hipMemcpyAsync(q->st + 1, &st, sizeof(QpState), hipMemcpyHostToDevice, s));
+    any_mixed |= p.precision == MPRGP_MIXED;
+    total_slices += (n_slices[i] = qps[i]->A->n_slices);
+    max_steps = std::max(max_steps, solve_step_bound(p));
+  }
+  const int ts = pick_ts(params[0], total_slices);
+  // one slice per wavefront measured fastest on the 256-QP batch (more slices
+  // per wave serialise the SpMV latency chains); callers may override
+  const int spw = params[0].reserved_i[0] > 0 ? params[0].reserved_i[0] : 1;
+  const int span = ts * spw;
+  std::vector<Tile> tiles;
+  for (int i = 0; i < count; ++i) append_tiles(i, n_slices[i], span, tiles);
+  BatchTables &tab = ctx->tab;
+  if (int rc = tab.ensure(count, (int)tiles.size())) return rc;
+  hipStream_t s = ctx->stream;
+  std::vector<QpDev> devs(count);
+  std::vector<QpState> states(2 * (size_t)count);
+  for (int i = 0; i < count; ++i) {
+    QpState &st = states[2 * (size_t)i];
+    if (int rc = prepare_qp(qps[i], params[stride ? i : 0], s, tiles_for(n_slices[i], span), tab.d_flags + i, 1,
+                            devs[i], st))
+      return rc;
+    states[2 * (size_t)i + 1] = st;
+    MPRGP_HIP(hipMemcpyAsync(qps[i]->st, &states[2 * (size_t)i], 2 * sizeof(QpState), hipMemcpyHostToDevice, s));
   }
   const int n_tiles = (int)tiles.size();
   int ctrl[2] = {n_tiles, count};
-  MPRGP_HIP(hipMemcpyAsync(ctx->d_qps, devs.data(), sizeof(QpDev) * count, hipMemcpyHostToDevice, s));
-  MPRGP_HIP(hipMemcpyAsync(ctx->d_tiles, tiles.data(), sizeof(Tile) * n_tiles, hipMemcpyHostToDevice, s));
-  MPRGP_HIP(hipMemcpyAsync(ctx->d_ctrl, ctrl, sizeof(ctrl), hipMemcpyHostToDevice, s));
-  MPRGP_HIP(hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * count, s));
+  MPRGP_HIP(hipMemcpyAsync(tab.d_qps, devs.data(), sizeof(QpDev) * count, hipMemcpyHostToDevice, s));
+  MPRGP_HIP(hipMemcpyAsync(tab.d_tiles, tiles.data(), sizeof(Tile) * n_tiles, hipMemcpyHostToDevice, s));
+  MPRGP_HIP(hipMemcpyAsync(tab.d_ctrl, ctrl, sizeof(ctrl), hipMemcpyHostToDevice, s));
+  MPRGP_HIP(hipMemsetAsync(tab.d_flags, 0, sizeof(int) * count, s));
   MPRGP_HIP(hipStreamSynchronize(s));
 
   int grid = round_grid(n_tiles);
@@ -919,7 +1023,7 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
   if (params[0].graph_steps < 0) steps = -params[0].graph_steps + (params[0].graph_steps & 1);
   hipGraphExec_t ex = nullptr;
   if (use_graph)
-    if (int rc = get_graph(ctx, ts, steps, grid, any_mixed, &ex)) return rc;
+    if (int rc = tab.graph(s, ts, steps, grid, any_mixed, &ex)) return rc;
   int in_table = count;  // QPs whose tiles are in the table
   std::vector<int> flags(count);
   std::vector<std::vector<Tile>> uploads;  // host buffers of pending table uploads
@@ -933,15 +1037,15 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
       MPRGP_HIP(hipGraphLaunch(ex, s));
     } else {
       for (int i = 0; i < steps; ++i)
-        MPRGP_HIP(launch_step(ts, grid, s, ctx->d_qps, ctx->d_tiles, ctx->d_ctrl, ctx->d_ctrl + 1, i & 1,
-                              any_mixed));
+        MPRGP_HIP(launch_step(ts, grid, s, tab.d_qps, tab.d_tiles, tab.d_ctrl, tab.d_ctrl + 1, i & 1, any_mixed));
     }
     launched += steps;
-    MPRGP_HIP(hipMemcpyAsync(ctx->h_running + slot, ctx->d_ctrl + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    MPRGP_HIP(hipMemcpyAsync(ctx->h_running + slot, tab.d_ctrl + 1, sizeof(int), hipMemcpyDeviceToHost, s));
     MPRGP_HIP(hipEventRecord(ctx->q[slot], s));
     return MPRGP_OK;
   };
   if (int rc = enqueue(0)) return rc;
+  std::vector<char> keep(count, 1);
   while (!done) {
     if (int rc = enqueue((k + 1) & 1)) return rc;
     MPRGP_HIP(hipEventSynchronize(ctx->q[k & 1]));
@@ -950,7 +1054,7 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
     ++k;
     // drop the tiles of finished QPs once a quarter of the table is idle
     if (!done && count > 1 && 4 * running < 3 * in_table) {
-      MPRGP_HIP(hipMemcpyAsync(flags.data(), ctx->d_flags, sizeof(int) * count, hipMemcpyDeviceToHost, s));
+      MPRGP_HIP(hipMemcpyAsync(flags.data(), tab.d_flags, sizeof(int) * count, hipMemcpyDeviceToHost, s));
       MPRGP_HIP(hipStreamSynchronize(s));
       int live = 0;
       for (int i = 0; i < count; ++i) {
@@ -959,16 +1063,16 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
       }
       if (live > 0 && live < in_table) {
         uploads.emplace_back();
-        build_tiles(n_slices, keep, sh, uploads.back());
+        for (int i = 0; i < count; ++i)
+          if (keep[i]) append_tiles(i, n_slices[i], span, uploads.back());
         const int nt = (int)uploads.back().size();
-        MPRGP_HIP(hipMemcpyAsync(ctx->d_tiles, uploads.back().data(), sizeof(Tile) * nt,
-                                 hipMemcpyHostToDevice, s));
-        MPRGP_HIP(hipMemcpyAsync(ctx->d_ctrl, &nt, sizeof(int), hipMemcpyHostToDevice, s));
+        MPRGP_HIP(hipMemcpyAsync(tab.d_tiles, uploads.back().data(), sizeof(Tile) * nt, hipMemcpyHostToDevice, s));
+        MPRGP_HIP(hipMemcpyAsync(tab.d_ctrl, &nt, sizeof(int), hipMemcpyHostToDevice, s));
         MPRGP_HIP(hipStreamSynchronize(s));
         in_table = live;
         grid = round_grid(nt);
         if (use_graph)
-          if (int rc = get_graph(ctx, ts, steps, grid, any_mixed, &ex)) return rc;
+          if (int rc = tab.graph(s, ts, steps, grid, any_mixed, &ex)) return rc;
       }
       // restart the two-deep polling pipeline
       if (int rc = enqueue(k & 1)) return rc;
@@ -988,32 +1092,9 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
   for (int i = 0; i < count; ++i) {
     QpState both[2];
     MPRGP_HIP(hipMemcpy(both, qps[i]->st, 2 * sizeof(QpState), hipMemcpyDeviceToHost));
-    // the copy that reached DONE (both do after one more launch)
-    const QpState &st = (both[1].op == OP_DONE && (both[0].op != OP_DONE || both[1].steps > both[0].steps))
-                            ? both[1] : both[0];
-    if (st.ev_valid) qps[i]->ev_valid = 1;
-    if (stats) {
-      mprgp_stats &o = stats[i];
-      std::memset(&o, 0, sizeof(o));
-      o.code = st.code;
-      o.refinements = st.mixed ? st.refine : 0;
-      o.iterations = st.iteration;
-      o.num_cg = st.num_cg;
-      o.num_exp = st.num_exp;
-      o.num_prop = st.num_prop;
-      o.active = st.active;
-      o.power_iters = st.power_it;
-      o.ev_valid = qps[i]->ev_valid;
-      o.steps = st.steps;
-      o.residual = st.residual;
-      o.alpha_bar = st.alpha_bar;
-      o.lambda_max = st.lambda_max;
-      o.elapsed_ms = ms;
-      o.reserved_i[0] = (int)launched;  // step-kernel launches issued for the whole batch
-      std::memcpy(qps[i]->opcnt, st.opcnt, sizeof(st.opcnt));
-    }
-    if (!std::isfinite(st.residual)) worst = MPRGP_ENAN;
-    else if (st.code != 0 && worst == MPRGP_OK) worst = MPRGP_NOT_CONVERGED;
+    int rc = fill_stats(qps[i], both, ms, launched, stats ? stats + i : nullptr);
+    if (rc == MPRGP_ENAN || rc == MPRGP_EHIP) worst = rc;
+    else if (rc == MPRGP_NOT_CONVERGED && worst == MPRGP_OK) worst = rc;
   }
   return worst;
 }

@@ -126,7 +126,8 @@ struct QpDev {
   const unsigned char *cf;
   unsigned char *face;
   QpState *st;          // st[0], st[1] (launch parity)
-  int *done_flag;       // set once when the QP reaches DONE (tile compaction)
+  int *done_flag;       // receives `tag` once the QP reaches DONE
+  int tag;              // nonzero; distinguishes successive occupants of a batch slot
   double *part;         // [2][n_tiles][kNumPart] partial sums (launch parity)
 };
 

@@ -345,6 +345,7 @@ extern "C" int mprgp_fem_create(mprgp_ctx *ctx, int N, int T, const int *tets_in
   if ((long long)T * 16 >= (1LL << 31)) return fail(MPRGP_EBADINPUT, "too many tets");
   *out = nullptr;
   MPRGP_HIP(hipSetDevice(ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
   std::vector<int> tets(tets_in, tets_in + 4 * (size_t)T);
   std::vector<double> dinv(9 * (size_t)T), vol(T), lam_mu(2 * (size_t)T), mass(N, 0.0);
   for (int t = 0; t < T; ++t) {
@@ -488,14 +489,19 @@ extern "C" mprgp_matrix *mprgp_fem_matrix(mprgp_fem *f) { return f ? f->A : null
 extern "C" int mprgp_fem_assemble(mprgp_fem *f, int material, const double *x_n, const double *v_n,
                                   const double *gravity, double h, mprgp_qp *qp) {
   if (!f || !x_n || !v_n || !gravity || !qp) return fail(MPRGP_EBADINPUT, "NULL argument");
+  const bool rhs_only = (material & MPRGP_RHS_ONLY) != 0;
+  material &= ~MPRGP_RHS_ONLY;
   if (material != MPRGP_LINEAR && material != MPRGP_COROTATIONAL)
     return fail(MPRGP_EBADINPUT, "unknown material %d", material);
+  if (rhs_only && f->A->version == 0) return fail(MPRGP_ESTATE, "RHS-only assembly before any full assembly");
   if (qp->A != f->A) return fail(MPRGP_EBADINPUT, "qp was not created on this FEM's matrix");
+  MPRGP_NOT_BUSY(qp);
   if (!(h > 0.0) || !std::isfinite(h)) return fail(MPRGP_EBADINPUT, "h must be positive");
   for (size_t i = 0; i < 3 * (size_t)f->N; ++i)
     if (!std::isfinite(x_n[i]) || !std::isfinite(v_n[i])) return fail(MPRGP_EBADINPUT, "non-finite x_n/v_n");
   mprgp_ctx *ctx = f->ctx;
   MPRGP_HIP(hipSetDevice(ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
   const size_t need = sizeof(double) * ((size_t)kElemOut * f->T + 3 * 3 * (size_t)f->N) + 1024;
   if (int rc = ctx->scratch_reserve(need)) return rc;
   const int N = f->N, T = f->T;
@@ -510,9 +516,11 @@ extern "C" int mprgp_fem_assemble(mprgp_fem *f, int material, const double *x_n,
   fem_element<<<(T + 127) / 128, 128, 0, s>>>(T, f->tets, f->dinv, f->vol, f->lam_mu, dx,
                                               material == MPRGP_COROTATIONAL, h2, elem);
   MPRGP_HIP(hipGetLastError());
-  fem_blocks<<<(unsigned)((f->nb + 255) / 256), 256, 0, s>>>(f->nb, T, f->blk_ptr, f->blk_ctr, f->blk_vbase,
-                                                              f->blk_diag, f->mass_rho, elem, f->A->d_val);
-  MPRGP_HIP(hipGetLastError());
+  if (!rhs_only) {
+    fem_blocks<<<(unsigned)((f->nb + 255) / 256), 256, 0, s>>>(f->nb, T, f->blk_ptr, f->blk_ctr, f->blk_vbase,
+                                                                f->blk_diag, f->mass_rho, elem, f->A->d_val);
+    MPRGP_HIP(hipGetLastError());
+  }
   fem_rhs<<<(N + 255) / 256, 256, 0, s>>>(N, T, f->node_ptr, f->node_ctr, f->node_mass, f->density, elem,
                                           dx, dv, gravity[0], gravity[1], gravity[2], h, db);
   MPRGP_HIP(hipGetLastError());
@@ -520,9 +528,9 @@ extern "C" int mprgp_fem_assemble(mprgp_fem *f, int material, const double *x_n,
   MPRGP_HIP(launch_gather_nodes(s, db, qp->b, f->A->d_perm, N));
   MPRGP_HIP(launch_gather_nodes(s, dx, qp->stage, f->A->d_perm, N));
   MPRGP_HIP(launch_spmv_plain(s, f->A->d_sl_off, f->A->d_col, f->A->d_val, qp->stage, qp->b, N, 1));
-  MPRGP_HIP(launch_setup_diag(s, f->A->d_sl_off, f->A->d_col, f->A->d_val, f->A->d_dinv, N));
+  if (!rhs_only) MPRGP_HIP(launch_setup_diag(s, f->A->d_sl_off, f->A->d_col, f->A->d_val, f->A->d_dinv, N));
   MPRGP_HIP(hipStreamSynchronize(s));
-  f->A->version++;
+  if (!rhs_only) f->A->version++;
   qp->has_rhs = true;
   return MPRGP_OK;
 }
@@ -531,6 +539,7 @@ extern "C" void mprgp_fem_destroy(mprgp_fem *f) {
   if (!f) return;
   mprgp_ctx *ctx = f->ctx;
   (void)hipSetDevice(ctx->device);
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
   (void)hipStreamSynchronize(ctx->stream);
   ctx->release(f->dev, f->dev_bytes);
   mprgp_matrix_destroy(f->A);

@@ -2,9 +2,11 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdarg>
 #include <cstdint>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -47,6 +49,38 @@ hipError_t launch_to_f32(hipStream_t s, const double *in, float *out, size_t n);
 int build_matrix(mprgp_ctx *ctx, int n_nodes, const int *rowptr, const int *colidx,
                  const double *blocks, mprgp_matrix **out);
 
+// Device launch tables of one batch driver (a context's solve_batch, or an
+// engine): QP descriptors, tiles, control words, done flags, captured graphs.
+struct BatchTables {
+  mprgp_ctx *ctx = nullptr;
+  QpDev *d_qps = nullptr;
+  int qps_cap = 0;
+  Tile *d_tiles = nullptr;
+  int tiles_cap = 0;
+  int *d_ctrl = nullptr;   // [0] = tile count, [1] = running QPs
+  int *d_flags = nullptr;  // per batch slot: tag of the QP that reached DONE
+  int flags_cap = 0;
+  std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
+  std::map<std::tuple<int, int, int, int>, hipGraph_t> graph_defs;
+  int init(mprgp_ctx *c);
+  int ensure(int n_qps, int n_tiles);
+  int graph(hipStream_t s, int ts, int steps, int grid, bool mixed, hipGraphExec_t *out);
+  void drop_graphs();
+  void destroy();
+};
+
+// Pieces of the batch solve shared by solve_batch and the engine.
+int check_batch_qp(const mprgp_ctx *ctx, const mprgp_qp *q, const mprgp_params &p, int i);
+int pick_ts(const mprgp_params &p, long total_slices);
+void append_tiles(int slot, int n_slices, int span, std::vector<Tile> &tiles);
+int tiles_for(int n_slices, int span);
+int round_grid(int n_tiles);
+long solve_step_bound(const mprgp_params &p);
+size_t solve_bytes(const mprgp_qp *q, const mprgp_params &p);
+int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, int *done_flag, int tag,
+               QpDev &d, QpState &st);
+int fill_stats(mprgp_qp *q, const QpState both[2], float ms, long launched, mprgp_stats *out);
+
 }  // namespace mprgp
 
 struct mprgp_ctx {
@@ -54,21 +88,12 @@ struct mprgp_ctx {
   hipStream_t stream = nullptr;
   size_t budget = 0;
   size_t in_use = 0;
-  // batch launch tables (device), grown on demand
-  mprgp::QpDev *d_qps = nullptr;
-  int qps_cap = 0;
-  mprgp::Tile *d_tiles = nullptr;
-  int tiles_cap = 0;
-  int *d_ctrl = nullptr;      // [0] = tile count, [1] = running QPs
-  int *d_flags = nullptr;     // per batch QP: reached DONE
-  int flags_cap = 0;
-  int *h_running = nullptr;   // pinned, 2 slots
+  std::mutex amu;            // guards budget accounting (alloc/release)
+  std::recursive_mutex mu;   // serialises calls that use `stream`
+  mprgp::BatchTables tab;    // solve_batch's launch tables
+  int *h_running = nullptr;  // pinned, 2 slots
   hipEvent_t t0 = nullptr, t1 = nullptr, q[2] = {nullptr, nullptr};
-  std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
-  std::map<std::tuple<int, int, int, int>, hipGraph_t> graph_defs;
-  // autotuner cache: node-count bucket -> (tiles per workgroup)
-  std::map<int, int> tuned_ts;
-  // scratch shared by calls on this context (calls on one context never overlap)
+  // scratch shared by calls on this context (serialised by `mu`)
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
   int scratch_reserve(size_t bytes);
@@ -124,4 +149,11 @@ struct mprgp_qp {
   double *fb = nullptr, *fg = nullptr, *fphi = nullptr, *fbeta = nullptr;  // FP64 gradient side
   float *fp = nullptr, *fz = nullptr, *fap = nullptr;   // FP32 direction side
   double *fcc = nullptr;  // shifted offsets c - n.x of the correction QP
+  std::atomic<int> busy{0};  // submitted to an engine and not yet completed
 };
+
+// Rejects calls on a QP an engine is solving.
+#define MPRGP_NOT_BUSY(q)                                                        \
+  do {                                                                           \
+    if ((q)->busy) return ::mprgp::fail(MPRGP_ESTATE, "qp is busy in an engine"); \
+  } while (0)

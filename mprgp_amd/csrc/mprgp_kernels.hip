@@ -790,11 +790,10 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
     if (r >= q.n_nodes) continue;
     if (f64) {
       row_work_fp64(q, s, lane, r, op, st, P);
+    } else if (MX && st.mixed) {
+      row_work<float>(q, q.f, s, lane, r, op, st, P);
     } else {
-      if constexpr (MX)
-        row_work<float>(q, q.f, s, lane, r, op, st, P);
-      else
-        row_work<double>(q, q.d, s, lane, r, op, st, P);
+      row_work<double>(q, q.d, s, lane, r, op, st, P);
     }
   }
 }
@@ -803,7 +802,8 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
 #define MPRGP_STEP_MIN_WAVES 3
 #endif
 
-// MX: the batch runs the mixed form (inner ops on the FP32 copies).
+// MX: the batch holds QPs of the mixed form (their inner ops run on the FP32
+// copies); FP64 QPs of the same batch take the FP64 path.
 template <int TS, bool MX>
 __global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(const QpDev *__restrict__ qps,
                                                            const Tile *__restrict__ tiles,
@@ -883,7 +883,7 @@ __global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(
           sst.opcnt[op]++;
         } else if (!prev_done) {
           atomicSub(running, 1);
-          *q.done_flag = 1;
+          *q.done_flag = q.tag;
         }
       }
       __syncthreads();

@@ -1,0 +1,127 @@
+"""Serving engine: continuous batching of device-resident session QPs.
+
+The engine fixes the tile shape, so a QP's partial sums are reduced in the
+same order whatever else is in the batch: every result must be bit-identical
+alone and in any mix, and identical to the direct solve with that shape."""
+import numpy as np
+import pytest
+
+import mprgp_amd as M
+from mprgp_amd import serving
+
+pytestmark = pytest.mark.gpu
+
+
+def sessions(ctx, seed=11, count=6):
+    return serving.make_sessions(ctx, count=count, seed=seed, dof_range=(1.5e3, 9e3), reassemble=2)
+
+
+def prec(s):
+    return M.MIXED if s.sid % 2 == 0 else M.FP64
+
+
+def test_engine_equals_direct_solve(ctx):
+    a, b = sessions(ctx, seed=12, count=2), sessions(ctx, seed=12, count=2)
+    eng = M.Engine(ctx, max_active=4, tiles=4)
+    for sa, sb in zip(a, b):
+        sa.prepare()
+        sb.prepare()
+        rc, st = sa.qp.solve(serving.solve_params(sa, 1e-6, prec(sa), tiles=4))
+        t = eng.submit(sb.qp, serving.solve_params(sb, 1e-6, prec(sb)))
+        rc2, st2 = eng.wait(t)
+        assert rc == rc2 == 0
+        assert st["iterations"] == st2["iterations"]
+        np.testing.assert_array_equal(sa.qp.get_x(), sb.qp.get_x())
+    eng.close()
+
+
+def test_sessions_alone_equal_mix(ctx):
+    steps = 5
+    alone, mix = sessions(ctx), sessions(ctx)
+    eng = M.Engine(ctx, max_active=8, tiles=4, graph_steps=8)
+    for s in alone:  # one session at a time
+        for _ in range(steps):
+            s.prepare()
+            rc, st = eng.wait(eng.submit(s.qp, serving.solve_params(s, 1e-6, prec(s))))
+            assert rc == 0, st
+            s.complete(keep_history=True)
+    # all sessions at once; a finished session immediately issues its next
+    # step, so the sessions drift apart and every step meets a different mix
+    rng = np.random.default_rng(3)
+    order = rng.permutation(len(mix))
+    live = {}
+    for i in order:
+        s = mix[i]
+        s.prepare()
+        live[eng.submit(s.qp, serving.solve_params(s, 1e-6, prec(s)))] = s
+    peak = 0
+    while live:
+        peak = max(peak, eng.info()["active"])
+        t, rc, st = eng.wait_any(-1)
+        assert rc == 0, st
+        s = live.pop(t)
+        s.complete(keep_history=True)
+        if s.step < steps:
+            s.prepare()
+            live[eng.submit(s.qp, serving.solve_params(s, 1e-6, prec(s)))] = s
+    info = eng.info()
+    print(f"engine: {info}")
+    assert info["peak_active"] > 1
+    for sa, sm in zip(alone, mix):
+        assert len(sa.history) == len(sm.history) == steps
+        for k, (xa, xm) in enumerate(zip(sa.history, sm.history)):
+            np.testing.assert_array_equal(xa, xm, err_msg=f"session {sa.sid} step {k}")
+    eng.close()
+
+
+def test_budget_caps_admission(ctx):
+    ss = sessions(ctx, seed=13, count=5)
+    for s in ss:
+        s.prepare()
+    need = [ss[0].qp, ss[1].qp]
+    import mprgp_amd._lib as L
+    # budget: the two largest working sets side by side, never three
+    sizes = sorted(((s.fem.matrix.info()["bytes"]) for s in ss), reverse=True)
+    eng = M.Engine(ctx, max_active=8, tiles=4, hbm_budget=int(3.2 * sizes[0]))
+    ts = [eng.submit(s.qp, serving.solve_params(s, 1e-6, M.FP64)) for s in ss]
+    for t in ts:
+        rc, st = eng.wait(t)
+        assert rc == 0
+    info = eng.info()
+    print(f"budget {3.2 * sizes[0]:.0f} B: {info}")
+    assert info["peak_bytes"] <= 3.2 * sizes[0]
+    assert info["completed"] == len(ss)
+    eng.close()
+
+
+def test_submit_errors(ctx):
+    ss = sessions(ctx, seed=14, count=1)
+    s = ss[0]
+    s.prepare()
+    eng = M.Engine(ctx, max_active=2, tiles=4, hbm_budget=1024)
+    with pytest.raises(M.MprgpError) as e:
+        eng.submit(s.qp, serving.solve_params(s, 1e-6, M.FP64))
+    assert e.value.code == -7  # EBUDGET
+    eng.close()
+    eng = M.Engine(ctx, max_active=2, tiles=4)
+    t = eng.submit(s.qp, serving.solve_params(s, 1e-6, M.FP64))
+    with pytest.raises(M.MprgpError) as e:
+        eng.submit(s.qp, serving.solve_params(s, 1e-6, M.FP64))
+    assert e.value.code == -8  # busy
+    with pytest.raises(M.MprgpError) as e:
+        s.qp.set_x(s.scene.x_n.ravel())
+    assert e.value.code == -8
+    rc, st = eng.wait(t)
+    assert rc == 0
+    s.qp.set_x(s.scene.x_n.ravel())  # free again once collected
+    eng.close()
+
+
+def test_open_loop_small(ctx):
+    ss = sessions(ctx, seed=15, count=8)
+    eng = M.Engine(ctx, max_active=16, tiles=4)
+    out = serving.run_open_loop(eng, ss, rate=300.0, duration=1.0, precision=M.MIXED, workers=2)
+    print(out)
+    assert out["converged"] == out["requests"]
+    assert out["p99_ms"] >= out["p50_ms"] > 0
+    eng.close()
