@@ -52,7 +52,9 @@ extern "C" {
 #define MPRGP_LINEAR 1
 #define MPRGP_COROTATIONAL 2
 /* flag or-ed into the material of mprgp_fem_assemble: write b only and keep
- * the matrix of the last full assembly (lagged stiffness between re-assemblies) */
+ * the matrix of the last full assembly (lagged stiffness between
+ * re-assemblies).  COROTATIONAL forces then use the element rotations of that
+ * assembly, so b and the kept matrix are one linearisation. */
 #define MPRGP_RHS_ONLY 0x100
 
 /* precision forms */
@@ -215,8 +217,9 @@ typedef struct mprgp_engine_info {
 
 void mprgp_engine_config_default(mprgp_engine_config *cfg);
 int mprgp_engine_create(mprgp_ctx *ctx, const mprgp_engine_config *cfg, mprgp_engine **out);
-/* Queues a solve of qp (rhs, rows and x set).  The QP is busy until its ticket
- * is waited for.  MPRGP_EBUDGET if this QP alone exceeds the budget. */
+/* Queues a solve of qp (rhs, rows and x set; any context on the engine's
+ * device).  The QP is busy until its ticket is waited for.  MPRGP_EBUDGET if
+ * this QP alone exceeds the budget. */
 int mprgp_engine_submit(mprgp_engine *eng, mprgp_qp *qp, const mprgp_params *params, long long *ticket);
 /* Waits up to timeout_ms (< 0: forever) for a ticket; returns its solve code
  * (0, -1, or an error) and fills stats (elapsed_ms = submit-to-completion

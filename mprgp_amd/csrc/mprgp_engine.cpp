@@ -370,7 +370,10 @@ extern "C" int mprgp_engine_submit(mprgp_engine *e, mprgp_qp *q, const mprgp_par
     mprgp_params_default(&def);
     params = &def;
   }
-  if (int rc = check_batch_qp(e->ctx, q, *params, 0)) return rc;
+  // QPs of any context on the engine's device (each session may keep its own
+  // context, so its uploads do not serialise with other sessions')
+  if (q->ctx->device != e->ctx->device) return fail(MPRGP_EBADINPUT, "qp lives on another device");
+  if (int rc = check_batch_qp(q->ctx, q, *params, 0)) return rc;
   const size_t bytes = solve_bytes(q, *params);
   if (e->budget && bytes > e->budget)
     return fail(MPRGP_EBUDGET, "QP needs %zu bytes, engine budget is %zu", bytes, e->budget);

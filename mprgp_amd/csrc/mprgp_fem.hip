@@ -161,11 +161,16 @@ __device__ void polar_rotation(const double *F, double *R) {
     }
 }
 
+// rot: per-tet rotations, [entry][tet].  corot && !lag: R is the polar
+// rotation of F and is stored; corot && lag: R is the stored rotation of the
+// last full assembly (the element is then linear in x with exactly the
+// stiffness of that assembly).  rhs_only skips the stiffness blocks.
 __global__ void __launch_bounds__(128) fem_element(int T, const int4 *__restrict__ tets,
                                                    const double *__restrict__ dinv,
                                                    const double *__restrict__ vol,
                                                    const double *__restrict__ lam_mu,
-                                                   const double *__restrict__ x, int corot, double h2,
+                                                   const double *__restrict__ x, int corot, int lag,
+                                                   int rhs_only, double h2, double *__restrict__ rot,
                                                    double *__restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= T) return;
@@ -192,8 +197,11 @@ __global__ void __launch_bounds__(128) fem_element(int T, const int4 *__restrict
       F[r * 3 + c] = s;
     }
   double R[9];
-  if (corot) {
+  if (corot && lag) {
+    for (int k = 0; k < 9; ++k) R[k] = rot[(size_t)k * T + t];
+  } else if (corot) {
     polar_rotation(F, R);
+    for (int k = 0; k < 9; ++k) rot[(size_t)k * T + t] = R[k];
   } else {
     for (int i = 0; i < 9; ++i) R[i] = (i % 4 == 0) ? 1.0 : 0.0;
   }
@@ -234,7 +242,7 @@ __global__ void __launch_bounds__(128) fem_element(int T, const int4 *__restrict
       u[I][r] = s;
     }
   // h^2 K_e blocks
-  for (int I = 0; I < 4; ++I)
+  for (int I = 0; I < 4 && !rhs_only; ++I)
     for (int J = 0; J < 4; ++J) {
       const double bb = (B[I][0] * B[J][0] + B[I][1] * B[J][1]) + B[I][2] * B[J][2];
       for (int r = 0; r < 3; ++r)
@@ -322,6 +330,8 @@ struct mprgp_fem {
   double *dinv = nullptr, *vol = nullptr, *lam_mu = nullptr, *node_mass = nullptr, *mass_rho = nullptr;
   int *blk_ptr = nullptr, *blk_ctr = nullptr, *blk_vbase = nullptr, *blk_diag = nullptr;
   int *node_ptr = nullptr, *node_ctr = nullptr;
+  double *rot = nullptr;   // per-tet rotations of the last corotational assembly
+  bool rot_valid = false;
   double density = 0.0;
 };
 
@@ -441,7 +451,7 @@ extern "C" int mprgp_fem_create(mprgp_ctx *ctx, int N, int T, const int *tets_in
   f->T = T;
   f->nb = nb;
   f->density = density;
-  size_t bytes = 256 * 16 + sizeof(int4) * T + sizeof(double) * (9 + 1 + 2) * (size_t)T +
+  size_t bytes = 256 * 17 + sizeof(int4) * T + sizeof(double) * (9 + 1 + 2 + 9) * (size_t)T +
                  sizeof(double) * 2 * N + sizeof(int) * ((nb + 1) + bctr.size() + 2 * nb + (N + 1) + nctr.size());
   if (int rc = ctx->alloc(&f->dev, bytes)) {
     mprgp_matrix_destroy(A);
@@ -462,6 +472,7 @@ extern "C" int mprgp_fem_create(mprgp_ctx *ctx, int N, int T, const int *tets_in
   f->blk_diag = carve<int>(p, nb);
   f->node_ptr = carve<int>(p, N + 1);
   f->node_ctr = carve<int>(p, nctr.size());
+  f->rot = carve<double>(p, 9 * (size_t)T);
   hipStream_t s = ctx->stream;
   hipError_t e = hipMemcpyAsync(f->tets, tets.data(), sizeof(int) * 4 * (size_t)T, hipMemcpyHostToDevice, s);
   if (!e) e = hipMemcpyAsync(f->dinv, dinv.data(), sizeof(double) * dinv.size(), hipMemcpyHostToDevice, s);
@@ -494,6 +505,8 @@ extern "C" int mprgp_fem_assemble(mprgp_fem *f, int material, const double *x_n,
   if (material != MPRGP_LINEAR && material != MPRGP_COROTATIONAL)
     return fail(MPRGP_EBADINPUT, "unknown material %d", material);
   if (rhs_only && f->A->version == 0) return fail(MPRGP_ESTATE, "RHS-only assembly before any full assembly");
+  if (rhs_only && material == MPRGP_COROTATIONAL && !f->rot_valid)
+    return fail(MPRGP_ESTATE, "corotational RHS-only assembly needs a full corotational assembly first");
   if (qp->A != f->A) return fail(MPRGP_EBADINPUT, "qp was not created on this FEM's matrix");
   MPRGP_NOT_BUSY(qp);
   if (!(h > 0.0) || !std::isfinite(h)) return fail(MPRGP_EBADINPUT, "h must be positive");
@@ -513,8 +526,9 @@ extern "C" int mprgp_fem_assemble(mprgp_fem *f, int material, const double *x_n,
   const double h2 = h * h;
   MPRGP_HIP(hipMemcpyAsync(dx, x_n, sizeof(double) * 3 * N, hipMemcpyHostToDevice, s));
   MPRGP_HIP(hipMemcpyAsync(dv, v_n, sizeof(double) * 3 * N, hipMemcpyHostToDevice, s));
-  fem_element<<<(T + 127) / 128, 128, 0, s>>>(T, f->tets, f->dinv, f->vol, f->lam_mu, dx,
-                                              material == MPRGP_COROTATIONAL, h2, elem);
+  const int corot = material == MPRGP_COROTATIONAL;
+  fem_element<<<(T + 127) / 128, 128, 0, s>>>(T, f->tets, f->dinv, f->vol, f->lam_mu, dx, corot,
+                                              rhs_only ? 1 : 0, rhs_only ? 1 : 0, h2, f->rot, elem);
   MPRGP_HIP(hipGetLastError());
   if (!rhs_only) {
     fem_blocks<<<(unsigned)((f->nb + 255) / 256), 256, 0, s>>>(f->nb, T, f->blk_ptr, f->blk_ctr, f->blk_vbase,
@@ -530,7 +544,10 @@ extern "C" int mprgp_fem_assemble(mprgp_fem *f, int material, const double *x_n,
   MPRGP_HIP(launch_spmv_plain(s, f->A->d_sl_off, f->A->d_col, f->A->d_val, qp->stage, qp->b, N, 1));
   if (!rhs_only) MPRGP_HIP(launch_setup_diag(s, f->A->d_sl_off, f->A->d_col, f->A->d_val, f->A->d_dinv, N));
   MPRGP_HIP(hipStreamSynchronize(s));
-  if (!rhs_only) f->A->version++;
+  if (!rhs_only) {
+    f->A->version++;
+    f->rot_valid = corot;
+  }
   qp->has_rhs = true;
   return MPRGP_OK;
 }

@@ -79,3 +79,33 @@ def test_assembled_system_solves_like_oracle(ctx):
 def test_assembly_config2_size(ctx):
     sc = problems.config2()
     check_system(ctx, sc, sc.x_n, M.LINEAR)
+
+
+@pytest.mark.parametrize("mtype", [M.LINEAR, M.COROTATIONAL])
+def test_rhs_only_is_the_kept_linearisation(ctx, mtype):
+    """RHS-only assembly at x' keeps A (and, for COROT, the element rotations)
+    of the full assembly at x0.  The element energy is then quadratic in x with
+    exactly the kept stiffness, so f(x') = f(x0) - K (x' - x0) and
+    b(x') = b(x0) + M (x' - x0) with the lumped mass M: a closed form that a
+    wrong rotation, a stale force or a missing A x' term all break."""
+    sc = problems.config1_flat()
+    x0 = deformed(sc, 3)
+    fem = M.Fem(ctx, sc.X0, sc.tets, sc.young, sc.poisson, sc.density)
+    q = fem.matrix.qp()
+    if mtype == M.COROTATIONAL:
+        with pytest.raises(M.MprgpError):  # no rotations kept yet
+            fem.assemble(q, x0.ravel(), sc.v_n.ravel(), sc.gravity, sc.h, mtype | M.RHS_ONLY)
+    fem.assemble(q, x0.ravel(), sc.v_n.ravel(), sc.gravity, sc.h, mtype)
+    b0 = q.get_rhs()
+    rng = np.random.default_rng(4)
+    x1 = x0 + 0.01 * rng.normal(size=x0.shape) + np.array([0.0, -0.05, 0.02])
+    probe = rng.normal(size=3 * sc.n_nodes)
+    Ax = fem.matrix.spmv(probe)
+    fem.assemble(q, x1.ravel(), sc.v_n.ravel(), sc.gravity, sc.h, mtype | M.RHS_ONLY)
+    b1 = q.get_rhs()
+    np.testing.assert_array_equal(fem.matrix.spmv(probe), Ax)  # matrix kept
+    _, _, _, mass = oracle.fem_setup(sc.n_nodes, sc.tets, sc.X0)
+    m3 = np.repeat(mass * sc.density, 3)
+    expect = b0 + m3 * (x1 - x0).ravel()
+    err = np.abs(b1 - expect).max() / np.abs(b1).max()
+    assert err <= 1e-12, err

@@ -44,7 +44,7 @@ def test_sessions_alone_equal_mix(ctx):
             s.prepare()
             rc, st = eng.wait(eng.submit(s.qp, serving.solve_params(s, 1e-6, prec(s))))
             assert rc == 0, st
-            s.complete(keep_history=True)
+            s.complete(keep_history=True, rc=rc, stats=st)
     # all sessions at once; a finished session immediately issues its next
     # step, so the sessions drift apart and every step meets a different mix
     rng = np.random.default_rng(3)
@@ -60,7 +60,7 @@ def test_sessions_alone_equal_mix(ctx):
         t, rc, st = eng.wait_any(-1)
         assert rc == 0, st
         s = live.pop(t)
-        s.complete(keep_history=True)
+        s.complete(keep_history=True, rc=rc, stats=st)
         if s.step < steps:
             s.prepare()
             live[eng.submit(s.qp, serving.solve_params(s, 1e-6, prec(s)))] = s
