@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64-batch", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-seconds", type=float, default=3.0)
     ap.add_argument("--ts", type=int, default=0, help="wavefronts per workgroup (0: library choice)")
     ap.add_argument("--spw", type=int, default=0, help="slices per wavefront per tile (0: library choice)")
     ap.add_argument("--graph-steps", type=int, default=0)
@@ -79,6 +81,8 @@ def main():
     log(f"power method: {time.perf_counter() - t1:.2f}s, "
         f"iterations max {max(s['power_iters'] for s in st)}")
 
+    copy_gbps = ctx.copy_probe(1 << 30, 10)
+    log(f"copy probe: {copy_gbps:.0f} GB/s")
     prec = M.MIXED if args.precision == "mixed" else M.FP64
     mixed = prec == M.MIXED
 
@@ -121,6 +125,7 @@ def main():
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None, "bytes_per_launch": int(alg / max(launches, 1)),
             "avg_launch_us": round(1e6 * dev_s / max(launches, 1), 2), "launches": launches,
+            "copy_gbps": round(copy_gbps, 1),
             "iterations_total": int(sum(s["iterations"] for s in stats))}
     log(f"roofline: {roof}")
 
@@ -190,6 +195,21 @@ def main():
         err["vs_oracle_sample"] = done
         log(f"cpu oracle: {cpu}")
 
+    serve = None
+    if not args.no_serving:
+        from mprgp_amd import serving
+        ss = serving.make_sessions(ctx, count=64, seed=5, own_contexts=True)
+        eng = M.Engine(ctx, max_active=256, tiles=4, hbm_budget=int(200e9))
+        serving.run_open_loop(eng, ss, rate=2000.0, duration=0.3, precision=prec, seed=105)  # warm-up
+        serve = serving.run_open_loop(eng, ss, rate=2000.0, duration=args.serving_seconds,
+                                      precision=prec, seed=6)
+        serve.pop("engine", None)
+        serve["sessions"] = len(ss)
+        serve["dof_total"] = int(sum(3 * s.n_nodes for s in ss))
+        serve["plain_fp64_one_at_a_time"] = serving.run_one_at_a_time(ss, n_req=150, seed=7)
+        eng.close()
+        log(f"serving: {serve}")
+
     out = {
         "metric": "MPRGP solves/s at KKT res \u22641e-6, batched FP32/FP64 vs plain FP64; p50/p99 ms",
         "value": round(value, 2),
@@ -213,6 +233,7 @@ def main():
         "plain_fp64_one_at_a_time": plain,
         "speedup_vs_plain": round(value / plain["solves_per_s"], 2) if plain else None,
         "solution_rel_err": err,
+        "serving": serve,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -187,6 +187,13 @@ int mprgp_fem_assemble(mprgp_fem *fem, int material, const double *x_n, const do
 mprgp_matrix *mprgp_fem_matrix(mprgp_fem *fem);
 void mprgp_fem_destroy(mprgp_fem *fem);
 
+/* ---- bandwidth probe ------------------------------------------------------
+ * No counterpart in the reference: copies `bytes` between two device buffers
+ * `iters` times with a streaming kernel and returns the HBM rate (read +
+ * write bytes per second, GB/s) -- the achievable bound beside the published
+ * peak, and the known-size kernel that calibrates profiler byte counters. */
+int mprgp_copy_probe(mprgp_ctx *ctx, size_t bytes, int iters, double *gbps);
+
 /* ---- serving engine (continuous batching) --------------------------------
  * No counterpart in the reference, which calls MPRGPDecoupledCon::solve once
  * per frame and per body (MprgpFemSolver.cpp:532-533); the engine is the

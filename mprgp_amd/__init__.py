@@ -90,6 +90,12 @@ class Context:
         check(load().mprgp_ctx_memory(self.h, C.byref(u), C.byref(b)))
         return u.value, b.value
 
+    def copy_probe(self, nbytes=1 << 30, iters=20) -> float:
+        """Device copy rate in GB/s (read + write) of a streaming kernel."""
+        g = C.c_double()
+        check(load().mprgp_copy_probe(self.h, C.c_size_t(int(nbytes)), int(iters), C.byref(g)))
+        return g.value
+
     def matrix_bsr3(self, n_nodes, rowptr, colidx, blocks) -> "Matrix":
         rowptr, colidx = _i32(rowptr), _i32(colidx)
         blocks = _f64(blocks)

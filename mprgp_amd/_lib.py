@@ -117,6 +117,7 @@ SIGNATURES = {
     "mprgp_fem_assemble": (I, [P, I, P, P, P, D, P]),
     "mprgp_fem_matrix": (P, [P]),
     "mprgp_fem_destroy": (None, [P]),
+    "mprgp_copy_probe": (I, [P, C.c_size_t, I, C.POINTER(D)]),
     "mprgp_engine_config_default": (None, [C.POINTER(EngineConfig)]),
     "mprgp_engine_create": (I, [P, C.POINTER(EngineConfig), PP]),
     "mprgp_engine_submit": (I, [P, P, C.POINTER(Params), C.POINTER(C.c_longlong)]),

@@ -1120,6 +1120,34 @@ extern "C" int mprgp_solve_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps
   return run_batch(ctx, count, qps, params, params_stride ? 1 : 0, stats);
 }
 
+extern "C" int mprgp_copy_probe(mprgp_ctx *ctx, size_t bytes, int iters, double *gbps) {
+  if (!ctx || !gbps || iters <= 0 || bytes < 4096) return fail(MPRGP_EBADINPUT, "bad probe arguments");
+  bytes &= ~(size_t)15;
+  MPRGP_HIP(hipSetDevice(ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
+  void *a = nullptr, *b = nullptr;
+  if (int rc = ctx->alloc(&a, bytes)) return rc;
+  if (int rc = ctx->alloc(&b, bytes)) {
+    ctx->release(a, bytes);
+    return rc;
+  }
+  hipStream_t s = ctx->stream;
+  int rc = MPRGP_OK;
+  hipError_t e = hipMemsetAsync(a, 0, bytes, s);
+  if (!e) e = launch_copy_stream(s, a, b, bytes);  // warm-up
+  if (!e) e = hipEventRecord(ctx->t0, s);
+  for (int i = 0; i < iters && !e; ++i) e = launch_copy_stream(s, (i & 1) ? b : a, (i & 1) ? a : b, bytes);
+  if (!e) e = hipEventRecord(ctx->t1, s);
+  if (!e) e = hipStreamSynchronize(s);
+  float ms = 0.f;
+  if (!e) e = hipEventElapsedTime(&ms, ctx->t0, ctx->t1);
+  if (e) rc = fail(MPRGP_EHIP, "copy probe: %s", hipGetErrorString(e));
+  else *gbps = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
+  ctx->release(a, bytes);
+  ctx->release(b, bytes);
+  return rc;
+}
+
 extern "C" int mprgp_decoupled_solve(int device, int n, const int *a_outer, const int *a_inner,
                                      const double *a_val, const double *B, int m, const int *j_outer,
                                      const int *j_inner, const double *j_val, const double *c, double *x,

@@ -43,6 +43,7 @@ hipError_t launch_setup_rows(hipStream_t s, const double *dinv, const double *cn
 hipError_t launch_spmv_plain(hipStream_t s, const int *sl_off, const int *col, const double *val,
                              const double *x, double *y, int n, int accumulate);
 hipError_t launch_to_f32(hipStream_t s, const double *in, float *out, size_t n);
+hipError_t launch_copy_stream(hipStream_t s, const void *in, void *out, size_t bytes);
 
 // Builds the SELL-64 device matrix from a node-order block CSR (blocks may be
 // NULL for a zero matrix of that pattern).

@@ -138,11 +138,13 @@ __device__ __forceinline__ V3 precond_of(const Vecs<T> &v, bool on, const Row &R
 // Accumulates A v for the lane's block row over the slice's blocks; `gather`
 // returns NV input vectors at a neighbour node.  Padding slots hold a valid
 // column and zero values, so the loop has no branch and the column indices,
-// matrix values and gathers of U blocks are all in flight together.
+// matrix values and gathers of U blocks are all in flight together.  U is
+// kept small enough (2 blocks for one vector, 1 for two) that the step kernel
+// needs no scratch: spilled registers would go to HBM on every launch.
 template <int NV, class T, class G>
 __device__ __forceinline__ void spmv_row(const int *sl_off, const int *col, const T *val, int s, int lane,
                                          G gather, V3 (&acc)[NV]) {
-  constexpr int U = NV == 1 ? 4 : 2;
+  constexpr int U = NV == 1 ? 2 : 1;
 #pragma unroll
   for (int t = 0; t < NV; ++t) acc[t] = V3{0.0, 0.0, 0.0};
   const int b0 = sl_off[s];
@@ -799,7 +801,7 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
 }
 
 #ifndef MPRGP_STEP_MIN_WAVES
-#define MPRGP_STEP_MIN_WAVES 3
+#define MPRGP_STEP_MIN_WAVES 4
 #endif
 
 // MX: the batch holds QPs of the mixed form (their inner ops run on the FP32
@@ -967,6 +969,21 @@ __global__ void to_f32(const double *__restrict__ in, float *__restrict__ out, s
   for (; i < n; i += stride) out[i] = (float)in[i];
 }
 
+// Streaming copy (16 B per lane per iteration): the bandwidth probe that
+// calibrates the roofline and the profiler's HBM byte counters.
+__global__ void __launch_bounds__(256) copy_stream(const double2 *__restrict__ in, double2 *__restrict__ out,
+                                                   size_t n) {
+  // four independent 16 B loads in flight per lane, each a coalesced 4 KiB block row
+  const size_t i0 = (size_t)blockIdx.x * 1024 + threadIdx.x;
+  double2 v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (i0 + 256 * k < n) v[k] = in[i0 + 256 * k];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (i0 + 256 * k < n) out[i0 + 256 * k] = v[k];
+}
+
 // Plain SELL SpMV y = A x (used by tests and by the assembly's A x1 term).
 __global__ void __launch_bounds__(256) spmv_plain(const int *__restrict__ sl_off,
                                                   const int *__restrict__ col,
@@ -1031,6 +1048,14 @@ hipError_t launch_to_f32(hipStream_t s, const double *in, float *out, size_t n) 
   if (n == 0) return hipSuccess;
   size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
   to_f32<<<(unsigned)blocks, 256, 0, s>>>(in, out, n);
+  return hipGetLastError();
+}
+hipError_t launch_copy_stream(hipStream_t s, const void *in, void *out, size_t bytes) {
+  const size_t n = bytes / sizeof(double2);
+  if (n == 0) return hipSuccess;
+  const size_t blocks = (n + 1023) / 1024;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  copy_stream<<<(unsigned)blocks, 256, 0, s>>>((const double2 *)in, (double2 *)out, n);
   return hipGetLastError();
 }
 hipError_t launch_spmv_plain(hipStream_t s, const int *sl_off, const int *col, const double *val,
