@@ -20,10 +20,14 @@ Prints one JSON line.
 from __future__ import annotations
 
 import argparse
+import csv
 import json
 import os
 import platform
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -36,6 +40,81 @@ HBM_PEAK_GBS = 8000.0
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def pmc_child(args):
+    """Run under rocprofv3 --pmc by measure_traffic: the copy probe of known
+    size, or one batched solve of the benchmark workload (direct launches)."""
+    import mprgp_amd as M
+    ctx = M.Context(0)
+    if args.pmc_child == "copy":
+        ctx.copy_probe(COPY_BYTES, 2)
+        return
+    from mprgp_amd import problems, workloads
+    scenes = problems.batch_scenes(args.qps, seed=4, tol=args.tol, max_it=4000)
+    qps = [workloads.DeviceQP(ctx, sc) for sc in scenes]
+    rc, st = M.solve_batch(ctx, [q.qp for q in qps], M.params(tol=args.tol, max_it=0, power_tol=1e-6))
+    prec = M.MIXED if args.precision == "mixed" else M.FP64
+    per = [M.params(tol=args.tol, max_it=4000, alpha_bar=s["alpha_bar"], project_first=1, precision=prec,
+                    tiles=4, graph_steps=-16) for s in st]
+    rc, stats = M.solve_batch(ctx, [q.qp for q in qps], per_qp=per)
+    print(json.dumps({"launches": stats[0]["launches"]}), flush=True)
+
+
+COPY_BYTES = 1 << 28
+
+
+def measure_traffic(args):
+    """HBM bytes per launch of the step kernel in one batched solve: two
+    rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over a child run of
+    the workload, each scaled by what the same counter reads for the copy
+    probe (COPY_BYTES read and written per dispatch).  Runs before this
+    process touches the GPU.  Returns None when rocprofv3 is unavailable."""
+    rocprof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(rocprof):
+        return None
+    tmp = tempfile.mkdtemp(prefix="mprgp_pmc_")
+    kernel = "mprgp_step<4, true>" if args.precision == "mixed" else "mprgp_step<4, false>"
+
+    def run(counter, child, tag):
+        d = os.path.join(tmp, tag)
+        cmd = ["timeout", "-s", "KILL", "300", rocprof, "--pmc", counter, "--output-format", "csv",
+               "-d", d, "-o", "run", "--", sys.executable, "-u", os.path.abspath(__file__),
+               "--pmc-child", child, "--qps", str(args.qps), "--tol", str(args.tol),
+               "--precision", args.precision]
+        env = dict(os.environ, MPRGP_NO_GRAPH="1")
+        r = subprocess.run(cmd, capture_output=True, text=True, env=env)
+        if r.returncode != 0:
+            raise RuntimeError(f"rocprofv3 {counter} {child} failed ({r.returncode}): {r.stderr[-400:]}")
+        launches = None
+        for line in r.stdout.splitlines():
+            if line.startswith("{"):
+                launches = json.loads(line)["launches"]
+        vals = []
+        for root, _, files in os.walk(d):
+            for fn in files:
+                if fn.endswith("counter_collection.csv"):
+                    with open(os.path.join(root, fn)) as f:
+                        for row in csv.DictReader(f):
+                            name = row["Kernel_Name"]
+                            if (child == "copy" and "copy_stream" in name) or (child == "solve" and kernel in name):
+                                vals.append(float(row["Counter_Value"]))
+        if child == "solve" and launches:
+            vals = vals[-launches:]  # the timed solve's launches (the power method runs first)
+        return sum(vals) / len(vals) * 1024.0 if vals else None  # KiB -> bytes
+
+    try:
+        cal_f = COPY_BYTES / run("FETCH_SIZE", "copy", "cf")
+        cal_w = COPY_BYTES / run("WRITE_SIZE", "copy", "cw")
+        fetch = run("FETCH_SIZE", "solve", "sf") * cal_f
+        write = run("WRITE_SIZE", "solve", "sw") * cal_w
+    except (RuntimeError, TypeError, ZeroDivisionError) as e:
+        log(f"traffic measurement skipped: {e}")
+        return None
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return {"fetch_bytes": int(fetch), "write_bytes": int(write), "fetch_scale": round(cal_f, 4),
+            "write_scale": round(cal_w, 4)}
 
 
 def main():
@@ -55,11 +134,21 @@ def main():
     ap.add_argument("--ts", type=int, default=0, help="wavefronts per workgroup (0: library choice)")
     ap.add_argument("--spw", type=int, default=0, help="slices per wavefront per tile (0: library choice)")
     ap.add_argument("--graph-steps", type=int, default=0)
+    ap.add_argument("--no-tune", action="store_true", help="library default shape instead of the autotuner's")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM counter passes")
+    ap.add_argument("--pmc-child", choices=["copy", "solve"], help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_child:
+        pmc_child(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the batched QPs shard across "
                           "GPUs with no communication (see DESIGN.md), but only --gpus 1 is run"}))
         sys.exit(2)
+
+    traffic = None if args.no_traffic else measure_traffic(args)
+    if traffic:
+        log(f"traffic per launch: {traffic}")
 
     import mprgp_amd as M
     from mprgp_amd import problems, workloads
@@ -94,6 +183,12 @@ def main():
         return per
 
     per = batch_params(prec)
+    shape = None
+    if not args.no_tune and not (args.ts or args.spw or args.graph_steps):
+        from mprgp_amd import tune
+        t1 = time.perf_counter()
+        per, shape, hit = tune.tuned_params(ctx, [q.qp for q in qps], per, prec, log=log)
+        log(f"launch shape {shape} ({'cached' if hit else 'tuned'}, {time.perf_counter() - t1:.1f}s)")
 
     def batch_step(params):
         for q in qps:
@@ -123,7 +218,8 @@ def main():
     achieved = alg / dev_s / 1e9
     roof = {"kernel": "mprgp_step", "bound": "hbm", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None, "bytes_per_launch": int(alg / max(launches, 1)),
+            "traffic": (traffic["fetch_bytes"] + traffic["write_bytes"]) if traffic else None,
+            "traffic_detail": traffic, "bytes_per_launch": int(alg / max(launches, 1)),
             "avg_launch_us": round(1e6 * dev_s / max(launches, 1), 2), "launches": launches,
             "copy_gbps": round(copy_gbps, 1),
             "iterations_total": int(sum(s["iterations"] for s in stats))}
@@ -132,6 +228,9 @@ def main():
     other = None
     if mixed and not args.no_fp64_batch:
         per64 = batch_params(M.FP64)
+        if shape is not None:
+            from mprgp_amd import tune
+            per64, _, _ = tune.tuned_params(ctx, [q.qp for q in qps], per64, M.FP64, log=log)
         batch_step(per64)
         k = max(1, args.steps // 2)
         s0 = time.perf_counter()
@@ -224,6 +323,7 @@ def main():
         "dtype": "f64" if not mixed else "f32+f64",
         "data": "synthetic",
         "config": {"workload": "256 beam contact QPs batched, DoF 5k-60k each", "qps": args.qps,
+                   "launch_shape": shape,
                    "dof_total": 3 * total_nodes, "blocks_total": total_blocks,
                    "dof_range": [5000, 60000], "tol": args.tol, "precision": args.precision,
                    "converged": conv},

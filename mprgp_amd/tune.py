@@ -1,0 +1,101 @@
+"""Autotuner of the batched solve's launch shape.
+
+The step kernel's shape is (tiles = wavefronts per workgroup, slices per
+wavefront) -- one wavefront streams 64 block rows per slice -- and the
+number of launches captured per hipGraph replay.  Every candidate changes
+only the schedule (and, through the tile grouping of partial sums, the last
+bits of the reductions), never the algorithm.  ``tune`` times each candidate
+on the caller's own batch from the same warm start and keeps the fastest;
+``tuned_params`` looks the answer up in a JSON cache keyed by device, batch
+size class and precision, tuning on a miss.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import time
+
+import mprgp_amd as M
+
+DEFAULT_CANDIDATES = [
+    {"tiles": 4, "spw": 1, "graph_steps": 16},
+    {"tiles": 4, "spw": 1, "graph_steps": 32},
+    {"tiles": 4, "spw": 1, "graph_steps": 8},
+    {"tiles": 2, "spw": 1, "graph_steps": 16},
+    {"tiles": 8, "spw": 1, "graph_steps": 16},
+    {"tiles": 4, "spw": 2, "graph_steps": 16},
+    {"tiles": 1, "spw": 1, "graph_steps": 16},
+]
+
+
+def cache_path():
+    return os.environ.get("MPRGP_TUNE_CACHE") or os.path.join(
+        os.path.expanduser("~"), ".cache", "mprgp_amd", "tune.json")
+
+
+def batch_key(qps, precision, device=0):
+    """Device, precision and the batch's size class (log2 of total slices and
+    of the QP count): batches in one class share a launch shape."""
+    slices = sum(-(-q.n // 3 // 64) for q in qps)
+    return f"dev{device}/{'mixed' if precision == M.MIXED else 'fp64'}/" \
+           f"s{int(math.log2(max(slices, 1)))}/q{int(math.log2(max(len(qps), 1)))}"
+
+
+def apply(p: M.Params, shape):
+    p.tiles = int(shape["tiles"])
+    p.graph_steps = int(shape["graph_steps"])
+    p.reserved_i[0] = int(shape["spw"])
+    return p
+
+
+def tune(ctx, qps, per_qp, candidates=None, repeats=1, log=None):
+    """Times every candidate shape on the batch (each QP restarted from its
+    warm start with reset_x) and returns (best_shape, [(shape, ms)])."""
+    candidates = candidates or DEFAULT_CANDIDATES
+    results = []
+    for shape in candidates:
+        ps = [apply(M.Params.from_buffer_copy(p), shape) for p in per_qp]
+        best = math.inf
+        for rep in range(repeats + 1):  # the first run captures the graphs
+            for q in qps:
+                q.reset_x()
+            t0 = time.perf_counter()
+            M.solve_batch(ctx, qps, per_qp=ps)
+            ms = 1e3 * (time.perf_counter() - t0)
+            if rep > 0:
+                best = min(best, ms)
+        results.append((shape, best))
+        if log:
+            log(f"tune {shape}: {best:.2f} ms")
+    for q in qps:
+        q.reset_x()
+    return min(results, key=lambda r: r[1])[0], results
+
+
+def tuned_params(ctx, qps, per_qp, precision, device=0, path=None, retune=False, log=None):
+    """per_qp with the cached (or freshly tuned) shape applied; returns
+    (params list, shape, from_cache)."""
+    path = path or cache_path()
+    key = batch_key(qps, precision, device)
+    cache = {}
+    if os.path.exists(path):
+        try:
+            with open(path) as f:
+                cache = json.load(f)
+        except (OSError, ValueError):
+            cache = {}
+    hit = key in cache and not retune
+    if hit:
+        shape = cache[key]["shape"]
+    else:
+        shape, results = tune(ctx, qps, per_qp, log=log)
+        cache[key] = {"shape": shape, "ms": [[s, round(ms, 3)] for s, ms in results]}
+        try:
+            os.makedirs(os.path.dirname(path), exist_ok=True)
+            with open(path + ".tmp", "w") as f:
+                json.dump(cache, f, indent=1)
+            os.replace(path + ".tmp", path)
+        except OSError:
+            pass  # read-only home: tuned for this process only
+    return [apply(M.Params.from_buffer_copy(p), shape) for p in per_qp], shape, hit
