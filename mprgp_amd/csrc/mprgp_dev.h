@@ -33,8 +33,10 @@ enum Op : int {
   OP_INIT,       // g = Ax - b, face(x), phi, z, p = z
   OP_TOP,        // phi, beta, |phi+beta|^2, beta.beta, phitphi
   OP_CG_AP,      // AP = A p, p.AP, z.g
-  OP_CG_PY,      // y = x - alpha p, py = P(y), feasibility of y
-  OP_CG_Y,       // A py (and A x at CG entry): objective values
+  OP_CG_PY,      // y = x - alpha p, py = P(y), feasibility of y (mixed: p.g,
+                 // and f(x) at CG entry)
+  OP_CG_Y,       // A py (and A x at CG entry): objective values; the mixed
+                 // form's inner solve skips it when y is feasible
   OP_CG_A,       // x = y, g -= alpha AP, phi, z, z.AP
   OP_CG_B,       // p = z - beta p, beta, |phi+beta|^2, beta.beta, phitphi
   OP_EXP1,       // y = P(x)
@@ -90,6 +92,7 @@ struct alignas(128) QpState {
   double alpha, beta_cg, pd, zg, residual, fx, fy;
   double alpha_f, nu, power_eps, lambda_max;
   double tol_outer, res64, inner_rel, best_res;
+  double pg, fx_entry;      // p.g of the CG step; f(x) at CG entry (mixed inner)
   int opcnt[OP_NUM];  // launches this QP spent in each op
 };
 

@@ -249,6 +249,29 @@ __device__ __forceinline__ bool inner_stalled(QpState &st) {
   return st.iteration - st.since_best > 128;
 }
 
+// CGMonotonicStep's acceptance test (MPRGPSolver.h:284-292) once f(P(y)) is
+// known: fnew = f(P(y)), fx_entry = f(x) when the CG loop is entered.
+__device__ __forceinline__ void cg_decide(QpState &st, double fnew, double fx_entry) {
+  if (st.cg_entry) {
+    st.fx = fx_entry;
+    st.fy = fnew;
+    st.prop = 1;
+    st.cg_entry = 0;
+  } else {
+    st.fx = st.fy;
+    st.fy = fnew;
+    st.num_cg++;
+    st.iteration++;
+  }
+  if (st.prop && st.residual > st.tol && st.fy <= st.fx && st.iteration < st.max_it) {
+    st.op = OP_CG_A;
+  } else if (st.residual <= st.tol) {
+    st.op = OP_FINAL;
+  } else {
+    st.op = (st.prop || !st.y_valid || !st.y_feas) ? OP_EXP1 : OP_PROP1;
+  }
+}
+
 __device__ __forceinline__ void finish_power(QpState &st) {
   st.lambda_max = st.nu;
   st.alpha_bar = 2.0 / st.nu;
@@ -327,30 +350,20 @@ __device__ void control(QpState &st, const Sums &S) {
     case OP_CG_PY:
       st.y_valid = 1;
       st.y_feas = S.v[0] == 0.0;
-      st.op = OP_CG_Y;
-      break;
-    case OP_CG_Y: {
-      double fnew = 0.5 * S.v[0] - S.v[1];
-      if (st.cg_entry) {
-        st.fx = 0.5 * S.v[2] - S.v[3];
-        st.fy = fnew;
-        st.prop = 1;
-        st.cg_entry = 0;
+      st.pg = S.v[1];
+      if (st.cg_entry) st.fx_entry = 0.5 * S.v[2];  // f(x) = x'(g - b)/2 with g = A x - b
+      if (st.mixed && st.y_feas) {
+        // mixed inner solve: P(y) = y, so f(P(y)) = f(x) - alpha p'g + alpha^2 p'Ap/2
+        // from the step's own sums; no A P(y) product
+        const double f0 = st.cg_entry ? st.fx_entry : st.fy;
+        cg_decide(st, f0 - st.alpha * st.pg + 0.5 * st.alpha * st.alpha * st.pd, st.fx_entry);
       } else {
-        st.fx = st.fy;
-        st.fy = fnew;
-        st.num_cg++;
-        st.iteration++;
-      }
-      if (st.prop && st.residual > st.tol && st.fy <= st.fx && st.iteration < st.max_it) {
-        st.op = OP_CG_A;
-      } else if (st.residual <= st.tol) {
-        st.op = OP_FINAL;
-      } else {
-        st.op = (st.prop || !st.y_valid || !st.y_feas) ? OP_EXP1 : OP_PROP1;
+        st.op = OP_CG_Y;
       }
       break;
-    }
+    case OP_CG_Y:
+      cg_decide(st, 0.5 * S.v[0] - S.v[1], 0.5 * S.v[2] - S.v[3]);
+      break;
     case OP_CG_A:
       st.beta_cg = S.v[0] / st.pd;
       st.op = OP_CG_B;
@@ -532,10 +545,16 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
     }
     case OP_CG_PY: {
       Row R = ldrow(q, v, r);
-      V3 y = axmy(ld3(v.x, r), st.alpha, ld3(v.p, r));
+      const V3 xr = ld3(v.x, r), pr = ld3(v.p, r);
+      V3 y = axmy(xr, st.alpha, pr);
       st3(v.y, r, y);
       st3(v.py, r, proj(R, y));
       P[0] += infeasible(R, y);
+      if (st.mixed) {  // sums for f(y) without A y (mixed inner solve)
+        const V3 gr = ld3(v.g, r);
+        P[1] += dot3(pr, gr);
+        if (st.cg_entry) P[2] += dot3(xr, sub(gr, ld3(v.b, r)));
+      }
       break;
     }
     case OP_CG_Y: {

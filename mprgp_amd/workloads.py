@@ -73,6 +73,8 @@ _STREAMS = {
 
 # FP32 streams of the mixed form's inner solve; everything else is FP64.
 MIXED_F32 = {"M", "p", "z", "ap"}
+# streams only the mixed form's inner solve adds (p.g for f(y) without A y)
+MIXED_EXTRA = {"CG_PY": ("g",)}
 
 
 def spmv_bytes(N, nb, s=8):
@@ -86,7 +88,7 @@ def op_bytes(op, N, nb, m, mixed=False, precond=True):
     FP32 matrix and FP32 direction vectors."""
     name = OPS[op]
     total = 0
-    for st in _STREAMS.get(name, ()):
+    for st in _STREAMS.get(name, ()) + (MIXED_EXTRA.get(name, ()) if mixed else ()):
         if st == "rows":
             total += N + 48 * m
         elif st == "cc":
