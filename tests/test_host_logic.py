@@ -1,0 +1,76 @@
+"""Host-side logic that needs no GPU: the op table shared by the device state
+machine and the byte accounting, the accounting itself, the serving
+workload's scenes, and the autotuner's cache keys."""
+import os
+import re
+
+import numpy as np
+
+import mprgp_amd as M
+from mprgp_amd import problems, serving, tune, workloads
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def device_ops():
+    text = open(os.path.join(ROOT, "mprgp_amd", "csrc", "mprgp_dev.h")).read()
+    body = text[text.index("enum Op : int {"):text.index("OP_NUM")]
+    return re.findall(r"\bOP_([A-Z0-9_]+)\b", body)
+
+
+def test_op_names_match_device_enum():
+    assert device_ops() == workloads.OPS
+
+
+def test_byte_accounting():
+    N, nb, m = 10000, 150000, 500
+    mat64 = nb * (9 * 8 + 4) + 4 * (N + 1)
+    mat32 = nb * (9 * 4 + 4) + 4 * (N + 1)
+    v64, v32 = 24 * N, 12 * N
+    op = workloads.OPS.index
+    # the power method's SpMV is FP64 in both forms: matrix + 2 vectors
+    for mixed in (False, True):
+        assert workloads.op_bytes(op("POW_MV"), N, nb, m, mixed) == mat64 + 2 * v64
+    # CG_AP: matrix + p (gathered once) + AP + z + g
+    assert workloads.op_bytes(op("CG_AP"), N, nb, m) == mat64 + 4 * v64
+    assert workloads.op_bytes(op("CG_AP"), N, nb, m, True) == mat32 + 3 * v32 + v64
+    # the mixed form's CG_PY also reads g
+    assert (workloads.op_bytes(op("CG_PY"), N, nb, m, True)
+            - workloads.op_bytes(op("CG_PY"), N, nb, m, False)) == v64 - v32  # p FP32, + g
+    # no preconditioner: no D^-1 stream
+    assert (workloads.op_bytes(op("CG_A"), N, nb, m) - workloads.op_bytes(op("CG_A"), N, nb, m, precond=False)
+            == v64)
+    counts = np.zeros(32, np.int32)
+    counts[op("CG_AP")] = 3
+    counts[op("TOP")] = 2
+    assert workloads.solve_bytes(counts, N, nb, m) == 3 * workloads.op_bytes(op("CG_AP"), N, nb, m) + \
+        2 * workloads.op_bytes(op("TOP"), N, nb, m)
+
+
+def test_serving_scenes_start_clear_of_their_planes():
+    rng = np.random.default_rng(5)
+    for i in range(12):
+        sc = serving.session_scene(rng, i, dof_range=(3e3, 2e4))
+        assert sc.material == M.COROTATIONAL
+        assert np.all(problems.signed_volumes(sc.X0, sc.tets) > 0)
+        for n, p, delta in sc.planes:
+            n = np.asarray(n) / np.linalg.norm(n)
+            s = sc.x_n @ n + p
+            assert s.min() > 0.0            # no node starts inside a plane
+            assert s.min() <= delta + 0.02  # ... and the body is close enough to land on it
+
+
+def test_tune_keys_and_shapes():
+    class Q:
+        def __init__(self, n):
+            self.n = n
+    a = [Q(3 * 1000)] * 10
+    b = [Q(3 * 1000)] * 10 + [Q(30)]
+    assert tune.batch_key(a, M.MIXED) == tune.batch_key(a, M.MIXED)
+    assert tune.batch_key(a, M.MIXED) != tune.batch_key(a, M.FP64)
+    assert tune.batch_key(a, M.MIXED).startswith("dev0/mixed/")
+    assert tune.batch_key(b, M.MIXED) == tune.batch_key(a, M.MIXED)  # same size class
+    p = tune.apply(M.params(), {"tiles": 8, "spw": 2, "graph_steps": 32})
+    assert (p.tiles, p.reserved_i[0], p.graph_steps) == (8, 2, 32)
+    for s in tune.DEFAULT_CANDIDATES:
+        assert s["tiles"] in (1, 2, 4, 8) and s["spw"] >= 1 and s["graph_steps"] > 0
