@@ -413,11 +413,11 @@ static int ensure_matrix32(mprgp_matrix *A, hipStream_t s) {
 static int ensure_qp32(mprgp_qp *q) {
   if (q->base32) return MPRGP_OK;
   const size_t n3 = 3 * (size_t)q->n;
-  const size_t bytes = (7 * n3 + (size_t)q->n) * sizeof(double) + (3 * n3 + 64) * sizeof(float);
+  const size_t bytes = (3 * n3 + (size_t)q->n) * sizeof(double) + (7 * n3 + 64) * sizeof(float);
   if (int rc = q->ctx->alloc(&q->base32, bytes)) return rc;
   q->bytes32 = bytes;
   double *dp = (double *)q->base32;
-  double **dvecs[] = {&q->fx, &q->fy, &q->fpy, &q->fb, &q->fg, &q->fphi, &q->fbeta};
+  double **dvecs[] = {&q->fx, &q->fy, &q->fpy};
   for (double **v : dvecs) {
     *v = dp;
     dp += n3;
@@ -425,7 +425,7 @@ static int ensure_qp32(mprgp_qp *q) {
   q->fcc = dp;
   dp += q->n;
   float *fp32 = (float *)dp;
-  float **fvecs[] = {&q->fp, &q->fz, &q->fap};
+  float **fvecs[] = {&q->fp, &q->fz, &q->fap, &q->fb, &q->fg, &q->fphi, &q->fbeta};
   for (float **v : fvecs) {
     *v = fp32;
     fp32 += n3;

@@ -98,19 +98,21 @@ struct alignas(128) QpState {
 
 // Storage of one precision form: FP64 for the plain solve and the outer
 // refinement; for the inner solve of the mixed form, the matrix values and
-// the direction vectors p, z, A p are FP32 (the matrix is ~85% of the
-// traffic).  Everything that carries the contact forces stays FP64: on an
-// active node the gradient's normal component is the multiplier, orders of
-// magnitude above the tangential residual, and any FP32 rounding of g, b, or
-// of the tangent-Jacobi operands (1/diag against the FP64 factor s) leaks
-// eps32*|lambda| into the tangential part, which either floors the KKT norm
-// or pushes CG steps through the planes.
+// the gradient- and direction-side vectors (b, g, phi, beta, p, z, A p) are
+// FP32.  The iterate and trial points (x, y, P(y)), the per-node rows and
+// 1/diag stay FP64: the active set is decided on x against the 1e-9 plane
+// tolerance, and the tangent-Jacobi factor s is computed from the FP64
+// 1/diag, so the preconditioner must use that same 1/diag or it leaves a
+// normal component eps32*|lambda| in z that pushes every CG step through the
+// planes.  FP32 g itself carries eps32*|lambda| into the tangential part: the
+// inner solve's KKT floor, far below the refinement's tolerance unless the
+// contact forces lambda reach tol/eps32.
 template <class T>
 struct Vecs {
   const T *val;               // SELL values
   const double *dinv;         // 1/diag(A)
   double *x, *y, *py;         // iterate and trial points
-  double *b, *g, *phi, *beta; // gradient side
+  T *b, *g, *phi, *beta;      // gradient side
   T *p, *z, *ap;              // search direction side
   const double *cn, *cj, *cs; // normal, |n|^2, tangent-Jacobi factor
   double *cc;                 // offset (mixed inner: shifted c - n.x)

@@ -147,7 +147,7 @@ struct mprgp_qp {
   void *base32 = nullptr;
   size_t bytes32 = 0;
   double *fx = nullptr, *fy = nullptr, *fpy = nullptr;  // FP64 iterate/trial points of the inner solve
-  double *fb = nullptr, *fg = nullptr, *fphi = nullptr, *fbeta = nullptr;  // FP64 gradient side
+  float *fb = nullptr, *fg = nullptr, *fphi = nullptr, *fbeta = nullptr;  // FP32 gradient side
   float *fp = nullptr, *fz = nullptr, *fap = nullptr;   // FP32 direction side
   double *fcc = nullptr;  // shifted offsets c - n.x of the correction QP
   std::atomic<int> busy{0};  // submitted to an engine and not yet completed

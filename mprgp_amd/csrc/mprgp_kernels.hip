@@ -641,7 +641,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
       // g = A x - b ; AD = A beta ; beta.AD ; g.beta  (MPRGPSolver.h:435-437, 224-231)
       V3 acc[2];
       const double *x = v.x;
-      const double *be = v.beta;
+      const T *be = v.beta;
       spmv_row<2>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[2]) {
         xs[0] = ld3(x, j);
         xs[1] = ld3(be, j);
@@ -656,7 +656,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
     }
     case OP_CL_PROP1: {
       V3 acc[1];
-      const double *be = v.beta;
+      const T *be = v.beta;
       spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(be, j); }, acc);
       st3(v.ap, r, acc[0]);
       V3 d = ld3(v.beta, r);
