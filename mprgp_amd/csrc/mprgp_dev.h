@@ -36,7 +36,8 @@ enum Op : int {
   OP_CG_PY,      // y = x - alpha p, py = P(y), feasibility of y (mixed: p.g,
                  // and f(x) at CG entry)
   OP_CG_Y,       // A py (and A x at CG entry): objective values; the mixed
-                 // form's inner solve skips it when y is feasible
+                 // form's inner solve skips it when y is feasible and otherwise
+                 // forms only d'A d over the rows P moved (d = P(y) - y)
   OP_CG_A,       // x = y, g -= alpha AP, phi, z, z.AP
   OP_CG_B,       // p = z - beta p, beta, |phi+beta|^2, beta.beta, phitphi
   OP_EXP1,       // y = P(x)
@@ -93,6 +94,7 @@ struct alignas(128) QpState {
   double alpha_f, nu, power_eps, lambda_max;
   double tol_outer, res64, inner_rel, best_res;
   double pg, fx_entry;      // p.g of the CG step; f(x) at CG entry (mixed inner)
+  double f_trial;           // f(y) of the CG trial point (mixed inner)
   int opcnt[OP_NUM];  // launches this QP spent in each op
 };
 

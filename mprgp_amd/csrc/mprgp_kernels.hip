@@ -141,10 +141,9 @@ __device__ __forceinline__ V3 precond_of(const Vecs<T> &v, bool on, const Row &R
 // matrix values and gathers of U blocks are all in flight together.  U is
 // kept small enough (2 blocks for one vector, 1 for two) that the step kernel
 // needs no scratch: spilled registers would go to HBM on every launch.
-template <int NV, class T, class G>
+template <int NV, int U = (NV == 1 ? 2 : 1), class T, class G>
 __device__ __forceinline__ void spmv_row(const int *sl_off, const int *col, const T *val, int s, int lane,
                                          G gather, V3 (&acc)[NV]) {
-  constexpr int U = NV == 1 ? 2 : 1;
 #pragma unroll
   for (int t = 0; t < NV; ++t) acc[t] = V3{0.0, 0.0, 0.0};
   const int b0 = sl_off[s];
@@ -352,17 +351,20 @@ __device__ void control(QpState &st, const Sums &S) {
       st.y_feas = S.v[0] == 0.0;
       st.pg = S.v[1];
       if (st.cg_entry) st.fx_entry = 0.5 * S.v[2];  // f(x) = x'(g - b)/2 with g = A x - b
-      if (st.mixed && st.y_feas) {
-        // mixed inner solve: P(y) = y, so f(P(y)) = f(x) - alpha p'g + alpha^2 p'Ap/2
-        // from the step's own sums; no A P(y) product
+      if (st.mixed) {
+        // mixed inner solve: f(y) from the step's own sums; when P(y) = y that
+        // is f(P(y)), otherwise CG_Y adds d'A d/2 over the projected rows
         const double f0 = st.cg_entry ? st.fx_entry : st.fy;
-        cg_decide(st, f0 - st.alpha * st.pg + 0.5 * st.alpha * st.alpha * st.pd, st.fx_entry);
+        st.f_trial = f0 - st.alpha * st.pg + 0.5 * st.alpha * st.alpha * st.pd;
+        if (st.y_feas) cg_decide(st, st.f_trial, st.fx_entry);
+        else st.op = OP_CG_Y;
       } else {
         st.op = OP_CG_Y;
       }
       break;
     case OP_CG_Y:
-      cg_decide(st, 0.5 * S.v[0] - S.v[1], 0.5 * S.v[2] - S.v[3]);
+      if (st.mixed) cg_decide(st, st.f_trial + S.v[1] + 0.5 * S.v[0], st.fx_entry);
+      else cg_decide(st, 0.5 * S.v[0] - S.v[1], 0.5 * S.v[2] - S.v[3]);
       break;
     case OP_CG_A:
       st.beta_cg = S.v[0] / st.pd;
@@ -476,9 +478,12 @@ __device__ void control(QpState &st, const Sums &S) {
 
 // One row (= node r of slice s) of a solve op on storage T; partial sums
 // accumulate into P (FP64).
-template <class T>
+// MX: the kernel instance may hold mixed-form QPs (otherwise their branches
+// are compiled out).
+template <class T, bool MX>
 __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s, int lane, int r, int op,
                                          const QpState &st, double (&P)[kNumPart]) {
+  const bool mixed = MX && st.mixed;
   const bool pre = st.precond && st.has_rows;
   switch (op) {
     case OP_INIT:
@@ -547,10 +552,11 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
       Row R = ldrow(q, v, r);
       const V3 xr = ld3(v.x, r), pr = ld3(v.p, r);
       V3 y = axmy(xr, st.alpha, pr);
+      const V3 pyr = proj(R, y);
       st3(v.y, r, y);
-      st3(v.py, r, proj(R, y));
+      st3(v.py, r, pyr);
       P[0] += infeasible(R, y);
-      if (st.mixed) {  // sums for f(y) without A y (mixed inner solve)
+      if (mixed) {  // mixed inner solve: sums for f(y) = f(x) - alpha p'g + alpha^2 p'Ap/2
         const V3 gr = ld3(v.g, r);
         P[1] += dot3(pr, gr);
         if (st.cg_entry) P[2] += dot3(xr, sub(gr, ld3(v.b, r)));
@@ -558,6 +564,21 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
       break;
     }
     case OP_CG_Y: {
+      if (mixed) {
+        // mixed inner solve: with d = P(y) - y, nonzero only on the rows the
+        // projection moved, f(P(y)) = f(y) + d'(A y - b) + d'A d/2 where
+        // A y - b = g - alpha A p; every other row reads no matrix at all
+        const V3 dr = sub(ld3(v.py, r), ld3(v.y, r));
+        if (dr.x != 0.0 || dr.y != 0.0 || dr.z != 0.0) {
+          P[1] += dot3(dr, axmy(ld3(v.g, r), st.alpha, ld3(v.ap, r)));
+          const double *py = v.py, *yy = v.y;
+          V3 acc[1];
+          spmv_row<1, 1>(q.sl_off, q.col, v.val, s, lane,
+                         [&](int j, V3 (&xs)[1]) { xs[0] = sub(ld3(py, j), ld3(yy, j)); }, acc);
+          P[0] += dot3(dr, acc[0]);
+        }
+        break;
+      }
       const double *py = v.py;
       V3 pyr = ld3(v.py, r);
       V3 br = ld3(v.b, r);
@@ -812,9 +833,9 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
     if (f64) {
       row_work_fp64(q, s, lane, r, op, st, P);
     } else if (MX && st.mixed) {
-      row_work<float>(q, q.f, s, lane, r, op, st, P);
+      row_work<float, MX>(q, q.f, s, lane, r, op, st, P);
     } else {
-      row_work<double>(q, q.d, s, lane, r, op, st, P);
+      row_work<double, MX>(q, q.d, s, lane, r, op, st, P);
     }
   }
 }

@@ -75,6 +75,10 @@ _STREAMS = {
 MIXED_F32 = {"M", "p", "z", "ap"}
 # streams only the mixed form's inner solve adds (p.g for f(y) without A y)
 MIXED_EXTRA = {"CG_PY": ("g",)}
+# ops the mixed form's inner solve runs differently: CG_Y reads P(y) and y of
+# every row and the matrix only on the few rows the projection moved (not
+# counted: a lower bound)
+MIXED_STREAMS = {"CG_Y": ("py", "y")}
 
 
 def spmv_bytes(N, nb, s=8):
@@ -88,7 +92,8 @@ def op_bytes(op, N, nb, m, mixed=False, precond=True):
     FP32 matrix and FP32 direction vectors."""
     name = OPS[op]
     total = 0
-    for st in _STREAMS.get(name, ()) + (MIXED_EXTRA.get(name, ()) if mixed else ()):
+    streams = MIXED_STREAMS.get(name, _STREAMS.get(name, ())) if mixed else _STREAMS.get(name, ())
+    for st in streams + (MIXED_EXTRA.get(name, ()) if mixed else ()):
         if st == "rows":
             total += N + 48 * m
         elif st == "cc":
