@@ -38,7 +38,7 @@ enum Op : int {
   OP_CG_Y,       // A py (and A x at CG entry): objective values; the mixed
                  // form's inner solve skips it when y is feasible and otherwise
                  // forms only d'A d over the rows P moved (d = P(y) - y)
-  OP_CG_A,       // x = y, g -= alpha AP, phi, z, z.AP
+  OP_CG_A,       // (x = y by buffer swap) g -= alpha AP, phi, z, z.AP
   OP_CG_B,       // p = z - beta p, beta, |phi+beta|^2, beta.beta, phitphi
   OP_EXP1,       // y = P(x)
   OP_EXP2,       // g = A y - b, face(y), phi, y' = y - abar phi, x = P(y')
@@ -89,6 +89,7 @@ struct alignas(128) QpState {
   int iter_total;           // mixed: inner iterations over all sweeps
   int num_cg_t, num_exp_t, num_prop_t;
   int since_best;           // mixed inner: iterations without halving the residual
+  int xb;                   // 1: x lives in the y buffer and y in the x buffer
   double tol, alpha_bar, gamma;
   double alpha, beta_cg, pd, zg, residual, fx, fy;
   double alpha_f, nu, power_eps, lambda_max;

@@ -263,6 +263,7 @@ __device__ __forceinline__ void cg_decide(QpState &st, double fnew, double fx_en
     st.iteration++;
   }
   if (st.prop && st.residual > st.tol && st.fy <= st.fx && st.iteration < st.max_it) {
+    st.xb ^= 1;  // x = y: the trial point's buffer becomes the iterate's
     st.op = OP_CG_A;
   } else if (st.residual <= st.tol) {
     st.op = OP_FINAL;
@@ -451,6 +452,7 @@ __device__ void control(QpState &st, const Sums &S) {
       } else {
         double t = st.inner_rel * st.res64;
         st.tol = t > 0.5 * st.tol_outer ? t : 0.5 * st.tol_outer;
+        st.xb = 0;  // the correction d = 0 was written to the first buffer
         st.op = OP_INIT;
       }
       break;
@@ -484,6 +486,10 @@ template <class T, bool MX>
 __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s, int lane, int r, int op,
                                          const QpState &st, double (&P)[kNumPart]) {
   const bool mixed = MX && st.mixed;
+  // x and y swap buffers when a CG step is accepted (x = y without a copy):
+  // st.xb names the buffer that holds x
+  double *const xc = st.xb ? v.y : v.x;
+  double *const yc = st.xb ? v.x : v.y;
   const bool pre = st.precond && st.has_rows;
   switch (op) {
     case OP_INIT:
@@ -491,11 +497,11 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
     case OP_CL_EXP2: {
       // g = A x - b ; DECIDE_FACE(x) ; phi ; z ; p = z   (MPRGPSolver.h:148-156, 263-268, 210-215)
       V3 acc[1];
-      const double *x = v.x;
+      const double *x = xc;
       spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
       V3 g = sub(acc[0], ld3(v.b, r));
       Row R = ldrow(q, v, r);
-      bool f = face_of(R, ld3(v.x, r));
+      bool f = face_of(R, ld3(xc, r));
       V3 ph = phi_of(R, f, g);
       V3 z = precond_of(v, pre, R, f, r, g, ph);
       st3(v.g, r, g);
@@ -503,7 +509,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
       st3(v.z, r, z);
       st3(v.p, r, z);
       q.face[r] = f;
-      if (op == OP_EXP3) st3(v.y, r, ld3(v.py, r));
+      if (op == OP_EXP3) st3(yc, r, ld3(v.py, r));
       break;
     }
     case OP_TOP: {
@@ -521,7 +527,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
       V3 gp = add(ph, be);
       P[0] += dot3(gp, gp);
       P[1] += dot3(be, be);
-      V3 x = ld3(v.x, r);
+      V3 x = ld3(xc, r);
       V3 px = proj(R, axmy(x, st.alpha_bar, ph));
       P[2] += dot3(sub(x, px), ph);
       break;
@@ -540,7 +546,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
         Row R = ldrow(q, v, r);
         double lim = kDblMax;
         if (R.has) {
-          V3 xr = ld3(v.x, r);
+          V3 xr = ld3(xc, r);
           double jx = dot3(R.n, xr), jd = dot3(R.n, pr);
           if (jd > kEps && jx > R.c) lim = (jx - R.c) / jd;
         }
@@ -550,10 +556,10 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
     }
     case OP_CG_PY: {
       Row R = ldrow(q, v, r);
-      const V3 xr = ld3(v.x, r), pr = ld3(v.p, r);
+      const V3 xr = ld3(xc, r), pr = ld3(v.p, r);
       V3 y = axmy(xr, st.alpha, pr);
       const V3 pyr = proj(R, y);
-      st3(v.y, r, y);
+      st3(yc, r, y);
       st3(v.py, r, pyr);
       P[0] += infeasible(R, y);
       if (mixed) {  // mixed inner solve: sums for f(y) = f(x) - alpha p'g + alpha^2 p'Ap/2
@@ -568,10 +574,10 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
         // mixed inner solve: with d = P(y) - y, nonzero only on the rows the
         // projection moved, f(P(y)) = f(y) + d'(A y - b) + d'A d/2 where
         // A y - b = g - alpha A p; every other row reads no matrix at all
-        const V3 dr = sub(ld3(v.py, r), ld3(v.y, r));
+        const V3 dr = sub(ld3(v.py, r), ld3(yc, r));
         if (dr.x != 0.0 || dr.y != 0.0 || dr.z != 0.0) {
           P[1] += dot3(dr, axmy(ld3(v.g, r), st.alpha, ld3(v.ap, r)));
-          const double *py = v.py, *yy = v.y;
+          const double *py = v.py, *yy = yc;
           V3 acc[1];
           spmv_row<1, 1>(q.sl_off, q.col, v.val, s, lane,
                          [&](int j, V3 (&xs)[1]) { xs[0] = sub(ld3(py, j), ld3(yy, j)); }, acc);
@@ -583,13 +589,13 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
       V3 pyr = ld3(v.py, r);
       V3 br = ld3(v.b, r);
       if (st.cg_entry) {
-        const double *x = v.x;
+        const double *x = xc;
         V3 acc[2];
         spmv_row<2>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[2]) {
           xs[0] = ld3(py, j);
           xs[1] = ld3(x, j);
         }, acc);
-        V3 xr = ld3(v.x, r);
+        V3 xr = ld3(xc, r);
         P[0] += dot3(pyr, acc[0]);
         P[1] += dot3(pyr, br);
         P[2] += dot3(xr, acc[1]);
@@ -606,8 +612,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
       // CGMonotonicStep body, first half (MPRGPSolver.h:293-297)
       Row R = ldrow(q, v, r);
       bool f = q.face[r];
-      V3 ap = ld3(v.ap, r);
-      st3(v.x, r, ld3(v.y, r));
+      V3 ap = ld3(v.ap, r);  // x = y happened in the control (buffer swap)
       V3 g = axmy(ld3(v.g, r), st.alpha, ap);
       V3 ph = phi_of(R, f, g);
       V3 z = precond_of(v, pre, R, f, r, g, ph);
@@ -629,31 +634,31 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
       V3 gp = add(ph, be);
       P[0] += dot3(gp, gp);
       P[1] += dot3(be, be);
-      V3 x = ld3(v.x, r);
+      V3 x = ld3(xc, r);
       V3 px = proj(R, axmy(x, st.alpha_bar, ph));
       P[2] += dot3(sub(x, px), ph);
       break;
     }
     case OP_EXP1: {
       Row R = ldrow(q, v, r);
-      st3(v.y, r, proj(R, ld3(v.x, r)));
+      st3(yc, r, proj(R, ld3(xc, r)));
       break;
     }
     case OP_EXP2: {
       // ExpMonotonicStep, first half (MPRGPSolver.h:255-261): the new y goes
       // to py because neighbours still gather y in this launch.
       V3 acc[1];
-      const double *y = v.y;
+      const double *y = yc;
       spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(y, j); }, acc);
       V3 g = sub(acc[0], ld3(v.b, r));
       Row R = ldrow(q, v, r);
-      V3 yr = ld3(v.y, r);
+      V3 yr = ld3(yc, r);
       bool f = face_of(R, yr);
       V3 ph = phi_of(R, f, g);
       V3 yn = axmy(yr, st.alpha_bar, ph);
       st3(v.g, r, g);
       st3(v.py, r, yn);
-      st3(v.x, r, proj(R, yn));
+      st3(xc, r, proj(R, yn));
       q.face[r] = f;
       P[0] += infeasible(R, yn);
       break;
@@ -661,7 +666,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
     case OP_PROP1: {
       // g = A x - b ; AD = A beta ; beta.AD ; g.beta  (MPRGPSolver.h:435-437, 224-231)
       V3 acc[2];
-      const double *x = v.x;
+      const double *x = xc;
       const T *be = v.beta;
       spmv_row<2>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[2]) {
         xs[0] = ld3(x, j);
@@ -688,12 +693,12 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
     case OP_PROP2: {
       // PropStep tail (MPRGPSolver.h:232-241)
       Row R = ldrow(q, v, r);
-      V3 xn = proj(R, axmy(ld3(v.x, r), st.alpha, ld3(v.beta, r)));
+      V3 xn = proj(R, axmy(ld3(xc, r), st.alpha, ld3(v.beta, r)));
       V3 g = axmy(ld3(v.g, r), st.alpha, ld3(v.ap, r));
       bool f = face_of(R, xn);
       V3 ph = phi_of(R, f, g);
       V3 z = precond_of(v, pre, R, f, r, g, ph);
-      st3(v.x, r, xn);
+      st3(xc, r, xn);
       st3(v.g, r, g);
       st3(v.phi, r, ph);
       st3(v.z, r, z);
@@ -706,7 +711,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
       Row R = ldrow(q, v, r);
       bool f = q.face[r];
       V3 ap = ld3(v.ap, r);
-      st3(v.x, r, axmy(ld3(v.x, r), st.alpha, ld3(v.p, r)));
+      st3(xc, r, axmy(ld3(xc, r), st.alpha, ld3(v.p, r)));
       V3 g = axmy(ld3(v.g, r), st.alpha, ap);
       V3 ph = phi_of(R, f, g);
       V3 z = precond_of(v, pre, R, f, r, g, ph);
@@ -719,21 +724,21 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
     case OP_CL_EXP: {
       // ExpStep first half (MPRGPSolver.h:202-208)
       Row R = ldrow(q, v, r);
-      V3 xt = axmy(ld3(v.x, r), st.alpha_f, ld3(v.p, r));
+      V3 xt = axmy(ld3(xc, r), st.alpha_f, ld3(v.p, r));
       V3 g = axmy(ld3(v.g, r), st.alpha_f, ld3(v.ap, r));
       bool f = face_of(R, xt);
       V3 ph = phi_of(R, f, g);
       xt = axmy(xt, st.alpha_bar, ph);
-      st3(v.x, r, proj(R, xt));
+      st3(xc, r, proj(R, xt));
       st3(v.g, r, g);
       q.face[r] = f;
       break;
     }
     case OP_FINAL: {
-      if (st.variant != VAR_CLASSIC) {
-        Row R = ldrow(q, v, r);
-        st3(v.x, r, proj(R, ld3(v.x, r)));
-      }
+      // the result goes to the first buffer, where the caller reads x
+      V3 xr = ld3(xc, r);
+      if (st.variant != VAR_CLASSIC) xr = proj(ldrow(q, v, r), xr);
+      st3(v.x, r, xr);
       P[0] += q.face[r] ? 1.0 : 0.0;
       break;
     }

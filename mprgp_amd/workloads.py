@@ -56,7 +56,7 @@ _STREAMS = {
     "CL_AP": ("M", "p", "ap", "z", "g", "rows", "x"),
     "CG_PY": ("rows", "x", "p", "y", "py"),
     "CG_Y": ("M", "py", "b"),
-    "CG_A": ("rows", "face", "ap", "y", "g", "dinv", "x", "g", "phi", "z"),
+    "CG_A": ("rows", "face", "ap", "g", "dinv", "g", "phi", "z"),
     "CG_B": ("rows", "face", "z", "p", "g", "phi", "x", "p", "beta"),
     "EXP1": ("rows", "x", "y"),
     "EXP2": ("M", "y", "b", "rows", "g", "py", "x", "face"),
