@@ -21,7 +21,7 @@
 namespace mprgp {
 
 constexpr int kSlice = 64;       // rows per SELL slice = lanes per wavefront
-constexpr int kNumPart = 8;      // partial sums per tile
+constexpr int kNumPart = 4;      // partial sums per tile (most any op produces)
 constexpr double kEps = 1e-9;    // ScalarUtil<double>::scalar_eps of the simulator
 
 enum Op : int {
@@ -136,7 +136,7 @@ struct QpDev {
   QpState *st;          // st[0], st[1] (launch parity)
   int *done_flag;       // receives `tag` once the QP reaches DONE
   int tag;              // nonzero; distinguishes successive occupants of a batch slot
-  double *part;         // [2][n_tiles][kNumPart] partial sums (launch parity)
+  double *part;         // [2][kNumPart][n_tiles] partial sums (launch parity, slot-major)
 };
 
 // One work item of a launch: up to `slices` consecutive slices of one QP.

@@ -213,8 +213,24 @@ __device__ __forceinline__ double wave_min(double v) {
   return v;
 }
 
-// Partial slot 7 reduces with min (max feasible step); slots 0..6 with sum.
-constexpr int kMinSlot = 7;
+// Partial sums an op produces: slots 0..n-1 reduce with sum.  The classic
+// step-length search (CL_AP) also reduces slot 3 with min.
+constexpr int kMinSlot = 3;
+__device__ __forceinline__ int sum_slots(int op) {
+  switch (op) {
+    case OP_TOP: case OP_CG_PY: case OP_CG_B:
+      return 3;
+    case OP_CG_Y:
+      return 4;
+    case OP_CG_AP: case OP_CL_AP: case OP_PROP1: case OP_CL_PROP1:
+      return 2;
+    case OP_POW_MV: case OP_POW_NRM: case OP_CG_A: case OP_EXP2: case OP_FINAL: case OP_CL_CG:
+    case OP_R_START: case OP_R_FINAL:
+      return 1;
+    default:
+      return 0;
+  }
+}
 
 // ------------------------------------------------------------- control
 
@@ -869,26 +885,33 @@ __global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(
     //    (state words and partials are loaded together by wave 0)
     constexpr int kWords = sizeof(QpState) / sizeof(int);
     if (wave == 0) {
+      // partials are slot-major ([parity][slot][tile]): every slot's values
+      // are contiguous, and only the slots the previous op wrote are reduced
       const int nt = q.n_tiles;
-      const double *pp = q.part + (size_t)(parity ^ 1) * nt * kNumPart;
+      const double *pp = q.part + (size_t)(parity ^ 1) * kNumPart * nt;
       double acc[kNumPart];
+      double mn = kDblMax;
 #pragma unroll
-      for (int j = 0; j < kNumPart; ++j) acc[j] = (j == kMinSlot) ? kDblMax : 0.0;
+      for (int j = 0; j < kNumPart; ++j) acc[j] = 0.0;
       for (int k = lane; k < nt; k += kSlice) {
 #pragma unroll
         for (int j = 0; j < kNumPart; ++j) {
-          const double v = pp[(size_t)k * kNumPart + j];
-          acc[j] = (j == kMinSlot) ? fmin(acc[j], v) : acc[j] + v;
+          const double v = pp[(size_t)j * nt + k];
+          acc[j] += v;
+          if (j == kMinSlot) mn = fmin(mn, v);
         }
       }
       const int *src = reinterpret_cast<const int *>(q.st + (parity ^ 1));
       int *dst = reinterpret_cast<int *>(&sst);
       for (int i = lane; i < kWords; i += kSlice) dst[i] = src[i];
-      Sums S;
-#pragma unroll
-      for (int j = 0; j < kNumPart; ++j) S.v[j] = (j == kMinSlot) ? wave_min(acc[j]) : wave_sum(acc[j]);
       __builtin_amdgcn_s_waitcnt(0);
       __builtin_amdgcn_wave_barrier();
+      const int prev = sst.op;  // the op whose partials these are (same in every lane)
+      const int ns = sum_slots(prev);
+      Sums S;
+#pragma unroll
+      for (int j = 0; j < kNumPart; ++j) S.v[j] = (j < ns) ? wave_sum(acc[j]) : 0.0;
+      if (prev == OP_CL_AP) S.v[kMinSlot] = wave_min(mn);
       if (lane == 0) {
         const bool skip = sst.fresh || sst.op == OP_DONE;
         prev_done = sst.op == OP_DONE;
@@ -900,25 +923,30 @@ __global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(
     const QpState &st = sst;
     const int op = st.op;
     if (op != OP_DONE) {
+      const bool has_min = op == OP_CL_AP;
       double P[kNumPart];
 #pragma unroll
       for (int j = 0; j < kNumPart; ++j) P[j] = 0.0;
-      P[kMinSlot] = kDblMax;
+      if (has_min) P[kMinSlot] = kDblMax;
       tile_work<TS, MX>(q, t, op, st, P);
-      // wave then block reduction in a fixed order
+      // wave then block reduction in a fixed order, over the op's slots only
+      const int ns = has_min ? kNumPart : sum_slots(op);
+      if (ns > 0) {
 #pragma unroll
-      for (int j = 0; j < kMinSlot; ++j) P[j] = wave_sum(P[j]);
-      P[kMinSlot] = wave_min(P[kMinSlot]);
-      if (lane == 0) {
+        for (int j = 0; j < kNumPart; ++j)
+          if (j < ns) P[j] = (has_min && j == kMinSlot) ? wave_min(P[j]) : wave_sum(P[j]);
+        if (lane == 0) {
 #pragma unroll
-        for (int j = 0; j < kNumPart; ++j) red[wave][j] = P[j];
-      }
-      __syncthreads();
-      if (wave == 0 && lane < kNumPart) {
-        const int j = lane;
-        double acc = red[0][j];
-        for (int w = 1; w < TS; ++w) acc = (j == kMinSlot) ? fmin(acc, red[w][j]) : acc + red[w][j];
-        q.part[((size_t)parity * st.n_tiles + t.local) * kNumPart + j] = acc;
+          for (int j = 0; j < kNumPart; ++j) red[wave][j] = P[j];
+        }
+        __syncthreads();
+        if (wave == 0 && lane < ns) {
+          const int j = lane;
+          const bool m = has_min && j == kMinSlot;
+          double acc = red[0][j];
+          for (int w = 1; w < TS; ++w) acc = m ? fmin(acc, red[w][j]) : acc + red[w][j];
+          q.part[((size_t)parity * kNumPart + j) * st.n_tiles + t.local] = acc;
+        }
       }
     }
     // 2. the QP's first tile persists the state this launch executes
