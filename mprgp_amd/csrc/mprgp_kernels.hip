@@ -874,7 +874,6 @@ __global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(
                                                            int *running, int parity) {
   __shared__ double red[TS][kNumPart];
   __shared__ QpState sst;
-  __shared__ int prev_done;
   const int n_tiles = *n_tiles_dev;
   const int wave = threadIdx.x / kSlice;
   const int lane = threadIdx.x % kSlice;
@@ -914,9 +913,18 @@ __global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(
       if (prev == OP_CL_AP) S.v[kMinSlot] = wave_min(mn);
       if (lane == 0) {
         const bool skip = sst.fresh || sst.op == OP_DONE;
-        prev_done = sst.op == OP_DONE;
+        const bool was_done = sst.op == OP_DONE;
         if (!skip) control(sst, S);
         sst.fresh = 0;
+        if (t.local == 0) {  // the QP's first tile counts the launch (or its end)
+          if (sst.op != OP_DONE) {
+            sst.steps++;
+            sst.opcnt[sst.op]++;
+          } else if (!was_done) {
+            atomicSub(running, 1);
+            *q.done_flag = q.tag;
+          }
+        }
       }
     }
     __syncthreads();
@@ -949,26 +957,14 @@ __global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(
         }
       }
     }
-    // 2. the QP's first tile persists the state this launch executes
-    __syncthreads();
-    if (t.local == 0) {
-      if (threadIdx.x == 0) {
-        if (op != OP_DONE) {
-          sst.steps++;
-          sst.opcnt[op]++;
-        } else if (!prev_done) {
-          atomicSub(running, 1);
-          *q.done_flag = q.tag;
-        }
-      }
-      __syncthreads();
-      if (wave == 0) {
-        const int *src = reinterpret_cast<const int *>(&sst);
-        int *dst = reinterpret_cast<int *>(q.st + parity);
-        for (int i = lane; i < kWords; i += kSlice) dst[i] = src[i];
-      }
+    // 2. the QP's first tile persists the state this launch executes (wave 0
+    //    wrote it in the prologue; the other waves only read it)
+    if (t.local == 0 && wave == 0) {
+      const int *src = reinterpret_cast<const int *>(&sst);
+      int *dst = reinterpret_cast<int *>(q.st + parity);
+      for (int i = lane; i < kWords; i += kSlice) dst[i] = src[i];
     }
-    __syncthreads();
+    __syncthreads();  // before the next tile of this workgroup reuses sst
   }
 }
 
