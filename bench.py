@@ -132,7 +132,6 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serving-seconds", type=float, default=3.0)
     ap.add_argument("--ts", type=int, default=0, help="wavefronts per workgroup (0: library choice)")
-    ap.add_argument("--spw", type=int, default=0, help="slices per wavefront per tile (0: library choice)")
     ap.add_argument("--graph-steps", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true", help="library default shape instead of the autotuner's")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM counter passes")
@@ -176,15 +175,12 @@ def main():
     mixed = prec == M.MIXED
 
     def batch_params(precision):
-        per = [M.params(tol=args.tol, max_it=4000, alpha_bar=q.alpha_bar, project_first=1,
-                        precision=precision, tiles=args.ts, graph_steps=args.graph_steps) for q in qps]
-        for p in per:
-            p.reserved_i[0] = args.spw
-        return per
+        return [M.params(tol=args.tol, max_it=4000, alpha_bar=q.alpha_bar, project_first=1,
+                         precision=precision, tiles=args.ts, graph_steps=args.graph_steps) for q in qps]
 
     per = batch_params(prec)
     shape = None
-    if not args.no_tune and not (args.ts or args.spw or args.graph_steps):
+    if not args.no_tune and not (args.ts or args.graph_steps):
         from mprgp_amd import tune
         t1 = time.perf_counter()
         per, shape, hit = tune.tuned_params(ctx, [q.qp for q in qps], per, prec, log=log)

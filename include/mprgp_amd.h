@@ -79,7 +79,7 @@ typedef struct {
   int tiles;           /* SELL slices per workgroup (1,2,4,8); 0 = autotuned         */
   int graph_steps;     /* launches per captured hipGraph; 0 = autotuned              */
   int max_refine;      /* MIXED: outer refinement sweeps (0 = default 8)             */
-  int reserved_i[5];   /* [0]: SELL slices per wavefront per tile (0 = autotuned)    */
+  int reserved_i[5];   /* set to 0                                                   */
   double inner_tol;    /* MIXED: inner tolerance factor (0 = default 1e-3 relative)  */
   double reserved_d[3];
 } mprgp_params;

@@ -800,19 +800,24 @@ int pick_ts(const mprgp_params &p, long total_slices) {
   return 4;
 }
 
-// Tiles of one QP (slot `slot` of the batch table): `span` = TS*spw slices each.
-void append_tiles(int slot, int n_slices, int span, std::vector<Tile> &tiles) {
-  int local = 0;
-  for (int s0 = 0; s0 < n_slices; s0 += span, ++local)
-    tiles.push_back(Tile{slot, s0, std::min(span, n_slices - s0), local});
-}
-
 int tiles_for(int n_slices, int span) { return (n_slices + span - 1) / span; }
 
+// Tiles of one QP (slot `slot` of the batch table): `span` = TS slices each,
+// one per wavefront.
+void append_tiles(int slot, const mprgp_qp *q, int span, std::vector<Tile> &tiles) {
+  const int n_slices = q->A->n_slices;
+  const int nt = tiles_for(n_slices, span);
+  int local = 0;
+  for (int s0 = 0; s0 < n_slices; s0 += span, ++local)
+    tiles.push_back(Tile{q->st, q->part, slot, s0, std::min(span, n_slices - s0), local, nt, 0});
+}
+
+// Workgroups per launch: one per tile, rounded up to a power of two so that
+// a captured graph serves every tile count up to its grid.
 int round_grid(int n_tiles) {
   int g = 64;
-  while (g < n_tiles && g < 8192) g *= 2;
-  return std::min(g, 8192);
+  while (g < n_tiles) g *= 2;
+  return g;
 }
 
 int check_batch_qp(const mprgp_ctx *ctx, const mprgp_qp *q, const mprgp_params &p, int i) {
@@ -860,7 +865,6 @@ int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, i
   std::memset(&d, 0, sizeof(d));
   d.n_nodes = q->n;
   d.n_slices = q->A->n_slices;
-  d.n_tiles = n_tiles;
   d.done_flag = done_flag;
   d.tag = tag;
   d.sl_off = q->A->d_sl_off;
@@ -872,8 +876,6 @@ int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, i
   d.d.cn = q->cn; d.d.cc = q->cc; d.d.cj = q->cj; d.d.cs = q->cs;
   d.ev = q->ev; d.pv = q->pv; d.pw = q->pw;
   d.cf = q->cf; d.face = q->face;
-  d.st = q->st;
-  d.part = q->part;
   const bool mixed = p.precision == MPRGP_MIXED;
   if (mixed) {
     if (int rc = ensure_matrix32(q->A, s)) return rc;
@@ -986,12 +988,9 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
     max_steps = std::max(max_steps, solve_step_bound(p));
   }
   const int ts = pick_ts(params[0], total_slices);
-  // one slice per wavefront measured fastest on the 256-QP batch (more slices
-  // per wave serialise the SpMV latency chains); callers may override
-  const int spw = params[0].reserved_i[0] > 0 ? params[0].reserved_i[0] : 1;
-  const int span = ts * spw;
+  const int span = ts;
   std::vector<Tile> tiles;
-  for (int i = 0; i < count; ++i) append_tiles(i, n_slices[i], span, tiles);
+  for (int i = 0; i < count; ++i) append_tiles(i, qps[i], span, tiles);
   BatchTables &tab = ctx->tab;
   if (int rc = tab.ensure(count, (int)tiles.size())) return rc;
   hipStream_t s = ctx->stream;
@@ -1064,7 +1063,7 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
       if (live > 0 && live < in_table) {
         uploads.emplace_back();
         for (int i = 0; i < count; ++i)
-          if (keep[i]) append_tiles(i, n_slices[i], span, uploads.back());
+          if (keep[i]) append_tiles(i, qps[i], span, uploads.back());
         const int nt = (int)uploads.back().size();
         MPRGP_HIP(hipMemcpyAsync(tab.d_tiles, uploads.back().data(), sizeof(Tile) * nt, hipMemcpyHostToDevice, s));
         MPRGP_HIP(hipMemcpyAsync(tab.d_ctrl, &nt, sizeof(int), hipMemcpyHostToDevice, s));

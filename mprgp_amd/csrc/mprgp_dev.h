@@ -98,6 +98,8 @@ struct alignas(128) QpState {
   double f_trial;           // f(y) of the CG trial point (mixed inner)
   int opcnt[OP_NUM];  // launches this QP spent in each op
 };
+// The launch prologue moves a state copy as one 8-byte word per lane.
+static_assert(sizeof(QpState) == 8 * kSlice, "QpState must stay 512 bytes");
 
 // Storage of one precision form: FP64 for the plain solve and the outer
 // refinement; for the inner solve of the mixed form, the matrix values and
@@ -124,8 +126,6 @@ struct Vecs {
 struct QpDev {
   int n_nodes;
   int n_slices;
-  int tile0;         // first entry of this QP in the batch tile table
-  int n_tiles;       // tiles of this QP (partial-sum slots per launch)
   const int *sl_off;
   const int *col;
   Vecs<double> d;
@@ -133,18 +133,23 @@ struct QpDev {
   double *ev, *pv, *pw; // power method vectors (FP64)
   const unsigned char *cf;
   unsigned char *face;
-  QpState *st;          // st[0], st[1] (launch parity)
   int *done_flag;       // receives `tag` once the QP reaches DONE
   int tag;              // nonzero; distinguishes successive occupants of a batch slot
-  double *part;         // [2][kNumPart][n_tiles] partial sums (launch parity, slot-major)
 };
 
 // One work item of a launch: up to `slices` consecutive slices of one QP.
+// The tile carries its QP's state and partial-sum pointers itself, so the
+// launch prologue reaches them with one dependent load instead of going
+// through the QP table.
 struct Tile {
+  QpState *st;   // the QP's st[0], st[1] (launch parity)
+  double *part;  // the QP's [2][kNumPart][nt] partial sums (launch parity, slot-major)
   int qp;        // index into the batch's QpDev table
   int slice0;
   int slices;
-  int local;     // tile index inside its QP (partials slot)
+  int local;     // tile index inside its QP (partials column)
+  int nt;        // tiles of the QP
+  int pad;
 };
 
 }  // namespace mprgp

@@ -200,7 +200,7 @@ int mprgp_engine::round_once(long r, bool &idle) {
     std::vector<Tile> tv;
     tv.reserve(n_tiles);
     for (int i = 0; i < (int)slots.size(); ++i)
-      if (slots[i].req && slots[i].retiring == 0) append_tiles(i, slots[i].req->q->A->n_slices, ts, tv);
+      if (slots[i].req && slots[i].retiring == 0) append_tiles(i, slots[i].req->q, ts, tv);
     std::memcpy(tl, tv.data(), sizeof(Tile) * tv.size());
     int *ctrl = (int *)(tl + n_tiles);
     ctrl[0] = n_tiles;

@@ -73,7 +73,7 @@ struct BatchTables {
 // Pieces of the batch solve shared by solve_batch and the engine.
 int check_batch_qp(const mprgp_ctx *ctx, const mprgp_qp *q, const mprgp_params &p, int i);
 int pick_ts(const mprgp_params &p, long total_slices);
-void append_tiles(int slot, int n_slices, int span, std::vector<Tile> &tiles);
+void append_tiles(int slot, const mprgp_qp *q, int span, std::vector<Tile> &tiles);
 int tiles_for(int n_slices, int span);
 int round_grid(int n_tiles);
 long solve_step_bound(const mprgp_params &p);

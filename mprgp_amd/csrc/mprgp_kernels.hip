@@ -202,15 +202,40 @@ __device__ __forceinline__ void spmv_row(const int *sl_off, const int *col, cons
 
 // ----------------------------------------------------------- reductions
 
+// Cross-lane moves without the LDS crossbar: DPP within rows of 16 lanes,
+// then readlane of the four row totals.
+constexpr int kDppSwap1 = 0xb1;        // quad_perm [1,0,3,2]: lane ^ 1
+constexpr int kDppSwap2 = 0x4e;        // quad_perm [2,3,0,1]: lane ^ 2
+constexpr int kDppHalfMirror = 0x141;  // lane i of each 8 reads lane 7 - i
+constexpr int kDppMirror = 0x140;      // lane i of each 16 reads lane 15 - i
+
+template <int kCtrl>
+__device__ __forceinline__ double dpp(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+// Sum over the wavefront in a fixed order (pairs, quads, eighths, rows of
+// 16, then the four rows), the same value in every lane: each combining
+// step adds the same two operands in both lanes of a partner pair.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-  return v;
+  v += dpp<kDppSwap1>(v);
+  v += dpp<kDppSwap2>(v);
+  v += dpp<kDppHalfMirror>(v);
+  v += dpp<kDppMirror>(v);
+  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
 }
 __device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v = fmin(v, __shfl_xor(v, m, 64));
-  return v;
+  v = fmin(v, dpp<kDppSwap1>(v));
+  v = fmin(v, dpp<kDppSwap2>(v));
+  v = fmin(v, dpp<kDppHalfMirror>(v));
+  v = fmin(v, dpp<kDppMirror>(v));
+  return fmin(fmin(lane_value(v, 0), lane_value(v, 16)), fmin(lane_value(v, 32), lane_value(v, 48)));
 }
 
 // Partial sums an op produces: slots 0..n-1 reduce with sum.  The classic
@@ -843,23 +868,25 @@ __device__ __forceinline__ bool fp64_op(int op) {
 // A tile's rows: wave w takes slices w, w+TS, w+2TS, ... of the tile, so the
 // waves of a workgroup stream neighbouring slices at the same time.
 template <int TS, bool MX>
+// One slice per wavefront: a tile is up to TS consecutive slices.
 __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &st, double (&P)[kNumPart]) {
   const int wave = threadIdx.x / kSlice;
   const int lane = threadIdx.x % kSlice;
-  const bool f64 = fp64_op(op);
-  for (int i = wave; i < t.slices; i += TS) {
-    const int s = t.slice0 + i;
-    const int r = s * kSlice + lane;
-    if (r >= q.n_nodes) continue;
-    if (f64) {
-      row_work_fp64(q, s, lane, r, op, st, P);
-    } else if (MX && st.mixed) {
-      row_work<float, MX>(q, q.f, s, lane, r, op, st, P);
-    } else {
-      row_work<double, MX>(q, q.d, s, lane, r, op, st, P);
-    }
+  const int s = t.slice0 + wave;
+  const int r = s * kSlice + lane;
+  if (wave >= t.slices || r >= q.n_nodes) return;
+  if (fp64_op(op)) {
+    row_work_fp64(q, s, lane, r, op, st, P);
+  } else if (MX && st.mixed) {
+    row_work<float, MX>(q, q.f, s, lane, r, op, st, P);
+  } else {
+    row_work<double, MX>(q, q.d, s, lane, r, op, st, P);
   }
 }
+
+// A QpState copy moves as 64 words of 8 bytes, one per lane; the type may
+// alias the state's int and double fields.
+typedef uint64_t __attribute__((__may_alias__)) StateWord;
 
 #ifndef MPRGP_STEP_MIN_WAVES
 #define MPRGP_STEP_MIN_WAVES 4
@@ -874,98 +901,99 @@ __global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(
                                                            int *running, int parity) {
   __shared__ double red[TS][kNumPart];
   __shared__ QpState sst;
-  const int n_tiles = *n_tiles_dev;
   const int wave = threadIdx.x / kSlice;
   const int lane = threadIdx.x % kSlice;
-  for (int ti = blockIdx.x; ti < n_tiles; ti += gridDim.x) {
-    const Tile t = tiles[ti];
-    const QpDev &q = qps[t.qp];
-    // 1. state the previous launch executed + its partials -> this launch's op
-    //    (state words and partials are loaded together by wave 0)
-    constexpr int kWords = sizeof(QpState) / sizeof(int);
-    if (wave == 0) {
-      // partials are slot-major ([parity][slot][tile]): every slot's values
-      // are contiguous, and only the slots the previous op wrote are reduced
-      const int nt = q.n_tiles;
-      const double *pp = q.part + (size_t)(parity ^ 1) * kNumPart * nt;
-      double acc[kNumPart];
-      double mn = kDblMax;
+  // one tile per workgroup: the grid is a power of two >= the tile count
+  // (graphs replay with a shrinking table) and the surplus workgroups exit
+  const int ti = blockIdx.x;
+  if (ti >= *n_tiles_dev) return;
+  const Tile t = tiles[ti];
+  const QpDev &q = qps[t.qp];
+  // 1. state the previous launch executed + its partials -> this launch's op.
+  //    Wave 0 issues every load (one 8-byte state word per lane, the
+  //    partials slot-major: [parity][slot][tile]) before the first wait.
+  if (wave == 0) {
+    const StateWord w = reinterpret_cast<const StateWord *>(t.st + (parity ^ 1))[lane];
+    const int nt = t.nt;
+    const double *pp = t.part + (size_t)(parity ^ 1) * kNumPart * nt;
+    double a[2][kNumPart];
 #pragma unroll
-      for (int j = 0; j < kNumPart; ++j) acc[j] = 0.0;
-      for (int k = lane; k < nt; k += kSlice) {
+    for (int u = 0; u < 2; ++u) {
+      const int k = lane + u * kSlice;
 #pragma unroll
-        for (int j = 0; j < kNumPart; ++j) {
-          const double v = pp[(size_t)j * nt + k];
-          acc[j] += v;
-          if (j == kMinSlot) mn = fmin(mn, v);
-        }
-      }
-      const int *src = reinterpret_cast<const int *>(q.st + (parity ^ 1));
-      int *dst = reinterpret_cast<int *>(&sst);
-      for (int i = lane; i < kWords; i += kSlice) dst[i] = src[i];
-      __builtin_amdgcn_s_waitcnt(0);
-      __builtin_amdgcn_wave_barrier();
-      const int prev = sst.op;  // the op whose partials these are (same in every lane)
-      const int ns = sum_slots(prev);
-      Sums S;
+      for (int j = 0; j < kNumPart; ++j) a[u][j] = (k < nt) ? pp[(size_t)j * nt + k] : 0.0;
+    }
+    double acc[kNumPart];
+    double mn = fmin(lane < nt ? a[0][kMinSlot] : kDblMax, lane + kSlice < nt ? a[1][kMinSlot] : kDblMax);
 #pragma unroll
-      for (int j = 0; j < kNumPart; ++j) S.v[j] = (j < ns) ? wave_sum(acc[j]) : 0.0;
-      if (prev == OP_CL_AP) S.v[kMinSlot] = wave_min(mn);
-      if (lane == 0) {
-        const bool skip = sst.fresh || sst.op == OP_DONE;
-        const bool was_done = sst.op == OP_DONE;
-        if (!skip) control(sst, S);
-        sst.fresh = 0;
-        if (t.local == 0) {  // the QP's first tile counts the launch (or its end)
-          if (sst.op != OP_DONE) {
-            sst.steps++;
-            sst.opcnt[sst.op]++;
-          } else if (!was_done) {
-            atomicSub(running, 1);
-            *q.done_flag = q.tag;
-          }
-        }
+    for (int j = 0; j < kNumPart; ++j) acc[j] = a[0][j] + a[1][j];
+    for (int k = lane + 2 * kSlice; k < nt; k += kSlice) {  // QPs of more than 128 tiles
+#pragma unroll
+      for (int j = 0; j < kNumPart; ++j) {
+        const double v = pp[(size_t)j * nt + k];
+        acc[j] += v;
+        if (j == kMinSlot) mn = fmin(mn, v);
       }
     }
-    __syncthreads();
-    const QpState &st = sst;
-    const int op = st.op;
-    if (op != OP_DONE) {
-      const bool has_min = op == OP_CL_AP;
-      double P[kNumPart];
+    reinterpret_cast<StateWord *>(&sst)[lane] = w;
+    // the op whose partials these are: the state's first word, held by lane 0
+    const int prev = __builtin_amdgcn_readfirstlane((int)(uint32_t)w);
+    const int ns = sum_slots(prev);
+    Sums S;
 #pragma unroll
-      for (int j = 0; j < kNumPart; ++j) P[j] = 0.0;
-      if (has_min) P[kMinSlot] = kDblMax;
-      tile_work<TS, MX>(q, t, op, st, P);
-      // wave then block reduction in a fixed order, over the op's slots only
-      const int ns = has_min ? kNumPart : sum_slots(op);
-      if (ns > 0) {
-#pragma unroll
-        for (int j = 0; j < kNumPart; ++j)
-          if (j < ns) P[j] = (has_min && j == kMinSlot) ? wave_min(P[j]) : wave_sum(P[j]);
-        if (lane == 0) {
-#pragma unroll
-          for (int j = 0; j < kNumPart; ++j) red[wave][j] = P[j];
-        }
-        __syncthreads();
-        if (wave == 0 && lane < ns) {
-          const int j = lane;
-          const bool m = has_min && j == kMinSlot;
-          double acc = red[0][j];
-          for (int w = 1; w < TS; ++w) acc = m ? fmin(acc, red[w][j]) : acc + red[w][j];
-          q.part[((size_t)parity * kNumPart + j) * st.n_tiles + t.local] = acc;
+    for (int j = 0; j < kNumPart; ++j) S.v[j] = (j < ns) ? wave_sum(acc[j]) : 0.0;
+    if (prev == OP_CL_AP) S.v[kMinSlot] = wave_min(mn);
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) {
+      const bool skip = sst.fresh || sst.op == OP_DONE;
+      const bool was_done = sst.op == OP_DONE;
+      if (!skip) control(sst, S);
+      sst.fresh = 0;
+      if (t.local == 0) {  // the QP's first tile counts the launch (or its end)
+        if (sst.op != OP_DONE) {
+          sst.steps++;
+          sst.opcnt[sst.op]++;
+        } else if (!was_done) {
+          atomicSub(running, 1);
+          *q.done_flag = q.tag;
         }
       }
     }
-    // 2. the QP's first tile persists the state this launch executes (wave 0
-    //    wrote it in the prologue; the other waves only read it)
-    if (t.local == 0 && wave == 0) {
-      const int *src = reinterpret_cast<const int *>(&sst);
-      int *dst = reinterpret_cast<int *>(q.st + parity);
-      for (int i = lane; i < kWords; i += kSlice) dst[i] = src[i];
-    }
-    __syncthreads();  // before the next tile of this workgroup reuses sst
   }
+  __syncthreads();
+  const QpState &st = sst;
+  const int op = st.op;
+  if (op != OP_DONE) {
+    const bool has_min = op == OP_CL_AP;
+    double P[kNumPart];
+#pragma unroll
+    for (int j = 0; j < kNumPart; ++j) P[j] = 0.0;
+    if (has_min) P[kMinSlot] = kDblMax;
+    tile_work<TS, MX>(q, t, op, st, P);
+    // wave then block reduction in a fixed order, over the op's slots only
+    const int ns = has_min ? kNumPart : sum_slots(op);
+    if (ns > 0) {
+#pragma unroll
+      for (int j = 0; j < kNumPart; ++j)
+        if (j < ns) P[j] = (has_min && j == kMinSlot) ? wave_min(P[j]) : wave_sum(P[j]);
+      if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < kNumPart; ++j) red[wave][j] = P[j];
+      }
+      __syncthreads();
+      if (wave == 0 && lane < ns) {
+        const int j = lane;
+        const bool m = has_min && j == kMinSlot;
+        double acc = red[0][j];
+        for (int w = 1; w < TS; ++w) acc = m ? fmin(acc, red[w][j]) : acc + red[w][j];
+        t.part[((size_t)parity * kNumPart + j) * t.nt + t.local] = acc;
+      }
+    }
+  }
+  // 2. the QP's first tile persists the state this launch executes (wave 0
+  //    wrote it in the prologue; the other waves only read it)
+  if (t.local == 0 && wave == 0)
+    reinterpret_cast<StateWord *>(t.st + parity)[lane] = reinterpret_cast<const StateWord *>(&sst)[lane];
 }
 
 // ------------------------------------------------------- helper kernels

@@ -1,8 +1,8 @@
 """Autotuner of the batched solve's launch shape.
 
-The step kernel's shape is (tiles = wavefronts per workgroup, slices per
-wavefront) -- one wavefront streams 64 block rows per slice -- and the
-number of launches captured per hipGraph replay.  Every candidate changes
+The step kernel's shape is the number of wavefronts per workgroup (one
+SELL slice of 64 block rows per wavefront, so a workgroup's tile is that many
+slices) and the number of launches captured per hipGraph replay.  Every candidate changes
 only the schedule (and, through the tile grouping of partial sums, the last
 bits of the reductions), never the algorithm.  ``tune`` times each candidate
 on the caller's own batch from the same warm start and keeps the fastest;
@@ -19,14 +19,17 @@ import time
 import mprgp_amd as M
 
 DEFAULT_CANDIDATES = [
-    {"tiles": 4, "spw": 1, "graph_steps": 16},
-    {"tiles": 4, "spw": 1, "graph_steps": 32},
-    {"tiles": 4, "spw": 1, "graph_steps": 8},
-    {"tiles": 2, "spw": 1, "graph_steps": 16},
-    {"tiles": 8, "spw": 1, "graph_steps": 16},
-    {"tiles": 4, "spw": 2, "graph_steps": 16},
-    {"tiles": 1, "spw": 1, "graph_steps": 16},
+    {"tiles": 4, "graph_steps": 16},
+    {"tiles": 2, "graph_steps": 16},
+    {"tiles": 2, "graph_steps": 32},
+    {"tiles": 2, "graph_steps": 8},
+    {"tiles": 4, "graph_steps": 32},
+    {"tiles": 8, "graph_steps": 16},
+    {"tiles": 1, "graph_steps": 16},
 ]
+
+# Bumped whenever the step kernel changes in a way that moves the best shape.
+CACHE_VERSION = 2
 
 
 def cache_path():
@@ -38,14 +41,13 @@ def batch_key(qps, precision, device=0):
     """Device, precision and the batch's size class (log2 of total slices and
     of the QP count): batches in one class share a launch shape."""
     slices = sum(-(-q.n // 3 // 64) for q in qps)
-    return f"dev{device}/{'mixed' if precision == M.MIXED else 'fp64'}/" \
+    return f"v{CACHE_VERSION}/dev{device}/{'mixed' if precision == M.MIXED else 'fp64'}/" \
            f"s{int(math.log2(max(slices, 1)))}/q{int(math.log2(max(len(qps), 1)))}"
 
 
 def apply(p: M.Params, shape):
     p.tiles = int(shape["tiles"])
     p.graph_steps = int(shape["graph_steps"])
-    p.reserved_i[0] = int(shape["spw"])
     return p
 
 

@@ -68,9 +68,9 @@ def test_tune_keys_and_shapes():
     b = [Q(3 * 1000)] * 10 + [Q(30)]
     assert tune.batch_key(a, M.MIXED) == tune.batch_key(a, M.MIXED)
     assert tune.batch_key(a, M.MIXED) != tune.batch_key(a, M.FP64)
-    assert tune.batch_key(a, M.MIXED).startswith("dev0/mixed/")
+    assert tune.batch_key(a, M.MIXED).startswith(f"v{tune.CACHE_VERSION}/dev0/mixed/")
     assert tune.batch_key(b, M.MIXED) == tune.batch_key(a, M.MIXED)  # same size class
-    p = tune.apply(M.params(), {"tiles": 8, "spw": 2, "graph_steps": 32})
-    assert (p.tiles, p.reserved_i[0], p.graph_steps) == (8, 2, 32)
+    p = tune.apply(M.params(), {"tiles": 8, "graph_steps": 32})
+    assert (p.tiles, p.graph_steps) == (8, 32)
     for s in tune.DEFAULT_CANDIDATES:
-        assert s["tiles"] in (1, 2, 4, 8) and s["spw"] >= 1 and s["graph_steps"] > 0
+        assert s["tiles"] in (1, 2, 4, 8) and s["graph_steps"] > 0
