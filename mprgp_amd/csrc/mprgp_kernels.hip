@@ -245,7 +245,7 @@ __device__ __forceinline__ int sum_slots(int op) {
   switch (op) {
     case OP_TOP: case OP_CG_PY: case OP_CG_B:
       return 3;
-    case OP_CG_Y:
+    case OP_CG_Y: case OP_CG_PYG:
       return 4;
     case OP_CG_AP: case OP_CL_AP: case OP_PROP1: case OP_CL_PROP1:
       return 2;
@@ -305,7 +305,12 @@ __device__ __forceinline__ void cg_decide(QpState &st, double fnew, double fx_en
   }
   if (st.prop && st.residual > st.tol && st.fy <= st.fx && st.iteration < st.max_it) {
     st.xb ^= 1;  // x = y: the trial point's buffer becomes the iterate's
-    st.op = OP_CG_A;
+    if (st.mixed) {  // CG_PYG already applied g -= alpha AP, phi, z
+      st.beta_cg = st.zap / st.pd;
+      st.op = OP_CG_B;
+    } else {
+      st.op = OP_CG_A;
+    }
   } else if (st.residual <= st.tol) {
     st.op = OP_FINAL;
   } else {
@@ -386,24 +391,28 @@ __device__ void control(QpState &st, const Sums &S) {
       st.pd = S.v[0];
       st.zg = S.v[1];
       st.alpha = st.zg / st.pd;
-      st.op = OP_CG_PY;
+      st.op = st.mixed ? OP_CG_PYG : OP_CG_PY;
       break;
     case OP_CG_PY:
       st.y_valid = 1;
       st.y_feas = S.v[0] == 0.0;
+      st.op = OP_CG_Y;
+      break;
+    case OP_CG_PYG: {
+      // mixed inner solve: f(y) = f(x) - alpha p'g + alpha^2 p'Ap/2 from the
+      // step's own sums; when P(y) = y that is f(P(y)), otherwise CG_Y adds
+      // the projected rows' share
+      st.y_valid = 1;
+      st.y_feas = S.v[0] == 0.0;
       st.pg = S.v[1];
       if (st.cg_entry) st.fx_entry = 0.5 * S.v[2];  // f(x) = x'(g - b)/2 with g = A x - b
-      if (st.mixed) {
-        // mixed inner solve: f(y) from the step's own sums; when P(y) = y that
-        // is f(P(y)), otherwise CG_Y adds d'A d/2 over the projected rows
-        const double f0 = st.cg_entry ? st.fx_entry : st.fy;
-        st.f_trial = f0 - st.alpha * st.pg + 0.5 * st.alpha * st.alpha * st.pd;
-        if (st.y_feas) cg_decide(st, st.f_trial, st.fx_entry);
-        else st.op = OP_CG_Y;
-      } else {
-        st.op = OP_CG_Y;
-      }
+      st.zap = S.v[3];
+      const double f0 = st.cg_entry ? st.fx_entry : st.fy;
+      st.f_trial = f0 - st.alpha * st.pg + 0.5 * st.alpha * st.alpha * st.pd;
+      if (st.y_feas) cg_decide(st, st.f_trial, st.fx_entry);
+      else st.op = OP_CG_Y;
       break;
+    }
     case OP_CG_Y:
       if (st.mixed) cg_decide(st, st.f_trial + S.v[1] + 0.5 * S.v[0], st.fx_entry);
       else cg_decide(st, 0.5 * S.v[0] - S.v[1], 0.5 * S.v[2] - S.v[3]);
@@ -597,27 +606,49 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
     }
     case OP_CG_PY: {
       Row R = ldrow(q, v, r);
-      const V3 xr = ld3(xc, r), pr = ld3(v.p, r);
-      V3 y = axmy(xr, st.alpha, pr);
-      const V3 pyr = proj(R, y);
+      V3 y = axmy(ld3(xc, r), st.alpha, ld3(v.p, r));
       st3(yc, r, y);
-      st3(v.py, r, pyr);
+      st3(v.py, r, proj(R, y));
       P[0] += infeasible(R, y);
-      if (mixed) {  // mixed inner solve: sums for f(y) = f(x) - alpha p'g + alpha^2 p'Ap/2
-        const V3 gr = ld3(v.g, r);
-        P[1] += dot3(pr, gr);
-        if (st.cg_entry) P[2] += dot3(xr, sub(gr, ld3(v.b, r)));
-      }
+      break;
+    }
+    case OP_CG_PYG: {
+      // Mixed inner solve, CGMonotonicStep (MPRGPSolver.h:281-297) in one
+      // pass: the trial point with the sums for f(y), and the gradient
+      // update of an accepted step applied in place.  A rejected step leaves
+      // the CG loop through EXP1/EXP2, PROP1 or FINAL, and the first two
+      // recompute g = A x - b, phi and z before reading them (the reference
+      // does the same: MPRGPSolver.h:255-261, 435-437); FINAL reads x only.
+      // The face stays fixed inside the CG loop.
+      if (!MX) break;
+      Row R = ldrow(q, v, r);
+      const bool f = q.face[r];
+      const V3 xr = ld3(xc, r), pr = ld3(v.p, r);
+      const V3 y = axmy(xr, st.alpha, pr);
+      st3(yc, r, y);
+      st3(v.py, r, proj(R, y));
+      P[0] += infeasible(R, y);
+      const V3 gr = ld3(v.g, r), ap = ld3(v.ap, r);
+      P[1] += dot3(pr, gr);
+      if (st.cg_entry) P[2] += dot3(xr, sub(gr, ld3(v.b, r)));
+      const V3 g = axmy(gr, st.alpha, ap);
+      const V3 ph = phi_of(R, f, g);
+      const V3 z = precond_of(v, pre, R, f, r, g, ph);
+      st3(v.g, r, g);
+      st3(v.phi, r, ph);
+      st3(v.z, r, z);
+      P[3] += dot3(z, ap);
       break;
     }
     case OP_CG_Y: {
       if (mixed) {
         // mixed inner solve: with d = P(y) - y, nonzero only on the rows the
         // projection moved, f(P(y)) = f(y) + d'(A y - b) + d'A d/2 where
-        // A y - b = g - alpha A p; every other row reads no matrix at all
+        // A y - b = g - alpha A p, the gradient CG_PYG left in g; every other
+        // row reads no matrix at all
         const V3 dr = sub(ld3(v.py, r), ld3(yc, r));
         if (dr.x != 0.0 || dr.y != 0.0 || dr.z != 0.0) {
-          P[1] += dot3(dr, axmy(ld3(v.g, r), st.alpha, ld3(v.ap, r)));
+          P[1] += dot3(dr, ld3(v.g, r));
           const double *py = v.py, *yy = yc;
           V3 acc[1];
           spmv_row<1, 1>(q.sl_off, q.col, v.val, s, lane,
@@ -865,10 +896,9 @@ __device__ __forceinline__ bool fp64_op(int op) {
          op == OP_R_START || op == OP_R_UPD || op == OP_R_FINAL;
 }
 
-// A tile's rows: wave w takes slices w, w+TS, w+2TS, ... of the tile, so the
-// waves of a workgroup stream neighbouring slices at the same time.
+// One slice per wavefront: a tile is up to TS consecutive slices, streamed
+// by the workgroup's waves at the same time.
 template <int TS, bool MX>
-// One slice per wavefront: a tile is up to TS consecutive slices.
 __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &st, double (&P)[kNumPart]) {
   const int wave = threadIdx.x / kSlice;
   const int lane = threadIdx.x % kSlice;

@@ -16,7 +16,7 @@ from mprgp_amd import problems
 # op indices of the device state machine (mprgp_dev.h)
 OPS = ["DONE", "PROJX", "POW_MV", "POW_NRM", "POW_SAVE", "INIT", "TOP", "CG_AP", "CG_PY", "CG_Y",
        "CG_A", "CG_B", "EXP1", "EXP2", "EXP3", "PROP1", "PROP2", "FINAL", "CL_AP", "CL_CG",
-       "CL_EXP", "CL_EXP2", "CL_PROP1", "R_START", "R_UPD", "R_FINAL"]
+       "CL_EXP", "CL_EXP2", "CL_PROP1", "R_START", "R_UPD", "R_FINAL", "CG_PYG"]
 
 
 class DeviceQP:
@@ -57,6 +57,7 @@ _STREAMS = {
     "CG_PY": ("rows", "x", "p", "y", "py"),
     "CG_Y": ("M", "py", "b"),
     "CG_A": ("rows", "face", "ap", "g", "dinv", "g", "phi", "z"),
+    "CG_PYG": ("rows", "face", "x", "p", "y", "py", "g", "ap", "dinv", "g", "phi", "z"),
     "CG_B": ("rows", "face", "z", "p", "g", "phi", "x", "p", "beta"),
     "EXP1": ("rows", "x", "y"),
     "EXP2": ("M", "y", "b", "rows", "g", "py", "x", "face"),
@@ -71,13 +72,13 @@ _STREAMS = {
     "R_FINAL": ("rows", "x64", "face", "x64"),
 }
 
-# FP32 streams of the mixed form's inner solve; everything else is FP64.
-MIXED_F32 = {"M", "p", "z", "ap"}
-# streams only the mixed form's inner solve adds (p.g for f(y) without A y)
-MIXED_EXTRA = {"CG_PY": ("g",)}
+# FP32 streams of the mixed form's inner solve; everything else (x, y, P(y),
+# 1/diag, the rows) is FP64.
+MIXED_F32 = {"M", "b", "g", "phi", "beta", "p", "z", "ap"}
 # ops the mixed form's inner solve runs differently: CG_Y reads P(y) and y of
-# every row and the matrix only on the few rows the projection moved (not
-# counted: a lower bound)
+# every row, and g and the matrix only on the few rows the projection moved
+# (not counted: a lower bound).  CG_PYG's read of b at CG entry is not
+# counted either.
 MIXED_STREAMS = {"CG_Y": ("py", "y")}
 
 
@@ -93,7 +94,7 @@ def op_bytes(op, N, nb, m, mixed=False, precond=True):
     name = OPS[op]
     total = 0
     streams = MIXED_STREAMS.get(name, _STREAMS.get(name, ())) if mixed else _STREAMS.get(name, ())
-    for st in streams + (MIXED_EXTRA.get(name, ()) if mixed else ()):
+    for st in streams:
         if st == "rows":
             total += N + 48 * m
         elif st == "cc":

@@ -33,10 +33,10 @@ def test_byte_accounting():
         assert workloads.op_bytes(op("POW_MV"), N, nb, m, mixed) == mat64 + 2 * v64
     # CG_AP: matrix + p (gathered once) + AP + z + g
     assert workloads.op_bytes(op("CG_AP"), N, nb, m) == mat64 + 4 * v64
-    assert workloads.op_bytes(op("CG_AP"), N, nb, m, True) == mat32 + 3 * v32 + v64
-    # the mixed form's CG_PY also reads g
-    assert (workloads.op_bytes(op("CG_PY"), N, nb, m, True)
-            - workloads.op_bytes(op("CG_PY"), N, nb, m, False)) == v64 - v32  # p FP32, + g
+    assert workloads.op_bytes(op("CG_AP"), N, nb, m, True) == mat32 + 4 * v32  # all FP32 in the inner solve
+    # the mixed form's fused CG step: rows (flag byte + 6 words per row), face,
+    # x, y, P(y) and 1/diag in FP64; p, g read and written, AP, phi, z in FP32
+    assert workloads.op_bytes(op("CG_PYG"), N, nb, m, True) == (N + 48 * m) + N + 4 * v64 + 6 * v32
     # no preconditioner: no D^-1 stream
     assert (workloads.op_bytes(op("CG_A"), N, nb, m) - workloads.op_bytes(op("CG_A"), N, nb, m, precond=False)
             == v64)
