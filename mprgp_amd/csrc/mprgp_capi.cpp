@@ -413,7 +413,7 @@ static int ensure_matrix32(mprgp_matrix *A, hipStream_t s) {
 static int ensure_qp32(mprgp_qp *q) {
   if (q->base32) return MPRGP_OK;
   const size_t n3 = 3 * (size_t)q->n;
-  const size_t bytes = (3 * n3 + (size_t)q->n) * sizeof(double) + (7 * n3 + 64) * sizeof(float);
+  const size_t bytes = (3 * n3 + (size_t)q->n) * sizeof(double) + (9 * n3 + 64) * sizeof(float);
   if (int rc = q->ctx->alloc(&q->base32, bytes)) return rc;
   q->bytes32 = bytes;
   double *dp = (double *)q->base32;
@@ -425,7 +425,7 @@ static int ensure_qp32(mprgp_qp *q) {
   q->fcc = dp;
   dp += q->n;
   float *fp32 = (float *)dp;
-  float **fvecs[] = {&q->fp, &q->fz, &q->fap, &q->fb, &q->fg, &q->fphi, &q->fbeta};
+  float **fvecs[] = {&q->fp, &q->fp2, &q->fz, &q->fap, &q->fb, &q->fg, &q->fphi, &q->fbeta, &q->fbeta2};
   for (float **v : fvecs) {
     *v = fp32;
     fp32 += n3;
@@ -883,7 +883,8 @@ int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, i
     d.f.val = q->A->d_val32;
     d.f.dinv = q->A->d_dinv;
     d.f.x = q->fx; d.f.b = q->fb; d.f.g = q->fg; d.f.p = q->fp; d.f.z = q->fz; d.f.phi = q->fphi;
-    d.f.beta = q->fbeta; d.f.y = q->fy; d.f.ap = q->fap; d.f.py = q->fpy;
+    d.f.beta = q->fbeta; d.f.y = q->fy; d.f.ap = q->fap; d.f.py = q->fpy; d.f.p2 = q->fp2;
+    d.f.beta2 = q->fbeta2;
     d.f.cn = q->cn; d.f.cj = q->cj; d.f.cs = q->cs; d.f.cc = q->fcc;
   }
   std::memset(&st, 0, sizeof(st));

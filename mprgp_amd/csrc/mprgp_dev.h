@@ -21,7 +21,7 @@
 namespace mprgp {
 
 constexpr int kSlice = 64;       // rows per SELL slice = lanes per wavefront
-constexpr int kNumPart = 4;      // partial sums per tile (most any op produces)
+constexpr int kNumPart = 8;      // partial sums per tile (most any op produces: 7)
 constexpr double kEps = 1e-9;    // ScalarUtil<double>::scalar_eps of the simulator
 
 enum Op : int {
@@ -56,7 +56,10 @@ enum Op : int {
   OP_CG_PYG,     // mixed inner CG step in one pass: y = x - alpha p, py = P(y),
                  // feasibility, p.g (and f(x) at CG entry), and the gradient
                  // update of an accepted step applied in place: g -= alpha AP,
-                 // phi, z, z.AP
+                 // phi, z, z.AP, z.g; then beta into the other beta buffer,
+                 // |phi+beta|^2, beta.beta, phitphi
+  OP_CG_BAP,     // mixed inner: p' = z - beta p into the other p buffer and
+                 // AP = A p' with p' formed at the gather from z and p, p'.AP
   OP_NUM
 };
 
@@ -93,6 +96,9 @@ struct alignas(128) QpState {
   int num_cg_t, num_exp_t, num_prop_t;
   int since_best;           // mixed inner: iterations without halving the residual
   int xb;                   // 1: x lives in the y buffer and y in the x buffer
+  int pb;                   // 1: p lives in the p2 buffer (mixed inner)
+  int bb;                   // 1: beta lives in the beta2 buffer (mixed inner)
+  int prop_next;            // proportioning test at the CG trial point (mixed inner)
   double tol, alpha_bar, gamma;
   double alpha, beta_cg, pd, zg, residual, fx, fy;
   double alpha_f, nu, power_eps, lambda_max;
@@ -100,6 +106,7 @@ struct alignas(128) QpState {
   double pg, fx_entry;      // p.g of the CG step; f(x) at CG entry (mixed inner)
   double f_trial;           // f(y) of the CG trial point (mixed inner)
   double zap;               // z.AP after the in-place gradient update (mixed inner)
+  double res_next;          // KKT residual at the CG trial point (mixed inner)
   int opcnt[OP_NUM];  // launches this QP spent in each op
 };
 // The launch prologue moves a state copy as one 8-byte word per lane.
@@ -123,6 +130,10 @@ struct Vecs {
   double *x, *y, *py;         // iterate and trial points
   T *b, *g, *phi, *beta;      // gradient side
   T *p, *z, *ap;              // search direction side
+  T *p2;                      // second p buffer (mixed inner: p' = z - beta p
+                              // is written while neighbours still gather p)
+  T *beta2;                   // second beta buffer (mixed inner: the trial
+                              // point's beta, kept apart until it is accepted)
   const double *cn, *cj, *cs; // normal, |n|^2, tangent-Jacobi factor
   double *cc;                 // offset (mixed inner: shifted c - n.x)
 };

@@ -148,7 +148,8 @@ struct mprgp_qp {
   size_t bytes32 = 0;
   double *fx = nullptr, *fy = nullptr, *fpy = nullptr;  // FP64 iterate/trial points of the inner solve
   float *fb = nullptr, *fg = nullptr, *fphi = nullptr, *fbeta = nullptr;  // FP32 gradient side
-  float *fp = nullptr, *fz = nullptr, *fap = nullptr;   // FP32 direction side
+  float *fbeta2 = nullptr;
+  float *fp = nullptr, *fp2 = nullptr, *fz = nullptr, *fap = nullptr;   // FP32 direction side
   double *fcc = nullptr;  // shifted offsets c - n.x of the correction QP
   std::atomic<int> busy{0};  // submitted to an engine and not yet completed
 };

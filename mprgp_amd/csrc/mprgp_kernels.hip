@@ -35,14 +35,27 @@ struct V3 {
   double x, y, z;
 };
 
+// The data pointers of the QP table are generic pointers; every access
+// through them goes via gmem() to the global address space, so it compiles to
+// global memory instructions rather than flat ones (which would also count
+// against the LDS/scalar wait counter and take 64-bit VGPR addresses).
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T *gmem(const T *p) {
+  return (const __attribute__((address_space(1))) T *)p;
+}
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gmem(T *p) {
+  return (__attribute__((address_space(1))) T *)p;
+}
+
 template <class T>
 __device__ __forceinline__ V3 ld3(const T *v, int r) {
-  const T *p = v + 3 * (size_t)r;
+  const auto *p = gmem(v) + 3 * (size_t)r;
   return V3{(double)p[0], (double)p[1], (double)p[2]};
 }
 template <class T>
 __device__ __forceinline__ void st3(T *v, int r, V3 a) {
-  T *p = v + 3 * (size_t)r;
+  auto *p = gmem(v) + 3 * (size_t)r;
   p[0] = (T)a.x;
   p[1] = (T)a.y;
   p[2] = (T)a.z;
@@ -65,12 +78,12 @@ struct Row {
 template <class T>
 __device__ __forceinline__ Row ldrow(const QpDev &q, const Vecs<T> &v, int r) {
   Row R;
-  R.has = q.cf[r] != 0;
+  R.has = gmem(q.cf)[r] != 0;
   if (R.has) {
     R.n = ld3(v.cn, r);
-    R.c = v.cc[r];
-    R.jj = v.cj[r];
-    R.s = v.cs[r];
+    R.c = gmem(v.cc)[r];
+    R.jj = gmem(v.cj)[r];
+    R.s = gmem(v.cs)[r];
   } else {
     R.n = V3{0.0, 0.0, 0.0};
     R.c = R.jj = R.s = 0.0;
@@ -139,17 +152,17 @@ __device__ __forceinline__ V3 precond_of(const Vecs<T> &v, bool on, const Row &R
 // returns NV input vectors at a neighbour node.  Padding slots hold a valid
 // column and zero values, so the loop has no branch and the column indices,
 // matrix values and gathers of U blocks are all in flight together.  U is
-// kept small enough (2 blocks for one vector, 1 for two) that the step kernel
+// kept small enough (3 blocks for one vector, 1 for two) that the step kernel
 // needs no scratch: spilled registers would go to HBM on every launch.
-template <int NV, int U = (NV == 1 ? 2 : 1), class T, class G>
+template <int NV, int U = (NV == 1 ? 3 : 1), class T, class G>
 __device__ __forceinline__ void spmv_row(const int *sl_off, const int *col, const T *val, int s, int lane,
                                          G gather, V3 (&acc)[NV]) {
 #pragma unroll
   for (int t = 0; t < NV; ++t) acc[t] = V3{0.0, 0.0, 0.0};
-  const int b0 = sl_off[s];
-  const int W = (sl_off[s + 1] - b0) / kSlice;
-  const int *cp = col + b0 + lane;
-  const T *vp = val + (size_t)b0 * 9 + lane;
+  const int b0 = gmem(sl_off)[s];
+  const int W = (gmem(sl_off)[s + 1] - b0) / kSlice;
+  const auto *cp = gmem(col) + b0 + lane;
+  const auto *vp = gmem(val) + (size_t)b0 * 9 + lane;
   int k = 0;
   for (; k + U <= W; k += U) {
     int j[U];
@@ -243,13 +256,15 @@ __device__ __forceinline__ double wave_min(double v) {
 constexpr int kMinSlot = 3;
 __device__ __forceinline__ int sum_slots(int op) {
   switch (op) {
-    case OP_TOP: case OP_CG_PY: case OP_CG_B:
+    case OP_CG_PYG:
+      return 7;
+    case OP_TOP: case OP_CG_PY: case OP_CG_B: case OP_CG_AP:
       return 3;
-    case OP_CG_Y: case OP_CG_PYG:
+    case OP_CG_Y:
       return 4;
-    case OP_CG_AP: case OP_CL_AP: case OP_PROP1: case OP_CL_PROP1:
+    case OP_CL_AP: case OP_PROP1: case OP_CL_PROP1:
       return 2;
-    case OP_POW_MV: case OP_POW_NRM: case OP_CG_A: case OP_EXP2: case OP_FINAL: case OP_CL_CG:
+    case OP_POW_MV: case OP_POW_NRM: case OP_CG_A: case OP_EXP2: case OP_FINAL: case OP_CL_CG: case OP_CG_BAP:
     case OP_R_START: case OP_R_FINAL:
       return 1;
     default:
@@ -305,9 +320,15 @@ __device__ __forceinline__ void cg_decide(QpState &st, double fnew, double fx_en
   }
   if (st.prop && st.residual > st.tol && st.fy <= st.fx && st.iteration < st.max_it) {
     st.xb ^= 1;  // x = y: the trial point's buffer becomes the iterate's
-    if (st.mixed) {  // CG_PYG already applied g -= alpha AP, phi, z
+    if (st.mixed) {
+      // CG_PYG already applied g -= alpha AP, phi, z and computed beta and
+      // the KKT and proportioning sums at y (MPRGPSolver.h:293-302)
+      st.bb ^= 1;
       st.beta_cg = st.zap / st.pd;
-      st.op = OP_CG_B;
+      st.residual = st.res_next;
+      st.prop = st.prop_next;
+      // a stalled inner solve ends through the loop's exit path
+      st.op = inner_stalled(st) ? OP_FINAL : OP_CG_BAP;
     } else {
       st.op = OP_CG_A;
     }
@@ -391,6 +412,7 @@ __device__ void control(QpState &st, const Sums &S) {
       st.pd = S.v[0];
       st.zg = S.v[1];
       st.alpha = st.zg / st.pd;
+      if (st.mixed && st.cg_entry) st.fx_entry = 0.5 * S.v[2];  // f(x) = x'(g - b)/2, g = A x - b
       st.op = st.mixed ? OP_CG_PYG : OP_CG_PY;
       break;
     case OP_CG_PY:
@@ -405,8 +427,10 @@ __device__ void control(QpState &st, const Sums &S) {
       st.y_valid = 1;
       st.y_feas = S.v[0] == 0.0;
       st.pg = S.v[1];
-      if (st.cg_entry) st.fx_entry = 0.5 * S.v[2];  // f(x) = x'(g - b)/2 with g = A x - b
+      st.zg = S.v[2];  // z.g after the update: the next step's alpha numerator
       st.zap = S.v[3];
+      st.res_next = sqrt(S.v[4]);
+      st.prop_next = proportional(st, S.v[5], S.v[6]);
       const double f0 = st.cg_entry ? st.fx_entry : st.fy;
       st.f_trial = f0 - st.alpha * st.pg + 0.5 * st.alpha * st.alpha * st.pd;
       if (st.y_feas) cg_decide(st, st.f_trial, st.fx_entry);
@@ -420,6 +444,12 @@ __device__ void control(QpState &st, const Sums &S) {
     case OP_CG_A:
       st.beta_cg = S.v[0] / st.pd;
       st.op = OP_CG_B;
+      break;
+    case OP_CG_BAP:
+      st.pb ^= 1;  // p' went to the other buffer
+      st.pd = S.v[0];
+      st.alpha = st.zg / st.pd;
+      st.op = OP_CG_PYG;
       break;
     case OP_CG_B:
       st.residual = sqrt(S.v[0]);
@@ -532,10 +562,25 @@ __device__ void control(QpState &st, const Sums &S) {
 // accumulate into P (FP64).
 // MX: the kernel instance may hold mixed-form QPs (otherwise their branches
 // are compiled out).
+template <class T>
+__device__ __forceinline__ V3 round_to(V3 a) {  // the value a store to T storage keeps
+  return V3{(double)(T)a.x, (double)(T)a.y, (double)(T)a.z};
+}
+
 template <class T, bool MX>
-__device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s, int lane, int r, int op,
+__device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int s, int lane, int r, int op,
                                          const QpState &st, double (&P)[kNumPart]) {
   const bool mixed = MX && st.mixed;
+  // p and p2, beta and beta2 swap roles with st.pb, st.bb (mixed inner solve)
+  Vecs<T> v = vin;
+  if (MX && st.pb) {
+    v.p = vin.p2;
+    v.p2 = vin.p;
+  }
+  if (MX && st.bb) {
+    v.beta = vin.beta2;
+    v.beta2 = vin.beta;
+  }
   // x and y swap buffers when a CG step is accepted (x = y without a copy):
   // st.xb names the buffer that holds x
   double *const xc = st.xb ? v.y : v.x;
@@ -558,14 +603,14 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
       st3(v.phi, r, ph);
       st3(v.z, r, z);
       st3(v.p, r, z);
-      q.face[r] = f;
+      gmem(q.face)[r] = f;
       if (op == OP_EXP3) st3(yc, r, ld3(v.py, r));
       break;
     }
     case OP_TOP: {
       // computeGradients (MPRGPSolver.h:163-173) + proportional (:174-181)
       Row R = ldrow(q, v, r);
-      bool f = q.face[r];
+      bool f = gmem(q.face)[r];
       if (st.p_pending) {  // classic CGStep tail: p = z - beta p (MPRGPSolver.h:192-193)
         st3(v.p, r, axmy(ld3(v.z, r), st.beta_cg, ld3(v.p, r)));
       }
@@ -590,7 +635,9 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
       st3(v.ap, r, acc[0]);
       V3 pr = ld3(v.p, r);
       P[0] += dot3(pr, acc[0]);
-      P[1] += dot3(ld3(v.z, r), ld3(v.g, r));
+      const V3 gr = ld3(v.g, r);
+      P[1] += dot3(ld3(v.z, r), gr);
+      if (mixed && st.cg_entry) P[2] += dot3(ld3(xc, r), sub(gr, ld3(v.b, r)));
       if (op == OP_CL_AP) {
         // stepLimit (MPRGPProjection.h:263-282)
         Row R = ldrow(q, v, r);
@@ -622,7 +669,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
       // The face stays fixed inside the CG loop.
       if (!MX) break;
       Row R = ldrow(q, v, r);
-      const bool f = q.face[r];
+      const bool f = gmem(q.face)[r];
       const V3 xr = ld3(xc, r), pr = ld3(v.p, r);
       const V3 y = axmy(xr, st.alpha, pr);
       st3(yc, r, y);
@@ -630,14 +677,44 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
       P[0] += infeasible(R, y);
       const V3 gr = ld3(v.g, r), ap = ld3(v.ap, r);
       P[1] += dot3(pr, gr);
-      if (st.cg_entry) P[2] += dot3(xr, sub(gr, ld3(v.b, r)));
       const V3 g = axmy(gr, st.alpha, ap);
       const V3 ph = phi_of(R, f, g);
       const V3 z = precond_of(v, pre, R, f, r, g, ph);
       st3(v.g, r, g);
       st3(v.phi, r, ph);
       st3(v.z, r, z);
+      // sums over the values later reads of z, g and phi see
+      const V3 gs = round_to<T>(g), phs = round_to<T>(ph);
+      P[2] += dot3(round_to<T>(z), gs);
       P[3] += dot3(z, ap);
+      // computeGradients(g, false) and proportional at x = y; beta goes to
+      // the other buffer, which only an accepted step makes current (a
+      // rejected step's proportioning reads the old beta)
+      const V3 be = beta_of(R, f, gs, phs);
+      st3(v.beta2, r, be);
+      const V3 gp = add(phs, be);
+      P[4] += dot3(gp, gp);
+      P[5] += dot3(be, be);
+      const V3 px = proj(R, axmy(y, st.alpha_bar, phs));
+      P[6] += dot3(sub(y, px), phs);
+      break;
+    }
+    case OP_CG_BAP: {
+      // Mixed inner solve: p = z - beta p and the next step's A p
+      // (MPRGPSolver.h:298, 304-306) in one pass.  p' is formed again at
+      // every gather from z and p, which no tile writes in this launch (p'
+      // goes to the other p buffer), rounded as its store.
+      if (!MX) break;
+      const T *zz = v.z, *pp = v.p;
+      const double beta = st.beta_cg;
+      V3 acc[1];
+      spmv_row<1, 3>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) {
+        xs[0] = round_to<T>(axmy(ld3(zz, j), beta, ld3(pp, j)));
+      }, acc);
+      const V3 pn = round_to<T>(axmy(ld3(v.z, r), beta, ld3(v.p, r)));
+      st3(v.p2, r, pn);
+      st3(v.ap, r, acc[0]);
+      P[0] += dot3(pn, acc[0]);
       break;
     }
     case OP_CG_Y: {
@@ -683,7 +760,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
     case OP_CG_A: {
       // CGMonotonicStep body, first half (MPRGPSolver.h:293-297)
       Row R = ldrow(q, v, r);
-      bool f = q.face[r];
+      bool f = gmem(q.face)[r];
       V3 ap = ld3(v.ap, r);  // x = y happened in the control (buffer swap)
       V3 g = axmy(ld3(v.g, r), st.alpha, ap);
       V3 ph = phi_of(R, f, g);
@@ -697,7 +774,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
     case OP_CG_B: {
       // p = z - beta p ; computeGradients(g, false) ; proportional (MPRGPSolver.h:298-302)
       Row R = ldrow(q, v, r);
-      bool f = q.face[r];
+      bool f = gmem(q.face)[r];
       st3(v.p, r, axmy(ld3(v.z, r), st.beta_cg, ld3(v.p, r)));
       V3 g = ld3(v.g, r);
       V3 ph = ld3(v.phi, r);
@@ -731,7 +808,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
       st3(v.g, r, g);
       st3(v.py, r, yn);
       st3(xc, r, proj(R, yn));
-      q.face[r] = f;
+      gmem(q.face)[r] = f;
       P[0] += infeasible(R, yn);
       break;
     }
@@ -775,13 +852,13 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
       st3(v.phi, r, ph);
       st3(v.z, r, z);
       st3(v.p, r, z);
-      q.face[r] = f;
+      gmem(q.face)[r] = f;
       break;
     }
     case OP_CL_CG: {
       // CGStep (MPRGPSolver.h:188-191); p is updated at the next TOP
       Row R = ldrow(q, v, r);
-      bool f = q.face[r];
+      bool f = gmem(q.face)[r];
       V3 ap = ld3(v.ap, r);
       st3(xc, r, axmy(ld3(xc, r), st.alpha, ld3(v.p, r)));
       V3 g = axmy(ld3(v.g, r), st.alpha, ap);
@@ -803,7 +880,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
       xt = axmy(xt, st.alpha_bar, ph);
       st3(xc, r, proj(R, xt));
       st3(v.g, r, g);
-      q.face[r] = f;
+      gmem(q.face)[r] = f;
       break;
     }
     case OP_FINAL: {
@@ -811,7 +888,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &v, int s
       V3 xr = ld3(xc, r);
       if (st.variant != VAR_CLASSIC) xr = proj(ldrow(q, v, r), xr);
       st3(v.x, r, xr);
-      P[0] += q.face[r] ? 1.0 : 0.0;
+      P[0] += gmem(q.face)[r] ? 1.0 : 0.0;
       break;
     }
     default:
@@ -870,10 +947,10 @@ __device__ __forceinline__ void row_work_fp64(const QpDev &q, int s, int lane, i
       V3 be = beta_of(R, f, g, ph);
       V3 gp = add(ph, be);
       P[0] += dot3(gp, gp);
-      q.face[r] = f;
+      gmem(q.face)[r] = f;
       st3(q.f.b, r, V3{-g.x, -g.y, -g.z});
       st3(q.f.x, r, V3{0.0, 0.0, 0.0});
-      if (R.has) q.f.cc[r] = R.c - dot3(R.n, xr);
+      if (R.has) gmem(q.f.cc)[r] = R.c - dot3(R.n, xr);
       break;
     }
     case OP_R_UPD: {
@@ -883,7 +960,7 @@ __device__ __forceinline__ void row_work_fp64(const QpDev &q, int s, int lane, i
     case OP_R_FINAL: {
       Row R = ldrow(q, v, r);
       st3(v.x, r, proj(R, ld3(v.x, r)));
-      P[0] += q.face[r] ? 1.0 : 0.0;
+      P[0] += gmem(q.face)[r] ? 1.0 : 0.0;
       break;
     }
     default:
@@ -911,6 +988,36 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
     row_work<float, MX>(q, q.f, s, lane, r, op, st, P);
   } else {
     row_work<double, MX>(q, q.d, s, lane, r, op, st, P);
+  }
+}
+
+// Partial sums of one QP's tiles, slots J0..J0+kGroup-1, summed per lane:
+// every load of a lane's first two columns is issued before the first use.
+constexpr int kGroup = 4;
+template <int J0>
+__device__ __forceinline__ void sum_part(const double *pp, int nt, int lane, double (&acc)[kNumPart],
+                                         double &mn) {
+  double a[2][kGroup];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int k = lane + u * kSlice;
+#pragma unroll
+    for (int j = 0; j < kGroup; ++j) a[u][j] = (k < nt) ? gmem(pp)[(size_t)(J0 + j) * nt + k] : 0.0;
+  }
+#pragma unroll
+  for (int j = 0; j < kGroup; ++j) acc[J0 + j] = a[0][j] + a[1][j];
+  if (J0 == 0) {
+#pragma unroll
+    for (int j = kGroup; j < kNumPart; ++j) acc[j] = 0.0;
+    mn = fmin(lane < nt ? a[0][kMinSlot] : kDblMax, lane + kSlice < nt ? a[1][kMinSlot] : kDblMax);
+  }
+  for (int k = lane + 2 * kSlice; k < nt; k += kSlice) {  // QPs of more than 128 tiles
+#pragma unroll
+    for (int j = 0; j < kGroup; ++j) {
+      const double v = gmem(pp)[(size_t)(J0 + j) * nt + k];
+      acc[J0 + j] += v;
+      if (J0 == 0 && j == kMinSlot) mn = fmin(mn, v);
+    }
   }
 }
 
@@ -943,32 +1050,17 @@ __global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(
   //    Wave 0 issues every load (one 8-byte state word per lane, the
   //    partials slot-major: [parity][slot][tile]) before the first wait.
   if (wave == 0) {
-    const StateWord w = reinterpret_cast<const StateWord *>(t.st + (parity ^ 1))[lane];
+    const StateWord w = gmem(reinterpret_cast<const StateWord *>(t.st + (parity ^ 1)))[lane];
     const int nt = t.nt;
     const double *pp = t.part + (size_t)(parity ^ 1) * kNumPart * nt;
-    double a[2][kNumPart];
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int k = lane + u * kSlice;
-#pragma unroll
-      for (int j = 0; j < kNumPart; ++j) a[u][j] = (k < nt) ? pp[(size_t)j * nt + k] : 0.0;
-    }
     double acc[kNumPart];
-    double mn = fmin(lane < nt ? a[0][kMinSlot] : kDblMax, lane + kSlice < nt ? a[1][kMinSlot] : kDblMax);
-#pragma unroll
-    for (int j = 0; j < kNumPart; ++j) acc[j] = a[0][j] + a[1][j];
-    for (int k = lane + 2 * kSlice; k < nt; k += kSlice) {  // QPs of more than 128 tiles
-#pragma unroll
-      for (int j = 0; j < kNumPart; ++j) {
-        const double v = pp[(size_t)j * nt + k];
-        acc[j] += v;
-        if (j == kMinSlot) mn = fmin(mn, v);
-      }
-    }
+    double mn;
+    sum_part<0>(pp, nt, lane, acc, mn);
     reinterpret_cast<StateWord *>(&sst)[lane] = w;
     // the op whose partials these are: the state's first word, held by lane 0
     const int prev = __builtin_amdgcn_readfirstlane((int)(uint32_t)w);
     const int ns = sum_slots(prev);
+    if (ns > kGroup) sum_part<kGroup>(pp, nt, lane, acc, mn);  // the wide ops' upper slots
     Sums S;
 #pragma unroll
     for (int j = 0; j < kNumPart; ++j) S.v[j] = (j < ns) ? wave_sum(acc[j]) : 0.0;
@@ -985,7 +1077,7 @@ __global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(
           sst.opcnt[sst.op]++;
         } else if (!was_done) {
           atomicSub(running, 1);
-          *q.done_flag = q.tag;
+          *gmem(q.done_flag) = q.tag;
         }
       }
     }
@@ -1001,7 +1093,7 @@ __global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(
     if (has_min) P[kMinSlot] = kDblMax;
     tile_work<TS, MX>(q, t, op, st, P);
     // wave then block reduction in a fixed order, over the op's slots only
-    const int ns = has_min ? kNumPart : sum_slots(op);
+    const int ns = has_min ? kMinSlot + 1 : sum_slots(op);
     if (ns > 0) {
 #pragma unroll
       for (int j = 0; j < kNumPart; ++j)
@@ -1016,14 +1108,14 @@ __global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(
         const bool m = has_min && j == kMinSlot;
         double acc = red[0][j];
         for (int w = 1; w < TS; ++w) acc = m ? fmin(acc, red[w][j]) : acc + red[w][j];
-        t.part[((size_t)parity * kNumPart + j) * t.nt + t.local] = acc;
+        gmem(t.part)[((size_t)parity * kNumPart + j) * t.nt + t.local] = acc;
       }
     }
   }
   // 2. the QP's first tile persists the state this launch executes (wave 0
   //    wrote it in the prologue; the other waves only read it)
   if (t.local == 0 && wave == 0)
-    reinterpret_cast<StateWord *>(t.st + parity)[lane] = reinterpret_cast<const StateWord *>(&sst)[lane];
+    gmem(reinterpret_cast<StateWord *>(t.st + parity))[lane] = reinterpret_cast<const StateWord *>(&sst)[lane];
 }
 
 // ------------------------------------------------------- helper kernels

@@ -16,7 +16,8 @@ from mprgp_amd import problems
 # op indices of the device state machine (mprgp_dev.h)
 OPS = ["DONE", "PROJX", "POW_MV", "POW_NRM", "POW_SAVE", "INIT", "TOP", "CG_AP", "CG_PY", "CG_Y",
        "CG_A", "CG_B", "EXP1", "EXP2", "EXP3", "PROP1", "PROP2", "FINAL", "CL_AP", "CL_CG",
-       "CL_EXP", "CL_EXP2", "CL_PROP1", "R_START", "R_UPD", "R_FINAL", "CG_PYG"]
+       "CL_EXP", "CL_EXP2", "CL_PROP1", "R_START", "R_UPD", "R_FINAL", "CG_PYG",
+       "CG_BAP"]
 
 
 class DeviceQP:
@@ -57,7 +58,8 @@ _STREAMS = {
     "CG_PY": ("rows", "x", "p", "y", "py"),
     "CG_Y": ("M", "py", "b"),
     "CG_A": ("rows", "face", "ap", "g", "dinv", "g", "phi", "z"),
-    "CG_PYG": ("rows", "face", "x", "p", "y", "py", "g", "ap", "dinv", "g", "phi", "z"),
+    "CG_PYG": ("rows", "face", "x", "p", "y", "py", "g", "ap", "dinv", "g", "phi", "z", "beta"),
+    "CG_BAP": ("M", "z", "p", "p", "ap"),
     "CG_B": ("rows", "face", "z", "p", "g", "phi", "x", "p", "beta"),
     "EXP1": ("rows", "x", "y"),
     "EXP2": ("M", "y", "b", "rows", "g", "py", "x", "face"),
@@ -77,8 +79,8 @@ _STREAMS = {
 MIXED_F32 = {"M", "b", "g", "phi", "beta", "p", "z", "ap"}
 # ops the mixed form's inner solve runs differently: CG_Y reads P(y) and y of
 # every row, and g and the matrix only on the few rows the projection moved
-# (not counted: a lower bound).  CG_PYG's read of b at CG entry is not
-# counted either.
+# (not counted: a lower bound).  CG_AP's reads of x and b at CG entry (the
+# mixed form's f(x)) are not counted either.
 MIXED_STREAMS = {"CG_Y": ("py", "y")}
 
 
