@@ -56,7 +56,7 @@ def pmc_child(args):
     rc, st = M.solve_batch(ctx, [q.qp for q in qps], M.params(tol=args.tol, max_it=0, power_tol=1e-6))
     prec = M.MIXED if args.precision == "mixed" else M.FP64
     per = [M.params(tol=args.tol, max_it=4000, alpha_bar=s["alpha_bar"], project_first=1, precision=prec,
-                    tiles=4, graph_steps=-16) for s in st]
+                    tiles=args.ts or 4, graph_steps=-16) for s in st]
     rc, stats = M.solve_batch(ctx, [q.qp for q in qps], per_qp=per)
     print(json.dumps({"launches": stats[0]["launches"]}), flush=True)
 
@@ -64,24 +64,25 @@ def pmc_child(args):
 COPY_BYTES = 1 << 28
 
 
-def measure_traffic(args):
-    """HBM bytes per launch of the step kernel in one batched solve: two
+def measure_traffic(args, tiles):
+    """HBM bytes per launch of the step kernel in one batched solve at the
+    headline's launch shape (``tiles`` wavefronts per workgroup): two
     rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over a child run of
     the workload, each scaled by what the same counter reads for the copy
-    probe (COPY_BYTES read and written per dispatch).  Runs before this
-    process touches the GPU.  Returns None when rocprofv3 is unavailable."""
+    probe (COPY_BYTES read and written per dispatch).  The children run while
+    this process waits.  Returns None when rocprofv3 is unavailable."""
     rocprof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
     if not os.path.exists(rocprof):
         return None
     tmp = tempfile.mkdtemp(prefix="mprgp_pmc_")
-    kernel = "mprgp_step<4, true>" if args.precision == "mixed" else "mprgp_step<4, false>"
+    kernel = f"mprgp_step<{tiles}, {'true' if args.precision == 'mixed' else 'false'}>"
 
     def run(counter, child, tag):
         d = os.path.join(tmp, tag)
         cmd = ["timeout", "-s", "KILL", "300", rocprof, "--pmc", counter, "--output-format", "csv",
                "-d", d, "-o", "run", "--", sys.executable, "-u", os.path.abspath(__file__),
                "--pmc-child", child, "--qps", str(args.qps), "--tol", str(args.tol),
-               "--precision", args.precision]
+               "--precision", args.precision, "--ts", str(tiles)]
         env = dict(os.environ, MPRGP_NO_GRAPH="1")
         r = subprocess.run(cmd, capture_output=True, text=True, env=env)
         if r.returncode != 0:
@@ -145,10 +146,6 @@ def main():
                           "GPUs with no communication (see DESIGN.md), but only --gpus 1 is run"}))
         sys.exit(2)
 
-    traffic = None if args.no_traffic else measure_traffic(args)
-    if traffic:
-        log(f"traffic per launch: {traffic}")
-
     import mprgp_amd as M
     from mprgp_amd import problems, workloads
 
@@ -205,6 +202,11 @@ def main():
     log(f"batched {args.precision}: {value:.1f} solves/s, {1e3 * total / args.steps:.2f} ms/step, "
         f"converged {conv}/{args.qps}")
     x_head = [q.qp.get_x() for q in qps]
+
+    tiles = shape["tiles"] if shape else (args.ts or 4)
+    traffic = None if args.no_traffic else measure_traffic(args, tiles)
+    if traffic:
+        log(f"traffic per launch (mprgp_step<{tiles}, ...>): {traffic}")
 
     # roofline of mprgp_step over the last batched solve: algorithmic bytes /
     # device time (HIP events on the solve stream around the launches)

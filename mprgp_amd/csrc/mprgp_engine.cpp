@@ -190,30 +190,27 @@ int mprgp_engine::round_once(long r, bool &idle) {
       n_tiles += sl.n_tiles;
       any_mixed |= sl.req->p.precision == MPRGP_MIXED;
     }
+  const int grid = round_grid(n_tiles);
   if (changed) {
-    if (int rc = h_tiles[par].reserve(sizeof(Tile) * (size_t)(n_tiles + 1), s)) return rc;
+    if (int rc = h_tiles[par].reserve(sizeof(Tile) * (size_t)grid, s)) return rc;
     // a larger tile table frees the old one and the graphs that captured it:
     // let the replays in flight finish first
-    if (n_tiles > tab.tiles_cap) MPRGP_HIP(hipStreamSynchronize(s));
-    if (int rc = tab.ensure(max_active, n_tiles)) return rc;
+    if (grid > tab.tiles_cap) MPRGP_HIP(hipStreamSynchronize(s));
+    if (int rc = tab.ensure(max_active, grid)) return rc;
     Tile *tl = (Tile *)h_tiles[par].p;
     std::vector<Tile> tv;
     tv.reserve(n_tiles);
     for (int i = 0; i < (int)slots.size(); ++i)
       if (slots[i].req && slots[i].retiring == 0) append_tiles(i, slots[i].req->q, ts, tv);
-    std::memcpy(tl, tv.data(), sizeof(Tile) * tv.size());
-    int *ctrl = (int *)(tl + n_tiles);
-    ctrl[0] = n_tiles;
+    layout_tiles(tv, grid, tl);
     std::memcpy(h_qps[par].p, devs.data(), sizeof(QpDev) * devs.size());
     MPRGP_HIP(hipMemcpyAsync(tab.d_qps, h_qps[par].p, sizeof(QpDev) * devs.size(), hipMemcpyHostToDevice, s));
-    if (n_tiles)
-      MPRGP_HIP(hipMemcpyAsync(tab.d_tiles, tl, sizeof(Tile) * n_tiles, hipMemcpyHostToDevice, s));
-    MPRGP_HIP(hipMemcpyAsync(tab.d_ctrl, ctrl, sizeof(int), hipMemcpyHostToDevice, s));
+    MPRGP_HIP(hipMemcpyAsync(tab.d_tiles, tl, sizeof(Tile) * grid, hipMemcpyHostToDevice, s));
   }
   // 2. replay
   if (n_tiles > 0) {
     hipGraphExec_t ex = nullptr;
-    if (int rc = tab.graph(s, ts, steps, round_grid(n_tiles), any_mixed, &ex)) return rc;
+    if (int rc = tab.graph(s, ts, steps, grid, any_mixed, &ex)) return rc;
     MPRGP_HIP(hipGraphLaunch(ex, s));
     launches += steps;
   }

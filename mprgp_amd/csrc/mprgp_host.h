@@ -33,7 +33,7 @@ int fail(int code, const char *fmt, ...);
 
 // kernel launch wrappers (mprgp_kernels.hip)
 hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
-                       const int *n_tiles, int *running, int parity, bool mixed);
+                       int *running, int parity, bool mixed);
 hipError_t launch_gather_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n);
 hipError_t launch_scatter_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n);
 hipError_t launch_setup_diag(hipStream_t s, const int *sl_off, const int *col, const double *val,
@@ -56,15 +56,15 @@ struct BatchTables {
   mprgp_ctx *ctx = nullptr;
   QpDev *d_qps = nullptr;
   int qps_cap = 0;
-  Tile *d_tiles = nullptr;
+  Tile *d_tiles = nullptr;  // one slot per workgroup of a launch (layout_tiles)
   int tiles_cap = 0;
-  int *d_ctrl = nullptr;   // [0] = tile count, [1] = running QPs
+  int *d_ctrl = nullptr;   // [1] = running QPs
   int *d_flags = nullptr;  // per batch slot: tag of the QP that reached DONE
   int flags_cap = 0;
   std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
   std::map<std::tuple<int, int, int, int>, hipGraph_t> graph_defs;
   int init(mprgp_ctx *c);
-  int ensure(int n_qps, int n_tiles);
+  int ensure(int n_qps, int n_slots);
   int graph(hipStream_t s, int ts, int steps, int grid, bool mixed, hipGraphExec_t *out);
   void drop_graphs();
   void destroy();
@@ -76,6 +76,7 @@ int pick_ts(const mprgp_params &p, long total_slices);
 void append_tiles(int slot, const mprgp_qp *q, int span, std::vector<Tile> &tiles);
 int tiles_for(int n_slices, int span);
 int round_grid(int n_tiles);
+void layout_tiles(const std::vector<Tile> &tiles, int grid, Tile *table);
 long solve_step_bound(const mprgp_params &p);
 size_t solve_bytes(const mprgp_qp *q, const mprgp_params &p);
 int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, int *done_flag, int tag,

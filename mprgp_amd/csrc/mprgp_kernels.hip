@@ -1034,17 +1034,15 @@ typedef uint64_t __attribute__((__may_alias__)) StateWord;
 template <int TS, bool MX>
 __global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(const QpDev *__restrict__ qps,
                                                            const Tile *__restrict__ tiles,
-                                                           const int *__restrict__ n_tiles_dev,
                                                            int *running, int parity) {
   __shared__ double red[TS][kNumPart];
   __shared__ QpState sst;
   const int wave = threadIdx.x / kSlice;
   const int lane = threadIdx.x % kSlice;
-  // one tile per workgroup: the grid is a power of two >= the tile count
-  // (graphs replay with a shrinking table) and the surplus workgroups exit
-  const int ti = blockIdx.x;
-  if (ti >= *n_tiles_dev) return;
-  const Tile t = tiles[ti];
+  // one tile per workgroup: the table has one slot per workgroup, laid out
+  // for the XCDs, and the workgroups of unused slots exit
+  const Tile t = tiles[blockIdx.x];
+  if (t.qp < 0) return;
   const QpDev &q = qps[t.qp];
   // 1. state the previous launch executed + its partials -> this launch's op.
   //    Wave 0 issues every load (one 8-byte state word per lane, the
@@ -1224,21 +1222,21 @@ __global__ void __launch_bounds__(256) spmv_plain(const int *__restrict__ sl_off
 
 template <bool MX>
 static hipError_t launch_step_t(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
-                                const int *n_tiles, int *running, int parity) {
+                                int *running, int parity) {
   switch (ts) {
-    case 1: mprgp_step<1, MX><<<grid, 1 * kSlice, 0, s>>>(qps, tiles, n_tiles, running, parity); break;
-    case 2: mprgp_step<2, MX><<<grid, 2 * kSlice, 0, s>>>(qps, tiles, n_tiles, running, parity); break;
-    case 4: mprgp_step<4, MX><<<grid, 4 * kSlice, 0, s>>>(qps, tiles, n_tiles, running, parity); break;
-    case 8: mprgp_step<8, MX><<<grid, 8 * kSlice, 0, s>>>(qps, tiles, n_tiles, running, parity); break;
+    case 1: mprgp_step<1, MX><<<grid, 1 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
+    case 2: mprgp_step<2, MX><<<grid, 2 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
+    case 4: mprgp_step<4, MX><<<grid, 4 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
+    case 8: mprgp_step<8, MX><<<grid, 8 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
 hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
-                       const int *n_tiles, int *running, int parity, bool mixed) {
-  return mixed ? launch_step_t<true>(ts, grid, s, qps, tiles, n_tiles, running, parity)
-               : launch_step_t<false>(ts, grid, s, qps, tiles, n_tiles, running, parity);
+                       int *running, int parity, bool mixed) {
+  return mixed ? launch_step_t<true>(ts, grid, s, qps, tiles, running, parity)
+               : launch_step_t<false>(ts, grid, s, qps, tiles, running, parity);
 }
 
 hipError_t launch_gather_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n) {
