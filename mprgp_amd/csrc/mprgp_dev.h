@@ -34,9 +34,7 @@ enum Op : int {
   OP_TOP,        // phi, beta, |phi+beta|^2, beta.beta, phitphi
   OP_CG_AP,      // AP = A p, p.AP, z.g
   OP_CG_PY,      // y = x - alpha p, py = P(y), feasibility of y
-  OP_CG_Y,       // A py (and A x at CG entry): objective values; the mixed
-                 // form's inner solve skips it when y is feasible and otherwise
-                 // forms only d'A d over the rows P moved (d = P(y) - y)
+  OP_CG_Y,       // A py (and A x at CG entry): objective values
   OP_CG_A,       // (x = y by buffer swap) g -= alpha AP, phi, z, z.AP
   OP_CG_B,       // p = z - beta p, beta, |phi+beta|^2, beta.beta, phitphi
   OP_EXP1,       // y = P(x)
@@ -53,8 +51,8 @@ enum Op : int {
   OP_R_START,    // mixed: FP64 g = A x - b, FP64 KKT norm, correction QP set-up
   OP_R_UPD,      // mixed: x += d (FP64)
   OP_R_FINAL,    // mixed: x = P(x) in FP64, active count
-  OP_CG_PYG,     // mixed inner CG step in one pass: y = x - alpha p, py = P(y),
-                 // feasibility, p.g (and f(x) at CG entry), and the gradient
+  OP_CG_PYG,     // mixed inner CG step in one pass: y = x - alpha p,
+                 // feasibility, p.g and f(P(y)) - f(y), and the gradient
                  // update of an accepted step applied in place: g -= alpha AP,
                  // phi, z, z.AP, z.g; then beta into the other beta buffer,
                  // |phi+beta|^2, beta.beta, phitphi
@@ -104,7 +102,6 @@ struct alignas(128) QpState {
   double alpha_f, nu, power_eps, lambda_max;
   double tol_outer, res64, inner_rel, best_res;
   double pg, fx_entry;      // p.g of the CG step; f(x) at CG entry (mixed inner)
-  double f_trial;           // f(y) of the CG trial point (mixed inner)
   double zap;               // z.AP after the in-place gradient update (mixed inner)
   double res_next;          // KKT residual at the CG trial point (mixed inner)
   int opcnt[OP_NUM];  // launches this QP spent in each op

@@ -257,7 +257,7 @@ constexpr int kMinSlot = 3;
 __device__ __forceinline__ int sum_slots(int op) {
   switch (op) {
     case OP_CG_PYG:
-      return 7;
+      return 8;
     case OP_TOP: case OP_CG_PY: case OP_CG_B: case OP_CG_AP:
       return 3;
     case OP_CG_Y:
@@ -421,9 +421,8 @@ __device__ void control(QpState &st, const Sums &S) {
       st.op = OP_CG_Y;
       break;
     case OP_CG_PYG: {
-      // mixed inner solve: f(y) = f(x) - alpha p'g + alpha^2 p'Ap/2 from the
-      // step's own sums; when P(y) = y that is f(P(y)), otherwise CG_Y adds
-      // the projected rows' share
+      // mixed inner solve: f(P(y)) = f(x) - alpha p'g + alpha^2 p'Ap/2 + the
+      // projected rows' share, all from the step's own sums
       st.y_valid = 1;
       st.y_feas = S.v[0] == 0.0;
       st.pg = S.v[1];
@@ -432,14 +431,11 @@ __device__ void control(QpState &st, const Sums &S) {
       st.res_next = sqrt(S.v[4]);
       st.prop_next = proportional(st, S.v[5], S.v[6]);
       const double f0 = st.cg_entry ? st.fx_entry : st.fy;
-      st.f_trial = f0 - st.alpha * st.pg + 0.5 * st.alpha * st.alpha * st.pd;
-      if (st.y_feas) cg_decide(st, st.f_trial, st.fx_entry);
-      else st.op = OP_CG_Y;
+      cg_decide(st, f0 - st.alpha * st.pg + 0.5 * st.alpha * st.alpha * st.pd + S.v[7], st.fx_entry);
       break;
     }
     case OP_CG_Y:
-      if (st.mixed) cg_decide(st, st.f_trial + S.v[1] + 0.5 * S.v[0], st.fx_entry);
-      else cg_decide(st, 0.5 * S.v[0] - S.v[1], 0.5 * S.v[2] - S.v[3]);
+      cg_decide(st, 0.5 * S.v[0] - S.v[1], 0.5 * S.v[2] - S.v[3]);
       break;
     case OP_CG_A:
       st.beta_cg = S.v[0] / st.pd;
@@ -673,8 +669,35 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       const V3 xr = ld3(xc, r), pr = ld3(v.p, r);
       const V3 y = axmy(xr, st.alpha, pr);
       st3(yc, r, y);
-      st3(v.py, r, proj(R, y));
-      P[0] += infeasible(R, y);
+      // f(P(y)) = f(y) + d'(A y - b) + d'A d/2 for d = P(y) - y
+      // (projectFunValue, MPRGPSolver.h:245-249), with d taken on the rows
+      // that violate their plane by more than the feasibility tolerance
+      // (isFeasible's eps): the rows on their plane, whose round-off P would
+      // move by ~1e-17, count as feasible.  A violated row gathers every
+      // neighbour's d by the same rule, formed from its x, p and row (none of
+      // which this launch writes, all loaded at once); no other row reads the
+      // matrix.
+      const bool viol = infeasible(R, y) != 0.0;
+      P[0] += viol ? 1.0 : 0.0;
+      V3 dr{0.0, 0.0, 0.0};
+      double dad = 0.0;
+      if (viol) {
+        const double lam = (R.c - dot3(R.n, y)) / R.jj;
+        dr = V3{R.n.x * lam, R.n.y * lam, R.n.z * lam};
+        const double *xx = xc;
+        const T *pp = v.p;
+        const double alpha = st.alpha;
+        V3 acc[1];
+        spmv_row<1, 1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) {
+          const V3 yj = axmy(ld3(xx, j), alpha, ld3(pp, j));
+          const V3 nj = ld3(v.cn, j);
+          const double vj = gmem(v.cc)[j] - dot3(nj, yj);
+          const bool on = gmem(q.cf)[j] != 0 && vj > kEps;
+          const double lj = vj / gmem(v.cj)[j];
+          xs[0] = on ? V3{nj.x * lj, nj.y * lj, nj.z * lj} : V3{0.0, 0.0, 0.0};
+        }, acc);
+        dad = dot3(dr, acc[0]);
+      }
       const V3 gr = ld3(v.g, r), ap = ld3(v.ap, r);
       P[1] += dot3(pr, gr);
       const V3 g = axmy(gr, st.alpha, ap);
@@ -697,6 +720,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       P[5] += dot3(be, be);
       const V3 px = proj(R, axmy(y, st.alpha_bar, phs));
       P[6] += dot3(sub(y, px), phs);
+      P[7] += dot3(dr, gs) + 0.5 * dad;  // A y - b = g - alpha AP, as stored
       break;
     }
     case OP_CG_BAP: {
@@ -718,22 +742,6 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       break;
     }
     case OP_CG_Y: {
-      if (mixed) {
-        // mixed inner solve: with d = P(y) - y, nonzero only on the rows the
-        // projection moved, f(P(y)) = f(y) + d'(A y - b) + d'A d/2 where
-        // A y - b = g - alpha A p, the gradient CG_PYG left in g; every other
-        // row reads no matrix at all
-        const V3 dr = sub(ld3(v.py, r), ld3(yc, r));
-        if (dr.x != 0.0 || dr.y != 0.0 || dr.z != 0.0) {
-          P[1] += dot3(dr, ld3(v.g, r));
-          const double *py = v.py, *yy = yc;
-          V3 acc[1];
-          spmv_row<1, 1>(q.sl_off, q.col, v.val, s, lane,
-                         [&](int j, V3 (&xs)[1]) { xs[0] = sub(ld3(py, j), ld3(yy, j)); }, acc);
-          P[0] += dot3(dr, acc[0]);
-        }
-        break;
-      }
       const double *py = v.py;
       V3 pyr = ld3(v.py, r);
       V3 br = ld3(v.b, r);

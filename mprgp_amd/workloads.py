@@ -58,7 +58,7 @@ _STREAMS = {
     "CG_PY": ("rows", "x", "p", "y", "py"),
     "CG_Y": ("M", "py", "b"),
     "CG_A": ("rows", "face", "ap", "g", "dinv", "g", "phi", "z"),
-    "CG_PYG": ("rows", "face", "x", "p", "y", "py", "g", "ap", "dinv", "g", "phi", "z", "beta"),
+    "CG_PYG": ("rows", "face", "x", "p", "y", "g", "ap", "dinv", "g", "phi", "z", "beta"),
     "CG_BAP": ("M", "z", "p", "p", "ap"),
     "CG_B": ("rows", "face", "z", "p", "g", "phi", "x", "p", "beta"),
     "EXP1": ("rows", "x", "y"),
@@ -77,11 +77,9 @@ _STREAMS = {
 # FP32 streams of the mixed form's inner solve; everything else (x, y, P(y),
 # 1/diag, the rows) is FP64.
 MIXED_F32 = {"M", "b", "g", "phi", "beta", "p", "z", "ap"}
-# ops the mixed form's inner solve runs differently: CG_Y reads P(y) and y of
-# every row, and g and the matrix only on the few rows the projection moved
-# (not counted: a lower bound).  CG_AP's reads of x and b at CG entry (the
-# mixed form's f(x)) are not counted either.
-MIXED_STREAMS = {"CG_Y": ("py", "y")}
+# Not counted (a lower bound): CG_PYG's matrix and neighbour reads on the few
+# rows the projection moves, and CG_AP's reads of x and b at CG entry (the
+# mixed form's f(x)).
 
 
 def spmv_bytes(N, nb, s=8):
@@ -95,7 +93,7 @@ def op_bytes(op, N, nb, m, mixed=False, precond=True):
     FP32 matrix and FP32 direction vectors."""
     name = OPS[op]
     total = 0
-    streams = MIXED_STREAMS.get(name, _STREAMS.get(name, ())) if mixed else _STREAMS.get(name, ())
+    streams = _STREAMS.get(name, ())
     for st in streams:
         if st == "rows":
             total += N + 48 * m

@@ -35,8 +35,8 @@ def test_byte_accounting():
     assert workloads.op_bytes(op("CG_AP"), N, nb, m) == mat64 + 4 * v64
     assert workloads.op_bytes(op("CG_AP"), N, nb, m, True) == mat32 + 4 * v32  # all FP32 in the inner solve
     # the mixed form's fused CG step: rows (flag byte + 6 words per row), face,
-    # x, y, P(y) and 1/diag in FP64; p, g read and written, AP, phi, z, beta in FP32
-    assert workloads.op_bytes(op("CG_PYG"), N, nb, m, True) == (N + 48 * m) + N + 4 * v64 + 7 * v32
+    # x, y and 1/diag in FP64; p, g read and written, AP, phi, z, beta in FP32
+    assert workloads.op_bytes(op("CG_PYG"), N, nb, m, True) == (N + 48 * m) + N + 3 * v64 + 7 * v32
     # no preconditioner: no D^-1 stream
     assert (workloads.op_bytes(op("CG_A"), N, nb, m) - workloads.op_bytes(op("CG_A"), N, nb, m, precond=False)
             == v64)
