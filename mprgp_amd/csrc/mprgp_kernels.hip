@@ -166,49 +166,49 @@ __device__ __forceinline__ void spmv_row(const int *sl_off, const int *col, cons
   int k = 0;
   for (; k + U <= W; k += U) {
     int j[U];
-    double a[U][9];
+    T a[U][9];  // kept in the storage precision until its multiply
     V3 xs[U][NV];
 #pragma unroll
     for (int u = 0; u < U; ++u) j[u] = cp[(k + u) * kSlice];
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
-      for (int e = 0; e < 9; ++e) a[u][e] = (double)vp[(size_t)(k + u) * (9 * kSlice) + e * kSlice];
+      for (int e = 0; e < 9; ++e) a[u][e] = vp[(size_t)(k + u) * (9 * kSlice) + e * kSlice];
 #pragma unroll
     for (int u = 0; u < U; ++u) gather(j[u], xs[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
       for (int t = 0; t < NV; ++t) {
-        acc[t].x += a[u][0] * xs[u][t].x;
-        acc[t].x += a[u][1] * xs[u][t].y;
-        acc[t].x += a[u][2] * xs[u][t].z;
-        acc[t].y += a[u][3] * xs[u][t].x;
-        acc[t].y += a[u][4] * xs[u][t].y;
-        acc[t].y += a[u][5] * xs[u][t].z;
-        acc[t].z += a[u][6] * xs[u][t].x;
-        acc[t].z += a[u][7] * xs[u][t].y;
-        acc[t].z += a[u][8] * xs[u][t].z;
+        acc[t].x += (double)a[u][0] * xs[u][t].x;
+        acc[t].x += (double)a[u][1] * xs[u][t].y;
+        acc[t].x += (double)a[u][2] * xs[u][t].z;
+        acc[t].y += (double)a[u][3] * xs[u][t].x;
+        acc[t].y += (double)a[u][4] * xs[u][t].y;
+        acc[t].y += (double)a[u][5] * xs[u][t].z;
+        acc[t].z += (double)a[u][6] * xs[u][t].x;
+        acc[t].z += (double)a[u][7] * xs[u][t].y;
+        acc[t].z += (double)a[u][8] * xs[u][t].z;
       }
   }
   for (; k < W; ++k) {
     const int j = cp[k * kSlice];
-    double a[9];
+    T a[9];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) a[e] = (double)vp[(size_t)k * (9 * kSlice) + e * kSlice];
+    for (int e = 0; e < 9; ++e) a[e] = vp[(size_t)k * (9 * kSlice) + e * kSlice];
     V3 xs[NV];
     gather(j, xs);
 #pragma unroll
     for (int t = 0; t < NV; ++t) {
-      acc[t].x += a[0] * xs[t].x;
-      acc[t].x += a[1] * xs[t].y;
-      acc[t].x += a[2] * xs[t].z;
-      acc[t].y += a[3] * xs[t].x;
-      acc[t].y += a[4] * xs[t].y;
-      acc[t].y += a[5] * xs[t].z;
-      acc[t].z += a[6] * xs[t].x;
-      acc[t].z += a[7] * xs[t].y;
-      acc[t].z += a[8] * xs[t].z;
+      acc[t].x += (double)a[0] * xs[t].x;
+      acc[t].x += (double)a[1] * xs[t].y;
+      acc[t].x += (double)a[2] * xs[t].z;
+      acc[t].y += (double)a[3] * xs[t].x;
+      acc[t].y += (double)a[4] * xs[t].y;
+      acc[t].y += (double)a[5] * xs[t].z;
+      acc[t].z += (double)a[6] * xs[t].x;
+      acc[t].z += (double)a[7] * xs[t].y;
+      acc[t].z += (double)a[8] * xs[t].z;
     }
   }
 }
