@@ -449,7 +449,7 @@ extern "C" int mprgp_qp_create(mprgp_matrix *A, mprgp_qp **out) {
   q->A = A;
   q->n = A->n_nodes;
   const size_t n3 = 3 * (size_t)q->n;
-  const int n_vec = 15;      // x x0 b g p z phi beta y ap py ev pv pw stage
+  const int n_vec = 17;      // x x0 b g p z phi beta y ap py ev pv pw stage p2 beta2
   const int n_row = 3 + 3;   // cn(3) cc cj cs
   q->part_cap = (int)qp_tiles_max(q->n);
   size_t doubles = n_vec * n3 + n_row * (size_t)q->n + 2 * (size_t)q->part_cap * kNumPart;
@@ -462,7 +462,7 @@ extern "C" int mprgp_qp_create(mprgp_matrix *A, mprgp_qp **out) {
   q->bytes = bytes;
   double *p = q->base;
   double **vecs[] = {&q->x, &q->x0, &q->b, &q->g, &q->p, &q->z, &q->phi, &q->beta,
-                     &q->y, &q->ap, &q->py, &q->ev, &q->pv, &q->pw, &q->stage};
+                     &q->y, &q->ap, &q->py, &q->ev, &q->pv, &q->pw, &q->stage, &q->p2, &q->beta2};
   for (double **v : vecs) {
     *v = p;
     p += n3;
@@ -895,6 +895,7 @@ int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, i
   d.d.dinv = q->A->d_dinv;
   d.d.x = q->x; d.d.b = q->b; d.d.g = q->g; d.d.p = q->p; d.d.z = q->z; d.d.phi = q->phi;
   d.d.beta = q->beta; d.d.y = q->y; d.d.ap = q->ap; d.d.py = q->py;
+  d.d.p2 = q->p2; d.d.beta2 = q->beta2;
   d.d.cn = q->cn; d.d.cc = q->cc; d.d.cj = q->cj; d.d.cs = q->cs;
   d.ev = q->ev; d.pv = q->pv; d.pw = q->pw;
   d.cf = q->cf; d.face = q->face;

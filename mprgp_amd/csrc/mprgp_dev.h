@@ -35,8 +35,8 @@ enum Op : int {
   OP_CG_AP,      // AP = A p, p.AP, z.g
   OP_CG_PY,      // y = x - alpha p, py = P(y), feasibility of y
   OP_CG_Y,       // A py (and A x at CG entry): objective values
-  OP_CG_A,       // (x = y by buffer swap) g -= alpha AP, phi, z, z.AP
-  OP_CG_B,       // p = z - beta p, beta, |phi+beta|^2, beta.beta, phitphi
+  OP_CG_A,       // (x = y by buffer swap) g -= alpha AP, phi, z, z.AP, z.g,
+                 // beta, |phi+beta|^2, beta.beta, phitphi
   OP_EXP1,       // y = P(x)
   OP_EXP2,       // g = A y - b, face(y), phi, y' = y - abar phi, x = P(y')
   OP_EXP3,       // g = A x - b, face(x), phi, z, p = z, y = y'
@@ -56,8 +56,8 @@ enum Op : int {
                  // update of an accepted step applied in place: g -= alpha AP,
                  // phi, z, z.AP, z.g; then beta into the other beta buffer,
                  // |phi+beta|^2, beta.beta, phitphi
-  OP_CG_BAP,     // mixed inner: p' = z - beta p into the other p buffer and
-                 // AP = A p' with p' formed at the gather from z and p, p'.AP
+  OP_CG_BAP,     // p' = z - beta p into the other p buffer and AP = A p' with
+                 // p' formed at the gather from z and p, p'.AP
   OP_NUM
 };
 

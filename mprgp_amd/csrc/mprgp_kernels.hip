@@ -258,13 +258,15 @@ __device__ __forceinline__ int sum_slots(int op) {
   switch (op) {
     case OP_CG_PYG:
       return 8;
-    case OP_TOP: case OP_CG_PY: case OP_CG_B: case OP_CG_AP:
+    case OP_CG_A:
+      return 5;
+    case OP_TOP: case OP_CG_PY: case OP_CG_AP:
       return 3;
     case OP_CG_Y:
       return 4;
     case OP_CL_AP: case OP_PROP1: case OP_CL_PROP1:
       return 2;
-    case OP_POW_MV: case OP_POW_NRM: case OP_CG_A: case OP_EXP2: case OP_FINAL: case OP_CL_CG: case OP_CG_BAP:
+    case OP_POW_MV: case OP_POW_NRM: case OP_EXP2: case OP_FINAL: case OP_CL_CG: case OP_CG_BAP:
     case OP_R_START: case OP_R_FINAL:
       return 1;
     default:
@@ -439,19 +441,16 @@ __device__ void control(QpState &st, const Sums &S) {
       break;
     case OP_CG_A:
       st.beta_cg = S.v[0] / st.pd;
-      st.op = OP_CG_B;
+      st.zg = S.v[1];
+      st.residual = sqrt(S.v[2]);
+      st.prop = proportional(st, S.v[3], S.v[4]);
+      st.op = OP_CG_BAP;
       break;
     case OP_CG_BAP:
       st.pb ^= 1;  // p' went to the other buffer
       st.pd = S.v[0];
       st.alpha = st.zg / st.pd;
-      st.op = OP_CG_PYG;
-      break;
-    case OP_CG_B:
-      st.residual = sqrt(S.v[0]);
-      st.prop = proportional(st, S.v[1], S.v[2]);
-      // a stalled inner solve (mixed form) ends through the loop's exit path
-      st.op = inner_stalled(st) ? OP_FINAL : OP_CG_AP;
+      st.op = st.mixed ? OP_CG_PYG : OP_CG_PY;
       break;
     case OP_EXP1:
       st.op = OP_EXP2;
@@ -567,13 +566,13 @@ template <class T, bool MX>
 __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int s, int lane, int r, int op,
                                          const QpState &st, double (&P)[kNumPart]) {
   const bool mixed = MX && st.mixed;
-  // p and p2, beta and beta2 swap roles with st.pb, st.bb (mixed inner solve)
+  // p and p2, beta and beta2 swap roles with st.pb, st.bb
   Vecs<T> v = vin;
-  if (MX && st.pb) {
+  if (st.pb) {
     v.p = vin.p2;
     v.p2 = vin.p;
   }
-  if (MX && st.bb) {
+  if (st.bb) {
     v.beta = vin.beta2;
     v.beta2 = vin.beta;
   }
@@ -724,11 +723,10 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       break;
     }
     case OP_CG_BAP: {
-      // Mixed inner solve: p = z - beta p and the next step's A p
-      // (MPRGPSolver.h:298, 304-306) in one pass.  p' is formed again at
-      // every gather from z and p, which no tile writes in this launch (p'
-      // goes to the other p buffer), rounded as its store.
-      if (!MX) break;
+      // p = z - beta p and the next step's A p (MPRGPSolver.h:298, 304-306)
+      // in one pass.  p' is formed again at every gather from z and p, which
+      // no tile writes in this launch (p' goes to the other p buffer),
+      // rounded as its store.
       const T *zz = v.z, *pp = v.p;
       const double beta = st.beta_cg;
       V3 acc[1];
@@ -766,7 +764,8 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       break;
     }
     case OP_CG_A: {
-      // CGMonotonicStep body, first half (MPRGPSolver.h:293-297)
+      // CGMonotonicStep body (MPRGPSolver.h:293-302) but p = z - beta p,
+      // which needs beta first and goes with the next A p (CG_BAP)
       Row R = ldrow(q, v, r);
       bool f = gmem(q.face)[r];
       V3 ap = ld3(v.ap, r);  // x = y happened in the control (buffer swap)
@@ -777,23 +776,16 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       st3(v.phi, r, ph);
       st3(v.z, r, z);
       P[0] += dot3(z, ap);
-      break;
-    }
-    case OP_CG_B: {
-      // p = z - beta p ; computeGradients(g, false) ; proportional (MPRGPSolver.h:298-302)
-      Row R = ldrow(q, v, r);
-      bool f = gmem(q.face)[r];
-      st3(v.p, r, axmy(ld3(v.z, r), st.beta_cg, ld3(v.p, r)));
-      V3 g = ld3(v.g, r);
-      V3 ph = ld3(v.phi, r);
-      V3 be = beta_of(R, f, g, ph);
+      P[1] += dot3(z, g);  // the next step's alpha numerator
+      // computeGradients(g, false) and proportional at the new x
+      const V3 be = beta_of(R, f, g, ph);
       st3(v.beta, r, be);
-      V3 gp = add(ph, be);
-      P[0] += dot3(gp, gp);
-      P[1] += dot3(be, be);
-      V3 x = ld3(xc, r);
-      V3 px = proj(R, axmy(x, st.alpha_bar, ph));
-      P[2] += dot3(sub(x, px), ph);
+      const V3 gp = add(ph, be);
+      P[2] += dot3(gp, gp);
+      P[3] += dot3(be, be);
+      const V3 x = ld3(xc, r);
+      const V3 px = proj(R, axmy(x, st.alpha_bar, ph));
+      P[4] += dot3(sub(x, px), ph);
       break;
     }
     case OP_EXP1: {

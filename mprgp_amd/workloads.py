@@ -15,7 +15,7 @@ from mprgp_amd import problems
 
 # op indices of the device state machine (mprgp_dev.h)
 OPS = ["DONE", "PROJX", "POW_MV", "POW_NRM", "POW_SAVE", "INIT", "TOP", "CG_AP", "CG_PY", "CG_Y",
-       "CG_A", "CG_B", "EXP1", "EXP2", "EXP3", "PROP1", "PROP2", "FINAL", "CL_AP", "CL_CG",
+       "CG_A", "EXP1", "EXP2", "EXP3", "PROP1", "PROP2", "FINAL", "CL_AP", "CL_CG",
        "CL_EXP", "CL_EXP2", "CL_PROP1", "R_START", "R_UPD", "R_FINAL", "CG_PYG",
        "CG_BAP"]
 
@@ -57,10 +57,9 @@ _STREAMS = {
     "CL_AP": ("M", "p", "ap", "z", "g", "rows", "x"),
     "CG_PY": ("rows", "x", "p", "y", "py"),
     "CG_Y": ("M", "py", "b"),
-    "CG_A": ("rows", "face", "ap", "g", "dinv", "g", "phi", "z"),
+    "CG_A": ("rows", "face", "ap", "g", "dinv", "x", "g", "phi", "z", "beta"),
     "CG_PYG": ("rows", "face", "x", "p", "y", "g", "ap", "dinv", "g", "phi", "z", "beta"),
     "CG_BAP": ("M", "z", "p", "p", "ap"),
-    "CG_B": ("rows", "face", "z", "p", "g", "phi", "x", "p", "beta"),
     "EXP1": ("rows", "x", "y"),
     "EXP2": ("M", "y", "b", "rows", "g", "py", "x", "face"),
     "PROP1": ("M", "x", "beta", "b", "g", "ap"),
