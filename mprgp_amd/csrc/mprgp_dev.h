@@ -58,6 +58,8 @@ enum Op : int {
                  // |phi+beta|^2, beta.beta, phitphi
   OP_CG_BAP,     // p' = z - beta p into the other p buffer and AP = A p' with
                  // p' formed at the gather from z and p, p'.AP
+  OP_CG_YA,      // FP64, after the first CG step: CG_Y, then CG_A applied
+                 // speculatively (beta into the other beta buffer)
   OP_NUM
 };
 

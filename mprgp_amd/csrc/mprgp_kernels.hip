@@ -258,6 +258,8 @@ __device__ __forceinline__ int sum_slots(int op) {
   switch (op) {
     case OP_CG_PYG:
       return 8;
+    case OP_CG_YA:
+      return 7;
     case OP_CG_A:
       return 5;
     case OP_TOP: case OP_CG_PY: case OP_CG_AP:
@@ -308,7 +310,10 @@ __device__ __forceinline__ bool inner_stalled(QpState &st) {
 
 // CGMonotonicStep's acceptance test (MPRGPSolver.h:284-292) once f(P(y)) is
 // known: fnew = f(P(y)), fx_entry = f(x) when the CG loop is entered.
-__device__ __forceinline__ void cg_decide(QpState &st, double fnew, double fx_entry) {
+// applied: the trial launch already applied the accepted step's update
+// (g -= alpha AP, phi, z in place, beta into the other buffer) and reduced
+// its sums (CG_PYG, CG_YA); otherwise CG_A does that after acceptance.
+__device__ __forceinline__ void cg_decide(QpState &st, double fnew, double fx_entry, bool applied) {
   if (st.cg_entry) {
     st.fx = fx_entry;
     st.fy = fnew;
@@ -322,9 +327,9 @@ __device__ __forceinline__ void cg_decide(QpState &st, double fnew, double fx_en
   }
   if (st.prop && st.residual > st.tol && st.fy <= st.fx && st.iteration < st.max_it) {
     st.xb ^= 1;  // x = y: the trial point's buffer becomes the iterate's
-    if (st.mixed) {
-      // CG_PYG already applied g -= alpha AP, phi, z and computed beta and
-      // the KKT and proportioning sums at y (MPRGPSolver.h:293-302)
+    if (applied) {
+      // g -= alpha AP, phi, z, beta and the KKT and proportioning sums at y
+      // are done (MPRGPSolver.h:293-302)
       st.bb ^= 1;
       st.beta_cg = st.zap / st.pd;
       st.residual = st.res_next;
@@ -420,7 +425,8 @@ __device__ void control(QpState &st, const Sums &S) {
     case OP_CG_PY:
       st.y_valid = 1;
       st.y_feas = S.v[0] == 0.0;
-      st.op = OP_CG_Y;
+      // the CG entry needs f(x) too (four sums): its step takes CG_Y, CG_A
+      st.op = st.cg_entry ? OP_CG_Y : OP_CG_YA;
       break;
     case OP_CG_PYG: {
       // mixed inner solve: f(P(y)) = f(x) - alpha p'g + alpha^2 p'Ap/2 + the
@@ -433,11 +439,18 @@ __device__ void control(QpState &st, const Sums &S) {
       st.res_next = sqrt(S.v[4]);
       st.prop_next = proportional(st, S.v[5], S.v[6]);
       const double f0 = st.cg_entry ? st.fx_entry : st.fy;
-      cg_decide(st, f0 - st.alpha * st.pg + 0.5 * st.alpha * st.alpha * st.pd + S.v[7], st.fx_entry);
+      cg_decide(st, f0 - st.alpha * st.pg + 0.5 * st.alpha * st.alpha * st.pd + S.v[7], st.fx_entry, true);
       break;
     }
     case OP_CG_Y:
-      cg_decide(st, 0.5 * S.v[0] - S.v[1], 0.5 * S.v[2] - S.v[3]);
+      cg_decide(st, 0.5 * S.v[0] - S.v[1], 0.5 * S.v[2] - S.v[3], false);
+      break;
+    case OP_CG_YA:
+      st.zap = S.v[2];
+      st.zg = S.v[3];  // z.g after the update: the next step's alpha numerator
+      st.res_next = sqrt(S.v[4]);
+      st.prop_next = proportional(st, S.v[5], S.v[6]);
+      cg_decide(st, 0.5 * S.v[0] - S.v[1], 0.0, true);  // fx_entry: not the CG entry
       break;
     case OP_CG_A:
       st.beta_cg = S.v[0] / st.pd;
@@ -761,6 +774,40 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
         P[0] += dot3(pyr, acc[0]);
         P[1] += dot3(pyr, br);
       }
+      break;
+    }
+    case OP_CG_YA: {
+      // CGMonotonicStep (MPRGPSolver.h:283-302) after the CG entry: f(P(y))
+      // by A P(y) as CG_Y, then CG_A's update applied speculatively at x = y.
+      // A rejected step leaves the CG loop through EXP1/EXP2, PROP1 or FINAL,
+      // and the first two recompute g, phi and z before reading them
+      // (MPRGPSolver.h:255-261, 435-437); FINAL reads x only; beta goes to
+      // the other buffer, current only once the step is accepted.
+      const double *py = v.py;
+      V3 acc[1];
+      spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(py, j); }, acc);
+      const V3 pyr = ld3(v.py, r);
+      P[0] += dot3(pyr, acc[0]);
+      P[1] += dot3(pyr, ld3(v.b, r));
+      Row R = ldrow(q, v, r);
+      const bool f = gmem(q.face)[r];
+      const V3 ap = ld3(v.ap, r);
+      const V3 g = axmy(ld3(v.g, r), st.alpha, ap);
+      const V3 ph = phi_of(R, f, g);
+      const V3 z = precond_of(v, pre, R, f, r, g, ph);
+      st3(v.g, r, g);
+      st3(v.phi, r, ph);
+      st3(v.z, r, z);
+      P[2] += dot3(z, ap);
+      P[3] += dot3(z, g);
+      const V3 be = beta_of(R, f, g, ph);
+      st3(v.beta2, r, be);
+      const V3 gp = add(ph, be);
+      P[4] += dot3(gp, gp);
+      P[5] += dot3(be, be);
+      const V3 y = ld3(yc, r);
+      const V3 px = proj(R, axmy(y, st.alpha_bar, ph));
+      P[6] += dot3(sub(y, px), ph);
       break;
     }
     case OP_CG_A: {

@@ -17,7 +17,7 @@ from mprgp_amd import problems
 OPS = ["DONE", "PROJX", "POW_MV", "POW_NRM", "POW_SAVE", "INIT", "TOP", "CG_AP", "CG_PY", "CG_Y",
        "CG_A", "EXP1", "EXP2", "EXP3", "PROP1", "PROP2", "FINAL", "CL_AP", "CL_CG",
        "CL_EXP", "CL_EXP2", "CL_PROP1", "R_START", "R_UPD", "R_FINAL", "CG_PYG",
-       "CG_BAP"]
+       "CG_BAP", "CG_YA"]
 
 
 class DeviceQP:
@@ -60,6 +60,7 @@ _STREAMS = {
     "CG_A": ("rows", "face", "ap", "g", "dinv", "x", "g", "phi", "z", "beta"),
     "CG_PYG": ("rows", "face", "x", "p", "y", "g", "ap", "dinv", "g", "phi", "z", "beta"),
     "CG_BAP": ("M", "z", "p", "p", "ap"),
+    "CG_YA": ("M", "py", "b", "rows", "face", "ap", "g", "dinv", "y", "g", "phi", "z", "beta"),
     "EXP1": ("rows", "x", "y"),
     "EXP2": ("M", "y", "b", "rows", "g", "py", "x", "face"),
     "PROP1": ("M", "x", "beta", "b", "g", "ap"),
