@@ -943,8 +943,10 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
   }
 }
 
-// Ops on the FP64 data only: projection of the warm start, the power
-// method, and the outer refinement of the mixed form.
+// Ops on FP64 vectors in both forms: projection of the warm start, the power
+// method (its SpMV reads the matrix of the QP's form), and the outer
+// refinement of the mixed form.
+template <bool MX>
 __device__ __forceinline__ void row_work_fp64(const QpDev &q, int s, int lane, int r, int op,
                                               const QpState &st, double (&P)[kNumPart]) {
   const Vecs<double> &v = q.d;
@@ -955,11 +957,16 @@ __device__ __forceinline__ void row_work_fp64(const QpDev &q, int s, int lane, i
       break;
     }
     case OP_POW_MV: {
-      // v and w alternate between pv and pw with the iteration parity
+      // v and w alternate between pv and pw with the iteration parity.  The
+      // mixed form multiplies by the FP32 copy of A, the matrix its inner
+      // solve iterates with (vectors and norms stay FP64).
       const double *pv = (st.power_it & 1) ? q.pw : q.pv;
       double *w = (st.power_it & 1) ? q.pv : q.pw;
       V3 acc[1];
-      spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(pv, j); }, acc);
+      if (MX && st.mixed)
+        spmv_row<1>(q.sl_off, q.col, q.f.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(pv, j); }, acc);
+      else
+        spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(pv, j); }, acc);
       st3(w, r, acc[0]);
       P[0] += dot3(acc[0], acc[0]);
       break;
@@ -1030,7 +1037,7 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
   const int r = s * kSlice + lane;
   if (wave >= t.slices || r >= q.n_nodes) return;
   if (fp64_op(op)) {
-    row_work_fp64(q, s, lane, r, op, st, P);
+    row_work_fp64<MX>(q, s, lane, r, op, st, P);
   } else if (MX && st.mixed) {
     row_work<float, MX>(q, q.f, s, lane, r, op, st, P);
   } else {

@@ -46,7 +46,7 @@ class DeviceQP:
 # row; "face" one byte per node.
 _STREAMS = {
     "PROJX": ("rows", "x", "x"),
-    "POW_MV": ("M64", "pv", "pw"),
+    "POW_MV": ("M", "pv", "pw"),
     "POW_NRM": ("pw", "pv", "pw"),
     "POW_SAVE": ("pv", "ev"),
     "INIT": ("M", "x", "b", "rows", "dinv", "g", "phi", "z", "p", "face"),
