@@ -449,7 +449,7 @@ extern "C" int mprgp_qp_create(mprgp_matrix *A, mprgp_qp **out) {
   q->A = A;
   q->n = A->n_nodes;
   const size_t n3 = 3 * (size_t)q->n;
-  const int n_vec = 17;      // x x0 b g p z phi beta y ap py ev pv pw stage p2 beta2
+  const int n_vec = 19;      // x x0 b g p z phi beta y ap py ev pv pw stage p2 beta2 pu pz
   const int n_row = 3 + 3;   // cn(3) cc cj cs
   q->part_cap = (int)qp_tiles_max(q->n);
   size_t doubles = n_vec * n3 + n_row * (size_t)q->n + 2 * (size_t)q->part_cap * kNumPart;
@@ -462,7 +462,8 @@ extern "C" int mprgp_qp_create(mprgp_matrix *A, mprgp_qp **out) {
   q->bytes = bytes;
   double *p = q->base;
   double **vecs[] = {&q->x, &q->x0, &q->b, &q->g, &q->p, &q->z, &q->phi, &q->beta,
-                     &q->y, &q->ap, &q->py, &q->ev, &q->pv, &q->pw, &q->stage, &q->p2, &q->beta2};
+                     &q->y, &q->ap, &q->py, &q->ev, &q->pv, &q->pw, &q->stage, &q->p2, &q->beta2,
+                     &q->pu, &q->pz};
   for (double **v : vecs) {
     *v = p;
     p += n3;
@@ -897,7 +898,7 @@ int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, i
   d.d.beta = q->beta; d.d.y = q->y; d.d.ap = q->ap; d.d.py = q->py;
   d.d.p2 = q->p2; d.d.beta2 = q->beta2;
   d.d.cn = q->cn; d.d.cc = q->cc; d.d.cj = q->cj; d.d.cs = q->cs;
-  d.ev = q->ev; d.pv = q->pv; d.pw = q->pw;
+  d.ev = q->ev; d.pv = q->pv; d.pw = q->pw; d.pu = q->pu; d.pz = q->pz;
   d.cf = q->cf; d.face = q->face;
   const bool mixed = p.precision == MPRGP_MIXED;
   if (mixed) {
@@ -923,7 +924,7 @@ int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, i
   st.alpha_bar = power ? 0.0 : p.alpha_bar;
   st.lambda_max = power ? 0.0 : 2.0 / p.alpha_bar;
   st.power_eps = p.power_tol > 0.0 ? p.power_tol : p.tol;
-  st.pow_next = power ? OP_POW_MV : (mixed ? OP_R_START : OP_INIT);
+  st.pow_next = power ? OP_POW_IT : (mixed ? OP_R_START : OP_INIT);
   st.op = p.project_first ? OP_PROJX : st.pow_next;
   st.fresh = 1;
   if (mixed) {

@@ -27,8 +27,8 @@ constexpr double kEps = 1e-9;    // ScalarUtil<double>::scalar_eps of the simula
 enum Op : int {
   OP_DONE = 0,
   OP_PROJX,      // x = P(x) before the solve (J/c facade overload)
-  OP_POW_MV,     // power method: w = A v, |w|^2
-  OP_POW_NRM,    // w /= nu, |w - v|^2
+  OP_POW_IT,     // power iteration k: v_k = w_{k-1}/nu_{k-1} (formed at the
+                 // gather), |v_k - v_{k-1}|^2, w_k = A v_k, |w_k|^2
   OP_POW_SAVE,   // ev = v
   OP_INIT,       // g = Ax - b, face(x), phi, z, p = z
   OP_TOP,        // phi, beta, |phi+beta|^2, beta.beta, phitphi
@@ -144,7 +144,9 @@ struct QpDev {
   const int *col;
   Vecs<double> d;
   Vecs<float> f;
-  double *ev, *pv, *pw; // power method vectors (FP64)
+  double *ev;            // power vector kept for the next solve (FP64)
+  double *pv, *pw;       // power iterates v_k by parity of k
+  double *pu, *pz;       // products w_k = A v_k by parity of k
   const unsigned char *cf;
   unsigned char *face;
   int *done_flag;       // receives `tag` once the QP reaches DONE

@@ -134,7 +134,8 @@ struct mprgp_qp {
   size_t bytes = 0;
   double *x = nullptr, *x0 = nullptr, *b = nullptr, *g = nullptr, *p = nullptr, *z = nullptr,
          *phi = nullptr, *beta = nullptr, *y = nullptr, *ap = nullptr, *py = nullptr, *ev = nullptr,
-         *pv = nullptr, *pw = nullptr, *stage = nullptr, *p2 = nullptr, *beta2 = nullptr;
+         *pv = nullptr, *pw = nullptr, *stage = nullptr, *p2 = nullptr, *beta2 = nullptr, *pu = nullptr,
+         *pz = nullptr;
   double *cn = nullptr, *cc = nullptr, *cj = nullptr, *cs = nullptr;
   unsigned char *cf = nullptr, *face = nullptr;
   mprgp::QpState *st = nullptr;

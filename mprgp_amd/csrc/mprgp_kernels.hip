@@ -266,9 +266,9 @@ __device__ __forceinline__ int sum_slots(int op) {
       return 3;
     case OP_CG_Y:
       return 4;
-    case OP_CL_AP: case OP_PROP1: case OP_CL_PROP1:
+    case OP_CL_AP: case OP_PROP1: case OP_CL_PROP1: case OP_POW_IT:
       return 2;
-    case OP_POW_MV: case OP_POW_NRM: case OP_EXP2: case OP_FINAL: case OP_CL_CG: case OP_CG_BAP:
+    case OP_EXP2: case OP_FINAL: case OP_CL_CG: case OP_CG_BAP:
     case OP_R_START: case OP_R_FINAL:
       return 1;
     default:
@@ -360,29 +360,30 @@ __device__ void control(QpState &st, const Sums &S) {
     case OP_PROJX:
       st.op = st.pow_next;
       break;
-    case OP_POW_MV: {
-      double nu = sqrt(S.v[0]);
-      st.nu = nu;
-      if (nu < kEps) {  // specRad: tiny norm ends the iteration, ev = v
-        st.nu = kEps;
-        st.op = OP_POW_SAVE;
-      } else {
-        st.op = OP_POW_NRM;
-      }
-      break;
-    }
-    case OP_POW_NRM: {
-      double delta = sqrt(S.v[0]);
-      if (delta <= st.power_eps) {
-        st.op = OP_POW_SAVE;
-      } else {
-        st.power_it++;
-        if (st.power_it > 1000) {  // cap: ev untouched
+    case OP_POW_IT: {
+      // specRad (MPRGPSolver.h:57-76) one launch per iteration: launch k
+      // closes iteration k-1 (its delta test) and runs iteration k's product
+      const int k = st.power_it;
+      if (k >= 1) {
+        if (sqrt(S.v[1]) <= st.power_eps) {  // converged at k-1: ev = v_{k-1}, nu_{k-1}
+          st.power_it = k - 1;
+          st.op = OP_POW_SAVE;
+          break;
+        }
+        if (k > 1000) {  // iterations 0..1000 done: ev untouched
           finish_power(st);
-        } else {
-          st.op = OP_POW_MV;  // v and w swap roles (power_it parity)
+          break;
         }
       }
+      const double nu = sqrt(S.v[0]);
+      st.nu = nu;
+      if (nu < kEps) {  // tiny norm ends the iteration, ev = v_k
+        st.nu = kEps;
+        st.op = OP_POW_SAVE;
+        break;
+      }
+      st.power_it = k + 1;
+      st.op = OP_POW_IT;
       break;
     }
     case OP_POW_SAVE:
@@ -956,29 +957,38 @@ __device__ __forceinline__ void row_work_fp64(const QpDev &q, int s, int lane, i
       st3(v.x, r, proj(R, ld3(v.x, r)));
       break;
     }
-    case OP_POW_MV: {
-      // v and w alternate between pv and pw with the iteration parity.  The
-      // mixed form multiplies by the FP32 copy of A, the matrix its inner
-      // solve iterates with (vectors and norms stay FP64).
-      const double *pv = (st.power_it & 1) ? q.pw : q.pv;
-      double *w = (st.power_it & 1) ? q.pv : q.pw;
+    case OP_POW_IT: {
+      // Iteration k: v_k = w_{k-1}/nu_{k-1} (v_0 is the start vector);
+      // |v_k - v_{k-1}|^2 closes iteration k-1; w_k = A v_k, formed as
+      // (A w_{k-1})/nu_{k-1} so that the gathers read w_{k-1}, which no tile
+      // writes in this launch (equal to A v_k up to rounding).  v and w each
+      // alternate between two buffers with the parity of k.  The mixed form
+      // multiplies by the FP32 copy of A, the matrix its inner solve
+      // iterates with (vectors and norms stay FP64).
+      const int k = st.power_it;
+      double *vk = (k & 1) ? q.pw : q.pv;
+      const double *vp = (k & 1) ? q.pv : q.pw;
+      double *wk = (k & 1) ? q.pz : q.pu;
+      const double *wp = (k & 1) ? q.pu : q.pz;
+      const double nu = st.nu;
+      if (k >= 1) {
+        const V3 a = ld3(wp, r);
+        const V3 vr = V3{a.x / nu, a.y / nu, a.z / nu};
+        st3(vk, r, vr);
+        const V3 dlt = sub(vr, ld3(vp, r));
+        P[1] += dot3(dlt, dlt);
+      }
+      if (k > 1000) break;  // the last launch only closes iteration 1000
+      const double *in = k == 0 ? vk : wp;
       V3 acc[1];
+      auto gather = [&](int j, V3 (&xs)[1]) { xs[0] = ld3(in, j); };
       if (MX && st.mixed)
-        spmv_row<1>(q.sl_off, q.col, q.f.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(pv, j); }, acc);
+        spmv_row<1>(q.sl_off, q.col, q.f.val, s, lane, gather, acc);
       else
-        spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(pv, j); }, acc);
-      st3(w, r, acc[0]);
-      P[0] += dot3(acc[0], acc[0]);
-      break;
-    }
-    case OP_POW_NRM: {
-      const double *pv = (st.power_it & 1) ? q.pw : q.pv;
-      double *w = (st.power_it & 1) ? q.pv : q.pw;
-      V3 a = ld3(w, r), b = ld3(pv, r);
-      a = V3{a.x / st.nu, a.y / st.nu, a.z / st.nu};
-      st3(w, r, a);
-      V3 dlt = sub(a, b);
-      P[0] += dot3(dlt, dlt);
+        spmv_row<1>(q.sl_off, q.col, v.val, s, lane, gather, acc);
+      const V3 w = k == 0 ? acc[0] : V3{acc[0].x / nu, acc[0].y / nu, acc[0].z / nu};
+      st3(wk, r, w);
+      P[0] += dot3(w, w);
       break;
     }
     case OP_POW_SAVE: {
@@ -1023,7 +1033,7 @@ __device__ __forceinline__ void row_work_fp64(const QpDev &q, int s, int lane, i
 }
 
 __device__ __forceinline__ bool fp64_op(int op) {
-  return op == OP_PROJX || op == OP_POW_MV || op == OP_POW_NRM || op == OP_POW_SAVE ||
+  return op == OP_PROJX || op == OP_POW_IT || op == OP_POW_SAVE ||
          op == OP_R_START || op == OP_R_UPD || op == OP_R_FINAL;
 }
 

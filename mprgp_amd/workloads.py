@@ -14,7 +14,7 @@ import mprgp_amd as M
 from mprgp_amd import problems
 
 # op indices of the device state machine (mprgp_dev.h)
-OPS = ["DONE", "PROJX", "POW_MV", "POW_NRM", "POW_SAVE", "INIT", "TOP", "CG_AP", "CG_PY", "CG_Y",
+OPS = ["DONE", "PROJX", "POW_IT", "POW_SAVE", "INIT", "TOP", "CG_AP", "CG_PY", "CG_Y",
        "CG_A", "EXP1", "EXP2", "EXP3", "PROP1", "PROP2", "FINAL", "CL_AP", "CL_CG",
        "CL_EXP", "CL_EXP2", "CL_PROP1", "R_START", "R_UPD", "R_FINAL", "CG_PYG",
        "CG_BAP", "CG_YA"]
@@ -46,8 +46,7 @@ class DeviceQP:
 # row; "face" one byte per node.
 _STREAMS = {
     "PROJX": ("rows", "x", "x"),
-    "POW_MV": ("M", "pv", "pw"),
-    "POW_NRM": ("pw", "pv", "pw"),
+    "POW_IT": ("M", "w", "v", "v", "w"),
     "POW_SAVE": ("pv", "ev"),
     "INIT": ("M", "x", "b", "rows", "dinv", "g", "phi", "z", "p", "face"),
     "EXP3": ("M", "x", "b", "rows", "dinv", "g", "phi", "z", "p", "face", "py", "y"),

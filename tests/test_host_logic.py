@@ -28,9 +28,10 @@ def test_byte_accounting():
     mat32 = nb * (9 * 4 + 4) + 4 * (N + 1)
     v64, v32 = 24 * N, 12 * N
     op = workloads.OPS.index
-    # the power method's SpMV: the form's matrix + 2 FP64 vectors
-    assert workloads.op_bytes(op("POW_MV"), N, nb, m) == mat64 + 2 * v64
-    assert workloads.op_bytes(op("POW_MV"), N, nb, m, True) == mat32 + 2 * v64
+    # a power iteration: the form's matrix, w_{k-1} (gathered) and v_{k-1}
+    # read, v_k and w_k written, all FP64
+    assert workloads.op_bytes(op("POW_IT"), N, nb, m) == mat64 + 4 * v64
+    assert workloads.op_bytes(op("POW_IT"), N, nb, m, True) == mat32 + 4 * v64
     # CG_AP: matrix + p (gathered once) + AP + z + g
     assert workloads.op_bytes(op("CG_AP"), N, nb, m) == mat64 + 4 * v64
     assert workloads.op_bytes(op("CG_AP"), N, nb, m, True) == mat32 + 4 * v32  # all FP32 in the inner solve
