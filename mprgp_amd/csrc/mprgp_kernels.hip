@@ -262,7 +262,7 @@ __device__ __forceinline__ int sum_slots(int op) {
       return 7;
     case OP_CG_A:
       return 5;
-    case OP_TOP: case OP_CG_PY: case OP_CG_AP:
+    case OP_TOP: case OP_CG_PY: case OP_CG_AP: case OP_INIT: case OP_EXP3: case OP_PROP2:
       return 3;
     case OP_CG_Y:
       return 4;
@@ -352,6 +352,34 @@ __device__ __forceinline__ void finish_power(QpState &st) {
   st.op = st.mixed ? OP_R_START : OP_INIT;
 }
 
+// The outer loop head (MPRGPSolver.h:417-437): KKT residual and
+// proportioning from the sums of computeGradients + proportional (slots 0-2).
+// TOP reduces them; INIT, EXP3 and PROP2 reduce them too, at the gradient
+// they produce, so that no separate TOP launch follows them.
+__device__ __forceinline__ void loop_head(QpState &st, const Sums &S) {
+  st.p_pending = 0;
+  st.residual = sqrt(S.v[0]);
+  if (st.residual <= st.tol || inner_stalled(st)) {
+    st.op = OP_FINAL;
+    return;
+  }
+  st.prop = proportional(st, S.v[1], S.v[2]);
+  if (st.variant == VAR_CLASSIC) {
+    st.op = st.prop ? OP_CL_AP : OP_CL_PROP1;
+  } else if (st.prop) {
+    st.cg_entry = 1;
+    st.op = OP_CG_AP;
+  } else {
+    st.op = (!st.y_valid || !st.y_feas) ? OP_EXP1 : OP_PROP1;
+  }
+}
+
+// next_outer for an op that carried the loop head's sums.
+__device__ __forceinline__ void next_outer_head(QpState &st, const Sums &S) {
+  if (st.iteration < st.max_it) loop_head(st, S);
+  else st.op = OP_FINAL;
+}
+
 // Consumes the reduced partial sums of the op a QP executed in the previous
 // launch and moves its state machine to the next op (MPRGPSolver.h:42-79,
 // 148-456).  Every tile of the QP runs it on identical inputs.
@@ -396,26 +424,12 @@ __device__ void control(QpState &st, const Sums &S) {
       st.y_valid = 0;
       st.best_res = kDblMax;
       st.since_best = 0;
-      st.op = (st.max_it > 0) ? OP_TOP : OP_FINAL;
+      if (st.max_it > 0) loop_head(st, S);  // INIT carried the loop head's sums
+      else st.op = OP_FINAL;
       break;
-    case OP_TOP: {
-      st.p_pending = 0;
-      st.residual = sqrt(S.v[0]);
-      if (st.residual <= st.tol || inner_stalled(st)) {
-        st.op = OP_FINAL;
-        break;
-      }
-      st.prop = proportional(st, S.v[1], S.v[2]);
-      if (st.variant == VAR_CLASSIC) {
-        st.op = st.prop ? OP_CL_AP : OP_CL_PROP1;
-      } else if (st.prop) {
-        st.cg_entry = 1;
-        st.op = OP_CG_AP;
-      } else {
-        st.op = (!st.y_valid || !st.y_feas) ? OP_EXP1 : OP_PROP1;
-      }
+    case OP_TOP:
+      loop_head(st, S);
       break;
-    }
     case OP_CG_AP:
       st.pd = S.v[0];
       st.zg = S.v[1];
@@ -477,7 +491,7 @@ __device__ void control(QpState &st, const Sums &S) {
     case OP_EXP3:
       st.num_exp++;
       st.iteration++;
-      next_outer(st);
+      next_outer_head(st, S);
       break;
     case OP_PROP1:
       st.alpha = S.v[1] / S.v[0];
@@ -486,7 +500,7 @@ __device__ void control(QpState &st, const Sums &S) {
     case OP_PROP2:
       st.num_prop++;
       st.iteration++;
-      next_outer(st);
+      next_outer_head(st, S);
       break;
     case OP_CL_AP: {
       st.pd = S.v[0];
@@ -576,6 +590,24 @@ __device__ __forceinline__ V3 round_to(V3 a) {  // the value a store to T storag
   return V3{(double)(T)a.x, (double)(T)a.y, (double)(T)a.z};
 }
 
+// computeGradients(g) + proportional (MPRGPSolver.h:163-181) for one row at
+// gradient g (as stored), face f and iterate x: phi and beta stored, the
+// loop head's sums in slots 0-2.
+template <class T>
+__device__ __forceinline__ void loop_head_row(const Vecs<T> &v, const Row &R, bool f, V3 g, V3 x,
+                                              double alpha_bar, int r, double (&P)[kNumPart]) {
+  const V3 gs = round_to<T>(g);
+  const V3 ph = phi_of(R, f, gs);
+  const V3 be = beta_of(R, f, gs, ph);
+  st3(v.phi, r, ph);
+  st3(v.beta, r, be);
+  const V3 gp = add(ph, be);
+  P[0] += dot3(gp, gp);
+  P[1] += dot3(be, be);
+  const V3 px = proj(R, axmy(x, alpha_bar, ph));
+  P[2] += dot3(sub(x, px), ph);
+}
+
 template <class T, bool MX>
 __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int s, int lane, int r, int op,
                                          const QpState &st, double (&P)[kNumPart]) {
@@ -605,15 +637,19 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
       V3 g = sub(acc[0], ld3(v.b, r));
       Row R = ldrow(q, v, r);
-      bool f = face_of(R, ld3(xc, r));
+      const V3 xr = ld3(xc, r);
+      bool f = face_of(R, xr);
       V3 ph = phi_of(R, f, g);
       V3 z = precond_of(v, pre, R, f, r, g, ph);
       st3(v.g, r, g);
-      st3(v.phi, r, ph);
       st3(v.z, r, z);
       st3(v.p, r, z);
       gmem(q.face)[r] = f;
       if (op == OP_EXP3) st3(yc, r, ld3(v.py, r));
+      // INIT and EXP3 end with the next outer iteration's head (its phi
+      // replaces this one); the classic CL_EXP2 is followed by TOP
+      if (op == OP_CL_EXP2) st3(v.phi, r, ph);
+      else loop_head_row(v, R, f, g, xr, st.alpha_bar, r, P);
       break;
     }
     case OP_TOP: {
@@ -623,17 +659,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       if (st.p_pending) {  // classic CGStep tail: p = z - beta p (MPRGPSolver.h:192-193)
         st3(v.p, r, axmy(ld3(v.z, r), st.beta_cg, ld3(v.p, r)));
       }
-      V3 g = ld3(v.g, r);
-      V3 ph = phi_of(R, f, g);
-      V3 be = beta_of(R, f, g, ph);
-      st3(v.phi, r, ph);
-      st3(v.beta, r, be);
-      V3 gp = add(ph, be);
-      P[0] += dot3(gp, gp);
-      P[1] += dot3(be, be);
-      V3 x = ld3(xc, r);
-      V3 px = proj(R, axmy(x, st.alpha_bar, ph));
-      P[2] += dot3(sub(x, px), ph);
+      loop_head_row(v, R, f, ld3(v.g, r), ld3(xc, r), st.alpha_bar, r, P);
       break;
     }
     case OP_CG_AP:
@@ -897,10 +923,10 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       V3 z = precond_of(v, pre, R, f, r, g, ph);
       st3(xc, r, xn);
       st3(v.g, r, g);
-      st3(v.phi, r, ph);
       st3(v.z, r, z);
       st3(v.p, r, z);
       gmem(q.face)[r] = f;
+      loop_head_row(v, R, f, g, xn, st.alpha_bar, r, P);  // the next iteration's head
       break;
     }
     case OP_CL_CG: {

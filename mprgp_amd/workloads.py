@@ -48,8 +48,8 @@ _STREAMS = {
     "PROJX": ("rows", "x", "x"),
     "POW_IT": ("M", "w", "v", "v", "w"),
     "POW_SAVE": ("pv", "ev"),
-    "INIT": ("M", "x", "b", "rows", "dinv", "g", "phi", "z", "p", "face"),
-    "EXP3": ("M", "x", "b", "rows", "dinv", "g", "phi", "z", "p", "face", "py", "y"),
+    "INIT": ("M", "x", "b", "rows", "dinv", "g", "phi", "z", "p", "face", "beta"),
+    "EXP3": ("M", "x", "b", "rows", "dinv", "g", "phi", "z", "p", "face", "py", "y", "beta"),
     "CL_EXP2": ("M", "x", "b", "rows", "dinv", "g", "phi", "z", "p", "face"),
     "TOP": ("rows", "face", "g", "x", "phi", "beta"),
     "CG_AP": ("M", "p", "ap", "z", "g"),
@@ -63,7 +63,7 @@ _STREAMS = {
     "EXP1": ("rows", "x", "y"),
     "EXP2": ("M", "y", "b", "rows", "g", "py", "x", "face"),
     "PROP1": ("M", "x", "beta", "b", "g", "ap"),
-    "PROP2": ("rows", "x", "beta", "g", "ap", "dinv", "x", "g", "phi", "z", "p", "face"),
+    "PROP2": ("rows", "x", "beta", "g", "ap", "dinv", "x", "g", "phi", "z", "p", "face", "beta"),
     "FINAL": ("rows", "x", "face", "x"),
     "CL_CG": ("rows", "face", "ap", "x", "p", "g", "dinv", "x", "g", "phi", "z"),
     "CL_EXP": ("rows", "x", "p", "g", "ap", "x", "g", "face"),
