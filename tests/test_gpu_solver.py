@@ -137,6 +137,34 @@ def test_power_method_matches_oracle(ctx):
         assert sg2["power_iters"] <= 2
 
 
+def test_power_method_cap_matches_oracle(ctx):
+    # specRad runs iterations 0..1000 when its stopping test is never met
+    # (MPRGPSolver.h:58-76): the last estimate is returned and ev is left as is
+    sc = problems.config1()
+    A, b, node, normal, c, x0 = scene_qp(sc)
+    lam, it, ev = oracle.spec_rad(A, 1e-300)
+    assert it == 1001 and ev is None
+    q = make_qp(ctx, A, b, node, normal, c, x0)
+    rc, sg = q.solve(M.params(tol=1e-8, max_it=0, power_tol=1e-300))
+    assert sg["power_iters"] == 1001
+    assert sg["ev_valid"] == 0
+    assert sg["lambda_max"] == pytest.approx(lam, rel=1e-10)
+    assert sg["alpha_bar"] == pytest.approx(2.0 / lam, rel=1e-10)
+
+
+def test_mixed_power_method_uses_the_fp32_matrix(ctx):
+    # the mixed form's power method multiplies by the FP32 copy of A: its
+    # estimate agrees with the FP64 one to the FP32 rounding of A's values
+    sc = problems.config1()
+    A, b, node, normal, c, x0 = scene_qp(sc)
+    lam64 = oracle.spec_rad(A, 1e-12)[0]
+    lam32 = oracle.spec_rad(sp.csr_matrix(A.astype(np.float32).astype(np.float64)), 1e-12)[0]
+    q = make_qp(ctx, A, b, node, normal, c, x0)
+    rc, sm = q.solve(M.params(tol=1e-8, max_it=0, power_tol=1e-12, precision=M.MIXED))
+    assert sm["lambda_max"] == pytest.approx(lam32, rel=1e-10)
+    assert sm["lambda_max"] == pytest.approx(lam64, rel=1e-6)
+
+
 def test_classic_variant_parity(ctx):
     sc = problems.config1_flat()
     A, b, node, normal, c, x0 = scene_qp(sc)
