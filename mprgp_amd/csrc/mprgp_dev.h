@@ -27,11 +27,11 @@ constexpr double kEps = 1e-9;    // ScalarUtil<double>::scalar_eps of the simula
 enum Op : int {
   OP_DONE = 0,
   OP_PROJX,      // x = P(x) before the solve (J/c facade overload)
-  OP_POW_IT,     // power iteration k: v_k = w_{k-1}/nu_{k-1} (formed at the
-                 // gather), |v_k - v_{k-1}|^2, w_k = A v_k, |w_k|^2
+  OP_POW_IT,     // power iteration k: v_k = w_{k-1}/nu_{k-1}, |v_k - v_{k-1}|^2,
+                 // w_k = (A w_{k-1})/nu_{k-1} = A v_k, |w_k|^2
   OP_POW_SAVE,   // ev = v
-  OP_INIT,       // g = Ax - b, face(x), phi, z, p = z
-  OP_TOP,        // phi, beta, |phi+beta|^2, beta.beta, phitphi
+  OP_INIT,       // g = Ax - b, face(x), phi, z, p = z, and the loop head (TOP)
+  OP_TOP,        // loop head: phi, beta, |phi+beta|^2, beta.beta, phitphi
   OP_CG_AP,      // AP = A p, p.AP, z.g
   OP_CG_PY,      // y = x - alpha p, py = P(y), feasibility of y
   OP_CG_Y,       // A py (and A x at CG entry): objective values
@@ -39,9 +39,10 @@ enum Op : int {
                  // beta, |phi+beta|^2, beta.beta, phitphi
   OP_EXP1,       // y = P(x)
   OP_EXP2,       // g = A y - b, face(y), phi, y' = y - abar phi, x = P(y')
-  OP_EXP3,       // g = A x - b, face(x), phi, z, p = z, y = y'
+  OP_EXP3,       // g = A x - b, face(x), phi, z, p = z, y = y', and the loop head
   OP_PROP1,      // g = A x - b, AD = A beta, beta.AD, g.beta
-  OP_PROP2,      // x = P(x - a beta), g -= a AD, face(x), phi, z, p = z
+  OP_PROP2,      // x = P(x - a beta), g -= a AD, face(x), phi, z, p = z, and
+                 // the loop head
   OP_FINAL,      // x = P(x) (monotone only), active count
   OP_CL_AP,      // classic: AP = A p, p.AP, z.g, max feasible step
   OP_CL_CG,      // classic CG step: x -= a p, g -= a AP, phi, z, z.AP

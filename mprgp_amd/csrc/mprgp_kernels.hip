@@ -18,9 +18,9 @@
 //
 // Precision: arithmetic and every reduction run in FP64.  The plain form
 // stores matrix and vectors in FP64; the mixed form runs the inner solve on
-// an FP32 copy of the matrix (and FP32 search directions) inside an FP64
-// iterative refinement: the residual b - A x, the KKT norm and the update
-// x += d use the FP64 data.
+// an FP32 copy of the matrix (and FP32 gradient- and direction-side vectors;
+// iterates, rows and 1/diag stay FP64) inside an FP64 iterative refinement:
+// the residual b - A x, the KKT norm and the update x += d use the FP64 data.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -262,13 +262,13 @@ __device__ __forceinline__ int sum_slots(int op) {
       return 7;
     case OP_CG_A:
       return 5;
-    case OP_TOP: case OP_CG_PY: case OP_CG_AP: case OP_INIT: case OP_EXP3: case OP_PROP2:
+    case OP_TOP: case OP_CG_AP: case OP_INIT: case OP_EXP3: case OP_PROP2:
       return 3;
     case OP_CG_Y:
       return 4;
     case OP_CL_AP: case OP_PROP1: case OP_CL_PROP1: case OP_POW_IT:
       return 2;
-    case OP_EXP2: case OP_FINAL: case OP_CL_CG: case OP_CG_BAP:
+    case OP_CG_PY: case OP_EXP2: case OP_FINAL: case OP_CL_CG: case OP_CG_BAP:
     case OP_R_START: case OP_R_FINAL:
       return 1;
     default:
