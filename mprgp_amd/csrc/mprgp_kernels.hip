@@ -727,7 +727,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
         const T *pp = v.p;
         const double alpha = st.alpha;
         V3 acc[1];
-        spmv_row<1, 1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) {
+        spmv_row<1, 2>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) {
           const V3 yj = axmy(ld3(xx, j), alpha, ld3(pp, j));
           const V3 nj = ld3(v.cn, j);
           const double vj = gmem(v.cc)[j] - dot3(nj, yj);
