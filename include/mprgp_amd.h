@@ -169,6 +169,30 @@ int mprgp_decoupled_solve(int device, int n, const int *a_outer, const int *a_in
                           const int *j_inner, const double *j_val, const double *c, double *x,
                           double tol, int max_it, double *ev, int *ev_valid, mprgp_stats *stats);
 
+/* ---- QP files ----------------------------------------------------------------
+ * The reference's binary QP file of A x = B s.t. J x >= c with initial value
+ * x0 (writeQP / loadQP, MPRGPUtility.h:406-487): A and J as {size_t rows,
+ * size_t cols, size_t nnz, nnz x Eigen::Triplet<double> (int32 row, int32
+ * col, double value)}, then B, c and x0 as doubles, in the order A, B, J, c,
+ * x0.  The readers return CSR arrays with columns ascending and duplicate
+ * triplets summed (setFromTriplets); a file that cannot be read or is
+ * malformed gives MPRGP_EBADINPUT (the reference's solve returns -1). */
+/* n = A's rows, m = J's rows, the CSR nonzero counts of A and J */
+int mprgp_qp_file_info(const char *file_name, int *n, int *m, long long *a_nnz, long long *j_nnz);
+/* a_ptr[n+1], a_col/a_val[a_nnz], B[n], j_ptr[m+1], j_col/j_val[j_nnz], c[m], x0[n] */
+int mprgp_qp_file_read(const char *file_name, int *a_ptr, int *a_col, double *a_val, double *B,
+                       int *j_ptr, int *j_col, double *j_val, double *c, double *x0);
+/* writeQP (MPRGPUtility.h:406-443) from CSR A (n x n) and CSR J (m x n) */
+int mprgp_write_qp(const char *file_name, int n, const int *a_ptr, const int *a_col, const double *a_val,
+                   const double *B, int m, const int *j_ptr, const int *j_col, const double *j_val,
+                   const double *c, const double *x0);
+/* MPRGPDecoupledCon<double>::solve(file_name, x, tol, max_it, name, ev)
+ * (MPRGPSolver.h:576-589): reads the file, then solves as
+ * mprgp_decoupled_solve from its x0 (projected first).  x (capacity n_x)
+ * receives the solution; *n_out (may be NULL) the QP's size n. */
+int mprgp_decoupled_solve_file(int device, const char *file_name, double *x, int n_x, int *n_out,
+                               double tol, int max_it, double *ev, int *ev_valid, mprgp_stats *stats);
+
 /* ---- FEM assembly on the device ------------------------------------------
  * FEMSystem::buildSystem (FEMSystem.cpp:167-192) + MprgpFemSolver::
  * buildLinearSystem (MprgpFemSolver.cpp:315-356) for one body at the first

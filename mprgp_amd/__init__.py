@@ -20,6 +20,7 @@ library (or without a GPU) the calls raise instead of falling back.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -28,6 +29,7 @@ from ._lib import (CLASSIC, COROTATIONAL, FP64, LINEAR, MIXED, MONOTONIC, RHS_ON
                    Params, Stats, check, last_error, load)
 
 __all__ = ["Context", "Matrix", "QP", "Fem", "Engine", "MPRGPDecoupledCon", "solve_batch", "params",
+           "read_qp", "write_qp", "qp_file_info",
            "MprgpError", "MONOTONIC", "CLASSIC", "FP64", "MIXED", "LINEAR", "COROTATIONAL",
            "RHS_ONLY", "device_count"]
 
@@ -371,3 +373,63 @@ class MPRGPDecoupledCon:
         if stats_out is not None:
             stats_out.update(st.as_dict())
         return 0 if rc == _lib.OK else -1
+
+    @staticmethod
+    def solve_file(file_name, tol=1e-3, max_it=1000, solver_name="MPRGP", ev=None, device=0,
+                   stats_out=None):
+        """File overload (MPRGPSolver.h:576-589): reads A, B, J, c and x0 from a
+        QP file in the reference's binary layout (loadQP, MPRGPUtility.h:446-487)
+        and solves from x0 as ``solve``.  The reference fills its x argument;
+        here the solution is returned: ``(code, x)``."""
+        n, m, _, _ = qp_file_info(file_name)
+        x = np.zeros(n)
+        evb = np.zeros(n) if ev is None else _f64(ev, n)
+        valid = C.c_int(0 if ev is None or not np.any(ev) else 1)
+        st = Stats()
+        n_out = C.c_int(0)
+        rc = load().mprgp_decoupled_solve_file(int(device), os.fsencode(file_name), _p(x), n,
+                                               C.byref(n_out), float(tol), int(max_it), _p(evb),
+                                               C.byref(valid), C.byref(st))
+        check(rc, allow=(_lib.OK, _lib.NOT_CONVERGED, _lib.ENAN))
+        if ev is not None and isinstance(ev, np.ndarray) and ev.dtype == np.float64 and ev.size == n:
+            ev[:] = evb
+        if stats_out is not None:
+            stats_out.update(st.as_dict())
+        return (0 if rc == _lib.OK else -1), x
+
+
+def qp_file_info(file_name):
+    """(n, m, nnz of A, nnz of J) of a QP file (duplicates summed)."""
+    n, m = C.c_int(), C.c_int()
+    an, jn = C.c_longlong(), C.c_longlong()
+    check(load().mprgp_qp_file_info(os.fsencode(file_name), C.byref(n), C.byref(m), C.byref(an),
+                                    C.byref(jn)))
+    return n.value, m.value, an.value, jn.value
+
+
+def read_qp(file_name):
+    """(A, B, J, c, x0) of a QP file in the reference's binary layout, A and J
+    as scipy CSR matrices (loadQP, MPRGPUtility.h:446-487)."""
+    import scipy.sparse as sp
+    n, m, an, jn = qp_file_info(file_name)
+    ap, ac, av = np.zeros(n + 1, np.int32), np.zeros(an, np.int32), np.zeros(an)
+    jp, jc, jv = np.zeros(m + 1, np.int32), np.zeros(jn, np.int32), np.zeros(jn)
+    B, c, x0 = np.zeros(n), np.zeros(m), np.zeros(n)
+    check(load().mprgp_qp_file_read(os.fsencode(file_name), _p(ap), _p(ac), _p(av), _p(B), _p(jp),
+                                    _p(jc), _p(jv), _p(c), _p(x0)))
+    return (sp.csr_matrix((av, ac, ap), shape=(n, n)), B, sp.csr_matrix((jv, jc, jp), shape=(m, n)),
+            c, x0)
+
+
+def write_qp(file_name, A, B, J, c, x0):
+    """writeQP (MPRGPUtility.h:406-443): A (n x n), J (m x n) scipy sparse."""
+    import scipy.sparse as sp
+    A = sp.csr_matrix(A)
+    n = A.shape[0]
+    J = sp.csr_matrix(J, shape=(J.shape[0], n))
+    m = J.shape[0]
+    ap, ac, av = _i32(A.indptr), _i32(A.indices), _f64(A.data)
+    jp, jc, jv = _i32(J.indptr), _i32(J.indices), _f64(J.data)
+    B, c, x0 = _f64(B, n), _f64(c, m), _f64(x0, n)
+    check(load().mprgp_write_qp(os.fsencode(file_name), n, _p(ap), _p(ac), _p(av), _p(B), m, _p(jp),
+                                _p(jc), _p(jv), _p(c), _p(x0)))

@@ -113,6 +113,12 @@ SIGNATURES = {
     "mprgp_solve_batch": (I, [P, I, PP, C.POINTER(Params), I, C.POINTER(Stats)]),
     "mprgp_decoupled_solve": (I, [I, I, P, P, P, P, I, P, P, P, P, P, D, I, P, C.POINTER(I),
                                   C.POINTER(Stats)]),
+    "mprgp_qp_file_info": (I, [C.c_char_p, C.POINTER(I), C.POINTER(I), C.POINTER(C.c_longlong),
+                               C.POINTER(C.c_longlong)]),
+    "mprgp_qp_file_read": (I, [C.c_char_p, P, P, P, P, P, P, P, P, P]),
+    "mprgp_write_qp": (I, [C.c_char_p, I, P, P, P, P, I, P, P, P, P, P]),
+    "mprgp_decoupled_solve_file": (I, [I, C.c_char_p, P, I, C.POINTER(I), D, I, P, C.POINTER(I),
+                                       C.POINTER(Stats)]),
     "mprgp_fem_create": (I, [P, I, I, P, P, P, P, D, PP]),
     "mprgp_fem_assemble": (I, [P, I, P, P, P, D, P]),
     "mprgp_fem_matrix": (P, [P]),

@@ -1173,10 +1173,12 @@ extern "C" int mprgp_copy_probe(mprgp_ctx *ctx, size_t bytes, int iters, double 
   return rc;
 }
 
-extern "C" int mprgp_decoupled_solve(int device, int n, const int *a_outer, const int *a_inner,
-                                     const double *a_val, const double *B, int m, const int *j_outer,
-                                     const int *j_inner, const double *j_val, const double *c, double *x,
-                                     double tol, int max_it, double *ev, int *ev_valid, mprgp_stats *stats) {
+// The one-shot drop-in on a context of its own; J by rows (CSR) or by
+// columns (CSC, Eigen's column-major storage).
+int mprgp::decoupled_solve(int device, int n, const int *a_outer, const int *a_inner, const double *a_val,
+                           const double *B, int m, int j_col_major, const int *j_outer, const int *j_inner,
+                           const double *j_val, const double *c, double *x, double tol, int max_it, double *ev,
+                           int *ev_valid, mprgp_stats *stats) {
   if (!B || !x) return fail(MPRGP_EBADINPUT, "NULL B or x");
   mprgp_ctx *ctx = nullptr;
   int rc = mprgp_ctx_create(device, 0, &ctx);
@@ -1185,7 +1187,7 @@ extern "C" int mprgp_decoupled_solve(int device, int n, const int *a_outer, cons
   mprgp_qp *q = nullptr;
   rc = mprgp_matrix_create_csr(ctx, n, a_outer, a_inner, a_val, &A);
   if (!rc) rc = mprgp_qp_create(A, &q);
-  if (!rc) rc = mprgp_qp_set_rows_sparse(q, m, 1, j_outer, j_inner, j_val, c);
+  if (!rc) rc = mprgp_qp_set_rows_sparse(q, m, j_col_major, j_outer, j_inner, j_val, c);
   if (!rc) rc = mprgp_qp_set_rhs(q, B);
   if (!rc) rc = mprgp_qp_set_x(q, x);
   if (!rc && ev && ev_valid && *ev_valid) rc = mprgp_qp_set_ev(q, ev);
@@ -1207,4 +1209,12 @@ extern "C" int mprgp_decoupled_solve(int device, int n, const int *a_outer, cons
   mprgp_matrix_destroy(A);
   mprgp_ctx_destroy(ctx);
   return code;
+}
+
+extern "C" int mprgp_decoupled_solve(int device, int n, const int *a_outer, const int *a_inner,
+                                     const double *a_val, const double *B, int m, const int *j_outer,
+                                     const int *j_inner, const double *j_val, const double *c, double *x,
+                                     double tol, int max_it, double *ev, int *ev_valid, mprgp_stats *stats) {
+  return mprgp::decoupled_solve(device, n, a_outer, a_inner, a_val, B, m, /*j_col_major*/ 1, j_outer, j_inner,
+                                j_val, c, x, tol, max_it, ev, ev_valid, stats);
 }

@@ -22,6 +22,10 @@ namespace mprgp {
 // last-error plumbing
 void set_error(const char *fmt, ...);
 int fail(int code, const char *fmt, ...);
+int decoupled_solve(int device, int n, const int *a_outer, const int *a_inner, const double *a_val,
+                    const double *B, int m, int j_col_major, const int *j_outer, const int *j_inner,
+                    const double *j_val, const double *c, double *x, double tol, int max_it, double *ev,
+                    int *ev_valid, mprgp_stats *stats);
 
 #define MPRGP_HIP(call)                                                                   \
   do {                                                                                    \

@@ -165,6 +165,32 @@ def test_mixed_power_method_uses_the_fp32_matrix(ctx):
     assert sm["lambda_max"] == pytest.approx(lam64, rel=1e-6)
 
 
+def test_file_overload_equals_the_drop_in(ctx):
+    # MPRGPDecoupledCon::solve(file_name, x, ...) (MPRGPSolver.h:576-589) reads
+    # the QP and solves it exactly as the J/c overload does
+    import os
+    import shutil
+    sc = problems.config1()
+    A, b, node, normal, c, x0 = scene_qp(sc)
+    J = problems.rows_to_J(node, normal, sc.n_nodes)
+    d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_tmp", "gpu_qp")
+    os.makedirs(d, exist_ok=True)
+    try:
+        path = os.path.join(d, "config1.qp")
+        M.write_qp(path, A, b, J, c, x0)
+        st_f, st_m = {}, {}
+        code_f, xf = M.MPRGPDecoupledCon.solve_file(path, tol=1e-8, max_it=5000, stats_out=st_f)
+        xm = x0.copy()
+        code_m = M.MPRGPDecoupledCon.solve(A, b, J, c, xm, tol=1e-8, max_it=5000, stats_out=st_m)
+        assert code_f == code_m == 0
+        assert np.array_equal(xf, xm)
+        assert st_f["iterations"] == st_m["iterations"]
+        code, xr, sr, _ = oracle.decoupled_solve(A, b, J, c, x0, tol=1e-8, max_it=5000)
+        assert_parity(A, b, node, normal, c, xf, xr, st_f, sr)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def test_classic_variant_parity(ctx):
     sc = problems.config1_flat()
     A, b, node, normal, c, x0 = scene_qp(sc)
