@@ -16,6 +16,7 @@
  *   MprgpFemSolver::buildLinearSystem    src/CollisionHandle/MprgpFemSolver.cpp:315-356
  *   FEMSystem::buildSystem               src/ElasticSolver/FEMSystem.cpp:167-192
  *   MaterialEnergy::eval (LINEAR/COROT)  src/ElasticSolver/FEMEnergy.cpp:263-304, 657-1057
+ *   LinearConCollider::handle (planes)   src/CollisionHandle/LinearConCollider.cpp:140-161
  *
  * Threading: a context owns one HIP stream; calls that use it are serialised
  * by the context (several threads may share a context).  An engine runs its
@@ -140,6 +141,21 @@ int mprgp_qp_set_rows(mprgp_qp *qp, int m, const int *node, const double *normal
  * default).  Each row must lie inside one node's three columns. */
 int mprgp_qp_set_rows_sparse(mprgp_qp *qp, int m, int col_major, const int *outer, const int *inner,
                              const double *val, const double *c);
+/* One time step's discrete plane collision, decoupled (MprgpFemSolver.cpp:255-271
+ * calling LinearConCollider::handle(body, plane, delta), LinearConCollider.cpp:140-161,
+ * then getConstraints, :206-230), on the device from the QP's current x (the
+ * last set_x or solve): every node with n.x + p - delta <= 0 for some plane
+ * gets one row, that of the first such plane in the given order, J row = n
+ * on the node, c = -p.  Replaces the QP's rows.  planes: n_planes x 5 doubles
+ * {nx, ny, nz, p, delta}; delta may be +inf.  A normal whose length is not 1
+ * within 1e-8 (the reference's assertion, :145) gives MPRGP_EBADINPUT; no x
+ * set gives MPRGP_ESTATE.  *m_out (may be NULL) receives the row count. */
+int mprgp_qp_collide_planes(mprgp_qp *qp, int n_planes, const double *planes, int *m_out);
+/* The QP's rows in node order (ascending user node; the reference lists them
+ * plane by plane): *m receives the count; node/normal/c hold cap rows (normal
+ * 3 per row); with all three NULL only *m is written.  More rows than cap
+ * gives MPRGP_EBADINPUT with *m set. */
+int mprgp_qp_get_rows(mprgp_qp *qp, int cap, int *m, int *node, double *normal, double *c);
 int mprgp_qp_set_rhs(mprgp_qp *qp, const double *b);
 int mprgp_qp_get_rhs(mprgp_qp *qp, double *b);       /* node order               */
 int mprgp_qp_set_x(mprgp_qp *qp, const double *x);   /* also kept as the reset point */

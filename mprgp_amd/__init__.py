@@ -49,6 +49,17 @@ def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def _planes(planes):
+    """(K, 5) float64 {nx, ny, nz, p, delta} from an array or (normal, p, delta) tuples."""
+    if isinstance(planes, np.ndarray):
+        return _f64(planes).reshape(-1, 5)
+    planes = list(planes)
+    if all(len(pl) == 5 and all(np.isscalar(v) for v in pl) for pl in planes):
+        return _f64(np.array(planes, np.float64).reshape(-1, 5))
+    rows = [[*np.asarray(n, np.float64).ravel(), float(p), float(d)] for n, p, d in planes]
+    return _f64(np.array(rows, np.float64).reshape(-1, 5))
+
+
 def device_count() -> int:
     n = C.c_int(0)
     rc = load().mprgp_device_count(C.byref(n))
@@ -214,6 +225,28 @@ class QP:
         outer, inner, val = _i32(J.indptr), _i32(J.indices), _f64(J.data)
         check(load().mprgp_qp_set_rows_sparse(self.h, int(J.shape[0]), 1, _p(outer), _p(inner),
                                               _p(val), _p(c)))
+
+    def collide_planes(self, planes):
+        """Rows of the decoupled plane collision at the QP's current x, made on
+        the device (LinearConCollider::handle per plane, first plane wins).
+        planes: (K, 5) {nx, ny, nz, p, delta} or a list of (normal, p, delta)
+        with unit normals.  Returns the row count."""
+        P = _planes(planes)
+        m = C.c_int(0)
+        check(load().mprgp_qp_collide_planes(self.h, int(P.shape[0]), _p(P), C.byref(m)))
+        return m.value
+
+    def get_rows(self):
+        """(node, normal (m, 3), c) of the QP's rows, by ascending node."""
+        m = C.c_int(0)
+        L = load()
+        check(L.mprgp_qp_get_rows(self.h, 0, C.byref(m), None, None, None))
+        node = np.zeros(max(m.value, 1), np.int32)
+        normal = np.zeros((max(m.value, 1), 3))
+        c = np.zeros(max(m.value, 1))
+        check(L.mprgp_qp_get_rows(self.h, m.value, C.byref(m), _p(node), _p(normal), _p(c)))
+        k = m.value
+        return node[:k], normal[:k], c[:k]
 
     def set_rhs(self, b):
         b = _f64(b, self.n)

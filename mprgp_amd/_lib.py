@@ -100,6 +100,8 @@ SIGNATURES = {
     "mprgp_qp_destroy": (None, [P]),
     "mprgp_qp_set_rows": (I, [P, I, P, P, P]),
     "mprgp_qp_set_rows_sparse": (I, [P, I, I, P, P, P, P]),
+    "mprgp_qp_collide_planes": (I, [P, I, P, P]),
+    "mprgp_qp_get_rows": (I, [P, I, P, P, P, P]),
     "mprgp_qp_set_rhs": (I, [P, P]),
     "mprgp_qp_get_rhs": (I, [P, P]),
     "mprgp_qp_set_x": (I, [P, P]),

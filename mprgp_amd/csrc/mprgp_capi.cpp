@@ -697,6 +697,73 @@ extern "C" int mprgp_qp_set_rows_sparse(mprgp_qp *q, int m, int col_major, const
   return upload_rows(q, m, node, nrm, cv);
 }
 
+extern "C" int mprgp_qp_collide_planes(mprgp_qp *q, int n_planes, const double *planes, int *m_out) {
+  if (!q || n_planes < 0 || (n_planes > 0 && !planes)) return fail(MPRGP_EBADINPUT, "bad plane arguments");
+  for (int k = 0; k < n_planes; ++k) {
+    const double *pl = planes + 5 * (size_t)k;
+    if (!std::isfinite(pl[0]) || !std::isfinite(pl[1]) || !std::isfinite(pl[2]) || !std::isfinite(pl[3]) ||
+        std::isnan(pl[4]))
+      return fail(MPRGP_EBADINPUT, "plane %d: non-finite normal or offset, or NaN band", k);
+    const double nn = std::sqrt(pl[0] * pl[0] + pl[1] * pl[1] + pl[2] * pl[2]);
+    if (!(nn >= 1.0 - 1e-8 && nn <= 1.0 + 1e-8))  // LinearConCollider.cpp:145
+      return fail(MPRGP_EBADINPUT, "plane %d: |n| = %.17g is not 1 within 1e-8", k, nn);
+  }
+  if (!q->has_x) return fail(MPRGP_ESTATE, "x was never set");
+  MPRGP_NOT_BUSY(q);
+  MPRGP_HIP(hipSetDevice(q->ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
+  mprgp_ctx *ctx = q->ctx;
+  hipStream_t s = ctx->stream;
+  // scratch: the row counter, then the planes from byte 256
+  const size_t pbytes = 5 * sizeof(double) * (size_t)n_planes;
+  if (int rc = ctx->scratch_reserve(256 + pbytes)) return rc;
+  int *d_count = (int *)ctx->scratch;
+  double *d_planes = (double *)((char *)ctx->scratch + 256);
+  MPRGP_HIP(hipMemsetAsync(d_count, 0, sizeof(int), s));
+  if (pbytes) MPRGP_HIP(hipMemcpyAsync(d_planes, planes, pbytes, hipMemcpyHostToDevice, s));
+  MPRGP_HIP(launch_collide_planes(s, q->x, d_planes, n_planes, q->cn, q->cc, q->cj, q->cf, d_count, q->n));
+  int m = 0;
+  MPRGP_HIP(hipMemcpyAsync(ctx->h_running, d_count, sizeof(int), hipMemcpyDeviceToHost, s));
+  MPRGP_HIP(hipStreamSynchronize(s));
+  m = ctx->h_running[0];
+  q->m = m;
+  if (m_out) *m_out = m;
+  return MPRGP_OK;
+}
+
+extern "C" int mprgp_qp_get_rows(mprgp_qp *q, int cap, int *m, int *node, double *normal, double *c) {
+  if (!q || !m || cap < 0) return fail(MPRGP_EBADINPUT, "bad arguments");
+  *m = q->m;
+  if (!node && !normal && !c) return MPRGP_OK;  // count query
+  if (q->m > cap) return fail(MPRGP_EBADINPUT, "the QP has %d rows, the arrays hold %d", q->m, cap);
+  if (q->m > 0 && (!node || !normal || !c)) return fail(MPRGP_EBADINPUT, "NULL argument");
+  MPRGP_NOT_BUSY(q);
+  MPRGP_HIP(hipSetDevice(q->ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
+  const int N = q->n;
+  std::vector<double> cn(3 * (size_t)N), cc(N);
+  std::vector<unsigned char> cf(N);
+  hipStream_t s = q->ctx->stream;
+  MPRGP_HIP(hipMemcpyAsync(cn.data(), q->cn, cn.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  MPRGP_HIP(hipMemcpyAsync(cc.data(), q->cc, cc.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  MPRGP_HIP(hipMemcpyAsync(cf.data(), q->cf, cf.size(), hipMemcpyDeviceToHost, s));
+  MPRGP_HIP(hipStreamSynchronize(s));
+  int i = 0;
+  for (int k = 0; k < N; ++k) {
+    const int r = q->A->iperm[k];
+    if (!cf[r]) continue;
+    if (i >= q->m) return fail(MPRGP_ESTATE, "row storage holds more than %d rows", q->m);
+    node[i] = k;
+    normal[3 * (size_t)i + 0] = cn[3 * (size_t)r + 0];
+    normal[3 * (size_t)i + 1] = cn[3 * (size_t)r + 1];
+    normal[3 * (size_t)i + 2] = cn[3 * (size_t)r + 2];
+    c[i] = cc[r];
+    ++i;
+  }
+  if (i != q->m) return fail(MPRGP_ESTATE, "row storage holds %d rows, expected %d", i, q->m);
+  return MPRGP_OK;
+}
+
 // ----------------------------------------------------------- batch solve
 
 namespace mprgp {

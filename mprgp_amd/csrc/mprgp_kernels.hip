@@ -1269,6 +1269,49 @@ __global__ void setup_rows(const double *__restrict__ dinv, const double *__rest
   cs[r] = 1.0 / s;
 }
 
+// Decoupled plane collision at the current iterate: node r gets the row of the
+// first plane k (in the given order) with n_k.x_r + p_k - delta_k <= 0, i.e.
+// J row = n_k on the node's 3 columns and c = -p_k (LinearConCollider.cpp:140-161
+// per plane, MprgpFemSolver.cpp:268-271 over the planes, getConstraints
+// LinearConCollider.cpp:57-72); every other node's row storage is cleared.  With
+// one row per node, the reference's per-node list of distinct planes (the 1e-4
+// test of addConPlane) never changes which plane wins.  planes: 5 doubles each
+// {nx, ny, nz, p, delta}.  count must be zero on entry; it receives the rows.
+__global__ void __launch_bounds__(256) collide_planes(const double *__restrict__ x,
+                                                      const double *__restrict__ planes, int n_planes,
+                                                      double *__restrict__ cn, double *__restrict__ cc,
+                                                      double *__restrict__ cj, unsigned char *__restrict__ cf,
+                                                      int *__restrict__ count, int n_nodes) {
+#pragma clang fp contract(off)
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  bool hit = false;
+  if (r < n_nodes) {
+    const V3 xr = ld3(x, r);
+    V3 n = {0.0, 0.0, 0.0};
+    double c = 0.0;
+    for (int k = 0; k < n_planes; ++k) {
+      const double *pl = planes + 5 * k;
+      // n.dot(pos) + plane[3] - delta, the dot summed left to right
+      double d = pl[0] * xr.x + pl[1] * xr.y;
+      d = d + pl[2] * xr.z;
+      d = d + pl[3];
+      d = d - pl[4];
+      if (d <= 0.0) {
+        hit = true;
+        n = V3{pl[0], pl[1], pl[2]};
+        c = -pl[3];
+        break;
+      }
+    }
+    st3(cn, r, n);
+    cc[r] = c;
+    cj[r] = hit ? (n.x * n.x + n.y * n.y) + n.z * n.z : 0.0;
+    cf[r] = hit ? 1 : 0;
+  }
+  const unsigned long long b = __ballot(hit);
+  if ((threadIdx.x % kSlice) == 0 && b != 0) atomicAdd(count, (int)__popcll(b));
+}
+
 // FP32 copies for the mixed form.
 __global__ void to_f32(const double *__restrict__ in, float *__restrict__ out, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1349,6 +1392,12 @@ hipError_t launch_setup_rows(hipStream_t s, const double *dinv, const double *cn
                              const unsigned char *cf, double *cs, int n) {
   if (n <= 0) return hipSuccess;
   setup_rows<<<(n + 255) / 256, 256, 0, s>>>(dinv, cn, cf, cs, n);
+  return hipGetLastError();
+}
+hipError_t launch_collide_planes(hipStream_t s, const double *x, const double *planes, int n_planes,
+                                 double *cn, double *cc, double *cj, unsigned char *cf, int *count, int n) {
+  if (n <= 0) return hipSuccess;
+  collide_planes<<<(n + 255) / 256, 256, 0, s>>>(x, planes, n_planes, cn, cc, cj, cf, count, n);
   return hipGetLastError();
 }
 hipError_t launch_to_f32(hipStream_t s, const double *in, float *out, size_t n) {

@@ -101,6 +101,18 @@ def plane_rows(x, planes):
             np.concatenate(cs))
 
 
+def plane_array(planes):
+    """(K, 5) {nx, ny, nz, p, delta} of a scene's (normal, p, delta) planes,
+    normals scaled to unit length as ``plane_rows`` uses them."""
+    out = np.zeros((len(planes), 5))
+    for k, (n, p, delta) in enumerate(planes):
+        n = np.asarray(n, dtype=np.float64)
+        out[k, :3] = n / np.linalg.norm(n)
+        out[k, 3] = p
+        out[k, 4] = delta
+    return out
+
+
 def rows_to_J(node, normal, n_nodes):
     """Sparse J (m x 3N) of per-node rows, as LinearConCollider::getConstraints builds it."""
     import scipy.sparse as sp

@@ -9,8 +9,10 @@ device-resident QP.  A timestep (one request) is
   centroid (then lifted clear of the planes), full corotational assembly of
   A and b at x_n, and a fresh alpha_bar;
 * otherwise: right-hand side only (``RHS_ONLY``: the lagged matrix is kept);
-* every step: the collider's rows from x_n (host, a node within the planes'
-  band gets one row, as the reference's collision handling does per step);
+* every step: the collider's rows from x_n (a node within the planes' band
+  gets one row, as the reference's collision handling does per step), made
+  on the device from the QP's iterate (``QP.collide_planes``), or on the
+  host and uploaded (``device_rows=False``);
 * the solve from the warm start x_n (projected first, as
   DecoupledMprgpFemSolver::forward does, MprgpFemSolver.cpp:528-533);
 * x_{n+1} = x, v_{n+1} = (x - x_n) / h.
@@ -47,6 +49,7 @@ class Session:
     history: list = dataclasses.field(default_factory=list)
     pending: collections.deque = dataclasses.field(default_factory=collections.deque)
     busy: bool = False
+    device_rows: bool = True
 
     @property
     def n_nodes(self):
@@ -74,10 +77,13 @@ class Session:
             sc.v_n = sc.v_n @ R.T
         material = M.COROTATIONAL if full else (M.COROTATIONAL | M.RHS_ONLY)
         self.fem.assemble(self.qp, sc.x_n.ravel(), sc.v_n.ravel(), sc.gravity, sc.h, material)
-        node, normal, c = sc.rows()
-        self.qp.set_rows(node, normal, c)
         if self.step == 0 or full:  # otherwise the QP's x is the last solution = x_n
             self.qp.set_x(sc.x_n.ravel())
+        if self.device_rows:
+            self.qp.collide_planes(problems.plane_array(sc.planes))
+        else:
+            node, normal, c = sc.rows()
+            self.qp.set_rows(node, normal, c)
 
     def complete(self, keep_history=False, rc=0, stats=None):
         sc = self.scene
@@ -131,7 +137,8 @@ def session_scene(rng, sid, dof_range=(3e3, 1.5e5)):
     return sc
 
 
-def make_sessions(ctx, count=64, seed=5, dof_range=(3e3, 1.5e5), reassemble=10, own_contexts=False):
+def make_sessions(ctx, count=64, seed=5, dof_range=(3e3, 1.5e5), reassemble=10, own_contexts=False,
+                  device_rows=True):
     """Sessions on `ctx`, or each on a context of its own (same device) so
     that the host work of different sessions does not share one stream."""
     rng = np.random.default_rng(seed)
@@ -144,7 +151,7 @@ def make_sessions(ctx, count=64, seed=5, dof_range=(3e3, 1.5e5), reassemble=10, 
         fem = M.Fem(sctx, sc.X0, sc.tets, sc.young, sc.poisson, sc.density)
         qp = fem.matrix.qp()
         out.append(Session(i, sc, fem, qp, reassemble=reassemble,
-                           rng=np.random.default_rng([seed, i])))
+                           rng=np.random.default_rng([seed, i]), device_rows=device_rows))
     # alpha_bar per session from its first matrix (power method, batched)
     for s in out:
         s.prepare()

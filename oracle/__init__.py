@@ -95,6 +95,8 @@ def lib():
                                            C.POINTER(C.POINTER(C.c_int)),
                                            C.POINTER(C.POINTER(C.c_double)), P]
         L.orc_free.argtypes = [P]
+        L.orc_plane_collide.argtypes = [I, P, I, P, P, P, P]
+        L.orc_plane_collide.restype = I
         _lib = L
     return _lib
 
@@ -313,3 +315,21 @@ def fem_build_system(n_nodes, tets_oriented, d, vol, mass, young, poisson, densi
     L.orc_free(C.cast(ip, C.c_void_p))
     L.orc_free(C.cast(vp, C.c_void_p))
     return sp.csr_matrix((val, idx, ptr), shape=(n, n)), b
+
+
+def plane_collide(x, planes):
+    """Rows of the decoupled plane collision at positions x (N, 3).
+
+    planes: (K, 5) rows {nx, ny, nz, p, delta}.  Returns (node, normal (m, 3),
+    c) in the reference's push order, or raises ValueError on a non-unit
+    normal (the reference's assertion)."""
+    x = _f64(np.reshape(x, (-1, 3)))
+    N = x.shape[0]
+    planes = _f64(np.reshape(planes, (-1, 5)))
+    node = np.zeros(max(N, 1), np.int32)
+    normal = np.zeros((max(N, 1), 3))
+    c = np.zeros(max(N, 1))
+    m = lib().orc_plane_collide(N, _ptr(x), planes.shape[0], _ptr(planes), _ptr(node), _ptr(normal), _ptr(c))
+    if m < 0:
+        raise ValueError("plane normal is not unit length")
+    return node[:m].copy(), normal[:m].copy(), c[:m].copy()

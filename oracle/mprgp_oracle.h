@@ -123,6 +123,16 @@ int orc_fem_build_system(int n_nodes, int n_tets, const int *tets_oriented,
                          int **ptr_out, int **idx_out, double **val_out, double *b_out);
 void orc_free(void *p);
 
+/* ---- plane collider (collide_oracle.c) --------------------------------- */
+/* Decoupled discrete collision of one body's n_nodes positions x (3 per node)
+ * with n_planes planes {nx, ny, nz, p, delta} in order: the rows of
+ * LinearConCollider::getConstraints after handle(body, plane, delta) for each
+ * plane.  node/normal/c receive at most n_nodes rows (normal 3 per row).
+ * Returns the row count, or -1 when a normal's length is outside
+ * [1 - 1e-8, 1 + 1e-8]. */
+int orc_plane_collide(int n_nodes, const double *x, int n_planes, const double *planes,
+                      int *node, double *normal, double *c);
+
 #ifdef __cplusplus
 }
 #endif
