@@ -226,6 +226,57 @@ int mprgp_fem_assemble(mprgp_fem *fem, int material, const double *x_n, const do
                        const double *gravity, double h, mprgp_qp *qp);
 mprgp_matrix *mprgp_fem_matrix(mprgp_fem *fem);
 void mprgp_fem_destroy(mprgp_fem *fem);
+/* mprgp_fem_assemble at a Newton iterate x1 (buildLinearSystem after
+ * updatePos, MprgpFemSolver.cpp:315-356): elements evaluated at x1,
+ * b = h^2 f(x1) + h^2 M g + M (x_n + h v_n - x1) + A x1.  x1 = NULL means x_n. */
+int mprgp_fem_assemble_at(mprgp_fem *fem, int material, const double *x_n, const double *v_n,
+                          const double *x1, const double *gravity, double h, mprgp_qp *qp);
+
+/* ---- time step: MprgpFemSolver::advance (MprgpFemSolver.cpp:193-203) -----
+ * One body's positions and velocities kept on the device (node order), with
+ * the simulator's integrator, implicit Euler (Simulator.cpp:313 calling
+ * FEMSolver::resetImplicitEuler, FEMSystem.h:140-142: beta = 1, beta2 = 0.5,
+ * gamma = 1), no damping.  A step is the planes' collision at x_n
+ * (handleCollDetection/DCDhandle, :255-271), DecoupledMprgpFemSolver::forward's
+ * Newton loop (:488-538: assemble at x1, x = P(x1), MPRGP solve, updatePos
+ * :358-375, stop when max|x - x1| < eps or after maxIter solves), then
+ * updateMesh (:377-390): a = (x - (x_n + h v_n)) / (h h), v = v_n + a h,
+ * x_n = x. */
+typedef struct {
+  double h;               /* time step                                              */
+  double gravity[3];
+  int material;           /* MPRGP_LINEAR or MPRGP_COROTATIONAL, | MPRGP_RHS_ONLY  */
+  int n_planes;           /* planes: n_planes x {nx, ny, nz, p, delta}, unit normals */
+  const double *planes;
+  int newton_max_it;      /* forward's "maxIter" (resetImplicitEuler default 10)    */
+  double newton_eps;      /* forward's "eps" on max|x - x1| (default 1e-4)          */
+  int reserved_i[4];      /* set to 0                                               */
+  double reserved_d[4];
+} mprgp_step_params;
+
+typedef struct {
+  int newton_iters;       /* solves run                                             */
+  int rows;               /* constraint rows of the step's collision                */
+  int code;               /* MPRGP_OK, or MPRGP_NOT_CONVERGED if any solve was not  */
+  int iterations;         /* MPRGP iterations over the step's solves                */
+  double delta;           /* last max|x - x1| (updatePos's tol_err)                 */
+  mprgp_stats last;       /* the last solve's statistics                            */
+} mprgp_step_stats;
+
+int mprgp_fem_set_state(mprgp_fem *fem, const double *x, const double *v);
+int mprgp_fem_get_state(mprgp_fem *fem, double *x, double *v); /* either may be NULL */
+/* The whole step, solving with mprgp_qp_solve (params->project_first is
+ * forced to 1, as forward projects x1 before each solve).  Returns
+ * stats->code; the state advances also when a solve did not converge. */
+int mprgp_fem_advance(mprgp_fem *fem, mprgp_qp *qp, const mprgp_step_params *sp,
+                      const mprgp_params *params, mprgp_step_stats *stats);
+/* A one-solve step split around a solve made elsewhere (an engine):
+ * begin_step collides the planes at x_n (*rows, may be NULL), assembles at
+ * x1 = x_n and sets the QP's x to x_n; end_step takes the QP's x as the new
+ * positions, returns max|x - x_n| in *delta (may be NULL) and updates the
+ * velocities and positions as updateMesh does. */
+int mprgp_fem_begin_step(mprgp_fem *fem, mprgp_qp *qp, const mprgp_step_params *sp, int *rows);
+int mprgp_fem_end_step(mprgp_fem *fem, mprgp_qp *qp, const mprgp_step_params *sp, double *delta);
 
 /* ---- bandwidth probe ------------------------------------------------------
  * No counterpart in the reference: copies `bytes` between two device buffers

@@ -26,7 +26,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (CLASSIC, COROTATIONAL, FP64, LINEAR, MIXED, MONOTONIC, RHS_ONLY, MprgpError,
-                   Params, Stats, check, last_error, load)
+                   Params, Stats, StepParams, StepStats, check, last_error, load)
 
 __all__ = ["Context", "Matrix", "QP", "Fem", "Engine", "MPRGPDecoupledCon", "solve_batch", "params",
            "read_qp", "write_qp", "qp_file_info",
@@ -133,7 +133,9 @@ class Context:
 
 
 class Fem:
-    """Device FEM assembly of one body (buildLinearSystem at x1 = x_n)."""
+    """Device FEM assembly of one body (buildLinearSystem) and, once a state
+    is set, its implicit-Euler time step (MprgpFemSolver::advance) with the
+    positions and velocities kept on the device."""
 
     def __init__(self, ctx: Context, rest_pos, tets, young, poisson, density):
         X = _f64(rest_pos)
@@ -161,6 +163,59 @@ class Fem:
         v = _f64(v_n, 3 * self.n_nodes)
         g = _f64(gravity, 3)
         check(load().mprgp_fem_assemble(self.h, int(material), _p(x), _p(v), _p(g), float(h), qp.h))
+
+    def assemble_at(self, qp: "QP", x_n, v_n, x1, gravity=(0.0, -9.8, 0.0), h=0.01, material=LINEAR):
+        """A and b at the Newton iterate x1 (x1 = None: x_n)."""
+        x = _f64(x_n, 3 * self.n_nodes)
+        v = _f64(v_n, 3 * self.n_nodes)
+        y = None if x1 is None else _f64(x1, 3 * self.n_nodes)
+        g = _f64(gravity, 3)
+        check(load().mprgp_fem_assemble_at(self.h, int(material), _p(x), _p(v), None if y is None else _p(y),
+                                           _p(g), float(h), qp.h))
+
+    def set_state(self, x, v):
+        check(load().mprgp_fem_set_state(self.h, _p(_f64(x, 3 * self.n_nodes)),
+                                         _p(_f64(v, 3 * self.n_nodes))))
+
+    def get_state(self):
+        x = np.empty(3 * self.n_nodes)
+        v = np.empty(3 * self.n_nodes)
+        check(load().mprgp_fem_get_state(self.h, _p(x), _p(v)))
+        return x, v
+
+    def step_params(self, h=0.01, gravity=(0.0, -9.8, 0.0), material=LINEAR, planes=(), newton_max_it=10,
+                    newton_eps=1e-4):
+        """StepParams (the planes array is kept alive on the returned object)."""
+        sp = StepParams()
+        sp.h = float(h)
+        for k in range(3):
+            sp.gravity[k] = float(gravity[k])
+        sp.material = int(material)
+        P = _planes(planes)
+        sp._planes = P
+        sp.n_planes = int(P.shape[0])
+        sp.planes = P.ctypes.data if P.size else None
+        sp.newton_max_it = int(newton_max_it)
+        sp.newton_eps = float(newton_eps)
+        return sp
+
+    def advance(self, qp: "QP", sp: StepParams, p: Params | None = None, **kw):
+        """One time step; returns (code, step stats dict)."""
+        p = params(**kw) if p is None else p
+        st = StepStats()
+        rc = load().mprgp_fem_advance(self.h, qp.h, C.byref(sp), C.byref(p), C.byref(st))
+        check(rc, allow=(_lib.OK, _lib.NOT_CONVERGED))
+        return rc, st.as_dict()
+
+    def begin_step(self, qp: "QP", sp: StepParams):
+        rows = C.c_int(0)
+        check(load().mprgp_fem_begin_step(self.h, qp.h, C.byref(sp), C.byref(rows)))
+        return rows.value
+
+    def end_step(self, qp: "QP", sp: StepParams):
+        delta = C.c_double(0.0)
+        check(load().mprgp_fem_end_step(self.h, qp.h, C.byref(sp), C.byref(delta)))
+        return delta.value
 
 
 class Matrix:

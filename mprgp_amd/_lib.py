@@ -54,6 +54,22 @@ class Stats(C.Structure):
         return d
 
 
+class StepParams(C.Structure):
+    _fields_ = [("h", C.c_double), ("gravity", C.c_double * 3), ("material", C.c_int),
+                ("n_planes", C.c_int), ("planes", C.c_void_p), ("newton_max_it", C.c_int),
+                ("newton_eps", C.c_double), ("reserved_i", C.c_int * 4), ("reserved_d", C.c_double * 4)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("newton_iters", C.c_int), ("rows", C.c_int), ("code", C.c_int), ("iterations", C.c_int),
+                ("delta", C.c_double), ("last", Stats)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "last"}
+        d["last"] = self.last.as_dict()
+        return d
+
+
 class EngineConfig(C.Structure):
     _fields_ = [("max_active", C.c_int), ("tiles", C.c_int), ("graph_steps", C.c_int),
                 ("reserved_i", C.c_int * 5), ("hbm_budget", C.c_size_t), ("reserved_d", C.c_double * 4)]
@@ -123,6 +139,12 @@ SIGNATURES = {
                                        C.POINTER(Stats)]),
     "mprgp_fem_create": (I, [P, I, I, P, P, P, P, D, PP]),
     "mprgp_fem_assemble": (I, [P, I, P, P, P, D, P]),
+    "mprgp_fem_assemble_at": (I, [P, I, P, P, P, P, D, P]),
+    "mprgp_fem_set_state": (I, [P, P, P]),
+    "mprgp_fem_get_state": (I, [P, P, P]),
+    "mprgp_fem_advance": (I, [P, P, C.POINTER(StepParams), C.POINTER(Params), C.POINTER(StepStats)]),
+    "mprgp_fem_begin_step": (I, [P, P, C.POINTER(StepParams), P]),
+    "mprgp_fem_end_step": (I, [P, P, C.POINTER(StepParams), P]),
     "mprgp_fem_matrix": (P, [P]),
     "mprgp_fem_destroy": (None, [P]),
     "mprgp_copy_probe": (I, [P, C.c_size_t, I, C.POINTER(D)]),

@@ -287,13 +287,15 @@ __global__ void fem_blocks(long long nb, int T, const int *__restrict__ ptr, con
   for (int e = 0; e < 9; ++e) o[(size_t)e * kSlice] = acc[e];
 }
 
-// b (node order, before A x_n): sum of h^2 f_e in tet order, then gravity
-// h^2 rho m g, then M (x_n + h v_n - x_n)  (FEMSystem.cpp:176-186).
+// b (node order, before A x1): sum of h^2 f_e(x1) in tet order, then gravity
+// h^2 rho m g, then M (x_n + h v_n - x1)  (FEMSystem.cpp:176-186 with
+// MRHSL = PSI - X1, MprgpFemSolver.cpp:336-340).  x1 is the Newton iterate
+// (x_n on the first iteration).
 __global__ void fem_rhs(int N, int T, const int *__restrict__ ptr, const int *__restrict__ ctr,
                         const double *__restrict__ node_mass, double density,
                         const double *__restrict__ elem, const double *__restrict__ x,
-                        const double *__restrict__ v, double g0, double g1, double g2, double h,
-                        double *__restrict__ b) {
+                        const double *__restrict__ v, const double *__restrict__ x1, double g0, double g1,
+                        double g2, double h, double *__restrict__ b) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= N) return;
   double s[3] = {0.0, 0.0, 0.0};
@@ -308,11 +310,41 @@ __global__ void fem_rhs(int N, int T, const int *__restrict__ ptr, const int *__
   const double h2 = h * h;
   for (int r = 0; r < 3; ++r) s[r] += ((g[r] * density) * m) * h2;
   for (int r = 0; r < 3; ++r) {
-    const double xr = x[3 * (size_t)a + r];
-    const double psi = xr + h * v[3 * (size_t)a + r];
-    s[r] += M * (psi - xr);
+    const double psi = x[3 * (size_t)a + r] + h * v[3 * (size_t)a + r];
+    s[r] += M * (psi - x1[3 * (size_t)a + r]);
   }
   for (int r = 0; r < 3; ++r) b[3 * (size_t)a + r] = s[r];
+}
+
+// max_i |a_i - b_i| (updatePos's tol_err, MprgpFemSolver.cpp:368-374): one
+// workgroup, a fixed reduction order, so the result is reproducible.
+__global__ void __launch_bounds__(1024) max_abs_diff(const double *__restrict__ a, const double *__restrict__ b,
+                                                     size_t n, double *__restrict__ out) {
+  __shared__ double red[1024];
+  double m = 0.0;
+  for (size_t i = threadIdx.x; i < n; i += 1024) m = fmax(m, fabs(a[i] - b[i]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// updateMesh (MprgpFemSolver.cpp:377-390) with initVel's PHI and PSI
+// (:295-313) under implicit Euler (beta = 1, beta2 = 0.5, gamma = 1,
+// FEMSystem.h:140-142): the previous acceleration enters with the factors
+// 1 - gamma = 0 and 1 - 2 beta2 = 0, so a = (x - (x_n + h v_n)) / (h h) and
+// v = v_n + a h.  Then x_n = x, v_n = v.
+__global__ void euler_update(size_t n, const double *__restrict__ x, double *__restrict__ xn,
+                             double *__restrict__ vn, double h) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double psi = xn[i] + h * vn[i];
+  const double acc = (x[i] - psi) / ((1.0 * h) * h);
+  vn[i] = vn[i] + (acc * 1.0) * h;
+  xn[i] = x[i];
 }
 
 }  // namespace mprgp
@@ -333,6 +365,12 @@ struct mprgp_fem {
   double *rot = nullptr;   // per-tet rotations of the last corotational assembly
   bool rot_valid = false;
   double density = 0.0;
+  // device-resident body state (node order): positions x_n, velocities v_n,
+  // the Newton iterate x1 and the next one x2 (allocated by set_state)
+  double *state = nullptr;
+  double *xn = nullptr, *vn = nullptr, *x1 = nullptr, *x2 = nullptr;
+  bool has_state = false;
+  bool step_open = false;  // begin_step ran and end_step has not
 };
 
 namespace {
@@ -497,9 +535,9 @@ extern "C" int mprgp_fem_create(mprgp_ctx *ctx, int N, int T, const int *tets_in
 
 extern "C" mprgp_matrix *mprgp_fem_matrix(mprgp_fem *f) { return f ? f->A : nullptr; }
 
-extern "C" int mprgp_fem_assemble(mprgp_fem *f, int material, const double *x_n, const double *v_n,
-                                  const double *gravity, double h, mprgp_qp *qp) {
-  if (!f || !x_n || !v_n || !gravity || !qp) return fail(MPRGP_EBADINPUT, "NULL argument");
+namespace {
+
+int check_material(const mprgp_fem *f, int material) {
   const bool rhs_only = (material & MPRGP_RHS_ONLY) != 0;
   material &= ~MPRGP_RHS_ONLY;
   if (material != MPRGP_LINEAR && material != MPRGP_COROTATIONAL)
@@ -507,27 +545,32 @@ extern "C" int mprgp_fem_assemble(mprgp_fem *f, int material, const double *x_n,
   if (rhs_only && f->A->version == 0) return fail(MPRGP_ESTATE, "RHS-only assembly before any full assembly");
   if (rhs_only && material == MPRGP_COROTATIONAL && !f->rot_valid)
     return fail(MPRGP_ESTATE, "corotational RHS-only assembly needs a full corotational assembly first");
-  if (qp->A != f->A) return fail(MPRGP_EBADINPUT, "qp was not created on this FEM's matrix");
-  MPRGP_NOT_BUSY(qp);
-  if (!(h > 0.0) || !std::isfinite(h)) return fail(MPRGP_EBADINPUT, "h must be positive");
-  for (size_t i = 0; i < 3 * (size_t)f->N; ++i)
-    if (!std::isfinite(x_n[i]) || !std::isfinite(v_n[i])) return fail(MPRGP_EBADINPUT, "non-finite x_n/v_n");
+  return MPRGP_OK;
+}
+
+int check_finite(const double *v, size_t n, const char *what) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(v[i])) return fail(MPRGP_EBADINPUT, "non-finite %s", what);
+  return MPRGP_OK;
+}
+
+// A (unless RHS_ONLY) and b at the Newton iterate x1 from device arrays in
+// node order (buildLinearSystem, MprgpFemSolver.cpp:315-356).  The caller holds
+// the context lock, has checked the material and keeps the arrays alive.
+int assemble_dev(mprgp_fem *f, int material, const double *xn, const double *vn, const double *x1,
+                 const double *gravity, double h, mprgp_qp *qp) {
+  const bool rhs_only = (material & MPRGP_RHS_ONLY) != 0;
+  material &= ~MPRGP_RHS_ONLY;
   mprgp_ctx *ctx = f->ctx;
-  MPRGP_HIP(hipSetDevice(ctx->device));
-  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
-  const size_t need = sizeof(double) * ((size_t)kElemOut * f->T + 3 * 3 * (size_t)f->N) + 1024;
+  const size_t need = sizeof(double) * ((size_t)kElemOut * f->T + 3 * (size_t)f->N) + 1024;
   if (int rc = ctx->scratch_reserve(need)) return rc;
   const int N = f->N, T = f->T;
   double *elem = (double *)ctx->scratch;
-  double *dx = elem + (size_t)kElemOut * T;
-  double *dv = dx + 3 * (size_t)N;
-  double *db = dv + 3 * (size_t)N;
+  double *db = elem + (size_t)kElemOut * T;
   hipStream_t s = ctx->stream;
   const double h2 = h * h;
-  MPRGP_HIP(hipMemcpyAsync(dx, x_n, sizeof(double) * 3 * N, hipMemcpyHostToDevice, s));
-  MPRGP_HIP(hipMemcpyAsync(dv, v_n, sizeof(double) * 3 * N, hipMemcpyHostToDevice, s));
   const int corot = material == MPRGP_COROTATIONAL;
-  fem_element<<<(T + 127) / 128, 128, 0, s>>>(T, f->tets, f->dinv, f->vol, f->lam_mu, dx, corot,
+  fem_element<<<(T + 127) / 128, 128, 0, s>>>(T, f->tets, f->dinv, f->vol, f->lam_mu, x1, corot,
                                               rhs_only ? 1 : 0, rhs_only ? 1 : 0, h2, f->rot, elem);
   MPRGP_HIP(hipGetLastError());
   if (!rhs_only) {
@@ -535,12 +578,12 @@ extern "C" int mprgp_fem_assemble(mprgp_fem *f, int material, const double *x_n,
                                                                 f->blk_diag, f->mass_rho, elem, f->A->d_val);
     MPRGP_HIP(hipGetLastError());
   }
-  fem_rhs<<<(N + 255) / 256, 256, 0, s>>>(N, T, f->node_ptr, f->node_ctr, f->node_mass, f->density, elem,
-                                          dx, dv, gravity[0], gravity[1], gravity[2], h, db);
+  fem_rhs<<<(N + 255) / 256, 256, 0, s>>>(N, T, f->node_ptr, f->node_ctr, f->node_mass, f->density, elem, xn,
+                                          vn, x1, gravity[0], gravity[1], gravity[2], h, db);
   MPRGP_HIP(hipGetLastError());
-  // b += A x_n (MprgpFemSolver.cpp:349), in the QP's internal node order
+  // b += A x1 (MprgpFemSolver.cpp:349), in the QP's internal node order
   MPRGP_HIP(launch_gather_nodes(s, db, qp->b, f->A->d_perm, N));
-  MPRGP_HIP(launch_gather_nodes(s, dx, qp->stage, f->A->d_perm, N));
+  MPRGP_HIP(launch_gather_nodes(s, x1, qp->stage, f->A->d_perm, N));
   MPRGP_HIP(launch_spmv_plain(s, f->A->d_sl_off, f->A->d_col, f->A->d_val, qp->stage, qp->b, N, 1));
   if (!rhs_only) MPRGP_HIP(launch_setup_diag(s, f->A->d_sl_off, f->A->d_col, f->A->d_val, f->A->d_dinv, N));
   MPRGP_HIP(hipStreamSynchronize(s));
@@ -552,6 +595,194 @@ extern "C" int mprgp_fem_assemble(mprgp_fem *f, int material, const double *x_n,
   return MPRGP_OK;
 }
 
+}  // namespace
+
+extern "C" int mprgp_fem_assemble_at(mprgp_fem *f, int material, const double *x_n, const double *v_n,
+                                     const double *x1, const double *gravity, double h, mprgp_qp *qp) {
+  if (!f || !x_n || !v_n || !gravity || !qp) return fail(MPRGP_EBADINPUT, "NULL argument");
+  if (int rc = check_material(f, material)) return rc;
+  if (qp->A != f->A) return fail(MPRGP_EBADINPUT, "qp was not created on this FEM's matrix");
+  MPRGP_NOT_BUSY(qp);
+  if (!(h > 0.0) || !std::isfinite(h)) return fail(MPRGP_EBADINPUT, "h must be positive");
+  const size_t n = 3 * (size_t)f->N;
+  if (int rc = check_finite(x_n, n, "x_n")) return rc;
+  if (int rc = check_finite(v_n, n, "v_n")) return rc;
+  if (x1)
+    if (int rc = check_finite(x1, n, "x1")) return rc;
+  if (int rc = check_finite(gravity, 3, "gravity")) return rc;
+  mprgp_ctx *ctx = f->ctx;
+  MPRGP_HIP(hipSetDevice(ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
+  // scratch: assemble_dev's element array and b, then the staged host arrays
+  const size_t elem_b = (size_t)kElemOut * f->T + n;
+  if (int rc = ctx->scratch_reserve(sizeof(double) * (elem_b + 3 * n) + 1024)) return rc;
+  double *stage = (double *)ctx->scratch + elem_b;
+  hipStream_t s = ctx->stream;
+  MPRGP_HIP(hipMemcpyAsync(stage, x_n, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  MPRGP_HIP(hipMemcpyAsync(stage + n, v_n, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  if (x1) MPRGP_HIP(hipMemcpyAsync(stage + 2 * n, x1, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  return assemble_dev(f, material, stage, stage + n, x1 ? stage + 2 * n : stage, gravity, h, qp);
+}
+
+extern "C" int mprgp_fem_assemble(mprgp_fem *f, int material, const double *x_n, const double *v_n,
+                                  const double *gravity, double h, mprgp_qp *qp) {
+  return mprgp_fem_assemble_at(f, material, x_n, v_n, nullptr, gravity, h, qp);
+}
+
+// ---------------------------------------------------------------- time step
+
+extern "C" int mprgp_fem_set_state(mprgp_fem *f, const double *x, const double *v) {
+  if (!f || !x || !v) return fail(MPRGP_EBADINPUT, "NULL argument");
+  const size_t n = 3 * (size_t)f->N;
+  if (int rc = check_finite(x, n, "x")) return rc;
+  if (int rc = check_finite(v, n, "v")) return rc;
+  mprgp_ctx *ctx = f->ctx;
+  MPRGP_HIP(hipSetDevice(ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
+  if (!f->state) {
+    if (int rc = ctx->alloc((void **)&f->state, sizeof(double) * 4 * n)) return rc;
+    f->xn = f->state;
+    f->vn = f->xn + n;
+    f->x1 = f->vn + n;
+    f->x2 = f->x1 + n;
+  }
+  hipStream_t s = ctx->stream;
+  MPRGP_HIP(hipMemcpyAsync(f->xn, x, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  MPRGP_HIP(hipMemcpyAsync(f->vn, v, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  MPRGP_HIP(hipStreamSynchronize(s));
+  f->has_state = true;
+  f->step_open = false;
+  return MPRGP_OK;
+}
+
+extern "C" int mprgp_fem_get_state(mprgp_fem *f, double *x, double *v) {
+  if (!f) return fail(MPRGP_EBADINPUT, "NULL argument");
+  if (!f->has_state) return fail(MPRGP_ESTATE, "no state was set");
+  mprgp_ctx *ctx = f->ctx;
+  MPRGP_HIP(hipSetDevice(ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
+  const size_t n = 3 * (size_t)f->N;
+  hipStream_t s = ctx->stream;
+  if (x) MPRGP_HIP(hipMemcpyAsync(x, f->xn, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  if (v) MPRGP_HIP(hipMemcpyAsync(v, f->vn, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  MPRGP_HIP(hipStreamSynchronize(s));
+  return MPRGP_OK;
+}
+
+namespace {
+
+int check_step(mprgp_fem *f, mprgp_qp *qp, const mprgp_step_params *sp) {
+  if (!f || !qp || !sp) return fail(MPRGP_EBADINPUT, "NULL argument");
+  if (qp->A != f->A) return fail(MPRGP_EBADINPUT, "qp was not created on this FEM's matrix");
+  if (!f->has_state) return fail(MPRGP_ESTATE, "no state was set");
+  if (!(sp->h > 0.0) || !std::isfinite(sp->h)) return fail(MPRGP_EBADINPUT, "h must be positive");
+  if (int rc = check_finite(sp->gravity, 3, "gravity")) return rc;
+  if (sp->n_planes < 0 || (sp->n_planes > 0 && !sp->planes)) return fail(MPRGP_EBADINPUT, "bad planes");
+  return check_material(f, sp->material);
+}
+
+}  // namespace
+
+extern "C" int mprgp_fem_begin_step(mprgp_fem *f, mprgp_qp *qp, const mprgp_step_params *sp, int *rows) {
+  if (int rc = check_step(f, qp, sp)) return rc;
+  MPRGP_NOT_BUSY(qp);
+  mprgp_ctx *ctx = f->ctx;
+  MPRGP_HIP(hipSetDevice(ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
+  hipStream_t s = ctx->stream;
+  const int N = f->N;
+  const size_t n = 3 * (size_t)N;
+  // initPos/initVel: x1 = x_n; the QP's iterate (and reset point) is x_n
+  MPRGP_HIP(hipMemcpyAsync(f->x1, f->xn, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  MPRGP_HIP(launch_gather_nodes(s, f->xn, qp->x, f->A->d_perm, N));
+  MPRGP_HIP(hipMemcpyAsync(qp->x0, qp->x, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  qp->has_x = true;
+  // handleCollDetection: the planes at x_n (DCDhandle, MprgpFemSolver.cpp:255-271)
+  int m = 0;
+  if (int rc = mprgp_qp_collide_planes(qp, sp->n_planes, sp->planes, &m)) return rc;
+  if (rows) *rows = m;
+  // the first buildLinearSystem of forward's Newton loop, at x1 = x_n
+  if (int rc = assemble_dev(f, sp->material, f->xn, f->vn, f->x1, sp->gravity, sp->h, qp)) return rc;
+  f->step_open = true;
+  return MPRGP_OK;
+}
+
+extern "C" int mprgp_fem_end_step(mprgp_fem *f, mprgp_qp *qp, const mprgp_step_params *sp, double *delta) {
+  if (int rc = check_step(f, qp, sp)) return rc;
+  if (!f->step_open) return fail(MPRGP_ESTATE, "no step was begun");
+  MPRGP_NOT_BUSY(qp);
+  mprgp_ctx *ctx = f->ctx;
+  MPRGP_HIP(hipSetDevice(ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
+  hipStream_t s = ctx->stream;
+  const int N = f->N;
+  const size_t n = 3 * (size_t)N;
+  // updatePos: x2 = the solve's x (node order), tol_err = max |x2 - x1|
+  MPRGP_HIP(launch_scatter_nodes(s, qp->x, f->x2, f->A->d_perm, N));
+  if (delta) {
+    if (int rc = ctx->scratch_reserve(256)) return rc;
+    max_abs_diff<<<1, 1024, 0, s>>>(f->x2, f->x1, n, (double *)ctx->scratch);
+    MPRGP_HIP(hipGetLastError());
+    MPRGP_HIP(hipMemcpyAsync(delta, ctx->scratch, sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  // updateMesh: velocities from x2, then x_n = x2
+  euler_update<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, f->x2, f->xn, f->vn, sp->h);
+  MPRGP_HIP(hipGetLastError());
+  MPRGP_HIP(hipStreamSynchronize(s));
+  f->step_open = false;
+  return MPRGP_OK;
+}
+
+extern "C" int mprgp_fem_advance(mprgp_fem *f, mprgp_qp *qp, const mprgp_step_params *sp, const mprgp_params *p,
+                                 mprgp_step_stats *out) {
+  if (int rc = check_step(f, qp, sp)) return rc;
+  if (!p) return fail(MPRGP_EBADINPUT, "NULL params");
+  if (sp->newton_max_it < 1) return fail(MPRGP_EBADINPUT, "newton_max_it must be at least 1");
+  if (!(sp->newton_eps >= 0.0)) return fail(MPRGP_EBADINPUT, "newton_eps must be >= 0");
+  MPRGP_NOT_BUSY(qp);
+  mprgp_ctx *ctx = f->ctx;
+  MPRGP_HIP(hipSetDevice(ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
+  mprgp_step_stats st;
+  std::memset(&st, 0, sizeof(st));
+  if (int rc = mprgp_fem_begin_step(f, qp, sp, &st.rows)) return rc;
+  // forward (MprgpFemSolver.cpp:514-537): x = P(x1), solve, updatePos, until
+  // max |x - x1| < eps or maxIter solves
+  mprgp_params sp_solve = *p;
+  sp_solve.project_first = 1;
+  hipStream_t s = ctx->stream;
+  const int N = f->N;
+  const size_t n = 3 * (size_t)N;
+  int worst = MPRGP_OK;
+  for (int it = 0; it < sp->newton_max_it; ++it) {
+    if (it > 0)
+      if (int rc = assemble_dev(f, sp->material, f->xn, f->vn, f->x1, sp->gravity, sp->h, qp)) return rc;
+    mprgp_stats ss;
+    const int code = mprgp_qp_solve(qp, &sp_solve, &ss);
+    if (code != MPRGP_OK && code != MPRGP_NOT_CONVERGED) return code;
+    if (code != MPRGP_OK) worst = code;
+    st.newton_iters = it + 1;
+    st.iterations += ss.iterations;
+    st.last = ss;
+    MPRGP_HIP(launch_scatter_nodes(s, qp->x, f->x2, f->A->d_perm, N));
+    if (int rc = ctx->scratch_reserve(256)) return rc;
+    max_abs_diff<<<1, 1024, 0, s>>>(f->x2, f->x1, n, (double *)ctx->scratch);
+    MPRGP_HIP(hipGetLastError());
+    MPRGP_HIP(hipMemcpyAsync(&st.delta, ctx->scratch, sizeof(double), hipMemcpyDeviceToHost, s));
+    MPRGP_HIP(hipStreamSynchronize(s));
+    std::swap(f->x1, f->x2);  // x1 = the new iterate
+    if (st.delta < sp->newton_eps) break;
+  }
+  // updateMesh from the last iterate (now in x1)
+  euler_update<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, f->x1, f->xn, f->vn, sp->h);
+  MPRGP_HIP(hipGetLastError());
+  MPRGP_HIP(hipStreamSynchronize(s));
+  f->step_open = false;
+  st.code = worst;
+  if (out) *out = st;
+  return worst;
+}
+
 extern "C" void mprgp_fem_destroy(mprgp_fem *f) {
   if (!f) return;
   mprgp_ctx *ctx = f->ctx;
@@ -559,6 +790,7 @@ extern "C" void mprgp_fem_destroy(mprgp_fem *f) {
   std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
   (void)hipStreamSynchronize(ctx->stream);
   ctx->release(f->dev, f->dev_bytes);
+  if (f->state) ctx->release(f->state, sizeof(double) * 4 * 3 * (size_t)f->N);
   mprgp_matrix_destroy(f->A);
   delete f;
 }

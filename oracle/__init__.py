@@ -95,6 +95,11 @@ def lib():
                                            C.POINTER(C.POINTER(C.c_int)),
                                            C.POINTER(C.POINTER(C.c_double)), P]
         L.orc_free.argtypes = [P]
+        L.orc_fem_build_system_at.argtypes = [I, I, P, P, P, P, P, P, D, P, P, P, P, D, I,
+                                              C.POINTER(C.POINTER(C.c_int)),
+                                              C.POINTER(C.POINTER(C.c_int)),
+                                              C.POINTER(C.POINTER(C.c_double)), P]
+        L.orc_implicit_euler_velocity.argtypes = [I, P, P, P, D, P]
         L.orc_plane_collide.argtypes = [I, P, I, P, P, P, P]
         L.orc_plane_collide.restype = I
         _lib = L
@@ -289,24 +294,25 @@ def fem_element(mtype, xe, d, vol, lam, mu):
 
 
 def fem_build_system(n_nodes, tets_oriented, d, vol, mass, young, poisson, density,
-                     x_n, v_n, gravity, h, mtype):
-    """Returns (A as scipy CSR, b)."""
+                     x_n, v_n, gravity, h, mtype, x1=None):
+    """Returns (A as scipy CSR, b) at the Newton iterate x1 (None: x_n)."""
     tets = _i32(tets_oriented)
     T = tets.shape[0]
     d, vol, mass = _f64(d), _f64(vol), _f64(mass)
     young = _f64(np.broadcast_to(young, (T,)))
     poisson = _f64(np.broadcast_to(poisson, (T,)))
     x_n, v_n, gravity = _f64(x_n), _f64(v_n), _f64(gravity)
+    x1 = x_n if x1 is None else _f64(x1)
     n = 3 * n_nodes
     b = np.empty(n)
     pp = C.POINTER(C.c_int)()
     ip = C.POINTER(C.c_int)()
     vp = C.POINTER(C.c_double)()
     L = lib()
-    L.orc_fem_build_system(int(n_nodes), int(T), _ptr(tets), _ptr(d), _ptr(vol), _ptr(mass),
-                           _ptr(young), _ptr(poisson), float(density), _ptr(x_n), _ptr(v_n),
-                           _ptr(gravity), float(h), int(mtype), C.byref(pp), C.byref(ip),
-                           C.byref(vp), _ptr(b))
+    L.orc_fem_build_system_at(int(n_nodes), int(T), _ptr(tets), _ptr(d), _ptr(vol), _ptr(mass),
+                              _ptr(young), _ptr(poisson), float(density), _ptr(x_n), _ptr(v_n),
+                              _ptr(x1), _ptr(gravity), float(h), int(mtype), C.byref(pp), C.byref(ip),
+                              C.byref(vp), _ptr(b))
     ptr = np.ctypeslib.as_array(pp, shape=(n + 1,)).copy()
     nnz = int(ptr[-1])
     idx = np.ctypeslib.as_array(ip, shape=(nnz,)).copy()
@@ -333,3 +339,44 @@ def plane_collide(x, planes):
     if m < 0:
         raise ValueError("plane normal is not unit length")
     return node[:m].copy(), normal[:m].copy(), c[:m].copy()
+
+
+def implicit_euler_velocity(x_n, v_n, x, h):
+    """updateMesh's velocity under implicit Euler (MprgpFemSolver.cpp:377-390)."""
+    x_n, v_n, x = _f64(x_n), _f64(v_n), _f64(x)
+    v = np.empty(x.size)
+    lib().orc_implicit_euler_velocity(int(x.size), _ptr(x_n), _ptr(v_n), _ptr(x), float(h), _ptr(v))
+    return v
+
+
+def advance(n_nodes, tets_oriented, d, vol, mass, young, poisson, density, x_n, v_n, gravity, h, mtype,
+            planes, newton_max_it=10, newton_eps=1e-4, tol=1e-6, max_it=1000, alpha_bar=0.0, ev=None):
+    """One time step of MprgpFemSolver::advance (MprgpFemSolver.cpp:193-203)
+    with the simulator's implicit Euler (Simulator.cpp:313): the planes'
+    collision at x_n (DCDhandle, :255-271), DecoupledMprgpFemSolver::forward's
+    Newton loop (:514-537: buildLinearSystem at x1, x = P(x1), MPRGP solve,
+    updatePos :358-375 until max|x - x1| < eps or maxIter solves), then
+    updateMesh (:377-390).  Returns (x, v, info dict)."""
+    x_n, v_n = _f64(x_n).ravel(), _f64(v_n).ravel()
+    node, normal, c = plane_collide(x_n.reshape(-1, 3), planes)
+    m = node.size
+    J = sp.csr_matrix((normal.ravel(), (np.repeat(np.arange(m), 3),
+                                        (3 * node[:, None] + np.arange(3)).ravel())),
+                      shape=(m, 3 * n_nodes))
+    x1 = x_n.copy()
+    info = {"rows": m, "newton_iters": 0, "codes": [], "iterations": 0, "delta": 0.0}
+    for it in range(newton_max_it):
+        A, b = fem_build_system(n_nodes, tets_oriented, d, vol, mass, young, poisson, density, x_n, v_n,
+                                gravity, h, mtype, x1=x1)
+        code, x, st, ev = decoupled_solve(A, b, J, c, x1, tol=tol, max_it=max_it, alpha_bar=alpha_bar, ev=ev,
+                                          project_first=True)
+        info["codes"].append(code)
+        info["iterations"] += st["iterations"]
+        info["newton_iters"] = it + 1
+        delta = float(np.max(np.abs(x - x1))) if x.size else 0.0
+        info["delta"] = delta
+        x1 = x
+        if delta < newton_eps:
+            break
+    info["ev"] = ev
+    return x1, implicit_euler_velocity(x_n, v_n, x1, h), info

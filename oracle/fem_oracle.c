@@ -22,6 +22,14 @@
  *                        MprgpFemSolver.cpp:295-356
  *                        -> for LINEAR, b equals the closed form M(x_n + h v_n) + h^2(K X0 + M g);
  *                           A symmetric; A - M = h^2 sum K_e; sum of masses = density * volume
+ *   orc_fem_build_system_at  the same at a Newton iterate x1 (MprgpFemSolver.cpp:315-356)
+ *                        (tests/test_oracle_step.py)
+ *                        -> for LINEAR, b does not depend on x1 (the closed form above); for
+ *                           COROT, b - A x1 - M(x_n + h v_n - x1) - h^2 M g equals h^2 times
+ *                           minus the finite-difference gradient of the total energy at x1
+ *   orc_implicit_euler_velocity  MprgpFemSolver.cpp:295-313, 377-390 with FEMSystem.h:140-142
+ *                        -> equals (x - x_n)/h to rounding; a rigid free fall of one step from
+ *                           the rest shape gives x = X0 + h v + h^2 g and v' = v + h g
  */
 #include "mprgp_oracle.h"
 
@@ -291,12 +299,12 @@ static int cmp_int(const void *a, const void *b) {
   return (x > y) - (x < y);
 }
 
-int orc_fem_build_system(int n_nodes, int n_tets, const int *tets,
-                         const double *d, const double *vol, const double *node_mass,
-                         const double *young, const double *poisson, double density,
-                         const double *x_n, const double *v_n, const double *gravity,
-                         double h, int type,
-                         int **ptr_out, int **idx_out, double **val_out, double *b) {
+int orc_fem_build_system_at(int n_nodes, int n_tets, const int *tets,
+                            const double *d, const double *vol, const double *node_mass,
+                            const double *young, const double *poisson, double density,
+                            const double *x_n, const double *v_n, const double *x1,
+                            const double *gravity, double h, int type,
+                            int **ptr_out, int **idx_out, double **val_out, double *b) {
   /* node adjacency (including self), sorted */
   int *deg = (int *)xcalloc((size_t)n_nodes + 1, sizeof(int));
   for (int t = 0; t < n_tets; ++t)
@@ -344,7 +352,7 @@ int orc_fem_build_system(int n_nodes, int n_tets, const int *tets,
     const int *v = tets + 4 * t;
     double xe[12], fB[12], HB[144];
     for (int i = 0; i < 4; ++i)
-      for (int r = 0; r < 3; ++r) xe[i * 3 + r] = x_n[3 * v[i] + r];
+      for (int r = 0; r < 3; ++r) xe[i * 3 + r] = x1[3 * v[i] + r];
     double E = young[t], nu = poisson[t];
     double mu = E / (2.0 * (1.0 + nu));
     double lam = nu * E / ((1.0 + nu) * (1.0 - 2.0 * nu));
@@ -373,7 +381,7 @@ int orc_fem_build_system(int n_nodes, int n_tets, const int *tets,
     }
     for (int r = 0; r < 3; ++r) {
       double psi = x_n[3 * a + r] + h * v_n[3 * a + r];
-      b[3 * a + r] += M * (psi - x_n[3 * a + r]);
+      b[3 * a + r] += M * (psi - x1[3 * a + r]);
       /* diagonal entry of row 3a+r */
       int lo = nptr[a], hi = nptr[a + 1] - 1;
       while (lo < hi) {
@@ -384,13 +392,35 @@ int orc_fem_build_system(int n_nodes, int n_tets, const int *tets,
       val[ptr[3 * a + r] + 3 * kb + r] += M;
     }
   }
-  /* RHS += LHS * x1 (MprgpFemSolver.cpp:349), x1 = x_n */
+  /* RHS += LHS * x1 (MprgpFemSolver.cpp:349) */
   for (int i = 0; i < n; ++i) {
     double s = 0.0;
-    for (int k = ptr[i]; k < ptr[i + 1]; ++k) s += val[k] * x_n[idx[k]];
+    for (int k = ptr[i]; k < ptr[i + 1]; ++k) s += val[k] * x1[idx[k]];
     b[i] += s;
   }
   free(nptr); free(nadj);
   *ptr_out = ptr; *idx_out = idx; *val_out = val;
   return 0;
+}
+
+int orc_fem_build_system(int n_nodes, int n_tets, const int *tets,
+                         const double *d, const double *vol, const double *node_mass,
+                         const double *young, const double *poisson, double density,
+                         const double *x_n, const double *v_n, const double *gravity,
+                         double h, int type,
+                         int **ptr_out, int **idx_out, double **val_out, double *b) {
+  return orc_fem_build_system_at(n_nodes, n_tets, tets, d, vol, node_mass, young, poisson, density, x_n,
+                                 v_n, x_n, gravity, h, type, ptr_out, idx_out, val_out, b);
+}
+
+/* updateMesh under implicit Euler (beta = 1, beta2 = 0.5, gamma = 1): the
+ * previous acceleration enters PHI and PSI with the factors 1 - gamma = 0 and
+ * 1 - 2 beta2 = 0, so a = (x - (x_n + h v_n)) / (h h), v = v_n + a h. */
+void orc_implicit_euler_velocity(int n, const double *x_n, const double *v_n, const double *x,
+                                 double h, double *v) {
+  for (int i = 0; i < n; ++i) {
+    double psi = x_n[i] + h * v_n[i];
+    double a = (x[i] - psi) / ((1.0 * h) * h);
+    v[i] = v_n[i] + (a * 1.0) * h;
+  }
 }

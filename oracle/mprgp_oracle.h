@@ -121,6 +121,19 @@ int orc_fem_build_system(int n_nodes, int n_tets, const int *tets_oriented,
                          const double *x_n, const double *v_n, const double *gravity,
                          double h, int type,
                          int **ptr_out, int **idx_out, double **val_out, double *b_out);
+/* The same at a Newton iterate x1 (buildLinearSystem after updatePos,
+ * MprgpFemSolver.cpp:315-356): elements evaluated at x1,
+ * b = h^2 f(x1) + h^2 M g + M (x_n + h v_n - x1) + A x1. */
+int orc_fem_build_system_at(int n_nodes, int n_tets, const int *tets_oriented,
+                            const double *d, const double *vol, const double *node_mass,
+                            const double *young, const double *poisson, double density,
+                            const double *x_n, const double *v_n, const double *x1,
+                            const double *gravity, double h, int type,
+                            int **ptr_out, int **idx_out, double **val_out, double *b_out);
+/* MprgpFemSolver::updateMesh (MprgpFemSolver.cpp:377-390) with the implicit
+ * Euler parameters of FEMSolver::resetImplicitEuler (FEMSystem.h:140-142). */
+void orc_implicit_euler_velocity(int n, const double *x_n, const double *v_n, const double *x,
+                                 double h, double *v);
 void orc_free(void *p);
 
 /* ---- plane collider (collide_oracle.c) --------------------------------- */
