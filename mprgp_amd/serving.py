@@ -10,12 +10,18 @@ device-resident QP.  A timestep (one request) is
   A and b at x_n, and a fresh alpha_bar;
 * otherwise: right-hand side only (``RHS_ONLY``: the lagged matrix is kept);
 * every step: the collider's rows from x_n (a node within the planes' band
-  gets one row, as the reference's collision handling does per step), made
-  on the device from the QP's iterate (``QP.collide_planes``), or on the
-  host and uploaded (``device_rows=False``);
+  gets one row, as the reference's collision handling does per step);
 * the solve from the warm start x_n (projected first, as
   DecoupledMprgpFemSolver::forward does, MprgpFemSolver.cpp:528-533);
-* x_{n+1} = x, v_{n+1} = (x - x_n) / h.
+* x_{n+1} = x and updateMesh's implicit-Euler velocity,
+  v_{n+1} = v_n + ((x - (x_n + h v_n)) / (h h)) h.
+
+By default a session's positions and velocities live on the device
+(``Fem.begin_step`` before the solve: collision, assembly, warm start;
+``Fem.end_step`` after it: positions and velocities), and the host only
+reads them back for the rotation of a re-assembly step.  With
+``device_state=False`` the host keeps them, assembles from host arrays and
+uploads the host collider's rows; the two kinds step bit-identically.
 
 ``run_open_loop`` drives the sessions with a Poisson stream of requests
 through an :class:`mprgp_amd.Engine`; ``run_one_at_a_time`` serves the same
@@ -49,7 +55,8 @@ class Session:
     history: list = dataclasses.field(default_factory=list)
     pending: collections.deque = dataclasses.field(default_factory=collections.deque)
     busy: bool = False
-    device_rows: bool = True
+    device_state: bool = True
+    sp: object = None  # StepParams of the open step (device state)
 
     @property
     def n_nodes(self):
@@ -59,6 +66,9 @@ class Session:
         """Host + device work before the solve of the current step."""
         sc = self.scene
         full = self.step % self.reassemble == 0
+        if self.device_state and self.step > 0 and full:
+            x, v = self.fem.get_state()
+            sc.x_n, sc.v_n = x.reshape(-1, 3), v.reshape(-1, 3)
         if full and self.step > 0:
             ang = math.radians(self.rng.uniform(1.0, 5.0))
             axis = self.rng.normal(size=3)
@@ -76,26 +86,42 @@ class Session:
             sc.x_n = x
             sc.v_n = sc.v_n @ R.T
         material = M.COROTATIONAL if full else (M.COROTATIONAL | M.RHS_ONLY)
+        if self.device_state:
+            if self.step == 0 or full:
+                self.fem.set_state(sc.x_n.ravel(), sc.v_n.ravel())
+            self.sp = self.fem.step_params(h=sc.h, gravity=sc.gravity, material=material,
+                                           planes=problems.plane_array(sc.planes), newton_max_it=1)
+            self.fem.begin_step(self.qp, self.sp)
+            return
         self.fem.assemble(self.qp, sc.x_n.ravel(), sc.v_n.ravel(), sc.gravity, sc.h, material)
         if self.step == 0 or full:  # otherwise the QP's x is the last solution = x_n
             self.qp.set_x(sc.x_n.ravel())
-        if self.device_rows:
-            self.qp.collide_planes(problems.plane_array(sc.planes))
-        else:
-            node, normal, c = sc.rows()
-            self.qp.set_rows(node, normal, c)
+        node, normal, c = sc.rows()
+        self.qp.set_rows(node, normal, c)
 
     def complete(self, keep_history=False, rc=0, stats=None):
         sc = self.scene
         if stats is not None and self.step % self.reassemble == 0:
             self.alpha_bar = stats["alpha_bar"]
+        if self.device_state:
+            if stats is not None and not math.isfinite(stats["residual"]):
+                raise FloatingPointError(f"session {self.sid} ({sc.name}, {3 * self.n_nodes} DoF) step "
+                                         f"{self.step}: non-finite residual, code {rc}, stats {stats}")
+            self.fem.end_step(self.qp, self.sp)
+            if keep_history:
+                self.history.append(self.fem.get_state()[0].reshape(-1, 3))
+            self.step += 1
+            return
         x = self.qp.get_x().reshape(-1, 3)
         if not np.all(np.isfinite(x)):
             raise FloatingPointError(f"session {self.sid} ({sc.name}, {3 * self.n_nodes} DoF) step "
                                      f"{self.step}: non-finite solution, code {rc}, stats {stats}")
         if keep_history:
             self.history.append(x.copy())
-        sc.v_n = (x - sc.x_n) / sc.h
+        # updateMesh under implicit Euler, in the reference's arithmetic
+        psi = sc.x_n + sc.h * sc.v_n
+        acc = (x - psi) / ((1.0 * sc.h) * sc.h)
+        sc.v_n = sc.v_n + (acc * 1.0) * sc.h
         sc.x_n = x
         self.step += 1
 
@@ -138,7 +164,7 @@ def session_scene(rng, sid, dof_range=(3e3, 1.5e5)):
 
 
 def make_sessions(ctx, count=64, seed=5, dof_range=(3e3, 1.5e5), reassemble=10, own_contexts=False,
-                  device_rows=True):
+                  device_state=True):
     """Sessions on `ctx`, or each on a context of its own (same device) so
     that the host work of different sessions does not share one stream."""
     rng = np.random.default_rng(seed)
@@ -151,7 +177,7 @@ def make_sessions(ctx, count=64, seed=5, dof_range=(3e3, 1.5e5), reassemble=10, 
         fem = M.Fem(sctx, sc.X0, sc.tets, sc.young, sc.poisson, sc.density)
         qp = fem.matrix.qp()
         out.append(Session(i, sc, fem, qp, reassemble=reassemble,
-                           rng=np.random.default_rng([seed, i]), device_rows=device_rows))
+                           rng=np.random.default_rng([seed, i]), device_state=device_state))
     # alpha_bar per session from its first matrix (power method, batched)
     for s in out:
         s.prepare()

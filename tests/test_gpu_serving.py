@@ -127,20 +127,28 @@ def test_open_loop_small(ctx):
     eng.close()
 
 
-def test_device_collider_sessions_equal_host_rows(ctx):
-    """Sessions whose rows the device collider makes step bit-identically to
-    sessions that upload the host collider's rows."""
+def test_device_sessions_equal_host_sessions(ctx):
+    """Sessions whose state, rows and velocity updates stay on the device
+    (begin_step/end_step) step bit-identically to sessions that keep the
+    state on the host, assemble from host arrays and upload the host
+    collider's rows; re-assembly steps (every 2nd) read the device state back
+    for the rotation."""
     dev = serving.make_sessions(ctx, count=3, seed=15, dof_range=(1.5e3, 9e3), reassemble=2)
     host = serving.make_sessions(ctx, count=3, seed=15, dof_range=(1.5e3, 9e3), reassemble=2,
-                                 device_rows=False)
+                                 device_state=False)
     for sd, sh in zip(dev, host):
-        for step in range(4):
+        for step in range(5):
             for s in (sd, sh):
                 s.prepare()
                 rc, st = s.qp.solve(serving.solve_params(s, 1e-6, prec(s), tiles=4))
                 assert rc == 0, (s.sid, step, st)
+                if s is sd:
+                    nd = s.qp.get_rows()
+                else:
+                    nh = s.qp.get_rows()
                 s.complete(keep_history=True, rc=rc, stats=st)
-            nd, nh = sd.qp.get_rows(), sh.qp.get_rows()
             for a, b in zip(nd, nh):
                 np.testing.assert_array_equal(a, b)
             np.testing.assert_array_equal(sd.history[-1], sh.history[-1], err_msg=f"{sd.sid} {step}")
+        xd, vd = sd.fem.get_state()
+        np.testing.assert_array_equal(vd.reshape(-1, 3), sh.scene.v_n)
