@@ -226,6 +226,13 @@ int mprgp_fem_assemble(mprgp_fem *fem, int material, const double *x_n, const do
                        const double *gravity, double h, mprgp_qp *qp);
 mprgp_matrix *mprgp_fem_matrix(mprgp_fem *fem);
 void mprgp_fem_destroy(mprgp_fem *fem);
+/* Rayleigh damping of the body (FEMSystem::setDamping, FEMSystem.cpp:122-126,
+ * as Simulator.cpp:159-167 sets it from the scene's alpha_k and alpha_m; a
+ * new FEM has none, as a new FEMSystem).  Assemblies then write
+ * A = M (1 + h alpha_k) + K (h^2 + h alpha_m) and add
+ * alpha_m K (x_n - x1) h + M (x_n - x1) h alpha_k to b (buildSystem,
+ * FEMSystem.cpp:167-192, with CRHS = (x_n - x1) h). */
+int mprgp_fem_set_damping(mprgp_fem *fem, double alpha_k, double alpha_m);
 /* mprgp_fem_assemble at a Newton iterate x1 (buildLinearSystem after
  * updatePos, MprgpFemSolver.cpp:315-356): elements evaluated at x1,
  * b = h^2 f(x1) + h^2 M g + M (x_n + h v_n - x1) + A x1.  x1 = NULL means x_n. */

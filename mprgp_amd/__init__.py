@@ -173,6 +173,10 @@ class Fem:
         check(load().mprgp_fem_assemble_at(self.h, int(material), _p(x), _p(v), None if y is None else _p(y),
                                            _p(g), float(h), qp.h))
 
+    def set_damping(self, alpha_k=0.0, alpha_m=0.0):
+        """Rayleigh damping: mass term alpha_k, stiffness term alpha_m (the scene's keys)."""
+        check(load().mprgp_fem_set_damping(self.h, float(alpha_k), float(alpha_m)))
+
     def set_state(self, x, v):
         check(load().mprgp_fem_set_state(self.h, _p(_f64(x, 3 * self.n_nodes)),
                                          _p(_f64(v, 3 * self.n_nodes))))

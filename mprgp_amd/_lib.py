@@ -140,6 +140,7 @@ SIGNATURES = {
     "mprgp_fem_create": (I, [P, I, I, P, P, P, P, D, PP]),
     "mprgp_fem_assemble": (I, [P, I, P, P, P, D, P]),
     "mprgp_fem_assemble_at": (I, [P, I, P, P, P, P, D, P]),
+    "mprgp_fem_set_damping": (I, [P, D, D]),
     "mprgp_fem_set_state": (I, [P, P, P]),
     "mprgp_fem_get_state": (I, [P, P, P]),
     "mprgp_fem_advance": (I, [P, P, C.POINTER(StepParams), C.POINTER(Params), C.POINTER(StepStats)]),

@@ -39,7 +39,8 @@
 
 namespace mprgp {
 
-constexpr int kElemOut = 16 * 9 + 12;  // h^2 K_e blocks, then h^2 f_e
+// per tet: kc K_e blocks, then h^2 f_e, then K_e c_e (the damping term)
+constexpr int kElemOut = 16 * 9 + 12 + 12;
 
 __device__ inline double det3(const double *M) {
   return M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) +
@@ -165,13 +166,17 @@ __device__ void polar_rotation(const double *F, double *R) {
 // rotation of F and is stored; corot && lag: R is the stored rotation of the
 // last full assembly (the element is then linear in x with exactly the
 // stiffness of that assembly).  rhs_only skips the stiffness blocks.
+// Outputs (FEMSystem::buildSystem, FEMSystem.cpp:167-192): the stiffness
+// blocks times kc = h^2 + h alpha_m, the force times h^2, and with damp set
+// the unscaled K_e c_e for c = (x_n - x1) h (the stiffness-damping term, CRHS).
 __global__ void __launch_bounds__(128) fem_element(int T, const int4 *__restrict__ tets,
                                                    const double *__restrict__ dinv,
                                                    const double *__restrict__ vol,
                                                    const double *__restrict__ lam_mu,
-                                                   const double *__restrict__ x, int corot, int lag,
-                                                   int rhs_only, double h2, double *__restrict__ rot,
-                                                   double *__restrict__ out) {
+                                                   const double *__restrict__ x,
+                                                   const double *__restrict__ xn, int corot, int lag,
+                                                   int rhs_only, int damp, double kc, double h2, double h,
+                                                   double *__restrict__ rot, double *__restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= T) return;
   const int4 v = tets[t];
@@ -241,16 +246,25 @@ __global__ void __launch_bounds__(128) fem_element(int T, const int4 *__restrict
       for (int c = 0; c < 3; ++c) s += R[r * 3 + c] * B[I][c];
       u[I][r] = s;
     }
-  // h^2 K_e blocks
-  for (int I = 0; I < 4 && !rhs_only; ++I)
+  // kc K_e blocks; K_e c_e for the damping term
+  double ce[12];
+  if (damp)
+    for (int i = 0; i < 4; ++i)
+      for (int r = 0; r < 3; ++r) ce[i * 3 + r] = (xn[3 * (size_t)vi[i] + r] - xe[i * 3 + r]) * h;
+  for (int I = 0; I < 4 && (!rhs_only || damp); ++I) {
+    double kd[3] = {0.0, 0.0, 0.0};
     for (int J = 0; J < 4; ++J) {
       const double bb = (B[I][0] * B[J][0] + B[I][1] * B[J][1]) + B[I][2] * B[J][2];
       for (int r = 0; r < 3; ++r)
         for (int s = 0; s < 3; ++s) {
           double k = V * (lam * u[I][r] * u[J][s] + mu * (RRt[r * 3 + s] * bb + u[J][r] * u[I][s]));
-          out[(size_t)(((I * 4 + J) * 9) + r * 3 + s) * T + t] = k * h2;
+          if (!rhs_only) out[(size_t)(((I * 4 + J) * 9) + r * 3 + s) * T + t] = k * kc;
+          if (damp) kd[r] += k * ce[J * 3 + s];
         }
     }
+    if (damp)
+      for (int r = 0; r < 3; ++r) out[(size_t)(156 + I * 3 + r) * T + t] = kd[r];
+  }
   // h^2 f_e
   for (int I = 0; I < 4; ++I)
     for (int r = 0; r < 3; ++r) {
@@ -265,7 +279,7 @@ __global__ void __launch_bounds__(128) fem_element(int T, const int4 *__restrict
 // its SELL slot (value base vbase[b], entry planes 64 apart).
 __global__ void fem_blocks(long long nb, int T, const int *__restrict__ ptr, const int *__restrict__ ctr,
                            const int *__restrict__ vbase, const int *__restrict__ diag_node,
-                           const double *__restrict__ mass, const double *__restrict__ elem,
+                           const double *__restrict__ mass, double mc, const double *__restrict__ elem,
                            double *__restrict__ val) {
   const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
@@ -278,7 +292,7 @@ __global__ void fem_blocks(long long nb, int T, const int *__restrict__ ptr, con
   }
   const int dn = diag_node[b];
   if (dn >= 0) {
-    const double M = mass[dn];
+    const double M = mass[dn] * mc;  // M (1 + h alpha_k)
     acc[0] += M;
     acc[4] += M;
     acc[8] += M;
@@ -287,15 +301,17 @@ __global__ void fem_blocks(long long nb, int T, const int *__restrict__ ptr, con
   for (int e = 0; e < 9; ++e) o[(size_t)e * kSlice] = acc[e];
 }
 
-// b (node order, before A x1): sum of h^2 f_e(x1) in tet order, then gravity
-// h^2 rho m g, then M (x_n + h v_n - x1)  (FEMSystem.cpp:176-186 with
-// MRHSL = PSI - X1, MprgpFemSolver.cpp:336-340).  x1 is the Newton iterate
-// (x_n on the first iteration).
+// b (node order, before A x1): sum of h^2 f_e(x1) in tet order, then the
+// stiffness damping alpha_m K c, then gravity h^2 rho m g, then
+// M ((x_n + h v_n - x1) + c alpha_k) with c = (x_n - x1) h  (FEMSystem.cpp:176-186
+// with MRHSL = PSI - X1 and CRHS = c, MprgpFemSolver.cpp:336-340).  x1 is the
+// Newton iterate (x_n on the first iteration, where c = 0).
 __global__ void fem_rhs(int N, int T, const int *__restrict__ ptr, const int *__restrict__ ctr,
                         const double *__restrict__ node_mass, double density,
                         const double *__restrict__ elem, const double *__restrict__ x,
                         const double *__restrict__ v, const double *__restrict__ x1, double g0, double g1,
-                        double g2, double h, double *__restrict__ b) {
+                        double g2, double h, int damp, double alpha_k, double alpha_m,
+                        double *__restrict__ b) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= N) return;
   double s[3] = {0.0, 0.0, 0.0};
@@ -304,14 +320,24 @@ __global__ void fem_rhs(int N, int T, const int *__restrict__ ptr, const int *__
     const int t = code >> 2, I = code & 3;
     for (int r = 0; r < 3; ++r) s[r] += elem[(size_t)(144 + I * 3 + r) * T + t];
   }
+  if (damp) {
+    double kd[3] = {0.0, 0.0, 0.0};
+    for (int k = ptr[a]; k < ptr[a + 1]; ++k) {
+      const int code = ctr[k];
+      const int t = code >> 2, I = code & 3;
+      for (int r = 0; r < 3; ++r) kd[r] += elem[(size_t)(156 + I * 3 + r) * T + t];
+    }
+    for (int r = 0; r < 3; ++r) s[r] += kd[r] * alpha_m;
+  }
   const double g[3] = {g0, g1, g2};
   const double m = node_mass[a];
   const double M = m * density;
   const double h2 = h * h;
   for (int r = 0; r < 3; ++r) s[r] += ((g[r] * density) * m) * h2;
   for (int r = 0; r < 3; ++r) {
-    const double psi = x[3 * (size_t)a + r] + h * v[3 * (size_t)a + r];
-    s[r] += M * (psi - x1[3 * (size_t)a + r]);
+    const double xn = x[3 * (size_t)a + r], y = x1[3 * (size_t)a + r];
+    const double psi = xn + h * v[3 * (size_t)a + r];
+    s[r] += M * ((psi - y) + ((xn - y) * h) * alpha_k);
   }
   for (int r = 0; r < 3; ++r) b[3 * (size_t)a + r] = s[r];
 }
@@ -365,6 +391,7 @@ struct mprgp_fem {
   double *rot = nullptr;   // per-tet rotations of the last corotational assembly
   bool rot_valid = false;
   double density = 0.0;
+  double alpha_k = 0.0, alpha_m = 0.0;  // Rayleigh damping (FEMSystem::setDamping)
   // device-resident body state (node order): positions x_n, velocities v_n,
   // the Newton iterate x1 and the next one x2 (allocated by set_state)
   double *state = nullptr;
@@ -568,18 +595,26 @@ int assemble_dev(mprgp_fem *f, int material, const double *xn, const double *vn,
   double *elem = (double *)ctx->scratch;
   double *db = elem + (size_t)kElemOut * T;
   hipStream_t s = ctx->stream;
-  const double h2 = h * h;
+  // buildLinearSystem's coefficients with beta = 1 (MprgpFemSolver.cpp:331-339):
+  // bdt = h, KCoef = FCoef = bdt h, CCoef = bdt, MCoef = 1
+  const double bdt = 1.0 * h;
+  const double h2 = bdt * h;
+  const double kc = h2 + bdt * f->alpha_m;
+  const double mc = 1.0 + bdt * f->alpha_k;
+  const int damp = f->alpha_m != 0.0 ? 1 : 0;
   const int corot = material == MPRGP_COROTATIONAL;
-  fem_element<<<(T + 127) / 128, 128, 0, s>>>(T, f->tets, f->dinv, f->vol, f->lam_mu, x1, corot,
-                                              rhs_only ? 1 : 0, rhs_only ? 1 : 0, h2, f->rot, elem);
+  fem_element<<<(T + 127) / 128, 128, 0, s>>>(T, f->tets, f->dinv, f->vol, f->lam_mu, x1, xn, corot,
+                                              rhs_only ? 1 : 0, rhs_only ? 1 : 0, damp, kc, h2, bdt, f->rot,
+                                              elem);
   MPRGP_HIP(hipGetLastError());
   if (!rhs_only) {
     fem_blocks<<<(unsigned)((f->nb + 255) / 256), 256, 0, s>>>(f->nb, T, f->blk_ptr, f->blk_ctr, f->blk_vbase,
-                                                                f->blk_diag, f->mass_rho, elem, f->A->d_val);
+                                                                f->blk_diag, f->mass_rho, mc, elem, f->A->d_val);
     MPRGP_HIP(hipGetLastError());
   }
   fem_rhs<<<(N + 255) / 256, 256, 0, s>>>(N, T, f->node_ptr, f->node_ctr, f->node_mass, f->density, elem, xn,
-                                          vn, x1, gravity[0], gravity[1], gravity[2], h, db);
+                                          vn, x1, gravity[0], gravity[1], gravity[2], bdt, damp, f->alpha_k,
+                                          f->alpha_m, db);
   MPRGP_HIP(hipGetLastError());
   // b += A x1 (MprgpFemSolver.cpp:349), in the QP's internal node order
   MPRGP_HIP(launch_gather_nodes(s, db, qp->b, f->A->d_perm, N));
@@ -627,6 +662,15 @@ extern "C" int mprgp_fem_assemble_at(mprgp_fem *f, int material, const double *x
 extern "C" int mprgp_fem_assemble(mprgp_fem *f, int material, const double *x_n, const double *v_n,
                                   const double *gravity, double h, mprgp_qp *qp) {
   return mprgp_fem_assemble_at(f, material, x_n, v_n, nullptr, gravity, h, qp);
+}
+
+extern "C" int mprgp_fem_set_damping(mprgp_fem *f, double alpha_k, double alpha_m) {
+  if (!f) return fail(MPRGP_EBADINPUT, "NULL argument");
+  if (!(alpha_k >= 0.0) || !(alpha_m >= 0.0) || !std::isfinite(alpha_k) || !std::isfinite(alpha_m))
+    return fail(MPRGP_EBADINPUT, "damping must be finite and >= 0");  // Simulator.cpp:162-163
+  f->alpha_k = alpha_k;
+  f->alpha_m = alpha_m;
+  return MPRGP_OK;
 }
 
 // ---------------------------------------------------------------- time step

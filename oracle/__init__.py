@@ -95,10 +95,10 @@ def lib():
                                            C.POINTER(C.POINTER(C.c_int)),
                                            C.POINTER(C.POINTER(C.c_double)), P]
         L.orc_free.argtypes = [P]
-        L.orc_fem_build_system_at.argtypes = [I, I, P, P, P, P, P, P, D, P, P, P, P, D, I,
-                                              C.POINTER(C.POINTER(C.c_int)),
-                                              C.POINTER(C.POINTER(C.c_int)),
-                                              C.POINTER(C.POINTER(C.c_double)), P]
+        L.orc_fem_build_system_damped.argtypes = [I, I, P, P, P, P, P, P, D, P, P, P, P, D, I, D, D,
+                                                  C.POINTER(C.POINTER(C.c_int)),
+                                                  C.POINTER(C.POINTER(C.c_int)),
+                                                  C.POINTER(C.POINTER(C.c_double)), P]
         L.orc_implicit_euler_velocity.argtypes = [I, P, P, P, D, P]
         L.orc_plane_collide.argtypes = [I, P, I, P, P, P, P]
         L.orc_plane_collide.restype = I
@@ -294,8 +294,9 @@ def fem_element(mtype, xe, d, vol, lam, mu):
 
 
 def fem_build_system(n_nodes, tets_oriented, d, vol, mass, young, poisson, density,
-                     x_n, v_n, gravity, h, mtype, x1=None):
-    """Returns (A as scipy CSR, b) at the Newton iterate x1 (None: x_n)."""
+                     x_n, v_n, gravity, h, mtype, x1=None, alpha_k=0.0, alpha_m=0.0):
+    """Returns (A as scipy CSR, b) at the Newton iterate x1 (None: x_n), with
+    Rayleigh damping alpha_k (mass) and alpha_m (stiffness)."""
     tets = _i32(tets_oriented)
     T = tets.shape[0]
     d, vol, mass = _f64(d), _f64(vol), _f64(mass)
@@ -309,10 +310,10 @@ def fem_build_system(n_nodes, tets_oriented, d, vol, mass, young, poisson, densi
     ip = C.POINTER(C.c_int)()
     vp = C.POINTER(C.c_double)()
     L = lib()
-    L.orc_fem_build_system_at(int(n_nodes), int(T), _ptr(tets), _ptr(d), _ptr(vol), _ptr(mass),
-                              _ptr(young), _ptr(poisson), float(density), _ptr(x_n), _ptr(v_n),
-                              _ptr(x1), _ptr(gravity), float(h), int(mtype), C.byref(pp), C.byref(ip),
-                              C.byref(vp), _ptr(b))
+    L.orc_fem_build_system_damped(int(n_nodes), int(T), _ptr(tets), _ptr(d), _ptr(vol), _ptr(mass),
+                                  _ptr(young), _ptr(poisson), float(density), _ptr(x_n), _ptr(v_n),
+                                  _ptr(x1), _ptr(gravity), float(h), int(mtype), float(alpha_k),
+                                  float(alpha_m), C.byref(pp), C.byref(ip), C.byref(vp), _ptr(b))
     ptr = np.ctypeslib.as_array(pp, shape=(n + 1,)).copy()
     nnz = int(ptr[-1])
     idx = np.ctypeslib.as_array(ip, shape=(nnz,)).copy()
@@ -350,7 +351,8 @@ def implicit_euler_velocity(x_n, v_n, x, h):
 
 
 def advance(n_nodes, tets_oriented, d, vol, mass, young, poisson, density, x_n, v_n, gravity, h, mtype,
-            planes, newton_max_it=10, newton_eps=1e-4, tol=1e-6, max_it=1000, alpha_bar=0.0, ev=None):
+            planes, newton_max_it=10, newton_eps=1e-4, tol=1e-6, max_it=1000, alpha_bar=0.0, ev=None,
+            alpha_k=0.0, alpha_m=0.0):
     """One time step of MprgpFemSolver::advance (MprgpFemSolver.cpp:193-203)
     with the simulator's implicit Euler (Simulator.cpp:313): the planes'
     collision at x_n (DCDhandle, :255-271), DecoupledMprgpFemSolver::forward's
@@ -367,7 +369,7 @@ def advance(n_nodes, tets_oriented, d, vol, mass, young, poisson, density, x_n, 
     info = {"rows": m, "newton_iters": 0, "codes": [], "iterations": 0, "delta": 0.0}
     for it in range(newton_max_it):
         A, b = fem_build_system(n_nodes, tets_oriented, d, vol, mass, young, poisson, density, x_n, v_n,
-                                gravity, h, mtype, x1=x1)
+                                gravity, h, mtype, x1=x1, alpha_k=alpha_k, alpha_m=alpha_m)
         code, x, st, ev = decoupled_solve(A, b, J, c, x1, tol=tol, max_it=max_it, alpha_bar=alpha_bar, ev=ev,
                                           project_first=True)
         info["codes"].append(code)

@@ -27,6 +27,11 @@
  *                        -> for LINEAR, b does not depend on x1 (the closed form above); for
  *                           COROT, b - A x1 - M(x_n + h v_n - x1) - h^2 M g equals h^2 times
  *                           minus the finite-difference gradient of the total energy at x1
+ *   orc_fem_build_system_damped  the same with Rayleigh damping (FEMSystem.cpp:122-126, 167-192)
+ *                        -> A = M (1 + h alpha_k) + (h^2 + h alpha_m)/h^2 (A_0 - M) against the
+ *                           undamped A_0; the damping terms of b against the undamped b; a
+ *                           damped free vibration meets implicit Euler's discrete energy identity
+ *                           E_{n+1} - E_n = -|dv|_M^2/2 - |du|_K^2/2 - h v' (alpha_k M + alpha_m K) v
  *   orc_implicit_euler_velocity  MprgpFemSolver.cpp:295-313, 377-390 with FEMSystem.h:140-142
  *                        -> equals (x - x_n)/h to rounding; a rigid free fall of one step from
  *                           the rest shape gives x = X0 + h v + h^2 g and v' = v + h g
@@ -299,12 +304,13 @@ static int cmp_int(const void *a, const void *b) {
   return (x > y) - (x < y);
 }
 
-int orc_fem_build_system_at(int n_nodes, int n_tets, const int *tets,
-                            const double *d, const double *vol, const double *node_mass,
-                            const double *young, const double *poisson, double density,
-                            const double *x_n, const double *v_n, const double *x1,
-                            const double *gravity, double h, int type,
-                            int **ptr_out, int **idx_out, double **val_out, double *b) {
+int orc_fem_build_system_damped(int n_nodes, int n_tets, const int *tets,
+                                const double *d, const double *vol, const double *node_mass,
+                                const double *young, const double *poisson, double density,
+                                const double *x_n, const double *v_n, const double *x1,
+                                const double *gravity, double h, int type, double alpha_k,
+                                double alpha_m, int **ptr_out, int **idx_out, double **val_out,
+                                double *b) {
   /* node adjacency (including self), sorted */
   int *deg = (int *)xcalloc((size_t)n_nodes + 1, sizeof(int));
   for (int t = 0; t < n_tets; ++t)
@@ -338,6 +344,7 @@ int orc_fem_build_system_at(int n_nodes, int n_tets, const int *tets,
   for (int i = 0; i < n; ++i) ptr[i + 1] += ptr[i];
   int *idx = (int *)xcalloc((size_t)ptr[n], sizeof(int));
   double *val = (double *)xcalloc((size_t)ptr[n], sizeof(double));
+  double *kval = (double *)xcalloc((size_t)ptr[n], sizeof(double)); /* unscaled K */
   for (int a = 0; a < n_nodes; ++a)
     for (int r = 0; r < 3; ++r) {
       int o = ptr[3 * a + r];
@@ -346,8 +353,13 @@ int orc_fem_build_system_at(int n_nodes, int n_tets, const int *tets,
     }
   for (int i = 0; i < n; ++i) b[i] = 0.0;
   /* materialEnergyEval: per tet, in tet order (FEMEnergy.cpp:241-250, 98-106);
-     stiffness triplets are scaled by h^2 (FEMSystem.cpp:180-181) before summation */
-  double h2 = h * h;
+     stiffness triplets are scaled by KCoef + CCoef beta = h^2 + h alpha_m
+     (FEMSystem.cpp:180-181) before summation.  buildLinearSystem's
+     coefficients with beta = 1 (MprgpFemSolver.cpp:331-339): bdt = h,
+     KCoef = FCoef = bdt h, CCoef = bdt, MCoef = 1. */
+  double bdt = 1.0 * h;
+  double h2 = bdt * h;
+  double kc = h2 + bdt * alpha_m;
   for (int t = 0; t < n_tets; ++t) {
     const int *v = tets + 4 * t;
     double xe[12], fB[12], HB[144];
@@ -368,11 +380,20 @@ int orc_fem_build_system_at(int n_nodes, int n_tets, const int *tets,
         }
         int kb = lo - nptr[v[i]];
         for (int r = 0; r < 3; ++r)
-          for (int c = 0; c < 3; ++c)
-            val[ptr[3 * v[i] + r] + 3 * kb + c] += HB[(3 * i + r) * 12 + 3 * j + c] * h2;
+          for (int c = 0; c < 3; ++c) {
+            val[ptr[3 * v[i] + r] + 3 * kb + c] += HB[(3 * i + r) * 12 + 3 * j + c] * kc;
+            kval[ptr[3 * v[i] + r] + 3 * kb + c] += HB[(3 * i + r) * 12 + 3 * j + c];
+          }
       }
     }
   }
+  /* RHS += (K CRHS) beta, CRHS = (x0 - x1) bdt with x0 = x_n (FEMSystem.cpp:179) */
+  for (int i = 0; i < n; ++i) {
+    double s = 0.0;
+    for (int k = ptr[i]; k < ptr[i + 1]; ++k) s += kval[k] * ((x_n[idx[k]] - x1[idx[k]]) * bdt);
+    b[i] += s * alpha_m;
+  }
+  free(kval);
   /* gravity (FixedForceEnergy, FEMEnergy.cpp:108-115), mass term and diagonal */
   for (int a = 0; a < n_nodes; ++a) {
     double M = node_mass[a] * density;
@@ -381,7 +402,8 @@ int orc_fem_build_system_at(int n_nodes, int n_tets, const int *tets,
     }
     for (int r = 0; r < 3; ++r) {
       double psi = x_n[3 * a + r] + h * v_n[3 * a + r];
-      b[3 * a + r] += M * (psi - x1[3 * a + r]);
+      double crhs = (x_n[3 * a + r] - x1[3 * a + r]) * bdt;
+      b[3 * a + r] += M * ((psi - x1[3 * a + r]) + crhs * alpha_k);
       /* diagonal entry of row 3a+r */
       int lo = nptr[a], hi = nptr[a + 1] - 1;
       while (lo < hi) {
@@ -389,7 +411,7 @@ int orc_fem_build_system_at(int n_nodes, int n_tets, const int *tets,
         if (nadj[mid] < a) lo = mid + 1; else hi = mid;
       }
       int kb = lo - nptr[a];
-      val[ptr[3 * a + r] + 3 * kb + r] += M;
+      val[ptr[3 * a + r] + 3 * kb + r] += M * (1.0 + bdt * alpha_k);
     }
   }
   /* RHS += LHS * x1 (MprgpFemSolver.cpp:349) */
@@ -401,6 +423,17 @@ int orc_fem_build_system_at(int n_nodes, int n_tets, const int *tets,
   free(nptr); free(nadj);
   *ptr_out = ptr; *idx_out = idx; *val_out = val;
   return 0;
+}
+
+int orc_fem_build_system_at(int n_nodes, int n_tets, const int *tets,
+                            const double *d, const double *vol, const double *node_mass,
+                            const double *young, const double *poisson, double density,
+                            const double *x_n, const double *v_n, const double *x1,
+                            const double *gravity, double h, int type,
+                            int **ptr_out, int **idx_out, double **val_out, double *b) {
+  return orc_fem_build_system_damped(n_nodes, n_tets, tets, d, vol, node_mass, young, poisson, density,
+                                     x_n, v_n, x1, gravity, h, type, 0.0, 0.0, ptr_out, idx_out,
+                                     val_out, b);
 }
 
 int orc_fem_build_system(int n_nodes, int n_tets, const int *tets,

@@ -130,6 +130,17 @@ int orc_fem_build_system_at(int n_nodes, int n_tets, const int *tets_oriented,
                             const double *x_n, const double *v_n, const double *x1,
                             const double *gravity, double h, int type,
                             int **ptr_out, int **idx_out, double **val_out, double *b_out);
+/* The same with Rayleigh damping alpha_k (mass) and alpha_m (stiffness)
+ * (FEMSystem::setDamping as Simulator.cpp:159-167 sets it): A = M (1 + h
+ * alpha_k) + K (h^2 + h alpha_m), b += alpha_m K c + M c alpha_k with
+ * c = (x_n - x1) h. */
+int orc_fem_build_system_damped(int n_nodes, int n_tets, const int *tets_oriented,
+                                const double *d, const double *vol, const double *node_mass,
+                                const double *young, const double *poisson, double density,
+                                const double *x_n, const double *v_n, const double *x1,
+                                const double *gravity, double h, int type, double alpha_k,
+                                double alpha_m, int **ptr_out, int **idx_out, double **val_out,
+                                double *b_out);
 /* MprgpFemSolver::updateMesh (MprgpFemSolver.cpp:377-390) with the implicit
  * Euler parameters of FEMSolver::resetImplicitEuler (FEMSystem.h:140-142). */
 void orc_implicit_euler_velocity(int n, const double *x_n, const double *v_n, const double *x,

@@ -1,6 +1,6 @@
 """GPU parity of the time step (mprgp_fem_advance, begin_step/end_step) and
-of the assembly at a Newton iterate (mprgp_fem_assemble_at) against the
-oracle's.
+of the assembly at a Newton iterate (mprgp_fem_assemble_at), with and without
+Rayleigh damping, against the oracle's.
 
 Bars: assembly A x and b within 1e-12 (relative to the largest entry, as the
 x1 = x_n assembly tests); a step's positions within 1e-8 relative (the FP64
@@ -29,15 +29,18 @@ def deformed(sc, seed, scale=0.002):
     return ((sc.X0 - c) @ Q.T + c + scale * rng.normal(size=sc.X0.shape)).ravel()
 
 
+@pytest.mark.parametrize("damping", [(0.0, 0.0), (0.02, 0.01)])
 @pytest.mark.parametrize("mtype", [M.LINEAR, M.COROTATIONAL])
-def test_assembly_at_newton_iterate_matches_oracle(ctx, mtype):
+def test_assembly_at_newton_iterate_matches_oracle(ctx, mtype, damping):
     sc = problems.config1_flat()
     tets, d, vol, mass = oracle_body(sc)
     x_n = deformed(sc, 4)
     x1 = x_n + 0.003 * np.random.default_rng(5).normal(size=x_n.size)
+    ak, am = damping
     A, b = oracle.fem_build_system(sc.n_nodes, tets, d, vol, mass, sc.young, sc.poisson, sc.density, x_n,
-                                   sc.v_n.ravel(), sc.gravity, sc.h, mtype, x1=x1)
+                                   sc.v_n.ravel(), sc.gravity, sc.h, mtype, x1=x1, alpha_k=ak, alpha_m=am)
     fem = M.Fem(ctx, sc.X0, sc.tets, sc.young, sc.poisson, sc.density)
+    fem.set_damping(ak, am)
     q = fem.matrix.qp()
     fem.assemble_at(q, x_n, sc.v_n.ravel(), x1, sc.gravity, sc.h, mtype)
     rng = np.random.default_rng(0)
@@ -56,12 +59,15 @@ def make(ctx, sc):
     return fem, q
 
 
-@pytest.mark.parametrize("mtype", [M.LINEAR, M.COROTATIONAL])
-def test_steps_match_oracle(ctx, mtype):
+@pytest.mark.parametrize("mtype,damping", [(M.LINEAR, (0.0, 0.0)), (M.COROTATIONAL, (0.0, 0.0)),
+                                           (M.COROTATIONAL, (0.02, 0.01))])
+def test_steps_match_oracle(ctx, mtype, damping):
     sc = problems.config1()
     tets, d, vol, mass = oracle_body(sc)
     planes = problems.plane_array(sc.planes)
     fem, q = make(ctx, sc)
+    ak, am = damping
+    fem.set_damping(ak, am)
     sp = fem.step_params(h=sc.h, gravity=sc.gravity, material=mtype, planes=planes, newton_max_it=4,
                          newton_eps=1e-7)
     x_r, v_r = sc.x_n.ravel(), sc.v_n.ravel()
@@ -70,7 +76,8 @@ def test_steps_match_oracle(ctx, mtype):
         rc, st = fem.advance(q, sp, tol=1e-8, max_it=1000, alpha_bar=alpha_bar)
         x_r, v_r, info = oracle.advance(sc.n_nodes, tets, d, vol, mass, sc.young, sc.poisson, sc.density,
                                         x_r, v_r, sc.gravity, sc.h, mtype, planes, newton_max_it=4,
-                                        newton_eps=1e-7, tol=1e-8, max_it=1000, alpha_bar=alpha_bar)
+                                        newton_eps=1e-7, tol=1e-8, max_it=1000, alpha_bar=alpha_bar,
+                                        alpha_k=ak, alpha_m=am)
         x_g, v_g = fem.get_state()
         assert rc == 0 and all(c == 0 for c in info["codes"]), (k, st, info["codes"])
         assert st["rows"] == info["rows"], k
@@ -138,3 +145,7 @@ def test_step_errors(ctx):
     with pytest.raises(M.MprgpError) as e:
         fem.advance(q, bad)
     assert e.value.code == _lib.EBADINPUT
+    for ak, am in ((-0.1, 0.0), (0.0, float("nan"))):
+        with pytest.raises(M.MprgpError) as e:
+            fem.set_damping(ak, am)
+        assert e.value.code == _lib.EBADINPUT

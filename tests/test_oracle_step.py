@@ -6,7 +6,9 @@ energy); for a COROTATIONAL body b - A x1 - M(x_n + h v_n - x1) - h^2 M g is
 h^2 times minus the finite-difference gradient of the total energy at x1; the
 velocity is (x - x_n)/h to rounding; a rigid free fall of one step from the
 rest shape lands at X0 + h v + h^2 g with velocity v + h g after two Newton
-solves (the second one finds the first's solution)."""
+solves (the second one finds the first's solution).  Rayleigh damping: the
+damped A and b against the undamped ones, and implicit Euler's discrete
+energy identity for a damped free vibration."""
 import numpy as np
 import pytest
 
@@ -116,3 +118,68 @@ def test_newton_loop_on_the_ground(mtype):
     else:
         assert 2 <= info["newton_iters"] <= 6 and info["delta"] < 1e-7
     assert np.all(x.reshape(-1, 3)[:, 1] >= -1e-9)
+
+
+@pytest.mark.parametrize("mtype", [oracle.LINEAR, oracle.COROTATIONAL])
+def test_damped_system_against_the_undamped_one(mtype):
+    rng = np.random.default_rng(6)
+    b = body(4, 2, 2)
+    X = b["X"]
+    Q = problems.random_rotation(rng)
+    x_n = ((X - X.mean(0)) @ Q.T + 0.002 * rng.normal(size=X.shape)).ravel()
+    v_n = rng.normal(size=x_n.size)
+    x1 = x_n + 0.003 * rng.normal(size=x_n.size)
+    h, ak, am = 0.01, 0.02, 0.01
+    g = np.array([0.0, -9.8, 0.0])
+    Mv = np.repeat(b["rho"] * b["mass"], 3)
+    A0, b0 = system(b, x_n, v_n, x1, mtype, h=h, g=g)
+    kw = dict(x1=x1, alpha_k=ak, alpha_m=am)
+    A, rhs = oracle.fem_build_system(b["N"], b["tets"], b["d"], b["vol"], b["mass"], b["young"], b["poisson"],
+                                     b["rho"], x_n, v_n, g, h, mtype, **kw)
+    K = (A0 - oracle_sp_diags(Mv)) / (h * h)
+    A_ref = K * (h * h + h * am) + oracle_sp_diags(Mv * (1 + h * ak))
+    assert abs(A - A_ref).max() <= 1e-12 * abs(A).max()
+    # the damping terms of b: alpha_m K c + M c alpha_k, c = (x_n - x1) h; and A x1 moves with A
+    c = (x_n - x1) * h
+    b_ref = b0 + am * (K @ c) + Mv * c * ak + (A - A0) @ x1
+    np.testing.assert_allclose(rhs, b_ref, rtol=0, atol=1e-11 * np.abs(b0).max())
+    # at x1 = x_n the damping leaves b but for A x_n
+    A1, b1 = oracle.fem_build_system(b["N"], b["tets"], b["d"], b["vol"], b["mass"], b["young"], b["poisson"],
+                                     b["rho"], x_n, v_n, g, h, mtype, alpha_k=ak, alpha_m=am)
+    A00, b00 = system(b, x_n, v_n, None, mtype, h=h, g=g)
+    np.testing.assert_allclose(b1, b00 + (A1 - A00) @ x_n, rtol=0, atol=1e-11 * np.abs(b00).max())
+
+
+def oracle_sp_diags(v):
+    import scipy.sparse as sp
+    return sp.diags(v)
+
+
+def test_damped_steps_satisfy_the_discrete_energy_identity():
+    """A stretched linear bar released with no gravity and no planes.  Implicit
+    Euler with Rayleigh damping C = alpha_k M + alpha_m K satisfies, step by
+    step, E_{n+1} - E_n = -|dv|_M^2/2 - |du|_K^2/2 - h v_{n+1}' C v_{n+1}
+    with E = v'Mv/2 + u'Ku/2, u = x - X0 (exactly, for an exact linear solve)."""
+    b = body(4, 1, 1)
+    X = b["X"].ravel()
+    x = (b["X"] * np.array([1.05, 1.0, 1.0])).ravel()
+    v = np.zeros_like(x)
+    h, ak, am = 0.001, 0.5, 0.002
+    Mv = np.repeat(b["rho"] * b["mass"], 3)
+    A0, _ = system(b, X, v, None, oracle.LINEAR, h=h, g=np.zeros(3))
+    K = (A0 - oracle_sp_diags(Mv)) / (h * h)
+
+    def energy(x, v):
+        u = x - X
+        return 0.5 * v @ (Mv * v) + 0.5 * u @ (K @ u)
+
+    e0 = energy(x, v)
+    for k in range(10):
+        x1, v1, info = oracle.advance(b["N"], b["tets"], b["d"], b["vol"], b["mass"], b["young"], b["poisson"],
+                                      b["rho"], x, v, np.zeros(3), h, oracle.LINEAR, np.zeros((0, 5)),
+                                      newton_max_it=2, newton_eps=1e-12, tol=1e-12, alpha_k=ak, alpha_m=am)
+        dv, du = v1 - v, x1 - x
+        loss = 0.5 * dv @ (Mv * dv) + 0.5 * du @ (K @ du) + h * (v1 @ (ak * Mv * v1 + am * (K @ v1)))
+        assert energy(x1, v1) - energy(x, v) == pytest.approx(-loss, rel=1e-6, abs=1e-9 * e0), k
+        assert loss > 0
+        x, v = x1, v1
