@@ -1319,19 +1319,14 @@ __global__ void to_f32(const double *__restrict__ in, float *__restrict__ out, s
   for (; i < n; i += stride) out[i] = (float)in[i];
 }
 
-// Streaming copy (16 B per lane per iteration): the bandwidth probe that
-// calibrates the roofline and the profiler's HBM byte counters.
+// Streaming copy, one 16 B element per lane: the bandwidth probe that
+// calibrates the roofline and the profiler's HBM byte counters.  (One element
+// per lane over many workgroups measured faster on MI355X, 6.2 TB/s, than four
+// elements per lane, 5.6 TB/s.)
 __global__ void __launch_bounds__(256) copy_stream(const double2 *__restrict__ in, double2 *__restrict__ out,
                                                    size_t n) {
-  // four independent 16 B loads in flight per lane, each a coalesced 4 KiB block row
-  const size_t i0 = (size_t)blockIdx.x * 1024 + threadIdx.x;
-  double2 v[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k)
-    if (i0 + 256 * k < n) v[k] = in[i0 + 256 * k];
-#pragma unroll
-  for (int k = 0; k < 4; ++k)
-    if (i0 + 256 * k < n) out[i0 + 256 * k] = v[k];
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = in[i];
 }
 
 // Plain SELL SpMV y = A x (used by tests and by the assembly's A x1 term).
@@ -1409,7 +1404,7 @@ hipError_t launch_to_f32(hipStream_t s, const double *in, float *out, size_t n) 
 hipError_t launch_copy_stream(hipStream_t s, const void *in, void *out, size_t bytes) {
   const size_t n = bytes / sizeof(double2);
   if (n == 0) return hipSuccess;
-  const size_t blocks = (n + 1023) / 1024;
+  const size_t blocks = (n + 255) / 256;
   if (blocks > 0x7fffffff) return hipErrorInvalidValue;
   copy_stream<<<(unsigned)blocks, 256, 0, s>>>((const double2 *)in, (double2 *)out, n);
   return hipGetLastError();

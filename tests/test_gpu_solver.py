@@ -303,3 +303,11 @@ def test_config2_full_size_parity(ctx):
     assert_parity(A, b, node, normal, c, q.get_x(), xr, sg, sr)
     rep = kkt_report(A, b, node, normal, c, q.get_x())
     assert rep["infeas"] <= 1e-12 and rep["neg_mult"] <= 1e-6
+
+
+def test_copy_probe_rate_is_plausible(ctx):
+    """The HBM probe behind the roofline: between a fifth of the published
+    8 TB/s and the peak itself."""
+    gbps = ctx.copy_probe(1 << 28, 4)
+    print(f"copy probe {gbps:.0f} GB/s")
+    assert 1600.0 < gbps < 8000.0
