@@ -703,9 +703,11 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       // does the same: MPRGPSolver.h:255-261, 435-437); FINAL reads x only.
       // The face stays fixed inside the CG loop.
       if (!MX) break;
-      Row R = ldrow(q, v, r);
+      // the row's own x and p first, so their loads are in flight while the
+      // row flag (and, on rows, the row data behind it) arrives
       const bool f = gmem(q.face)[r];
       const V3 xr = ld3(xc, r), pr = ld3(v.p, r);
+      Row R = ldrow(q, v, r);
       const V3 y = axmy(xr, st.alpha, pr);
       st3(yc, r, y);
       // f(P(y)) = f(y) + d'(A y - b) + d'A d/2 for d = P(y) - y
