@@ -151,6 +151,26 @@ int mprgp_qp_set_rows_sparse(mprgp_qp *qp, int m, int col_major, const int *oute
  * within 1e-8 (the reference's assertion, :145) gives MPRGP_EBADINPUT; no x
  * set gives MPRGP_ESTATE.  *m_out (may be NULL) receives the row count. */
 int mprgp_qp_collide_planes(mprgp_qp *qp, int n_planes, const double *planes, int *m_out);
+/* The projector and preconditioner of the QP's rows, applied once to host
+ * vectors in node order with the step kernel's own per-node code; the face
+ * (active set) is the QP's, as the projector keeps it: set by decide_face or
+ * by the last solve, all free for a new QP.
+ *   project      DecoupledConProjector::project   MPRGPProjection.h:333-348
+ *   is_feasible  BaseGeneralConProjector::isFeasible  :298-310 (1 or 0)
+ *   decide_face  DECIDE_FACE :283-297 (*active may be NULL; COUNT_CONSTRAINTS)
+ *   phi, beta    PHI :349-367, BETA :368-383 on the QP's face
+ *   phitphi      PHITPHI :384-393 (alpha_bar > 1e-9, as it asserts)
+ *   step_limit   stepLimit :263-282 (alpha_cg = DBL_MAX means none)
+ *   precond      DiagonalDecouplePreconSolver::solve, MPRGPPrecondition.h:91-105,
+ *                with 1/diag(A) of the QP's matrix; z = phi when the QP has no rows */
+int mprgp_qp_project(mprgp_qp *qp, const double *x, double *y);
+int mprgp_qp_is_feasible(mprgp_qp *qp, const double *x, int *feasible);
+int mprgp_qp_decide_face(mprgp_qp *qp, const double *x, int *active);
+int mprgp_qp_phi(mprgp_qp *qp, const double *g, double *phi);
+int mprgp_qp_beta(mprgp_qp *qp, const double *g, const double *phi, double *beta);
+int mprgp_qp_phitphi(mprgp_qp *qp, const double *x, double alpha_bar, const double *phi, double *out);
+int mprgp_qp_step_limit(mprgp_qp *qp, const double *x, const double *d, double alpha_cg, double *alpha_f);
+int mprgp_qp_precond(mprgp_qp *qp, const double *g, const double *phi, double *z);
 /* The QP's rows in node order (ascending user node; the reference lists them
  * plane by plane): *m receives the count; node/normal/c hold cap rows (normal
  * 3 per row); with all three NULL only *m is written.  More rows than cap

@@ -307,6 +307,50 @@ class QP:
         k = m.value
         return node[:k], normal[:k], c[:k]
 
+    # DecoupledConProjector / DiagonalDecouplePreconSolver on this QP's rows
+    def project(self, x):
+        y = np.empty(self.n)
+        check(load().mprgp_qp_project(self.h, _p(_f64(x, self.n)), _p(y)))
+        return y
+
+    def is_feasible(self, x):
+        ok = C.c_int(0)
+        check(load().mprgp_qp_is_feasible(self.h, _p(_f64(x, self.n)), C.byref(ok)))
+        return bool(ok.value)
+
+    def decide_face(self, x):
+        """Sets the QP's face from x; returns the active count."""
+        k = C.c_int(0)
+        check(load().mprgp_qp_decide_face(self.h, _p(_f64(x, self.n)), C.byref(k)))
+        return k.value
+
+    def phi(self, g):
+        out = np.empty(self.n)
+        check(load().mprgp_qp_phi(self.h, _p(_f64(g, self.n)), _p(out)))
+        return out
+
+    def beta(self, g, phi):
+        out = np.empty(self.n)
+        check(load().mprgp_qp_beta(self.h, _p(_f64(g, self.n)), _p(_f64(phi, self.n)), _p(out)))
+        return out
+
+    def phitphi(self, x, alpha_bar, phi):
+        out = C.c_double(0.0)
+        check(load().mprgp_qp_phitphi(self.h, _p(_f64(x, self.n)), float(alpha_bar), _p(_f64(phi, self.n)),
+                                      C.byref(out)))
+        return out.value
+
+    def step_limit(self, x, d, alpha_cg=np.finfo(np.float64).max):
+        out = C.c_double(0.0)
+        check(load().mprgp_qp_step_limit(self.h, _p(_f64(x, self.n)), _p(_f64(d, self.n)), float(alpha_cg),
+                                         C.byref(out)))
+        return out.value
+
+    def precond(self, g, phi):
+        out = np.empty(self.n)
+        check(load().mprgp_qp_precond(self.h, _p(_f64(g, self.n)), _p(_f64(phi, self.n)), _p(out)))
+        return out
+
     def set_rhs(self, b):
         b = _f64(b, self.n)
         check(load().mprgp_qp_set_rhs(self.h, _p(b)))

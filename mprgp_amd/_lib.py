@@ -118,6 +118,14 @@ SIGNATURES = {
     "mprgp_qp_set_rows_sparse": (I, [P, I, I, P, P, P, P]),
     "mprgp_qp_collide_planes": (I, [P, I, P, P]),
     "mprgp_qp_get_rows": (I, [P, I, P, P, P, P]),
+    "mprgp_qp_project": (I, [P, P, P]),
+    "mprgp_qp_is_feasible": (I, [P, P, P]),
+    "mprgp_qp_decide_face": (I, [P, P, P]),
+    "mprgp_qp_phi": (I, [P, P, P]),
+    "mprgp_qp_beta": (I, [P, P, P, P]),
+    "mprgp_qp_phitphi": (I, [P, P, D, P, P]),
+    "mprgp_qp_step_limit": (I, [P, P, P, D, P]),
+    "mprgp_qp_precond": (I, [P, P, P, P]),
     "mprgp_qp_set_rhs": (I, [P, P]),
     "mprgp_qp_get_rhs": (I, [P, P]),
     "mprgp_qp_set_x": (I, [P, P]),

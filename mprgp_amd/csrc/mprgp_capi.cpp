@@ -731,6 +731,113 @@ extern "C" int mprgp_qp_collide_planes(mprgp_qp *q, int n_planes, const double *
   return MPRGP_OK;
 }
 
+// ------------------------------------------- projector and preconditioner
+
+// One projector_op over host vectors in node order: a (and b) in, out and/or
+// the reduction result back.  The face is the QP's own (internal order).
+static int projector_call(mprgp_qp *q, int op, const double *a, const double *b, double *out, double s0,
+                          double *result) {
+  if (!q || !a) return fail(MPRGP_EBADINPUT, "NULL argument");
+  MPRGP_NOT_BUSY(q);
+  MPRGP_HIP(hipSetDevice(q->ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
+  mprgp_ctx *ctx = q->ctx;
+  hipStream_t s = ctx->stream;
+  const int N = q->n;
+  const size_t n = 3 * (size_t)N;
+  const int blocks = (N + 255) / 256;
+  if (int rc = ctx->scratch_reserve(sizeof(double) * (5 * n + blocks + 1) + 1024)) return rc;
+  double *ua = (double *)ctx->scratch, *ub = ua + n, *ia = ub + n, *ib = ia + n, *io = ib + n;
+  double *part = io + n, *res = part + blocks;
+  MPRGP_HIP(hipMemcpyAsync(ua, a, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  MPRGP_HIP(launch_gather_nodes(s, ua, ia, q->A->d_perm, N));
+  if (b) {
+    MPRGP_HIP(hipMemcpyAsync(ub, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    MPRGP_HIP(launch_gather_nodes(s, ub, ib, q->A->d_perm, N));
+  }
+  if (op == PJ_PRECOND) MPRGP_HIP(launch_setup_rows(s, q->A->d_dinv, q->cn, q->cf, q->cs, N));
+  ProjArgs g{};
+  g.op = op;
+  g.n_nodes = N;
+  g.cf = q->cf;
+  g.cn = q->cn;
+  g.cc = q->cc;
+  g.cj = q->cj;
+  g.cs = q->cs;
+  g.dinv = q->A->d_dinv;
+  g.face = q->face;
+  g.a = ia;
+  g.b = b ? ib : ia;
+  g.out = io;
+  g.s0 = s0;
+  g.part = result ? part : nullptr;
+  MPRGP_HIP(launch_projector(s, g, res));
+  if (out) {
+    MPRGP_HIP(launch_scatter_nodes(s, io, ua, q->A->d_perm, N));
+    MPRGP_HIP(hipMemcpyAsync(out, ua, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  }
+  double r = 0.0;
+  if (result) MPRGP_HIP(hipMemcpyAsync(&r, res, sizeof(double), hipMemcpyDeviceToHost, s));
+  MPRGP_HIP(hipStreamSynchronize(s));
+  if (result) *result = N > 0 ? r : (op == PJ_STEP_LIMIT ? kDblMax : 0.0);
+  return MPRGP_OK;
+}
+
+extern "C" int mprgp_qp_project(mprgp_qp *q, const double *x, double *y) {
+  if (!y) return fail(MPRGP_EBADINPUT, "NULL argument");
+  return projector_call(q, PJ_PROJECT, x, nullptr, y, 0.0, nullptr);
+}
+
+extern "C" int mprgp_qp_is_feasible(mprgp_qp *q, const double *x, int *feasible) {
+  if (!feasible) return fail(MPRGP_EBADINPUT, "NULL argument");
+  double violated = 0.0;
+  if (int rc = projector_call(q, PJ_FEASIBLE, x, nullptr, nullptr, 0.0, &violated)) return rc;
+  *feasible = violated == 0.0;
+  return MPRGP_OK;
+}
+
+extern "C" int mprgp_qp_decide_face(mprgp_qp *q, const double *x, int *active) {
+  double count = 0.0;
+  if (int rc = projector_call(q, PJ_FACE, x, nullptr, nullptr, 0.0, &count)) return rc;
+  if (active) *active = (int)count;
+  return MPRGP_OK;
+}
+
+extern "C" int mprgp_qp_phi(mprgp_qp *q, const double *g, double *phi) {
+  if (!phi) return fail(MPRGP_EBADINPUT, "NULL argument");
+  return projector_call(q, PJ_PHI, g, nullptr, phi, 0.0, nullptr);
+}
+
+extern "C" int mprgp_qp_beta(mprgp_qp *q, const double *g, const double *phi, double *beta) {
+  if (!phi || !beta) return fail(MPRGP_EBADINPUT, "NULL argument");
+  return projector_call(q, PJ_BETA, g, phi, beta, 0.0, nullptr);
+}
+
+extern "C" int mprgp_qp_phitphi(mprgp_qp *q, const double *x, double alpha_bar, const double *phi, double *out) {
+  if (!phi || !out) return fail(MPRGP_EBADINPUT, "NULL argument");
+  if (!(alpha_bar > kEps)) return fail(MPRGP_EBADINPUT, "alpha_bar must exceed 1e-9");  // :387 assertion
+  double s = 0.0;
+  if (int rc = projector_call(q, PJ_PHITPHI, x, phi, nullptr, alpha_bar, &s)) return rc;
+  const double r = s * (1.0 / alpha_bar);
+  *out = r > 0.0 ? r : 0.0;
+  return MPRGP_OK;
+}
+
+extern "C" int mprgp_qp_step_limit(mprgp_qp *q, const double *x, const double *d, double alpha_cg,
+                                   double *alpha_f) {
+  if (!d || !alpha_f) return fail(MPRGP_EBADINPUT, "NULL argument");
+  double lim = kDblMax;
+  if (int rc = projector_call(q, PJ_STEP_LIMIT, x, d, nullptr, 0.0, &lim)) return rc;
+  double af = (alpha_cg == kDblMax) ? alpha_cg : (alpha_cg + kEps);
+  *alpha_f = lim < af ? lim : af;
+  return MPRGP_OK;
+}
+
+extern "C" int mprgp_qp_precond(mprgp_qp *q, const double *g, const double *phi, double *z) {
+  if (!phi || !z) return fail(MPRGP_EBADINPUT, "NULL argument");
+  return projector_call(q, PJ_PRECOND, g, phi, z, q && q->m > 0 ? 1.0 : 0.0, nullptr);
+}
+
 extern "C" int mprgp_qp_get_rows(mprgp_qp *q, int cap, int *m, int *node, double *normal, double *c) {
   if (!q || !m || cap < 0) return fail(MPRGP_EBADINPUT, "bad arguments");
   *m = q->m;

@@ -23,6 +23,7 @@ namespace mprgp {
 constexpr int kSlice = 64;       // rows per SELL slice = lanes per wavefront
 constexpr int kNumPart = 8;      // partial sums per tile (most any op produces: 7)
 constexpr double kEps = 1e-9;    // ScalarUtil<double>::scalar_eps of the simulator
+constexpr double kDblMax = 1.7976931348623157e308;  // ScalarUtil<double>::scalar_max
 
 enum Op : int {
   OP_DONE = 0,
@@ -152,6 +153,20 @@ struct QpDev {
   unsigned char *face;
   int *done_flag;       // receives `tag` once the QP reaches DONE
   int tag;              // nonzero; distinguishes successive occupants of a batch slot
+};
+
+// One application of the projector or preconditioner (projector_op).
+enum ProjOp { PJ_PROJECT, PJ_FEASIBLE, PJ_FACE, PJ_PHI, PJ_BETA, PJ_PHITPHI, PJ_STEP_LIMIT, PJ_PRECOND };
+
+struct ProjArgs {
+  int op, n_nodes;
+  const unsigned char *cf;
+  const double *cn, *cc, *cj, *cs, *dinv;
+  unsigned char *face;  // written by PJ_FACE, read by PJ_PHI, PJ_BETA, PJ_PRECOND
+  const double *a, *b;  // x or g; phi or d
+  double *out;          // per-node result
+  double s0;            // alpha_bar (PJ_PHITPHI); rows present (PJ_PRECOND, 0 or 1)
+  double *part;         // one partial per workgroup (ops that reduce), else null
 };
 
 // One work item of a launch: up to `slices` consecutive slices of one QP.

@@ -48,6 +48,8 @@ hipError_t launch_spmv_plain(hipStream_t s, const int *sl_off, const int *col, c
                              const double *x, double *y, int n, int accumulate);
 hipError_t launch_collide_planes(hipStream_t s, const double *x, const double *planes, int n_planes,
                                  double *cn, double *cc, double *cj, unsigned char *cf, int *count, int n);
+// part (if non-null) must hold one double per 256 nodes; result receives the reduction
+hipError_t launch_projector(hipStream_t s, const ProjArgs &a, double *result);
 hipError_t launch_to_f32(hipStream_t s, const double *in, float *out, size_t n);
 hipError_t launch_copy_stream(hipStream_t s, const void *in, void *out, size_t bytes);
 

@@ -29,7 +29,6 @@
 
 namespace mprgp {
 
-constexpr double kDblMax = 1.7976931348623157e308;
 
 struct V3 {
   double x, y, z;
@@ -1314,6 +1313,84 @@ __global__ void __launch_bounds__(256) collide_planes(const double *__restrict__
   if ((threadIdx.x % kSlice) == 0 && b != 0) atomicAdd(count, (int)__popcll(b));
 }
 
+// The projector and preconditioner of a QP's rows applied once, per node,
+// with the step kernel's own per-node functions (proj, infeasible, face_of,
+// phi_of, beta_of, precond_of and CL_AP's step-limit rule).  Vectors are in
+// internal node order.  Ops that reduce write one partial per workgroup into
+// part (a sum, or a minimum for PJ_STEP_LIMIT), which reduce_parts finishes in
+// a fixed order.
+__global__ void __launch_bounds__(256) projector_op(ProjArgs g) {
+  __shared__ double red[256];
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool is_min = g.op == PJ_STEP_LIMIT;
+  double acc = is_min ? kDblMax : 0.0;
+  if (r < g.n_nodes) {
+    Row R;
+    R.has = gmem(g.cf)[r] != 0;
+    R.n = R.has ? ld3(g.cn, r) : V3{0.0, 0.0, 0.0};
+    R.c = R.has ? gmem(g.cc)[r] : 0.0;
+    R.jj = R.has ? gmem(g.cj)[r] : 0.0;
+    R.s = R.has ? gmem(g.cs)[r] : 0.0;
+    const V3 a = ld3(g.a, r);
+    switch (g.op) {
+      case PJ_PROJECT: st3(g.out, r, proj(R, a)); break;
+      case PJ_FEASIBLE: acc = infeasible(R, a); break;
+      case PJ_FACE: {
+        const bool f = face_of(R, a);
+        gmem(g.face)[r] = f ? 1 : 0;
+        acc = f ? 1.0 : 0.0;
+        break;
+      }
+      case PJ_PHI: st3(g.out, r, phi_of(R, gmem(g.face)[r] != 0, a)); break;
+      case PJ_BETA: st3(g.out, r, beta_of(R, gmem(g.face)[r] != 0, a, ld3(g.b, r))); break;
+      case PJ_PHITPHI: {
+        const V3 ph = ld3(g.b, r);
+        const V3 px = proj(R, axmy(a, g.s0, ph));
+        acc = dot3(sub(a, px), ph);
+        break;
+      }
+      case PJ_STEP_LIMIT:
+        if (R.has) {
+          const double jx = dot3(R.n, a), jd = dot3(R.n, ld3(g.b, r));
+          if (jd > kEps && jx > R.c) acc = (jx - R.c) / jd;
+        }
+        break;
+      case PJ_PRECOND: {
+        Vecs<double> v{};
+        v.dinv = g.dinv;
+        st3(g.out, r, precond_of(v, g.s0 != 0.0, R, gmem(g.face)[r] != 0, r, a, ld3(g.b, r)));
+        break;
+      }
+      default: break;
+    }
+  }
+  if (!g.part) return;  // uniform: no op that reduces passes a null part
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      red[threadIdx.x] = is_min ? fmin(red[threadIdx.x], red[threadIdx.x + w]) : red[threadIdx.x] + red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) gmem(g.part)[blockIdx.x] = red[0];
+}
+
+// part[0..n) to out[0] in a fixed order (one workgroup).
+__global__ void __launch_bounds__(256) reduce_parts(const double *__restrict__ part, int n, int is_min,
+                                                    double *__restrict__ out) {
+  __shared__ double red[256];
+  double acc = is_min ? kDblMax : 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) acc = is_min ? fmin(acc, part[i]) : acc + part[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      red[threadIdx.x] = is_min ? fmin(red[threadIdx.x], red[threadIdx.x + w]) : red[threadIdx.x] + red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
 // FP32 copies for the mixed form.
 __global__ void to_f32(const double *__restrict__ in, float *__restrict__ out, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1395,6 +1472,13 @@ hipError_t launch_collide_planes(hipStream_t s, const double *x, const double *p
                                  double *cn, double *cc, double *cj, unsigned char *cf, int *count, int n) {
   if (n <= 0) return hipSuccess;
   collide_planes<<<(n + 255) / 256, 256, 0, s>>>(x, planes, n_planes, cn, cc, cj, cf, count, n);
+  return hipGetLastError();
+}
+hipError_t launch_projector(hipStream_t s, const ProjArgs &a, double *result) {
+  if (a.n_nodes <= 0) return hipSuccess;
+  const int blocks = (a.n_nodes + 255) / 256;
+  projector_op<<<blocks, 256, 0, s>>>(a);
+  if (a.part) reduce_parts<<<1, 256, 0, s>>>(a.part, blocks, a.op == PJ_STEP_LIMIT ? 1 : 0, result);
   return hipGetLastError();
 }
 hipError_t launch_to_f32(hipStream_t s, const double *in, float *out, size_t n) {
