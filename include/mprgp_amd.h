@@ -127,6 +127,10 @@ int mprgp_matrix_update_bsr3(mprgp_matrix *A, const double *blocks);
 int mprgp_matrix_info(const mprgp_matrix *A, int *n_nodes, long long *n_blocks, long long *bytes);
 /* y = A x on host vectors (node order), FixedSparseMatrix::multiply (:469-473) */
 int mprgp_matrix_spmv(mprgp_matrix *A, const double *x, double *y);
+/* 1/2 x'Ax - x'b, FixedSparseMatrix::funcValue (:475-482), host vectors in node order */
+int mprgp_matrix_func_value(mprgp_matrix *A, const double *x, const double *b, double *f);
+/* diag(A), 3 per node in node order: getDiagonal (MPRGPUtility.h:503-515), FixedSparseMatrix::diag */
+int mprgp_matrix_diagonal(mprgp_matrix *A, double *diag);
 void mprgp_matrix_destroy(mprgp_matrix *A);
 
 /* ---- QP: matrix + rows + right-hand side + iterate, device-resident ------- */

@@ -248,6 +248,18 @@ class Matrix:
         check(load().mprgp_matrix_spmv(self.h, _p(x), _p(y)))
         return y
 
+    def func_value(self, x, b):
+        """1/2 x'Ax - x'b (FixedSparseMatrix::funcValue)."""
+        f = C.c_double(0.0)
+        check(load().mprgp_matrix_func_value(self.h, _p(_f64(x, 3 * self.n_nodes)),
+                                             _p(_f64(b, 3 * self.n_nodes)), C.byref(f)))
+        return f.value
+
+    def diagonal(self):
+        d = np.empty(3 * self.n_nodes)
+        check(load().mprgp_matrix_diagonal(self.h, _p(d)))
+        return d
+
     def qp(self) -> "QP":
         h = C.c_void_p()
         check(load().mprgp_qp_create(self.h, C.byref(h)))

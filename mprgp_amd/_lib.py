@@ -119,6 +119,8 @@ SIGNATURES = {
     "mprgp_qp_collide_planes": (I, [P, I, P, P]),
     "mprgp_qp_get_rows": (I, [P, I, P, P, P, P]),
     "mprgp_qp_project": (I, [P, P, P]),
+    "mprgp_matrix_func_value": (I, [P, P, P, P]),
+    "mprgp_matrix_diagonal": (I, [P, P]),
     "mprgp_qp_is_feasible": (I, [P, P, P]),
     "mprgp_qp_decide_face": (I, [P, P, P]),
     "mprgp_qp_phi": (I, [P, P, P]),

@@ -382,6 +382,50 @@ extern "C" int mprgp_matrix_spmv(mprgp_matrix *A, const double *x, double *y) {
   return MPRGP_OK;
 }
 
+extern "C" int mprgp_matrix_diagonal(mprgp_matrix *A, double *diag) {
+  if (!A || !diag) return fail(MPRGP_EBADINPUT, "NULL argument");
+  mprgp_ctx *ctx = A->ctx;
+  MPRGP_HIP(hipSetDevice(ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
+  const int n = A->n_nodes;
+  const size_t vb = sizeof(double) * 3 * n;
+  if (int rc = ctx->scratch_reserve(2 * vb + 256)) return rc;
+  double *dd = (double *)ctx->scratch, *du = dd + 3 * (size_t)n;
+  hipStream_t s = ctx->stream;
+  MPRGP_HIP(launch_get_diag(s, A->d_sl_off, A->d_col, A->d_val, dd, n));
+  MPRGP_HIP(launch_scatter_nodes(s, dd, du, A->d_perm, n));
+  MPRGP_HIP(hipMemcpyAsync(diag, du, vb, hipMemcpyDeviceToHost, s));
+  MPRGP_HIP(hipStreamSynchronize(s));
+  return MPRGP_OK;
+}
+
+extern "C" int mprgp_matrix_func_value(mprgp_matrix *A, const double *x, const double *b, double *f) {
+  if (!A || !x || !b || !f) return fail(MPRGP_EBADINPUT, "NULL argument");
+  mprgp_ctx *ctx = A->ctx;
+  MPRGP_HIP(hipSetDevice(ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
+  const int n = A->n_nodes;
+  const size_t N3 = 3 * (size_t)n;
+  const size_t blocks = (N3 + 255) / 256;
+  if (int rc = ctx->scratch_reserve(sizeof(double) * (4 * N3 + blocks + 2) + 256)) return rc;
+  double *du = (double *)ctx->scratch, *dx = du + N3, *db = dx + N3, *dy = db + N3;
+  double *part = dy + N3, *res = part + blocks;
+  hipStream_t s = ctx->stream;
+  MPRGP_HIP(hipMemcpyAsync(du, x, sizeof(double) * N3, hipMemcpyHostToDevice, s));
+  MPRGP_HIP(launch_gather_nodes(s, du, dx, A->d_perm, n));
+  MPRGP_HIP(hipMemcpyAsync(du, b, sizeof(double) * N3, hipMemcpyHostToDevice, s));
+  MPRGP_HIP(launch_gather_nodes(s, du, db, A->d_perm, n));
+  MPRGP_HIP(launch_spmv_plain(s, A->d_sl_off, A->d_col, A->d_val, dx, dy, n, 0));
+  // 0.5 x.(A x) - x.b  (FixedSparseMatrix::funcValue, MPRGPSolver.h:475-482)
+  MPRGP_HIP(launch_dot(s, dx, dy, N3, part, res));
+  MPRGP_HIP(launch_dot(s, dx, db, N3, part, res + 1));
+  double r[2] = {0.0, 0.0};
+  MPRGP_HIP(hipMemcpyAsync(r, res, sizeof(r), hipMemcpyDeviceToHost, s));
+  MPRGP_HIP(hipStreamSynchronize(s));
+  *f = 0.5 * r[0] - r[1];
+  return MPRGP_OK;
+}
+
 extern "C" void mprgp_matrix_destroy(mprgp_matrix *A) {
   if (!A) return;
   mprgp_ctx *ctx = A->ctx;

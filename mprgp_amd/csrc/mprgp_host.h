@@ -48,6 +48,10 @@ hipError_t launch_spmv_plain(hipStream_t s, const int *sl_off, const int *col, c
                              const double *x, double *y, int n, int accumulate);
 hipError_t launch_collide_planes(hipStream_t s, const double *x, const double *planes, int n_planes,
                                  double *cn, double *cc, double *cj, unsigned char *cf, int *count, int n);
+hipError_t launch_get_diag(hipStream_t s, const int *sl_off, const int *col, const double *val, double *diag,
+                           int n);
+// a.b into out[0] (device); part holds one double per 256 entries
+hipError_t launch_dot(hipStream_t s, const double *a, const double *b, size_t n, double *part, double *out);
 // part (if non-null) must hold one double per 256 nodes; result receives the reduction
 hipError_t launch_projector(hipStream_t s, const ProjArgs &a, double *result);
 hipError_t launch_to_f32(hipStream_t s, const double *in, float *out, size_t n);

@@ -1253,6 +1253,43 @@ __global__ void setup_diag(const int *__restrict__ sl_off, const int *__restrict
   dinv[3 * (size_t)r + 2] = 1.0 / d2;
 }
 
+// diag(A) from the diagonal block of each row (getDiagonal, MPRGPUtility.h:503-515).
+__global__ void get_diag(const int *__restrict__ sl_off, const int *__restrict__ col,
+                         const double *__restrict__ val, double *__restrict__ diag, int n_nodes) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_nodes) return;
+  int s = r / kSlice, lane = r % kSlice;
+  int b0 = sl_off[s];
+  int W = (sl_off[s + 1] - b0) / kSlice;
+  double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+  for (int k = 0; k < W; ++k) {
+    if (col[b0 + k * kSlice + lane] == r) {
+      const double *v = val + (size_t)b0 * 9 + lane + (size_t)k * 9 * kSlice;
+      d0 = v[0 * kSlice];
+      d1 = v[4 * kSlice];
+      d2 = v[8 * kSlice];
+      break;
+    }
+  }
+  diag[3 * (size_t)r + 0] = d0;
+  diag[3 * (size_t)r + 1] = d1;
+  diag[3 * (size_t)r + 2] = d2;
+}
+
+// One partial of a.b per workgroup, for reduce_parts.
+__global__ void __launch_bounds__(256) dot_parts(const double *__restrict__ a, const double *__restrict__ b,
+                                                 size_t n, double *__restrict__ part) {
+  __shared__ double red[256];
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  red[threadIdx.x] = i < n ? a[i] * b[i] : 0.0;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
 // cs = 1/((n o dinv).n) per row (MPRGPPrecondition.h:81-88).
 __global__ void setup_rows(const double *__restrict__ dinv, const double *__restrict__ cn,
                            const unsigned char *__restrict__ cf, double *__restrict__ cs, int n_nodes) {
@@ -1472,6 +1509,20 @@ hipError_t launch_collide_planes(hipStream_t s, const double *x, const double *p
                                  double *cn, double *cc, double *cj, unsigned char *cf, int *count, int n) {
   if (n <= 0) return hipSuccess;
   collide_planes<<<(n + 255) / 256, 256, 0, s>>>(x, planes, n_planes, cn, cc, cj, cf, count, n);
+  return hipGetLastError();
+}
+hipError_t launch_get_diag(hipStream_t s, const int *sl_off, const int *col, const double *val, double *diag,
+                           int n) {
+  if (n <= 0) return hipSuccess;
+  get_diag<<<(n + 255) / 256, 256, 0, s>>>(sl_off, col, val, diag, n);
+  return hipGetLastError();
+}
+hipError_t launch_dot(hipStream_t s, const double *a, const double *b, size_t n, double *part, double *out) {
+  if (n == 0) return hipSuccess;
+  const size_t blocks = (n + 255) / 256;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  dot_parts<<<(unsigned)blocks, 256, 0, s>>>(a, b, n, part);
+  reduce_parts<<<1, 256, 0, s>>>(part, (int)blocks, 0, out);
   return hipGetLastError();
 }
 hipError_t launch_projector(hipStream_t s, const ProjArgs &a, double *result) {

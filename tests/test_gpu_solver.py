@@ -60,6 +60,18 @@ def test_spmv_matches_oracle(ctx):
     assert info["n_nodes"] == 525 and info["n_blocks"] == 525 + 2 * 2476
 
 
+def test_func_value_and_diagonal_match_oracle(ctx):
+    A, b, node, normal, c, x0 = scene_qp(problems.config1())
+    Ad = ctx.matrix_csr(A)
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=A.shape[0])
+    f = Ad.func_value(x, b)
+    fr = oracle.func_value(A, x, b)
+    scale = 0.5 * np.abs(x) @ (abs(A) @ np.abs(x)) + np.abs(x) @ np.abs(b)
+    assert abs(f - fr) <= 1e-14 * scale, (f, fr)
+    assert np.array_equal(Ad.diagonal(), A.diagonal())
+
+
 def test_bsr3_and_csr_uploads_agree(ctx):
     A, *_ = scene_qp(problems.config1())
     N = A.shape[0] // 3
