@@ -635,8 +635,8 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       const double *x = xc;
       spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
       V3 g = sub(acc[0], ld3(v.b, r));
+      const V3 xr = ld3(xc, r);  // in flight with the row flag
       Row R = ldrow(q, v, r);
-      const V3 xr = ld3(xc, r);
       bool f = face_of(R, xr);
       V3 ph = phi_of(R, f, g);
       V3 z = precond_of(v, pre, R, f, r, g, ph);
@@ -686,8 +686,9 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       break;
     }
     case OP_CG_PY: {
+      const V3 xr = ld3(xc, r), pr = ld3(v.p, r);  // in flight with the row flag
       Row R = ldrow(q, v, r);
-      V3 y = axmy(ld3(xc, r), st.alpha, ld3(v.p, r));
+      V3 y = axmy(xr, st.alpha, pr);
       st3(yc, r, y);
       st3(v.py, r, proj(R, y));
       P[0] += infeasible(R, y);
@@ -817,10 +818,10 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       const V3 pyr = ld3(v.py, r);
       P[0] += dot3(pyr, acc[0]);
       P[1] += dot3(pyr, ld3(v.b, r));
+      const bool f = gmem(q.face)[r];  // in flight with the row flag
+      const V3 ap = ld3(v.ap, r), gr = ld3(v.g, r);
       Row R = ldrow(q, v, r);
-      const bool f = gmem(q.face)[r];
-      const V3 ap = ld3(v.ap, r);
-      const V3 g = axmy(ld3(v.g, r), st.alpha, ap);
+      const V3 g = axmy(gr, st.alpha, ap);
       const V3 ph = phi_of(R, f, g);
       const V3 z = precond_of(v, pre, R, f, r, g, ph);
       st3(v.g, r, g);
@@ -841,10 +842,11 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
     case OP_CG_A: {
       // CGMonotonicStep body (MPRGPSolver.h:293-302) but p = z - beta p,
       // which needs beta first and goes with the next A p (CG_BAP)
+      bool f = gmem(q.face)[r];  // in flight with the row flag
+      V3 ap = ld3(v.ap, r);      // x = y happened in the control (buffer swap)
+      const V3 gr = ld3(v.g, r);
       Row R = ldrow(q, v, r);
-      bool f = gmem(q.face)[r];
-      V3 ap = ld3(v.ap, r);  // x = y happened in the control (buffer swap)
-      V3 g = axmy(ld3(v.g, r), st.alpha, ap);
+      V3 g = axmy(gr, st.alpha, ap);
       V3 ph = phi_of(R, f, g);
       V3 z = precond_of(v, pre, R, f, r, g, ph);
       st3(v.g, r, g);
@@ -980,8 +982,9 @@ __device__ __forceinline__ void row_work_fp64(const QpDev &q, int s, int lane, i
   const Vecs<double> &v = q.d;
   switch (op) {
     case OP_PROJX: {
+      const V3 xr = ld3(v.x, r);  // in flight with the row flag
       Row R = ldrow(q, v, r);
-      st3(v.x, r, proj(R, ld3(v.x, r)));
+      st3(v.x, r, proj(R, xr));
       break;
     }
     case OP_POW_IT: {
@@ -1031,8 +1034,8 @@ __device__ __forceinline__ void row_work_fp64(const QpDev &q, int s, int lane, i
       const double *x = v.x;
       spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
       V3 g = sub(acc[0], ld3(v.b, r));
+      V3 xr = ld3(v.x, r);  // in flight with the row flag
       Row R = ldrow(q, v, r);
-      V3 xr = ld3(v.x, r);
       bool f = face_of(R, xr);
       V3 ph = phi_of(R, f, g);
       V3 be = beta_of(R, f, g, ph);
@@ -1049,9 +1052,10 @@ __device__ __forceinline__ void row_work_fp64(const QpDev &q, int s, int lane, i
       break;
     }
     case OP_R_FINAL: {
-      Row R = ldrow(q, v, r);
-      st3(v.x, r, proj(R, ld3(v.x, r)));
+      const V3 xr = ld3(v.x, r);  // in flight with the row flag
       P[0] += gmem(q.face)[r] ? 1.0 : 0.0;
+      Row R = ldrow(q, v, r);
+      st3(v.x, r, proj(R, xr));
       break;
     }
     default:
