@@ -51,9 +51,11 @@ def apply(p: M.Params, shape):
     return p
 
 
-def tune(ctx, qps, per_qp, candidates=None, repeats=1, log=None):
+def tune(ctx, qps, per_qp, candidates=None, repeats=2, log=None):
     """Times every candidate shape on the batch (each QP restarted from its
-    warm start with reset_x) and returns (best_shape, [(shape, ms)])."""
+    warm start with reset_x), keeps each candidate's best of `repeats` timed
+    runs (single runs differ by about 1%, as much as the candidates do) and
+    returns (best_shape, [(shape, ms)])."""
     candidates = candidates or DEFAULT_CANDIDATES
     results = []
     for shape in candidates:
