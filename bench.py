@@ -142,8 +142,9 @@ def main():
         pmc_child(args)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the batched QPs shard across "
-                          "GPUs with no communication (see DESIGN.md), but only --gpus 1 is run"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU: the batched QPs are independent "
+                          "and would shard across GPUs with no communication, one process per "
+                          "GPU, but only --gpus 1 is measured"}))
         sys.exit(2)
 
     import mprgp_amd as M
