@@ -266,7 +266,7 @@ def main():
     if not args.no_cpu:
         import oracle
         order = np.argsort([q.n_nodes for q in qps])
-        done, spent, dof, rel_o, sample = 0, 0.0, 0, [], []
+        done, spent, dof, rel_o, sample, per_ms, ostats = 0, 0.0, 0, [], [], [], []
         for i in order:
             sc = scenes[i]
             tets, d, vol, mass = oracle.fem_setup(sc.n_nodes, sc.tets, sc.X0)
@@ -276,9 +276,12 @@ def main():
             node, normal, c = sc.rows()
             J = problems.rows_to_J(node, normal, sc.n_nodes)
             s0 = time.perf_counter()
-            _, xo, _, _ = oracle.decoupled_solve(A, b, J, c, sc.x_n.ravel(), tol=args.tol, max_it=4000,
-                                                 alpha_bar=qps[i].alpha_bar, project_first=True)
-            spent += time.perf_counter() - s0
+            _, xo, so, _ = oracle.decoupled_solve(A, b, J, c, sc.x_n.ravel(), tol=args.tol, max_it=4000,
+                                                  alpha_bar=qps[i].alpha_bar, project_first=True)
+            dt = time.perf_counter() - s0
+            spent += dt
+            per_ms.append(1e3 * dt)
+            ostats.append(so)
             sample.append((A, b, J, c, sc.x_n.ravel(), qps[i].alpha_bar))
             rel_o.append(np.linalg.norm(x_head[i] - xo) / np.linalg.norm(xo))
             done += 1
@@ -287,7 +290,11 @@ def main():
                 break
         cpu = {"value": round(done / spent, 3), "unit": "solves/s", "cores": 1, "kind": "port",
                "sample": f"{done} smallest of the {args.qps} QPs ({dof} DoF total), oracle FP64, "
-                         f"same alpha_bar, {spent:.1f}s; host {platform.processor() or platform.machine()}"}
+                         f"same alpha_bar, {spent:.1f}s; host {platform.processor() or platform.machine()}",
+               "per_solve_ms": {"p50": round(float(np.percentile(per_ms, 50)), 3),
+                                "p99": round(float(np.percentile(per_ms, 99)), 3)},
+               "steps": {k: int(sum(o[k] for o in ostats))
+                         for k in ("iterations", "num_cg", "num_exp", "num_prop")}}
         if err is None:
             err = {}
         err["vs_oracle_max"] = float(max(rel_o))
