@@ -77,3 +77,30 @@ def test_mixed_batch_equals_alone(ctx):
     for (q, _), (xa, sa), sb in zip(qs, alone, stats):
         np.testing.assert_array_equal(q.get_x(), xa)
         assert sb["iterations"] == sa["iterations"] and sb["refinements"] == sa["refinements"]
+
+
+def test_full_batch_of_another_seed(ctx):
+    """The benchmark's workload shape (256 beams, 5k-60k DoF each) from a
+    seed the benchmark does not use: every QP converges in both forms, the
+    FP64 KKT residual at the returned x is <= tol, and the mixed solutions
+    stay within 1e-5 of the FP64 ones (tol 1e-6 on the KKT residual bounds
+    the error by tol / lambda_min(A) ~ 1e-4 absolute on coordinates of
+    order 1)."""
+    from mprgp_amd import workloads
+    scenes = problems.batch_scenes(256, seed=12, tol=1e-6, max_it=4000)
+    qps = [workloads.DeviceQP(ctx, sc) for sc in scenes]
+    rc, st = M.solve_batch(ctx, [q.qp for q in qps], M.params(tol=1e-6, max_it=0, power_tol=1e-6))
+    ab = [s["alpha_bar"] for s in st]
+    xs = {}
+    for prec in (M.FP64, M.MIXED):
+        per = [M.params(tol=1e-6, max_it=4000, alpha_bar=a, project_first=1, precision=prec) for a in ab]
+        for q in qps:
+            q.qp.reset_x()
+        rc, st = M.solve_batch(ctx, [q.qp for q in qps], per_qp=per)
+        assert rc == 0 and all(s["code"] == 0 for s in st), prec
+        assert max(s["residual"] for s in st) <= 1e-6, prec
+        xs[prec] = [q.qp.get_x() for q in qps]
+        assert all(np.all(np.isfinite(x)) for x in xs[prec])
+    rel = [np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(xs[M.MIXED], xs[M.FP64])]
+    print(f"mixed vs fp64: max {max(rel):.2e}, median {np.median(rel):.2e}")
+    assert max(rel) <= 1e-5
