@@ -210,7 +210,7 @@ int mprgp_engine::round_once(long r, bool &idle) {
   // 2. replay
   if (n_tiles > 0) {
     hipGraphExec_t ex = nullptr;
-    if (int rc = tab.graph(s, ts, steps, grid, any_mixed, &ex)) return rc;
+    if (int rc = tab.graph(s, ts, steps, grid, any_mixed, n_tiles <= kTailTiles, &ex)) return rc;
     MPRGP_HIP(hipGraphLaunch(ex, s));
     launches += steps;
   }

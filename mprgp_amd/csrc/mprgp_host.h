@@ -36,8 +36,9 @@ int decoupled_solve(int device, int n, const int *a_outer, const int *a_inner, c
   } while (0)
 
 // kernel launch wrappers (mprgp_kernels.hip)
+// tail: the variant for launches with few tiles (longer SpMV unroll, same results)
 hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
-                       int *running, int parity, bool mixed);
+                       int *running, int parity, bool mixed, bool tail = false);
 hipError_t launch_gather_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n);
 hipError_t launch_scatter_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n);
 hipError_t launch_setup_diag(hipStream_t s, const int *sl_off, const int *col, const double *val,
@@ -73,11 +74,11 @@ struct BatchTables {
   int *d_ctrl = nullptr;   // [1] = running QPs
   int *d_flags = nullptr;  // per batch slot: tag of the QP that reached DONE
   int flags_cap = 0;
-  std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
-  std::map<std::tuple<int, int, int, int>, hipGraph_t> graph_defs;
+  std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> graphs;
+  std::map<std::tuple<int, int, int, int, int>, hipGraph_t> graph_defs;
   int init(mprgp_ctx *c);
   int ensure(int n_qps, int n_slots);
-  int graph(hipStream_t s, int ts, int steps, int grid, bool mixed, hipGraphExec_t *out);
+  int graph(hipStream_t s, int ts, int steps, int grid, bool mixed, bool tail, hipGraphExec_t *out);
   void drop_graphs();
   void destroy();
 };
@@ -88,6 +89,9 @@ int pick_ts(const mprgp_params &p, long total_slices);
 void append_tiles(int slot, const mprgp_qp *q, int span, std::vector<Tile> &tiles);
 int tiles_for(int n_slices, int span);
 int round_grid(int n_tiles);
+// Launches with at most this many tiles (one workgroup per CU of an MI355X)
+// use the step kernel's tail variant.
+constexpr int kTailTiles = 256;
 void layout_tiles(const std::vector<Tile> &tiles, int grid, Tile *table);
 long solve_step_bound(const mprgp_params &p);
 size_t solve_bytes(const mprgp_qp *q, const mprgp_params &p);

@@ -607,7 +607,7 @@ __device__ __forceinline__ void loop_head_row(const Vecs<T> &v, const Row &R, bo
   P[2] += dot3(sub(x, px), ph);
 }
 
-template <class T, bool MX>
+template <class T, bool MX, int UB>
 __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int s, int lane, int r, int op,
                                          const QpState &st, double (&P)[kNumPart]) {
   const bool mixed = MX && st.mixed;
@@ -633,7 +633,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       // g = A x - b ; DECIDE_FACE(x) ; phi ; z ; p = z   (MPRGPSolver.h:148-156, 263-268, 210-215)
       V3 acc[1];
       const double *x = xc;
-      spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
+      spmv_row<1, UB>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
       V3 g = sub(acc[0], ld3(v.b, r));
       const V3 xr = ld3(xc, r);  // in flight with the row flag
       Row R = ldrow(q, v, r);
@@ -665,7 +665,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
     case OP_CL_AP: {
       V3 acc[1];
       const T *p = v.p;
-      spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(p, j); }, acc);
+      spmv_row<1, UB>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(p, j); }, acc);
       st3(v.ap, r, acc[0]);
       V3 pr = ld3(v.p, r);
       P[0] += dot3(pr, acc[0]);
@@ -772,7 +772,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       const T *zz = v.z, *pp = v.p;
       const double beta = st.beta_cg;
       V3 acc[1];
-      spmv_row<1, 3>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) {
+      spmv_row<1, UB>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) {
         xs[0] = round_to<T>(axmy(ld3(zz, j), beta, ld3(pp, j)));
       }, acc);
       const V3 pn = round_to<T>(axmy(ld3(v.z, r), beta, ld3(v.p, r)));
@@ -799,7 +799,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
         P[3] += dot3(xr, br);
       } else {
         V3 acc[1];
-        spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(py, j); }, acc);
+        spmv_row<1, UB>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(py, j); }, acc);
         P[0] += dot3(pyr, acc[0]);
         P[1] += dot3(pyr, br);
       }
@@ -814,7 +814,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       // the other buffer, current only once the step is accepted.
       const double *py = v.py;
       V3 acc[1];
-      spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(py, j); }, acc);
+      spmv_row<1, UB>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(py, j); }, acc);
       const V3 pyr = ld3(v.py, r);
       P[0] += dot3(pyr, acc[0]);
       P[1] += dot3(pyr, ld3(v.b, r));
@@ -875,7 +875,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       // to py because neighbours still gather y in this launch.
       V3 acc[1];
       const double *y = yc;
-      spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(y, j); }, acc);
+      spmv_row<1, UB>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(y, j); }, acc);
       V3 g = sub(acc[0], ld3(v.b, r));
       Row R = ldrow(q, v, r);
       V3 yr = ld3(yc, r);
@@ -909,7 +909,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
     case OP_CL_PROP1: {
       V3 acc[1];
       const T *be = v.beta;
-      spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(be, j); }, acc);
+      spmv_row<1, UB>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(be, j); }, acc);
       st3(v.ap, r, acc[0]);
       V3 d = ld3(v.beta, r);
       P[0] += dot3(d, acc[0]);
@@ -976,7 +976,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
 // Ops on FP64 vectors in both forms: projection of the warm start, the power
 // method (its SpMV reads the matrix of the QP's form), and the outer
 // refinement of the mixed form.
-template <bool MX>
+template <bool MX, int UB>
 __device__ __forceinline__ void row_work_fp64(const QpDev &q, int s, int lane, int r, int op,
                                               const QpState &st, double (&P)[kNumPart]) {
   const Vecs<double> &v = q.d;
@@ -1013,9 +1013,9 @@ __device__ __forceinline__ void row_work_fp64(const QpDev &q, int s, int lane, i
       V3 acc[1];
       auto gather = [&](int j, V3 (&xs)[1]) { xs[0] = ld3(in, j); };
       if (MX && st.mixed)
-        spmv_row<1>(q.sl_off, q.col, q.f.val, s, lane, gather, acc);
+        spmv_row<1, UB>(q.sl_off, q.col, q.f.val, s, lane, gather, acc);
       else
-        spmv_row<1>(q.sl_off, q.col, v.val, s, lane, gather, acc);
+        spmv_row<1, UB>(q.sl_off, q.col, v.val, s, lane, gather, acc);
       const V3 w = k == 0 ? acc[0] : V3{acc[0].x / nu, acc[0].y / nu, acc[0].z / nu};
       st3(wk, r, w);
       P[0] += dot3(w, w);
@@ -1032,7 +1032,7 @@ __device__ __forceinline__ void row_work_fp64(const QpDev &q, int s, int lane, i
       // is set up in the FP32 storage with d = 0.
       V3 acc[1];
       const double *x = v.x;
-      spmv_row<1>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
+      spmv_row<1, UB>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
       V3 g = sub(acc[0], ld3(v.b, r));
       V3 xr = ld3(v.x, r);  // in flight with the row flag
       Row R = ldrow(q, v, r);
@@ -1070,7 +1070,7 @@ __device__ __forceinline__ bool fp64_op(int op) {
 
 // One slice per wavefront: a tile is up to TS consecutive slices, streamed
 // by the workgroup's waves at the same time.
-template <int TS, bool MX>
+template <int TS, bool MX, int UB>
 __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &st, double (&P)[kNumPart]) {
   const int wave = threadIdx.x / kSlice;
   const int lane = threadIdx.x % kSlice;
@@ -1078,11 +1078,11 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
   const int r = s * kSlice + lane;
   if (wave >= t.slices || r >= q.n_nodes) return;
   if (fp64_op(op)) {
-    row_work_fp64<MX>(q, s, lane, r, op, st, P);
+    row_work_fp64<MX, UB>(q, s, lane, r, op, st, P);
   } else if (MX && st.mixed) {
-    row_work<float, MX>(q, q.f, s, lane, r, op, st, P);
+    row_work<float, MX, UB>(q, q.f, s, lane, r, op, st, P);
   } else {
-    row_work<double, MX>(q, q.d, s, lane, r, op, st, P);
+    row_work<double, MX, UB>(q, q.d, s, lane, r, op, st, P);
   }
 }
 
@@ -1124,10 +1124,19 @@ typedef uint64_t __attribute__((__may_alias__)) StateWord;
 #define MPRGP_STEP_MIN_WAVES 4
 #endif
 
+// The one-vector SpMVs' unroll: the blocks of UB rows' slots are loaded per
+// round.  The sums still run over the blocks in order, so UB changes no bit
+// of a result.  kUnrollTail, with more registers and fewer waves, is for
+// launches with few tiles, where a lone row's chain of load rounds sets the
+// launch time.
+constexpr int kUnroll = 3;
+constexpr int kUnrollTail = 6;
+
 // MX: the batch holds QPs of the mixed form (their inner ops run on the FP32
 // copies); FP64 QPs of the same batch take the FP64 path.
-template <int TS, bool MX>
-__global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(const QpDev *__restrict__ qps,
+template <int TS, bool MX, int UB>
+__global__ void __launch_bounds__(TS * kSlice, UB == kUnroll ? MPRGP_STEP_MIN_WAVES : 2)
+    mprgp_step(const QpDev *__restrict__ qps,
                                                            const Tile *__restrict__ tiles,
                                                            int *running, int parity) {
   __shared__ double red[TS][kNumPart];
@@ -1184,7 +1193,7 @@ __global__ void __launch_bounds__(TS * kSlice, MPRGP_STEP_MIN_WAVES) mprgp_step(
 #pragma unroll
     for (int j = 0; j < kNumPart; ++j) P[j] = 0.0;
     if (has_min) P[kMinSlot] = kDblMax;
-    tile_work<TS, MX>(q, t, op, st, P);
+    tile_work<TS, MX, UB>(q, t, op, st, P);
     // wave then block reduction in a fixed order, over the op's slots only
     const int ns = has_min ? kMinSlot + 1 : sum_slots(op);
     if (ns > 0) {
@@ -1468,23 +1477,26 @@ __global__ void __launch_bounds__(256) spmv_plain(const int *__restrict__ sl_off
 
 // Host-callable launch wrappers ------------------------------------------
 
-template <bool MX>
+template <bool MX, int UB>
 static hipError_t launch_step_t(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
                                 int *running, int parity) {
   switch (ts) {
-    case 1: mprgp_step<1, MX><<<grid, 1 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
-    case 2: mprgp_step<2, MX><<<grid, 2 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
-    case 4: mprgp_step<4, MX><<<grid, 4 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
-    case 8: mprgp_step<8, MX><<<grid, 8 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
+    case 1: mprgp_step<1, MX, UB><<<grid, 1 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
+    case 2: mprgp_step<2, MX, UB><<<grid, 2 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
+    case 4: mprgp_step<4, MX, UB><<<grid, 4 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
+    case 8: mprgp_step<8, MX, UB><<<grid, 8 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
 hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
-                       int *running, int parity, bool mixed) {
-  return mixed ? launch_step_t<true>(ts, grid, s, qps, tiles, running, parity)
-               : launch_step_t<false>(ts, grid, s, qps, tiles, running, parity);
+                       int *running, int parity, bool mixed, bool tail) {
+  if (tail)
+    return mixed ? launch_step_t<true, kUnrollTail>(ts, grid, s, qps, tiles, running, parity)
+                 : launch_step_t<false, kUnrollTail>(ts, grid, s, qps, tiles, running, parity);
+  return mixed ? launch_step_t<true, kUnroll>(ts, grid, s, qps, tiles, running, parity)
+               : launch_step_t<false, kUnroll>(ts, grid, s, qps, tiles, running, parity);
 }
 
 hipError_t launch_gather_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n) {
