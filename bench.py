@@ -75,7 +75,8 @@ def measure_traffic(args, tiles):
     if not os.path.exists(rocprof):
         return None
     tmp = tempfile.mkdtemp(prefix="mprgp_pmc_")
-    kernel = f"mprgp_step<{tiles}, {'true' if args.precision == 'mixed' else 'false'}>"
+    # both unroll variants of the solve's kernel (mprgp_step<TS, MX, 3 or 6>)
+    kernel = f"mprgp_step<{tiles}, {'true' if args.precision == 'mixed' else 'false'},"
 
     def run(counter, child, tag):
         d = os.path.join(tmp, tag)
