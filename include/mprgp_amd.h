@@ -138,7 +138,8 @@ int mprgp_qp_create(mprgp_matrix *A, mprgp_qp **out);
 void mprgp_qp_destroy(mprgp_qp *qp);
 /* DecoupledConProjector(J, JJt, c) (MPRGPProjection.h:330-331) in per-node
  * form: row i constrains node[i] with normal[3i..3i+2] and offset c[i].
- * Duplicate nodes, zero normals or bad indices give MPRGP_EBADINPUT. */
+ * Duplicate nodes (a second row on a node, even with disjoint DoFs as in
+ * J = I), zero normals or bad indices give MPRGP_EBADINPUT. */
 int mprgp_qp_set_rows(mprgp_qp *qp, int m, const int *node, const double *normal, const double *c);
 /* Same from the sparse J the reference builds (LinearConCollider.cpp:206-230):
  * J is m x 3*nodes in CSR (col_major = 0) or CSC (col_major = 1, Eigen's

@@ -671,7 +671,9 @@ static int upload_rows(mprgp_qp *q, int m, const std::vector<int> &node, const s
     int k = node[i];
     if (k < 0 || k >= N) return fail(MPRGP_EBADINPUT, "row %d: node %d out of range", i, k);
     int r = q->A->iperm[k];
-    if (cf[r]) return fail(MPRGP_EBADINPUT, "row %d: node %d already has a row (J J^T not diagonal)", i, k);
+    if (cf[r])
+      return fail(MPRGP_EBADINPUT, "row %d: node %d already has a row (one half-space row per node is supported)",
+                  i, k);
     double n0 = nrm[3 * i], n1 = nrm[3 * i + 1], n2 = nrm[3 * i + 2];
     if (!std::isfinite(n0) || !std::isfinite(n1) || !std::isfinite(n2) || !std::isfinite(c[i]))
       return fail(MPRGP_EBADINPUT, "row %d: non-finite normal or offset", i);

@@ -281,6 +281,11 @@ def test_edge_cases(ctx):
     Jbad = sp.csr_matrix(([1.0, 1.0], ([0, 0], [0, 3])), shape=(1, A.shape[0]))  # spans 2 nodes
     with pytest.raises(M.MprgpError):
         q.set_rows_sparse(Jbad, np.zeros(1))
+    # two rows with disjoint DoFs on one node (J J' diagonal, as in J = I): one row per node only
+    Jbox = sp.csr_matrix(([1.0, 1.0], ([0, 1], [0, 1])), shape=(2, A.shape[0]))
+    with pytest.raises(M.MprgpError) as e:
+        q.set_rows_sparse(Jbox, np.zeros(2))
+    assert e.value.code == M._lib.EBADINPUT and "one half-space row per node" in str(e.value)
 
 
 def test_batch_equals_alone_bitwise(ctx):
