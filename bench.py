@@ -119,6 +119,38 @@ def measure_traffic(args, tiles):
             "write_scale": round(cal_w, 4)}
 
 
+def spmv_leg(M, workloads, ctx, qps, tiles, graph_steps):
+    """Bandwidth of the batched SpMV (FixedSparseMatrix::multiply,
+    MPRGPSolver.h:469-473) on the whole batch: every QP runs the power method
+    (specRad, MPRGPSolver.h:42-79) to its 1001-iteration cap (a power_tol of
+    1e-300 is never reached), so each of the ~1000 launches multiplies every
+    QP's matrix once and normalises (op POW_IT: the matrix, w read, v read and
+    written, w written).  Achieved = algorithmic bytes of the POW_IT launches
+    (from the op counts) / device time of the whole launch sequence (HIP events
+    on the solve stream), so the few set-up launches count against it.  The
+    batch's FP64 matrices total about 1.8 GB, far above the 256 MB last-level
+    cache; the FP32 copies (mixed form) about 1 GB."""
+    pow_it = workloads.OPS.index("POW_IT")
+    out = {}
+    for name, prec in (("fp64_matrix", M.FP64), ("fp32_matrix", M.MIXED)):
+        mixed = prec == M.MIXED
+        per = [M.params(tol=1e-6, max_it=0, power_tol=1e-300, precision=prec, tiles=tiles,
+                        graph_steps=graph_steps) for _ in qps]
+        _, st = M.solve_batch(ctx, [q.qp for q in qps], per_qp=per)
+        alg, its = 0, 0
+        for q in qps:
+            k = int(q.qp.op_counts()[pow_it])
+            its = max(its, k)
+            alg += k * workloads.op_bytes(pow_it, q.n_nodes, q.n_blocks, q.m, mixed=mixed)
+        dev_s = st[0]["elapsed_ms"] / 1e3
+        launches = st[0]["launches"]
+        out[name] = {"achieved": round(alg / dev_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(alg / dev_s / 1e9 / HBM_PEAK_GBS, 4), "launches": launches,
+                     "power_iterations": its, "avg_launch_us": round(1e6 * dev_s / max(launches, 1), 2),
+                     "bytes_per_launch": int(alg / max(its, 1))}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +169,7 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true", help="library default shape instead of the autotuner's")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM counter passes")
+    ap.add_argument("--no-spmv", action="store_true", help="skip the batched SpMV bandwidth leg")
     ap.add_argument("--pmc-child", choices=["copy", "solve"], help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -224,6 +257,11 @@ def main():
             "copy_gbps": round(copy_gbps, 1),
             "iterations_total": int(sum(s["iterations"] for s in stats))}
     log(f"roofline: {roof}")
+
+    spmv = None
+    if not args.no_spmv:
+        spmv = spmv_leg(M, workloads, ctx, qps, tiles, shape["graph_steps"] if shape else args.graph_steps)
+        log(f"batched spmv (power method to its cap): {spmv}")
 
     other = None
     if mixed and not args.no_fp64_batch:
@@ -361,6 +399,7 @@ def main():
         "solution_rel_err": err,
         "serving": serve,
         "roofline": roof,
+        "spmv_roofline": spmv,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
     }

