@@ -980,8 +980,8 @@ int BatchTables::ensure(int n_qps, int n_slots) {
   return MPRGP_OK;
 }
 
-int BatchTables::graph(hipStream_t s, int ts, int steps, int grid, bool mixed, bool tail, hipGraphExec_t *out) {
-  auto key = std::make_tuple(ts, steps, grid, (int)mixed, (int)tail);
+int BatchTables::graph(hipStream_t s, int ts, int steps, int grid, bool mixed, int variant, hipGraphExec_t *out) {
+  auto key = std::make_tuple(ts, steps, grid, (int)mixed, variant);
   auto it = graphs.find(key);
   if (it != graphs.end()) {
     *out = it->second;
@@ -989,7 +989,7 @@ int BatchTables::graph(hipStream_t s, int ts, int steps, int grid, bool mixed, b
   }
   MPRGP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   for (int i = 0; i < steps; ++i) {
-    hipError_t e = launch_step(ts, grid, s, d_qps, d_tiles, d_ctrl + 1, i & 1, mixed, tail);
+    hipError_t e = launch_step(ts, grid, s, d_qps, d_tiles, d_ctrl + 1, i & 1, mixed, variant);
     if (e != hipSuccess) {
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(s, &g);
@@ -1267,11 +1267,18 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
   const char *ng = getenv("MPRGP_NO_GRAPH");
   const bool use_graph = params[0].graph_steps >= 0 && !(ng && ng[0] && ng[0] != '0');
   if (params[0].graph_steps < 0) steps = -params[0].graph_steps + (params[0].graph_steps & 1);
-  // few tiles: the tail variant (same results, shorter chain of load rounds per row)
-  bool tail = n_tiles <= kTailTiles;
+  // the kernel variant for the table's tiles and the bytes of their matrices
+  // (same results; few tiles: the tail variant, large matrices: streamed)
+  auto live_bytes = [&](const std::vector<char> *keep) {
+    double b = 0.0;
+    for (int i = 0; i < count; ++i)
+      if (!keep || (*keep)[i]) b += matrix_stream_bytes(qps[i]->A, params[stride ? i : 0].precision == MPRGP_MIXED);
+    return b;
+  };
+  int variant = step_variant(n_tiles, live_bytes(nullptr));
   hipGraphExec_t ex = nullptr;
   if (use_graph)
-    if (int rc = tab.graph(s, ts, steps, grid, any_mixed, tail, &ex)) return rc;
+    if (int rc = tab.graph(s, ts, steps, grid, any_mixed, variant, &ex)) return rc;
   int in_table = count;  // QPs whose tiles are in the table
   std::vector<int> flags(count);
 
@@ -1284,7 +1291,7 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
       MPRGP_HIP(hipGraphLaunch(ex, s));
     } else {
       for (int i = 0; i < steps; ++i)
-        MPRGP_HIP(launch_step(ts, grid, s, tab.d_qps, tab.d_tiles, tab.d_ctrl + 1, i & 1, any_mixed, tail));
+        MPRGP_HIP(launch_step(ts, grid, s, tab.d_qps, tab.d_tiles, tab.d_ctrl + 1, i & 1, any_mixed, variant));
     }
     launched += steps;
     MPRGP_HIP(hipMemcpyAsync(ctx->h_running + slot, tab.d_ctrl + 1, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1318,9 +1325,9 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
         MPRGP_HIP(hipMemcpyAsync(tab.d_tiles, uploads.back().data(), sizeof(Tile) * grid, hipMemcpyHostToDevice, s));
         MPRGP_HIP(hipStreamSynchronize(s));
         in_table = live;
-        tail = (int)kept.size() <= kTailTiles;
+        variant = step_variant((int)kept.size(), live_bytes(&keep));
         if (use_graph)
-          if (int rc = tab.graph(s, ts, steps, grid, any_mixed, tail, &ex)) return rc;
+          if (int rc = tab.graph(s, ts, steps, grid, any_mixed, variant, &ex)) return rc;
       }
       // restart the two-deep polling pipeline
       if (int rc = enqueue(k & 1)) return rc;

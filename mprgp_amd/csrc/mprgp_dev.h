@@ -156,6 +156,13 @@ struct QpDev {
 };
 
 // One application of the projector or preconditioner (projector_op).
+// Variants of the step kernel; all give the same results, bit for bit.
+//   kStepCached: SpMV unroll 3, matrices read through the caches;
+//   kStepStream: the same with matrix values and column indices loaded
+//                nontemporal (for launches whose matrices exceed the cache);
+//   kStepTail:   SpMV unroll 6 and fewer waves (for launches with few tiles).
+enum StepVariant { kStepCached = 0, kStepStream = 1, kStepTail = 2 };
+
 enum ProjOp { PJ_PROJECT, PJ_FEASIBLE, PJ_FACE, PJ_PHI, PJ_BETA, PJ_PHITPHI, PJ_STEP_LIMIT, PJ_PRECOND };
 
 struct ProjArgs {

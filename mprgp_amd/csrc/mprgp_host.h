@@ -36,9 +36,9 @@ int decoupled_solve(int device, int n, const int *a_outer, const int *a_inner, c
   } while (0)
 
 // kernel launch wrappers (mprgp_kernels.hip)
-// tail: the variant for launches with few tiles (longer SpMV unroll, same results)
+// variant: a StepVariant (mprgp_dev.h), from step_variant() below
 hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
-                       int *running, int parity, bool mixed, bool tail = false);
+                       int *running, int parity, bool mixed, int variant = kStepCached);
 hipError_t launch_gather_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n);
 hipError_t launch_scatter_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n);
 hipError_t launch_setup_diag(hipStream_t s, const int *sl_off, const int *col, const double *val,
@@ -78,7 +78,7 @@ struct BatchTables {
   std::map<std::tuple<int, int, int, int, int>, hipGraph_t> graph_defs;
   int init(mprgp_ctx *c);
   int ensure(int n_qps, int n_slots);
-  int graph(hipStream_t s, int ts, int steps, int grid, bool mixed, bool tail, hipGraphExec_t *out);
+  int graph(hipStream_t s, int ts, int steps, int grid, bool mixed, int variant, hipGraphExec_t *out);
   void drop_graphs();
   void destroy();
 };
@@ -92,6 +92,14 @@ int round_grid(int n_tiles);
 // Launches with at most this many tiles (one workgroup per CU of an MI355X)
 // use the step kernel's tail variant.
 constexpr int kTailTiles = 256;
+// Launches whose matrices (SELL slots of the live QPs, at each QP's storage
+// precision) exceed the MI355X's 256 MB last-level cache stream them
+// nontemporal.
+constexpr double kStreamBytes = 256.0 * 1024 * 1024;
+inline int step_variant(int n_tiles, double matrix_bytes) {
+  if (n_tiles <= kTailTiles) return kStepTail;
+  return matrix_bytes > kStreamBytes ? kStepStream : kStepCached;
+}
 void layout_tiles(const std::vector<Tile> &tiles, int grid, Tile *table);
 long solve_step_bound(const mprgp_params &p);
 size_t solve_bytes(const mprgp_qp *q, const mprgp_params &p);
@@ -141,6 +149,12 @@ struct mprgp_matrix {
   float *d_val32 = nullptr;
   int version32 = -1;
 };
+
+// Bytes of A one SpMV of the inner solve streams (values and column indices
+// of every SELL slot).
+inline double matrix_stream_bytes(const mprgp_matrix *A, bool mixed) {
+  return (double)A->slots * (9.0 * (mixed ? sizeof(float) : sizeof(double)) + sizeof(int));
+}
 
 struct mprgp_qp {
   mprgp_ctx *ctx = nullptr;

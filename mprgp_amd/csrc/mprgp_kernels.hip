@@ -52,6 +52,14 @@ __device__ __forceinline__ V3 ld3(const T *v, int r) {
   const auto *p = gmem(v) + 3 * (size_t)r;
   return V3{(double)p[0], (double)p[1], (double)p[2]};
 }
+// One element; NT: nontemporal (streamed past the caches).
+template <bool NT, class P>
+__device__ __forceinline__ auto ldm(const P *p) {
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
 template <class T>
 __device__ __forceinline__ void st3(T *v, int r, V3 a) {
   auto *p = gmem(v) + 3 * (size_t)r;
@@ -153,11 +161,12 @@ __device__ __forceinline__ V3 precond_of(const Vecs<T> &v, bool on, const Row &R
 // matrix values and gathers of U blocks are all in flight together.  U is
 // kept small enough (3 blocks for one vector, 1 for two) that the step kernel
 // needs no scratch: spilled registers would go to HBM on every launch.
-// Matrix values and column indices are read once per launch and never by
-// another lane, so they are loaded nontemporal: the stream of a whole batch
-// (about 1 GB) then leaves the gathered vectors in L2 and the last-level
-// cache (measured: batched SpMV 4.70 -> 4.97 TB/s, FP64 matrices).
-template <int NV, int U = (NV == 1 ? 3 : 1), class T, class G>
+// NT: matrix values and column indices are loaded nontemporal.  Each is read
+// once per launch and by one lane only; when the launch's matrices are larger
+// than the last-level cache, streaming them past it keeps the gathered
+// vectors cached (measured on 1.8 GB of FP64 matrices: batched SpMV 4.70 ->
+// 4.97 TB/s).  Matrices that fit stay cached from launch to launch instead.
+template <int NV, int U = (NV == 1 ? 3 : 1), bool NT = false, class T, class G>
 __device__ __forceinline__ void spmv_row(const int *sl_off, const int *col, const T *val, int s, int lane,
                                          G gather, V3 (&acc)[NV]) {
 #pragma unroll
@@ -172,11 +181,11 @@ __device__ __forceinline__ void spmv_row(const int *sl_off, const int *col, cons
     T a[U][9];  // kept in the storage precision until its multiply
     V3 xs[U][NV];
 #pragma unroll
-    for (int u = 0; u < U; ++u) j[u] = __builtin_nontemporal_load(cp + (k + u) * kSlice);
+    for (int u = 0; u < U; ++u) j[u] = ldm<NT>(cp + (k + u) * kSlice);
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
-      for (int e = 0; e < 9; ++e) a[u][e] = __builtin_nontemporal_load(vp + (size_t)(k + u) * (9 * kSlice) + e * kSlice);
+      for (int e = 0; e < 9; ++e) a[u][e] = ldm<NT>(vp + (size_t)(k + u) * (9 * kSlice) + e * kSlice);
 #pragma unroll
     for (int u = 0; u < U; ++u) gather(j[u], xs[u]);
 #pragma unroll
@@ -195,10 +204,10 @@ __device__ __forceinline__ void spmv_row(const int *sl_off, const int *col, cons
       }
   }
   for (; k < W; ++k) {
-    const int j = __builtin_nontemporal_load(cp + k * kSlice);
+    const int j = ldm<NT>(cp + k * kSlice);
     T a[9];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) a[e] = __builtin_nontemporal_load(vp + (size_t)k * (9 * kSlice) + e * kSlice);
+    for (int e = 0; e < 9; ++e) a[e] = ldm<NT>(vp + (size_t)k * (9 * kSlice) + e * kSlice);
     V3 xs[NV];
     gather(j, xs);
 #pragma unroll
@@ -611,7 +620,7 @@ __device__ __forceinline__ void loop_head_row(const Vecs<T> &v, const Row &R, bo
   P[2] += dot3(sub(x, px), ph);
 }
 
-template <class T, bool MX, int UB>
+template <class T, bool MX, int UB, bool NT>
 __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int s, int lane, int r, int op,
                                          const QpState &st, double (&P)[kNumPart]) {
   const bool mixed = MX && st.mixed;
@@ -637,7 +646,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       // g = A x - b ; DECIDE_FACE(x) ; phi ; z ; p = z   (MPRGPSolver.h:148-156, 263-268, 210-215)
       V3 acc[1];
       const double *x = xc;
-      spmv_row<1, UB>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
+      spmv_row<1, UB, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
       V3 g = sub(acc[0], ld3(v.b, r));
       const V3 xr = ld3(xc, r);  // in flight with the row flag
       Row R = ldrow(q, v, r);
@@ -669,7 +678,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
     case OP_CL_AP: {
       V3 acc[1];
       const T *p = v.p;
-      spmv_row<1, UB>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(p, j); }, acc);
+      spmv_row<1, UB, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(p, j); }, acc);
       st3(v.ap, r, acc[0]);
       V3 pr = ld3(v.p, r);
       P[0] += dot3(pr, acc[0]);
@@ -733,7 +742,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
         const T *pp = v.p;
         const double alpha = st.alpha;
         V3 acc[1];
-        spmv_row<1, 2>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) {
+        spmv_row<1, 2, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) {
           const V3 yj = axmy(ld3(xx, j), alpha, ld3(pp, j));
           const V3 nj = ld3(v.cn, j);
           const double vj = gmem(v.cc)[j] - dot3(nj, yj);
@@ -776,7 +785,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       const T *zz = v.z, *pp = v.p;
       const double beta = st.beta_cg;
       V3 acc[1];
-      spmv_row<1, UB>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) {
+      spmv_row<1, UB, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) {
         xs[0] = round_to<T>(axmy(ld3(zz, j), beta, ld3(pp, j)));
       }, acc);
       const V3 pn = round_to<T>(axmy(ld3(v.z, r), beta, ld3(v.p, r)));
@@ -792,7 +801,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       if (st.cg_entry) {
         const double *x = xc;
         V3 acc[2];
-        spmv_row<2>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[2]) {
+        spmv_row<2, 1, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[2]) {
           xs[0] = ld3(py, j);
           xs[1] = ld3(x, j);
         }, acc);
@@ -803,7 +812,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
         P[3] += dot3(xr, br);
       } else {
         V3 acc[1];
-        spmv_row<1, UB>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(py, j); }, acc);
+        spmv_row<1, UB, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(py, j); }, acc);
         P[0] += dot3(pyr, acc[0]);
         P[1] += dot3(pyr, br);
       }
@@ -818,7 +827,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       // the other buffer, current only once the step is accepted.
       const double *py = v.py;
       V3 acc[1];
-      spmv_row<1, UB>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(py, j); }, acc);
+      spmv_row<1, UB, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(py, j); }, acc);
       const V3 pyr = ld3(v.py, r);
       P[0] += dot3(pyr, acc[0]);
       P[1] += dot3(pyr, ld3(v.b, r));
@@ -879,7 +888,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       // to py because neighbours still gather y in this launch.
       V3 acc[1];
       const double *y = yc;
-      spmv_row<1, UB>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(y, j); }, acc);
+      spmv_row<1, UB, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(y, j); }, acc);
       V3 g = sub(acc[0], ld3(v.b, r));
       Row R = ldrow(q, v, r);
       V3 yr = ld3(yc, r);
@@ -898,7 +907,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       V3 acc[2];
       const double *x = xc;
       const T *be = v.beta;
-      spmv_row<2>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[2]) {
+      spmv_row<2, 1, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[2]) {
         xs[0] = ld3(x, j);
         xs[1] = ld3(be, j);
       }, acc);
@@ -913,7 +922,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
     case OP_CL_PROP1: {
       V3 acc[1];
       const T *be = v.beta;
-      spmv_row<1, UB>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(be, j); }, acc);
+      spmv_row<1, UB, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(be, j); }, acc);
       st3(v.ap, r, acc[0]);
       V3 d = ld3(v.beta, r);
       P[0] += dot3(d, acc[0]);
@@ -980,7 +989,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
 // Ops on FP64 vectors in both forms: projection of the warm start, the power
 // method (its SpMV reads the matrix of the QP's form), and the outer
 // refinement of the mixed form.
-template <bool MX, int UB>
+template <bool MX, int UB, bool NT>
 __device__ __forceinline__ void row_work_fp64(const QpDev &q, int s, int lane, int r, int op,
                                               const QpState &st, double (&P)[kNumPart]) {
   const Vecs<double> &v = q.d;
@@ -1017,9 +1026,9 @@ __device__ __forceinline__ void row_work_fp64(const QpDev &q, int s, int lane, i
       V3 acc[1];
       auto gather = [&](int j, V3 (&xs)[1]) { xs[0] = ld3(in, j); };
       if (MX && st.mixed)
-        spmv_row<1, UB>(q.sl_off, q.col, q.f.val, s, lane, gather, acc);
+        spmv_row<1, UB, NT>(q.sl_off, q.col, q.f.val, s, lane, gather, acc);
       else
-        spmv_row<1, UB>(q.sl_off, q.col, v.val, s, lane, gather, acc);
+        spmv_row<1, UB, NT>(q.sl_off, q.col, v.val, s, lane, gather, acc);
       const V3 w = k == 0 ? acc[0] : V3{acc[0].x / nu, acc[0].y / nu, acc[0].z / nu};
       st3(wk, r, w);
       P[0] += dot3(w, w);
@@ -1036,7 +1045,7 @@ __device__ __forceinline__ void row_work_fp64(const QpDev &q, int s, int lane, i
       // is set up in the FP32 storage with d = 0.
       V3 acc[1];
       const double *x = v.x;
-      spmv_row<1, UB>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
+      spmv_row<1, UB, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
       V3 g = sub(acc[0], ld3(v.b, r));
       V3 xr = ld3(v.x, r);  // in flight with the row flag
       Row R = ldrow(q, v, r);
@@ -1074,7 +1083,7 @@ __device__ __forceinline__ bool fp64_op(int op) {
 
 // One slice per wavefront: a tile is up to TS consecutive slices, streamed
 // by the workgroup's waves at the same time.
-template <int TS, bool MX, int UB>
+template <int TS, bool MX, int UB, bool NT>
 __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &st, double (&P)[kNumPart]) {
   const int wave = threadIdx.x / kSlice;
   const int lane = threadIdx.x % kSlice;
@@ -1082,11 +1091,11 @@ __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &
   const int r = s * kSlice + lane;
   if (wave >= t.slices || r >= q.n_nodes) return;
   if (fp64_op(op)) {
-    row_work_fp64<MX, UB>(q, s, lane, r, op, st, P);
+    row_work_fp64<MX, UB, NT>(q, s, lane, r, op, st, P);
   } else if (MX && st.mixed) {
-    row_work<float, MX, UB>(q, q.f, s, lane, r, op, st, P);
+    row_work<float, MX, UB, NT>(q, q.f, s, lane, r, op, st, P);
   } else {
-    row_work<double, MX, UB>(q, q.d, s, lane, r, op, st, P);
+    row_work<double, MX, UB, NT>(q, q.d, s, lane, r, op, st, P);
   }
 }
 
@@ -1138,7 +1147,7 @@ constexpr int kUnrollTail = 6;
 
 // MX: the batch holds QPs of the mixed form (their inner ops run on the FP32
 // copies); FP64 QPs of the same batch take the FP64 path.
-template <int TS, bool MX, int UB>
+template <int TS, bool MX, int UB, bool NT>
 __global__ void __launch_bounds__(TS * kSlice, UB == kUnroll ? MPRGP_STEP_MIN_WAVES : 2)
     mprgp_step(const QpDev *__restrict__ qps,
                                                            const Tile *__restrict__ tiles,
@@ -1197,7 +1206,7 @@ __global__ void __launch_bounds__(TS * kSlice, UB == kUnroll ? MPRGP_STEP_MIN_WA
 #pragma unroll
     for (int j = 0; j < kNumPart; ++j) P[j] = 0.0;
     if (has_min) P[kMinSlot] = kDblMax;
-    tile_work<TS, MX, UB>(q, t, op, st, P);
+    tile_work<TS, MX, UB, NT>(q, t, op, st, P);
     // wave then block reduction in a fixed order, over the op's slots only
     const int ns = has_min ? kMinSlot + 1 : sum_slots(op);
     if (ns > 0) {
@@ -1481,26 +1490,33 @@ __global__ void __launch_bounds__(256) spmv_plain(const int *__restrict__ sl_off
 
 // Host-callable launch wrappers ------------------------------------------
 
-template <bool MX, int UB>
+template <bool MX, int UB, bool NT>
 static hipError_t launch_step_t(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
                                 int *running, int parity) {
   switch (ts) {
-    case 1: mprgp_step<1, MX, UB><<<grid, 1 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
-    case 2: mprgp_step<2, MX, UB><<<grid, 2 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
-    case 4: mprgp_step<4, MX, UB><<<grid, 4 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
-    case 8: mprgp_step<8, MX, UB><<<grid, 8 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
+    case 1: mprgp_step<1, MX, UB, NT><<<grid, 1 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
+    case 2: mprgp_step<2, MX, UB, NT><<<grid, 2 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
+    case 4: mprgp_step<4, MX, UB, NT><<<grid, 4 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
+    case 8: mprgp_step<8, MX, UB, NT><<<grid, 8 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
 hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
-                       int *running, int parity, bool mixed, bool tail) {
-  if (tail)
-    return mixed ? launch_step_t<true, kUnrollTail>(ts, grid, s, qps, tiles, running, parity)
-                 : launch_step_t<false, kUnrollTail>(ts, grid, s, qps, tiles, running, parity);
-  return mixed ? launch_step_t<true, kUnroll>(ts, grid, s, qps, tiles, running, parity)
-               : launch_step_t<false, kUnroll>(ts, grid, s, qps, tiles, running, parity);
+                       int *running, int parity, bool mixed, int variant) {
+  switch (variant) {
+    case kStepCached:
+      return mixed ? launch_step_t<true, kUnroll, false>(ts, grid, s, qps, tiles, running, parity)
+                   : launch_step_t<false, kUnroll, false>(ts, grid, s, qps, tiles, running, parity);
+    case kStepStream:
+      return mixed ? launch_step_t<true, kUnroll, true>(ts, grid, s, qps, tiles, running, parity)
+                   : launch_step_t<false, kUnroll, true>(ts, grid, s, qps, tiles, running, parity);
+    case kStepTail:
+      return mixed ? launch_step_t<true, kUnrollTail, false>(ts, grid, s, qps, tiles, running, parity)
+                   : launch_step_t<false, kUnrollTail, false>(ts, grid, s, qps, tiles, running, parity);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_gather_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n) {
