@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <atomic>
 #include <cstdarg>
 #include <cstdint>
@@ -94,11 +96,16 @@ int round_grid(int n_tiles);
 constexpr int kTailTiles = 256;
 // Launches whose matrices (SELL slots of the live QPs, at each QP's storage
 // precision) exceed the MI355X's 256 MB last-level cache stream them
-// nontemporal.
+// nontemporal.  MPRGP_STREAM_BYTES, when set, replaces the threshold (a
+// tuning knob; the variants give the same results).
 constexpr double kStreamBytes = 256.0 * 1024 * 1024;
+inline double stream_threshold() {
+  const char *e = getenv("MPRGP_STREAM_BYTES");
+  return (e && e[0]) ? atof(e) : kStreamBytes;
+}
 inline int step_variant(int n_tiles, double matrix_bytes) {
   if (n_tiles <= kTailTiles) return kStepTail;
-  return matrix_bytes > kStreamBytes ? kStepStream : kStepCached;
+  return matrix_bytes > stream_threshold() ? kStepStream : kStepCached;
 }
 void layout_tiles(const std::vector<Tile> &tiles, int grid, Tile *table);
 long solve_step_bound(const mprgp_params &p);

@@ -104,3 +104,30 @@ def test_full_batch_of_another_seed(ctx):
     rel = [np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(xs[M.MIXED], xs[M.FP64])]
     print(f"mixed vs fp64: max {max(rel):.2e}, median {np.median(rel):.2e}")
     assert max(rel) <= 1e-5
+
+
+def test_streamed_and_cached_variants_agree_bitwise(ctx, monkeypatch):
+    """The step kernel's streamed variant (matrix values and column indices
+    loaded nontemporal, launched while the live QPs' matrices exceed the
+    last-level cache) and its cached variant give the same bits: one batch
+    solved with the streaming threshold at 0 bytes (every launch above the
+    tail size streams) and at 1e18 bytes (none does), in both forms."""
+    from mprgp_amd import workloads
+    scenes = problems.batch_scenes(16, seed=31, tol=1e-6, max_it=4000)
+    qps = [workloads.DeviceQP(ctx, sc) for sc in scenes]
+    rc, st = M.solve_batch(ctx, [q.qp for q in qps], M.params(tol=1e-6, max_it=0, power_tol=1e-6))
+    ab = [s["alpha_bar"] for s in st]
+    for prec in (M.FP64, M.MIXED):
+        out = {}
+        for thr in ("0", "1e18"):
+            monkeypatch.setenv("MPRGP_STREAM_BYTES", thr)
+            for q in qps:
+                q.qp.reset_x()
+            per = [M.params(tol=1e-6, max_it=4000, alpha_bar=a, project_first=1, precision=prec, tiles=1)
+                   for a in ab]
+            rc, st = M.solve_batch(ctx, [q.qp for q in qps], per_qp=per)
+            assert all(s["code"] == 0 for s in st), (prec, thr)
+            out[thr] = ([q.qp.get_x() for q in qps], [s["iterations"] for s in st])
+        for a, b in zip(out["0"][0], out["1e18"][0]):
+            np.testing.assert_array_equal(a, b)
+        assert out["0"][1] == out["1e18"][1]
