@@ -29,7 +29,7 @@ DEFAULT_CANDIDATES = [
 ]
 
 # Bumped whenever the step kernel changes in a way that moves the best shape.
-CACHE_VERSION = 3
+CACHE_VERSION = 4
 
 
 def cache_path():
