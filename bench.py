@@ -44,7 +44,8 @@ def log(*a):
 
 def pmc_child(args):
     """Run under rocprofv3 --pmc by measure_traffic: the copy probe of known
-    size, or one batched solve of the benchmark workload (direct launches)."""
+    size, one batched solve of the benchmark workload, or the FP64 batched
+    SpMV leg (power method to its cap), all with direct launches."""
     import mprgp_amd as M
     ctx = M.Context(0)
     if args.pmc_child == "copy":
@@ -53,6 +54,12 @@ def pmc_child(args):
     from mprgp_amd import problems, workloads
     scenes = problems.batch_scenes(args.qps, seed=4, tol=args.tol, max_it=4000)
     qps = [workloads.DeviceQP(ctx, sc) for sc in scenes]
+    if args.pmc_child == "spmv":
+        per = [M.params(tol=1e-6, max_it=0, power_tol=1e-300, precision=M.FP64, tiles=args.ts or 4,
+                        graph_steps=-16) for _ in qps]
+        rc, stats = M.solve_batch(ctx, [q.qp for q in qps], per_qp=per)
+        print(json.dumps({"launches": stats[0]["launches"]}), flush=True)
+        return
     rc, st = M.solve_batch(ctx, [q.qp for q in qps], M.params(tol=args.tol, max_it=0, power_tol=1e-6))
     prec = M.MIXED if args.precision == "mixed" else M.FP64
     per = [M.params(tol=args.tol, max_it=4000, alpha_bar=s["alpha_bar"], project_first=1, precision=prec,
@@ -64,19 +71,21 @@ def pmc_child(args):
 COPY_BYTES = 1 << 28
 
 
-def measure_traffic(args, tiles):
-    """HBM bytes per launch of the step kernel in one batched solve at the
-    headline's launch shape (``tiles`` wavefronts per workgroup): two
-    rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over a child run of
-    the workload, each scaled by what the same counter reads for the copy
-    probe (COPY_BYTES read and written per dispatch).  The children run while
-    this process waits.  Returns None when rocprofv3 is unavailable."""
+def measure_traffic(args, tiles, what="solve"):
+    """HBM bytes per launch of the step kernel at the headline's launch shape
+    (``tiles`` wavefronts per workgroup) in one batched solve (``what`` =
+    "solve") or in the FP64 batched SpMV leg ("spmv"): two rocprofv3 --pmc
+    passes (FETCH_SIZE, then WRITE_SIZE) over a child run of the workload,
+    each scaled by what the same counter reads for the copy probe
+    (COPY_BYTES read and written per dispatch).  The children run while this
+    process waits.  Returns None when rocprofv3 is unavailable."""
     rocprof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
     if not os.path.exists(rocprof):
         return None
     tmp = tempfile.mkdtemp(prefix="mprgp_pmc_")
     # both unroll variants of the solve's kernel (mprgp_step<TS, MX, 3 or 6>)
-    kernel = f"mprgp_step<{tiles}, {'true' if args.precision == 'mixed' else 'false'},"
+    mixed = args.precision == "mixed" and what == "solve"
+    kernel = f"mprgp_step<{tiles}, {'true' if mixed else 'false'},"
 
     def run(counter, child, tag):
         d = os.path.join(tmp, tag)
@@ -99,17 +108,17 @@ def measure_traffic(args, tiles):
                     with open(os.path.join(root, fn)) as f:
                         for row in csv.DictReader(f):
                             name = row["Kernel_Name"]
-                            if (child == "copy" and "copy_stream" in name) or (child == "solve" and kernel in name):
+                            if (child == "copy" and "copy_stream" in name) or (child != "copy" and kernel in name):
                                 vals.append(float(row["Counter_Value"]))
-        if child == "solve" and launches:
-            vals = vals[-launches:]  # the timed solve's launches (the power method runs first)
+        if child != "copy" and launches:
+            vals = vals[-launches:]  # the measured call's launches (the solve's power method runs first)
         return sum(vals) / len(vals) * 1024.0 if vals else None  # KiB -> bytes
 
     try:
         cal_f = COPY_BYTES / run("FETCH_SIZE", "copy", "cf")
         cal_w = COPY_BYTES / run("WRITE_SIZE", "copy", "cw")
-        fetch = run("FETCH_SIZE", "solve", "sf") * cal_f
-        write = run("WRITE_SIZE", "solve", "sw") * cal_w
+        fetch = run("FETCH_SIZE", what, "sf") * cal_f
+        write = run("WRITE_SIZE", what, "sw") * cal_w
     except (RuntimeError, TypeError, ZeroDivisionError) as e:
         log(f"traffic measurement skipped: {e}")
         return None
@@ -170,7 +179,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="library default shape instead of the autotuner's")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM counter passes")
     ap.add_argument("--no-spmv", action="store_true", help="skip the batched SpMV bandwidth leg")
-    ap.add_argument("--pmc-child", choices=["copy", "solve"], help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-child", choices=["copy", "solve", "spmv"], help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
         pmc_child(args)
@@ -261,6 +270,12 @@ def main():
     spmv = None
     if not args.no_spmv:
         spmv = spmv_leg(M, workloads, ctx, qps, tiles, shape["graph_steps"] if shape else args.graph_steps)
+        st = None if args.no_traffic else measure_traffic(args, tiles, "spmv")
+        if st:
+            f64 = spmv["fp64_matrix"]
+            f64["traffic"] = st["fetch_bytes"] + st["write_bytes"]
+            f64["traffic_detail"] = st
+            f64["traffic_vs_algorithmic"] = round(f64["traffic"] / f64["bytes_per_launch"], 3)
         log(f"batched spmv (power method to its cap): {spmv}")
 
     other = None
