@@ -12,8 +12,8 @@ from mprgp_amd import serving
 pytestmark = pytest.mark.gpu
 
 
-def sessions(ctx, seed=11, count=6):
-    return serving.make_sessions(ctx, count=count, seed=seed, dof_range=(1.5e3, 9e3), reassemble=2)
+def sessions(ctx, seed=11, count=6, dof_range=(1.5e3, 9e3)):
+    return serving.make_sessions(ctx, count=count, seed=seed, dof_range=dof_range, reassemble=2)
 
 
 def prec(s):
@@ -35,10 +35,19 @@ def test_engine_equals_direct_solve(ctx):
     eng.close()
 
 
-def test_sessions_alone_equal_mix(ctx):
+# "small": every launch has few tiles (the tail variant of the step kernel);
+# "streamed": sessions of 30k-60k DoF in one-slice tiles, so the mix launches
+# the streamed variant (threshold 0 bytes) and a session alone the tail one.
+@pytest.mark.parametrize("case", ["small", "streamed"])
+def test_sessions_alone_equal_mix(ctx, case, monkeypatch):
     steps = 5
-    alone, mix = sessions(ctx), sessions(ctx)
-    eng = M.Engine(ctx, max_active=8, tiles=4, graph_steps=8)
+    if case == "small":
+        alone, mix = sessions(ctx), sessions(ctx)
+        eng = M.Engine(ctx, max_active=8, tiles=4, graph_steps=8)
+    else:
+        monkeypatch.setenv("MPRGP_STREAM_BYTES", "0")
+        alone, mix = sessions(ctx, seed=13, dof_range=(3e4, 6e4)), sessions(ctx, seed=13, dof_range=(3e4, 6e4))
+        eng = M.Engine(ctx, max_active=8, tiles=1, graph_steps=8)
     for s in alone:  # one session at a time
         for _ in range(steps):
             s.prepare()
