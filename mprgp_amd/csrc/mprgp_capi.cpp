@@ -941,11 +941,36 @@ void BatchTables::destroy() {
   ctx->release(d_tiles, sizeof(Tile) * (size_t)tiles_cap);
   ctx->release(d_ctrl, 4 * sizeof(int));
   ctx->release(d_flags, sizeof(int) * (size_t)flags_cap);
+  ctx->release(d_states, 2 * sizeof(QpState) * (size_t)states_cap);
+  ctx->release(d_stptr, sizeof(QpState *) * (size_t)states_cap);
+  if (h_states) (void)hipHostFree(h_states);
+  if (h_stptr) (void)hipHostFree(h_stptr);
   d_qps = nullptr;
   d_tiles = nullptr;
   d_ctrl = nullptr;
   d_flags = nullptr;
-  qps_cap = tiles_cap = flags_cap = 0;
+  d_states = h_states = nullptr;
+  d_stptr = h_stptr = nullptr;
+  qps_cap = tiles_cap = flags_cap = states_cap = 0;
+}
+
+// Grows solve_batch's state staging buffers (called with the stream idle).
+int BatchTables::ensure_states(int n_qps) {
+  if (n_qps <= states_cap) return MPRGP_OK;
+  const int cap = std::max(n_qps, 2 * states_cap);
+  ctx->release(d_states, 2 * sizeof(QpState) * (size_t)states_cap);
+  ctx->release(d_stptr, sizeof(QpState *) * (size_t)states_cap);
+  if (h_states) (void)hipHostFree(h_states);
+  if (h_stptr) (void)hipHostFree(h_stptr);
+  d_states = h_states = nullptr;
+  d_stptr = h_stptr = nullptr;
+  states_cap = 0;
+  if (int rc = ctx->alloc((void **)&d_states, 2 * sizeof(QpState) * (size_t)cap)) return rc;
+  if (int rc = ctx->alloc((void **)&d_stptr, sizeof(QpState *) * (size_t)cap)) return rc;
+  MPRGP_HIP(hipHostMalloc((void **)&h_states, 2 * sizeof(QpState) * (size_t)cap, hipHostMallocDefault));
+  MPRGP_HIP(hipHostMalloc((void **)&h_stptr, sizeof(QpState *) * (size_t)cap, hipHostMallocDefault));
+  states_cap = cap;
+  return MPRGP_OK;
 }
 
 // Grows the tables; graphs captured the old table pointers, so a regrowth
@@ -1240,19 +1265,24 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
   const int n_tiles = (int)tiles.size();
   int grid = round_grid(n_tiles);
   if (int rc = tab.ensure(count, grid)) return rc;
+  if (int rc = tab.ensure_states(count)) return rc;
   std::vector<std::vector<Tile>> uploads(1, std::vector<Tile>(grid));  // host buffers of table uploads
   layout_tiles(tiles, grid, uploads.back().data());
   hipStream_t s = ctx->stream;
   std::vector<QpDev> devs(count);
-  std::vector<QpState> states(2 * (size_t)count);
+  QpState *states = tab.h_states;  // pinned; [i][2]
   for (int i = 0; i < count; ++i) {
     QpState &st = states[2 * (size_t)i];
     if (int rc = prepare_qp(qps[i], params[stride ? i : 0], s, tiles_for(n_slices[i], span), tab.d_flags + i, 1,
                             devs[i], st))
       return rc;
     states[2 * (size_t)i + 1] = st;
-    MPRGP_HIP(hipMemcpyAsync(qps[i]->st, &states[2 * (size_t)i], 2 * sizeof(QpState), hipMemcpyHostToDevice, s));
+    tab.h_stptr[i] = qps[i]->st;
   }
+  // every QP's two state copies in one upload and one scatter
+  MPRGP_HIP(hipMemcpyAsync(tab.d_states, states, 2 * sizeof(QpState) * (size_t)count, hipMemcpyHostToDevice, s));
+  MPRGP_HIP(hipMemcpyAsync(tab.d_stptr, tab.h_stptr, sizeof(QpState *) * (size_t)count, hipMemcpyHostToDevice, s));
+  MPRGP_HIP(launch_scatter_states(s, tab.d_states, tab.d_stptr, count));
   const int running0 = count;
   MPRGP_HIP(hipMemcpyAsync(tab.d_qps, devs.data(), sizeof(QpDev) * count, hipMemcpyHostToDevice, s));
   MPRGP_HIP(hipMemcpyAsync(tab.d_tiles, uploads.back().data(), sizeof(Tile) * grid, hipMemcpyHostToDevice, s));
@@ -1343,10 +1373,13 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
   float ms = 0.f;
   MPRGP_HIP(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
 
+  // every QP's two state copies in one gather and one readback
+  MPRGP_HIP(launch_gather_states(s, tab.d_stptr, tab.d_states, count));
+  MPRGP_HIP(hipMemcpyAsync(tab.h_states, tab.d_states, 2 * sizeof(QpState) * (size_t)count, hipMemcpyDeviceToHost, s));
+  MPRGP_HIP(hipStreamSynchronize(s));
   int worst = MPRGP_OK;
   for (int i = 0; i < count; ++i) {
-    QpState both[2];
-    MPRGP_HIP(hipMemcpy(both, qps[i]->st, 2 * sizeof(QpState), hipMemcpyDeviceToHost));
+    const QpState *both = tab.h_states + 2 * (size_t)i;
     int rc = fill_stats(qps[i], both, ms, launched, stats ? stats + i : nullptr);
     if (rc == MPRGP_ENAN || rc == MPRGP_EHIP) worst = rc;
     else if (rc == MPRGP_NOT_CONVERGED && worst == MPRGP_OK) worst = rc;

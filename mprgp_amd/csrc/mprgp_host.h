@@ -42,6 +42,9 @@ int decoupled_solve(int device, int n, const int *a_outer, const int *a_inner, c
 hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
                        int *running, int parity, bool mixed, int variant = kStepCached);
 hipError_t launch_gather_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n);
+// the two state copies of n QPs: staging [i][2] <-> each QP's own (ptr[i])
+hipError_t launch_scatter_states(hipStream_t s, const QpState *src, QpState *const *dst, int n);
+hipError_t launch_gather_states(hipStream_t s, QpState *const *src, QpState *dst, int n);
 hipError_t launch_scatter_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n);
 hipError_t launch_setup_diag(hipStream_t s, const int *sl_off, const int *col, const double *val,
                              double *dinv, int n);
@@ -76,6 +79,12 @@ struct BatchTables {
   int *d_ctrl = nullptr;   // [1] = running QPs
   int *d_flags = nullptr;  // per batch slot: tag of the QP that reached DONE
   int flags_cap = 0;
+  // solve_batch's QP states in one staging copy each way: [qp][2] states,
+  // device and pinned host, and each QP's own state address
+  QpState *d_states = nullptr, *h_states = nullptr;
+  QpState **d_stptr = nullptr, **h_stptr = nullptr;
+  int states_cap = 0;
+  int ensure_states(int n_qps);
   std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> graphs;
   std::map<std::tuple<int, int, int, int, int>, hipGraph_t> graph_defs;
   int init(mprgp_ctx *c);

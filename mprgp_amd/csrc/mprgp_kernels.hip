@@ -1471,6 +1471,18 @@ __global__ void __launch_bounds__(256) copy_stream(const double2 *__restrict__ i
   if (i < n) out[i] = in[i];
 }
 
+// The two state copies (1024 bytes) of QP i between a staging buffer and the
+// QP's own state: one workgroup of 64 lanes per QP, one 16-byte word a lane.
+static_assert(2 * sizeof(QpState) == 64 * sizeof(double2), "two states: 64 words of 16 bytes");
+__global__ void __launch_bounds__(64) scatter_states(const double2 *__restrict__ src,
+                                                     double2 *const *__restrict__ dst, int n) {
+  if ((int)blockIdx.x < n) dst[blockIdx.x][threadIdx.x] = src[(size_t)blockIdx.x * 64 + threadIdx.x];
+}
+__global__ void __launch_bounds__(64) gather_states(const double2 *const *__restrict__ src,
+                                                    double2 *__restrict__ dst, int n) {
+  if ((int)blockIdx.x < n) dst[(size_t)blockIdx.x * 64 + threadIdx.x] = src[blockIdx.x][threadIdx.x];
+}
+
 // Plain SELL SpMV y = A x (used by tests and by the assembly's A x1 term).
 __global__ void __launch_bounds__(256) spmv_plain(const int *__restrict__ sl_off,
                                                   const int *__restrict__ col,
@@ -1519,6 +1531,17 @@ hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const 
   }
 }
 
+hipError_t launch_scatter_states(hipStream_t s, const QpState *src, QpState *const *dst, int n) {
+  if (n <= 0) return hipSuccess;
+  scatter_states<<<n, 64, 0, s>>>(reinterpret_cast<const double2 *>(src),
+                                  reinterpret_cast<double2 *const *>(dst), n);
+  return hipGetLastError();
+}
+hipError_t launch_gather_states(hipStream_t s, QpState *const *src, QpState *dst, int n) {
+  if (n <= 0) return hipSuccess;
+  gather_states<<<n, 64, 0, s>>>(reinterpret_cast<const double2 *const *>(src), reinterpret_cast<double2 *>(dst), n);
+  return hipGetLastError();
+}
 hipError_t launch_gather_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n) {
   if (n <= 0) return hipSuccess;
   gather_nodes<<<(n + 255) / 256, 256, 0, s>>>(in, out, perm, n);
