@@ -1129,7 +1129,7 @@ size_t solve_bytes(const mprgp_qp *q, const mprgp_params &p) {
 // A power start vector needs a host upload: it goes through `q->stage` and
 // synchronises `s`.
 int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, int *done_flag, int tag,
-               QpDev &d, QpState &st) {
+               QpDev &d, QpState &st, bool rows) {
   std::memset(&d, 0, sizeof(d));
   d.n_nodes = q->n;
   d.n_slices = q->A->n_slices;
@@ -1197,7 +1197,7 @@ int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, i
     }
   }
   // tangent-Jacobi factors depend on the current matrix values
-  MPRGP_HIP(launch_setup_rows(s, q->A->d_dinv, q->cn, q->cf, q->cs, q->n));
+  if (rows) MPRGP_HIP(launch_setup_rows(s, q->A->d_dinv, q->cn, q->cf, q->cs, q->n));
   return MPRGP_OK;
 }
 
@@ -1271,11 +1271,13 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
   hipStream_t s = ctx->stream;
   std::vector<QpDev> devs(count);
   QpState *states = tab.h_states;  // pinned; [i][2]
+  int max_nodes = 0;
   for (int i = 0; i < count; ++i) {
     QpState &st = states[2 * (size_t)i];
     if (int rc = prepare_qp(qps[i], params[stride ? i : 0], s, tiles_for(n_slices[i], span), tab.d_flags + i, 1,
-                            devs[i], st))
+                            devs[i], st, count > 65535))
       return rc;
+    max_nodes = std::max(max_nodes, qps[i]->n);
     states[2 * (size_t)i + 1] = st;
     tab.h_stptr[i] = qps[i]->st;
   }
@@ -1285,6 +1287,9 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
   MPRGP_HIP(launch_scatter_states(s, tab.d_states, tab.d_stptr, count));
   const int running0 = count;
   MPRGP_HIP(hipMemcpyAsync(tab.d_qps, devs.data(), sizeof(QpDev) * count, hipMemcpyHostToDevice, s));
+  // the rows' tangent-Jacobi factors of every QP (they follow the current
+  // matrix values) in one launch
+  if (count <= 65535) MPRGP_HIP(launch_setup_rows_batch(s, tab.d_qps, count, max_nodes));
   MPRGP_HIP(hipMemcpyAsync(tab.d_tiles, uploads.back().data(), sizeof(Tile) * grid, hipMemcpyHostToDevice, s));
   MPRGP_HIP(hipMemcpyAsync(tab.d_ctrl + 1, &running0, sizeof(int), hipMemcpyHostToDevice, s));
   MPRGP_HIP(hipMemsetAsync(tab.d_flags, 0, sizeof(int) * count, s));

@@ -42,6 +42,8 @@ int decoupled_solve(int device, int n, const int *a_outer, const int *a_inner, c
 hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
                        int *running, int parity, bool mixed, int variant = kStepCached);
 hipError_t launch_gather_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n);
+// setup_rows for `count` QPs of a device QpDev table (count <= 65535)
+hipError_t launch_setup_rows_batch(hipStream_t s, const QpDev *qps, int count, int max_nodes);
 // the two state copies of n QPs: staging [i][2] <-> each QP's own (ptr[i])
 hipError_t launch_scatter_states(hipStream_t s, const QpState *src, QpState *const *dst, int n);
 hipError_t launch_gather_states(hipStream_t s, QpState *const *src, QpState *dst, int n);
@@ -119,8 +121,10 @@ inline int step_variant(int n_tiles, double matrix_bytes) {
 void layout_tiles(const std::vector<Tile> &tiles, int grid, Tile *table);
 long solve_step_bound(const mprgp_params &p);
 size_t solve_bytes(const mprgp_qp *q, const mprgp_params &p);
+// rows = false: the caller runs the rows' tangent-Jacobi set-up itself
+// (solve_batch: one launch for the whole batch).
 int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, int *done_flag, int tag,
-               QpDev &d, QpState &st);
+               QpDev &d, QpState &st, bool rows = true);
 int fill_stats(mprgp_qp *q, const QpState both[2], float ms, long launched, mprgp_stats *out);
 
 }  // namespace mprgp

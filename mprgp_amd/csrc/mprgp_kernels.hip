@@ -1317,20 +1317,31 @@ __global__ void __launch_bounds__(256) dot_parts(const double *__restrict__ a, c
 }
 
 // cs = 1/((n o dinv).n) per row (MPRGPPrecondition.h:81-88).
-__global__ void setup_rows(const double *__restrict__ dinv, const double *__restrict__ cn,
-                           const unsigned char *__restrict__ cf, double *__restrict__ cs, int n_nodes) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n_nodes) return;
-  if (!cf[r]) {
-    cs[r] = 0.0;
-    return;
-  }
+__device__ __forceinline__ double row_factor(const double *dinv, const double *cn, unsigned char cf, int r) {
+  if (!cf) return 0.0;
   double s = 0.0;
   for (int d = 0; d < 3; ++d) {
     double n = cn[3 * (size_t)r + d];
     s += (n * dinv[3 * (size_t)r + d]) * n;
   }
-  cs[r] = 1.0 / s;
+  return 1.0 / s;
+}
+
+__global__ void setup_rows(const double *__restrict__ dinv, const double *__restrict__ cn,
+                           const unsigned char *__restrict__ cf, double *__restrict__ cs, int n_nodes) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_nodes) return;
+  cs[r] = row_factor(dinv, cn, cf[r], r);
+}
+
+// setup_rows for every QP of a batch table in one launch: blockIdx.y is the
+// QP, blockIdx.x a run of 256 of its nodes.  cs is the QP's own factor array
+// (read-only to the step kernel, written here).
+__global__ void __launch_bounds__(256) setup_rows_batch(const QpDev *__restrict__ qps) {
+  const QpDev &q = qps[blockIdx.y];
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= q.n_nodes) return;
+  const_cast<double *>(q.d.cs)[r] = row_factor(q.d.dinv, q.d.cn, q.cf[r], r);
 }
 
 // Decoupled plane collision at the current iterate: node r gets the row of the
@@ -1531,6 +1542,12 @@ hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const 
   }
 }
 
+hipError_t launch_setup_rows_batch(hipStream_t s, const QpDev *qps, int count, int max_nodes) {
+  if (count <= 0 || max_nodes <= 0) return hipSuccess;
+  if (count > 65535) return hipErrorInvalidValue;
+  setup_rows_batch<<<dim3((max_nodes + 255) / 256, count), 256, 0, s>>>(qps);
+  return hipGetLastError();
+}
 hipError_t launch_scatter_states(hipStream_t s, const QpState *src, QpState *const *dst, int n) {
   if (n <= 0) return hipSuccess;
   scatter_states<<<n, 64, 0, s>>>(reinterpret_cast<const double2 *>(src),
