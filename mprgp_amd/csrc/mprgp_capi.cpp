@@ -1275,21 +1275,26 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
   for (int i = 0; i < count; ++i) {
     QpState &st = states[2 * (size_t)i];
     if (int rc = prepare_qp(qps[i], params[stride ? i : 0], s, tiles_for(n_slices[i], span), tab.d_flags + i, 1,
-                            devs[i], st, count > 65535))
+                            devs[i], st, count == 1 || count > 65535))
       return rc;
     max_nodes = std::max(max_nodes, qps[i]->n);
     states[2 * (size_t)i + 1] = st;
     tab.h_stptr[i] = qps[i]->st;
   }
-  // every QP's two state copies in one upload and one scatter
-  MPRGP_HIP(hipMemcpyAsync(tab.d_states, states, 2 * sizeof(QpState) * (size_t)count, hipMemcpyHostToDevice, s));
-  MPRGP_HIP(hipMemcpyAsync(tab.d_stptr, tab.h_stptr, sizeof(QpState *) * (size_t)count, hipMemcpyHostToDevice, s));
-  MPRGP_HIP(launch_scatter_states(s, tab.d_states, tab.d_stptr, count));
+  // every QP's two state copies in one upload and one scatter (a lone QP:
+  // one direct copy)
+  if (count == 1) {
+    MPRGP_HIP(hipMemcpyAsync(qps[0]->st, states, 2 * sizeof(QpState), hipMemcpyHostToDevice, s));
+  } else {
+    MPRGP_HIP(hipMemcpyAsync(tab.d_states, states, 2 * sizeof(QpState) * (size_t)count, hipMemcpyHostToDevice, s));
+    MPRGP_HIP(hipMemcpyAsync(tab.d_stptr, tab.h_stptr, sizeof(QpState *) * (size_t)count, hipMemcpyHostToDevice, s));
+    MPRGP_HIP(launch_scatter_states(s, tab.d_states, tab.d_stptr, count));
+  }
   const int running0 = count;
   MPRGP_HIP(hipMemcpyAsync(tab.d_qps, devs.data(), sizeof(QpDev) * count, hipMemcpyHostToDevice, s));
   // the rows' tangent-Jacobi factors of every QP (they follow the current
   // matrix values) in one launch
-  if (count <= 65535) MPRGP_HIP(launch_setup_rows_batch(s, tab.d_qps, count, max_nodes));
+  if (count > 1 && count <= 65535) MPRGP_HIP(launch_setup_rows_batch(s, tab.d_qps, count, max_nodes));
   MPRGP_HIP(hipMemcpyAsync(tab.d_tiles, uploads.back().data(), sizeof(Tile) * grid, hipMemcpyHostToDevice, s));
   MPRGP_HIP(hipMemcpyAsync(tab.d_ctrl + 1, &running0, sizeof(int), hipMemcpyHostToDevice, s));
   MPRGP_HIP(hipMemsetAsync(tab.d_flags, 0, sizeof(int) * count, s));
@@ -1379,8 +1384,12 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
   MPRGP_HIP(hipEventElapsedTime(&ms, ctx->t0, ctx->t1));
 
   // every QP's two state copies in one gather and one readback
-  MPRGP_HIP(launch_gather_states(s, tab.d_stptr, tab.d_states, count));
-  MPRGP_HIP(hipMemcpyAsync(tab.h_states, tab.d_states, 2 * sizeof(QpState) * (size_t)count, hipMemcpyDeviceToHost, s));
+  if (count == 1) {
+    MPRGP_HIP(hipMemcpyAsync(tab.h_states, qps[0]->st, 2 * sizeof(QpState), hipMemcpyDeviceToHost, s));
+  } else {
+    MPRGP_HIP(launch_gather_states(s, tab.d_stptr, tab.d_states, count));
+    MPRGP_HIP(hipMemcpyAsync(tab.h_states, tab.d_states, 2 * sizeof(QpState) * (size_t)count, hipMemcpyDeviceToHost, s));
+  }
   MPRGP_HIP(hipStreamSynchronize(s));
   int worst = MPRGP_OK;
   for (int i = 0; i < count; ++i) {
