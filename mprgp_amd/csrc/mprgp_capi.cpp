@@ -10,29 +10,9 @@
 #include <numeric>
 
 #include "mprgp_host.h"
+#include "mprgp_qpfile.h"
 
 namespace mprgp {
-
-static thread_local std::string g_error;
-
-void set_error(const char *fmt, ...) {
-  char buf[1024];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof(buf), fmt, ap);
-  va_end(ap);
-  g_error = buf;
-}
-
-int fail(int code, const char *fmt, ...) {
-  char buf[1024];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof(buf), fmt, ap);
-  va_end(ap);
-  g_error = buf;
-  return code;
-}
 
 // splitmix64: deterministic start vector of the power method (replaces the
 // reference's unseeded setRandom()).
@@ -85,6 +65,7 @@ int mprgp_ctx::scratch_reserve(size_t bytes) {
 // ------------------------------------------------------------------ context
 
 extern "C" void mprgp_params_default(mprgp_params *p) {
+  MPRGP_TRY
   if (!p) return;
   std::memset(p, 0, sizeof(*p));
   p->tol = 1e-3;  // MPRGPDecoupledCon::solve default
@@ -96,11 +77,11 @@ extern "C" void mprgp_params_default(mprgp_params *p) {
   p->use_ev = 1;
   p->alpha_bar = 0.0;
   p->power_tol = 0.0;
+  MPRGP_CATCH_VOID
 }
 
-extern "C" const char *mprgp_last_error(void) { return g_error.c_str(); }
-
 extern "C" int mprgp_device_count(int *count) {
+  MPRGP_TRY
   if (!count) return fail(MPRGP_EBADINPUT, "count is NULL");
   int n = 0;
   hipError_t e = hipGetDeviceCount(&n);
@@ -111,9 +92,11 @@ extern "C" int mprgp_device_count(int *count) {
   }
   *count = n;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_ctx_create(int device, size_t hbm_budget_bytes, mprgp_ctx **out) {
+  MPRGP_TRY
   if (!out) return fail(MPRGP_EBADINPUT, "out is NULL");
   *out = nullptr;
   int n = 0;
@@ -140,9 +123,11 @@ extern "C" int mprgp_ctx_create(int device, size_t hbm_budget_bytes, mprgp_ctx *
   }
   *out = c;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" void mprgp_ctx_destroy(mprgp_ctx *c) {
+  MPRGP_TRY
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -155,13 +140,16 @@ extern "C" void mprgp_ctx_destroy(mprgp_ctx *c) {
   if (c->q[1]) (void)hipEventDestroy(c->q[1]);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+  MPRGP_CATCH_VOID
 }
 
 extern "C" int mprgp_ctx_memory(mprgp_ctx *c, size_t *in_use, size_t *budget) {
+  MPRGP_TRY
   if (!c) return fail(MPRGP_EBADINPUT, "ctx is NULL");
   if (in_use) *in_use = c->in_use;
   if (budget) *budget = c->budget;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 // ------------------------------------------------------------------ matrix
@@ -274,14 +262,17 @@ int mprgp::build_matrix(mprgp_ctx *ctx, int n_nodes, const int *rowptr, const in
 
 extern "C" int mprgp_matrix_create_bsr3(mprgp_ctx *ctx, int n_nodes, const int *rowptr, const int *colidx,
                                         const double *blocks, mprgp_matrix **out) {
+  MPRGP_TRY
   if (!ctx || !blocks) return fail(MPRGP_EBADINPUT, "ctx or blocks is NULL");
   MPRGP_HIP(hipSetDevice(ctx->device));
   std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
   return build_matrix(ctx, n_nodes, rowptr, colidx, blocks, out);
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_matrix_create_csr(mprgp_ctx *ctx, int n, const int *ptr, const int *idx,
                                        const double *val, mprgp_matrix **out) {
+  MPRGP_TRY
   if (!ctx || !ptr || !idx || !val || !out) return fail(MPRGP_EBADINPUT, "NULL argument");
   if (n <= 0 || n % 3) return fail(MPRGP_EBADINPUT, "n = %d is not a positive multiple of 3", n);
   const int N = n / 3;
@@ -319,9 +310,11 @@ extern "C" int mprgp_matrix_create_csr(mprgp_ctx *ctx, int n, const int *ptr, co
   MPRGP_HIP(hipSetDevice(ctx->device));
   std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
   return build_matrix(ctx, N, rowptr.data(), colidx.data(), blocks.data(), out);
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_matrix_update_bsr3(mprgp_matrix *A, const double *blocks) {
+  MPRGP_TRY
   if (!A || !blocks) return fail(MPRGP_EBADINPUT, "NULL argument");
   mprgp_ctx *ctx = A->ctx;
   MPRGP_HIP(hipSetDevice(ctx->device));
@@ -349,17 +342,21 @@ extern "C" int mprgp_matrix_update_bsr3(mprgp_matrix *A, const double *blocks) {
   MPRGP_HIP(hipStreamSynchronize(s));
   A->version++;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_matrix_info(const mprgp_matrix *A, int *n_nodes, long long *n_blocks, long long *bytes) {
+  MPRGP_TRY
   if (!A) return fail(MPRGP_EBADINPUT, "NULL matrix");
   if (n_nodes) *n_nodes = A->n_nodes;
   if (n_blocks) *n_blocks = A->nb;
   if (bytes) *bytes = (long long)A->bytes;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_matrix_spmv(mprgp_matrix *A, const double *x, double *y) {
+  MPRGP_TRY
   if (!A || !x || !y) return fail(MPRGP_EBADINPUT, "NULL argument");
   mprgp_ctx *ctx = A->ctx;
   MPRGP_HIP(hipSetDevice(ctx->device));
@@ -380,9 +377,11 @@ extern "C" int mprgp_matrix_spmv(mprgp_matrix *A, const double *x, double *y) {
   ctx->release(buf, 3 * vb);
   if (e != hipSuccess) return fail(MPRGP_EHIP, "spmv failed: %s", hipGetErrorString(e));
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_matrix_diagonal(mprgp_matrix *A, double *diag) {
+  MPRGP_TRY
   if (!A || !diag) return fail(MPRGP_EBADINPUT, "NULL argument");
   mprgp_ctx *ctx = A->ctx;
   MPRGP_HIP(hipSetDevice(ctx->device));
@@ -397,9 +396,11 @@ extern "C" int mprgp_matrix_diagonal(mprgp_matrix *A, double *diag) {
   MPRGP_HIP(hipMemcpyAsync(diag, du, vb, hipMemcpyDeviceToHost, s));
   MPRGP_HIP(hipStreamSynchronize(s));
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_matrix_func_value(mprgp_matrix *A, const double *x, const double *b, double *f) {
+  MPRGP_TRY
   if (!A || !x || !b || !f) return fail(MPRGP_EBADINPUT, "NULL argument");
   mprgp_ctx *ctx = A->ctx;
   MPRGP_HIP(hipSetDevice(ctx->device));
@@ -424,9 +425,11 @@ extern "C" int mprgp_matrix_func_value(mprgp_matrix *A, const double *x, const d
   MPRGP_HIP(hipStreamSynchronize(s));
   *f = 0.5 * r[0] - r[1];
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" void mprgp_matrix_destroy(mprgp_matrix *A) {
+  MPRGP_TRY
   if (!A) return;
   mprgp_ctx *ctx = A->ctx;
   (void)hipSetDevice(ctx->device);
@@ -439,6 +442,7 @@ extern "C" void mprgp_matrix_destroy(mprgp_matrix *A) {
   ctx->release(A->d_perm, sizeof(int) * A->n_nodes);
   ctx->release(A->d_val32, sizeof(float) * A->slots * 9);
   delete A;
+  MPRGP_CATCH_VOID
 }
 
 // FP32 copy of the values for the mixed form.
@@ -454,7 +458,9 @@ static int ensure_matrix32(mprgp_matrix *A, hipStream_t s) {
   return MPRGP_OK;
 }
 
-static int ensure_qp32(mprgp_qp *q) {
+// s: the stream the QP's solve will run on (the engine's own, for a QP
+// submitted to an engine), so the clearing is ordered before the solve.
+static int ensure_qp32(mprgp_qp *q, hipStream_t s) {
   if (q->base32) return MPRGP_OK;
   const size_t n3 = 3 * (size_t)q->n;
   const size_t bytes = (3 * n3 + (size_t)q->n) * sizeof(double) + (9 * n3 + 64) * sizeof(float);
@@ -474,7 +480,7 @@ static int ensure_qp32(mprgp_qp *q) {
     *v = fp32;
     fp32 += n3;
   }
-  MPRGP_HIP(hipMemsetAsync(q->base32, 0, q->bytes32, q->ctx->stream));
+  MPRGP_HIP(hipMemsetAsync(q->base32, 0, q->bytes32, s));
   return MPRGP_OK;
 }
 
@@ -483,6 +489,7 @@ static int ensure_qp32(mprgp_qp *q) {
 static size_t qp_tiles_max(int n_nodes) { return (size_t)((n_nodes + kSlice - 1) / kSlice); }
 
 extern "C" int mprgp_qp_create(mprgp_matrix *A, mprgp_qp **out) {
+  MPRGP_TRY
   if (!A || !out) return fail(MPRGP_EBADINPUT, "NULL argument");
   *out = nullptr;
   mprgp_ctx *ctx = A->ctx;
@@ -531,9 +538,11 @@ extern "C" int mprgp_qp_create(mprgp_matrix *A, mprgp_qp **out) {
   }
   *out = q;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" void mprgp_qp_destroy(mprgp_qp *q) {
+  MPRGP_TRY
   if (!q) return;
   (void)hipSetDevice(q->ctx->device);
   std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
@@ -541,6 +550,7 @@ extern "C" void mprgp_qp_destroy(mprgp_qp *q) {
   q->ctx->release(q->base, q->bytes);
   q->ctx->release(q->base32, q->bytes32);
   delete q;
+  MPRGP_CATCH_VOID
 }
 
 // host node-order vector -> device internal order
@@ -568,6 +578,7 @@ static int check_finite(const double *v, size_t n, const char *what) {
 }
 
 extern "C" int mprgp_qp_set_rhs(mprgp_qp *q, const double *b) {
+  MPRGP_TRY
   if (!q || !b) return fail(MPRGP_EBADINPUT, "NULL argument");
   if (int rc = check_finite(b, 3 * (size_t)q->n, "b")) return rc;
   MPRGP_NOT_BUSY(q);
@@ -576,9 +587,11 @@ extern "C" int mprgp_qp_set_rhs(mprgp_qp *q, const double *b) {
   int rc = upload_vec(q, b, q->b);
   if (!rc) q->has_rhs = true;
   return rc;
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_qp_set_x(mprgp_qp *q, const double *x) {
+  MPRGP_TRY
   if (!q || !x) return fail(MPRGP_EBADINPUT, "NULL argument");
   if (int rc = check_finite(x, 3 * (size_t)q->n, "x")) return rc;
   MPRGP_NOT_BUSY(q);
@@ -590,36 +603,46 @@ extern "C" int mprgp_qp_set_x(mprgp_qp *q, const double *x) {
   MPRGP_HIP(hipStreamSynchronize(q->ctx->stream));
   q->has_x = true;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_qp_reset_x(mprgp_qp *q) {
+  MPRGP_TRY
   if (!q) return fail(MPRGP_EBADINPUT, "NULL argument");
   if (!q->has_x) return fail(MPRGP_ESTATE, "x was never set");
   MPRGP_NOT_BUSY(q);
   MPRGP_HIP(hipSetDevice(q->ctx->device));
   std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
   MPRGP_HIP(hipMemcpyAsync(q->x, q->x0, sizeof(double) * 3 * q->n, hipMemcpyDeviceToDevice, q->ctx->stream));
+  // complete before returning, as set_x: an engine solves on its own stream
+  MPRGP_HIP(hipStreamSynchronize(q->ctx->stream));
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_qp_get_x(mprgp_qp *q, double *x) {
+  MPRGP_TRY
   if (!q || !x) return fail(MPRGP_EBADINPUT, "NULL argument");
   MPRGP_NOT_BUSY(q);
   MPRGP_HIP(hipSetDevice(q->ctx->device));
   std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
   return download_vec(q, q->x, x);
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_qp_get_rhs(mprgp_qp *q, double *b) {
+  MPRGP_TRY
   if (!q || !b) return fail(MPRGP_EBADINPUT, "NULL argument");
   if (!q->has_rhs) return fail(MPRGP_ESTATE, "rhs was never set");
   MPRGP_NOT_BUSY(q);
   MPRGP_HIP(hipSetDevice(q->ctx->device));
   std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
   return download_vec(q, q->b, b);
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_qp_set_ev(mprgp_qp *q, const double *ev) {
+  MPRGP_TRY
   if (!q) return fail(MPRGP_EBADINPUT, "NULL argument");
   if (!ev) {
     q->ev_valid = 0;
@@ -632,9 +655,11 @@ extern "C" int mprgp_qp_set_ev(mprgp_qp *q, const double *ev) {
   int rc = upload_vec(q, ev, q->ev);
   if (!rc) q->ev_valid = 1;
   return rc;
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_qp_get_ev(mprgp_qp *q, double *ev, int *valid) {
+  MPRGP_TRY
   if (!q) return fail(MPRGP_EBADINPUT, "NULL argument");
   if (valid) *valid = q->ev_valid;
   if (!ev || !q->ev_valid) return MPRGP_OK;
@@ -642,15 +667,19 @@ extern "C" int mprgp_qp_get_ev(mprgp_qp *q, double *ev, int *valid) {
   MPRGP_HIP(hipSetDevice(q->ctx->device));
   std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
   return download_vec(q, q->ev, ev);
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_qp_op_counts(mprgp_qp *q, int *counts, int n) {
+  MPRGP_TRY
   if (!q || !counts || n <= 0) return fail(MPRGP_EBADINPUT, "bad arguments");
   for (int i = 0; i < n; ++i) counts[i] = i < OP_NUM ? q->opcnt[i] : 0;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_qp_get_face(mprgp_qp *q, unsigned char *face) {
+  MPRGP_TRY
   if (!q || !face) return fail(MPRGP_EBADINPUT, "NULL argument");
   MPRGP_NOT_BUSY(q);
   MPRGP_HIP(hipSetDevice(q->ctx->device));
@@ -660,6 +689,7 @@ extern "C" int mprgp_qp_get_face(mprgp_qp *q, unsigned char *face) {
   MPRGP_HIP(hipStreamSynchronize(q->ctx->stream));
   for (int i = 0; i < q->n; ++i) face[q->A->perm[i]] = f[i];
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 static int upload_rows(mprgp_qp *q, int m, const std::vector<int> &node, const std::vector<double> &nrm,
@@ -697,6 +727,7 @@ static int upload_rows(mprgp_qp *q, int m, const std::vector<int> &node, const s
 }
 
 extern "C" int mprgp_qp_set_rows(mprgp_qp *q, int m, const int *node, const double *normal, const double *c) {
+  MPRGP_TRY
   if (!q || m < 0 || (m > 0 && (!node || !normal || !c))) return fail(MPRGP_EBADINPUT, "bad row arguments");
   if (m > q->n) return fail(MPRGP_EBADINPUT, "m = %d exceeds node count %d", m, q->n);
   MPRGP_NOT_BUSY(q);
@@ -705,10 +736,12 @@ extern "C" int mprgp_qp_set_rows(mprgp_qp *q, int m, const int *node, const doub
   std::vector<int> nd(node, node + m);
   std::vector<double> nr(normal, normal + 3 * (size_t)m), cv(c, c + m);
   return upload_rows(q, m, nd, nr, cv);
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_qp_set_rows_sparse(mprgp_qp *q, int m, int col_major, const int *outer, const int *inner,
                                         const double *val, const double *c) {
+  MPRGP_TRY
   if (!q || m < 0 || !outer || (m > 0 && (!inner || !val || !c)))
     return fail(MPRGP_EBADINPUT, "bad sparse row arguments");
   const int ncol = 3 * q->n;
@@ -741,9 +774,11 @@ extern "C" int mprgp_qp_set_rows_sparse(mprgp_qp *q, int m, int col_major, const
   MPRGP_HIP(hipSetDevice(q->ctx->device));
   std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
   return upload_rows(q, m, node, nrm, cv);
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_qp_collide_planes(mprgp_qp *q, int n_planes, const double *planes, int *m_out) {
+  MPRGP_TRY
   if (!q || n_planes < 0 || (n_planes > 0 && !planes)) return fail(MPRGP_EBADINPUT, "bad plane arguments");
   for (int k = 0; k < n_planes; ++k) {
     const double *pl = planes + 5 * (size_t)k;
@@ -775,6 +810,7 @@ extern "C" int mprgp_qp_collide_planes(mprgp_qp *q, int n_planes, const double *
   q->m = m;
   if (m_out) *m_out = m;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 // ------------------------------------------- projector and preconditioner
@@ -830,36 +866,47 @@ static int projector_call(mprgp_qp *q, int op, const double *a, const double *b,
 }
 
 extern "C" int mprgp_qp_project(mprgp_qp *q, const double *x, double *y) {
+  MPRGP_TRY
   if (!y) return fail(MPRGP_EBADINPUT, "NULL argument");
   return projector_call(q, PJ_PROJECT, x, nullptr, y, 0.0, nullptr);
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_qp_is_feasible(mprgp_qp *q, const double *x, int *feasible) {
+  MPRGP_TRY
   if (!feasible) return fail(MPRGP_EBADINPUT, "NULL argument");
   double violated = 0.0;
   if (int rc = projector_call(q, PJ_FEASIBLE, x, nullptr, nullptr, 0.0, &violated)) return rc;
   *feasible = violated == 0.0;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_qp_decide_face(mprgp_qp *q, const double *x, int *active) {
+  MPRGP_TRY
   double count = 0.0;
   if (int rc = projector_call(q, PJ_FACE, x, nullptr, nullptr, 0.0, &count)) return rc;
   if (active) *active = (int)count;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_qp_phi(mprgp_qp *q, const double *g, double *phi) {
+  MPRGP_TRY
   if (!phi) return fail(MPRGP_EBADINPUT, "NULL argument");
   return projector_call(q, PJ_PHI, g, nullptr, phi, 0.0, nullptr);
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_qp_beta(mprgp_qp *q, const double *g, const double *phi, double *beta) {
+  MPRGP_TRY
   if (!phi || !beta) return fail(MPRGP_EBADINPUT, "NULL argument");
   return projector_call(q, PJ_BETA, g, phi, beta, 0.0, nullptr);
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_qp_phitphi(mprgp_qp *q, const double *x, double alpha_bar, const double *phi, double *out) {
+  MPRGP_TRY
   if (!phi || !out) return fail(MPRGP_EBADINPUT, "NULL argument");
   if (!(alpha_bar > kEps)) return fail(MPRGP_EBADINPUT, "alpha_bar must exceed 1e-9");  // :387 assertion
   double s = 0.0;
@@ -867,24 +914,30 @@ extern "C" int mprgp_qp_phitphi(mprgp_qp *q, const double *x, double alpha_bar, 
   const double r = s * (1.0 / alpha_bar);
   *out = r > 0.0 ? r : 0.0;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_qp_step_limit(mprgp_qp *q, const double *x, const double *d, double alpha_cg,
                                    double *alpha_f) {
+  MPRGP_TRY
   if (!d || !alpha_f) return fail(MPRGP_EBADINPUT, "NULL argument");
   double lim = kDblMax;
   if (int rc = projector_call(q, PJ_STEP_LIMIT, x, d, nullptr, 0.0, &lim)) return rc;
   double af = (alpha_cg == kDblMax) ? alpha_cg : (alpha_cg + kEps);
   *alpha_f = lim < af ? lim : af;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_qp_precond(mprgp_qp *q, const double *g, const double *phi, double *z) {
+  MPRGP_TRY
   if (!phi || !z) return fail(MPRGP_EBADINPUT, "NULL argument");
   return projector_call(q, PJ_PRECOND, g, phi, z, q && q->m > 0 ? 1.0 : 0.0, nullptr);
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_qp_get_rows(mprgp_qp *q, int cap, int *m, int *node, double *normal, double *c) {
+  MPRGP_TRY
   if (!q || !m || cap < 0) return fail(MPRGP_EBADINPUT, "bad arguments");
   *m = q->m;
   if (!node && !normal && !c) return MPRGP_OK;  // count query
@@ -915,6 +968,7 @@ extern "C" int mprgp_qp_get_rows(mprgp_qp *q, int cap, int *m, int *node, double
   }
   if (i != q->m) return fail(MPRGP_ESTATE, "row storage holds %d rows, expected %d", i, q->m);
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 // ----------------------------------------------------------- batch solve
@@ -958,17 +1012,28 @@ void BatchTables::destroy() {
 int BatchTables::ensure_states(int n_qps) {
   if (n_qps <= states_cap) return MPRGP_OK;
   const int cap = std::max(n_qps, 2 * states_cap);
-  ctx->release(d_states, 2 * sizeof(QpState) * (size_t)states_cap);
-  ctx->release(d_stptr, sizeof(QpState *) * (size_t)states_cap);
-  if (h_states) (void)hipHostFree(h_states);
-  if (h_stptr) (void)hipHostFree(h_stptr);
-  d_states = h_states = nullptr;
-  d_stptr = h_stptr = nullptr;
-  states_cap = 0;
-  if (int rc = ctx->alloc((void **)&d_states, 2 * sizeof(QpState) * (size_t)cap)) return rc;
-  if (int rc = ctx->alloc((void **)&d_stptr, sizeof(QpState *) * (size_t)cap)) return rc;
-  MPRGP_HIP(hipHostMalloc((void **)&h_states, 2 * sizeof(QpState) * (size_t)cap, hipHostMallocDefault));
-  MPRGP_HIP(hipHostMalloc((void **)&h_stptr, sizeof(QpState *) * (size_t)cap, hipHostMallocDefault));
+  // the four buffers always have states_cap entries (or are all null), so a
+  // failure part-way leaves nothing allocated and nothing counted
+  auto drop = [&](int n) {
+    ctx->release(d_states, 2 * sizeof(QpState) * (size_t)n);
+    ctx->release(d_stptr, sizeof(QpState *) * (size_t)n);
+    if (h_states) (void)hipHostFree(h_states);
+    if (h_stptr) (void)hipHostFree(h_stptr);
+    d_states = h_states = nullptr;
+    d_stptr = h_stptr = nullptr;
+    states_cap = 0;
+  };
+  drop(states_cap);
+  int rc = ctx->alloc((void **)&d_states, 2 * sizeof(QpState) * (size_t)cap);
+  if (!rc) rc = ctx->alloc((void **)&d_stptr, sizeof(QpState *) * (size_t)cap);
+  hipError_t e = hipSuccess;
+  if (!rc) e = hipHostMalloc((void **)&h_states, 2 * sizeof(QpState) * (size_t)cap, hipHostMallocDefault);
+  if (!rc && e == hipSuccess) e = hipHostMalloc((void **)&h_stptr, sizeof(QpState *) * (size_t)cap, hipHostMallocDefault);
+  if (!rc && e != hipSuccess) rc = fail(MPRGP_ENOMEM, "pinned staging of %d states: %s", cap, hipGetErrorString(e));
+  if (rc) {
+    drop(cap);
+    return rc;
+  }
   states_cap = cap;
   return MPRGP_OK;
 }
@@ -1148,7 +1213,7 @@ int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, i
   const bool mixed = p.precision == MPRGP_MIXED;
   if (mixed) {
     if (int rc = ensure_matrix32(q->A, s)) return rc;
-    if (int rc = ensure_qp32(q)) return rc;
+    if (int rc = ensure_qp32(q, s)) return rc;
     d.f.val = q->A->d_val32;
     d.f.dinv = q->A->d_dinv;
     d.f.x = q->fx; d.f.b = q->fb; d.f.g = q->fg; d.f.p = q->fp; d.f.z = q->fz; d.f.phi = q->fphi;
@@ -1242,11 +1307,15 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
   if (!ctx || count <= 0 || !qps || !params) return fail(MPRGP_EBADINPUT, "bad batch arguments");
   MPRGP_HIP(hipSetDevice(ctx->device));
   std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
-  std::lock_guard<std::recursive_mutex> lock(ctx->mu);
-  for (int i = 0; i < count; ++i) {
+  for (int i = 0; i < count; ++i)
     if (int rc = check_batch_qp(ctx, qps[i], params[stride ? i : 0], i)) return rc;
-    for (int j = 0; j < i; ++j)
-      if (qps[j] == qps[i]) return fail(MPRGP_EBADINPUT, "qp %d appears twice in the batch", i);
+  {  // a QP may appear once: sort the handles and compare neighbours
+    std::vector<std::pair<const mprgp_qp *, int>> h(count);
+    for (int i = 0; i < count; ++i) h[i] = {qps[i], i};
+    std::sort(h.begin(), h.end());
+    for (int i = 1; i < count; ++i)
+      if (h[i].first == h[i - 1].first)
+        return fail(MPRGP_EBADINPUT, "qp %d appears twice in the batch", std::max(h[i].second, h[i - 1].second));
   }
   bool any_mixed = false;
   long total_slices = 0, max_steps = 0;
@@ -1402,6 +1471,7 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
 }
 
 extern "C" int mprgp_qp_solve(mprgp_qp *q, const mprgp_params *params, mprgp_stats *stats) {
+  MPRGP_TRY
   if (!q) return fail(MPRGP_EBADINPUT, "NULL qp");
   mprgp_params def;
   if (!params) {
@@ -1409,10 +1479,12 @@ extern "C" int mprgp_qp_solve(mprgp_qp *q, const mprgp_params *params, mprgp_sta
     params = &def;
   }
   return run_batch(q->ctx, 1, &q, params, 0, stats);
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_solve_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprgp_params *params,
                                  int params_stride, mprgp_stats *stats) {
+  MPRGP_TRY
   mprgp_params def;
   if (!params) {
     mprgp_params_default(&def);
@@ -1420,9 +1492,11 @@ extern "C" int mprgp_solve_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps
     params_stride = 0;
   }
   return run_batch(ctx, count, qps, params, params_stride ? 1 : 0, stats);
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_copy_probe(mprgp_ctx *ctx, size_t bytes, int iters, double *gbps) {
+  MPRGP_TRY
   if (!ctx || !gbps || iters <= 0 || bytes < 4096) return fail(MPRGP_EBADINPUT, "bad probe arguments");
   bytes &= ~(size_t)15;
   MPRGP_HIP(hipSetDevice(ctx->device));
@@ -1448,6 +1522,7 @@ extern "C" int mprgp_copy_probe(mprgp_ctx *ctx, size_t bytes, int iters, double 
   ctx->release(a, bytes);
   ctx->release(b, bytes);
   return rc;
+  MPRGP_CATCH
 }
 
 // The one-shot drop-in on a context of its own; J by rows (CSR) or by
@@ -1492,6 +1567,25 @@ extern "C" int mprgp_decoupled_solve(int device, int n, const int *a_outer, cons
                                      const double *a_val, const double *B, int m, const int *j_outer,
                                      const int *j_inner, const double *j_val, const double *c, double *x,
                                      double tol, int max_it, double *ev, int *ev_valid, mprgp_stats *stats) {
+  MPRGP_TRY
   return mprgp::decoupled_solve(device, n, a_outer, a_inner, a_val, B, m, /*j_col_major*/ 1, j_outer, j_inner,
                                 j_val, c, x, tol, max_it, ev, ev_valid, stats);
+  MPRGP_CATCH
+}
+
+// MPRGPDecoupledCon::solve(file_name, x, ...) (MPRGPSolver.h:576-589): the
+// file's A, B, J, c and x0, then the one-shot drop-in from x0.
+extern "C" int mprgp_decoupled_solve_file(int device, const char *file_name, double *x, int n_x, int *n_out,
+                                          double tol, int max_it, double *ev, int *ev_valid,
+                                          mprgp_stats *stats) {
+  MPRGP_TRY
+  QpCsr q;
+  if (int rc = read_qp_csr(file_name, q)) return rc;
+  if (n_out) *n_out = q.n;
+  if (!x || n_x < q.n) return fail(MPRGP_EBADINPUT, "x holds %d values, the QP has %d", n_x, q.n);
+  std::copy(q.x0.begin(), q.x0.end(), x);
+  return mprgp::decoupled_solve(device, q.n, q.ap.data(), q.ac.data(), q.av.data(), q.B.data(), q.m,
+                                /*j_col_major*/ 0, q.jp.data(), q.jc.data(), q.jv.data(), q.c.data(), x, tol,
+                                max_it, ev, ev_valid, stats);
+  MPRGP_CATCH
 }

@@ -108,6 +108,7 @@ struct alignas(128) QpState {
   double pg, fx_entry;      // p.g of the CG step; f(x) at CG entry (mixed inner)
   double zap;               // z.AP after the in-place gradient update (mixed inner)
   double res_next;          // KKT residual at the CG trial point (mixed inner)
+  double fxu;               // f(x) of the iterate, unprojected, from the step sums (mixed inner)
   int opcnt[OP_NUM];  // launches this QP spent in each op
 };
 // The launch prologue moves a state copy as one 8-byte word per lane.

@@ -274,7 +274,12 @@ void mprgp_engine::run() {
       }
     }
     bool idle = false;
-    int rc = round_once(r, idle);
+    int rc;
+    try {
+      rc = round_once(r, idle);
+    } catch (...) {  // e.g. std::bad_alloc: the worker fails its tickets instead of terminating
+      rc = on_exception();
+    }
     if (rc) {
       // fail every pending ticket with the error; the engine stops accepting work
       std::lock_guard<std::mutex> lk(mu);
@@ -303,14 +308,17 @@ void mprgp_engine::run() {
 }
 
 extern "C" void mprgp_engine_config_default(mprgp_engine_config *cfg) {
+  MPRGP_TRY
   if (!cfg) return;
   std::memset(cfg, 0, sizeof(*cfg));
   cfg->max_active = 256;
   cfg->tiles = 4;
   cfg->graph_steps = 8;
+  MPRGP_CATCH_VOID
 }
 
 extern "C" int mprgp_engine_create(mprgp_ctx *ctx, const mprgp_engine_config *cfg, mprgp_engine **out) {
+  MPRGP_TRY
   if (!ctx || !out) return fail(MPRGP_EBADINPUT, "NULL argument");
   *out = nullptr;
   mprgp_engine_config c;
@@ -359,9 +367,11 @@ extern "C" int mprgp_engine_create(mprgp_ctx *ctx, const mprgp_engine_config *cf
   e->worker = std::thread([e] { e->run(); });
   *out = e;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_engine_submit(mprgp_engine *e, mprgp_qp *q, const mprgp_params *params, long long *ticket) {
+  MPRGP_TRY
   if (!e || !q || !ticket) return fail(MPRGP_EBADINPUT, "NULL argument");
   *ticket = 0;
   mprgp_params def;
@@ -399,6 +409,7 @@ extern "C" int mprgp_engine_submit(mprgp_engine *e, mprgp_qp *q, const mprgp_par
   }
   e->cv_work.notify_one();
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 // Hands a completed request's result to the caller and forgets it (engine
@@ -416,6 +427,7 @@ static int collect(mprgp_engine *e, Request *rq, mprgp_stats *stats) {
 }
 
 extern "C" int mprgp_engine_wait(mprgp_engine *e, long long ticket, int timeout_ms, mprgp_stats *stats) {
+  MPRGP_TRY
   if (!e) return fail(MPRGP_EBADINPUT, "NULL engine");
   std::unique_lock<std::mutex> lk(e->mu);
   auto it = e->reqs.find(ticket);
@@ -427,9 +439,11 @@ extern "C" int mprgp_engine_wait(mprgp_engine *e, long long ticket, int timeout_
     return MPRGP_EPENDING;
   }
   return collect(e, rq, stats);
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_engine_wait_any(mprgp_engine *e, int timeout_ms, long long *ticket, mprgp_stats *stats) {
+  MPRGP_TRY
   if (!e || !ticket) return fail(MPRGP_EBADINPUT, "NULL argument");
   *ticket = 0;
   std::unique_lock<std::mutex> lk(e->mu);
@@ -443,9 +457,11 @@ extern "C" int mprgp_engine_wait_any(mprgp_engine *e, int timeout_ms, long long 
   auto it = e->reqs.find(e->done_q.front());
   *ticket = it->first;
   return collect(e, it->second, stats);
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_engine_get_info(mprgp_engine *e, mprgp_engine_info *info) {
+  MPRGP_TRY
   if (!e || !info) return fail(MPRGP_EBADINPUT, "NULL argument");
   std::lock_guard<std::mutex> lk(e->mu);
   std::memset(info, 0, sizeof(*info));
@@ -459,9 +475,11 @@ extern "C" int mprgp_engine_get_info(mprgp_engine *e, mprgp_engine_info *info) {
   info->active_bytes = e->active_bytes;
   info->peak_bytes = e->peak_bytes;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" void mprgp_engine_destroy(mprgp_engine *e) {
+  MPRGP_TRY
   if (!e) return;
   {
     std::lock_guard<std::mutex> lk(e->mu);
@@ -486,4 +504,5 @@ extern "C" void mprgp_engine_destroy(mprgp_engine *e) {
   }
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
+  MPRGP_CATCH_VOID
 }

@@ -414,6 +414,7 @@ T *carve(char *&p, size_t count) {
 
 extern "C" int mprgp_fem_create(mprgp_ctx *ctx, int N, int T, const int *tets_in, const double *X0,
                                 const double *young, const double *poisson, double density, mprgp_fem **out) {
+  MPRGP_TRY
   if (!ctx || !out || N <= 0 || T <= 0 || !tets_in || !X0 || !young || !poisson)
     return fail(MPRGP_EBADINPUT, "bad FEM arguments");
   if (!(density > 0.0) || !std::isfinite(density)) return fail(MPRGP_EBADINPUT, "density must be positive");
@@ -558,6 +559,7 @@ extern "C" int mprgp_fem_create(mprgp_ctx *ctx, int N, int T, const int *tets_in
   }
   *out = f;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" mprgp_matrix *mprgp_fem_matrix(mprgp_fem *f) { return f ? f->A : nullptr; }
@@ -634,6 +636,7 @@ int assemble_dev(mprgp_fem *f, int material, const double *xn, const double *vn,
 
 extern "C" int mprgp_fem_assemble_at(mprgp_fem *f, int material, const double *x_n, const double *v_n,
                                      const double *x1, const double *gravity, double h, mprgp_qp *qp) {
+  MPRGP_TRY
   if (!f || !x_n || !v_n || !gravity || !qp) return fail(MPRGP_EBADINPUT, "NULL argument");
   if (int rc = check_material(f, material)) return rc;
   if (qp->A != f->A) return fail(MPRGP_EBADINPUT, "qp was not created on this FEM's matrix");
@@ -657,25 +660,31 @@ extern "C" int mprgp_fem_assemble_at(mprgp_fem *f, int material, const double *x
   MPRGP_HIP(hipMemcpyAsync(stage + n, v_n, sizeof(double) * n, hipMemcpyHostToDevice, s));
   if (x1) MPRGP_HIP(hipMemcpyAsync(stage + 2 * n, x1, sizeof(double) * n, hipMemcpyHostToDevice, s));
   return assemble_dev(f, material, stage, stage + n, x1 ? stage + 2 * n : stage, gravity, h, qp);
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_fem_assemble(mprgp_fem *f, int material, const double *x_n, const double *v_n,
                                   const double *gravity, double h, mprgp_qp *qp) {
+  MPRGP_TRY
   return mprgp_fem_assemble_at(f, material, x_n, v_n, nullptr, gravity, h, qp);
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_fem_set_damping(mprgp_fem *f, double alpha_k, double alpha_m) {
+  MPRGP_TRY
   if (!f) return fail(MPRGP_EBADINPUT, "NULL argument");
   if (!(alpha_k >= 0.0) || !(alpha_m >= 0.0) || !std::isfinite(alpha_k) || !std::isfinite(alpha_m))
     return fail(MPRGP_EBADINPUT, "damping must be finite and >= 0");  // Simulator.cpp:162-163
   f->alpha_k = alpha_k;
   f->alpha_m = alpha_m;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 // ---------------------------------------------------------------- time step
 
 extern "C" int mprgp_fem_set_state(mprgp_fem *f, const double *x, const double *v) {
+  MPRGP_TRY
   if (!f || !x || !v) return fail(MPRGP_EBADINPUT, "NULL argument");
   const size_t n = 3 * (size_t)f->N;
   if (int rc = check_finite(x, n, "x")) return rc;
@@ -697,9 +706,11 @@ extern "C" int mprgp_fem_set_state(mprgp_fem *f, const double *x, const double *
   f->has_state = true;
   f->step_open = false;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_fem_get_state(mprgp_fem *f, double *x, double *v) {
+  MPRGP_TRY
   if (!f) return fail(MPRGP_EBADINPUT, "NULL argument");
   if (!f->has_state) return fail(MPRGP_ESTATE, "no state was set");
   mprgp_ctx *ctx = f->ctx;
@@ -711,6 +722,7 @@ extern "C" int mprgp_fem_get_state(mprgp_fem *f, double *x, double *v) {
   if (v) MPRGP_HIP(hipMemcpyAsync(v, f->vn, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   MPRGP_HIP(hipStreamSynchronize(s));
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 namespace {
@@ -728,6 +740,7 @@ int check_step(mprgp_fem *f, mprgp_qp *qp, const mprgp_step_params *sp) {
 }  // namespace
 
 extern "C" int mprgp_fem_begin_step(mprgp_fem *f, mprgp_qp *qp, const mprgp_step_params *sp, int *rows) {
+  MPRGP_TRY
   if (int rc = check_step(f, qp, sp)) return rc;
   MPRGP_NOT_BUSY(qp);
   mprgp_ctx *ctx = f->ctx;
@@ -749,9 +762,11 @@ extern "C" int mprgp_fem_begin_step(mprgp_fem *f, mprgp_qp *qp, const mprgp_step
   if (int rc = assemble_dev(f, sp->material, f->xn, f->vn, f->x1, sp->gravity, sp->h, qp)) return rc;
   f->step_open = true;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_fem_end_step(mprgp_fem *f, mprgp_qp *qp, const mprgp_step_params *sp, double *delta) {
+  MPRGP_TRY
   if (int rc = check_step(f, qp, sp)) return rc;
   if (!f->step_open) return fail(MPRGP_ESTATE, "no step was begun");
   MPRGP_NOT_BUSY(qp);
@@ -775,10 +790,12 @@ extern "C" int mprgp_fem_end_step(mprgp_fem *f, mprgp_qp *qp, const mprgp_step_p
   MPRGP_HIP(hipStreamSynchronize(s));
   f->step_open = false;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_fem_advance(mprgp_fem *f, mprgp_qp *qp, const mprgp_step_params *sp, const mprgp_params *p,
                                  mprgp_step_stats *out) {
+  MPRGP_TRY
   if (int rc = check_step(f, qp, sp)) return rc;
   if (!p) return fail(MPRGP_EBADINPUT, "NULL params");
   if (sp->newton_max_it < 1) return fail(MPRGP_EBADINPUT, "newton_max_it must be at least 1");
@@ -825,9 +842,11 @@ extern "C" int mprgp_fem_advance(mprgp_fem *f, mprgp_qp *qp, const mprgp_step_pa
   st.code = worst;
   if (out) *out = st;
   return worst;
+  MPRGP_CATCH
 }
 
 extern "C" void mprgp_fem_destroy(mprgp_fem *f) {
+  MPRGP_TRY
   if (!f) return;
   mprgp_ctx *ctx = f->ctx;
   (void)hipSetDevice(ctx->device);
@@ -837,4 +856,5 @@ extern "C" void mprgp_fem_destroy(mprgp_fem *f) {
   if (f->state) ctx->release(f->state, sizeof(double) * 4 * 3 * (size_t)f->N);
   mprgp_matrix_destroy(f->A);
   delete f;
+  MPRGP_CATCH_VOID
 }

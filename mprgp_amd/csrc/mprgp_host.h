@@ -15,15 +15,15 @@
 
 #include "../../include/mprgp_amd.h"
 #include "mprgp_dev.h"
+#include "mprgp_error.h"
 
 struct mprgp_ctx;
 struct mprgp_matrix;
 
 namespace mprgp {
 
-// last-error plumbing
-void set_error(const char *fmt, ...);
-int fail(int code, const char *fmt, ...);
+// the one-shot drop-in behind mprgp_decoupled_solve and the file overload;
+// J by rows (j_col_major = 0) or by columns (1)
 int decoupled_solve(int device, int n, const int *a_outer, const int *a_inner, const double *a_val,
                     const double *B, int m, int j_col_major, const int *j_outer, const int *j_inner,
                     const double *j_val, const double *c, double *x, double tol, int max_it, double *ev,

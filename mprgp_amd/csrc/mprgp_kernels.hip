@@ -456,8 +456,12 @@ __device__ void control(QpState &st, const Sums &S) {
       st.op = st.cg_entry ? OP_CG_Y : OP_CG_YA;
       break;
     case OP_CG_PYG: {
-      // mixed inner solve: f(P(y)) = f(x) - alpha p'g + alpha^2 p'Ap/2 + the
-      // projected rows' share, all from the step's own sums
+      // mixed inner solve: f(y) = f(x) - alpha p'g + alpha^2 p'Ap/2 and
+      // f(P(y)) = f(y) + the projected rows' share, all from the step's own
+      // sums.  f(x) is the objective at the iterate itself: at the CG entry
+      // f(x) = x'(g - b)/2; after an accepted step x = y, unprojected
+      // (MPRGPSolver.h:293), so f(x) is that step's f(y), not the f(P(y))
+      // the acceptance test compares (fx = fy, :307).
       st.y_valid = 1;
       st.y_feas = S.v[0] == 0.0;
       st.pg = S.v[1];
@@ -465,8 +469,9 @@ __device__ void control(QpState &st, const Sums &S) {
       st.zap = S.v[3];
       st.res_next = sqrt(S.v[4]);
       st.prop_next = proportional(st, S.v[5], S.v[6]);
-      const double f0 = st.cg_entry ? st.fx_entry : st.fy;
-      cg_decide(st, f0 - st.alpha * st.pg + 0.5 * st.alpha * st.alpha * st.pd + S.v[7], st.fx_entry, true);
+      const double f0 = st.cg_entry ? st.fx_entry : st.fxu;
+      st.fxu = f0 - st.alpha * st.pg + 0.5 * st.alpha * st.alpha * st.pd;  // f(y): f(x) once accepted
+      cg_decide(st, st.fxu + S.v[7], st.fx_entry, true);
       break;
     }
     case OP_CG_Y:

@@ -16,8 +16,9 @@
 #include <memory>
 #include <vector>
 
-#include "mprgp_amd.h"
-#include "mprgp_host.h"
+#include "../../include/mprgp_amd.h"
+#include "mprgp_error.h"
+#include "mprgp_qpfile.h"
 
 namespace {
 
@@ -41,13 +42,24 @@ struct FileCloser {
   void operator()(FILE *f) const { if (f) std::fclose(f); }
 };
 
+// Bytes from the current position to the end of the file (or 0 when the
+// stream cannot seek): every count a header states is checked against it
+// before anything is sized by that count.
+size_t bytes_left(FILE *f) {
+  const long pos = std::ftell(f);
+  if (pos < 0 || std::fseek(f, 0, SEEK_END) != 0) return 0;
+  const long end = std::ftell(f);
+  if (end < pos || std::fseek(f, pos, SEEK_SET) != 0) return 0;
+  return (size_t)(end - pos);
+}
+
 bool read_sparse(FILE *f, Sparse &s) {
   size_t hdr[3];
   if (std::fread(hdr, sizeof(size_t), 3, f) != 3) return false;
   s.rows = hdr[0];
   s.cols = hdr[1];
   const size_t nnz = hdr[2];
-  if (s.rows > 0x7fffffff || s.cols > 0x7fffffff || nnz > ((size_t)1 << 40)) return false;
+  if (s.rows > 0x7fffffff || s.cols > 0x7fffffff || nnz > bytes_left(f) / sizeof(Triplet)) return false;
   s.t.resize(nnz);
   if (nnz && std::fread(s.t.data(), sizeof(Triplet), nnz, f) != nnz) return false;
   for (const Triplet &e : s.t)
@@ -56,6 +68,7 @@ bool read_sparse(FILE *f, Sparse &s) {
 }
 
 bool read_doubles(FILE *f, std::vector<double> &v, size_t n) {
+  if (n > bytes_left(f) / sizeof(double)) return false;
   v.resize(n);
   return n == 0 || std::fread(v.data(), sizeof(double), n, f) == n;
 }
@@ -114,6 +127,7 @@ bool write_doubles(FILE *f, const double *v, size_t n) {
 }  // namespace
 
 extern "C" int mprgp_qp_file_info(const char *file_name, int *n, int *m, long long *a_nnz, long long *j_nnz) {
+  MPRGP_TRY
   if (!n || !m || !a_nnz || !j_nnz) return mprgp::fail(MPRGP_EBADINPUT, "NULL argument");
   QpFile q;
   if (int rc = load(file_name, q)) return rc;
@@ -126,10 +140,12 @@ extern "C" int mprgp_qp_file_info(const char *file_name, int *n, int *m, long lo
   *n = (int)q.A.rows;
   *m = (int)q.J.rows;
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_qp_file_read(const char *file_name, int *a_ptr, int *a_col, double *a_val, double *B,
                                   int *j_ptr, int *j_col, double *j_val, double *c, double *x0) {
+  MPRGP_TRY
   QpFile q;
   if (int rc = load(file_name, q)) return rc;
   if (!a_ptr || !a_col || !a_val || !B || !j_ptr || !x0 || (q.J.rows && !c) ||
@@ -149,11 +165,13 @@ extern "C" int mprgp_qp_file_read(const char *file_name, int *a_ptr, int *a_col,
   std::copy(q.c.begin(), q.c.end(), c);
   std::copy(q.x0.begin(), q.x0.end(), x0);
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
 extern "C" int mprgp_write_qp(const char *file_name, int n, const int *a_ptr, const int *a_col,
                               const double *a_val, const double *B, int m, const int *j_ptr, const int *j_col,
                               const double *j_val, const double *c, const double *x0) {
+  MPRGP_TRY
   if (!file_name || n < 0 || m < 0 || !a_ptr || !B || !x0 || !j_ptr || (m > 0 && !c))
     return mprgp::fail(MPRGP_EBADINPUT, "bad QP file arguments");
   std::unique_ptr<FILE, FileCloser> f(std::fopen(file_name, "wb"));
@@ -163,22 +181,19 @@ extern "C" int mprgp_write_qp(const char *file_name, int n, const int *a_ptr, co
                   write_doubles(f.get(), x0, n);
   if (!ok) return mprgp::fail(MPRGP_EBADINPUT, "writing QP file %s failed", file_name);
   return MPRGP_OK;
+  MPRGP_CATCH
 }
 
-extern "C" int mprgp_decoupled_solve_file(int device, const char *file_name, double *x, int n_x, int *n_out,
-                                          double tol, int max_it, double *ev, int *ev_valid,
-                                          mprgp_stats *stats) {
+// The file as CSR arrays for the solve-from-file overload.
+int mprgp::read_qp_csr(const char *file_name, QpCsr &out) {
   QpFile q;
   if (int rc = load(file_name, q)) return rc;
-  const int n = (int)q.A.rows;
-  if (n_out) *n_out = n;
-  if (!x || n_x < n) return mprgp::fail(MPRGP_EBADINPUT, "x holds %d values, the QP has %d", n_x, n);
-  std::vector<int> ap, ac, jp, jc;
-  std::vector<double> av, jv;
-  to_csr(q.A, ap, ac, av);
-  to_csr(q.J, jp, jc, jv);
-  std::copy(q.x0.begin(), q.x0.end(), x);
-  return mprgp::decoupled_solve(device, n, ap.data(), ac.data(), av.data(), q.B.data(), (int)q.J.rows,
-                                /*j_col_major*/ 0, jp.data(), jc.data(), jv.data(), q.c.data(), x, tol, max_it,
-                                ev, ev_valid, stats);
+  out.n = (int)q.A.rows;
+  out.m = (int)q.J.rows;
+  to_csr(q.A, out.ap, out.ac, out.av);
+  to_csr(q.J, out.jp, out.jc, out.jv);
+  out.B = std::move(q.B);
+  out.c = std::move(q.c);
+  out.x0 = std::move(q.x0);
+  return MPRGP_OK;
 }

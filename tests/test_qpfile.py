@@ -107,3 +107,21 @@ def test_malformed_files_are_rejected(qp_dir):
         with pytest.raises(M.MprgpError) as e:
             M.read_qp(path)
         assert e.value.code == -2, name  # MPRGP_EBADINPUT
+
+
+def test_oversized_header_is_rejected_not_fatal(qp_dir):
+    """A 24-byte file whose header claims 2^39 triplets: the counts are
+    checked against the bytes left in the file before anything is sized by
+    them, so the reader answers MPRGP_EBADINPUT.  Run in a child process, so a
+    regression (an exception escaping the C ABI) fails this test rather than
+    ending the test run."""
+    import subprocess
+    import sys
+    path = os.path.join(qp_dir, "huge_nnz.qp")
+    with open(path, "wb") as f:
+        f.write(struct.pack("<QQQ", 3, 3, 1 << 39))
+    code = ("import sys, mprgp_amd as M\n"
+            "try:\n    M.qp_file_info(sys.argv[1])\nexcept M.MprgpError as e:\n    print('code', e.code)\n")
+    r = subprocess.run([sys.executable, "-c", code, path], capture_output=True, text=True, cwd=ROOT, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert r.stdout.strip() == "code -2", r.stdout + r.stderr
