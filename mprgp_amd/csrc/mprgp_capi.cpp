@@ -14,13 +14,15 @@
 
 namespace mprgp {
 
-// splitmix64: deterministic start vector of the power method (replaces the
-// reference's unseeded setRandom()).
-static uint64_t splitmix64(uint64_t &s) {
-  uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-  return z ^ (z >> 31);
+// |v| of the power method's start vector (power_start_raw, summed in index
+// order; the device divides each element by it).
+double power_start_norm(size_t n) {
+  double nr = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    const double e = power_start_raw(i);
+    nr += e * e;
+  }
+  return std::sqrt(nr);
 }
 
 }  // namespace mprgp
@@ -1188,13 +1190,10 @@ size_t solve_bytes(const mprgp_qp *q, const mprgp_params &p) {
   return b;
 }
 
-// Fills the device descriptor and the initial state of a QP entering a batch
-// and enqueues its set-up on stream `s` (FP32 copies, power start vector,
-// tangent-Jacobi factors).  The caller uploads `st` into both state copies.
-// A power start vector needs a host upload: it goes through `q->stage` and
-// synchronises `s`.
-int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, int *done_flag, int tag,
-               QpDev &d, QpState &st, bool rows) {
+// The device descriptor and the initial state of a QP entering a batch (host
+// structs only; the mixed form's FP32 storage must exist).
+void describe_qp(const mprgp_qp *q, const mprgp_params &p, int n_tiles, int *done_flag, int tag, QpDev &d,
+                 QpState &st) {
   std::memset(&d, 0, sizeof(d));
   d.n_nodes = q->n;
   d.n_slices = q->A->n_slices;
@@ -1212,8 +1211,6 @@ int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, i
   d.cf = q->cf; d.face = q->face;
   const bool mixed = p.precision == MPRGP_MIXED;
   if (mixed) {
-    if (int rc = ensure_matrix32(q->A, s)) return rc;
-    if (int rc = ensure_qp32(q, s)) return rc;
     d.f.val = q->A->d_val32;
     d.f.dinv = q->A->d_dinv;
     d.f.x = q->fx; d.f.b = q->fb; d.f.g = q->fg; d.f.p = q->fp; d.f.z = q->fz; d.f.phi = q->fphi;
@@ -1243,22 +1240,27 @@ int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, i
     st.max_refine = p.max_refine > 0 ? p.max_refine : 8;
     st.inner_rel = p.inner_tol > 0.0 ? p.inner_tol : 1e-3;
   }
-  if (power) {
+}
+
+// describe_qp, and the QP's set-up enqueued on stream `s` (FP32 copies, power
+// start vector, tangent-Jacobi factors).  The caller uploads `st` into both
+// state copies.
+int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, int *done_flag, int tag,
+               QpDev &d, QpState &st, bool rows) {
+  if (p.precision == MPRGP_MIXED) {
+    if (int rc = ensure_matrix32(q->A, s)) return rc;
+    if (int rc = ensure_qp32(q, s)) return rc;
+  }
+  describe_qp(q, p, n_tiles, done_flag, tag, d, st);
+  if (!(p.alpha_bar > 0.0)) {
     if (p.use_ev && q->ev_valid) {
       MPRGP_HIP(hipMemcpyAsync(q->pv, q->ev, sizeof(double) * 3 * q->n, hipMemcpyDeviceToDevice, s));
     } else {
-      // deterministic start vector in user node order, normalised
-      std::vector<double> v(3 * (size_t)q->n);
-      uint64_t seed = 0x5EEDULL;
-      for (auto &e : v) e = 2.0 * ((double)(splitmix64(seed) >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
-      double nr = 0.0;
-      for (double e : v) nr += e * e;
-      nr = std::sqrt(nr);
-      if (nr > 0.0)
-        for (auto &e : v) e /= nr;
-      MPRGP_HIP(hipMemcpyAsync(q->stage, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, s));
+      // the deterministic start vector in user node order, normalised on the
+      // device, then in internal order
+      const size_t n3 = 3 * (size_t)q->n;
+      MPRGP_HIP(launch_power_start(s, q->stage, n3, power_start_norm(n3)));
       MPRGP_HIP(launch_gather_nodes(s, q->stage, q->pv, q->A->d_perm, q->n));
-      MPRGP_HIP(hipStreamSynchronize(s));
     }
   }
   // tangent-Jacobi factors depend on the current matrix values

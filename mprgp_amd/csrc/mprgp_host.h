@@ -37,7 +37,25 @@ int decoupled_solve(int device, int n, const int *a_outer, const int *a_inner, c
                            __FILE__, __LINE__);                                           \
   } while (0)
 
+// The power method's deterministic start vector, replacing the reference's
+// unseeded setRandom() (MPRGPSolver.h:47-50): element i is 2u - 1 for u the
+// top 53 bits of the i-th splitmix64 output from seed 0x5EED, i.e. of the
+// state 0x5EED + (i + 1) * 0x9E3779B97F4A7C15 mixed; the vector is then
+// divided by its norm (power_start_norm, mprgp_capi.cpp).  Every operation
+// is exact except the final division, which rounds the same on host and
+// device.
+__host__ __device__ inline double power_start_raw(uint64_t i) {
+  uint64_t z = 0x5EEDULL + (i + 1) * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z ^= z >> 31;
+  return 2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+}
+double power_start_norm(size_t n);
+
 // kernel launch wrappers (mprgp_kernels.hip)
+// out[i] = power_start_raw(i) / nr for i < n (out as is when nr = 0)
+hipError_t launch_power_start(hipStream_t s, double *out, size_t n, double nr);
 // variant: a StepVariant (mprgp_dev.h), from step_variant() below
 hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
                        int *running, int parity, bool mixed, int variant = kStepCached);
@@ -121,6 +139,10 @@ inline int step_variant(int n_tiles, double matrix_bytes) {
 void layout_tiles(const std::vector<Tile> &tiles, int grid, Tile *table);
 long solve_step_bound(const mprgp_params &p);
 size_t solve_bytes(const mprgp_qp *q, const mprgp_params &p);
+// the descriptor and initial state only (no device work; a mixed-form QP's
+// FP32 storage must exist)
+void describe_qp(const mprgp_qp *q, const mprgp_params &p, int n_tiles, int *done_flag, int tag, QpDev &d,
+                 QpState &st);
 // rows = false: the caller runs the rows' tangent-Jacobi set-up itself
 // (solve_batch: one launch for the whole batch).
 int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, int *done_flag, int tag,

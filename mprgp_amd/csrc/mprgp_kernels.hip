@@ -25,7 +25,7 @@
 
 #include <algorithm>
 
-#include "mprgp_dev.h"
+#include "mprgp_host.h"
 
 namespace mprgp {
 
@@ -1470,6 +1470,14 @@ __global__ void __launch_bounds__(256) reduce_parts(const double *__restrict__ p
   if (threadIdx.x == 0) out[0] = red[0];
 }
 
+// The power method's start vector (power_start_raw), divided by its norm.
+__global__ void __launch_bounds__(256) power_start(double *__restrict__ out, size_t n, double nr) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double e = power_start_raw(i);
+  out[i] = nr > 0.0 ? e / nr : e;
+}
+
 // FP32 copies for the mixed form.
 __global__ void to_f32(const double *__restrict__ in, float *__restrict__ out, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1611,6 +1619,13 @@ hipError_t launch_projector(hipStream_t s, const ProjArgs &a, double *result) {
   const int blocks = (a.n_nodes + 255) / 256;
   projector_op<<<blocks, 256, 0, s>>>(a);
   if (a.part) reduce_parts<<<1, 256, 0, s>>>(a.part, blocks, a.op == PJ_STEP_LIMIT ? 1 : 0, result);
+  return hipGetLastError();
+}
+hipError_t launch_power_start(hipStream_t s, double *out, size_t n, double nr) {
+  if (n == 0) return hipSuccess;
+  const size_t blocks = (n + 255) / 256;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  power_start<<<(unsigned)blocks, 256, 0, s>>>(out, n, nr);
   return hipGetLastError();
 }
 hipError_t launch_to_f32(hipStream_t s, const double *in, float *out, size_t n) {

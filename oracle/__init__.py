@@ -86,6 +86,8 @@ def lib():
         L.orc_spec_rad.restype = D
         L.orc_decoupled_solve.argtypes = [C.POINTER(_CSR), P, C.POINTER(_CSR), P, P, D, I, I, I,
                                           D, P, P, I, C.POINTER(Stats)]
+        L.orc_decoupled_solve_ex.argtypes = [C.POINTER(_CSR), P, C.POINTER(_CSR), P, P, D, I, I, I,
+                                             D, P, P, I, C.POINTER(_Options), C.POINTER(Stats)]
         L.orc_fem_setup.argtypes = [I, I, P, P, P, P, P]
         L.orc_polar_rotation.argtypes = [P, P]
         L.orc_fem_element.argtypes = [I, P, P, D, D, D, P, P]
@@ -261,6 +263,47 @@ def decoupled_solve(A, B, J, c, x, tol=1e-3, max_it=1000, variant=MONOTONIC,
                                      float(alpha_bar), _ptr(evb), C.byref(valid),
                                      int(project_first), C.byref(st))
     return code, x, st.as_dict(), (evb if valid.value else None)
+
+
+EV_RESIDUAL, EV_PROP, EV_CG_TEST, EV_FEASIBLE, EV_STEP, EV_Y_INFEASIBLE, EV_FY_CLOSED = range(7)
+STEP_CG, STEP_EXP, STEP_PROP = range(3)
+F_DIRECT, F_CLOSED, F_CLOSED_FROM_FPY = range(3)
+EVENT_DTYPE = np.dtype([("iteration", np.int32), ("kind", np.int32), ("outcome", np.int32),
+                        ("pad", np.int32), ("lhs", np.float64), ("rhs", np.float64)])
+
+
+class _Options(C.Structure):
+    _fields_ = [("events", C.c_void_p), ("cap", C.c_int), ("n", C.c_int), ("fmode", C.c_int)]
+
+
+def decoupled_solve_traced(A, B, J, c, x, tol=1e-3, max_it=1000, preconditioned=True, alpha_bar=0.0,
+                           ev=None, project_first=True, fmode=F_DIRECT, cap=1 << 20):
+    """The monotone solve with its decisions recorded (orc_decoupled_solve_ex).
+    Returns (code, x, stats-dict, events) with events a structured array of
+    EVENT_DTYPE in the order the control flow took them."""
+    a = _Csr(A)
+    j = _Csr(J, cols=A.shape[0])
+    B = _f64(B)
+    c = _f64(c)
+    x = _f64(x).copy()
+    n = A.shape[0]
+    evb = np.zeros(n) if ev is None else _f64(ev).copy()
+    valid = C.c_int(0 if ev is None else 1)
+    st = Stats()
+    events = np.zeros(cap, EVENT_DTYPE)
+    opt = _Options(events.ctypes.data, cap, 0, int(fmode))
+    code = lib().orc_decoupled_solve_ex(C.byref(a.s), _ptr(B), C.byref(j.s), _ptr(c), _ptr(x),
+                                        float(tol), int(max_it), MONOTONIC, int(preconditioned),
+                                        float(alpha_bar), _ptr(evb), C.byref(valid),
+                                        int(project_first), C.byref(opt), C.byref(st))
+    if opt.n > cap:
+        raise RuntimeError(f"{opt.n} events, capacity {cap}")
+    return code, x, st.as_dict(), events[:opt.n].copy()
+
+
+def step_sequence(events):
+    """The step types (STEP_*) of a traced solve, in order."""
+    return events["outcome"][events["kind"] == EV_STEP].copy()
 
 
 # ------------------------------------------------------------------ FEM

@@ -26,6 +26,12 @@
  *                                                  (<= 8 nodes), KKT conditions on beam QPs, the m = 0 case
  *                                                  against a dense Cholesky solve, the reference's own
  *                                                  test_solver case (J = I, n = 5) returning code 0
+ *   orc_decoupled_solve_ex  the same solve with its decisions recorded and an optional closed-form
+ *                       f(P(y)) (the device mixed form's): tracing changes no bit of the solve (identical
+ *                       x and counts with and without it), the recorded step counts equal the stats, the
+ *                       closed form equals the direct f(P(y)) to 1e-13 relative along trajectories that
+ *                       accept infeasible trial points, and a planted wrong continuation (from f(P(y)))
+ *                       is caught (tests/test_oracle_trace.py)
  */
 #include "mprgp_oracle.h"
 
@@ -388,7 +394,36 @@ typedef struct {
   int y_size;
   double residual_out;
   int result_code, num_cg, num_exp, numprop, iteration;
+  orc_options *opt;  /* events and the f(P(y)) form (NULL: reference form, no events) */
+  double *t2, *t3;   /* scratch of the closed-form objective */
 } mprgp;
+
+/* Appends one decision to the caller's event list (reads state only). */
+static void record(mprgp *S, int kind, int outcome, double lhs, double rhs) {
+  orc_options *o = S->opt;
+  if (!o) return;
+  if (o->events && o->n < o->cap) {
+    orc_event *e = &o->events[o->n];
+    e->iteration = S->iteration;
+    e->kind = kind;
+    e->outcome = outcome;
+    e->pad = 0;
+    e->lhs = lhs;
+    e->rhs = rhs;
+  }
+  o->n++;
+}
+
+/* min over rows of (J y - c)_i, DBL_MAX without rows (for events only). */
+static double min_slack(mprgp *S, const double *y) {
+  double m = ORC_MAX;
+  j_mul(S->P, y, S->P->lam);
+  for (int i = 0; i < S->P->m; ++i) {
+    double v = S->P->lam[i] - S->P->c[i];
+    m = v < m ? v : m;
+  }
+  return m;
+}
 
 static void mprgp_init(mprgp *S) {
   int n = S->n;
@@ -400,12 +435,14 @@ static void mprgp_init(mprgp *S) {
   S->gp = (double *)xcalloc((size_t)n, sizeof(double));
   S->y = (double *)xcalloc((size_t)n, sizeof(double));
   S->t1 = (double *)xcalloc((size_t)n, sizeof(double));
+  S->t2 = (double *)xcalloc((size_t)n, sizeof(double));
+  S->t3 = (double *)xcalloc((size_t)n, sizeof(double));
   S->y_size = 0;
 }
 
 static void mprgp_free(mprgp *S) {
   free(S->g); free(S->p); free(S->z); free(S->beta); free(S->phi); free(S->gp);
-  free(S->y); free(S->t1);
+  free(S->y); free(S->t1); free(S->t2); free(S->t3);
 }
 
 /* g = A x - B */
@@ -438,6 +475,7 @@ static int proportional(mprgp *S, const double *x) {
   double left = dot(S->n, S->beta, S->beta);
   double phitphi = orc_phitphi(S->P, x, S->alpha_bar, S->phi);
   double right = S->gamma * S->gamma * phitphi;
+  record(S, ORC_EV_PROP, left <= right, left, right);
   return left <= right;
 }
 
@@ -510,6 +548,50 @@ static void exp_monotonic_step(mprgp *S, double *x) {
   memcpy(S->p, S->z, sizeof(double) * (size_t)S->n);
 }
 
+/* f(P(y)) of the trial point y = x - alpha p from the step's sums (the
+ * device mixed form's closed form; see ORC_F_CLOSED): f0 = f(x) at the
+ * iterate, g = A x - B, AP = A p.  *fyu receives f(y) itself. */
+static double closed_fy(mprgp *S, double f0, double alpha, double pd, const double *AP, double *fyu) {
+  int n = S->n;
+  orc_projector *P = S->P;
+  *fyu = f0 - alpha * dot(n, S->p, S->g) + 0.5 * alpha * alpha * pd;
+  /* d = P(y) - y on the rows violated by more than the feasibility tolerance */
+  double *lam = P->lam, *d = S->t2, *ad = S->t3;
+  j_mul(P, S->y, lam);
+  for (int i = 0; i < P->m; ++i) {
+    double v = P->c[i] - lam[i];
+    lam[i] = (v > ORC_EPS) ? v / P->JJt[i] : 0.0;
+  }
+  jt_mul(P, lam, d);
+  orc_spmv(S->A, d, ad);
+  double share = 0.0;
+  for (int i = 0; i < n; ++i) share += d[i] * ((S->g[i] - alpha * AP[i]) + 0.5 * ad[i]);
+  return *fyu + share;
+}
+
+/* Closed forms tracked along the solve's own trajectory (traced solves):
+ * f0 of the correct form continues from f(y) of each accepted step, f0 of
+ * the planted wrong form from its own f(P(y)). */
+typedef struct {
+  double f0_fix, f0_bad;     /* f(x) of the next trial, per form      */
+  double fp_fix, fyu_fix;    /* this trial: f(P(y)) and f(y), correct */
+  double fp_bad;             /* this trial: f(P(y)), planted form      */
+} closed_track;
+
+/* f(P(y)) in the form the options ask for; with options, both closed forms
+ * are evaluated too and recorded (ORC_EV_FY_CLOSED) next to the decision. */
+static double trial_fy(mprgp *S, double alpha, double pd, const double *AP, closed_track *ct) {
+  if (!S->opt) return project_fun_value(S, S->y);
+  double fyu_bad;
+  ct->fp_fix = closed_fy(S, ct->f0_fix, alpha, pd, AP, &ct->fyu_fix);
+  ct->fp_bad = closed_fy(S, ct->f0_bad, alpha, pd, AP, &fyu_bad);
+  switch (S->opt->fmode) {
+    case ORC_F_CLOSED: return ct->fp_fix;
+    case ORC_F_CLOSED_FROM_FPY: return ct->fp_bad;
+    default: return project_fun_value(S, S->y);
+  }
+}
+
 /* MPRGPBase::CGMonotonicStep (MPRGPSolver.h:272-316); AP aliases gp. */
 static double cg_monotonic_step(mprgp *S, double *x, int *result_is_prop) {
   int n = S->n;
@@ -519,11 +601,24 @@ static double cg_monotonic_step(mprgp *S, double *x, int *result_is_prop) {
   double alpha_cg = dot(n, S->z, S->g) / pd;
   for (int i = 0; i < n; ++i) S->y[i] = x[i] - alpha_cg * S->p[i];
   S->y_size = n;
-  double fy = project_fun_value(S, S->y);
   double fx = orc_func_value(S->A, x, S->B);
+  closed_track ct = {fx, fx, 0.0, 0.0, 0.0};
+  double fy = trial_fy(S, alpha_cg, pd, AP, &ct);
   *result_is_prop = 1;
-  while (*result_is_prop && S->residual_out > S->tol && fy <= fx &&
-         S->iteration < S->max_iterations) {
+  for (;;) {
+    if (!(*result_is_prop && S->residual_out > S->tol)) break;
+    if (S->opt) {
+      const double ms = min_slack(S, S->y);
+      record(S, ORC_EV_CG_TEST, fy <= fx, fy, fx);
+      record(S, ORC_EV_FY_CLOSED, 0, ct.fp_fix, ct.fp_bad);
+      if (ms < -ORC_EPS) record(S, ORC_EV_Y_INFEASIBLE, 1, ms, -ORC_EPS);
+    }
+    if (!(fy <= fx && S->iteration < S->max_iterations)) break;
+    record(S, ORC_EV_STEP, ORC_STEP_CG, 0.0, 0.0);
+    /* the iterate becomes y, unprojected: the correct closed form continues
+     * from f(y), the planted one from its f(P(y)) */
+    ct.f0_fix = ct.fyu_fix;
+    ct.f0_bad = ct.fp_bad;
     memcpy(x, S->y, sizeof(double) * (size_t)n);
     for (int i = 0; i < n; ++i) S->g[i] -= alpha_cg * AP[i];
     orc_phi(S->P, S->g, S->phi);
@@ -531,13 +626,14 @@ static double cg_monotonic_step(mprgp *S, double *x, int *result_is_prop) {
     double b = dot(n, S->z, AP) / dot(n, S->p, AP);
     for (int i = 0; i < n; ++i) S->p[i] = S->z[i] - b * S->p[i];
     S->residual_out = compute_gradients(S, 0); /* overwrites gp (= AP) */
+    record(S, ORC_EV_RESIDUAL, S->residual_out <= S->tol, S->residual_out, S->tol);
     *result_is_prop = proportional(S, x);
     orc_spmv(S->A, S->p, AP);
     pd = dot(n, S->p, AP);
     alpha_cg = dot(n, S->z, S->g) / pd;
     for (int i = 0; i < n; ++i) S->y[i] = x[i] - alpha_cg * S->p[i];
     fx = fy;
-    fy = project_fun_value(S, S->y);
+    fy = trial_fy(S, alpha_cg, pd, AP, &ct);
     S->num_cg++;
     S->iteration++;
   }
@@ -549,14 +645,23 @@ static int solve_monotonic(mprgp *S, double *x) {
   initialize(S, x);
   for (S->iteration = 0; S->iteration < S->max_iterations;) {
     S->residual_out = compute_gradients(S, 1);
+    record(S, ORC_EV_RESIDUAL, S->residual_out <= S->tol, S->residual_out, S->tol);
     if (S->residual_out <= S->tol) break;
     int result_is_prop = proportional(S, x);
     if (result_is_prop) S->residual_out = cg_monotonic_step(S, x, &result_is_prop);
     if (S->residual_out <= S->tol) break;
-    if (result_is_prop || S->y_size <= 0 || !orc_is_feasible(S->P, S->y)) {
+    int expand = result_is_prop || S->y_size <= 0;
+    if (!expand) {
+      const int feas = orc_is_feasible(S->P, S->y);
+      if (S->opt) record(S, ORC_EV_FEASIBLE, feas, min_slack(S, S->y), -ORC_EPS);
+      expand = !feas;
+    }
+    if (expand) {
+      record(S, ORC_EV_STEP, ORC_STEP_EXP, 0.0, 0.0);
       exp_monotonic_step(S, x);
       S->iteration++;
     } else {
+      record(S, ORC_EV_STEP, ORC_STEP_PROP, 0.0, 0.0);
       grad(S, x);
       prop_step(S, x);
       S->iteration++;
@@ -597,6 +702,14 @@ int orc_decoupled_solve(const orc_csr *A, const double *B, const orc_csr *J, con
                         double *x, double tol, int max_it, int variant, int preconditioned,
                         double alpha_bar, double *ev, int *ev_valid, int project_first,
                         orc_stats *st) {
+  return orc_decoupled_solve_ex(A, B, J, c, x, tol, max_it, variant, preconditioned, alpha_bar, ev, ev_valid,
+                                project_first, NULL, st);
+}
+
+int orc_decoupled_solve_ex(const orc_csr *A, const double *B, const orc_csr *J, const double *c,
+                           double *x, double tol, int max_it, int variant, int preconditioned,
+                           double alpha_bar, double *ev, int *ev_valid, int project_first,
+                           orc_options *opt, orc_stats *st) {
   int n = A->rows;
   orc_projector *P = orc_projector_new(J, c);
   if (project_first) {
@@ -609,6 +722,8 @@ int orc_decoupled_solve(const orc_csr *A, const double *B, const orc_csr *J, con
   mprgp S;
   memset(&S, 0, sizeof(S));
   S.A = A; S.B = B; S.P = P; S.pc = pc; S.n = n;
+  S.opt = opt;
+  if (opt) opt->n = 0;
   S.max_iterations = max_it;
   S.tol = tol < 1e-30 ? 1e-30 : tol; /* setParameters (MPRGPSolver.h:34-39) */
   S.gamma = 1.0;

@@ -94,6 +94,53 @@ int orc_decoupled_solve(const orc_csr *A, const double *B, const orc_csr *J, con
                         double alpha_bar, double *ev, int *ev_valid, int project_first,
                         orc_stats *st);
 
+/* ---- traced solves ------------------------------------------------------ */
+/* One decision of the monotone solve, in the order the control flow takes
+ * them (iteration = the reference's `iteration` counter at that point):
+ *   ORC_EV_RESIDUAL  residual_out <= tol at the loop head or after a CG
+ *                    step (lhs = residual, rhs = tol, outcome = converged)
+ *   ORC_EV_PROP      proportional(x) (lhs = beta.beta, rhs = gamma^2 phitphi)
+ *   ORC_EV_CG_TEST   fy <= fx of the monotone CG loop (lhs = fy, rhs = fx,
+ *                    outcome = step accepted); when y is infeasible the next
+ *                    event is ORC_EV_Y_INFEASIBLE (lhs = min over rows of
+ *                    n.y - c)
+ *   ORC_EV_FEASIBLE  isFeasible(y) for the choice expansion/proportioning
+ *                    (lhs = min over rows of n.y - c, rhs = -1e-9)
+ *   ORC_EV_STEP      the step taken (outcome ORC_STEP_*)
+ *   ORC_EV_FY_CLOSED follows each ORC_EV_CG_TEST: f(P(y)) of the same trial
+ *                    point by the closed forms of ORC_F_CLOSED (lhs) and
+ *                    ORC_F_CLOSED_FROM_FPY (rhs), each tracked along this
+ *                    solve's own trajectory */
+enum { ORC_EV_RESIDUAL = 0, ORC_EV_PROP = 1, ORC_EV_CG_TEST = 2, ORC_EV_FEASIBLE = 3, ORC_EV_STEP = 4,
+       ORC_EV_Y_INFEASIBLE = 5, ORC_EV_FY_CLOSED = 6 };
+enum { ORC_STEP_CG = 0, ORC_STEP_EXP = 1, ORC_STEP_PROP = 2 };
+typedef struct {
+  int iteration, kind, outcome, pad;
+  double lhs, rhs;
+} orc_event;
+/* How the monotone CG loop obtains f(P(y)) (MPRGPSolver.h:283-308):
+ *   ORC_F_DIRECT  as the reference: project y, then 1/2 y'Ay - y'b by a SpMV;
+ *   ORC_F_CLOSED  from the step's sums, as the device's mixed form does:
+ *                 f(y) = f(x) - alpha p'g + alpha^2 p'Ap/2, with f(x) the
+ *                 objective at the (unprojected) iterate, plus
+ *                 d'(A y - b) + d'A d/2 for d = P(y) - y taken on the rows
+ *                 violated by more than the feasibility tolerance 1e-9;
+ *   ORC_F_CLOSED_FROM_FPY  the same, but continuing from f(P(y)) of the
+ *                 accepted step instead of f(y): a deliberately wrong form
+ *                 that tests use to show the checks catch it. */
+enum { ORC_F_DIRECT = 0, ORC_F_CLOSED = 1, ORC_F_CLOSED_FROM_FPY = 2 };
+typedef struct {
+  orc_event *events; /* may be NULL */
+  int cap;           /* capacity of events                                  */
+  int n;             /* out: events recorded (may exceed cap: the rest lost) */
+  int fmode;         /* ORC_F_*                                              */
+} orc_options;
+/* orc_decoupled_solve with options (the monotone variant records events). */
+int orc_decoupled_solve_ex(const orc_csr *A, const double *B, const orc_csr *J, const double *c,
+                           double *x, double tol, int max_it, int variant, int preconditioned,
+                           double alpha_bar, double *ev, int *ev_valid, int project_first,
+                           orc_options *opt, orc_stats *st);
+
 /* ---- FEM assembly (fem_oracle.c) -------------------------------------- */
 enum { ORC_LINEAR = 1, ORC_COROTATIONAL = 2 };
 

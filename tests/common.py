@@ -75,3 +75,22 @@ def brute_force_qp(A, b, node, normal, c):
                 if best is None or f < best[1] - 1e-14:
                     best = (x, f, S)
     return best
+
+
+def truncated_counts(seq, k):
+    """(num_cg, num_exp, num_prop) of a monotone solve run with max_it = k,
+    from the step sequence `seq` (oracle.STEP_*) of the same solve run to
+    convergence.  The truncated run takes the same first k steps; when its
+    k-th step is a CG step that the full run continues with another CG step,
+    the loop exits on `iteration < max_iterations` and one expansion follows
+    (MPRGPSolver.h:288-290, 431-433); when the full run's next step is an
+    expansion or proportioning step after a CG step, that step is taken too."""
+    seq = list(seq)
+    CG, EXP, PROP = 0, 1, 2
+    if k >= len(seq):
+        steps = seq
+    elif k >= 1 and seq[k - 1] == CG:
+        steps = seq[:k] + ([EXP] if seq[k] == CG else [seq[k]])
+    else:
+        steps = seq[:k]
+    return tuple(sum(1 for s in steps if s == t) for t in (CG, EXP, PROP))
