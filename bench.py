@@ -128,6 +128,93 @@ def measure_traffic(args, tiles, what="solve"):
             "write_scale": round(cal_w, 4)}
 
 
+def host_info():
+    """The host's CPU model and core counts (BASELINE's CPU-baseline plan asks
+    for them in every results file): /proc/cpuinfo, os, lscpu."""
+    info = {"cpu_model": None, "host_logical_cpus": os.cpu_count(),
+            "usable_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    info["cpu_model"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        r = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20)
+        keep = ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)",
+                "NUMA node(s)", "CPU max MHz")
+        info["lscpu"] = {k.strip(): v.strip() for k, v in
+                         (line.split(":", 1) for line in r.stdout.splitlines() if ":" in line)
+                         if k.strip() in keep}
+    except (OSError, subprocess.SubprocessError):
+        info["lscpu"] = None
+    return info
+
+
+def dropin_leg(M, problems, workloads, ctx, tol=1e-8, max_it=500, calls=3):
+    """The one-shot drop-in on the 133,623-DoF beam (the FP64 single-QP
+    workload): what DecoupledMprgpFemSolver::forward would run each frame
+    after swapping MprgpFemSolver.cpp:532-533 for mprgp_decoupled_solve --
+    host CSR arrays in (A from the device assembly, downloaded once), a fresh
+    context, SELL build and upload, solve, x and ev back.  The first call runs
+    the power method from scratch; the later ones pass the ev of the previous
+    call, as forward passes power_ev.  Beside it, the same solve with A, b,
+    rows and x resident (mprgp_qp_solve)."""
+    sc = problems.config2()
+    dq = workloads.DeviceQP(ctx, sc)
+    A = dq.fem.matrix.to_csr()
+    b = dq.qp.get_rhs()
+    node, normal, c = sc.rows()
+    J = problems.rows_to_J(node, normal, sc.n_nodes)
+    ev, ms, codes, st = np.zeros(A.shape[0]), [], [], {}
+    for _ in range(calls):
+        x = sc.x_n.ravel().copy()
+        st = {}
+        t0 = time.perf_counter()
+        codes.append(M.MPRGPDecoupledCon.solve(A, b, J, c, x, tol=tol, max_it=max_it, ev=ev, stats_out=st))
+        ms.append(1e3 * (time.perf_counter() - t0))
+    # the drop-in's phases through the object API, same QP: a context, the
+    # matrix (host SELL build + upload), the QP and its vectors, the solve
+    # (alpha_bar as the drop-in's last call found it), x back
+    ph = {}
+    t0 = time.perf_counter()
+    c2 = M.Context(0)
+    ph["context"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Am = c2.matrix_csr(A)
+    ph["matrix_build_upload"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    q2 = Am.qp()
+    q2.set_rows(node, normal, c)
+    q2.set_rhs(b)
+    q2.set_x(sc.x_n.ravel())
+    ph["qp_and_vectors"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rc, s2 = q2.solve(M.params(tol=tol, max_it=max_it, project_first=1, alpha_bar=st["alpha_bar"]))
+    ph["solve"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    q2.get_x()
+    ph["x_back"] = time.perf_counter() - t0
+    c2.close()
+    res = []
+    for _ in range(calls):
+        dq.qp.reset_x()
+        t0 = time.perf_counter()
+        rc, sr = dq.qp.solve(M.params(tol=tol, max_it=max_it, project_first=1, alpha_bar=st["alpha_bar"]))
+        res.append(1e3 * (time.perf_counter() - t0))
+    return {"workload": f"beam 100x20x20, {A.shape[0]} DoF, {A.nnz} nonzeros, {len(node)} rows, FP64, tol {tol:g}",
+            "dropin_ms": [round(t, 2) for t in ms], "codes": codes, "iterations": st.get("iterations"),
+            "power_iters": st.get("power_iters"),
+            "phases_ms": {k: round(1e3 * v, 2) for k, v in ph.items()},
+            "resident_solve_ms": [round(t, 2) for t in res], "resident_iterations": sr["iterations"],
+            "note": "drop-in = host arrays in, new context, SELL build + upload, power method (no ev passed: "
+                    "its 1001-iteration cap keeps ev invalid), solve, x back; resident = the same QP kept on "
+                    "the device, alpha_bar given"}
+
+
 def spmv_leg(M, workloads, ctx, qps, tiles, graph_steps):
     """Bandwidth of the batched SpMV (FixedSparseMatrix::multiply,
     MPRGPSolver.h:469-473) on the whole batch: every QP runs the power method
@@ -179,6 +266,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="library default shape instead of the autotuner's")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM counter passes")
     ap.add_argument("--no-spmv", action="store_true", help="skip the batched SpMV bandwidth leg")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the one-shot drop-in leg")
     ap.add_argument("--pmc-child", choices=["copy", "solve", "spmv"], help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -278,7 +366,7 @@ def main():
             f64["traffic_vs_algorithmic"] = round(f64["traffic"] / f64["bytes_per_launch"], 3)
         log(f"batched spmv (power method to its cap): {spmv}")
 
-    other = None
+    other, st64 = None, None
     if mixed and not args.no_fp64_batch:
         per64 = batch_params(M.FP64)
         if shape is not None:
@@ -320,7 +408,7 @@ def main():
     if not args.no_cpu:
         import oracle
         order = np.argsort([q.n_nodes for q in qps])
-        done, spent, dof, rel_o, sample, per_ms, ostats = 0, 0.0, 0, [], [], [], []
+        done, spent, dof, rel_o, sample, per_ms, ostats, work = 0, 0.0, 0, [], [], [], [], []
         for i in order:
             sc = scenes[i]
             tets, d, vol, mass = oracle.fem_setup(sc.n_nodes, sc.tets, sc.X0)
@@ -336,15 +424,45 @@ def main():
             spent += dt
             per_ms.append(1e3 * dt)
             ostats.append(so)
+            work.append(A.nnz * (so["iterations"] + 1))
             sample.append((A, b, J, c, sc.x_n.ravel(), qps[i].alpha_bar))
             rel_o.append(np.linalg.norm(x_head[i] - xo) / np.linalg.norm(xo))
             done += 1
             dof += 3 * sc.n_nodes
             if spent > args.cpu_seconds:
                 break
+        hinfo = host_info()
+        # the whole workload by extrapolation: seconds per (nonzero x
+        # iteration) from the sample, times each QP's nonzeros and the
+        # iterations of its FP64 GPU solve (the same algorithm); checked
+        # against one timed solve of the largest QP
+        rate = spent / float(sum(work))
+        it64 = [s["iterations"] for s in st64] if other is not None else [s["iterations"] for s in stats]
+        sampled = set(int(i) for i in order[:done])
+        est = spent + sum(rate * qps[j].n_blocks * 9 * (it64[j] + 1) for j in range(len(qps)) if j not in sampled)
+        # the QP with the most predicted work, timed to check the model
+        big = max(range(len(qps)), key=lambda j: qps[j].n_blocks * (it64[j] + 1))
+        scb = scenes[big]
+        tets, d, vol, mass = oracle.fem_setup(scb.n_nodes, scb.tets, scb.X0)
+        Ab, bb = oracle.fem_build_system(scb.n_nodes, tets, d, vol, mass, scb.young, scb.poisson, scb.density,
+                                         scb.x_n.ravel(), scb.v_n.ravel(), scb.gravity, scb.h, scb.material)
+        nb_, nrm_, cb_ = scb.rows()
+        s0 = time.perf_counter()
+        _, _, sob, _ = oracle.decoupled_solve(Ab, bb, problems.rows_to_J(nb_, nrm_, scb.n_nodes), cb_,
+                                              scb.x_n.ravel(), tol=args.tol, max_it=4000,
+                                              alpha_bar=qps[big].alpha_bar, project_first=True)
+        big_s = time.perf_counter() - s0
+        big_pred = rate * Ab.nnz * (sob["iterations"] + 1)
         cpu = {"value": round(done / spent, 3), "unit": "solves/s", "cores": 1, "kind": "port",
                "sample": f"{done} smallest of the {args.qps} QPs ({dof} DoF total), oracle FP64, "
-                         f"same alpha_bar, {spent:.1f}s; host {platform.processor() or platform.machine()}",
+                         f"same alpha_bar, {spent:.1f}s, 1 thread on {hinfo['cpu_model']}",
+               "host": hinfo,
+               "full_workload_estimate": {
+                   "solves_per_s": round(args.qps / est, 3), "seconds": round(est, 1),
+                   "model": "sample + (s per nonzero-iteration of the sample) x nonzeros x (FP64 GPU "
+                            "iterations + 1) for the QPs not timed",
+                   "costliest_qp": {"dof": 3 * scb.n_nodes, "measured_s": round(big_s, 2),
+                                    "model_s": round(big_pred, 2), "iterations": sob["iterations"]}},
                "per_solve_ms": {"p50": round(float(np.percentile(per_ms, 50)), 3),
                                 "p99": round(float(np.percentile(per_ms, 99)), 3)},
                "steps": {k: int(sum(o[k] for o in ostats))
@@ -357,7 +475,10 @@ def main():
         # the same sample over all host cores of this GPU's share: independent
         # QPs solved concurrently (the oracle's ctypes calls release the GIL)
         import concurrent.futures as cf
-        workers = max(1, min(int(os.environ.get("OMP_NUM_THREADS", "16")), os.cpu_count() or 1))
+        # every CPU this GPU's share of the host gives the process
+        # (OMP_NUM_THREADS caps it to that share where it is set)
+        usable = hinfo["usable_cpus"] or os.cpu_count() or 1
+        workers = max(1, min(usable, int(os.environ.get("OMP_NUM_THREADS") or usable)))
 
         def one(item):
             A, b, J, c, x0, ab = item
@@ -369,7 +490,9 @@ def main():
             list(pool.map(one, sample))
         wall = time.perf_counter() - s0
         cpu_all = {"value": round(done / wall, 3), "unit": "solves/s", "cores": workers, "kind": "port",
-                   "sample": f"the same {done} QPs, one per thread at a time, {wall:.1f}s wall"}
+                   "sample": f"the same {done} QPs, one per thread at a time, {wall:.1f}s wall, "
+                             f"{workers} threads = the CPUs this process may use (host: "
+                             f"{hinfo['host_logical_cpus']} logical CPUs)"}
         del sample
         log(f"cpu oracle, all cores: {cpu_all}")
 
@@ -384,9 +507,15 @@ def main():
         serve.pop("engine", None)
         serve["sessions"] = len(ss)
         serve["dof_total"] = int(sum(3 * s.n_nodes for s in ss))
+        serve["fp64_error_sample"] = serving.fp64_error_sample(eng, ss, n=16, precision=prec)
         serve["plain_fp64_one_at_a_time"] = serving.run_one_at_a_time(ss, n_req=150, seed=7)
         eng.close()
         log(f"serving: {serve}")
+
+    dropin = None
+    if not args.no_dropin:
+        dropin = dropin_leg(M, problems, workloads, ctx)
+        log(f"drop-in: {dropin}")
 
     out = {
         "metric": "MPRGP solves/s at KKT res \u22641e-6, batched FP32/FP64 vs plain FP64; p50/p99 ms",
@@ -417,6 +546,7 @@ def main():
         "spmv_roofline": spmv,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
+        "dropin_fp64_133k": dropin,
     }
     print(json.dumps(out), flush=True)
 

@@ -99,7 +99,17 @@ typedef struct {
   double alpha_bar;
   double lambda_max;  /* spectral-radius estimate behind alpha_bar                  */
   double elapsed_ms;  /* device time of the solve call (whole batch)                */
-  double reserved_d[4];
+  double queue_ms;    /* engine: wait from submission to admission                  */
+  /* No counterpart in the reference: a bound on |x - x*|_2, the distance of the
+   * returned x from the QP's exact minimiser, residual / lambda_min_lower with
+   * lambda_min_lower a lower bound on A's smallest eigenvalue
+   * (mprgp_matrix_set_lambda_min; FEM assembly sets it); -1 when none is known.
+   * For x feasible with its face exactly the rows it meets, strong convexity
+   * and the rows' nonnegative multipliers give
+   * lambda_min |x - x*|^2 <= g(x)'(x - x*) <= |phi + beta| |x - x*|.  It is
+   * the mixed form's per-solve error report: residual is its FP64 KKT norm. */
+  double error_bound;
+  double reserved_d[2];
 } mprgp_stats;
 
 /* ---- context ------------------------------------------------------------ */
@@ -122,6 +132,12 @@ int mprgp_matrix_create_bsr3(mprgp_ctx *ctx, int n_nodes, const int *rowptr, con
  * innerIndexPtr(), valuePtr(): its CSC arrays are the CSR arrays. */
 int mprgp_matrix_create_csr(mprgp_ctx *ctx, int n, const int *ptr, const int *idx,
                             const double *val, mprgp_matrix **out);
+/* The matrix back as the node-order block CSR it was created from (same
+ * block order; values as last written, e.g. by mprgp_fem assembly):
+ * rowptr[n_nodes+1], colidx[nb], blocks[9*nb] row-major (sizes from
+ * mprgp_matrix_info).  What the simulator's per-frame QP dump needs
+ * (saveQP, MprgpFemSolver.h:129-133, with mprgp_write_qp). */
+int mprgp_matrix_get_bsr3(mprgp_matrix *A, int *rowptr, int *colidx, double *blocks);
 /* new values, same pattern and block order as at creation */
 int mprgp_matrix_update_bsr3(mprgp_matrix *A, const double *blocks);
 int mprgp_matrix_info(const mprgp_matrix *A, int *n_nodes, long long *n_blocks, long long *bytes);
@@ -129,6 +145,12 @@ int mprgp_matrix_info(const mprgp_matrix *A, int *n_nodes, long long *n_blocks, 
 int mprgp_matrix_spmv(mprgp_matrix *A, const double *x, double *y);
 /* 1/2 x'Ax - x'b, FixedSparseMatrix::funcValue (:475-482), host vectors in node order */
 int mprgp_matrix_func_value(mprgp_matrix *A, const double *x, const double *b, double *f);
+/* No counterpart in the reference: declares lower > 0 a lower bound on A's
+ * smallest eigenvalue (for A = M + h^2 K with K positive semidefinite, the
+ * smallest lumped mass), which solves turn into mprgp_stats.error_bound;
+ * 0 withdraws it.  mprgp_fem assembly sets it to min(m_i) (1 + h alpha_k);
+ * new values (mprgp_matrix_update_bsr3) withdraw it. */
+int mprgp_matrix_set_lambda_min(mprgp_matrix *A, double lower);
 /* diag(A), 3 per node in node order: getDiagonal (MPRGPUtility.h:503-515), FixedSparseMatrix::diag */
 int mprgp_matrix_diagonal(mprgp_matrix *A, double *diag);
 void mprgp_matrix_destroy(mprgp_matrix *A);
@@ -353,8 +375,7 @@ int mprgp_engine_create(mprgp_ctx *ctx, const mprgp_engine_config *cfg, mprgp_en
 int mprgp_engine_submit(mprgp_engine *eng, mprgp_qp *qp, const mprgp_params *params, long long *ticket);
 /* Waits up to timeout_ms (< 0: forever) for a ticket; returns its solve code
  * (0, -1, or an error) and fills stats (elapsed_ms = submit-to-completion
- * latency, reserved_d[0] = queue wait in ms, reserved_i[0] = launches while
- * admitted), or MPRGP_EPENDING on time-out.  A ticket is waited for once. */
+ * latency, queue_ms = queue wait, reserved_i[0] = launches while admitted), or MPRGP_EPENDING on time-out.  A ticket is waited for once. */
 int mprgp_engine_wait(mprgp_engine *eng, long long ticket, int timeout_ms, mprgp_stats *stats);
 /* As mprgp_engine_wait for whichever ticket completed first (in completion
  * order); *ticket receives it.  MPRGP_ESTATE if no ticket is outstanding. */

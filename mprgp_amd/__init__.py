@@ -260,6 +260,26 @@ class Matrix:
         check(load().mprgp_matrix_diagonal(self.h, _p(d)))
         return d
 
+    def get_bsr3(self):
+        """(rowptr, colidx, blocks[nb, 3, 3]) in the creation's block order."""
+        i = self.info()
+        rp = np.zeros(self.n_nodes + 1, np.int32)
+        ci = np.zeros(i["n_blocks"], np.int32)
+        bl = np.zeros((i["n_blocks"], 3, 3))
+        check(load().mprgp_matrix_get_bsr3(self.h, _p(rp), _p(ci), _p(bl)))
+        return rp, ci, bl
+
+    def to_csr(self):
+        """The matrix as a scipy CSR of scalars (3 rows per node)."""
+        import scipy.sparse as sp
+        rp, ci, bl = self.get_bsr3()
+        return sp.bsr_matrix((bl, ci, rp), shape=(3 * self.n_nodes, 3 * self.n_nodes)).tocsr()
+
+    def set_lambda_min(self, lower: float):
+        """A lower bound on the smallest eigenvalue (0: none); solves then
+        report error_bound = residual / lower."""
+        check(load().mprgp_matrix_set_lambda_min(self.h, float(lower)))
+
     def qp(self) -> "QP":
         h = C.c_void_p()
         check(load().mprgp_qp_create(self.h, C.byref(h)))
@@ -467,7 +487,6 @@ class Engine:
             return None
         check(rc, allow=(_lib.OK, _lib.NOT_CONVERGED, _lib.ENAN))
         d = st.as_dict()
-        d["queue_ms"] = st.reserved_d[0]
         return rc, d
 
     def wait_any(self, timeout_ms: int = -1):
@@ -479,7 +498,6 @@ class Engine:
             return None
         check(rc, allow=(_lib.OK, _lib.NOT_CONVERGED, _lib.ENAN))
         d = st.as_dict()
-        d["queue_ms"] = st.reserved_d[0]
         return t.value, rc, d
 
     def info(self):

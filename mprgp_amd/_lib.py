@@ -46,7 +46,8 @@ class Stats(C.Structure):
                 ("power_iters", C.c_int), ("ev_valid", C.c_int), ("refinements", C.c_int),
                 ("steps", C.c_int), ("reserved_i", C.c_int * 2), ("residual", C.c_double),
                 ("alpha_bar", C.c_double), ("lambda_max", C.c_double),
-                ("elapsed_ms", C.c_double), ("reserved_d", C.c_double * 4)]
+                ("elapsed_ms", C.c_double), ("queue_ms", C.c_double), ("error_bound", C.c_double),
+                ("reserved_d", C.c_double * 2)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
@@ -110,6 +111,8 @@ SIGNATURES = {
     "mprgp_matrix_create_csr": (I, [P, I, P, P, P, PP]),
     "mprgp_matrix_update_bsr3": (I, [P, P]),
     "mprgp_matrix_info": (I, [P, C.POINTER(I), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "mprgp_matrix_set_lambda_min": (I, [P, C.c_double]),
+    "mprgp_matrix_get_bsr3": (I, [P, P, P, P]),
     "mprgp_matrix_spmv": (I, [P, P, P]),
     "mprgp_matrix_destroy": (None, [P]),
     "mprgp_qp_create": (I, [P, PP]),

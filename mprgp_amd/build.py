@@ -26,6 +26,27 @@ def sources():
     return sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
 
 
+ASAN_LIB = os.path.join(HERE, "libmprgp_amd_hostasan.so")
+
+
+def build_host_asan(verbose: bool = False) -> str:
+    """The same library with the host code under AddressSanitizer (device code
+    unchanged: -fsanitize applies to the host pass only), for the graph
+    life-cycle check tests/cpp/graph_churn.cpp.  Not loaded by the package."""
+    tmp = ASAN_LIB + ".tmp"
+    flags = [f for f in FLAGS if f != "-O3"] + ["-O1", "-g", "-Xarch_host", "-fsanitize=address",
+                                                "-Xarch_host", "-fno-omit-frame-pointer"]
+    cmd = [HIPCC, *flags, "-I", os.path.join(HERE, "..", "include"), "-o", tmp, *sources()]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building libmprgp_amd_hostasan.so")
+    os.replace(tmp, ASAN_LIB)
+    return ASAN_LIB
+
+
 def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     cmd = [HIPCC, *FLAGS, "-I", os.path.join(HERE, "..", "include"), "-o", tmp, *sources()]

@@ -114,6 +114,7 @@ extern "C" int mprgp_ctx_create(int device, size_t hbm_budget_bytes, mprgp_ctx *
   if (e == hipSuccess) e = hipEventCreate(&c->t1);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->q[0], hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->q[1], hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ready, hipEventDisableTiming);
   if (e != hipSuccess) {
     mprgp_ctx_destroy(c);
     return fail(MPRGP_EHIP, "context setup failed: %s", hipGetErrorString(e));
@@ -140,6 +141,7 @@ extern "C" void mprgp_ctx_destroy(mprgp_ctx *c) {
   if (c->t1) (void)hipEventDestroy(c->t1);
   if (c->q[0]) (void)hipEventDestroy(c->q[0]);
   if (c->q[1]) (void)hipEventDestroy(c->q[1]);
+  if (c->ready) (void)hipEventDestroy(c->ready);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   MPRGP_CATCH_VOID
@@ -343,6 +345,50 @@ extern "C" int mprgp_matrix_update_bsr3(mprgp_matrix *A, const double *blocks) {
   MPRGP_HIP(launch_setup_diag(s, A->d_sl_off, A->d_col, A->d_val, A->d_dinv, A->n_nodes));
   MPRGP_HIP(hipStreamSynchronize(s));
   A->version++;
+  A->lambda_min_lower = 0.0;  // new values: the old bound no longer holds
+  return MPRGP_OK;
+  MPRGP_CATCH
+}
+
+extern "C" int mprgp_matrix_get_bsr3(mprgp_matrix *A, int *rowptr, int *colidx, double *blocks) {
+  MPRGP_TRY
+  if (!A || !rowptr || !colidx || !blocks) return fail(MPRGP_EBADINPUT, "NULL argument");
+  mprgp_ctx *ctx = A->ctx;
+  MPRGP_HIP(hipSetDevice(ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
+  std::vector<int> col((size_t)A->slots);
+  std::vector<double> val((size_t)A->slots * 9);
+  hipStream_t s = ctx->stream;
+  MPRGP_HIP(hipMemcpyAsync(col.data(), A->d_col, sizeof(int) * col.size(), hipMemcpyDeviceToHost, s));
+  MPRGP_HIP(hipMemcpyAsync(val.data(), A->d_val, sizeof(double) * val.size(), hipMemcpyDeviceToHost, s));
+  MPRGP_HIP(hipStreamSynchronize(s));
+  // block b of the creation order sits in SELL slot blk_slot[b]: its row is
+  // the slot's lane in its slice, its column the slot's column index, both
+  // internal node numbers; creation order is by row, so the row counts give
+  // rowptr
+  std::fill(rowptr, rowptr + A->n_nodes + 1, 0);
+  for (long long b = 0; b < A->nb; ++b) {
+    const long long slot = A->blk_slot[b];
+    const int sl = (int)(std::upper_bound(A->sl_off_h.begin(), A->sl_off_h.end(), (int)slot) -
+                         A->sl_off_h.begin()) - 1;
+    const long long b0 = A->sl_off_h[sl];
+    const long long l = (slot - b0) % kSlice, k = (slot - b0) / kSlice;
+    const int row = A->perm[sl * kSlice + (int)l];
+    rowptr[row + 1]++;
+    colidx[b] = A->perm[col[slot]];
+    const long long vbase = b0 * 9 + k * 9 * kSlice + l;
+    for (int e = 0; e < 9; ++e) blocks[9 * b + e] = val[vbase + (long long)e * kSlice];
+  }
+  for (int i = 0; i < A->n_nodes; ++i) rowptr[i + 1] += rowptr[i];
+  return MPRGP_OK;
+  MPRGP_CATCH
+}
+
+extern "C" int mprgp_matrix_set_lambda_min(mprgp_matrix *A, double lower) {
+  MPRGP_TRY
+  if (!A) return fail(MPRGP_EBADINPUT, "NULL matrix");
+  if (!(lower >= 0.0) || !std::isfinite(lower)) return fail(MPRGP_EBADINPUT, "bound must be finite and >= 0");
+  A->lambda_min_lower = lower;
   return MPRGP_OK;
   MPRGP_CATCH
 }
@@ -615,9 +661,9 @@ extern "C" int mprgp_qp_reset_x(mprgp_qp *q) {
   MPRGP_NOT_BUSY(q);
   MPRGP_HIP(hipSetDevice(q->ctx->device));
   std::lock_guard<std::recursive_mutex> lk_(q->ctx->mu);
+  // queued on the context's stream; a solve on another stream (an engine's)
+  // is ordered after it by prepare_qp
   MPRGP_HIP(hipMemcpyAsync(q->x, q->x0, sizeof(double) * 3 * q->n, hipMemcpyDeviceToDevice, q->ctx->stream));
-  // complete before returning, as set_x: an engine solves on its own stream
-  MPRGP_HIP(hipStreamSynchronize(q->ctx->stream));
   return MPRGP_OK;
   MPRGP_CATCH
 }
@@ -1247,6 +1293,13 @@ void describe_qp(const mprgp_qp *q, const mprgp_params &p, int n_tiles, int *don
 // state copies.
 int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, int *done_flag, int tag,
                QpDev &d, QpState &st, bool rows) {
+  if (q->ctx->stream != s) {
+    // work the QP's own context queued (reset_x, a collider, an assembly
+    // not yet synchronised) precedes the solve on stream s
+    std::lock_guard<std::recursive_mutex> lk(q->ctx->mu);
+    MPRGP_HIP(hipEventRecord(q->ctx->ready, q->ctx->stream));
+    MPRGP_HIP(hipStreamWaitEvent(s, q->ctx->ready, 0));
+  }
   if (p.precision == MPRGP_MIXED) {
     if (int rc = ensure_matrix32(q->A, s)) return rc;
     if (int rc = ensure_qp32(q, s)) return rc;
@@ -1296,6 +1349,8 @@ int fill_stats(mprgp_qp *q, const QpState both[2], float ms, long launched, mprg
     o.lambda_max = st.lambda_max;
     o.elapsed_ms = ms;
     o.reserved_i[0] = (int)launched;  // step-kernel launches issued for the whole batch
+    const double lo = q->A->lambda_min_lower;
+    o.error_bound = (lo > 0.0 && std::isfinite(st.residual)) ? st.residual / lo : -1.0;
   }
   if (!std::isfinite(st.residual)) return MPRGP_ENAN;
   if (st.op != OP_DONE) return MPRGP_EHIP;

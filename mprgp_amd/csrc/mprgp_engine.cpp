@@ -115,7 +115,7 @@ void mprgp_engine::finish(Request *rq, int rc) {
   rq->rc = rc;
   rq->latency_ms = std::chrono::duration<double, std::milli>(Clock::now() - rq->t_submit).count();
   rq->stats.elapsed_ms = (float)rq->latency_ms;
-  rq->stats.reserved_d[0] = rq->queue_ms;
+  rq->stats.queue_ms = rq->queue_ms;
   rq->done = true;
   done_q.push_back(rq->ticket);
   completed++;

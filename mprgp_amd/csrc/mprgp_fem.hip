@@ -391,6 +391,7 @@ struct mprgp_fem {
   double *rot = nullptr;   // per-tet rotations of the last corotational assembly
   bool rot_valid = false;
   double density = 0.0;
+  double min_mass = 0.0;  // smallest lumped node mass rho sum(V/4)
   double alpha_k = 0.0, alpha_m = 0.0;  // Rayleigh damping (FEMSystem::setDamping)
   // device-resident body state (node order): positions x_n, velocities v_n,
   // the Newton iterate x1 and the next one x2 (allocated by set_state)
@@ -517,6 +518,7 @@ extern "C" int mprgp_fem_create(mprgp_ctx *ctx, int N, int T, const int *tets_in
   f->T = T;
   f->nb = nb;
   f->density = density;
+  f->min_mass = *std::min_element(mass_rho.begin(), mass_rho.end());
   size_t bytes = 256 * 17 + sizeof(int4) * T + sizeof(double) * (9 + 1 + 2 + 9) * (size_t)T +
                  sizeof(double) * 2 * N + sizeof(int) * ((nb + 1) + bctr.size() + 2 * nb + (N + 1) + nctr.size());
   if (int rc = ctx->alloc(&f->dev, bytes)) {
@@ -627,6 +629,10 @@ int assemble_dev(mprgp_fem *f, int material, const double *xn, const double *vn,
   if (!rhs_only) {
     f->A->version++;
     f->rot_valid = corot;
+    // A = M (1 + h alpha_k) + K (h^2 + h alpha_m), K positive semidefinite
+    // (LINEAR, or COROTATIONAL with the rotations held): its smallest
+    // eigenvalue is at least the smallest diagonal of the mass term
+    f->A->lambda_min_lower = kc >= 0.0 && mc > 0.0 ? f->min_mass * mc : 0.0;
   }
   qp->has_rhs = true;
   return MPRGP_OK;

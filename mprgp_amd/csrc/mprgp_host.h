@@ -161,6 +161,7 @@ struct mprgp_ctx {
   mprgp::BatchTables tab;    // solve_batch's launch tables
   int *h_running = nullptr;  // pinned, 2 slots
   hipEvent_t t0 = nullptr, t1 = nullptr, q[2] = {nullptr, nullptr};
+  hipEvent_t ready = nullptr;  // marks the stream's work for solves on another stream
   // scratch shared by calls on this context (serialised by `mu`)
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
@@ -190,6 +191,7 @@ struct mprgp_matrix {
   // FP32 copy of the values for the mixed form (made on first use, refreshed on updates)
   float *d_val32 = nullptr;
   int version32 = -1;
+  double lambda_min_lower = 0.0;  // > 0: a lower bound on the smallest eigenvalue (error_bound)
 };
 
 // Bytes of A one SpMV of the inner solve streams (values and column indices

@@ -164,13 +164,17 @@ def session_scene(rng, sid, dof_range=(3e3, 1.5e5)):
 
 
 def make_sessions(ctx, count=64, seed=5, dof_range=(3e3, 1.5e5), reassemble=10, own_contexts=False,
-                  device_state=True):
+                  device_state=True, only=None):
     """Sessions on `ctx`, or each on a context of its own (same device) so
-    that the host work of different sessions does not share one stream."""
+    that the host work of different sessions does not share one stream.
+    `only`: the session ids to build (the others' scenes are drawn and
+    dropped, so session i is the same whichever others are built)."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(count):
         sc = session_scene(rng, i, dof_range)
+        if only is not None and i not in only:
+            continue
         sctx = M.Context(0) if own_contexts else ctx
         if own_contexts:
             ctx._objs.append(sctx)
@@ -203,14 +207,18 @@ def run_open_loop(engine: M.Engine, sessions, rate=2000.0, duration=3.0, tol=1e-
                   precision=M.MIXED, seed=6, workers=4, keep_history=False, max_it=1000):
     """Poisson arrivals at `rate` requests/s for `duration` s, each to a
     uniformly chosen session; a session's requests are served in order.
-    Latency = completion of the solve - arrival of the request."""
+    Latency = completion of the solve - arrival of the request.
+    keep_history: True for every session, or a set of session ids whose
+    positions after every step are kept.  The result reports each solve's
+    error bound (mprgp_stats.error_bound: |x - x*| <= residual / lambda_min
+    for the session's mass-dominated matrix)."""
     rng = np.random.default_rng(seed)
     n_req = max(1, int(rng.poisson(rate * duration)))
     arr = np.sort(rng.uniform(0.0, duration, n_req))
     who = rng.integers(0, len(sessions), n_req)
     lock = threading.Lock()
     tickets = {}          # ticket -> (session, arrival)
-    lat, queue_ms, codes, iters = [], [], [], []
+    lat, queue_ms, codes, iters, bounds = [], [], [], [], []
     pool = cf.ThreadPoolExecutor(max_workers=workers)
     errors = []
 
@@ -232,7 +240,8 @@ def run_open_loop(engine: M.Engine, sessions, rate=2000.0, duration=3.0, tol=1e-
     def finish(s: Session, rc, st):
         try:
             t1 = time.perf_counter()
-            s.complete(keep_history, rc, st)
+            keep = keep_history if isinstance(keep_history, bool) else s.sid in keep_history
+            s.complete(keep, rc, st)
             host["complete"] += time.perf_counter() - t1
             with lock:
                 again = bool(s.pending)
@@ -277,6 +286,7 @@ def run_open_loop(engine: M.Engine, sessions, rate=2000.0, duration=3.0, tol=1e-
         queue_ms.append(st["queue_ms"])
         codes.append(rc)
         iters.append(st["iterations"])
+        bounds.append(st["error_bound"])
         done += 1
         pool.submit(finish, s, rc, st)
     wall = time.perf_counter() - t0
@@ -291,9 +301,47 @@ def run_open_loop(engine: M.Engine, sessions, rate=2000.0, duration=3.0, tol=1e-
             "queue_p99_ms": round(float(np.percentile(queue_ms, 99)), 3),
             "converged": int(sum(1 for c in codes if c == 0)),
             "iterations_mean": round(float(np.mean(iters)), 1),
+            "error_bound_m": {"max": float(np.max(bounds)), "median": float(np.median(bounds)),
+                              "reported": int(sum(1 for b in bounds if b >= 0.0))},
             "host_prepare_ms": round(1e3 * host["prepare"] / n_req, 3),
             "host_complete_ms": round(1e3 * host["complete"] / n_req, 3),
             "engine": engine.info()}
+
+
+def fp64_error_sample(engine: M.Engine, sessions, n=16, tol=1e-6, seed=8, precision=M.MIXED,
+                      ref_tol=1e-10, max_it=1000):
+    """The error of served solves against FP64 solves of the same QPs.
+    For `n` sessions drawn at random, one step each: the engine's solve in
+    `precision`, then, from the same warm start, the plain FP64 solve to
+    `ref_tol` (the reference solution); the session then continues from the
+    served solution.  Reports the relative and absolute errors and checks the
+    served solve's error bound against the measured error (both solutions
+    are within their bounds of the exact minimiser, so their distance is at
+    most the sum of the two bounds)."""
+    rng = np.random.default_rng(seed)
+    picks = rng.choice(len(sessions), size=min(n, len(sessions)), replace=False)
+    rel, ab, bound_ok, bounds = [], [], [], []
+    for i in picks:
+        s = sessions[int(i)]
+        s.prepare()
+        p = solve_params(s, tol, precision, max_it=max_it)
+        rc, st = engine.wait(engine.submit(s.qp, p))
+        x = s.qp.get_x()
+        s.qp.reset_x()
+        pr = solve_params(s, ref_tol, M.FP64, max_it=20 * max_it)
+        pr.alpha_bar = st["alpha_bar"]
+        rc64, st64 = s.qp.solve(pr)
+        x64 = s.qp.get_x()
+        s.qp.set_x(x)  # the session continues from the served solution
+        d = float(np.linalg.norm(x - x64))
+        rel.append(d / float(np.linalg.norm(x64)))
+        ab.append(d)
+        bounds.append(st["error_bound"])
+        bound_ok.append(st["error_bound"] >= 0.0 and d <= st["error_bound"] + max(st64["error_bound"], 0.0))
+        s.complete(rc=rc, stats=st)
+    return {"sample": len(picks), "rel_err_max": float(np.max(rel)), "rel_err_median": float(np.median(rel)),
+            "abs_err_max_m": float(np.max(ab)), "error_bound_max_m": float(np.max(bounds)),
+            "bound_holds": bool(all(bound_ok)), "reference": f"plain FP64 solve to {ref_tol:g}"}
 
 
 def run_one_at_a_time(sessions, n_req=200, tol=1e-6, seed=6, max_it=1000):

@@ -109,3 +109,24 @@ def test_rhs_only_is_the_kept_linearisation(ctx, mtype):
     expect = b0 + m3 * (x1 - x0).ravel()
     err = np.abs(b1 - expect).max() / np.abs(b1).max()
     assert err <= 1e-12, err
+
+
+def test_assembled_matrix_downloads_as_the_oracle_system(ctx):
+    """mprgp_matrix_get_bsr3 returns the device-assembled matrix in the
+    oracle's pattern and values (entrywise within 1e-12 of the row scale),
+    and a matrix uploaded from what it returns multiplies bit for bit as the
+    original; with mprgp_write_qp that is the simulator's per-frame QP dump
+    (saveQP, MprgpFemSolver.h:129-133)."""
+    sc = problems.config1()
+    fem, q, A, b = check_system(ctx, sc, deformed(sc, 4), M.COROTATIONAL)
+    Ad = fem.matrix.to_csr()
+    Ad.sort_indices()
+    Ar = A.tocsr()
+    Ar.sort_indices()
+    assert np.array_equal(Ad.indptr, Ar.indptr) and np.array_equal(Ad.indices, Ar.indices)
+    row_scale = np.repeat(np.abs(Ar).max(axis=1).toarray().ravel(), np.diff(Ar.indptr))
+    assert np.all(np.abs(Ad.data - Ar.data) <= 1e-12 * row_scale)
+    rp, ci, bl = fem.matrix.get_bsr3()
+    again = ctx.matrix_bsr3(sc.n_nodes, rp, ci, bl.reshape(-1))
+    x = np.random.default_rng(5).normal(size=3 * sc.n_nodes)
+    np.testing.assert_array_equal(again.spmv(x), fem.matrix.spmv(x))

@@ -131,3 +131,47 @@ def test_streamed_and_cached_variants_agree_bitwise(ctx, monkeypatch):
         for a, b in zip(out["0"][0], out["1e18"][0]):
             np.testing.assert_array_equal(a, b)
         assert out["0"][1] == out["1e18"][1]
+
+
+@pytest.mark.parametrize("which", ["config1", "config3"])
+def test_error_bound_of_every_solve(ctx, which):
+    """mprgp_stats.error_bound = residual / lambda_min_lower, with the bound
+    lambda_min(A) >= min lumped mass that the device FEM assembly records:
+    the mixed solve's distance from the oracle's solution (to 1e-11) is
+    within its bound plus the oracle's, and a matrix without a known bound
+    reports -1."""
+    from mprgp_amd import workloads
+    sc = getattr(problems, which)()
+    dq = workloads.DeviceQP(ctx, sc)
+    tets, d, vol, mass = oracle.fem_setup(sc.n_nodes, sc.tets, sc.X0)
+    A, b = oracle.fem_build_system(sc.n_nodes, tets, d, vol, mass, sc.young, sc.poisson, sc.density,
+                                   sc.x_n.ravel(), sc.v_n.ravel(), sc.gravity, sc.h, sc.material)
+    lam_lo = float(np.min(mass)) * sc.density
+    node, normal, c = sc.rows()
+    J = problems.rows_to_J(node, normal, sc.n_nodes)
+    ab = 2.0 / oracle.spec_rad(A, 1e-10)[0]
+    code, xr, sr, _ = oracle.decoupled_solve(A, b, J, c, sc.x_n.ravel(), tol=1e-11, max_it=50000,
+                                             alpha_bar=ab, project_first=True)
+    assert code == 0
+    # the bound really is below A's smallest eigenvalue
+    assert lam_lo <= np.linalg.eigvalsh(A.toarray())[0] * (1 + 1e-12)
+    for tol in (1e-4, 1e-6, 1e-8):
+        dq.qp.reset_x()
+        rc, st = dq.qp.solve(tol=tol, max_it=50000, alpha_bar=ab, precision=M.MIXED, project_first=1)
+        assert rc == 0
+        err = float(np.linalg.norm(dq.qp.get_x() - xr))
+        print(f"{which} tol {tol:g}: |x - x_oracle| {err:.2e}, error_bound {st['error_bound']:.2e} "
+              f"(residual {st['residual']:.2e} / lambda_min >= {lam_lo:.3e})")
+        assert st["error_bound"] == pytest.approx(st["residual"] / lam_lo, rel=1e-12)
+        assert err <= st["error_bound"] + 1e-11 / lam_lo
+    Ad = ctx.matrix_csr(A)  # uploaded as plain arrays: no bound known
+    q = Ad.qp()
+    q.set_rows(node, normal, c)
+    q.set_rhs(b)
+    q.set_x(sc.x_n.ravel())
+    rc, st = q.solve(tol=1e-6, max_it=50000, alpha_bar=ab, precision=M.MIXED, project_first=1)
+    assert rc == 0 and st["error_bound"] == -1.0
+    Ad.set_lambda_min(lam_lo)
+    q.reset_x()
+    rc, st = q.solve(tol=1e-6, max_it=50000, alpha_bar=ab, precision=M.MIXED, project_first=1)
+    assert st["error_bound"] == pytest.approx(st["residual"] / lam_lo, rel=1e-12)

@@ -161,3 +161,58 @@ def test_device_sessions_equal_host_sessions(ctx):
             np.testing.assert_array_equal(sd.history[-1], sh.history[-1], err_msg=f"{sd.sid} {step}")
         xd, vd = sd.fem.get_state()
         np.testing.assert_array_equal(vd.reshape(-1, 3), sh.scene.v_n)
+
+
+def test_config5_full_scale(ctx):
+    """The serving workload at its stated scale: 64 sessions (beams and
+    blobs, DoF log-uniform in [3k, 150k], each on its own context),
+    corotated re-assembly every 10 steps, a 200 GB HBM budget, Poisson
+    arrivals at 2000 requests/s for 1 s, mixed form, the simulator's
+    iteration cap (MprgpFemSolver.cpp:189).  Every request converges, p50/p99
+    are reported with an error bound per solve, and the 4 largest sessions,
+    replayed alone step by step on the same engine, reproduce their positions
+    in the mix bit for bit."""
+    ss = serving.make_sessions(ctx, count=64, seed=5, dof_range=(3e3, 1.5e5), reassemble=10,
+                               own_contexts=True)
+    dof = [3 * s.n_nodes for s in ss]
+    big = set(int(i) for i in np.argsort(dof)[-4:])
+    eng = M.Engine(ctx, max_active=256, tiles=4, hbm_budget=int(200e9))
+    out = serving.run_open_loop(eng, ss, rate=2000.0, duration=1.0, precision=M.MIXED, seed=16,
+                                keep_history=big)
+    print({k: v for k, v in out.items() if k != "engine"}, f"DoF {min(dof)}..{max(dof)} total {sum(dof)}")
+    assert min(dof) >= 3000 and max(dof) <= 150000 * 1.05 and max(dof) >= 60000
+    assert out["requests"] >= 1500
+    assert out["converged"] == out["requests"]
+    assert out["p99_ms"] >= out["p50_ms"] > 0
+    assert out["error_bound_m"]["reported"] == out["requests"]
+    assert out["engine"]["peak_bytes"] <= 200e9
+    solo = serving.make_sessions(ctx, count=64, seed=5, dof_range=(3e3, 1.5e5), reassemble=10,
+                                 own_contexts=True, only=big)
+    for s in solo:
+        mixed_run = ss[s.sid]
+        assert 3 * s.n_nodes == dof[s.sid] and len(mixed_run.history) >= 5
+        for k in range(len(mixed_run.history)):
+            s.prepare()
+            rc, st = eng.wait(eng.submit(s.qp, serving.solve_params(s, 1e-6, M.MIXED)))
+            assert rc == 0, st
+            s.complete(keep_history=True, rc=rc, stats=st)
+            np.testing.assert_array_equal(s.history[k], mixed_run.history[k],
+                                          err_msg=f"session {s.sid} ({dof[s.sid]} DoF) step {k}")
+    eng.close()
+
+
+def test_fp64_error_sample(ctx):
+    """Served mixed solves against FP64 solves of the same QPs to 1e-10: the
+    per-solve error bound holds and the error is small."""
+    ss = sessions(ctx, seed=17, count=6, dof_range=(3e3, 3e4))
+    eng = M.Engine(ctx, max_active=8, tiles=4)
+    for _ in range(3):  # a few steps in
+        for s in ss:
+            s.prepare()
+            rc, st = eng.wait(eng.submit(s.qp, serving.solve_params(s, 1e-6, M.MIXED)))
+            s.complete(rc=rc, stats=st)
+    out = serving.fp64_error_sample(eng, ss, n=6)
+    print(out)
+    assert out["bound_holds"] and out["error_bound_max_m"] > 0
+    assert out["rel_err_max"] <= 1e-4
+    eng.close()
