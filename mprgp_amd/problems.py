@@ -132,6 +132,45 @@ def project_rows(x, node, normal, c):
     return y.ravel()
 
 
+def self_contact_rows(scene, rng, m, gap=0.0):
+    """m coupled rows of 12 nonzeros in the reference's self-contact form
+    (SurfaceSelfConCache, LinearConCollider.cpp:74-126 and :186-217): a vertex
+    i against the triangle (j, k, l) of one tet face, unit normal n of that
+    face oriented towards i, barycentric weights (a, b, c):
+    n.(x_i - a x_j - b x_k - c x_l) >= rhs, with rhs the row's value at the
+    scene's current positions minus `gap` (so the warm start is feasible and
+    the dynamics press on about half of the rows).  Distinct vertices, and no
+    vertex also in another row's triangle.  Returns (J as an m x 3N CSR, rhs)."""
+    import scipy.sparse as sp
+    x = scene.x_n
+    N = scene.n_nodes
+    used = np.zeros(N, bool)
+    rows, cols, vals, rhs = [], [], [], []
+    faces = [(0, 1, 2, 3), (0, 1, 3, 2), (0, 2, 3, 1), (1, 2, 3, 0)]
+    for t in rng.permutation(len(scene.tets)):
+        if len(rhs) == m:
+            break
+        j, k, l, i = (int(scene.tets[t][q]) for q in faces[int(rng.integers(0, 4))])
+        if used[[i, j, k, l]].any():
+            continue
+        nrm = np.cross(x[k] - x[j], x[l] - x[j])
+        nrm /= np.linalg.norm(nrm)
+        if nrm @ (x[i] - x[j]) < 0.0:
+            nrm = -nrm
+        w = rng.dirichlet([3.0, 3.0, 3.0])
+        r = len(rhs)
+        for node, coef in ((i, 1.0), (j, -w[0]), (k, -w[1]), (l, -w[2])):
+            for d in range(3):
+                rows.append(r)
+                cols.append(3 * node + d)
+                vals.append(coef * nrm[d])
+        rhs.append(float(nrm @ (x[i] - w[0] * x[j] - w[1] * x[k] - w[2] * x[l])) - gap)
+        used[[i, j, k, l]] = True
+    J = sp.csr_matrix((vals, (rows, cols)), shape=(len(rhs), 3 * N))
+    J.sort_indices()
+    return J, np.array(rhs)
+
+
 @dataclasses.dataclass
 class Scene:
     """One body on planes: everything buildLinearSystem + the collider need."""

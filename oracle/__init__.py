@@ -88,6 +88,22 @@ def lib():
                                           D, P, P, I, C.POINTER(Stats)]
         L.orc_decoupled_solve_ex.argtypes = [C.POINTER(_CSR), P, C.POINTER(_CSR), P, P, D, I, I, I,
                                              D, P, P, I, C.POINTER(_Options), C.POINTER(Stats)]
+        L.orc_qp_nonneg.argtypes = [I, P, P, P]
+        L.orc_gprojector_new.argtypes = [C.POINTER(_CSR), P]
+        L.orc_gprojector_new.restype = P
+        L.orc_gprojector_free.argtypes = [P]
+        L.orc_gproject.argtypes = [P, P, P]
+        L.orc_gdecide_face.argtypes = [P, P]
+        L.orc_gis_feasible.argtypes = [P, P]
+        L.orc_gphi.argtypes = [P, P, P]
+        L.orc_gbeta.argtypes = [P, P, P, P]
+        L.orc_gphitphi.argtypes = [P, P, D, P]
+        L.orc_gphitphi.restype = D
+        L.orc_gface.argtypes = [P]
+        L.orc_gface.restype = C.POINTER(C.c_char)
+        L.orc_gnum_active.argtypes = [P]
+        L.orc_general_solve.argtypes = [C.POINTER(_CSR), P, C.POINTER(_CSR), P, P, D, I, I, D, P, P, I,
+                                        C.POINTER(Stats)]
         L.orc_fem_setup.argtypes = [I, I, P, P, P, P, P]
         L.orc_polar_rotation.argtypes = [P, P]
         L.orc_fem_element.argtypes = [I, P, P, D, D, D, P, P]
@@ -304,6 +320,82 @@ def decoupled_solve_traced(A, B, J, c, x, tol=1e-3, max_it=1000, preconditioned=
 def step_sequence(events):
     """The step types (STEP_*) of a traced solve, in order."""
     return events["outcome"][events["kind"] == EV_STEP].copy()
+
+
+# ---------------------------------------------------------- coupled rows
+
+def qp_nonneg(G, h):
+    """argmin 1/2 l'Gl - h'l s.t. l >= 0 (the reference's solveQP with J = I)."""
+    G = np.ascontiguousarray(G, np.float64)
+    h = _f64(h)
+    lam = np.zeros(len(h))
+    rc = lib().orc_qp_nonneg(len(h), _ptr(G), _ptr(h), _ptr(lam))
+    return rc, lam
+
+
+class GeneralProjector:
+    """GeneralConProjector (MPRGPProjection.h:400-520) over rows J x >= c."""
+
+    def __init__(self, J, c):
+        self._j = _Csr(J)
+        self.n = J.shape[1]
+        self.m = J.shape[0]
+        self._c = _f64(c).copy()
+        self.h = lib().orc_gprojector_new(C.byref(self._j.s), _ptr(self._c))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_gprojector_free(self.h)
+            self.h = None
+
+    def project(self, x):
+        y = np.zeros(self.n)
+        rc = lib().orc_gproject(self.h, _ptr(_f64(x)), _ptr(y))
+        return y if rc == 0 else None
+
+    def decide_face(self, x):
+        return lib().orc_gdecide_face(self.h, _ptr(_f64(x)))
+
+    def face(self):
+        f = lib().orc_gface(self.h)
+        return np.array([f[i][0] for i in range(self.m)], dtype=np.int8) if self.m else np.zeros(0, np.int8)
+
+    def num_active(self):
+        return lib().orc_gnum_active(self.h)
+
+    def is_feasible(self, x):
+        return bool(lib().orc_gis_feasible(self.h, _ptr(_f64(x))))
+
+    def phi(self, g):
+        out = np.zeros(self.n)
+        lib().orc_gphi(self.h, _ptr(_f64(g)), _ptr(out))
+        return out
+
+    def beta(self, g, phi):
+        out = np.zeros(self.n)
+        lib().orc_gbeta(self.h, _ptr(_f64(g)), _ptr(_f64(phi)), _ptr(out))
+        return out
+
+    def phitphi(self, x, alpha_bar, phi):
+        return lib().orc_gphitphi(self.h, _ptr(_f64(x)), float(alpha_bar), _ptr(_f64(phi)))
+
+
+def general_solve(A, B, J, c, x, tol=1e-3, max_it=1000, preconditioned=True, alpha_bar=0.0, ev=None,
+                  project_first=True):
+    """MPRGPGeneralCon<double>::solve restated.  Returns (code, x, stats-dict)."""
+    a = _Csr(A)
+    j = _Csr(J, cols=A.shape[0])
+    B = _f64(B)
+    c = _f64(c)
+    x = _f64(x).copy()
+    n = A.shape[0]
+    evb = np.zeros(n) if ev is None else _f64(ev).copy()
+    valid = C.c_int(0 if ev is None else 1)
+    st = Stats()
+    code = lib().orc_general_solve(C.byref(a.s), _ptr(B), C.byref(j.s), _ptr(c), _ptr(x), float(tol),
+                                   int(max_it), int(preconditioned), float(alpha_bar), _ptr(evb),
+                                   C.byref(valid), int(project_first), C.byref(st))
+    return code, x, st.as_dict()
 
 
 # ------------------------------------------------------------------ FEM

@@ -141,6 +141,29 @@ int orc_decoupled_solve_ex(const orc_csr *A, const double *B, const orc_csr *J, 
                            double alpha_bar, double *ev, int *ev_valid, int project_first,
                            orc_options *opt, orc_stats *st);
 
+/* ---- coupled rows (general_oracle.c) ----------------------------------- */
+/* GeneralConProjector (MPRGPProjection.h:400-520) for rows spanning several
+ * nodes, and MPRGPGeneralCon<double>::solve (MPRGPSolver.h:594-626). */
+typedef struct orc_gprojector orc_gprojector;
+/* exact argmin 1/2 l'Gl - h'l s.t. l >= 0 (G m x m row-major SPD); 0 or -1 */
+int orc_qp_nonneg(int m, const double *G, const double *h, double *lam);
+orc_gprojector *orc_gprojector_new(const orc_csr *J, const double *c);
+void orc_gprojector_free(orc_gprojector *P);
+int orc_gproject(const orc_gprojector *P, const double *x, double *y);
+int orc_gdecide_face(orc_gprojector *P, const double *x);
+int orc_gis_feasible(const orc_gprojector *P, const double *x);
+void orc_gphi(const orc_gprojector *P, const double *g, double *phi);
+int orc_gbeta(const orc_gprojector *P, const double *g, const double *phi, double *beta);
+double orc_gphitphi(const orc_gprojector *P, const double *x, double alpha_bar, const double *phi);
+const char *orc_gface(const orc_gprojector *P);
+int orc_gnum_active(const orc_gprojector *P);
+/* J/c overload: x projected first when project_first; the monotone variant;
+ * preconditioned = DiagonalInFacePreconSolver (z = phi D^-1), else z = phi.
+ * Returns 0, -1 (not converged) or -2 (a projection QP failed). */
+int orc_general_solve(const orc_csr *A, const double *B, const orc_csr *J, const double *c, double *x, double tol,
+                      int max_it, int preconditioned, double alpha_bar, double *ev, int *ev_valid,
+                      int project_first, orc_stats *st);
+
 /* ---- FEM assembly (fem_oracle.c) -------------------------------------- */
 enum { ORC_LINEAR = 1, ORC_COROTATIONAL = 2 };
 

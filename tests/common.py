@@ -94,3 +94,8 @@ def truncated_counts(seq, k):
     else:
         steps = seq[:k]
     return tuple(sum(1 for s in steps if s == t) for t in (CG, EXP, PROP))
+
+
+def coupled_rows(scene, rng, m):
+    """Self-contact rows on a scene (problems.self_contact_rows)."""
+    return problems.self_contact_rows(scene, rng, m)

@@ -14,6 +14,9 @@
 // Matrices: per QP a chain of n nodes, A = (1 + h^2 k) diag blocks with
 // -h^2 k/2 couplings to both neighbours (SPD, diagonally dominant); one
 // half-space row y >= 0 on every 7th node; b pulls the chain downwards.
+// alpha_bar = 2/(1 + 2k), the Gershgorin bound on |A| (the power method's
+// estimate from below converges slowly on a long chain's clustered top
+// eigenvalues).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -30,10 +33,13 @@
     }                                                                            \
   } while (0)
 
+constexpr double kChain = 40.0;
+constexpr double kAlphaBar = 2.0 / (1.0 + 2.0 * kChain);
+
 static mprgp_qp *chain_qp(mprgp_ctx *ctx, int n, mprgp_matrix **Aout) {
   std::vector<int> rowptr(n + 1), col;
   std::vector<double> blocks;
-  const double k = 40.0;
+  const double k = kChain;
   for (int i = 0; i < n; ++i) {
     rowptr[i] = (int)col.size();
     for (int j = i - 1; j <= i + 1; ++j) {
@@ -91,6 +97,7 @@ int main() {
           mprgp_params_default(&p[i]);
           p[i].tol = 1e-6;
           p[i].max_it = 20000;
+          p[i].alpha_bar = kAlphaBar;
           p[i].precision = prec ? MPRGP_MIXED : MPRGP_FP64;
           p[i].project_first = 1;
           p[i].tiles = sh[0];
@@ -98,7 +105,14 @@ int main() {
           CHECK(mprgp_qp_reset_x(qps[i]));
         }
         // all QPs, then every other one (a smaller table with the same graphs)
-        CHECK(mprgp_solve_batch(ctx, (int)qps.size(), qps.data(), p.data(), 1, st.data()));
+        const int rc = mprgp_solve_batch(ctx, (int)qps.size(), qps.data(), p.data(), 1, st.data());
+        if (rc != MPRGP_OK) {
+          for (size_t i = 0; i < qps.size(); ++i)
+            std::fprintf(stderr, "qp %zu (%d nodes) shape %d/%d %s: code %d, %d iterations, residual %g\n", i,
+                         sizes[i], sh[0], sh[1], prec ? "mixed" : "fp64", st[i].code, st[i].iterations,
+                         st[i].residual);
+          CHECK(rc);
+        }
         std::vector<mprgp_qp *> half;
         std::vector<mprgp_params> ph;
         for (size_t i = 0; i < qps.size(); i += 2) {
@@ -115,6 +129,7 @@ int main() {
     mprgp_params_default(&p);
     p.tol = 1e-6;
     p.max_it = 20000;
+    p.alpha_bar = kAlphaBar;
     p.graph_steps = g;
     CHECK(mprgp_qp_reset_x(qps[3]));
     CHECK(mprgp_qp_solve(qps[3], &p, &st[0]));
@@ -135,6 +150,7 @@ int main() {
         mprgp_params_default(&p);
         p.tol = 1e-6;
         p.max_it = 20000;
+        p.alpha_bar = kAlphaBar;
         p.precision = rep ? MPRGP_MIXED : MPRGP_FP64;
         CHECK(mprgp_qp_reset_x(q));
         long long t = 0;
