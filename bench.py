@@ -69,6 +69,9 @@ def pmc_child(args):
 
 
 COPY_BYTES = 1 << 28
+# Read requests from the L2 to the fabric by size (32, 64, 128 bytes), then
+# all of them: four TCC counters, one rocprofv3 pass.
+RDREQ_COUNTERS = ("TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum", "TCC_EA0_RDREQ_sum")
 
 
 def measure_traffic(args, tiles, what="solve"):
@@ -89,7 +92,7 @@ def measure_traffic(args, tiles, what="solve"):
 
     def run(counter, child, tag):
         d = os.path.join(tmp, tag)
-        cmd = ["timeout", "-s", "KILL", "300", rocprof, "--pmc", counter, "--output-format", "csv",
+        cmd = ["timeout", "-s", "KILL", "300", rocprof, "--pmc", *counter.split(","), "--output-format", "csv",
                "-d", d, "-o", "run", "--", sys.executable, "-u", os.path.abspath(__file__),
                "--pmc-child", child, "--qps", str(args.qps), "--tol", str(args.tol),
                "--precision", args.precision, "--ts", str(tiles)]
@@ -101,7 +104,7 @@ def measure_traffic(args, tiles, what="solve"):
         for line in r.stdout.splitlines():
             if line.startswith("{"):
                 launches = json.loads(line)["launches"]
-        vals = []
+        vals = {}
         for root, _, files in os.walk(d):
             for fn in files:
                 if fn.endswith("counter_collection.csv"):
@@ -109,23 +112,54 @@ def measure_traffic(args, tiles, what="solve"):
                         for row in csv.DictReader(f):
                             name = row["Kernel_Name"]
                             if (child == "copy" and "copy_stream" in name) or (child != "copy" and kernel in name):
-                                vals.append(float(row["Counter_Value"]))
-        if child != "copy" and launches:
-            vals = vals[-launches:]  # the measured call's launches (the solve's power method runs first)
-        return sum(vals) / len(vals) * 1024.0 if vals else None  # KiB -> bytes
+                                vals.setdefault(row["Counter_Name"], []).append(float(row["Counter_Value"]))
+        out = {}
+        for k, v in vals.items():
+            if child != "copy" and launches:
+                v = v[-launches:]  # the measured call's launches (the solve's power method runs first)
+            out[k] = sum(v) / len(v)
+        return out
+
+    def size(counter, child, tag):  # a *_SIZE counter, KiB per dispatch -> bytes
+        v = run(counter, child, tag)
+        return 1024.0 * v[counter] if counter in v else None
+
+    def by_request_size(child, tag):
+        """Read bytes per dispatch from the read requests counted by size
+        (32, 64 and 128 bytes), an estimate that needs no calibration."""
+        v = run(",".join(RDREQ_COUNTERS), child, tag)
+        if not all(c in v for c in RDREQ_COUNTERS):
+            log(f"request-size counters missing for {child}: got {sorted(v)}")
+            return None
+        b32, b64, b128, total = (v[c] for c in RDREQ_COUNTERS)
+        return {"bytes": int(32 * b32 + 64 * b64 + 128 * b128), "requests": int(total),
+                "requests_by_size": [int(b32), int(b64), int(b128)]}
 
     try:
-        cal_f = COPY_BYTES / run("FETCH_SIZE", "copy", "cf")
-        cal_w = COPY_BYTES / run("WRITE_SIZE", "copy", "cw")
-        fetch = run("FETCH_SIZE", what, "sf") * cal_f
-        write = run("WRITE_SIZE", what, "sw") * cal_w
+        cal_f = COPY_BYTES / size("FETCH_SIZE", "copy", "cf")
+        cal_w = COPY_BYTES / size("WRITE_SIZE", "copy", "cw")
+        fetch = size("FETCH_SIZE", what, "sf") * cal_f
+        write = size("WRITE_SIZE", what, "sw") * cal_w
     except (RuntimeError, TypeError, ZeroDivisionError) as e:
         log(f"traffic measurement skipped: {e}")
+        shutil.rmtree(tmp, ignore_errors=True)
         return None
+    out = {"fetch_bytes": int(fetch), "write_bytes": int(write), "fetch_scale": round(cal_f, 4),
+           "write_scale": round(cal_w, 4)}
+    try:  # independent check of the calibrated read bytes
+        copy_rq = by_request_size("copy", "cq")
+        rq = by_request_size(what, "sq")
+        if copy_rq and rq:
+            out["fetch_by_request_size"] = {
+                "bytes": rq["bytes"], "vs_calibrated_fetch": round(rq["bytes"] / fetch, 4),
+                "requests_by_size_32_64_128": rq["requests_by_size"],
+                "unsized_requests": rq["requests"] - sum(rq["requests_by_size"]),
+                "copy_probe_bytes_vs_true": round(copy_rq["bytes"] / COPY_BYTES, 4)}
+    except RuntimeError as e:
+        log(f"request-size pass skipped: {e}")
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
-    return {"fetch_bytes": int(fetch), "write_bytes": int(write), "fetch_scale": round(cal_f, 4),
-            "write_scale": round(cal_w, 4)}
+    return out
 
 
 def host_info():
@@ -330,6 +364,7 @@ def main():
         times.append(time.perf_counter() - s0)
     total = time.perf_counter() - tt0
     conv = sum(1 for s in stats if s["code"] == 0)
+    finished64 = sum(s["fp64_finish"] for s in stats)
     value = args.qps * args.steps / total
     log(f"batched {args.precision}: {value:.1f} solves/s, {1e3 * total / args.steps:.2f} ms/step, "
         f"converged {conv}/{args.qps}")
@@ -534,7 +569,7 @@ def main():
                    "launch_shape": shape,
                    "dof_total": 3 * total_nodes, "blocks_total": total_blocks,
                    "dof_range": [5000, 60000], "tol": args.tol, "precision": args.precision,
-                   "converged": conv},
+                   "converged": conv, "fp64_finish": finished64},
         "batch_p50_ms": round(1e3 * float(np.percentile(times, 50)), 3),
         "batch_p99_ms": round(1e3 * float(np.percentile(times, 99)), 3),
         "batched_fp64": other,

@@ -17,6 +17,8 @@
  *   FEMSystem::buildSystem               src/ElasticSolver/FEMSystem.cpp:167-192
  *   MaterialEnergy::eval (LINEAR/COROT)  src/ElasticSolver/FEMEnergy.cpp:263-304, 657-1057
  *   LinearConCollider::handle (planes)   src/CollisionHandle/LinearConCollider.cpp:140-161
+ *   MPRGPGeneralCon / GeneralConProjector src/MPRGPSolver/MPRGPSolver.h:594-626,
+ *                                        src/MPRGPSolver/MPRGPProjection.h:400-520
  *
  * Threading: a context owns one HIP stream; calls that use it are serialised
  * by the context (several threads may share a context).  An engine runs its
@@ -80,7 +82,9 @@ typedef struct {
   int tiles;           /* SELL slices per workgroup (1,2,4,8); 0 = autotuned         */
   int graph_steps;     /* launches per captured hipGraph; 0 = autotuned              */
   int max_refine;      /* MIXED: outer refinement sweeps (0 = default 8)             */
-  int reserved_i[5];   /* set to 0                                                   */
+  int tail_tiles;      /* launches of at most this many tiles run the step kernel's  */
+                       /* tail form (same results); 0 = 256, one per CU              */
+  int reserved_i[4];   /* set to 0                                                   */
   double inner_tol;    /* MIXED: inner tolerance factor (0 = default 1e-3 relative)  */
   double reserved_d[3];
 } mprgp_params;
@@ -94,7 +98,11 @@ typedef struct {
   int ev_valid;       /* power vector stored for the next solve                     */
   int refinements;    /* MIXED: outer refinement sweeps                             */
   int steps;          /* device launches this QP took part in                       */
-  int reserved_i[2];  /* [0]: step-kernel launches issued for the whole call        */
+  int launches;       /* step-kernel launches issued for the whole call             */
+  int fp64_finish;    /* MIXED: 1 when every refinement sweep ran and the residual  */
+                      /* was still above tol (the FP32 inner solve's floor), so the */
+                      /* FP64 form finished the solve from that x; then iterations  */
+                      /* and the step counts cover both                             */
   double residual;    /* final ||phi + beta||_2 (FP64)                              */
   double alpha_bar;
   double lambda_max;  /* spectral-radius estimate behind alpha_bar                  */
@@ -221,6 +229,23 @@ int mprgp_qp_solve(mprgp_qp *qp, const mprgp_params *params, mprgp_stats *stats)
  * converged, MPRGP_NOT_CONVERGED if any did not, an error otherwise. */
 int mprgp_solve_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprgp_params *params,
                       int params_stride, mprgp_stats *stats);
+
+/* ---- coupled rows -----------------------------------------------------------
+ * MPRGPGeneralCon<double>::solve<FixedSparseMatrix, preconditioned>(A, B, J,
+ * c, x, tol, max_it) (MPRGPSolver.h:611-623) with GeneralConProjector
+ * (MPRGPProjection.h:400-520): rows J x >= c that may span several nodes,
+ * such as the self-contact rows of LinearConCollider.cpp:186-217 (a vertex
+ * against a triangle, 12 nonzeros); J is m x 3*nodes CSR in user DoF order,
+ * m <= 2048, with J J^T nonsingular on every set of rows a projection
+ * frees.  Solves the QP's matrix, right-hand side and x (projected first when
+ * params->project_first; the QP's own per-node rows are not used); FP64,
+ * monotone variant; params->preconditioned selects the reference's
+ * DiagonalInFacePreconSolver (z = D^-1 phi).  alpha_bar <= 0: the power
+ * method from the QP's stored ev (params->use_ev) or the start vector.
+ * Returns 0, MPRGP_NOT_CONVERGED, or MPRGP_EBADINPUT when a projection's QP
+ * or the active rows' factorisation fails.  stats->error_bound is -1. */
+int mprgp_general_solve(mprgp_qp *qp, int m, const int *j_ptr, const int *j_col, const double *j_val,
+                        const double *c, const mprgp_params *params, mprgp_stats *stats);
 
 /* ---- one-shot drop-in ------------------------------------------------------
  * MPRGPDecoupledCon<double>::solve<FixedSparseMatrix<double>,true>(A, B, J, c,
@@ -375,7 +400,7 @@ int mprgp_engine_create(mprgp_ctx *ctx, const mprgp_engine_config *cfg, mprgp_en
 int mprgp_engine_submit(mprgp_engine *eng, mprgp_qp *qp, const mprgp_params *params, long long *ticket);
 /* Waits up to timeout_ms (< 0: forever) for a ticket; returns its solve code
  * (0, -1, or an error) and fills stats (elapsed_ms = submit-to-completion
- * latency, queue_ms = queue wait, reserved_i[0] = launches while admitted), or MPRGP_EPENDING on time-out.  A ticket is waited for once. */
+ * latency, queue_ms = queue wait, launches = launches while admitted), or MPRGP_EPENDING on time-out.  A ticket is waited for once. */
 int mprgp_engine_wait(mprgp_engine *eng, long long ticket, int timeout_ms, mprgp_stats *stats);
 /* As mprgp_engine_wait for whichever ticket completed first (in completion
  * order); *ticket receives it.  MPRGP_ESTATE if no ticket is outstanding. */

@@ -435,6 +435,22 @@ class QP:
         return rc, st.as_dict()
 
 
+def general_solve(qp, J, c, p: Params | None = None, **kw):
+    """MPRGPGeneralCon::solve on a QP's matrix, b and x with coupled rows
+    J x >= c (J: scipy sparse, m x 3*nodes, user DoF order).  Returns (rc, stats)."""
+    import scipy.sparse as sp
+    p = params(**kw) if p is None else p
+    Jc = sp.csr_matrix(J)
+    Jc.sort_indices()
+    jp, jc_, jv = _i32(Jc.indptr), _i32(Jc.indices), _f64(Jc.data)
+    c = _f64(c, Jc.shape[0])
+    st = Stats()
+    rc = load().mprgp_general_solve(qp.h, int(Jc.shape[0]), _p(jp), _p(jc_), _p(jv), _p(c), C.byref(p),
+                                    C.byref(st))
+    check(rc, allow=(_lib.OK, _lib.NOT_CONVERGED, _lib.ENAN))
+    return rc, st.as_dict()
+
+
 def solve_batch(ctx: Context, qps, p=None, per_qp=None, **kw):
     """Advances all QPs together; returns (rc, [stats dicts])."""
     n = len(qps)

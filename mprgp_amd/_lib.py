@@ -36,7 +36,7 @@ class Params(C.Structure):
                 ("precision", C.c_int), ("preconditioned", C.c_int), ("project_first", C.c_int),
                 ("use_ev", C.c_int), ("alpha_bar", C.c_double), ("power_tol", C.c_double),
                 ("tiles", C.c_int), ("graph_steps", C.c_int), ("max_refine", C.c_int),
-                ("reserved_i", C.c_int * 5), ("inner_tol", C.c_double),
+                ("tail_tiles", C.c_int), ("reserved_i", C.c_int * 4), ("inner_tol", C.c_double),
                 ("reserved_d", C.c_double * 3)]
 
 
@@ -44,15 +44,13 @@ class Stats(C.Structure):
     _fields_ = [("code", C.c_int), ("iterations", C.c_int), ("num_cg", C.c_int),
                 ("num_exp", C.c_int), ("num_prop", C.c_int), ("active", C.c_int),
                 ("power_iters", C.c_int), ("ev_valid", C.c_int), ("refinements", C.c_int),
-                ("steps", C.c_int), ("reserved_i", C.c_int * 2), ("residual", C.c_double),
+                ("steps", C.c_int), ("launches", C.c_int), ("fp64_finish", C.c_int), ("residual", C.c_double),
                 ("alpha_bar", C.c_double), ("lambda_max", C.c_double),
                 ("elapsed_ms", C.c_double), ("queue_ms", C.c_double), ("error_bound", C.c_double),
                 ("reserved_d", C.c_double * 2)]
 
     def as_dict(self):
-        d = {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
-        d["launches"] = self.reserved_i[0]
-        return d
+        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
 
 
 class StepParams(C.Structure):
@@ -113,6 +111,7 @@ SIGNATURES = {
     "mprgp_matrix_info": (I, [P, C.POINTER(I), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "mprgp_matrix_set_lambda_min": (I, [P, C.c_double]),
     "mprgp_matrix_get_bsr3": (I, [P, P, P, P]),
+    "mprgp_general_solve": (I, [P, I, P, P, P, P, C.POINTER(Params), C.POINTER(Stats)]),
     "mprgp_matrix_spmv": (I, [P, P, P]),
     "mprgp_matrix_destroy": (None, [P]),
     "mprgp_qp_create": (I, [P, PP]),

@@ -1221,7 +1221,8 @@ int check_batch_qp(const mprgp_ctx *ctx, const mprgp_qp *q, const mprgp_params &
 long solve_step_bound(const mprgp_params &p) {
   const bool mixed = p.precision == MPRGP_MIXED;
   const bool power = !(p.alpha_bar > 0.0);
-  const long sweeps = mixed ? (p.max_refine > 0 ? p.max_refine : 8) + 1 : 1;
+  // mixed: the sweeps, their final R_START, and a possible FP64 finish
+  const long sweeps = mixed ? (p.max_refine > 0 ? p.max_refine : 8) + 2 : 1;
   return sweeps * (8L * ((long)p.max_it + 4) + 8) + (power ? 2L * 1002 + 4 : 0) + 16;
 }
 
@@ -1335,7 +1336,8 @@ int fill_stats(mprgp_qp *q, const QpState both[2], float ms, long launched, mprg
     mprgp_stats &o = *out;
     std::memset(&o, 0, sizeof(o));
     o.code = st.code;
-    o.refinements = st.mixed ? st.refine : 0;
+    o.refinements = (st.mixed || st.fp64_finish) ? st.refine : 0;
+    o.fp64_finish = st.fp64_finish;
     o.iterations = st.iteration;
     o.num_cg = st.num_cg;
     o.num_exp = st.num_exp;
@@ -1348,7 +1350,7 @@ int fill_stats(mprgp_qp *q, const QpState both[2], float ms, long launched, mprg
     o.alpha_bar = st.alpha_bar;
     o.lambda_max = st.lambda_max;
     o.elapsed_ms = ms;
-    o.reserved_i[0] = (int)launched;  // step-kernel launches issued for the whole batch
+    o.launches = (int)launched;  // step-kernel launches issued for the whole batch
     const double lo = q->A->lambda_min_lower;
     o.error_bound = (lo > 0.0 && std::isfinite(st.residual)) ? st.residual / lo : -1.0;
   }
@@ -1441,7 +1443,8 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
       if (!keep || (*keep)[i]) b += matrix_stream_bytes(qps[i]->A, params[stride ? i : 0].precision == MPRGP_MIXED);
     return b;
   };
-  int variant = step_variant(n_tiles, live_bytes(nullptr));
+  const int tail_tiles = tail_tiles_of(params[0]);
+  int variant = step_variant(n_tiles, live_bytes(nullptr), tail_tiles);
   hipGraphExec_t ex = nullptr;
   if (use_graph)
     if (int rc = tab.graph(s, ts, steps, grid, any_mixed, variant, &ex)) return rc;
@@ -1491,7 +1494,7 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
         MPRGP_HIP(hipMemcpyAsync(tab.d_tiles, uploads.back().data(), sizeof(Tile) * grid, hipMemcpyHostToDevice, s));
         MPRGP_HIP(hipStreamSynchronize(s));
         in_table = live;
-        variant = step_variant((int)kept.size(), live_bytes(&keep));
+        variant = step_variant((int)kept.size(), live_bytes(&keep), tail_tiles);
         if (use_graph)
           if (int rc = tab.graph(s, ts, steps, grid, any_mixed, variant, &ex)) return rc;
       }

@@ -101,6 +101,7 @@ struct alignas(128) QpState {
   int pb;                   // 1: p lives in the p2 buffer (mixed inner)
   int bb;                   // 1: beta lives in the beta2 buffer (mixed inner)
   int prop_next;            // proportioning test at the CG trial point (mixed inner)
+  int fp64_finish;          // mixed: the refinement ran out of sweeps; the FP64 form finishes
   double tol, alpha_bar, gamma;
   double alpha, beta_cg, pd, zg, residual, fx, fy;
   double alpha_f, nu, power_eps, lambda_max;

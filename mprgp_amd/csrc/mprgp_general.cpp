@@ -1,0 +1,578 @@
+// mprgp_general.cpp -- the coupled-constraint solve: MPRGPGeneralCon<double>::
+// solve (MPRGPSolver.h:594-626) with GeneralConProjector
+// (MPRGPProjection.h:400-520) and DiagonalInFacePreconSolver
+// (MPRGPPrecondition.h:36-69), over MPRGPMonotonic::solve (MPRGPSolver.h:
+// 148-316, 412-454).
+//
+// Rows J x >= c may span several nodes (the self-contact rows of
+// LinearConCollider.cpp:186-217: a vertex against a triangle, 12 nonzeros),
+// so the projection is a QP in the rows' multipliers rather than a per-node
+// formula.  The n-sized work stays on the device: SpMVs with A (the QP's SELL
+// matrix), J v and J^T lambda (CSR, one thread per row), vector updates and
+// dot products.  The m-sized work runs on the host: the dense QP
+// min 1/2 l'(J J^T) l - h'l, l >= 0 of each projection and of BETA, and the
+// Cholesky factor of the active rows' J_a J_a^T for PHI -- where the
+// reference calls QuadProg++ and Eigen's SimplicialLLT.  J J^T is formed
+// once per solve.  Each of these decisions reads a few device scalars or an
+// m-vector back, so a solve synchronises the stream a few times per
+// iteration: this path is for coupled rows, which the simulator's decoupled
+// solve (the batched kernels) does not have.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mprgp_host.h"
+
+using namespace mprgp;
+
+namespace {
+
+constexpr int kMaxGeneralRows = 2048;  // J J^T is kept dense on the host
+
+// lam = argmin 1/2 l'Gl - h'l subject to l >= 0, G (k x k, row-major) SPD.
+// Primal active set on the free set F: free the bound variable with the
+// largest positive residual (h - G l)_i, solve G_FF s = h_F by Cholesky,
+// and while some s_i <= 0 step towards s up to the first zero and bound it.
+// Finite for SPD G; false if a free set's G_FF is not positive definite.
+bool chol_factor(std::vector<double> &L, int k) {
+  for (int j = 0; j < k; ++j) {
+    double d = L[(size_t)j * k + j];
+    for (int p = 0; p < j; ++p) d -= L[(size_t)j * k + p] * L[(size_t)j * k + p];
+    if (!(d > 0.0)) return false;
+    d = std::sqrt(d);
+    L[(size_t)j * k + j] = d;
+    for (int i = j + 1; i < k; ++i) {
+      double v = L[(size_t)i * k + j];
+      for (int p = 0; p < j; ++p) v -= L[(size_t)i * k + p] * L[(size_t)j * k + p];
+      L[(size_t)i * k + j] = v / d;
+    }
+  }
+  return true;
+}
+
+void chol_solve(const std::vector<double> &L, int k, double *b) {
+  for (int i = 0; i < k; ++i) {
+    double v = b[i];
+    for (int p = 0; p < i; ++p) v -= L[(size_t)i * k + p] * b[p];
+    b[i] = v / L[(size_t)i * k + i];
+  }
+  for (int i = k - 1; i >= 0; --i) {
+    double v = b[i];
+    for (int p = i + 1; p < k; ++p) v -= L[(size_t)p * k + i] * b[p];
+    b[i] = v / L[(size_t)i * k + i];
+  }
+}
+
+bool qp_nonneg(int k, const std::vector<double> &G, const std::vector<double> &h, std::vector<double> &lam) {
+  lam.assign(k, 0.0);
+  if (k == 0) return true;
+  std::vector<char> free_(k, 0);
+  std::vector<int> idx;
+  std::vector<double> L, s;
+  double hs = 0.0;
+  for (double v : h) hs = std::max(hs, std::fabs(v));
+  const double tol = 1e-14 * (hs > 0.0 ? hs : 1.0);
+  for (int outer = 0; outer < 10 * k + 10; ++outer) {
+    int best = -1;
+    double wbest = tol;
+    for (int i = 0; i < k; ++i) {
+      double r = h[i];
+      for (int j = 0; j < k; ++j) r -= G[(size_t)i * k + j] * lam[j];
+      if (!free_[i] && r > wbest) {
+        wbest = r;
+        best = i;
+      }
+    }
+    if (best < 0) return true;
+    free_[best] = 1;
+    for (int inner = 0; inner <= k; ++inner) {
+      idx.clear();
+      for (int i = 0; i < k; ++i)
+        if (free_[i]) idx.push_back(i);
+      const int f = (int)idx.size();
+      L.assign((size_t)f * f, 0.0);
+      s.assign(f, 0.0);
+      for (int a = 0; a < f; ++a) {
+        for (int b = 0; b < f; ++b) L[(size_t)a * f + b] = G[(size_t)idx[a] * k + idx[b]];
+        s[a] = h[idx[a]];
+      }
+      if (!chol_factor(L, f)) return false;
+      chol_solve(L, f, s.data());
+      double step = 1.0;
+      int hit = -1;
+      for (int a = 0; a < f; ++a)
+        if (s[a] <= 0.0) {
+          const double t = lam[idx[a]] / (lam[idx[a]] - s[a]);
+          if (t < step) {
+            step = t;
+            hit = a;
+          }
+        }
+      if (hit < 0) {
+        std::fill(lam.begin(), lam.end(), 0.0);
+        for (int a = 0; a < f; ++a) lam[idx[a]] = s[a];
+        break;
+      }
+      for (int a = 0; a < f; ++a) lam[idx[a]] += step * (s[a] - lam[idx[a]]);
+      free_[idx[hit]] = 0;
+      lam[idx[hit]] = 0.0;
+      for (int a = 0; a < f; ++a)
+        if (lam[idx[a]] <= 0.0) {
+          free_[idx[a]] = 0;
+          lam[idx[a]] = 0.0;
+        }
+    }
+  }
+  return false;
+}
+
+// One coupled-row solve on a QP: device vectors are the QP's own buffers
+// (internal node order); J and J^T live in one call-owned device block.
+struct General {
+  mprgp_qp *q = nullptr;
+  mprgp_ctx *ctx = nullptr;
+  hipStream_t s = nullptr;
+  int n = 0, m = 0;
+  // device
+  void *blk = nullptr;
+  size_t blk_bytes = 0;
+  int *jp = nullptr, *jc = nullptr, *tp = nullptr, *tc = nullptr;
+  double *jv = nullptr, *tv = nullptr, *jx = nullptr, *lam = nullptr, *part = nullptr, *res = nullptr;
+  // tl holds J^T lambda inside project/compute_phi/compute_beta only, so a
+  // caller may pass any of the other vectors (t included) as their input
+  double *x, *g, *p, *z, *phi, *beta, *y, *gp, *t, *t2, *t3, *tl, *pv, *pw;
+  const double *b = nullptr, *dinv = nullptr;
+  // host
+  std::vector<double> c, JJt, hm, hres;
+  std::vector<char> face;
+  std::vector<int> act;
+  std::vector<double> La;  // Cholesky factor of J_a J_a^T
+  bool qp_fail = false;
+  int y_size = 0;
+
+  ~General() {
+    if (blk) ctx->release(blk, blk_bytes);
+  }
+
+  // J v to the host
+  int jmul(const double *v) {
+    MPRGP_HIP(launch_csr_mul(s, m, jp, jc, jv, v, jx));
+    MPRGP_HIP(hipMemcpyAsync(hm.data(), jx, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+    MPRGP_HIP(hipStreamSynchronize(s));
+    return MPRGP_OK;
+  }
+  // out = J^T l for host l (the upload completes before l can go away)
+  int jtmul(const std::vector<double> &l, double *out) {
+    MPRGP_HIP(hipMemcpyAsync(lam, l.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
+    MPRGP_HIP(hipStreamSynchronize(s));
+    MPRGP_HIP(launch_csr_mul(s, n, tp, tc, tv, lam, out));
+    return MPRGP_OK;
+  }
+  // dot products of the pairs, all read back with one synchronisation
+  int dots(std::initializer_list<std::pair<const double *, const double *>> pairs, double *out) {
+    int i = 0;
+    for (const auto &pr : pairs) MPRGP_HIP(launch_dot(s, pr.first, pr.second, (size_t)n, part, res + i++));
+    MPRGP_HIP(hipMemcpyAsync(out, res, sizeof(double) * i, hipMemcpyDeviceToHost, s));
+    MPRGP_HIP(hipStreamSynchronize(s));
+    return MPRGP_OK;
+  }
+  int spmv(const double *v, double *out) {
+    const mprgp_matrix *A = q->A;
+    MPRGP_HIP(launch_spmv_plain(s, A->d_sl_off, A->d_col, A->d_val, v, out, A->n_nodes, 0));
+    return MPRGP_OK;
+  }
+  int grad(const double *v) {  // g = A v - b
+    if (int rc = spmv(v, g)) return rc;
+    MPRGP_HIP(launch_axpby(s, n, 1.0, g, -1.0, b, g));
+    return MPRGP_OK;
+  }
+  int func(const double *v, double *f) {  // 1/2 v'Av - v'b (FixedSparseMatrix::funcValue)
+    if (int rc = spmv(v, t3)) return rc;
+    double d[2];
+    if (int rc = dots({{v, t3}, {v, b}}, d)) return rc;
+    *f = 0.5 * d[0] - d[1];
+    return MPRGP_OK;
+  }
+
+  // GeneralConProjector::project (:412-431)
+  int project(const double *v, double *out) {
+    if (m == 0) {
+      if (out != v) MPRGP_HIP(hipMemcpyAsync(out, v, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+      return MPRGP_OK;
+    }
+    if (int rc = jmul(v)) return rc;
+    std::vector<double> h(m), l;
+    for (int i = 0; i < m; ++i) h[i] = c[i] - hm[i];
+    if (!qp_nonneg(m, JJt, h, l)) qp_fail = true;
+    if (int rc = jtmul(l, tl)) return rc;
+    MPRGP_HIP(launch_axpby(s, n, 1.0, v, 1.0, tl, out));
+    return MPRGP_OK;
+  }
+  // DECIDE_FACE + prepareActiveConMatrix (:485-520)
+  int decide_face(const double *v) {
+    act.clear();
+    if (m == 0) return MPRGP_OK;
+    if (int rc = jmul(v)) return rc;
+    for (int i = 0; i < m; ++i) {
+      face[i] = std::fabs(hm[i] - c[i]) < kEps ? 1 : 0;
+      if (face[i]) act.push_back(i);
+    }
+    const int k = (int)act.size();
+    La.assign((size_t)k * k, 0.0);
+    for (int a = 0; a < k; ++a)
+      for (int bb = 0; bb < k; ++bb) La[(size_t)a * k + bb] = JJt[(size_t)act[a] * m + act[bb]];
+    if (!chol_factor(La, k)) qp_fail = true;
+    return MPRGP_OK;
+  }
+  int feasible(const double *v, bool *ok) {  // isFeasible (:298-310)
+    *ok = true;
+    if (m == 0) return MPRGP_OK;
+    if (int rc = jmul(v)) return rc;
+    for (int i = 0; i < m; ++i)
+      if (hm[i] < c[i] - kEps) *ok = false;
+    return MPRGP_OK;
+  }
+  // PHI (:432-451)
+  int compute_phi() {
+    if (act.empty()) {
+      MPRGP_HIP(hipMemcpyAsync(phi, g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+      return MPRGP_OK;
+    }
+    if (int rc = jmul(g)) return rc;
+    const int k = (int)act.size();
+    std::vector<double> la(k), l(m, 0.0);
+    for (int a = 0; a < k; ++a) la[a] = -hm[act[a]];
+    chol_solve(La, k, la.data());
+    for (int a = 0; a < k; ++a) l[act[a]] = la[a];
+    if (int rc = jtmul(l, tl)) return rc;
+    MPRGP_HIP(launch_axpby(s, n, 1.0, g, 1.0, tl, phi));
+    return MPRGP_OK;
+  }
+  // BETA (:452-473)
+  int compute_beta() {
+    MPRGP_HIP(launch_axpby(s, n, 1.0, g, -1.0, phi, beta));
+    if (act.empty()) return MPRGP_OK;
+    if (int rc = jmul(beta)) return rc;
+    const int k = (int)act.size();
+    std::vector<double> G((size_t)k * k), h(k), la, l(m, 0.0);
+    for (int a = 0; a < k; ++a) {
+      h[a] = hm[act[a]];
+      for (int bb = 0; bb < k; ++bb) G[(size_t)a * k + bb] = JJt[(size_t)act[a] * m + act[bb]];
+    }
+    if (!qp_nonneg(k, G, h, la)) qp_fail = true;
+    for (int a = 0; a < k; ++a) l[act[a]] = la[a];
+    if (int rc = jtmul(l, tl)) return rc;
+    MPRGP_HIP(launch_axpby(s, n, 1.0, beta, -1.0, tl, beta));
+    return MPRGP_OK;
+  }
+  // PHITPHI (:474-484) into *r
+  int phitphi(const double *v, double ab, double *r) {
+    MPRGP_HIP(launch_axpby(s, n, 1.0, v, -ab, phi, t2));
+    if (int rc = project(t2, t2)) return rc;
+    MPRGP_HIP(launch_axpby(s, n, 1.0, v, -1.0, t2, t2));
+    double d;
+    if (int rc = dots({{t2, phi}}, &d)) return rc;
+    const double v2 = d * (1.0 / ab);
+    *r = v2 > 0.0 ? v2 : 0.0;
+    return MPRGP_OK;
+  }
+  int precond(bool on) {  // DiagonalInFacePreconSolver::solve (MPRGPPrecondition.h:53-64)
+    if (on) MPRGP_HIP(launch_vmul(s, n, phi, dinv, z));
+    else MPRGP_HIP(hipMemcpyAsync(z, phi, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    return MPRGP_OK;
+  }
+};
+
+}  // namespace
+
+extern "C" int mprgp_general_solve(mprgp_qp *q, int m, const int *j_ptr, const int *j_col, const double *j_val,
+                                   const double *c, const mprgp_params *params, mprgp_stats *stats) {
+  MPRGP_TRY
+  if (!q || m < 0 || (m > 0 && (!j_ptr || !j_col || !j_val || !c)))
+    return fail(MPRGP_EBADINPUT, "bad coupled-row arguments");
+  if (m > kMaxGeneralRows) return fail(MPRGP_EBADINPUT, "%d rows; the coupled solve takes at most %d", m, kMaxGeneralRows);
+  mprgp_params def;
+  if (!params) {
+    mprgp_params_default(&def);
+    params = &def;
+  }
+  const mprgp_params &p = *params;
+  if (!q->has_rhs || !q->has_x) return fail(MPRGP_ESTATE, "rhs and x must be set");
+  if (p.precision != MPRGP_FP64 || p.variant != MPRGP_MONOTONIC)
+    return fail(MPRGP_EBADINPUT, "the coupled solve runs the FP64 monotone variant");
+  if (!(p.tol == p.tol) || p.max_it < 0) return fail(MPRGP_EBADINPUT, "bad tol/max_it");
+  MPRGP_NOT_BUSY(q);
+  mprgp_ctx *ctx = q->ctx;
+  MPRGP_HIP(hipSetDevice(ctx->device));
+  std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
+  General G;
+  G.q = q;
+  G.ctx = ctx;
+  G.s = ctx->stream;
+  G.n = 3 * q->n;
+  G.m = m;
+  const int n = G.n;
+  // J in the internal node order; J^T as a CSR of n rows
+  const std::vector<int> &iperm = q->A->iperm;
+  const int nnz = m > 0 ? j_ptr[m] : 0;
+  if (m > 0 && (j_ptr[0] != 0 || nnz < 0)) return fail(MPRGP_EBADINPUT, "J row pointers must start at 0");
+  std::vector<int> jc(nnz), tcount(n + 1, 0);
+  for (int i = 0; i < m; ++i) {
+    if (j_ptr[i + 1] < j_ptr[i]) return fail(MPRGP_EBADINPUT, "J row pointers not monotone at %d", i);
+    if (j_ptr[i + 1] == j_ptr[i]) return fail(MPRGP_EBADINPUT, "J row %d is empty", i);
+    if (!std::isfinite(c[i])) return fail(MPRGP_EBADINPUT, "c[%d] is not finite", i);
+    for (int k = j_ptr[i]; k < j_ptr[i + 1]; ++k) {
+      const int col = j_col[k];
+      if (col < 0 || col >= n) return fail(MPRGP_EBADINPUT, "J column %d out of range", col);
+      if (!std::isfinite(j_val[k])) return fail(MPRGP_EBADINPUT, "J value of row %d is not finite", i);
+      jc[k] = 3 * iperm[col / 3] + col % 3;
+      tcount[jc[k] + 1]++;
+    }
+  }
+  for (int i = 0; i < n; ++i) tcount[i + 1] += tcount[i];
+  std::vector<int> tptr(tcount), tc(nnz), fill(tcount.begin(), tcount.end() - 1);
+  std::vector<double> tv(nnz);
+  for (int i = 0; i < m; ++i)
+    for (int k = j_ptr[i]; k < j_ptr[i + 1]; ++k) {
+      const int at = fill[jc[k]]++;
+      tc[at] = i;
+      tv[at] = j_val[k];
+    }
+  // J J^T (dense, host) by a column-dense pass over each row
+  G.c.assign(c, c + m);
+  G.JJt.assign((size_t)m * m, 0.0);
+  {
+    std::vector<double> row(n, 0.0);
+    for (int i = 0; i < m; ++i) {
+      for (int k = j_ptr[i]; k < j_ptr[i + 1]; ++k) row[jc[k]] += j_val[k];
+      for (int j = 0; j < m; ++j) {
+        double v = 0.0;
+        for (int k = j_ptr[j]; k < j_ptr[j + 1]; ++k) v += j_val[k] * row[jc[k]];
+        G.JJt[(size_t)i * m + j] = v;
+      }
+      for (int k = j_ptr[i]; k < j_ptr[i + 1]; ++k) row[jc[k]] = 0.0;
+    }
+  }
+  G.hm.assign(std::max(m, 1), 0.0);
+  G.face.assign(m, 0);
+  // device block: J, J^T, J v, lambda, dot partials and results
+  const size_t parts = ((size_t)n + 255) / 256 + 1;
+  const size_t bytes = sizeof(int) * ((size_t)m + 1 + (size_t)nnz + (size_t)n + 1 + (size_t)nnz) +
+                       sizeof(double) * (2 * (size_t)nnz + 2 * (size_t)m + parts + 16) + 1024;
+  if (int rc = ctx->alloc(&G.blk, bytes)) return rc;
+  G.blk_bytes = bytes;
+  {
+    char *b8 = (char *)G.blk;
+    auto carve = [&](size_t sz) {
+      uintptr_t u = ((uintptr_t)b8 + 7) & ~(uintptr_t)7;
+      b8 = (char *)(u + sz);
+      return (void *)u;
+    };
+    G.jv = (double *)carve(sizeof(double) * nnz);
+    G.tv = (double *)carve(sizeof(double) * nnz);
+    G.jx = (double *)carve(sizeof(double) * (m + 1));
+    G.lam = (double *)carve(sizeof(double) * (m + 1));
+    G.part = (double *)carve(sizeof(double) * parts);
+    G.res = (double *)carve(sizeof(double) * 16);
+    G.jp = (int *)carve(sizeof(int) * (m + 1));
+    G.jc = (int *)carve(sizeof(int) * nnz);
+    G.tp = (int *)carve(sizeof(int) * (n + 1));
+    G.tc = (int *)carve(sizeof(int) * nnz);
+  }
+  hipStream_t s = G.s;
+  if (m > 0) {
+    MPRGP_HIP(hipMemcpyAsync(G.jp, j_ptr, sizeof(int) * (m + 1), hipMemcpyHostToDevice, s));
+    MPRGP_HIP(hipMemcpyAsync(G.jc, jc.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, s));
+    MPRGP_HIP(hipMemcpyAsync(G.jv, j_val, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
+    MPRGP_HIP(hipMemcpyAsync(G.tc, tc.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, s));
+    MPRGP_HIP(hipMemcpyAsync(G.tv, tv.data(), sizeof(double) * nnz, hipMemcpyHostToDevice, s));
+  }
+  MPRGP_HIP(hipMemcpyAsync(G.tp, tptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+  MPRGP_HIP(hipStreamSynchronize(s));  // the host arrays above are locals
+  G.x = q->x; G.g = q->g; G.p = q->p; G.z = q->z; G.phi = q->phi; G.beta = q->beta; G.y = q->y;
+  G.gp = q->ap; G.t = q->py; G.t2 = q->p2; G.t3 = q->beta2; G.tl = q->pu; G.pv = q->pv; G.pw = q->pw;
+  G.b = q->b;
+  G.dinv = q->A->d_dinv;
+  const bool pre = p.preconditioned != 0;
+  const double tol = p.tol < 1e-30 ? 1e-30 : p.tol;
+  hipEvent_t t0 = ctx->t0, t1 = ctx->t1;
+  MPRGP_HIP(hipEventRecord(t0, s));
+  // the J/c overload projects x first (MPRGPSolver.h:620-622)
+  if (p.project_first) {
+    if (int rc = G.project(G.x, G.x)) return rc;
+  }
+  // alpha_bar: given, or 2/specRad(A, ev, tol) (MPRGPSolver.h:21-32, 42-79)
+  double alpha_bar, lmax;
+  int piters = 0;
+  if (p.alpha_bar > 0.0) {
+    alpha_bar = p.alpha_bar;
+    lmax = 2.0 / alpha_bar;
+  } else {
+    const double eps = p.power_tol > 0.0 ? p.power_tol : tol;
+    double *tmp = G.pv, *out = G.pw;
+    if (p.use_ev && q->ev_valid) {
+      MPRGP_HIP(hipMemcpyAsync(tmp, q->ev, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    } else {
+      MPRGP_HIP(launch_power_start(s, q->stage, (size_t)n, power_start_norm((size_t)n)));
+      MPRGP_HIP(launch_gather_nodes(s, q->stage, tmp, q->A->d_perm, q->n));
+    }
+    double nout = 1.0;
+    int it = 0;
+    for (; it <= 1000; ++it) {
+      if (int rc = G.spmv(tmp, out)) return rc;
+      double d;
+      if (int rc = G.dots({{out, out}}, &d)) return rc;
+      nout = std::sqrt(d);
+      if (nout < kEps) {
+        MPRGP_HIP(hipMemcpyAsync(q->ev, tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        q->ev_valid = 1;
+        nout = kEps;
+        break;
+      }
+      MPRGP_HIP(launch_axpby(s, n, 1.0 / nout, out, 0.0, out, out));
+      // |out/nout - tmp| as the reference forms it: out /= nout, then the difference
+      MPRGP_HIP(launch_axpby(s, n, 1.0, out, -1.0, tmp, G.t));
+      if (int rc = G.dots({{G.t, G.t}}, &d)) return rc;
+      if (std::sqrt(d) <= eps) {
+        MPRGP_HIP(hipMemcpyAsync(q->ev, tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        q->ev_valid = 1;
+        break;
+      }
+      std::swap(tmp, out);
+    }
+    piters = it;
+    lmax = nout;
+    alpha_bar = 2.0 / nout;
+  }
+  // MPRGPMonotonic::solve (:412-454)
+  double residual = 0.0;
+  int iteration = 0, num_cg = 0, num_exp = 0, num_prop = 0;
+  auto gradients = [&](bool comp_phi, double *r) -> int {  // computeGradients (:163-173)
+    if (comp_phi)
+      if (int rc = G.compute_phi()) return rc;
+    if (int rc = G.compute_beta()) return rc;
+    MPRGP_HIP(launch_axpby(s, n, 1.0, G.phi, 1.0, G.beta, G.gp));
+    double d;
+    if (int rc = G.dots({{G.gp, G.gp}}, &d)) return rc;
+    *r = std::sqrt(d);
+    return MPRGP_OK;
+  };
+  auto proportional = [&](const double *v, int *prop) -> int {  // :174-181
+    double bb, pp;
+    if (int rc = G.dots({{G.beta, G.beta}}, &bb)) return rc;
+    if (int rc = G.phitphi(v, alpha_bar, &pp)) return rc;
+    *prop = bb <= pp;  // gamma = 1
+    return MPRGP_OK;
+  };
+  // initialize (:148-162)
+  if (int rc = G.grad(G.x)) return rc;
+  if (int rc = G.decide_face(G.x)) return rc;
+  if (int rc = G.compute_phi()) return rc;
+  if (int rc = G.precond(pre)) return rc;
+  MPRGP_HIP(hipMemcpyAsync(G.p, G.z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  for (iteration = 0; iteration < p.max_it;) {
+    if (int rc = gradients(true, &residual)) return rc;
+    if (residual <= tol) break;
+    int prop = 0;
+    if (int rc = proportional(G.x, &prop)) return rc;
+    if (prop) {  // CGMonotonicStep (:272-316); AP in gp
+      double *AP = G.gp;
+      if (int rc = G.spmv(G.p, AP)) return rc;
+      double d[2];
+      if (int rc = G.dots({{G.p, AP}, {G.z, G.g}}, d)) return rc;
+      double alpha = d[1] / d[0];
+      MPRGP_HIP(launch_axpby(s, n, 1.0, G.x, -alpha, G.p, G.y));
+      G.y_size = n;
+      double fy, fx;
+      if (int rc = G.project(G.y, G.t)) return rc;
+      if (int rc = G.func(G.t, &fy)) return rc;
+      if (int rc = G.func(G.x, &fx)) return rc;
+      prop = 1;
+      while (prop && residual > tol && fy <= fx && iteration < p.max_it) {
+        MPRGP_HIP(hipMemcpyAsync(G.x, G.y, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        MPRGP_HIP(launch_axpby(s, n, 1.0, G.g, -alpha, AP, G.g));
+        if (int rc = G.compute_phi()) return rc;
+        if (int rc = G.precond(pre)) return rc;
+        double e[2];
+        if (int rc = G.dots({{G.z, AP}, {G.p, AP}}, e)) return rc;
+        const double beta = e[0] / e[1];
+        MPRGP_HIP(launch_axpby(s, n, 1.0, G.z, -beta, G.p, G.p));
+        if (int rc = gradients(false, &residual)) return rc;  // overwrites gp = AP
+        if (int rc = proportional(G.x, &prop)) return rc;
+        if (int rc = G.spmv(G.p, AP)) return rc;
+        if (int rc = G.dots({{G.p, AP}, {G.z, G.g}}, d)) return rc;
+        alpha = d[1] / d[0];
+        MPRGP_HIP(launch_axpby(s, n, 1.0, G.x, -alpha, G.p, G.y));
+        fx = fy;
+        if (int rc = G.project(G.y, G.t)) return rc;
+        if (int rc = G.func(G.t, &fy)) return rc;
+        num_cg++;
+        iteration++;
+      }
+    }
+    if (residual <= tol) break;
+    bool feas = false;
+    if (!prop && G.y_size > 0)
+      if (int rc = G.feasible(G.y, &feas)) return rc;
+    if (prop || G.y_size <= 0 || !feas) {  // ExpMonotonicStep (:250-271)
+      num_exp++;
+      if (int rc = G.project(G.x, G.y)) return rc;
+      G.y_size = n;
+      if (int rc = G.grad(G.y)) return rc;
+      if (int rc = G.decide_face(G.y)) return rc;
+      if (int rc = G.compute_phi()) return rc;
+      MPRGP_HIP(launch_axpby(s, n, 1.0, G.y, -alpha_bar, G.phi, G.y));
+      if (int rc = G.project(G.y, G.x)) return rc;
+      if (int rc = G.grad(G.x)) return rc;
+    } else {  // the caller's g = A x - b, then PropStep (:219-243, 435-437)
+      num_prop++;
+      if (int rc = G.grad(G.x)) return rc;
+      double *AD = G.gp;
+      if (int rc = G.spmv(G.beta, AD)) return rc;
+      double d[2];
+      if (int rc = G.dots({{G.beta, AD}, {G.g, G.beta}}, d)) return rc;
+      const double alpha = d[1] / d[0];
+      MPRGP_HIP(launch_axpby(s, n, 1.0, G.x, -alpha, G.beta, G.t));
+      if (int rc = G.project(G.t, G.x)) return rc;
+      MPRGP_HIP(launch_axpby(s, n, 1.0, G.g, -alpha, AD, G.g));
+    }
+    if (int rc = G.decide_face(G.x)) return rc;
+    if (int rc = G.compute_phi()) return rc;
+    if (int rc = G.precond(pre)) return rc;
+    MPRGP_HIP(hipMemcpyAsync(G.p, G.z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    iteration++;
+  }
+  if (int rc = G.project(G.x, G.t)) return rc;  // the final projection (:442-443)
+  MPRGP_HIP(hipMemcpyAsync(G.x, G.t, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  MPRGP_HIP(hipEventRecord(t1, s));
+  MPRGP_HIP(hipStreamSynchronize(s));
+  float ms = 0.f;
+  MPRGP_HIP(hipEventElapsedTime(&ms, t0, t1));
+  if (G.qp_fail) return fail(MPRGP_EBADINPUT, "a projection QP or the active rows' J J^T factorisation failed "
+                                              "(J J^T singular on the rows concerned)");
+  const int code = residual <= tol ? 0 : -1;
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->code = code;
+    stats->iterations = iteration;
+    stats->num_cg = num_cg;
+    stats->num_exp = num_exp;
+    stats->num_prop = num_prop;
+    stats->active = (int)G.act.size();
+    stats->power_iters = piters;
+    stats->ev_valid = q->ev_valid;
+    stats->residual = residual;
+    stats->alpha_bar = alpha_bar;
+    stats->lambda_max = lmax;
+    stats->elapsed_ms = ms;
+    const double lo = q->A->lambda_min_lower;
+    stats->error_bound = -1.0;  // the bound of mprgp_stats holds for decoupled rows
+    (void)lo;
+  }
+  if (!std::isfinite(residual)) return MPRGP_ENAN;
+  return code == 0 ? MPRGP_OK : MPRGP_NOT_CONVERGED;
+  MPRGP_CATCH
+}

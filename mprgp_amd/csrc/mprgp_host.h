@@ -54,6 +54,12 @@ __host__ __device__ inline double power_start_raw(uint64_t i) {
 double power_start_norm(size_t n);
 
 // kernel launch wrappers (mprgp_kernels.hip)
+// coupled-row solve: out = M x for a CSR M of `rows` rows; out = a x + b y;
+// out = x y (elementwise)
+hipError_t launch_csr_mul(hipStream_t s, int rows, const int *ptr, const int *col, const double *val,
+                          const double *x, double *out);
+hipError_t launch_axpby(hipStream_t s, size_t n, double a, const double *x, double b, const double *y, double *out);
+hipError_t launch_vmul(hipStream_t s, size_t n, const double *x, const double *y, double *out);
 // out[i] = power_start_raw(i) / nr for i < n (out as is when nr = 0)
 hipError_t launch_power_start(hipStream_t s, double *out, size_t n, double nr);
 // variant: a StepVariant (mprgp_dev.h), from step_variant() below
@@ -121,8 +127,10 @@ void append_tiles(int slot, const mprgp_qp *q, int span, std::vector<Tile> &tile
 int tiles_for(int n_slices, int span);
 int round_grid(int n_tiles);
 // Launches with at most this many tiles (one workgroup per CU of an MI355X)
-// use the step kernel's tail variant.
+// use the step kernel's tail variant, unless the call's tail_tiles says
+// otherwise.
 constexpr int kTailTiles = 256;
+inline int tail_tiles_of(const mprgp_params &p) { return p.tail_tiles > 0 ? p.tail_tiles : kTailTiles; }
 // Launches whose matrices (SELL slots of the live QPs, at each QP's storage
 // precision) exceed the MI355X's 256 MB last-level cache stream them
 // nontemporal.  MPRGP_STREAM_BYTES, when set, replaces the threshold (a
@@ -132,8 +140,8 @@ inline double stream_threshold() {
   const char *e = getenv("MPRGP_STREAM_BYTES");
   return (e && e[0]) ? atof(e) : kStreamBytes;
 }
-inline int step_variant(int n_tiles, double matrix_bytes) {
-  if (n_tiles <= kTailTiles) return kStepTail;
+inline int step_variant(int n_tiles, double matrix_bytes, int tail_tiles = kTailTiles) {
+  if (n_tiles <= tail_tiles) return kStepTail;
   return matrix_bytes > stream_threshold() ? kStepStream : kStepCached;
 }
 void layout_tiles(const std::vector<Tile> &tiles, int grid, Tile *table);

@@ -561,14 +561,31 @@ __device__ void control(QpState &st, const Sums &S) {
       }
       st.active = (int)S.v[0];
       st.code = st.residual <= st.tol ? 0 : -1;
+      if (st.fp64_finish) {  // the counts of the mixed sweeps before it
+        st.iteration += st.iter_total;
+        st.num_cg += st.num_cg_t;
+        st.num_exp += st.num_exp_t;
+        st.num_prop += st.num_prop_t;
+      }
       st.op = OP_DONE;
       break;
     case OP_R_START: {
       st.res64 = sqrt(S.v[0]);
       if (!(st.res64 == st.res64)) {  // NaN: stop
         st.op = OP_R_FINAL;
-      } else if (st.res64 <= st.tol_outer || st.refine >= st.max_refine) {
+      } else if (st.res64 <= st.tol_outer) {
         st.op = OP_R_FINAL;
+      } else if (st.refine >= st.max_refine) {
+        // every sweep used and still above tol: the inner solve's FP32 floor
+        // (about eps32 |lambda| per active row, the contact force leaking
+        // into the tangential gradient) is above what the sweeps need.  The
+        // FP64 form solves on from the current x (its own vectors, the rows'
+        // own offsets, x in the x buffer).
+        st.mixed = 0;
+        st.fp64_finish = 1;
+        st.tol = st.tol_outer;
+        st.xb = st.pb = st.bb = 0;
+        st.op = OP_INIT;
       } else {
         double t = st.inner_rel * st.res64;
         st.tol = t > 0.5 * st.tol_outer ? t : 0.5 * st.tol_outer;
@@ -1470,6 +1487,30 @@ __global__ void __launch_bounds__(256) reduce_parts(const double *__restrict__ p
   if (threadIdx.x == 0) out[0] = red[0];
 }
 
+// Coupled-row solve (mprgp_general.cpp): CSR products one thread per row,
+// terms in stored order; vector updates.
+__global__ void __launch_bounds__(256) csr_mul(int rows, const int *__restrict__ ptr, const int *__restrict__ col,
+                                               const double *__restrict__ val, const double *__restrict__ x,
+                                               double *__restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= rows) return;
+  double s = 0.0;
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k) s += val[k] * x[col[k]];
+  out[i] = s;
+}
+// out = a x + b y
+__global__ void __launch_bounds__(256) axpby(size_t n, double a, const double *__restrict__ x, double b,
+                                             const double *__restrict__ y, double *__restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = a * x[i] + b * y[i];
+}
+// out = x y (elementwise)
+__global__ void __launch_bounds__(256) vmul(size_t n, const double *__restrict__ x, const double *__restrict__ y,
+                                            double *__restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) out[i] = x[i] * y[i];
+}
+
 // The power method's start vector (power_start_raw), divided by its norm.
 __global__ void __launch_bounds__(256) power_start(double *__restrict__ out, size_t n, double nr) {
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -1619,6 +1660,26 @@ hipError_t launch_projector(hipStream_t s, const ProjArgs &a, double *result) {
   const int blocks = (a.n_nodes + 255) / 256;
   projector_op<<<blocks, 256, 0, s>>>(a);
   if (a.part) reduce_parts<<<1, 256, 0, s>>>(a.part, blocks, a.op == PJ_STEP_LIMIT ? 1 : 0, result);
+  return hipGetLastError();
+}
+hipError_t launch_csr_mul(hipStream_t s, int rows, const int *ptr, const int *col, const double *val,
+                          const double *x, double *out) {
+  if (rows <= 0) return hipSuccess;
+  csr_mul<<<(rows + 255) / 256, 256, 0, s>>>(rows, ptr, col, val, x, out);
+  return hipGetLastError();
+}
+hipError_t launch_axpby(hipStream_t s, size_t n, double a, const double *x, double b, const double *y, double *out) {
+  if (n == 0) return hipSuccess;
+  const size_t blocks = (n + 255) / 256;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  axpby<<<(unsigned)blocks, 256, 0, s>>>(n, a, x, b, y, out);
+  return hipGetLastError();
+}
+hipError_t launch_vmul(hipStream_t s, size_t n, const double *x, const double *y, double *out) {
+  if (n == 0) return hipSuccess;
+  const size_t blocks = (n + 255) / 256;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  vmul<<<(unsigned)blocks, 256, 0, s>>>(n, x, y, out);
   return hipGetLastError();
 }
 hipError_t launch_power_start(hipStream_t s, double *out, size_t n, double nr) {

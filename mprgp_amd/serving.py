@@ -219,6 +219,7 @@ def run_open_loop(engine: M.Engine, sessions, rate=2000.0, duration=3.0, tol=1e-
     lock = threading.Lock()
     tickets = {}          # ticket -> (session, arrival)
     lat, queue_ms, codes, iters, bounds = [], [], [], [], []
+    finish64 = 0
     pool = cf.ThreadPoolExecutor(max_workers=workers)
     errors = []
 
@@ -287,6 +288,7 @@ def run_open_loop(engine: M.Engine, sessions, rate=2000.0, duration=3.0, tol=1e-
         codes.append(rc)
         iters.append(st["iterations"])
         bounds.append(st["error_bound"])
+        finish64 += st["fp64_finish"]
         done += 1
         pool.submit(finish, s, rc, st)
     wall = time.perf_counter() - t0
@@ -300,6 +302,7 @@ def run_open_loop(engine: M.Engine, sessions, rate=2000.0, duration=3.0, tol=1e-
             "p99_ms": round(float(np.percentile(lat, 99)), 3),
             "queue_p99_ms": round(float(np.percentile(queue_ms, 99)), 3),
             "converged": int(sum(1 for c in codes if c == 0)),
+            "fp64_finish": int(finish64),
             "iterations_mean": round(float(np.mean(iters)), 1),
             "error_bound_m": {"max": float(np.max(bounds)), "median": float(np.median(bounds)),
                               "reported": int(sum(1 for b in bounds if b >= 0.0))},

@@ -2,10 +2,14 @@
 
 The step kernel's shape is the number of wavefronts per workgroup (one
 SELL slice of 64 block rows per wavefront, so a workgroup's tile is that many
-slices) and the number of launches captured per hipGraph replay.  Every candidate changes
-only the schedule (and, through the tile grouping of partial sums, the last
-bits of the reductions), never the algorithm.  ``tune`` times each candidate
-on the caller's own batch from the same warm start and keeps the fastest;
+slices), the number of launches captured per hipGraph replay, and the tile
+count at or below which a launch runs the kernel's tail form (the batch's
+last, latency-bound launches, once only the stiffest QPs are left).  Every
+candidate changes only the schedule (and, through the tile grouping of
+partial sums, the last bits of the reductions; the tail switch point not
+even those), never the algorithm.  ``tune`` times each candidate on the
+caller's own batch from the same warm start and keeps the fastest: first the
+wavefront/graph shapes, then the tail switch points at the best of them.
 ``tuned_params`` looks the answer up in a JSON cache keyed by device, batch
 size class and precision, tuning on a miss.
 """
@@ -28,8 +32,12 @@ DEFAULT_CANDIDATES = [
     {"tiles": 1, "graph_steps": 16},
 ]
 
+# Tail switch points tried at the best shape (0: the library's default, 256
+# tiles, one workgroup per CU).
+TAIL_CANDIDATES = [0, 64, 128, 512, 1024, 2048]
+
 # Bumped whenever the step kernel changes in a way that moves the best shape.
-CACHE_VERSION = 4
+CACHE_VERSION = 5
 
 
 def cache_path():
@@ -48,17 +56,33 @@ def batch_key(qps, precision, device=0):
 def apply(p: M.Params, shape):
     p.tiles = int(shape["tiles"])
     p.graph_steps = int(shape["graph_steps"])
+    p.tail_tiles = int(shape.get("tail_tiles", 0))
     return p
 
 
-def tune(ctx, qps, per_qp, candidates=None, repeats=2, log=None):
+def tune(ctx, qps, per_qp, candidates=None, tails=None, repeats=2, log=None):
     """Times every candidate shape on the batch (each QP restarted from its
     warm start with reset_x), keeps each candidate's best of `repeats` timed
-    runs (single runs differ by about 1%, as much as the candidates do) and
-    returns (best_shape, [(shape, ms)])."""
+    runs (single runs differ by about 1%, as much as the candidates do), then
+    the tail switch points `tails` at the best shape; returns (best_shape,
+    [(shape, ms)])."""
     candidates = candidates or DEFAULT_CANDIDATES
+    tails = TAIL_CANDIDATES if tails is None else tails
     results = []
+    timed = _timer(ctx, qps, per_qp, repeats, log, results)
     for shape in candidates:
+        timed(dict(shape))
+    best = min(results, key=lambda r: r[1])[0]
+    for t in tails:
+        if t != best.get("tail_tiles", 0):
+            timed(dict(best, tail_tiles=t))
+    for q in qps:
+        q.reset_x()
+    return min(results, key=lambda r: r[1])[0], results
+
+
+def _timer(ctx, qps, per_qp, repeats, log, results):
+    def timed(shape):
         ps = [apply(M.Params.from_buffer_copy(p), shape) for p in per_qp]
         best = math.inf
         for rep in range(repeats + 1):  # the first run captures the graphs
@@ -72,9 +96,7 @@ def tune(ctx, qps, per_qp, candidates=None, repeats=2, log=None):
         results.append((shape, best))
         if log:
             log(f"tune {shape}: {best:.2f} ms")
-    for q in qps:
-        q.reset_x()
-    return min(results, key=lambda r: r[1])[0], results
+    return timed
 
 
 def tuned_params(ctx, qps, per_qp, precision, device=0, path=None, retune=False, log=None):

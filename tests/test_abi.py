@@ -53,3 +53,29 @@ def test_workload_sizes():
     e = np.sort(np.concatenate([T[:, [i, j]] for i in range(4) for j in range(i + 1, 4)]), 1)
     edges = np.unique(e[:, 0].astype(np.int64) * 10**6 + e[:, 1]).size
     assert edges == 253340 and 44541 + 2 * edges == 551221   # blocks of A
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """Every ctypes mirror in mprgp_amd/_lib.py has the header's size and
+    field offsets (gcc compiles a program that prints them)."""
+    import ctypes
+    import subprocess
+    pairs = [("mprgp_params", _lib.Params), ("mprgp_stats", _lib.Stats),
+             ("mprgp_step_params", _lib.StepParams), ("mprgp_step_stats", _lib.StepStats),
+             ("mprgp_engine_config", _lib.EngineConfig), ("mprgp_engine_info", _lib.EngineInfo)]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mprgp_amd.h"', "int main(void) {"]
+    for cname, py in pairs:
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                          text=True).stdout.splitlines())
+    for cname, py in pairs:
+        assert int(got[f"{cname} size"]) == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got[f"{cname} {f}"]) == getattr(py, f).offset, (cname, f)

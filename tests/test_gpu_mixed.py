@@ -175,3 +175,55 @@ def test_error_bound_of_every_solve(ctx, which):
     q.reset_x()
     rc, st = q.solve(tol=1e-6, max_it=50000, alpha_bar=ab, precision=M.MIXED, project_first=1)
     assert st["error_bound"] == pytest.approx(st["residual"] / lam_lo, rel=1e-12)
+
+
+def chain_qp(n, k=40.0, load=1.0):
+    """A chain of n nodes: A = (1 + k) I on the diagonal blocks, -k/2 I to
+    each neighbour (eigenvalues in (1, 1 + 2k)); one row y >= 0 on every 7th
+    node; b pulls every node down by `load`, so each row carries about 7 load."""
+    import scipy.sparse as sp
+    T = sp.diags([np.full(n - 1, -0.5 * k), np.full(n, 1.0 + k), np.full(n - 1, -0.5 * k)], [-1, 0, 1])
+    A = sp.kron(T, sp.identity(3)).tocsr()
+    i = np.arange(n)
+    b = np.stack([0.01 * np.sin(0.1 * i), -load * np.ones(n), 0.01 * np.cos(0.1 * i)], 1).ravel()
+    x0 = np.zeros(3 * n)
+    x0[1::3] = 0.05
+    node = i[i % 7 == 0]
+    normal = np.tile([0.0, 1.0, 0.0], (node.size, 1))
+    return A, b, x0, node, normal, np.zeros(node.size)
+
+
+def test_stalled_refinement_finishes_in_fp64(ctx):
+    """Contact forces of about 7 per row put the FP32 inner solve's floor
+    (eps32 |lambda| per active row) above tol = 1e-6: every refinement sweep
+    runs and the residual stays above tol, so the FP64 form finishes from that
+    x.  The result converges (FP64 KKT residual <= tol) and lies within the
+    two solves' error bounds of the plain FP64 solve."""
+    A, b, x0, node, normal, c = chain_qp(12000)
+    ab = 2.0 / (1.0 + 2 * 40.0)  # Gershgorin bound on the largest eigenvalue
+    m = ctx.matrix_csr(A)
+    m.set_lambda_min(1.0)       # Gershgorin bound on the smallest
+    q = m.qp()
+    q.set_rows(node, normal, c)
+    q.set_rhs(b)
+    q.set_x(x0)
+    out = {}
+    for prec in (M.FP64, M.MIXED):
+        q.reset_x()
+        rc, st = q.solve(M.params(tol=1e-6, max_it=20000, alpha_bar=ab, precision=prec))
+        print(prec, {k: st[k] for k in ("code", "iterations", "refinements", "fp64_finish", "residual")})
+        assert rc == 0 and st["code"] == 0 and st["residual"] <= 1e-6
+        out[prec] = (q.get_x(), st)
+    xm, sm = out[M.MIXED]
+    xf, sf = out[M.FP64]
+    assert sm["fp64_finish"] == 1 and sm["refinements"] == 8 and sf["fp64_finish"] == 0
+    assert np.linalg.norm(xm - xf) <= sm["error_bound"] + sf["error_bound"]
+    # a QP inside the mixed form's range never takes the FP64 finish
+    sc = problems.config1()
+    Ab, bb, nb, nrm, cb, xb = scene_qp(sc)
+    qb = ctx.matrix_csr(Ab).qp()
+    qb.set_rows(nb, nrm, cb)
+    qb.set_rhs(bb)
+    qb.set_x(xb)
+    rc, st = qb.solve(M.params(tol=1e-6, max_it=20000, precision=M.MIXED, project_first=1))
+    assert rc == 0 and st["fp64_finish"] == 0

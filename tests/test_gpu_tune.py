@@ -1,0 +1,82 @@
+"""The batched solve's launch-shape autotuner (mprgp_amd/tune.py) on a GPU:
+the tail switch point changes no bits, the tuned shape is cached and found
+again, and every candidate shape's solution lies within the solves' own
+error bounds of the others."""
+import json
+
+import numpy as np
+import pytest
+
+import mprgp_amd as M
+from mprgp_amd import problems, tune, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def batch(ctx):
+    scenes = problems.batch_scenes(16, seed=37, tol=1e-6, max_it=4000)
+    qps = [workloads.DeviceQP(ctx, sc) for sc in scenes]
+    rc, st = M.solve_batch(ctx, [q.qp for q in qps], M.params(tol=1e-6, max_it=0, power_tol=1e-6))
+    return qps, [s["alpha_bar"] for s in st]
+
+
+def per_qp(ab, prec, **kw):
+    return [M.params(tol=1e-6, max_it=4000, alpha_bar=a, project_first=1, precision=prec, **kw) for a in ab]
+
+
+@pytest.mark.parametrize("prec", [M.FP64, M.MIXED])
+def test_tail_switch_point_changes_no_bits(ctx, batch, prec):
+    """tail_tiles = 1 (the tail form almost never), 0 (the default, 256) and
+    2^30 (every launch) give the same bits and iteration counts."""
+    qps, ab = batch
+    out = {}
+    for tail in (1, 0, 1 << 30):
+        for q in qps:
+            q.qp.reset_x()
+        rc, st = M.solve_batch(ctx, [q.qp for q in qps], per_qp=per_qp(ab, prec, tiles=2, graph_steps=16,
+                                                                        tail_tiles=tail))
+        assert all(s["code"] == 0 for s in st), tail
+        out[tail] = ([q.qp.get_x() for q in qps], [s["iterations"] for s in st])
+    for tail in (0, 1 << 30):
+        for a, b in zip(out[1][0], out[tail][0]):
+            np.testing.assert_array_equal(a, b)
+        assert out[tail][1] == out[1][1]
+
+
+def test_tuner_caches_and_candidates_agree(ctx, batch, tmp_path):
+    qps, ab = batch
+    path = str(tmp_path / "tune.json")
+    cands = [{"tiles": 2, "graph_steps": 16}, {"tiles": 4, "graph_steps": 8}, {"tiles": 1, "graph_steps": 32}]
+    base = per_qp(ab, M.FP64)
+    saved = (tune.DEFAULT_CANDIDATES, tune.TAIL_CANDIDATES)
+    tune.DEFAULT_CANDIDATES, tune.TAIL_CANDIDATES = cands, [0, 1024]
+    try:
+        ps, shape, hit = tune.tuned_params(ctx, [q.qp for q in qps], base, M.FP64, path=path)
+        assert not hit
+        ps2, shape2, hit2 = tune.tuned_params(ctx, [q.qp for q in qps], base, M.FP64, path=path)
+    finally:
+        tune.DEFAULT_CANDIDATES, tune.TAIL_CANDIDATES = saved
+    assert hit2 and shape2 == shape
+    with open(path) as f:
+        cache = json.load(f)
+    key = tune.batch_key([q.qp for q in qps], M.FP64)
+    assert cache[key]["shape"] == shape
+    tried = [s for s, _ in cache[key]["ms"]]
+    assert len(tried) == 4 and all(s in tried for s in cands)  # 3 shapes + 1 tail point at the best
+    assert all(p.tiles == shape["tiles"] and p.graph_steps == shape["graph_steps"]
+               and p.tail_tiles == shape.get("tail_tiles", 0) for p in ps)
+    # each tried shape: converged, and within the two solves' error bounds of
+    # the tuned shape's solution (|x - x*| <= error_bound for each)
+    sols = {}
+    for s in tried:
+        for q in qps:
+            q.qp.reset_x()
+        rc, st = M.solve_batch(ctx, [q.qp for q in qps], per_qp=[tune.apply(M.Params.from_buffer_copy(p), s)
+                                                                   for p in base])
+        assert all(x["code"] == 0 and x["error_bound"] > 0 for x in st), s
+        sols[json.dumps(s, sort_keys=True)] = ([q.qp.get_x() for q in qps], [x["error_bound"] for x in st])
+    ref_x, ref_b = sols[json.dumps(shape, sort_keys=True)]
+    for xs, bs in sols.values():
+        for x, xr, b, br in zip(xs, ref_x, bs, ref_b):
+            assert np.linalg.norm(x - xr) <= b + br
