@@ -224,6 +224,18 @@ int mprgp_qp_op_counts(mprgp_qp *qp, int *counts, int n);
 /* MPRGPDecoupledCon<double>::solve<FixedSparseMatrix, preconditioned>
  * (MPRGPSolver.h:544-559): x is read and overwritten on the device. */
 int mprgp_qp_solve(mprgp_qp *qp, const mprgp_params *params, mprgp_stats *stats);
+/* No counterpart in the reference: mprgp_qp_solve for callers that cannot
+ * allocate (serving loops that must not call hipMalloc, harnesses that forbid
+ * it).  The same solve, bit for bit, with no device or host allocation: its
+ * launch tables live in a static per-GPU scratch array of the library
+ * (657 KB, held until the process ends) and the step kernel is launched
+ * directly, without graph capture.  Limits: QPs of at most 16384 workgroup
+ * slots (2^20 nodes at params->tiles = 1, 2^22 at the default shape), else
+ * MPRGP_EBADINPUT; the mixed form needs the QP's FP32 storage from an earlier
+ * mprgp_qp_solve (else MPRGP_ESTATE).  Calls on one GPU share the array and
+ * are serialised within the process; concurrent calls from several processes
+ * each have their own copy. */
+int mprgp_qp_solve_noalloc(mprgp_qp *qp, const mprgp_params *params, mprgp_stats *stats);
 /* Many QPs advanced together (params: one per QP, or NULL-terminated use of
  * params[0] for all when params_stride == 0).  Returns MPRGP_OK when every QP
  * converged, MPRGP_NOT_CONVERGED if any did not, an error otherwise. */

@@ -140,6 +140,7 @@ SIGNATURES = {
     "mprgp_qp_get_face": (I, [P, P]),
     "mprgp_qp_op_counts": (I, [P, P, I]),
     "mprgp_qp_solve": (I, [P, C.POINTER(Params), C.POINTER(Stats)]),
+    "mprgp_qp_solve_noalloc": (I, [P, C.POINTER(Params), C.POINTER(Stats)]),
     "mprgp_solve_batch": (I, [P, I, PP, C.POINTER(Params), I, C.POINTER(Stats)]),
     "mprgp_decoupled_solve": (I, [I, I, P, P, P, P, I, P, P, P, P, P, D, I, P, C.POINTER(I),
                                   C.POINTER(Stats)]),

@@ -110,6 +110,7 @@ extern "C" int mprgp_ctx_create(int device, size_t hbm_budget_bytes, mprgp_ctx *
   c->budget = hbm_budget_bytes;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_running, 2 * sizeof(int), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc(&c->h_final, 2 * sizeof(QpState), hipHostMallocDefault);
   if (e == hipSuccess) e = hipEventCreate(&c->t0);
   if (e == hipSuccess) e = hipEventCreate(&c->t1);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->q[0], hipEventDisableTiming);
@@ -120,6 +121,7 @@ extern "C" int mprgp_ctx_create(int device, size_t hbm_budget_bytes, mprgp_ctx *
     return fail(MPRGP_EHIP, "context setup failed: %s", hipGetErrorString(e));
   }
   int rc = c->tab.init(c);
+  if (!rc) rc = noalloc_setup(device);
   if (rc) {
     mprgp_ctx_destroy(c);
     return rc;
@@ -137,6 +139,7 @@ extern "C" void mprgp_ctx_destroy(mprgp_ctx *c) {
   c->tab.destroy();
   c->release(c->scratch, c->scratch_bytes);
   if (c->h_running) (void)hipHostFree(c->h_running);
+  if (c->h_final) (void)hipHostFree(c->h_final);
   if (c->t0) (void)hipEventDestroy(c->t0);
   if (c->t1) (void)hipEventDestroy(c->t1);
   if (c->q[0]) (void)hipEventDestroy(c->q[0]);

@@ -156,6 +156,9 @@ void describe_qp(const mprgp_qp *q, const mprgp_params &p, int n_tiles, int *don
 int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, int *done_flag, int tag,
                QpDev &d, QpState &st, bool rows = true);
 int fill_stats(mprgp_qp *q, const QpState both[2], float ms, long launched, mprgp_stats *out);
+// The no-alloc solve's per-device scratch address (hipGetSymbolAddress with
+// the device current), looked up once per device at context creation.
+int noalloc_setup(int device);
 
 }  // namespace mprgp
 
@@ -168,6 +171,7 @@ struct mprgp_ctx {
   std::recursive_mutex mu;   // serialises calls that use `stream`
   mprgp::BatchTables tab;    // solve_batch's launch tables
   int *h_running = nullptr;  // pinned, 2 slots
+  void *h_final = nullptr;   // pinned: a QP's two state copies, read back by the no-alloc solve
   hipEvent_t t0 = nullptr, t1 = nullptr, q[2] = {nullptr, nullptr};
   hipEvent_t ready = nullptr;  // marks the stream's work for solves on another stream
   // scratch shared by calls on this context (serialised by `mu`)
