@@ -19,6 +19,7 @@
  *   LinearConCollider::handle (planes)   src/CollisionHandle/LinearConCollider.cpp:140-161
  *   MPRGPGeneralCon / GeneralConProjector src/MPRGPSolver/MPRGPSolver.h:594-626,
  *                                        src/MPRGPSolver/MPRGPProjection.h:400-520
+ *   ICASolver (the comparator)           src/MPRGPSolver/ICASolver.h:287-463
  *
  * Threading: a context owns one HIP stream; calls that use it are serialised
  * by the context (several threads may share a context).  An engine runs its
@@ -258,6 +259,27 @@ int mprgp_solve_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mpr
  * or the active rows' factorisation fails.  stats->error_bound is -1. */
 int mprgp_general_solve(mprgp_qp *qp, int m, const int *j_ptr, const int *j_col, const double *j_val,
                         const double *c, const mprgp_params *params, mprgp_stats *stats);
+
+/* ---- the ICA comparator ------------------------------------------------------
+ * ICASolver(max_it, tol, gs_max_it, gs_tol) .reset(A, b) .solve(J, p, x)
+ * (ICASolver.h:287-415), as ICAFemSolver::forward runs it
+ * (MprgpFemSolver.cpp:439-486): the solver the reference compares MPRGP
+ * with, for A x = b + J^T lambda, 0 <= lambda _|_ J x - p >= 0, on the QP's
+ * matrix, right-hand side and x (the start, overwritten; the reference
+ * starts from the unconstrained solution).  Rows J x >= p as for
+ * mprgp_general_solve (CSR over the user's DoFs, any coupling; the QP's own
+ * per-node rows are not used).  Each iteration: c = p + J Da^-1 (LaUa x - b),
+ * lambda by projected Gauss-Seidel on J Da^-1 J^T (at most gs_max_it sweeps,
+ * to gs_tol, from the previous lambda), then one forward 3x3-block
+ * Gauss-Seidel sweep in the user's node order; it stops when
+ * |A x - b - J^T lambda| <= tol.  lambda (m, may be NULL) receives the
+ * multipliers and *pgs_sweeps (may be NULL) the projected Gauss-Seidel
+ * sweeps run.  stats: code (0 met, -1 not), iterations (the reference's
+ * counter), residual, elapsed_ms; error_bound -1.  Returns MPRGP_OK or
+ * MPRGP_NOT_CONVERGED. */
+int mprgp_ica_solve(mprgp_qp *qp, int m, const int *j_ptr, const int *j_col, const double *j_val,
+                    const double *p, double tol, int max_it, int gs_max_it, double gs_tol, double *lambda,
+                    int *pgs_sweeps, mprgp_stats *stats);
 
 /* ---- one-shot drop-in ------------------------------------------------------
  * MPRGPDecoupledCon<double>::solve<FixedSparseMatrix<double>,true>(A, B, J, c,

@@ -10,6 +10,8 @@
  *   projector.project(x1, new_pos);
  *   mprgp_amd::MPRGPDecoupledCon<double>::solve<mprgp_amd::FixedSparseMatrix<double>, true>(
  *       A, RHS, projector, new_pos, tol, max_it, "decoupled MPRGP", &power_ev);
+ *   mprgp_amd::ICASolver ica(max_it, tol);          // ICAFemSolver's comparator
+ *   ica.reset(LHS_mat, RHS);  ica.solve(J, c, new_pos);
  *
  * Types are duck-typed on Eigen's interfaces: a sparse matrix is anything
  * with rows(), cols(), outerIndexPtr(), innerIndexPtr(), valuePtr() and
@@ -340,6 +342,83 @@ class MPRGPDecoupledCon<double> {
     if (code == MPRGP_OK || code == MPRGP_NOT_CONVERGED) evio.back(ev, ev_valid);
     return code;
   }
+};
+
+// ICASolver (ICASolver.h:287-463): the comparator, on the device through
+// mprgp_ica_solve.  reset(A, r) keeps a copy of A and r; solve(J, p, x) runs
+// from x and writes the result into it.
+class ICASolver {
+ public:
+  ICASolver(const int max_it = 1000, const double tol = 1e-4, const int gs_max_it = 20,
+            const double gs_tol = 1e-3)
+      : max_it_(max_it), tol_(tol), gs_max_it_(gs_max_it), gs_tol_(gs_tol) {}
+
+  template <typename SP, typename VEC>
+  void reset(const SP &A, const VEC &r) {
+    a_ = detail::Csr::rowmajor_of(A);
+    r_.assign(r.data(), r.data() + r.size());
+  }
+  template <typename SP>
+  void reset(const SP &A) {
+    a_ = detail::Csr::rowmajor_of(A);
+    r_.assign(a_.rows, 0.0);
+  }
+
+  // true when |A x - r - J^T lambda| <= tol was reached; a negative MPRGP_E*
+  // code leaves iterations at -1 and returns false
+  template <typename SP, typename VEC_P, typename VEC>
+  bool solve(const SP &J, const VEC_P &p, VEC &x) {
+    iterations_ = -1;
+    residual_ = -1.0;
+    status_ = MPRGP_EBADINPUT;
+    const detail::Csr j = detail::Csr::rowmajor_of(J);
+    if (a_.rows == 0 || a_.rows != a_.cols || (int)x.size() != a_.rows || (int)r_.size() != a_.rows ||
+        j.cols != a_.rows || (int)p.size() != j.rows)
+      return false;
+    mprgp_ctx *ctx = detail::context();
+    if (!ctx) {
+      status_ = MPRGP_ENODEV;
+      return false;
+    }
+    mprgp_matrix *m = nullptr;
+    mprgp_qp *q = nullptr;
+    int rc = mprgp_matrix_create_csr(ctx, a_.rows, a_.ptr.data(), a_.idx.data(), a_.val.data(), &m);
+    if (!rc) rc = mprgp_qp_create(m, &q);
+    if (!rc) rc = mprgp_qp_set_rhs(q, r_.data());
+    if (!rc) rc = mprgp_qp_set_x(q, x.data());
+    lambda_.assign(j.rows, 0.0);
+    if (!rc) {
+      mprgp_stats st;
+      int sweeps = 0;
+      rc = mprgp_ica_solve(q, j.rows, j.ptr.data(), j.idx.data(), j.val.data(), p.data(), tol_, max_it_,
+                           gs_max_it_, gs_tol_, lambda_.data(), &sweeps, &st);
+      if (rc == MPRGP_OK || rc == MPRGP_NOT_CONVERGED) {
+        iterations_ = st.iterations;
+        residual_ = st.residual;
+        if (mprgp_qp_get_x(q, x.data()) != MPRGP_OK) rc = MPRGP_EHIP;
+      }
+    }
+    if (q) mprgp_qp_destroy(q);
+    if (m) mprgp_matrix_destroy(m);
+    status_ = rc;
+    return rc == MPRGP_OK;
+  }
+
+  double getResidual() const { return residual_; }
+  int getIterations() const { return iterations_; }
+  const std::vector<double> &getLambda() const { return lambda_; }
+  int status() const { return status_; }  // the last solve's mprgp status code
+
+ private:
+  int max_it_;
+  double tol_;
+  int gs_max_it_;
+  double gs_tol_;
+  detail::Csr a_;
+  std::vector<double> r_, lambda_;
+  int iterations_ = -1;
+  double residual_ = -1.0;
+  int status_ = MPRGP_OK;
 };
 
 }  // namespace mprgp_amd

@@ -451,6 +451,26 @@ def general_solve(qp, J, c, p: Params | None = None, **kw):
     return rc, st.as_dict()
 
 
+def ica_solve(qp, J, p, tol=1e-4, max_it=1000, gs_max_it=20, gs_tol=1e-3):
+    """ICASolver(max_it, tol, gs_max_it, gs_tol).reset(A, b).solve(J, p, x)
+    (ICASolver.h:287-415) on a QP's matrix, b and x (x overwritten), rows
+    J x >= p (scipy sparse, user DoF order).  Returns (rc, stats, lambda,
+    projected Gauss-Seidel sweeps)."""
+    import scipy.sparse as sp
+    Jc = sp.csr_matrix(J)
+    Jc.sort_indices()
+    m = Jc.shape[0]
+    jp, jc_, jv = _i32(Jc.indptr), _i32(Jc.indices), _f64(Jc.data)
+    pv = _f64(p, m) if m else np.zeros(1)
+    lam = np.zeros(max(m, 1))
+    sweeps = C.c_int(0)
+    st = Stats()
+    rc = load().mprgp_ica_solve(qp.h, int(m), _p(jp), _p(jc_), _p(jv), _p(pv), float(tol), int(max_it),
+                                int(gs_max_it), float(gs_tol), _p(lam), C.byref(sweeps), C.byref(st))
+    check(rc, allow=(_lib.OK, _lib.NOT_CONVERGED))
+    return rc, st.as_dict(), lam[:m], sweeps.value
+
+
 def solve_batch(ctx: Context, qps, p=None, per_qp=None, **kw):
     """Advances all QPs together; returns (rc, [stats dicts])."""
     n = len(qps)

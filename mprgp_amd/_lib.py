@@ -112,6 +112,7 @@ SIGNATURES = {
     "mprgp_matrix_set_lambda_min": (I, [P, C.c_double]),
     "mprgp_matrix_get_bsr3": (I, [P, P, P, P]),
     "mprgp_general_solve": (I, [P, I, P, P, P, P, C.POINTER(Params), C.POINTER(Stats)]),
+    "mprgp_ica_solve": (I, [P, I, P, P, P, P, C.c_double, I, I, C.c_double, P, C.POINTER(I), C.POINTER(Stats)]),
     "mprgp_matrix_spmv": (I, [P, P, P]),
     "mprgp_matrix_destroy": (None, [P]),
     "mprgp_qp_create": (I, [P, PP]),

@@ -288,6 +288,44 @@ struct General {
 
 }  // namespace
 
+namespace mprgp {
+
+int rows_to_internal(const mprgp_matrix *A, int m, const int *j_ptr, const int *j_col, const double *j_val,
+                     const double *c, RowsInternal &out) {
+  const int n = 3 * A->n_nodes;
+  const std::vector<int> &iperm = A->iperm;
+  const int nnz = m > 0 ? j_ptr[m] : 0;
+  if (m > 0 && (j_ptr[0] != 0 || nnz < 0)) return fail(MPRGP_EBADINPUT, "J row pointers must start at 0");
+  out.jc.assign(nnz, 0);
+  std::vector<int> tcount(n + 1, 0);
+  for (int i = 0; i < m; ++i) {
+    if (j_ptr[i + 1] < j_ptr[i]) return fail(MPRGP_EBADINPUT, "J row pointers not monotone at %d", i);
+    if (j_ptr[i + 1] == j_ptr[i]) return fail(MPRGP_EBADINPUT, "J row %d is empty", i);
+    if (!std::isfinite(c[i])) return fail(MPRGP_EBADINPUT, "c[%d] is not finite", i);
+    for (int k = j_ptr[i]; k < j_ptr[i + 1]; ++k) {
+      const int col = j_col[k];
+      if (col < 0 || col >= n) return fail(MPRGP_EBADINPUT, "J column %d out of range", col);
+      if (!std::isfinite(j_val[k])) return fail(MPRGP_EBADINPUT, "J value of row %d is not finite", i);
+      out.jc[k] = 3 * iperm[col / 3] + col % 3;
+      tcount[out.jc[k] + 1]++;
+    }
+  }
+  for (int i = 0; i < n; ++i) tcount[i + 1] += tcount[i];
+  out.tptr = tcount;
+  out.tc.assign(nnz, 0);
+  out.tv.assign(nnz, 0.0);
+  std::vector<int> fill(tcount.begin(), tcount.end() - 1);
+  for (int i = 0; i < m; ++i)
+    for (int k = j_ptr[i]; k < j_ptr[i + 1]; ++k) {
+      const int at = fill[out.jc[k]]++;
+      out.tc[at] = i;
+      out.tv[at] = j_val[k];
+    }
+  return MPRGP_OK;
+}
+
+}  // namespace mprgp
+
 extern "C" int mprgp_general_solve(mprgp_qp *q, int m, const int *j_ptr, const int *j_col, const double *j_val,
                                    const double *c, const mprgp_params *params, mprgp_stats *stats) {
   MPRGP_TRY
@@ -316,31 +354,11 @@ extern "C" int mprgp_general_solve(mprgp_qp *q, int m, const int *j_ptr, const i
   G.m = m;
   const int n = G.n;
   // J in the internal node order; J^T as a CSR of n rows
-  const std::vector<int> &iperm = q->A->iperm;
+  RowsInternal R;
+  if (int rc = rows_to_internal(q->A, m, j_ptr, j_col, j_val, c, R)) return rc;
   const int nnz = m > 0 ? j_ptr[m] : 0;
-  if (m > 0 && (j_ptr[0] != 0 || nnz < 0)) return fail(MPRGP_EBADINPUT, "J row pointers must start at 0");
-  std::vector<int> jc(nnz), tcount(n + 1, 0);
-  for (int i = 0; i < m; ++i) {
-    if (j_ptr[i + 1] < j_ptr[i]) return fail(MPRGP_EBADINPUT, "J row pointers not monotone at %d", i);
-    if (j_ptr[i + 1] == j_ptr[i]) return fail(MPRGP_EBADINPUT, "J row %d is empty", i);
-    if (!std::isfinite(c[i])) return fail(MPRGP_EBADINPUT, "c[%d] is not finite", i);
-    for (int k = j_ptr[i]; k < j_ptr[i + 1]; ++k) {
-      const int col = j_col[k];
-      if (col < 0 || col >= n) return fail(MPRGP_EBADINPUT, "J column %d out of range", col);
-      if (!std::isfinite(j_val[k])) return fail(MPRGP_EBADINPUT, "J value of row %d is not finite", i);
-      jc[k] = 3 * iperm[col / 3] + col % 3;
-      tcount[jc[k] + 1]++;
-    }
-  }
-  for (int i = 0; i < n; ++i) tcount[i + 1] += tcount[i];
-  std::vector<int> tptr(tcount), tc(nnz), fill(tcount.begin(), tcount.end() - 1);
-  std::vector<double> tv(nnz);
-  for (int i = 0; i < m; ++i)
-    for (int k = j_ptr[i]; k < j_ptr[i + 1]; ++k) {
-      const int at = fill[jc[k]]++;
-      tc[at] = i;
-      tv[at] = j_val[k];
-    }
+  const std::vector<int> &jc = R.jc, &tptr = R.tptr, &tc = R.tc;
+  const std::vector<double> &tv = R.tv;
   // J J^T (dense, host) by a column-dense pass over each row
   G.c.assign(c, c + m);
   G.JJt.assign((size_t)m * m, 0.0);

@@ -53,6 +53,19 @@ __host__ __device__ inline double power_start_raw(uint64_t i) {
 }
 double power_start_norm(size_t n);
 
+// Rows J x >= c given over the user's DoFs (CSR, m rows): J's columns in a
+// QP's internal DoF order (jc, same entry order), and J^T as a CSR of
+// 3 * nodes rows whose entries run over J's rows in ascending order.
+// Validates the arrays (EBADINPUT on empty rows, bad columns, non-finite
+// values).
+struct RowsInternal {
+  std::vector<int> jc;
+  std::vector<int> tptr, tc;
+  std::vector<double> tv;
+};
+int rows_to_internal(const mprgp_matrix *A, int m, const int *j_ptr, const int *j_col, const double *j_val,
+                     const double *c, RowsInternal &out);
+
 // kernel launch wrappers (mprgp_kernels.hip)
 // coupled-row solve: out = M x for a CSR M of `rows` rows; out = a x + b y;
 // out = x y (elementwise)

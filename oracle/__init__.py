@@ -104,6 +104,14 @@ def lib():
         L.orc_gnum_active.argtypes = [P]
         L.orc_general_solve.argtypes = [C.POINTER(_CSR), P, C.POINTER(_CSR), P, P, D, I, I, D, P, P, I,
                                         C.POINTER(Stats)]
+        L.orc_inv3.argtypes = [P, P]
+        L.orc_pgs.argtypes = [C.POINTER(_CSR), P, P, I, D, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.orc_pgs.restype = I
+        L.orc_ica_b.argtypes = [C.POINTER(_CSR), I, P, C.POINTER(C.c_int), C.POINTER(C.POINTER(C.c_int)),
+                                C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.POINTER(C.c_double))]
+        L.orc_ica_free.argtypes = [P]
+        L.orc_ica_solve.argtypes = [C.POINTER(_CSR), P, C.POINTER(_CSR), P, P, D, I, I, D, P,
+                                    C.POINTER(IcaStats)]
         L.orc_fem_setup.argtypes = [I, I, P, P, P, P, P]
         L.orc_polar_rotation.argtypes = [P, P]
         L.orc_fem_element.argtypes = [I, P, P, D, D, D, P, P]
@@ -396,6 +404,67 @@ def general_solve(A, B, J, c, x, tol=1e-3, max_it=1000, preconditioned=True, alp
                                    int(max_it), int(preconditioned), float(alpha_bar), _ptr(evb),
                                    C.byref(valid), int(project_first), C.byref(st))
     return code, x, st.as_dict()
+
+
+# ------------------------------------------------------------------ ICA
+
+class IcaStats(C.Structure):
+    _fields_ = [("code", C.c_int), ("iterations", C.c_int), ("gs_iterations", C.c_int),
+                ("residual", C.c_double), ("gs_residual", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def inv3(m):
+    """3x3 inverse by the adjugate (the ICA comparator's Da^-1 blocks)."""
+    m = _f64(np.asarray(m).reshape(9))
+    out = np.empty(9)
+    lib().orc_inv3(_ptr(m), _ptr(out))
+    return out.reshape(3, 3)
+
+
+def pgs(B, c, lam0=None, max_it=20, tol=1e-3):
+    """ProjectedGaussSeidel::solve (ICASolver.h:68-111) on B (CSR) from lam0.
+    Returns (tolerance met, lambda, sweep index at the stop, residual)."""
+    b = _Csr(B)
+    c = _f64(c)
+    lam = np.zeros(B.shape[0]) if lam0 is None else _f64(lam0).copy()
+    it, res = C.c_int(), C.c_double()
+    met = lib().orc_pgs(C.byref(b.s), _ptr(c), _ptr(lam), int(max_it), float(tol), C.byref(it), C.byref(res))
+    return bool(met), lam, it.value, res.value
+
+
+def ica_b(J, n_nodes, dinv):
+    """B = J Da^-1 J^T (scipy CSR) from the per-node inverse blocks dinv (n_nodes x 3 x 3)."""
+    j = _Csr(J, cols=3 * n_nodes)
+    d = _f64(np.asarray(dinv).reshape(-1))
+    nnz = C.c_int()
+    pp, pi, pv = C.POINTER(C.c_int)(), C.POINTER(C.c_int)(), C.POINTER(C.c_double)()
+    lib().orc_ica_b(C.byref(j.s), int(n_nodes), _ptr(d), C.byref(nnz), C.byref(pp), C.byref(pi), C.byref(pv))
+    m = J.shape[0]
+    ptr = np.ctypeslib.as_array(pp, (m + 1,)).copy()
+    idx = np.ctypeslib.as_array(pi, (max(nnz.value, 1),))[:nnz.value].copy()
+    val = np.ctypeslib.as_array(pv, (max(nnz.value, 1),))[:nnz.value].copy()
+    for q in (pp, pi, pv):
+        lib().orc_ica_free(C.cast(q, C.c_void_p))
+    return sp.csr_matrix((val, idx, ptr), shape=(m, m))
+
+
+def ica_solve(A, R, J, p, x, tol=1e-4, max_it=1000, gs_max_it=20, gs_tol=1e-3):
+    """ICASolver(max_it, tol, gs_max_it, gs_tol).reset(A, R) + solve(J, p, x)
+    (ICASolver.h:287-415) restated.  Returns (code, x, lambda, stats-dict)."""
+    a = _Csr(A)
+    n = A.shape[0]
+    J = sp.csr_matrix((0, n)) if J is None else J
+    j = _Csr(J, cols=n)
+    R, p = _f64(R), _f64(p) if len(p) else np.zeros(1)
+    x = _f64(x).copy()
+    lam = np.zeros(max(J.shape[0], 1))
+    st = IcaStats()
+    code = lib().orc_ica_solve(C.byref(a.s), _ptr(R), C.byref(j.s), _ptr(p), _ptr(x), float(tol), int(max_it),
+                               int(gs_max_it), float(gs_tol), _ptr(lam), C.byref(st))
+    return code, x, lam[:J.shape[0]], st.as_dict()
 
 
 # ------------------------------------------------------------------ FEM

@@ -164,6 +164,23 @@ int orc_general_solve(const orc_csr *A, const double *B, const orc_csr *J, const
                       int max_it, int preconditioned, double alpha_bar, double *ev, int *ev_valid,
                       int project_first, orc_stats *st);
 
+/* ---- ICA comparator (ica_oracle.c) ------------------------------------ */
+/* ICASolver (ICASolver.h:287-463) with its ProjectedGaussSeidel (:20-150). */
+typedef struct {
+  int code;           /* 0: |A x - R - J^T lambda| <= tol reached, -1 not        */
+  int iterations;     /* the reference's `iterations` (index at the stop, or max) */
+  int gs_iterations;  /* projected Gauss-Seidel sweeps run, summed over calls    */
+  double residual;    /* |A x - R - J^T lambda| at exit                          */
+  double gs_residual; /* the last PGS call's |lambda - lambda_prev|              */
+} orc_ica_stats;
+void orc_inv3(const double *m, double *inv);
+int orc_pgs(const orc_csr *B, const double *c, double *lam, int max_it, double tol, int *iters, double *res);
+/* B = J Da^-1 J^T as CSR (arrays malloc'd; free with orc_ica_free) */
+int orc_ica_b(const orc_csr *J, int n_nodes, const double *dinv, int *nnz_out, int **ptr, int **idx, double **val);
+void orc_ica_free(void *p);
+int orc_ica_solve(const orc_csr *A, const double *R, const orc_csr *J, const double *p, double *x, double tol,
+                  int max_it, int gs_max_it, double gs_tol, double *lambda, orc_ica_stats *st);
+
 /* ---- FEM assembly (fem_oracle.c) -------------------------------------- */
 enum { ORC_LINEAR = 1, ORC_COROTATIONAL = 2 };
 

@@ -121,5 +121,15 @@ int main(int argc, char **argv) {
   d.resize(n);
   for (int i = 0; i < n; ++i) d.v[i] = FA.diag(i);
   emit("diag", 0, d);
+
+  // the ICA comparator as ICAFemSolver::forward calls it, from the start x0
+  mprgp_amd::ICASolver ica(max_it, tol);
+  ica.reset(A, B);
+  Vec xi = x0;
+  const bool ok = ica.solve(J, c, xi);
+  emit("ica", ok ? 0 : -1, xi);
+  Vec lam;
+  lam.v = ica.getLambda();
+  emit("ica_lambda", ica.getIterations(), lam);
   return 0;
 }

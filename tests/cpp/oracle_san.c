@@ -99,10 +99,20 @@ int main(int argc, char **argv) {
   orc_precond_solve(pc, P, g, z, phi);
   const double fv = orc_func_value(&A, y, b);
   orc_get_diagonal(&A, z);
+  /* the coupled-row solve and the ICA comparator on the same rows */
+  orc_stats gst;
+  double *xg = xmalloc(sizeof(double) * n), *lamv = xmalloc(sizeof(double) * (m + 1));
+  memcpy(xg, xn, sizeof(double) * n);
+  const int gcode = orc_general_solve(&A, b, &J, c, xg, 1e-8, 2000, 1, 2.0 / lam, NULL, NULL, 1, &gst);
+  orc_ica_stats ist;
+  memcpy(xg, xn, sizeof(double) * n);
+  const int icode = orc_ica_solve(&A, b, &J, c, xg, 1e-8, 2000, 20, 1e-3, lamv, &ist);
   printf("oracle_san ok: N=%d T=%d rows=%d lambda=%.6e codes=%d,%d,%d iters=%d,%d,%d active=%d "
-         "feasible=%d phitphi=%.3e step=%.3e f=%.6e\n",
+         "feasible=%d phitphi=%.3e step=%.3e f=%.6e general=%d/%d ica=%d/%d/%d\n",
          N, T, m, lam, codes[0], codes[1], codes[2], st[0].iterations, st[1].iterations, st[2].iterations,
-         orc_num_active(P), orc_is_feasible(P, y), pp, sl, fv);
+         orc_num_active(P), orc_is_feasible(P, y), pp, sl, fv, gcode, gst.iterations, icode, ist.iterations,
+         ist.gs_iterations);
+  free(xg); free(lamv);
   orc_precond_free(pc);
   orc_projector_free(P);
   orc_free(ptr); orc_free(idx); orc_free(val); orc_free(ptr2); orc_free(idx2); orc_free(val2);

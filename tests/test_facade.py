@@ -1,8 +1,8 @@
 """The C++ facade (include/mprgp_amd.hpp), the reference's operator API on
 the C ABI.  CPU: the header compiles on its own with the reference's call
 shapes.  GPU: tests/cpp/facade_check (built by build()) runs the three solve
-overloads, the projector and the matrix methods on a QP file; each result is
-checked against the oracle and the Python drop-in."""
+overloads, the projector, the matrix methods and the ICA comparator on a QP
+file; each result is checked against the oracle and the Python drop-in."""
 import os
 import shutil
 import subprocess
@@ -86,3 +86,11 @@ def test_facade_overloads_match_oracle():
     assert abs(fv - oracle.func_value(A, x, b)) <= 1e-13 * (np.abs(x) @ (abs(A) @ np.abs(x)) + np.abs(x) @ np.abs(b))
     _, dg = vec("diag")
     assert np.array_equal(dg, A.diagonal())
+
+    # ICASolver (ICAFemSolver's comparator) against the oracle's ICA
+    code_i, xi, li, si = oracle.ica_solve(A, b, J, c, x0.copy(), tol=tol, max_it=max_it)
+    ci, x_ica = vec("ica")
+    it_i, lam_ica = vec("ica_lambda")
+    assert code_i == 0 and ci == 0 and it_i == si["iterations"]
+    assert np.linalg.norm(x_ica - xi) <= 1e-10 * np.linalg.norm(xi)
+    np.testing.assert_allclose(lam_ica, li, rtol=1e-10, atol=1e-14)

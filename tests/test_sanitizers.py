@@ -39,7 +39,8 @@ def _build(out, cmd):
 
 @pytest.fixture(scope="module")
 def oracle_san():
-    src = [os.path.join(ROOT, "oracle", f) for f in ("mprgp_oracle.c", "fem_oracle.c", "collide_oracle.c")]
+    src = [os.path.join(ROOT, "oracle", f) for f in ("mprgp_oracle.c", "fem_oracle.c", "collide_oracle.c",
+                                                      "general_oracle.c", "ica_oracle.c")]
     return _build("oracle_san", ["gcc", "-std=c99", "-ffp-contract=off", *SAN, *src,
                                  os.path.join(ROOT, "tests", "cpp", "oracle_san.c"), "-lm"])
 
