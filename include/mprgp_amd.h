@@ -85,7 +85,9 @@ typedef struct {
   int max_refine;      /* MIXED: outer refinement sweeps (0 = default 8)             */
   int tail_tiles;      /* launches of at most this many tiles run the step kernel's  */
                        /* tail form (same results); 0 = 256, one per CU              */
-  int reserved_i[4];   /* set to 0                                                   */
+  int xcd_layout;      /* workgroups over the 8 XCDs (same results): 0 runs of 4     */
+                       /* tiles rotating, 1 one contiguous chunk of each QP per XCD  */
+  int reserved_i[3];   /* set to 0                                                   */
   double inner_tol;    /* MIXED: inner tolerance factor (0 = default 1e-3 relative)  */
   double reserved_d[3];
 } mprgp_params;

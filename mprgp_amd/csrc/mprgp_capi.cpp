@@ -191,8 +191,17 @@ int mprgp::build_matrix(mprgp_ctx *ctx, int n_nodes, const int *rowptr, const in
   A->perm.resize(n_nodes);
   std::iota(A->perm.begin(), A->perm.end(), 0);
   // internal order: descending block-row length, stable (keeps mesh locality
-  // within each length class)
-  std::stable_sort(A->perm.begin(), A->perm.end(), [&](int a, int b) { return len[a] > len[b]; });
+  // within each length class), over the whole matrix or, when
+  // MPRGP_SELL_SIGMA is set, within windows of that many consecutive nodes
+  {
+    const char *e = getenv("MPRGP_SELL_SIGMA");
+    const long long sigma = (e && e[0]) ? std::max(0LL, atoll(e)) : 0;
+    const long long w = sigma > 0 ? sigma : (long long)n_nodes;
+    for (long long a0 = 0; a0 < n_nodes; a0 += w) {
+      const long long a1 = std::min<long long>(n_nodes, a0 + w);
+      std::stable_sort(A->perm.begin() + a0, A->perm.begin() + a1, [&](int a, int b) { return len[a] > len[b]; });
+    }
+  }
   A->iperm.resize(n_nodes);
   for (int i = 0; i < n_nodes; ++i) A->iperm[A->perm[i]] = i;
   A->n_slices = (n_nodes + kSlice - 1) / kSlice;
@@ -1192,7 +1201,41 @@ int round_grid(int n_tiles) {
 // per-QP work differs by op and one XCD's share then sets the launch time.
 constexpr int kXcds = 8;
 constexpr int kXcdRun = 4;
-void layout_tiles(const std::vector<Tile> &tiles, int grid, Tile *table) {
+// layout 1 (per-QP chunks): each QP's tiles split into kXcds contiguous
+// chunks, one per XCD, so an XCD gathers from one contiguous part of each
+// QP's vectors; chunk sizes differ by at most one tile, the larger ones going
+// to the XCDs with the fewest tiles so far.  Falls back to the runs when the
+// chunks would not fit the table.
+static bool layout_chunks(const std::vector<Tile> &tiles, int grid, Tile *table) {
+  const int n = (int)tiles.size(), per = grid / kXcds;
+  std::vector<int> len(kXcds, 0);
+  std::vector<std::vector<int>> lists(kXcds);
+  for (int i0 = 0; i0 < n;) {
+    int i1 = i0;
+    while (i1 < n && tiles[i1].qp == tiles[i0].qp) ++i1;
+    const int nt = i1 - i0, base = nt / kXcds, rem = nt % kXcds;
+    // XCDs in ascending order of their lists' length (ties by index)
+    int order[kXcds];
+    for (int x = 0; x < kXcds; ++x) order[x] = x;
+    std::stable_sort(order, order + kXcds, [&](int a, int b) { return len[a] < len[b]; });
+    int at = i0;
+    for (int k = 0; k < kXcds; ++k) {
+      const int x = order[k], sz = base + (k < rem ? 1 : 0);
+      for (int j = 0; j < sz; ++j) lists[x].push_back(at++);
+      len[x] += sz;
+    }
+    i0 = i1;
+  }
+  for (int x = 0; x < kXcds; ++x)
+    if (len[x] > per) return false;
+  for (int b = 0; b < grid; ++b) table[b] = Tile{nullptr, nullptr, -1, 0, 0, 0, 0, 0};
+  for (int x = 0; x < kXcds; ++x)
+    for (int pos = 0; pos < len[x]; ++pos) table[pos * kXcds + x] = tiles[lists[x][pos]];
+  return true;
+}
+
+void layout_tiles(const std::vector<Tile> &tiles, int grid, Tile *table, int layout) {
+  if (layout == 1 && layout_chunks(tiles, grid, table)) return;
   const int n = (int)tiles.size();
   // grid is a power of two >= 64, so c * kXcds divides it and every slot
   // below lands inside the table
@@ -1398,7 +1441,8 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
   if (int rc = tab.ensure(count, grid)) return rc;
   if (int rc = tab.ensure_states(count)) return rc;
   std::vector<std::vector<Tile>> uploads(1, std::vector<Tile>(grid));  // host buffers of table uploads
-  layout_tiles(tiles, grid, uploads.back().data());
+  const int layout = params[0].xcd_layout;
+  layout_tiles(tiles, grid, uploads.back().data(), layout);
   hipStream_t s = ctx->stream;
   std::vector<QpDev> devs(count);
   QpState *states = tab.h_states;  // pinned; [i][2]
@@ -1493,7 +1537,7 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
           if (keep[i]) append_tiles(i, qps[i], span, kept);
         grid = round_grid((int)kept.size());  // <= the table's capacity: the tile count only shrinks
         uploads.emplace_back(grid);
-        layout_tiles(kept, grid, uploads.back().data());
+        layout_tiles(kept, grid, uploads.back().data(), layout);
         MPRGP_HIP(hipMemcpyAsync(tab.d_tiles, uploads.back().data(), sizeof(Tile) * grid, hipMemcpyHostToDevice, s));
         MPRGP_HIP(hipStreamSynchronize(s));
         in_table = live;

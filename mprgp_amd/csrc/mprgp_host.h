@@ -157,7 +157,9 @@ inline int step_variant(int n_tiles, double matrix_bytes, int tail_tiles = kTail
   if (n_tiles <= tail_tiles) return kStepTail;
   return matrix_bytes > stream_threshold() ? kStepStream : kStepCached;
 }
-void layout_tiles(const std::vector<Tile> &tiles, int grid, Tile *table);
+// layout 0: runs of kXcdRun tiles rotating over the XCDs; 1: per-QP chunks
+// (one contiguous chunk of every QP per XCD).  Results do not depend on it.
+void layout_tiles(const std::vector<Tile> &tiles, int grid, Tile *table, int layout = 0);
 long solve_step_bound(const mprgp_params &p);
 size_t solve_bytes(const mprgp_qp *q, const mprgp_params &p);
 // the descriptor and initial state only (no device work; a mixed-form QP's

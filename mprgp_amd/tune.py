@@ -2,14 +2,16 @@
 
 The step kernel's shape is the number of wavefronts per workgroup (one
 SELL slice of 64 block rows per wavefront, so a workgroup's tile is that many
-slices), the number of launches captured per hipGraph replay, and the tile
+slices), the number of launches captured per hipGraph replay, the tile
 count at or below which a launch runs the kernel's tail form (the batch's
-last, latency-bound launches, once only the stiffest QPs are left).  Every
-candidate changes only the schedule (and, through the tile grouping of
-partial sums, the last bits of the reductions; the tail switch point not
-even those), never the algorithm.  ``tune`` times each candidate on the
-caller's own batch from the same warm start and keeps the fastest: first the
-wavefront/graph shapes, then the tail switch points at the best of them.
+last, latency-bound launches, once only the stiffest QPs are left), and the
+placement of the tiles on the 8 XCDs.  Every candidate changes only the
+schedule (and, through the tile grouping of partial sums, the last bits of
+the reductions; the tail switch point and the placement not even those),
+never the algorithm.  ``tune`` times each candidate on the caller's own
+batch from the same warm start and keeps the fastest: first the
+wavefront/graph shapes, then the tail switch points at the best of them,
+then the placements.
 ``tuned_params`` looks the answer up in a JSON cache keyed by device, batch
 size class and precision, tuning on a miss.
 """
@@ -36,8 +38,12 @@ DEFAULT_CANDIDATES = [
 # tiles, one workgroup per CU).
 TAIL_CANDIDATES = [0, 64, 128, 512, 1024, 2048]
 
+# Placements of the tiles on the 8 XCDs tried last (0: runs of 4 tiles
+# rotating over the XCDs, 1: one contiguous chunk of each QP per XCD).
+LAYOUT_CANDIDATES = [0, 1]
+
 # Bumped whenever the step kernel changes in a way that moves the best shape.
-CACHE_VERSION = 5
+CACHE_VERSION = 6
 
 
 def cache_path():
@@ -57,17 +63,19 @@ def apply(p: M.Params, shape):
     p.tiles = int(shape["tiles"])
     p.graph_steps = int(shape["graph_steps"])
     p.tail_tiles = int(shape.get("tail_tiles", 0))
+    p.xcd_layout = int(shape.get("xcd_layout", 0))
     return p
 
 
-def tune(ctx, qps, per_qp, candidates=None, tails=None, repeats=2, log=None):
+def tune(ctx, qps, per_qp, candidates=None, tails=None, layouts=None, repeats=2, log=None):
     """Times every candidate shape on the batch (each QP restarted from its
     warm start with reset_x), keeps each candidate's best of `repeats` timed
     runs (single runs differ by about 1%, as much as the candidates do), then
-    the tail switch points `tails` at the best shape; returns (best_shape,
-    [(shape, ms)])."""
+    the tail switch points `tails` at the best shape, then the XCD layouts
+    `layouts` at the best of those; returns (best_shape, [(shape, ms)])."""
     candidates = candidates or DEFAULT_CANDIDATES
     tails = TAIL_CANDIDATES if tails is None else tails
+    layouts = LAYOUT_CANDIDATES if layouts is None else layouts
     results = []
     timed = _timer(ctx, qps, per_qp, repeats, log, results)
     for shape in candidates:
@@ -76,6 +84,10 @@ def tune(ctx, qps, per_qp, candidates=None, tails=None, repeats=2, log=None):
     for t in tails:
         if t != best.get("tail_tiles", 0):
             timed(dict(best, tail_tiles=t))
+    best = min(results, key=lambda r: r[1])[0]
+    for lay in layouts:
+        if lay != best.get("xcd_layout", 0):
+            timed(dict(best, xcd_layout=lay))
     for q in qps:
         q.reset_x()
     return min(results, key=lambda r: r[1])[0], results

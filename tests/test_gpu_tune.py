@@ -26,22 +26,24 @@ def per_qp(ab, prec, **kw):
 
 
 @pytest.mark.parametrize("prec", [M.FP64, M.MIXED])
-def test_tail_switch_point_changes_no_bits(ctx, batch, prec):
+def test_tail_switch_point_and_placement_change_no_bits(ctx, batch, prec):
     """tail_tiles = 1 (the tail form almost never), 0 (the default, 256) and
-    2^30 (every launch) give the same bits and iteration counts."""
+    2^30 (every launch), and both XCD placements, give the same bits and
+    iteration counts."""
     qps, ab = batch
     out = {}
-    for tail in (1, 0, 1 << 30):
+    for key in ((1, 0), (0, 0), (1 << 30, 0), (0, 1), (1 << 30, 1)):
+        tail, lay = key
         for q in qps:
             q.qp.reset_x()
         rc, st = M.solve_batch(ctx, [q.qp for q in qps], per_qp=per_qp(ab, prec, tiles=2, graph_steps=16,
-                                                                        tail_tiles=tail))
-        assert all(s["code"] == 0 for s in st), tail
-        out[tail] = ([q.qp.get_x() for q in qps], [s["iterations"] for s in st])
-    for tail in (0, 1 << 30):
-        for a, b in zip(out[1][0], out[tail][0]):
+                                                                        tail_tiles=tail, xcd_layout=lay))
+        assert all(s["code"] == 0 for s in st), key
+        out[key] = ([q.qp.get_x() for q in qps], [s["iterations"] for s in st])
+    for key in list(out)[1:]:
+        for a, b in zip(out[(1, 0)][0], out[key][0]):
             np.testing.assert_array_equal(a, b)
-        assert out[tail][1] == out[1][1]
+        assert out[key][1] == out[(1, 0)][1]
 
 
 def test_tuner_caches_and_candidates_agree(ctx, batch, tmp_path):
@@ -49,23 +51,25 @@ def test_tuner_caches_and_candidates_agree(ctx, batch, tmp_path):
     path = str(tmp_path / "tune.json")
     cands = [{"tiles": 2, "graph_steps": 16}, {"tiles": 4, "graph_steps": 8}, {"tiles": 1, "graph_steps": 32}]
     base = per_qp(ab, M.FP64)
-    saved = (tune.DEFAULT_CANDIDATES, tune.TAIL_CANDIDATES)
-    tune.DEFAULT_CANDIDATES, tune.TAIL_CANDIDATES = cands, [0, 1024]
+    saved = (tune.DEFAULT_CANDIDATES, tune.TAIL_CANDIDATES, tune.LAYOUT_CANDIDATES)
+    tune.DEFAULT_CANDIDATES, tune.TAIL_CANDIDATES, tune.LAYOUT_CANDIDATES = cands, [0, 1024], [0, 1]
     try:
         ps, shape, hit = tune.tuned_params(ctx, [q.qp for q in qps], base, M.FP64, path=path)
         assert not hit
         ps2, shape2, hit2 = tune.tuned_params(ctx, [q.qp for q in qps], base, M.FP64, path=path)
     finally:
-        tune.DEFAULT_CANDIDATES, tune.TAIL_CANDIDATES = saved
+        tune.DEFAULT_CANDIDATES, tune.TAIL_CANDIDATES, tune.LAYOUT_CANDIDATES = saved
     assert hit2 and shape2 == shape
     with open(path) as f:
         cache = json.load(f)
     key = tune.batch_key([q.qp for q in qps], M.FP64)
     assert cache[key]["shape"] == shape
     tried = [s for s, _ in cache[key]["ms"]]
-    assert len(tried) == 4 and all(s in tried for s in cands)  # 3 shapes + 1 tail point at the best
+    # 3 shapes, 1 more tail point at the best, 1 more placement at the best of those
+    assert len(tried) == 5 and all(s in tried for s in cands)
     assert all(p.tiles == shape["tiles"] and p.graph_steps == shape["graph_steps"]
-               and p.tail_tiles == shape.get("tail_tiles", 0) for p in ps)
+               and p.tail_tiles == shape.get("tail_tiles", 0) and p.xcd_layout == shape.get("xcd_layout", 0)
+               for p in ps)
     # each tried shape: converged, and within the two solves' error bounds of
     # the tuned shape's solution (|x - x*| <= error_bound for each)
     sols = {}

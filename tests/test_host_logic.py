@@ -73,8 +73,8 @@ def test_tune_keys_and_shapes():
     assert tune.batch_key(b, M.MIXED) == tune.batch_key(a, M.MIXED)  # same size class
     p = tune.apply(M.params(), {"tiles": 8, "graph_steps": 32})
     assert (p.tiles, p.graph_steps, p.tail_tiles) == (8, 32, 0)
-    p = tune.apply(M.params(), {"tiles": 2, "graph_steps": 16, "tail_tiles": 1024})
-    assert (p.tiles, p.graph_steps, p.tail_tiles) == (2, 16, 1024)
+    p = tune.apply(M.params(), {"tiles": 2, "graph_steps": 16, "tail_tiles": 1024, "xcd_layout": 1})
+    assert (p.tiles, p.graph_steps, p.tail_tiles, p.xcd_layout) == (2, 16, 1024, 1)
     assert 0 in tune.TAIL_CANDIDATES and all(t >= 0 for t in tune.TAIL_CANDIDATES)
     for s in tune.DEFAULT_CANDIDATES:
         assert s["tiles"] in (1, 2, 4, 8) and s["graph_steps"] > 0
