@@ -56,14 +56,16 @@ def pmc_child(args):
     qps = [workloads.DeviceQP(ctx, sc) for sc in scenes]
     if args.pmc_child == "spmv":
         per = [M.params(tol=1e-6, max_it=0, power_tol=1e-300, precision=M.FP64, tiles=args.ts or 4,
-                        graph_steps=-16, xcd_layout=args.xcd_layout) for _ in qps]
+                        graph_steps=-16, xcd_layout=args.xcd_layout, slices_per_wave=args.slices_per_wave)
+               for _ in qps]
         rc, stats = M.solve_batch(ctx, [q.qp for q in qps], per_qp=per)
         print(json.dumps({"launches": stats[0]["launches"]}), flush=True)
         return
     rc, st = M.solve_batch(ctx, [q.qp for q in qps], M.params(tol=args.tol, max_it=0, power_tol=1e-6))
     prec = M.MIXED if args.precision == "mixed" else M.FP64
     per = [M.params(tol=args.tol, max_it=4000, alpha_bar=s["alpha_bar"], project_first=1, precision=prec,
-                    tiles=args.ts or 4, graph_steps=-16, tail_tiles=args.tail_tiles, xcd_layout=args.xcd_layout)
+                    tiles=args.ts or 4, graph_steps=-16, tail_tiles=args.tail_tiles, xcd_layout=args.xcd_layout,
+                    slices_per_wave=args.slices_per_wave)
            for s in st]
     rc, stats = M.solve_batch(ctx, [q.qp for q in qps], per_qp=per)
     print(json.dumps({"launches": stats[0]["launches"]}), flush=True)
@@ -97,7 +99,7 @@ def measure_traffic(args, tiles, what="solve"):
                "-d", d, "-o", "run", "--", sys.executable, "-u", os.path.abspath(__file__),
                "--pmc-child", child, "--qps", str(args.qps), "--tol", str(args.tol),
                "--precision", args.precision, "--ts", str(tiles), "--tail-tiles", str(args.tail_tiles),
-               "--xcd-layout", str(args.xcd_layout)]
+               "--xcd-layout", str(args.xcd_layout), "--slices-per-wave", str(args.slices_per_wave)]
         env = dict(os.environ, MPRGP_NO_GRAPH="1")
         r = subprocess.run(cmd, capture_output=True, text=True, env=env)
         if r.returncode != 0:
@@ -301,6 +303,7 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=0)
     ap.add_argument("--tail-tiles", type=int, default=0, help="tail switch point (0: library default)")
     ap.add_argument("--xcd-layout", type=int, default=0, help="0 runs of tiles over the XCDs, 1 per-QP chunks")
+    ap.add_argument("--slices-per-wave", type=int, default=1, help="SELL slices each wavefront streams per launch")
     ap.add_argument("--no-tune", action="store_true", help="library default shape instead of the autotuner's")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM counter passes")
     ap.add_argument("--no-spmv", action="store_true", help="skip the batched SpMV bandwidth leg")
@@ -344,7 +347,8 @@ def main():
     def batch_params(precision):
         return [M.params(tol=args.tol, max_it=4000, alpha_bar=q.alpha_bar, project_first=1,
                          precision=precision, tiles=args.ts, graph_steps=args.graph_steps,
-                         tail_tiles=args.tail_tiles, xcd_layout=args.xcd_layout) for q in qps]
+                         tail_tiles=args.tail_tiles, xcd_layout=args.xcd_layout,
+                         slices_per_wave=args.slices_per_wave) for q in qps]
 
     per = batch_params(prec)
     shape = None
@@ -379,6 +383,7 @@ def main():
     if shape:  # the counter passes run the tuned shape
         args.tail_tiles = int(shape.get("tail_tiles", 0))
         args.xcd_layout = int(shape.get("xcd_layout", 0))
+        args.slices_per_wave = int(shape.get("slices_per_wave", 1))
     traffic = None if args.no_traffic else measure_traffic(args, tiles)
     if traffic:
         log(f"traffic per launch (mprgp_step<{tiles}, ...>): {traffic}")

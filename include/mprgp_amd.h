@@ -87,7 +87,10 @@ typedef struct {
                        /* tail form (same results); 0 = 256, one per CU              */
   int xcd_layout;      /* workgroups over the 8 XCDs (same results): 0 runs of 4     */
                        /* tiles rotating, 1 one contiguous chunk of each QP per XCD  */
-  int reserved_i[3];   /* set to 0                                                   */
+  int slices_per_wave; /* SELL slices (64 rows) each wavefront streams per launch:   */
+                       /* 0 or 1, or 2 (rows per wavefront = 64 x this; a tile is    */
+                       /* tiles x this slices; like tiles, it groups the sums)       */
+  int reserved_i[2];   /* set to 0                                                   */
   double inner_tol;    /* MIXED: inner tolerance factor (0 = default 1e-3 relative)  */
   double reserved_d[3];
 } mprgp_params;

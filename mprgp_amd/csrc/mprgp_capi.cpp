@@ -1130,8 +1130,9 @@ int BatchTables::ensure(int n_qps, int n_slots) {
   return MPRGP_OK;
 }
 
-int BatchTables::graph(hipStream_t s, int ts, int steps, int grid, bool mixed, int variant, hipGraphExec_t *out) {
-  auto key = std::make_tuple(ts, steps, grid, (int)mixed, variant);
+int BatchTables::graph(hipStream_t s, int ts, int steps, int grid, bool mixed, int variant, hipGraphExec_t *out,
+                       int spw) {
+  auto key = std::make_tuple(ts, steps, grid, (int)mixed, variant, spw);
   auto it = graphs.find(key);
   if (it != graphs.end()) {
     *out = it->second;
@@ -1139,7 +1140,7 @@ int BatchTables::graph(hipStream_t s, int ts, int steps, int grid, bool mixed, i
   }
   MPRGP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   for (int i = 0; i < steps; ++i) {
-    hipError_t e = launch_step(ts, grid, s, d_qps, d_tiles, d_ctrl + 1, i & 1, mixed, variant);
+    hipError_t e = launch_step(ts, grid, s, d_qps, d_tiles, d_ctrl + 1, i & 1, mixed, variant, spw);
     if (e != hipSuccess) {
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(s, &g);
@@ -1259,6 +1260,8 @@ int check_batch_qp(const mprgp_ctx *ctx, const mprgp_qp *q, const mprgp_params &
     return fail(MPRGP_EBADINPUT, "qp %d: unknown variant %d", i, p.variant);
   if (p.precision == MPRGP_MIXED && p.variant != MPRGP_MONOTONIC)
     return fail(MPRGP_EBADINPUT, "qp %d: the mixed form runs the monotone variant", i);
+  if (!(p.slices_per_wave == 0 || p.slices_per_wave == 1 || p.slices_per_wave == 2))
+    return fail(MPRGP_EBADINPUT, "qp %d: slices_per_wave must be 1 or 2 (0: 1)", i);
   return MPRGP_OK;
 }
 
@@ -1432,7 +1435,8 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
     max_steps = std::max(max_steps, solve_step_bound(p));
   }
   const int ts = pick_ts(params[0], total_slices);
-  const int span = ts;
+  const int spw = slices_per_wave_of(params[0]);
+  const int span = ts * spw;
   std::vector<Tile> tiles;
   for (int i = 0; i < count; ++i) append_tiles(i, qps[i], span, tiles);
   BatchTables &tab = ctx->tab;
@@ -1494,7 +1498,7 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
   int variant = step_variant(n_tiles, live_bytes(nullptr), tail_tiles);
   hipGraphExec_t ex = nullptr;
   if (use_graph)
-    if (int rc = tab.graph(s, ts, steps, grid, any_mixed, variant, &ex)) return rc;
+    if (int rc = tab.graph(s, ts, steps, grid, any_mixed, variant, &ex, spw)) return rc;
   int in_table = count;  // QPs whose tiles are in the table
   std::vector<int> flags(count);
 
@@ -1507,7 +1511,7 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
       MPRGP_HIP(hipGraphLaunch(ex, s));
     } else {
       for (int i = 0; i < steps; ++i)
-        MPRGP_HIP(launch_step(ts, grid, s, tab.d_qps, tab.d_tiles, tab.d_ctrl + 1, i & 1, any_mixed, variant));
+        MPRGP_HIP(launch_step(ts, grid, s, tab.d_qps, tab.d_tiles, tab.d_ctrl + 1, i & 1, any_mixed, variant, spw));
     }
     launched += steps;
     MPRGP_HIP(hipMemcpyAsync(ctx->h_running + slot, tab.d_ctrl + 1, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1543,7 +1547,7 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
         in_table = live;
         variant = step_variant((int)kept.size(), live_bytes(&keep), tail_tiles);
         if (use_graph)
-          if (int rc = tab.graph(s, ts, steps, grid, any_mixed, variant, &ex)) return rc;
+          if (int rc = tab.graph(s, ts, steps, grid, any_mixed, variant, &ex, spw)) return rc;
       }
       // restart the two-deep polling pipeline
       if (int rc = enqueue(k & 1)) return rc;

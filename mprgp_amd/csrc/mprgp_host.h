@@ -76,8 +76,9 @@ hipError_t launch_vmul(hipStream_t s, size_t n, const double *x, const double *y
 // out[i] = power_start_raw(i) / nr for i < n (out as is when nr = 0)
 hipError_t launch_power_start(hipStream_t s, double *out, size_t n, double nr);
 // variant: a StepVariant (mprgp_dev.h), from step_variant() below
+// spw: slices per wavefront (1 or 2)
 hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
-                       int *running, int parity, bool mixed, int variant = kStepCached);
+                       int *running, int parity, bool mixed, int variant = kStepCached, int spw = 1);
 hipError_t launch_gather_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n);
 // setup_rows for `count` QPs of a device QpDev table (count <= 65535)
 hipError_t launch_setup_rows_batch(hipStream_t s, const QpDev *qps, int count, int max_nodes);
@@ -124,11 +125,13 @@ struct BatchTables {
   QpState **d_stptr = nullptr, **h_stptr = nullptr;
   int states_cap = 0;
   int ensure_states(int n_qps);
-  std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> graphs;
-  std::map<std::tuple<int, int, int, int, int>, hipGraph_t> graph_defs;
+  // key: wavefronts per workgroup, launches, grid, mixed, kernel variant,
+  // slices per wavefront
+  std::map<std::tuple<int, int, int, int, int, int>, hipGraphExec_t> graphs;
+  std::map<std::tuple<int, int, int, int, int, int>, hipGraph_t> graph_defs;
   int init(mprgp_ctx *c);
   int ensure(int n_qps, int n_slots);
-  int graph(hipStream_t s, int ts, int steps, int grid, bool mixed, int variant, hipGraphExec_t *out);
+  int graph(hipStream_t s, int ts, int steps, int grid, bool mixed, int variant, hipGraphExec_t *out, int spw = 1);
   void drop_graphs();
   void destroy();
 };
@@ -136,6 +139,8 @@ struct BatchTables {
 // Pieces of the batch solve shared by solve_batch and the engine.
 int check_batch_qp(const mprgp_ctx *ctx, const mprgp_qp *q, const mprgp_params &p, int i);
 int pick_ts(const mprgp_params &p, long total_slices);
+// slices each wavefront streams per launch (params.slices_per_wave: 1 or 2)
+inline int slices_per_wave_of(const mprgp_params &p) { return p.slices_per_wave == 2 ? 2 : 1; }
 void append_tiles(int slot, const mprgp_qp *q, int span, std::vector<Tile> &tiles);
 int tiles_for(int n_slices, int span);
 int round_grid(int n_tiles);

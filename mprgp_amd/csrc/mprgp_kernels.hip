@@ -1103,21 +1103,33 @@ __device__ __forceinline__ bool fp64_op(int op) {
          op == OP_R_START || op == OP_R_UPD || op == OP_R_FINAL;
 }
 
-// One slice per wavefront: a tile is up to TS consecutive slices, streamed
-// by the workgroup's waves at the same time.
+// One slice of a tile per wavefront.
 template <int TS, bool MX, int UB, bool NT>
-__device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &st, double (&P)[kNumPart]) {
-  const int wave = threadIdx.x / kSlice;
-  const int lane = threadIdx.x % kSlice;
-  const int s = t.slice0 + wave;
+__device__ __forceinline__ void slice_work(const QpDev &q, int s, int lane, int op, const QpState &st,
+                                           double (&P)[kNumPart]) {
   const int r = s * kSlice + lane;
-  if (wave >= t.slices || r >= q.n_nodes) return;
+  if (r >= q.n_nodes) return;
   if (fp64_op(op)) {
     row_work_fp64<MX, UB, NT>(q, s, lane, r, op, st, P);
   } else if (MX && st.mixed) {
     row_work<float, MX, UB, NT>(q, q.f, s, lane, r, op, st, P);
   } else {
     row_work<double, MX, UB, NT>(q, q.d, s, lane, r, op, st, P);
+  }
+}
+
+// A tile is up to TS * SPW consecutive slices: wave w streams the tile's
+// slices w, w + TS, ..., w + (SPW - 1) TS one after the other (SPW slices
+// per wavefront, the call's slices_per_wave), adding each row's sums to its
+// lane's partials in that order.
+template <int TS, bool MX, int UB, bool NT, int SPW>
+__device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &st, double (&P)[kNumPart]) {
+  const int wave = threadIdx.x / kSlice;
+  const int lane = threadIdx.x % kSlice;
+  if (SPW == 1) {
+    if (wave < t.slices) slice_work<TS, MX, UB, NT>(q, t.slice0 + wave, lane, op, st, P);
+  } else {
+    for (int w = wave; w < t.slices; w += TS) slice_work<TS, MX, UB, NT>(q, t.slice0 + w, lane, op, st, P);
   }
 }
 
@@ -1164,12 +1176,15 @@ typedef uint64_t __attribute__((__may_alias__)) StateWord;
 // of a result.  kUnrollTail, with more registers and fewer waves, is for
 // launches with few tiles, where a lone row's chain of load rounds sets the
 // launch time.
-constexpr int kUnroll = 3;
+#ifndef MPRGP_STEP_UNROLL
+#define MPRGP_STEP_UNROLL 3
+#endif
+constexpr int kUnroll = MPRGP_STEP_UNROLL;
 constexpr int kUnrollTail = 6;
 
 // MX: the batch holds QPs of the mixed form (their inner ops run on the FP32
 // copies); FP64 QPs of the same batch take the FP64 path.
-template <int TS, bool MX, int UB, bool NT>
+template <int TS, bool MX, int UB, bool NT, int SPW>
 __global__ void __launch_bounds__(TS * kSlice, UB == kUnroll ? MPRGP_STEP_MIN_WAVES : 2)
     mprgp_step(const QpDev *__restrict__ qps,
                                                            const Tile *__restrict__ tiles,
@@ -1228,7 +1243,7 @@ __global__ void __launch_bounds__(TS * kSlice, UB == kUnroll ? MPRGP_STEP_MIN_WA
 #pragma unroll
     for (int j = 0; j < kNumPart; ++j) P[j] = 0.0;
     if (has_min) P[kMinSlot] = kDblMax;
-    tile_work<TS, MX, UB, NT>(q, t, op, st, P);
+    tile_work<TS, MX, UB, NT, SPW>(q, t, op, st, P);
     // wave then block reduction in a fixed order, over the op's slots only
     const int ns = has_min ? kMinSlot + 1 : sum_slots(op);
     if (ns > 0) {
@@ -1567,33 +1582,41 @@ __global__ void __launch_bounds__(256) spmv_plain(const int *__restrict__ sl_off
 
 // Host-callable launch wrappers ------------------------------------------
 
-template <bool MX, int UB, bool NT>
+template <bool MX, int UB, bool NT, int SPW>
 static hipError_t launch_step_t(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
                                 int *running, int parity) {
   switch (ts) {
-    case 1: mprgp_step<1, MX, UB, NT><<<grid, 1 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
-    case 2: mprgp_step<2, MX, UB, NT><<<grid, 2 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
-    case 4: mprgp_step<4, MX, UB, NT><<<grid, 4 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
-    case 8: mprgp_step<8, MX, UB, NT><<<grid, 8 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
+    case 1: mprgp_step<1, MX, UB, NT, SPW><<<grid, 1 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
+    case 2: mprgp_step<2, MX, UB, NT, SPW><<<grid, 2 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
+    case 4: mprgp_step<4, MX, UB, NT, SPW><<<grid, 4 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
+    case 8: mprgp_step<8, MX, UB, NT, SPW><<<grid, 8 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
-                       int *running, int parity, bool mixed, int variant) {
+template <int SPW>
+static hipError_t launch_step_v(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
+                                int *running, int parity, bool mixed, int variant) {
   switch (variant) {
     case kStepCached:
-      return mixed ? launch_step_t<true, kUnroll, false>(ts, grid, s, qps, tiles, running, parity)
-                   : launch_step_t<false, kUnroll, false>(ts, grid, s, qps, tiles, running, parity);
+      return mixed ? launch_step_t<true, kUnroll, false, SPW>(ts, grid, s, qps, tiles, running, parity)
+                   : launch_step_t<false, kUnroll, false, SPW>(ts, grid, s, qps, tiles, running, parity);
     case kStepStream:
-      return mixed ? launch_step_t<true, kUnroll, true>(ts, grid, s, qps, tiles, running, parity)
-                   : launch_step_t<false, kUnroll, true>(ts, grid, s, qps, tiles, running, parity);
+      return mixed ? launch_step_t<true, kUnroll, true, SPW>(ts, grid, s, qps, tiles, running, parity)
+                   : launch_step_t<false, kUnroll, true, SPW>(ts, grid, s, qps, tiles, running, parity);
     case kStepTail:
-      return mixed ? launch_step_t<true, kUnrollTail, false>(ts, grid, s, qps, tiles, running, parity)
-                   : launch_step_t<false, kUnrollTail, false>(ts, grid, s, qps, tiles, running, parity);
+      return mixed ? launch_step_t<true, kUnrollTail, false, SPW>(ts, grid, s, qps, tiles, running, parity)
+                   : launch_step_t<false, kUnrollTail, false, SPW>(ts, grid, s, qps, tiles, running, parity);
     default: return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
+                       int *running, int parity, bool mixed, int variant, int spw) {
+  if (spw == 2) return launch_step_v<2>(ts, grid, s, qps, tiles, running, parity, mixed, variant);
+  if (spw != 1) return hipErrorInvalidValue;
+  return launch_step_v<1>(ts, grid, s, qps, tiles, running, parity, mixed, variant);
 }
 
 hipError_t launch_setup_rows_batch(hipStream_t s, const QpDev *qps, int count, int max_nodes) {

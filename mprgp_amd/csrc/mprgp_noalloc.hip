@@ -115,7 +115,9 @@ extern "C" int mprgp_qp_solve_noalloc(mprgp_qp *q, const mprgp_params *params, m
   if (dev < 0 || dev >= kMaxDevices || !g_scratch[dev])
     return fail(MPRGP_ESTATE, "no scratch array address for device %d", dev);
   const int ts = pick_ts(p, q->A->n_slices);
-  const int n_tiles = tiles_for(q->A->n_slices, ts);
+  const int spw = slices_per_wave_of(p);
+  const int span = ts * spw;  // as mprgp_qp_solve
+  const int n_tiles = tiles_for(q->A->n_slices, span);
   const int grid = round_grid(n_tiles);
   if (grid > kNoAllocMaxTiles)
     return fail(MPRGP_EBADINPUT, "the QP needs %d workgroup slots of %d wavefronts, the no-alloc tables hold %d",
@@ -148,8 +150,8 @@ extern "C" int mprgp_qp_solve_noalloc(mprgp_qp *q, const mprgp_params *params, m
   const int run = std::min(4, grid / 8);  // layout_tiles' runs of tiles per XCD
   StateWords sw;
   std::memcpy(&sw, &st, sizeof(sw));
-  noalloc_tables<<<(grid + 255) / 256, 256, 0, s>>>(base, d, sw, q->st, q->part, q->A->n_slices, ts, n_tiles, grid,
-                                                    run);
+  noalloc_tables<<<(grid + 255) / 256, 256, 0, s>>>(base, d, sw, q->st, q->part, q->A->n_slices, span, n_tiles,
+                                                    grid, run);
   MPRGP_HIP(hipGetLastError());
   int steps = p.graph_steps > 0 ? p.graph_steps : 16;
   if (p.graph_steps < 0) steps = -p.graph_steps;
@@ -163,7 +165,7 @@ extern "C" int mprgp_qp_solve_noalloc(mprgp_qp *q, const mprgp_params *params, m
   long launched = 0;
   for (;;) {
     for (int i = 0; i < steps; ++i)
-      MPRGP_HIP(launch_step(ts, grid, s, qps, tiles, running, i & 1, mixed, variant));
+      MPRGP_HIP(launch_step(ts, grid, s, qps, tiles, running, i & 1, mixed, variant, spw));
     launched += steps;
     MPRGP_HIP(hipMemcpyAsync(ctx->h_running, running, sizeof(int), hipMemcpyDeviceToHost, s));
     MPRGP_HIP(hipStreamSynchronize(s));

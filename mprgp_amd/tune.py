@@ -1,8 +1,9 @@
 """Autotuner of the batched solve's launch shape.
 
-The step kernel's shape is the number of wavefronts per workgroup (one
-SELL slice of 64 block rows per wavefront, so a workgroup's tile is that many
-slices), the number of launches captured per hipGraph replay, the tile
+The step kernel's shape is the number of wavefronts per workgroup and the
+SELL slices (64 block rows) each wavefront streams per launch (rows per
+wavefront; a workgroup's tile is their product in slices), the number of
+launches captured per hipGraph replay, the tile
 count at or below which a launch runs the kernel's tail form (the batch's
 last, latency-bound launches, once only the stiffest QPs are left), and the
 placement of the tiles on the 8 XCDs.  Every candidate changes only the
@@ -34,6 +35,10 @@ DEFAULT_CANDIDATES = [
     {"tiles": 1, "graph_steps": 16},
 ]
 
+# Slices (64 rows) each wavefront streams per launch, tried at the best shape
+# (1: one slice per wavefront; a tile then holds tiles x this many slices).
+SPW_CANDIDATES = [1, 2]
+
 # Tail switch points tried at the best shape (0: the library's default, 256
 # tiles, one workgroup per CU).
 TAIL_CANDIDATES = [0, 64, 128, 512, 1024, 2048]
@@ -43,7 +48,7 @@ TAIL_CANDIDATES = [0, 64, 128, 512, 1024, 2048]
 LAYOUT_CANDIDATES = [0, 1]
 
 # Bumped whenever the step kernel changes in a way that moves the best shape.
-CACHE_VERSION = 6
+CACHE_VERSION = 7
 
 
 def cache_path():
@@ -64,22 +69,29 @@ def apply(p: M.Params, shape):
     p.graph_steps = int(shape["graph_steps"])
     p.tail_tiles = int(shape.get("tail_tiles", 0))
     p.xcd_layout = int(shape.get("xcd_layout", 0))
+    p.slices_per_wave = int(shape.get("slices_per_wave", 1))
     return p
 
 
-def tune(ctx, qps, per_qp, candidates=None, tails=None, layouts=None, repeats=2, log=None):
+def tune(ctx, qps, per_qp, candidates=None, spws=None, tails=None, layouts=None, repeats=2, log=None):
     """Times every candidate shape on the batch (each QP restarted from its
     warm start with reset_x), keeps each candidate's best of `repeats` timed
     runs (single runs differ by about 1%, as much as the candidates do), then
-    the tail switch points `tails` at the best shape, then the XCD layouts
-    `layouts` at the best of those; returns (best_shape, [(shape, ms)])."""
+    the slices per wavefront `spws` at the best shape, the tail switch points
+    `tails` at the best so far, then the XCD layouts `layouts`; returns
+    (best_shape, [(shape, ms)])."""
     candidates = candidates or DEFAULT_CANDIDATES
+    spws = SPW_CANDIDATES if spws is None else spws
     tails = TAIL_CANDIDATES if tails is None else tails
     layouts = LAYOUT_CANDIDATES if layouts is None else layouts
     results = []
     timed = _timer(ctx, qps, per_qp, repeats, log, results)
     for shape in candidates:
         timed(dict(shape))
+    best = min(results, key=lambda r: r[1])[0]
+    for k in spws:
+        if k != best.get("slices_per_wave", 1):
+            timed(dict(best, slices_per_wave=k))
     best = min(results, key=lambda r: r[1])[0]
     for t in tails:
         if t != best.get("tail_tiles", 0):

@@ -51,25 +51,26 @@ def test_tuner_caches_and_candidates_agree(ctx, batch, tmp_path):
     path = str(tmp_path / "tune.json")
     cands = [{"tiles": 2, "graph_steps": 16}, {"tiles": 4, "graph_steps": 8}, {"tiles": 1, "graph_steps": 32}]
     base = per_qp(ab, M.FP64)
-    saved = (tune.DEFAULT_CANDIDATES, tune.TAIL_CANDIDATES, tune.LAYOUT_CANDIDATES)
-    tune.DEFAULT_CANDIDATES, tune.TAIL_CANDIDATES, tune.LAYOUT_CANDIDATES = cands, [0, 1024], [0, 1]
+    saved = (tune.DEFAULT_CANDIDATES, tune.SPW_CANDIDATES, tune.TAIL_CANDIDATES, tune.LAYOUT_CANDIDATES)
+    tune.DEFAULT_CANDIDATES, tune.SPW_CANDIDATES = cands, [1, 2]
+    tune.TAIL_CANDIDATES, tune.LAYOUT_CANDIDATES = [0, 1024], [0, 1]
     try:
         ps, shape, hit = tune.tuned_params(ctx, [q.qp for q in qps], base, M.FP64, path=path)
         assert not hit
         ps2, shape2, hit2 = tune.tuned_params(ctx, [q.qp for q in qps], base, M.FP64, path=path)
     finally:
-        tune.DEFAULT_CANDIDATES, tune.TAIL_CANDIDATES, tune.LAYOUT_CANDIDATES = saved
+        tune.DEFAULT_CANDIDATES, tune.SPW_CANDIDATES, tune.TAIL_CANDIDATES, tune.LAYOUT_CANDIDATES = saved
     assert hit2 and shape2 == shape
     with open(path) as f:
         cache = json.load(f)
     key = tune.batch_key([q.qp for q in qps], M.FP64)
     assert cache[key]["shape"] == shape
     tried = [s for s, _ in cache[key]["ms"]]
-    # 3 shapes, 1 more tail point at the best, 1 more placement at the best of those
-    assert len(tried) == 5 and all(s in tried for s in cands)
+    # 3 shapes, 1 more slices-per-wave at the best, 1 more tail point, 1 more placement
+    assert len(tried) == 6 and all(s in tried for s in cands)
     assert all(p.tiles == shape["tiles"] and p.graph_steps == shape["graph_steps"]
                and p.tail_tiles == shape.get("tail_tiles", 0) and p.xcd_layout == shape.get("xcd_layout", 0)
-               for p in ps)
+               and p.slices_per_wave == shape.get("slices_per_wave", 1) for p in ps)
     # each tried shape: converged, and within the two solves' error bounds of
     # the tuned shape's solution (|x - x*| <= error_bound for each)
     sols = {}
@@ -84,3 +85,47 @@ def test_tuner_caches_and_candidates_agree(ctx, batch, tmp_path):
     for xs, bs in sols.values():
         for x, xr, b, br in zip(xs, ref_x, bs, ref_b):
             assert np.linalg.norm(x - xr) <= b + br
+
+
+@pytest.mark.parametrize("spw", [2])
+def test_slices_per_wave_batch_equals_alone(ctx, batch, spw):
+    """With a fixed slices_per_wave (and tiles), a QP's result is the same bits
+    alone and inside the batch, as for the other launch-shape parameters."""
+    qps, ab = batch
+    kw = dict(tiles=2, graph_steps=16, slices_per_wave=spw)
+    for q in qps:
+        q.qp.reset_x()
+    rc, st = M.solve_batch(ctx, [q.qp for q in qps], per_qp=per_qp(ab, M.FP64, **kw))
+    assert all(s["code"] == 0 for s in st)
+    xs = [q.qp.get_x() for q in qps]
+    for i in (0, 7, 15):
+        qps[i].qp.reset_x()
+        rc, s1 = M.solve_batch(ctx, [qps[i].qp], per_qp=per_qp(ab[i:i + 1], M.FP64, **kw))
+        assert rc == 0 and s1[0]["iterations"] == st[i]["iterations"]
+        np.testing.assert_array_equal(qps[i].qp.get_x(), xs[i])
+
+
+@pytest.mark.parametrize("spw", [1, 2])
+def test_slices_per_wave_parity_with_the_oracle(ctx, spw):
+    """FP64 bars against the oracle on config 1 (tol 1e-8): solution 1e-8
+    relative, objective 1e-10."""
+    import oracle
+    from tests.common import scene_qp
+    sc = problems.config1()
+    A, b, node, normal, c, x0 = scene_qp(sc)
+    J = problems.rows_to_J(node, normal, sc.n_nodes)
+    ab = 2.0 / oracle.spec_rad(A, 1e-10)[0]
+    code, xr, sr, _ = oracle.decoupled_solve(A, b, J, c, x0, tol=1e-8, max_it=20000, alpha_bar=ab)
+    q = ctx.matrix_csr(A).qp()
+    q.set_rows(node, normal, c)
+    q.set_rhs(b)
+    q.set_x(x0)
+    rc, st = q.solve(M.params(tol=1e-8, max_it=20000, alpha_bar=ab, tiles=1, slices_per_wave=spw))
+    x = q.get_x()
+    assert code == 0 and rc == 0
+    assert np.linalg.norm(x - xr) <= 1e-8 * np.linalg.norm(xr)
+    f = lambda v: 0.5 * v @ (A @ v) - b @ v
+    assert abs(f(x) - f(xr)) <= 1e-10 * abs(f(xr))
+    with pytest.raises(M.MprgpError) as e:
+        q.solve(M.params(tol=1e-8, max_it=10, alpha_bar=ab, slices_per_wave=3))
+    assert e.value.code == -2 and "slices_per_wave" in M.last_error()

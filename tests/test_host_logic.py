@@ -73,8 +73,11 @@ def test_tune_keys_and_shapes():
     assert tune.batch_key(b, M.MIXED) == tune.batch_key(a, M.MIXED)  # same size class
     p = tune.apply(M.params(), {"tiles": 8, "graph_steps": 32})
     assert (p.tiles, p.graph_steps, p.tail_tiles) == (8, 32, 0)
-    p = tune.apply(M.params(), {"tiles": 2, "graph_steps": 16, "tail_tiles": 1024, "xcd_layout": 1})
-    assert (p.tiles, p.graph_steps, p.tail_tiles, p.xcd_layout) == (2, 16, 1024, 1)
+    p = tune.apply(M.params(), {"tiles": 2, "graph_steps": 16, "tail_tiles": 1024, "xcd_layout": 1,
+                                "slices_per_wave": 2})
+    assert (p.tiles, p.graph_steps, p.tail_tiles, p.xcd_layout, p.slices_per_wave) == (2, 16, 1024, 1, 2)
+    assert tune.apply(M.params(), {"tiles": 2, "graph_steps": 16}).slices_per_wave == 1
+    assert set(tune.SPW_CANDIDATES) <= {1, 2}
     assert 0 in tune.TAIL_CANDIDATES and all(t >= 0 for t in tune.TAIL_CANDIDATES)
     for s in tune.DEFAULT_CANDIDATES:
         assert s["tiles"] in (1, 2, 4, 8) and s["graph_steps"] > 0

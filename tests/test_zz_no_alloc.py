@@ -68,6 +68,8 @@ def test_noalloc_equals_solve_on_two_calls_in_a_row(ctx):
         # mixed: its FP32 storage exists after the first solve_both's mprgp_qp_solve
         st = solve_both(q, tol=1e-6, max_it=5000, project_first=1, alpha_bar=ab, precision=M.MIXED)
         assert st["code"] == 0 and st["refinements"] >= 1
+        # two slices per wavefront: tiles of twice the rows, the same on both paths
+        solve_both(q, tol=1e-8, max_it=5000, project_first=1, alpha_bar=ab, tiles=1, slices_per_wave=2)
 
 
 def test_noalloc_limits_are_reported(ctx):
