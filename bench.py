@@ -253,6 +253,39 @@ def dropin_leg(M, problems, workloads, ctx, tol=1e-8, max_it=500, calls=3):
                     "the device, alpha_bar given"}
 
 
+def ica_leg(M, problems, workloads, ctx, tol=8e-5, max_it=5000):
+    """The reference's comparison (script/data/ica_mprgp.txt: both solvers on
+    the same QP at tol 8e-5): ICASolver (ICAFemSolver::forward,
+    MprgpFemSolver.cpp:439-486) against the decoupled MPRGP solve, on the
+    133,623-DoF beam with its plane rows, from the same start, each resident
+    on the device; ICA with the reference's inner defaults (20 projected
+    Gauss-Seidel sweeps to 1e-3).  MPRGP with alpha_bar from its power
+    method, timed once before."""
+    sc = problems.config2()
+    dq = workloads.DeviceQP(ctx, sc)
+    node, normal, c = sc.rows()
+    J = problems.rows_to_J(node, normal, sc.n_nodes)
+    rc, st = dq.qp.solve(M.params(tol=tol, max_it=0, project_first=1))
+    ab = st["alpha_bar"]
+    out = {}
+    for name in ("mprgp", "ica"):
+        ms, its, res = [], None, None
+        for _ in range(3):
+            dq.qp.reset_x()
+            t0 = time.perf_counter()
+            if name == "mprgp":
+                rc, s = dq.qp.solve(M.params(tol=tol, max_it=max_it, project_first=1, alpha_bar=ab))
+            else:
+                rc, s, lam, sweeps = M.ica_solve(dq.qp, J, c, tol=tol, max_it=max_it)
+            ms.append(1e3 * (time.perf_counter() - t0))
+            its, res = s["iterations"], s["residual"]
+        out[name] = {"ms": [round(t, 2) for t in ms], "iterations": its, "residual": res, "code": rc}
+    out["ica_over_mprgp_time"] = round(min(out["ica"]["ms"]) / min(out["mprgp"]["ms"]), 2)
+    out["workload"] = (f"beam 100x20x20, {3 * sc.n_nodes} DoF, {len(node)} plane rows, FP64, tol {tol:g}, "
+                       "both from the same start (x_n)")
+    return out
+
+
 def spmv_leg(M, workloads, ctx, qps, tiles, graph_steps):
     """Bandwidth of the batched SpMV (FixedSparseMatrix::multiply,
     MPRGPSolver.h:469-473) on the whole batch: every QP runs the power method
@@ -308,6 +341,7 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM counter passes")
     ap.add_argument("--no-spmv", action="store_true", help="skip the batched SpMV bandwidth leg")
     ap.add_argument("--no-dropin", action="store_true", help="skip the one-shot drop-in leg")
+    ap.add_argument("--no-ica", action="store_true", help="skip the ICA-versus-MPRGP comparison leg")
     ap.add_argument("--pmc-child", choices=["copy", "solve", "spmv"], help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -564,6 +598,10 @@ def main():
     if not args.no_dropin:
         dropin = dropin_leg(M, problems, workloads, ctx)
         log(f"drop-in: {dropin}")
+    ica = None
+    if not args.no_ica:
+        ica = ica_leg(M, problems, workloads, ctx)
+        log(f"ica vs mprgp: {ica}")
 
     out = {
         "metric": "MPRGP solves/s at KKT res \u22641e-6, batched FP32/FP64 vs plain FP64; p50/p99 ms",
@@ -595,6 +633,7 @@ def main():
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
         "dropin_fp64_133k": dropin,
+        "ica_vs_mprgp_133k": ica,
     }
     print(json.dumps(out), flush=True)
 

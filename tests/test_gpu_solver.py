@@ -203,8 +203,13 @@ def test_file_overload_equals_the_drop_in(ctx):
         shutil.rmtree(d, ignore_errors=True)
 
 
-def test_classic_variant_parity(ctx):
-    sc = problems.config1_flat()
+@pytest.mark.parametrize("which", ["config1_flat", "config1", "config3"])
+def test_classic_variant_parity(ctx, which):
+    """Classic MPRGP (MPRGP::solve with the max-feasible-step search): the
+    FP64 bars against the oracle on the beams, including config 3 (per-tet E
+    over four decades, kappa ~1e6); the same step counts on the flat beam,
+    where no decision falls within rounding of its threshold."""
+    sc = getattr(problems, which)()
     A, b, node, normal, c, x0 = scene_qp(sc)
     J = problems.rows_to_J(node, normal, sc.n_nodes)
     ab = 2.0 / oracle.spec_rad(A, 1e-10)[0]
@@ -212,8 +217,12 @@ def test_classic_variant_parity(ctx):
                                              variant=oracle.CLASSIC)
     q = make_qp(ctx, A, b, node, normal, c, x0)
     rc, sg = q.solve(tol=1e-8, max_it=5000, alpha_bar=ab, variant=M.CLASSIC)
+    print(which, "oracle", (sr["num_cg"], sr["num_exp"], sr["num_prop"]), "gpu",
+          (sg["num_cg"], sg["num_exp"], sg["num_prop"]), flush=True)
+    assert code == 0 and rc == 0
     assert_parity(A, b, node, normal, c, q.get_x(), xr, sg, sr)
-    assert (sg["num_cg"], sg["num_exp"], sg["num_prop"]) == (sr["num_cg"], sr["num_exp"], sr["num_prop"])
+    if which == "config1_flat":
+        assert (sg["num_cg"], sg["num_exp"], sg["num_prop"]) == (sr["num_cg"], sr["num_exp"], sr["num_prop"])
 
 
 @pytest.mark.parametrize("seed", range(4))
