@@ -123,37 +123,60 @@ __global__ void __launch_bounds__(256) ica_rows(const int *sl_off, const int *co
 }
 
 // The forward block Gauss-Seidel sweep (ICASolver.h:394-404) in one
-// workgroup: level by level, each node subtracts its blocks on nodes before
-// it (already updated, earlier levels) from its rhs and multiplies by Da^-1.
-// The barrier orders a level's x writes before the next level's reads; all
-// waves of the workgroup share the CU's caches.
-__global__ void __launch_bounds__(1024) ica_sweep(const int *sl_off, const int *col, const double *val,
-                                                  const int *perm, const double *dinv, const double *rhs,
-                                                  double *x, const int *lvl_ptr, const int *lvl_node,
-                                                  int n_levels) {
+// workgroup, level by level.  A level's lower blocks (each node's blocks on
+// nodes before it, already updated in earlier levels) are listed in sweep
+// order: lo_ptr[t] .. lo_ptr[t + 1] for the level-ordered node t, with each
+// block's SELL slot and column.  Each level runs in two phases:
+//   A. every lower block of the level, one per thread: its nine products
+//      v_ab * x_b, each rounded once, into prod;
+//   B. every node of the level, one per thread: rhs minus its blocks'
+//      products in block order, times Da^-1.
+// Each node's rows thus see the same rounded products subtracted in the same
+// order as the sequential sweep (r_a -= v_ab x_b over blocks, b = 0, 1, 2),
+// while the loads of a level are all issued at once instead of one block at
+// a time.  The barriers order phase A's products and phase B's x writes
+// before their readers; all waves of the workgroup share the CU's caches.
+__global__ void __launch_bounds__(1024) ica_sweep(const double *val, const int *lo_slot, const int *lo_col,
+                                                  const int *lo_ptr, const double *dinv, const double *rhs,
+                                                  double *x, double *prod, const int *lvl_ptr,
+                                                  const int *lvl_node, int n_levels) {
   for (int lv = 0; lv < n_levels; ++lv) {
-    const int end = lvl_ptr[lv + 1];
-    for (int t = lvl_ptr[lv] + (int)threadIdx.x; t < end; t += (int)blockDim.x) {
+    const int t0 = lvl_ptr[lv], t1 = lvl_ptr[lv + 1];
+    const int e1 = lo_ptr[t1];
+    for (int e = lo_ptr[t0] + (int)threadIdx.x; e < e1; e += (int)blockDim.x) {
+      const long long slot = lo_slot[e];
+      // a slot's 9 values: slice base b0 and block k give slot = b0 + 64 k + l
+      // and values at 9 (b0 + 64 k) + l + 64 e
+      const double *v = val + (9 * slot - 8 * (slot % kSlice));
+      const int c = lo_col[e];
+      const double x0 = x[3 * (size_t)c], x1 = x[3 * (size_t)c + 1], x2 = x[3 * (size_t)c + 2];
+      double *pr = prod + 9 * (size_t)e;
+      pr[0] = v[0 * kSlice] * x0;
+      pr[1] = v[1 * kSlice] * x1;
+      pr[2] = v[2 * kSlice] * x2;
+      pr[3] = v[3 * kSlice] * x0;
+      pr[4] = v[4 * kSlice] * x1;
+      pr[5] = v[5 * kSlice] * x2;
+      pr[6] = v[6 * kSlice] * x0;
+      pr[7] = v[7 * kSlice] * x1;
+      pr[8] = v[8 * kSlice] * x2;
+    }
+    __syncthreads();
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += (int)blockDim.x) {
       const int i = lvl_node[t];
-      const int s = i / kSlice, l = i % kSlice;
-      const int b0 = sl_off[s], W = (sl_off[s + 1] - b0) / kSlice;
-      const int ui = perm[i];
       const size_t r = 3 * (size_t)i;
       double r0 = rhs[r], r1 = rhs[r + 1], r2 = rhs[r + 2];
-      for (int k = 0; k < W; ++k) {
-        const int c = col[b0 + k * kSlice + l];
-        if (perm[c] >= ui) continue;
-        const double *v = val + (size_t)b0 * 9 + (size_t)k * 9 * kSlice + l;
-        const double x0 = x[3 * (size_t)c], x1 = x[3 * (size_t)c + 1], x2 = x[3 * (size_t)c + 2];
-        r0 -= v[0 * kSlice] * x0;
-        r0 -= v[1 * kSlice] * x1;
-        r0 -= v[2 * kSlice] * x2;
-        r1 -= v[3 * kSlice] * x0;
-        r1 -= v[4 * kSlice] * x1;
-        r1 -= v[5 * kSlice] * x2;
-        r2 -= v[6 * kSlice] * x0;
-        r2 -= v[7 * kSlice] * x1;
-        r2 -= v[8 * kSlice] * x2;
+      for (int e = lo_ptr[t]; e < lo_ptr[t + 1]; ++e) {
+        const double *pr = prod + 9 * (size_t)e;
+        r0 -= pr[0];
+        r0 -= pr[1];
+        r0 -= pr[2];
+        r1 -= pr[3];
+        r1 -= pr[4];
+        r1 -= pr[5];
+        r2 -= pr[6];
+        r2 -= pr[7];
+        r2 -= pr[8];
       }
       const double *D = dinv + 9 * (size_t)i;
       x[r] = D[0] * r0 + D[1] * r1 + D[2] * r2;
@@ -323,12 +346,29 @@ extern "C" int mprgp_ica_solve(mprgp_qp *q, int m, const int *j_ptr, const int *
     std::vector<int> fill(lvl_ptr.begin(), lvl_ptr.end() - 1);
     for (int u = 0; u < nn; ++u) lvl_node[fill[level[u]]++] = A->iperm[u];
   }
+  // each level-ordered node's lower blocks (nodes before it), in block
+  // order: SELL slot and column
+  std::vector<int> lo_ptr(nn + 1, 0), lo_slot, lo_col;
+  for (int t = 0; t < nn; ++t) {
+    const int i = lvl_node[t], u = A->perm[i], sl = i / kSlice, l = i % kSlice;
+    const int b0 = A->sl_off_h[sl], W = (A->sl_off_h[sl + 1] - b0) / kSlice;
+    for (int k = 0; k < W; ++k) {
+      const int slot = b0 + k * kSlice + l, c = col[(size_t)slot];
+      if (A->perm[c] < u) {
+        lo_slot.push_back(slot);
+        lo_col.push_back(c);
+      }
+    }
+    lo_ptr[t + 1] = (int)lo_slot.size();
+  }
+  const size_t n_lo = lo_slot.size();
 
-  // device block: Da^-1, levels, J and J^T, J u, lambda, dot partials
+  // device block: Da^-1, levels, the lower blocks and their products, J and
+  // J^T, J u, lambda, dot partials
   const size_t parts = ((size_t)n + 255) / 256 + 1;
-  const size_t bytes = sizeof(double) * (9 * (size_t)nn + 2 * (size_t)nnz + 2 * (size_t)m + parts + 8) +
-                       sizeof(int) * ((size_t)n_levels + 1 + (size_t)nn + (size_t)m + 1 + 2 * (size_t)nnz +
-                                      (size_t)n + 1) +
+  const size_t bytes = sizeof(double) * (9 * (size_t)nn + 9 * n_lo + 2 * (size_t)nnz + 2 * (size_t)m + parts + 8) +
+                       sizeof(int) * ((size_t)n_levels + 1 + (size_t)nn + (size_t)nn + 1 + 2 * n_lo + (size_t)m +
+                                      1 + 2 * (size_t)nnz + (size_t)n + 1) +
                        1024;
   void *blk = nullptr;
   if (int rc = ctx->alloc(&blk, bytes)) return rc;
@@ -356,6 +396,10 @@ extern "C" int mprgp_ica_solve(mprgp_qp *q, int m, const int *j_ptr, const int *
   double *d_res = (double *)carve(sizeof(double) * 8);
   int *d_lvl_ptr = (int *)carve(sizeof(int) * (n_levels + 1));
   int *d_lvl_node = (int *)carve(sizeof(int) * (size_t)nn);
+  double *d_prod = (double *)carve(sizeof(double) * 9 * n_lo);
+  int *d_lo_ptr = (int *)carve(sizeof(int) * ((size_t)nn + 1));
+  int *d_lo_slot = (int *)carve(sizeof(int) * n_lo);
+  int *d_lo_col = (int *)carve(sizeof(int) * n_lo);
   int *d_jp = (int *)carve(sizeof(int) * (m + 1));
   int *d_jc = (int *)carve(sizeof(int) * nnz);
   int *d_tp = (int *)carve(sizeof(int) * (n + 1));
@@ -370,6 +414,11 @@ extern "C" int mprgp_ica_solve(mprgp_qp *q, int m, const int *j_ptr, const int *
   MPRGP_HIP(hipMemcpyAsync(d_tp, Ri.tptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
   MPRGP_HIP(hipMemcpyAsync(d_lvl_ptr, lvl_ptr.data(), sizeof(int) * (n_levels + 1), hipMemcpyHostToDevice, s));
   MPRGP_HIP(hipMemcpyAsync(d_lvl_node, lvl_node.data(), sizeof(int) * (size_t)nn, hipMemcpyHostToDevice, s));
+  MPRGP_HIP(hipMemcpyAsync(d_lo_ptr, lo_ptr.data(), sizeof(int) * ((size_t)nn + 1), hipMemcpyHostToDevice, s));
+  if (n_lo > 0) {
+    MPRGP_HIP(hipMemcpyAsync(d_lo_slot, lo_slot.data(), sizeof(int) * n_lo, hipMemcpyHostToDevice, s));
+    MPRGP_HIP(hipMemcpyAsync(d_lo_col, lo_col.data(), sizeof(int) * n_lo, hipMemcpyHostToDevice, s));
+  }
   const int blocks = (nn + 255) / 256;
   if (nn > 0) {
     ica_dinv<<<blocks, 256, 0, s>>>(A->d_sl_off, A->d_col, A->d_val, nn, d_dinv);
@@ -416,7 +465,7 @@ extern "C" int mprgp_ica_solve(mprgp_qp *q, int m, const int *j_ptr, const int *
       ica_rows<<<blocks, 256, 0, s>>>(A->d_sl_off, A->d_col, A->d_val, A->d_perm, x_c, q->b, jtl, d_dinv, rh, nn,
                                       kUpper);
       MPRGP_HIP(hipGetLastError());
-      ica_sweep<<<1, kSweepThreads, 0, s>>>(A->d_sl_off, A->d_col, A->d_val, A->d_perm, d_dinv, rh, q->x,
+      ica_sweep<<<1, kSweepThreads, 0, s>>>(A->d_val, d_lo_slot, d_lo_col, d_lo_ptr, d_dinv, rh, q->x, d_prod,
                                             d_lvl_ptr, d_lvl_node, n_levels);
       MPRGP_HIP(hipGetLastError());
       ica_rows<<<blocks, 256, 0, s>>>(A->d_sl_off, A->d_col, A->d_val, A->d_perm, x_c, q->b, jtl, d_dinv, e, nn,
