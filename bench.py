@@ -382,7 +382,7 @@ def main():
         return [M.params(tol=args.tol, max_it=4000, alpha_bar=q.alpha_bar, project_first=1,
                          precision=precision, tiles=args.ts, graph_steps=args.graph_steps,
                          tail_tiles=args.tail_tiles, xcd_layout=args.xcd_layout,
-                         slices_per_wave=args.slices_per_wave) for q in qps]
+                         slices_per_wave=args.slices_per_wave, from_x0=1) for q in qps]
 
     per = batch_params(prec)
     shape = None
@@ -393,8 +393,8 @@ def main():
         log(f"launch shape {shape} ({'cached' if hit else 'tuned'}, {time.perf_counter() - t1:.1f}s)")
 
     def batch_step(params):
-        for q in qps:
-            q.qp.reset_x()
+        # every QP restarts from its stored warm start inside the solve
+        # (from_x0: the x = x0 copies ride on the batch's set-up launch)
         return M.solve_batch(ctx, [q.qp for q in qps], per_qp=params)
 
     for _ in range(args.warmup):

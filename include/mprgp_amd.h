@@ -90,7 +90,10 @@ typedef struct {
   int slices_per_wave; /* SELL slices (64 rows) each wavefront streams per launch:   */
                        /* 0 or 1, or 2 (rows per wavefront = 64 x this; a tile is    */
                        /* tiles x this slices; like tiles, it groups the sums)       */
-  int reserved_i[2];   /* set to 0                                                   */
+  int from_x0;         /* 1: start from the QP's stored warm start (its last set_x), */
+                       /* as mprgp_qp_reset_x would; a batch restarts all such QPs */
+                       /* in one launch                                            */
+  int reserved_i[1];   /* set to 0                                                   */
   double inner_tol;    /* MIXED: inner tolerance factor (0 = default 1e-3 relative)  */
   double reserved_d[3];
 } mprgp_params;

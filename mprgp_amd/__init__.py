@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+from collections.abc import Sequence
 
 import numpy as np
 
@@ -471,8 +472,39 @@ def ica_solve(qp, J, p, tol=1e-4, max_it=1000, gs_max_it=20, gs_tol=1e-3):
     return rc, st.as_dict(), lam[:m], sweeps.value
 
 
+class BatchStats(Sequence):
+    """The per-QP stats of a batch solve as a sequence of dicts, each made
+    from the returned C array on first access."""
+
+    __slots__ = ("_raw", "_dicts")
+
+    def __init__(self, raw):
+        self._raw = raw
+        self._dicts = [None] * len(raw)
+
+    def __len__(self):
+        return len(self._dicts)
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [self[j] for j in range(*i.indices(len(self)))]
+        n = len(self._dicts)
+        if i < 0:
+            i += n
+        if not 0 <= i < n:
+            raise IndexError("batch stats index out of range")
+        d = self._dicts[i]
+        if d is None:
+            d = self._dicts[i] = self._raw[i].as_dict()
+        return d
+
+    def __repr__(self):
+        return repr(list(self))
+
+
 def solve_batch(ctx: Context, qps, p=None, per_qp=None, **kw):
-    """Advances all QPs together; returns (rc, [stats dicts])."""
+    """Advances all QPs together; returns (rc, stats): a sequence of one
+    stats dict per QP."""
     n = len(qps)
     arr = (C.c_void_p * n)(*[q.h.value for q in qps])
     if per_qp is not None:
@@ -484,7 +516,7 @@ def solve_batch(ctx: Context, qps, p=None, per_qp=None, **kw):
     st = (Stats * n)()
     rc = load().mprgp_solve_batch(ctx.h, n, C.cast(arr, C.POINTER(C.c_void_p)), ps, stride, st)
     check(rc, allow=(_lib.OK, _lib.NOT_CONVERGED, _lib.ENAN))
-    return rc, [s.as_dict() for s in st]
+    return rc, BatchStats(st)
 
 
 class Engine:

@@ -37,7 +37,7 @@ class Params(C.Structure):
                 ("use_ev", C.c_int), ("alpha_bar", C.c_double), ("power_tol", C.c_double),
                 ("tiles", C.c_int), ("graph_steps", C.c_int), ("max_refine", C.c_int),
                 ("tail_tiles", C.c_int), ("xcd_layout", C.c_int), ("slices_per_wave", C.c_int),
-                ("reserved_i", C.c_int * 2),
+                ("from_x0", C.c_int), ("reserved_i", C.c_int * 1),
                 ("inner_tol", C.c_double),
                 ("reserved_d", C.c_double * 3)]
 

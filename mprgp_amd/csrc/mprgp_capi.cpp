@@ -1053,16 +1053,18 @@ void BatchTables::destroy() {
   drop_graphs();
   ctx->release(d_qps, sizeof(QpDev) * (size_t)qps_cap);
   ctx->release(d_tiles, sizeof(Tile) * (size_t)tiles_cap);
+  if (h_tiles) (void)hipHostFree(h_tiles);
   ctx->release(d_ctrl, 4 * sizeof(int));
   ctx->release(d_flags, sizeof(int) * (size_t)flags_cap);
+  if (h_flags) (void)hipHostFree(h_flags);
   ctx->release(d_states, 2 * sizeof(QpState) * (size_t)states_cap);
   ctx->release(d_stptr, sizeof(QpState *) * (size_t)states_cap);
   if (h_states) (void)hipHostFree(h_states);
   if (h_stptr) (void)hipHostFree(h_stptr);
   d_qps = nullptr;
-  d_tiles = nullptr;
+  d_tiles = h_tiles = nullptr;
   d_ctrl = nullptr;
-  d_flags = nullptr;
+  d_flags = h_flags = nullptr;
   d_states = h_states = nullptr;
   d_stptr = h_stptr = nullptr;
   qps_cap = tiles_cap = flags_cap = states_cap = 0;
@@ -1104,9 +1106,17 @@ int BatchTables::ensure(int n_qps, int n_slots) {
   if (n_qps > flags_cap) {
     int cap = std::max(n_qps, 2 * flags_cap);
     ctx->release(d_flags, sizeof(int) * (size_t)flags_cap);
-    d_flags = nullptr;
+    if (h_flags) (void)hipHostFree(h_flags);
+    d_flags = h_flags = nullptr;
     flags_cap = 0;
     if (int rc = ctx->alloc((void **)&d_flags, sizeof(int) * (size_t)cap)) return rc;
+    hipError_t e = hipHostMalloc((void **)&h_flags, 2 * sizeof(int) * (size_t)cap, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      h_flags = nullptr;
+      ctx->release(d_flags, sizeof(int) * (size_t)cap);
+      d_flags = nullptr;
+      return fail(MPRGP_ENOMEM, "pinned flags of %d QPs: %s", cap, hipGetErrorString(e));
+    }
     flags_cap = cap;
   }
   if (n_qps > qps_cap) {
@@ -1121,9 +1131,17 @@ int BatchTables::ensure(int n_qps, int n_slots) {
   if (n_slots > tiles_cap) {
     int cap = std::max(n_slots, 2 * tiles_cap);
     ctx->release(d_tiles, sizeof(Tile) * (size_t)tiles_cap);
-    d_tiles = nullptr;
+    if (h_tiles) (void)hipHostFree(h_tiles);
+    d_tiles = h_tiles = nullptr;
     tiles_cap = 0;
     if (int rc = ctx->alloc((void **)&d_tiles, sizeof(Tile) * (size_t)cap)) return rc;
+    hipError_t e = hipHostMalloc((void **)&h_tiles, sizeof(Tile) * (size_t)cap, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      h_tiles = nullptr;
+      ctx->release(d_tiles, sizeof(Tile) * (size_t)cap);
+      d_tiles = nullptr;
+      return fail(MPRGP_ENOMEM, "pinned staging of %d tiles: %s", cap, hipGetErrorString(e));
+    }
     tiles_cap = cap;
     drop_graphs();
   }
@@ -1138,6 +1156,9 @@ int BatchTables::graph(hipStream_t s, int ts, int steps, int grid, bool mixed, i
     *out = it->second;
     return MPRGP_OK;
   }
+  // a new shape is captured with the stream idle (callers may have replays
+  // in flight)
+  MPRGP_HIP(hipStreamSynchronize(s));
   MPRGP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   for (int i = 0; i < steps; ++i) {
     hipError_t e = launch_step(ts, grid, s, d_qps, d_tiles, d_ctrl + 1, i & 1, mixed, variant, spw);
@@ -1262,6 +1283,7 @@ int check_batch_qp(const mprgp_ctx *ctx, const mprgp_qp *q, const mprgp_params &
     return fail(MPRGP_EBADINPUT, "qp %d: the mixed form runs the monotone variant", i);
   if (!(p.slices_per_wave == 0 || p.slices_per_wave == 1 || p.slices_per_wave == 2))
     return fail(MPRGP_EBADINPUT, "qp %d: slices_per_wave must be 1 or 2 (0: 1)", i);
+  if (p.from_x0 != 0 && p.from_x0 != 1) return fail(MPRGP_EBADINPUT, "qp %d: from_x0 must be 0 or 1", i);
   return MPRGP_OK;
 }
 
@@ -1355,6 +1377,10 @@ int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, i
     if (int rc = ensure_qp32(q, s)) return rc;
   }
   describe_qp(q, p, n_tiles, done_flag, tag, d, st);
+  if (p.from_x0) {  // the warm start the last set_x stored, as mprgp_qp_reset_x
+    if (rows) MPRGP_HIP(hipMemcpyAsync(q->x, q->x0, sizeof(double) * 3 * q->n, hipMemcpyDeviceToDevice, s));
+    else d.x0 = q->x0;
+  }
   if (!(p.alpha_bar > 0.0)) {
     if (p.use_ev && q->ev_valid) {
       MPRGP_HIP(hipMemcpyAsync(q->pv, q->ev, sizeof(double) * 3 * q->n, hipMemcpyDeviceToDevice, s));
@@ -1444,9 +1470,8 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
   int grid = round_grid(n_tiles);
   if (int rc = tab.ensure(count, grid)) return rc;
   if (int rc = tab.ensure_states(count)) return rc;
-  std::vector<std::vector<Tile>> uploads(1, std::vector<Tile>(grid));  // host buffers of table uploads
   const int layout = params[0].xcd_layout;
-  layout_tiles(tiles, grid, uploads.back().data(), layout);
+  layout_tiles(tiles, grid, tab.h_tiles, layout);
   hipStream_t s = ctx->stream;
   std::vector<QpDev> devs(count);
   QpState *states = tab.h_states;  // pinned; [i][2]
@@ -1474,7 +1499,7 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
   // the rows' tangent-Jacobi factors of every QP (they follow the current
   // matrix values) in one launch
   if (count > 1 && count <= 65535) MPRGP_HIP(launch_setup_rows_batch(s, tab.d_qps, count, max_nodes));
-  MPRGP_HIP(hipMemcpyAsync(tab.d_tiles, uploads.back().data(), sizeof(Tile) * grid, hipMemcpyHostToDevice, s));
+  MPRGP_HIP(hipMemcpyAsync(tab.d_tiles, tab.h_tiles, sizeof(Tile) * grid, hipMemcpyHostToDevice, s));
   MPRGP_HIP(hipMemcpyAsync(tab.d_ctrl + 1, &running0, sizeof(int), hipMemcpyHostToDevice, s));
   MPRGP_HIP(hipMemsetAsync(tab.d_flags, 0, sizeof(int) * count, s));
   MPRGP_HIP(hipStreamSynchronize(s));
@@ -1500,7 +1525,6 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
   if (use_graph)
     if (int rc = tab.graph(s, ts, steps, grid, any_mixed, variant, &ex, spw)) return rc;
   int in_table = count;  // QPs whose tiles are in the table
-  std::vector<int> flags(count);
 
   MPRGP_HIP(hipEventRecord(ctx->t0, s));
   long launched = 0;
@@ -1515,21 +1539,30 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
     }
     launched += steps;
     MPRGP_HIP(hipMemcpyAsync(ctx->h_running + slot, tab.d_ctrl + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (count > 1)
+      MPRGP_HIP(hipMemcpyAsync(tab.h_flags + (size_t)slot * tab.flags_cap, tab.d_flags, sizeof(int) * count,
+                               hipMemcpyDeviceToHost, s));
     MPRGP_HIP(hipEventRecord(ctx->q[slot], s));
     return MPRGP_OK;
   };
   if (int rc = enqueue(0)) return rc;
   std::vector<char> keep(count, 1);
+  long shrunk_at = -2;  // the poll of the last table shrink
   while (!done) {
+    // two replays in flight: the next one is queued before this poll's wait
     if (int rc = enqueue((k + 1) & 1)) return rc;
     MPRGP_HIP(hipEventSynchronize(ctx->q[k & 1]));
     const int running = ctx->h_running[k & 1];
+    const int *flags = tab.h_flags + (size_t)(k & 1) * tab.flags_cap;
     if (running == 0) done = true;
-    ++k;
-    // drop the tiles of finished QPs once a quarter of the table is idle
-    if (!done && count > 1 && 4 * running < 3 * in_table) {
-      MPRGP_HIP(hipMemcpyAsync(flags.data(), tab.d_flags, sizeof(int) * count, hipMemcpyDeviceToHost, s));
-      MPRGP_HIP(hipStreamSynchronize(s));
+    // Drop the tiles of finished QPs once a quarter of the table is idle,
+    // without draining the stream: the flags of this poll name the QPs done
+    // by this replay (those the replay in flight finishes stay one replay
+    // longer and idle), and the new table is uploaded behind the replay in
+    // flight, which still reads the old one.  The staging buffer is written
+    // again only from the second poll on, whose replay follows the last
+    // upload.
+    if (!done && count > 1 && 4 * running < 3 * in_table && k >= shrunk_at + 2) {
       int live = 0;
       for (int i = 0; i < count; ++i) {
         keep[i] = flags[i] == 0;
@@ -1540,19 +1573,16 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
         for (int i = 0; i < count; ++i)
           if (keep[i]) append_tiles(i, qps[i], span, kept);
         grid = round_grid((int)kept.size());  // <= the table's capacity: the tile count only shrinks
-        uploads.emplace_back(grid);
-        layout_tiles(kept, grid, uploads.back().data(), layout);
-        MPRGP_HIP(hipMemcpyAsync(tab.d_tiles, uploads.back().data(), sizeof(Tile) * grid, hipMemcpyHostToDevice, s));
-        MPRGP_HIP(hipStreamSynchronize(s));
+        layout_tiles(kept, grid, tab.h_tiles, layout);
+        MPRGP_HIP(hipMemcpyAsync(tab.d_tiles, tab.h_tiles, sizeof(Tile) * grid, hipMemcpyHostToDevice, s));
         in_table = live;
+        shrunk_at = k;
         variant = step_variant((int)kept.size(), live_bytes(&keep), tail_tiles);
         if (use_graph)
           if (int rc = tab.graph(s, ts, steps, grid, any_mixed, variant, &ex, spw)) return rc;
       }
-      // restart the two-deep polling pipeline
-      if (int rc = enqueue(k & 1)) return rc;
-      continue;
     }
+    ++k;
     if (!done && launched > max_steps + 2L * steps) {
       (void)hipStreamSynchronize(s);
       return fail(MPRGP_EHIP, "solve did not finish within %ld device steps", launched);

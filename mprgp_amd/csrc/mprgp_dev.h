@@ -155,6 +155,7 @@ struct QpDev {
   unsigned char *face;
   int *done_flag;       // receives `tag` once the QP reaches DONE
   int tag;              // nonzero; distinguishes successive occupants of a batch slot
+  const double *x0;     // non-null: restart_x copies it into d.x before the solve
 };
 
 // One application of the projector or preconditioner (projector_op).

@@ -115,9 +115,11 @@ struct BatchTables {
   QpDev *d_qps = nullptr;
   int qps_cap = 0;
   Tile *d_tiles = nullptr;  // one slot per workgroup of a launch (layout_tiles)
+  Tile *h_tiles = nullptr;  // pinned staging of a table upload, tiles_cap slots
   int tiles_cap = 0;
   int *d_ctrl = nullptr;   // [1] = running QPs
   int *d_flags = nullptr;  // per batch slot: tag of the QP that reached DONE
+  int *h_flags = nullptr;  // pinned, [2][flags_cap]: d_flags read back with each poll
   int flags_cap = 0;
   // solve_batch's QP states in one staging copy each way: [qp][2] states,
   // device and pinned host, and each QP's own state address
@@ -171,8 +173,9 @@ size_t solve_bytes(const mprgp_qp *q, const mprgp_params &p);
 // FP32 storage must exist)
 void describe_qp(const mprgp_qp *q, const mprgp_params &p, int n_tiles, int *done_flag, int tag, QpDev &d,
                  QpState &st);
-// rows = false: the caller runs the rows' tangent-Jacobi set-up itself
-// (solve_batch: one launch for the whole batch).
+// rows = false: the caller runs the rows' tangent-Jacobi set-up and the
+// from_x0 restart itself (solve_batch: one launch for the whole batch, which
+// copies the x0 that d.x0 names).
 int prepare_qp(mprgp_qp *q, const mprgp_params &p, hipStream_t s, int n_tiles, int *done_flag, int tag,
                QpDev &d, QpState &st, bool rows = true);
 int fill_stats(mprgp_qp *q, const QpState both[2], float ms, long launched, mprgp_stats *out);

@@ -1373,12 +1373,19 @@ __global__ void setup_rows(const double *__restrict__ dinv, const double *__rest
 
 // setup_rows for every QP of a batch table in one launch: blockIdx.y is the
 // QP, blockIdx.x a run of 256 of its nodes.  cs is the QP's own factor array
-// (read-only to the step kernel, written here).
+// (read-only to the step kernel, written here).  A QP whose descriptor names
+// an x0 also restarts from it: x = x0 on the same nodes.
 __global__ void __launch_bounds__(256) setup_rows_batch(const QpDev *__restrict__ qps) {
   const QpDev &q = qps[blockIdx.y];
   const int r = blockIdx.x * 256 + threadIdx.x;
   if (r >= q.n_nodes) return;
   const_cast<double *>(q.d.cs)[r] = row_factor(q.d.dinv, q.d.cn, q.cf[r], r);
+  if (q.x0) {
+    const size_t i = 3 * (size_t)r;
+    q.d.x[i] = q.x0[i];
+    q.d.x[i + 1] = q.x0[i + 1];
+    q.d.x[i + 2] = q.x0[i + 2];
+  }
 }
 
 // Decoupled plane collision at the current iterate: node r gets the row of the

@@ -131,6 +131,7 @@ extern "C" int mprgp_qp_solve_noalloc(mprgp_qp *q, const mprgp_params *params, m
   QpDev d;
   QpState st;
   describe_qp(q, p, n_tiles, reinterpret_cast<int *>(base + kFlagOff), 1, d, st);
+  if (p.from_x0) MPRGP_HIP(hipMemcpyAsync(q->x, q->x0, sizeof(double) * 3 * q->n, hipMemcpyDeviceToDevice, s));
   // the QP's set-up, as prepare_qp: FP32 values if the matrix changed, the
   // power start vector, the rows' tangent-Jacobi factors
   if (mixed && q->A->version32 != q->A->version) {

@@ -317,6 +317,47 @@ def test_batch_equals_alone_bitwise(ctx):
             assert sb[k] == sa[k]
 
 
+@pytest.mark.parametrize("prec", [M.FP64, M.MIXED])
+def test_from_x0_equals_reset_then_solve(ctx, prec):
+    """from_x0 = 1 restarts each QP from its stored warm start inside the
+    solve (on the batch's set-up launch, or by a copy for a lone QP and the
+    no-alloc solve): the same bits as mprgp_qp_reset_x and then the solve."""
+    import ctypes as C
+    scenes = [problems.config1(seed=s) for s in (21, 22)] + [problems.config1_flat(seed=23)]
+    qs = []
+    for sc in scenes:
+        A, b, node, normal, c, x0 = scene_qp(sc)
+        qs.append((make_qp(ctx, A, b, node, normal, c, x0), 2.0 / oracle.spec_rad(A, 1e-10)[0]))
+    per = [M.params(tol=1e-8, max_it=3000, alpha_bar=ab, precision=prec, project_first=1) for _, ab in qs]
+    ref = []
+    for (q, _), p in zip(qs, per):
+        q.reset_x()
+        rc, st = q.solve(p)
+        assert rc == 0
+        ref.append((q.get_x(), st["iterations"]))
+    # every x now holds its solution: a solve from x would stop at once
+    for p in per:
+        p.from_x0 = 1
+    rc, stats = M.solve_batch(ctx, [q for q, _ in qs], per_qp=per)
+    assert rc == 0
+    for (q, _), (xr, it), s in zip(qs, ref, stats):
+        assert s["iterations"] == it
+        np.testing.assert_array_equal(q.get_x(), xr)
+    for (q, _), (xr, it), p in zip(qs, ref, per):
+        rc, st = q.solve(p)
+        assert rc == 0 and st["iterations"] == it
+        np.testing.assert_array_equal(q.get_x(), xr)
+        st = M.Stats()
+        assert M._lib.load().mprgp_qp_solve_noalloc(q.h, C.byref(p), C.byref(st)) == 0
+        assert st.iterations == it
+        np.testing.assert_array_equal(q.get_x(), xr)
+    bad = M.Params.from_buffer_copy(per[0])
+    bad.from_x0 = 2
+    with pytest.raises(M.MprgpError) as e:
+        qs[0][0].solve(bad)
+    assert e.value.code == M._lib.EBADINPUT and "from_x0" in M.last_error()
+
+
 def test_config2_full_size_parity(ctx):
     sc = problems.config2()
     A, b, node, normal, c, x0 = scene_qp(sc)

@@ -5,6 +5,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 import mprgp_amd as M
 from mprgp_amd import problems, serving, tune, workloads
@@ -81,3 +82,24 @@ def test_tune_keys_and_shapes():
     assert 0 in tune.TAIL_CANDIDATES and all(t >= 0 for t in tune.TAIL_CANDIDATES)
     for s in tune.DEFAULT_CANDIDATES:
         assert s["tiles"] in (1, 2, 4, 8) and s["graph_steps"] > 0
+
+
+def test_batch_stats_sequence():
+    """solve_batch's stats: one dict per QP, made from the C array on first
+    access; indexing, negative indices, slices, iteration and len behave as
+    for a list."""
+    import mprgp_amd as M
+    raw = (M.Stats * 3)()
+    for i in range(3):
+        raw[i].code, raw[i].iterations, raw[i].residual = -i, 10 * i, 0.5 * i
+    st = M.BatchStats(raw)
+    assert len(st) == 3
+    assert st[1]["iterations"] == 10 and st[-1]["code"] == -2 and st[1] is st[1]
+    assert [s["residual"] for s in st] == [0.0, 0.5, 1.0]
+    assert [s["code"] for s in st[1:]] == [-1, -2]
+    assert sum(1 for s in st if s["code"] == 0) == 1
+    assert "reserved_d" not in st[0] and set(st[0]) == set(raw[0].as_dict())
+    with pytest.raises(IndexError):
+        st[3]
+    with pytest.raises(IndexError):
+        st[-4]
