@@ -1121,15 +1121,19 @@ __device__ __forceinline__ void slice_work(const QpDev &q, int s, int lane, int 
 // A tile is up to TS * SPW consecutive slices: wave w streams the tile's
 // slices w, w + TS, ..., w + (SPW - 1) TS one after the other (SPW slices
 // per wavefront, the call's slices_per_wave), adding each row's sums to its
-// lane's partials in that order.
+// lane's partials in that order.  SPW is 1 or 2; the second slice is a
+// second inlined call rather than a loop, whose body the register allocator
+// spilled (110 VGPRs against 3).
 template <int TS, bool MX, int UB, bool NT, int SPW>
 __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &st, double (&P)[kNumPart]) {
   const int wave = threadIdx.x / kSlice;
   const int lane = threadIdx.x % kSlice;
-  if (SPW == 1) {
+  if constexpr (SPW == 1) {
     if (wave < t.slices) slice_work<TS, MX, UB, NT>(q, t.slice0 + wave, lane, op, st, P);
   } else {
-    for (int w = wave; w < t.slices; w += TS) slice_work<TS, MX, UB, NT>(q, t.slice0 + w, lane, op, st, P);
+    static_assert(SPW == 2, "one or two slices per wavefront");
+    if (wave < t.slices) slice_work<TS, MX, UB, NT>(q, t.slice0 + wave, lane, op, st, P);
+    if (wave + TS < t.slices) slice_work<TS, MX, UB, NT>(q, t.slice0 + wave + TS, lane, op, st, P);
   }
 }
 
