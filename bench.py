@@ -305,10 +305,11 @@ def spmv_leg(M, workloads, ctx, qps, tiles, graph_steps):
                         graph_steps=graph_steps) for _ in qps]
         _, st = M.solve_batch(ctx, [q.qp for q in qps], per_qp=per)
         alg, its = 0, 0
+        idx = max(workloads.index_bytes(q.n_nodes) for q in qps)  # the launch's column index width
         for q in qps:
             k = int(q.qp.op_counts()[pow_it])
             its = max(its, k)
-            alg += k * workloads.op_bytes(pow_it, q.n_nodes, q.n_blocks, q.m, mixed=mixed)
+            alg += k * workloads.op_bytes(pow_it, q.n_nodes, q.n_blocks, q.m, mixed=mixed, idx=idx)
         dev_s = st[0]["elapsed_ms"] / 1e3
         launches = st[0]["launches"]
         out[name] = {"achieved": round(alg / dev_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -426,12 +427,13 @@ def main():
     # device time (HIP events on the solve stream around the launches)
     launches = stats[0]["launches"]
     dev_s = stats[0]["elapsed_ms"] / 1e3
-    alg = sum(workloads.solve_bytes(q.qp.op_counts(), q.n_nodes, q.n_blocks, q.m, mixed) for q in qps)
+    idx = max(workloads.index_bytes(q.n_nodes, args.slices_per_wave) for q in qps)
+    alg = sum(workloads.solve_bytes(q.qp.op_counts(), q.n_nodes, q.n_blocks, q.m, mixed, idx=idx) for q in qps)
     achieved = alg / dev_s / 1e9
     roof = {"kernel": "mprgp_step", "bound": "hbm", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": (traffic["fetch_bytes"] + traffic["write_bytes"]) if traffic else None,
-            "traffic_detail": traffic, "bytes_per_launch": int(alg / max(launches, 1)),
+            "traffic_detail": traffic, "bytes_per_launch": int(alg / max(launches, 1)), "index_bytes": idx,
             "avg_launch_us": round(1e6 * dev_s / max(launches, 1), 2), "launches": launches,
             "copy_gbps": round(copy_gbps, 1),
             "iterations_total": int(sum(s["iterations"] for s in stats))}

@@ -251,12 +251,16 @@ int mprgp::build_matrix(mprgp_ctx *ctx, int n_nodes, const int *rowptr, const in
   }
   size_t b_off = sizeof(int) * (A->n_slices + 1), b_col = sizeof(int) * slots,
          b_val = sizeof(double) * slots * 9, b_d = sizeof(double) * 3 * n_nodes, b_p = sizeof(int) * n_nodes;
+  // 16-bit columns for matrices whose node indices fit
+  const bool c16 = n_nodes <= 65536;
+  const size_t b_c16 = c16 ? sizeof(unsigned short) * slots : 0;
   int rc = ctx->alloc((void **)&A->d_sl_off, b_off);
   if (!rc) rc = ctx->alloc((void **)&A->d_col, b_col);
+  if (!rc && c16) rc = ctx->alloc((void **)&A->d_col16, b_c16);
   if (!rc) rc = ctx->alloc((void **)&A->d_val, b_val);
   if (!rc) rc = ctx->alloc((void **)&A->d_dinv, b_d);
   if (!rc) rc = ctx->alloc((void **)&A->d_perm, b_p);
-  A->bytes = b_off + b_col + b_val + b_d + b_p;
+  A->bytes = b_off + b_col + b_c16 + b_val + b_d + b_p;
   if (rc) {
     mprgp_matrix_destroy(A);
     return rc;
@@ -264,6 +268,7 @@ int mprgp::build_matrix(mprgp_ctx *ctx, int n_nodes, const int *rowptr, const in
   hipStream_t s = ctx->stream;
   hipError_t e = hipMemcpyAsync(A->d_sl_off, A->sl_off_h.data(), b_off, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(A->d_col, col.data(), b_col, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess && c16) e = launch_to_u16(s, A->d_col, A->d_col16, (size_t)slots);
   if (e == hipSuccess) e = hipMemcpyAsync(A->d_val, val.data(), b_val, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(A->d_perm, A->perm.data(), b_p, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = launch_setup_diag(s, A->d_sl_off, A->d_col, A->d_val, A->d_dinv, n_nodes);
@@ -497,6 +502,7 @@ extern "C" void mprgp_matrix_destroy(mprgp_matrix *A) {
   (void)hipStreamSynchronize(ctx->stream);
   ctx->release(A->d_sl_off, sizeof(int) * (A->n_slices + 1));
   ctx->release(A->d_col, sizeof(int) * A->slots);
+  ctx->release(A->d_col16, sizeof(unsigned short) * A->slots);
   ctx->release(A->d_val, sizeof(double) * A->slots * 9);
   ctx->release(A->d_dinv, sizeof(double) * 3 * A->n_nodes);
   ctx->release(A->d_perm, sizeof(int) * A->n_nodes);
@@ -1149,8 +1155,8 @@ int BatchTables::ensure(int n_qps, int n_slots) {
 }
 
 int BatchTables::graph(hipStream_t s, int ts, int steps, int grid, bool mixed, int variant, hipGraphExec_t *out,
-                       int spw) {
-  auto key = std::make_tuple(ts, steps, grid, (int)mixed, variant, spw);
+                       int spw, bool c16) {
+  auto key = std::make_tuple(ts, steps, grid, (int)mixed, variant, spw, (int)c16);
   auto it = graphs.find(key);
   if (it != graphs.end()) {
     *out = it->second;
@@ -1161,7 +1167,7 @@ int BatchTables::graph(hipStream_t s, int ts, int steps, int grid, bool mixed, i
   MPRGP_HIP(hipStreamSynchronize(s));
   MPRGP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   for (int i = 0; i < steps; ++i) {
-    hipError_t e = launch_step(ts, grid, s, d_qps, d_tiles, d_ctrl + 1, i & 1, mixed, variant, spw);
+    hipError_t e = launch_step(ts, grid, s, d_qps, d_tiles, d_ctrl + 1, i & 1, mixed, variant, spw, c16);
     if (e != hipSuccess) {
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(s, &g);
@@ -1319,6 +1325,7 @@ void describe_qp(const mprgp_qp *q, const mprgp_params &p, int n_tiles, int *don
   d.tag = tag;
   d.sl_off = q->A->d_sl_off;
   d.col = q->A->d_col;
+  d.col16 = q->A->d_col16;
   d.d.val = q->A->d_val;
   d.d.dinv = q->A->d_dinv;
   d.d.x = q->x; d.d.b = q->b; d.d.g = q->g; d.d.p = q->p; d.d.z = q->z; d.d.phi = q->phi;
@@ -1451,18 +1458,20 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
       if (h[i].first == h[i - 1].first)
         return fail(MPRGP_EBADINPUT, "qp %d appears twice in the batch", std::max(h[i].second, h[i - 1].second));
   }
-  bool any_mixed = false;
+  bool any_mixed = false, all_c16 = true;
   long total_slices = 0, max_steps = 0;
   std::vector<int> n_slices(count);
   for (int i = 0; i < count; ++i) {
     const mprgp_params &p = params[stride ? i : 0];
     any_mixed |= p.precision == MPRGP_MIXED;
+    all_c16 &= qps[i]->A->d_col16 != nullptr;
     total_slices += (n_slices[i] = qps[i]->A->n_slices);
     max_steps = std::max(max_steps, solve_step_bound(p));
   }
   const int ts = pick_ts(params[0], total_slices);
   const int spw = slices_per_wave_of(params[0]);
   const int span = ts * spw;
+  const bool c16 = use_col16(all_c16, spw);
   std::vector<Tile> tiles;
   for (int i = 0; i < count; ++i) append_tiles(i, qps[i], span, tiles);
   BatchTables &tab = ctx->tab;
@@ -1523,7 +1532,7 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
   int variant = step_variant(n_tiles, live_bytes(nullptr), tail_tiles);
   hipGraphExec_t ex = nullptr;
   if (use_graph)
-    if (int rc = tab.graph(s, ts, steps, grid, any_mixed, variant, &ex, spw)) return rc;
+    if (int rc = tab.graph(s, ts, steps, grid, any_mixed, variant, &ex, spw, c16)) return rc;
   int in_table = count;  // QPs whose tiles are in the table
 
   MPRGP_HIP(hipEventRecord(ctx->t0, s));
@@ -1535,7 +1544,8 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
       MPRGP_HIP(hipGraphLaunch(ex, s));
     } else {
       for (int i = 0; i < steps; ++i)
-        MPRGP_HIP(launch_step(ts, grid, s, tab.d_qps, tab.d_tiles, tab.d_ctrl + 1, i & 1, any_mixed, variant, spw));
+        MPRGP_HIP(launch_step(ts, grid, s, tab.d_qps, tab.d_tiles, tab.d_ctrl + 1, i & 1, any_mixed, variant, spw,
+                              c16));
     }
     launched += steps;
     MPRGP_HIP(hipMemcpyAsync(ctx->h_running + slot, tab.d_ctrl + 1, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1579,7 +1589,7 @@ static int run_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mprg
         shrunk_at = k;
         variant = step_variant((int)kept.size(), live_bytes(&keep), tail_tiles);
         if (use_graph)
-          if (int rc = tab.graph(s, ts, steps, grid, any_mixed, variant, &ex, spw)) return rc;
+          if (int rc = tab.graph(s, ts, steps, grid, any_mixed, variant, &ex, spw, c16)) return rc;
       }
     }
     ++k;

@@ -146,6 +146,7 @@ struct QpDev {
   int n_slices;
   const int *sl_off;
   const int *col;
+  const unsigned short *col16;  // the same columns in 16 bits, or null
   Vecs<double> d;
   Vecs<float> f;
   double *ev;            // power vector kept for the next solve (FP64)

@@ -184,12 +184,13 @@ int mprgp_engine::round_once(long r, bool &idle) {
   }
   // 1c. table upload
   int n_tiles = 0;
-  bool any_mixed = false;
+  bool any_mixed = false, all_c16 = true;
   double matrix_bytes = 0.0;
   for (const Slot &sl : slots)
     if (sl.req && sl.retiring == 0) {
       n_tiles += sl.n_tiles;
       any_mixed |= sl.req->p.precision == MPRGP_MIXED;
+      all_c16 &= sl.req->q->A->d_col16 != nullptr;
       matrix_bytes += matrix_stream_bytes(sl.req->q->A, sl.req->p.precision == MPRGP_MIXED);
     }
   const int grid = round_grid(n_tiles);
@@ -212,7 +213,9 @@ int mprgp_engine::round_once(long r, bool &idle) {
   // 2. replay
   if (n_tiles > 0) {
     hipGraphExec_t ex = nullptr;
-    if (int rc = tab.graph(s, ts, steps, grid, any_mixed, step_variant(n_tiles, matrix_bytes), &ex)) return rc;
+    if (int rc = tab.graph(s, ts, steps, grid, any_mixed, step_variant(n_tiles, matrix_bytes), &ex, 1,
+                           use_col16(all_c16, 1)))
+      return rc;
     MPRGP_HIP(hipGraphLaunch(ex, s));
     launches += steps;
   }

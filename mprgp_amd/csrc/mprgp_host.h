@@ -77,8 +77,10 @@ hipError_t launch_vmul(hipStream_t s, size_t n, const double *x, const double *y
 hipError_t launch_power_start(hipStream_t s, double *out, size_t n, double nr);
 // variant: a StepVariant (mprgp_dev.h), from step_variant() below
 // spw: slices per wavefront (1 or 2)
+// c16: every QP of the table has 16-bit column indices (ignored with spw 2)
 hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
-                       int *running, int parity, bool mixed, int variant = kStepCached, int spw = 1);
+                       int *running, int parity, bool mixed, int variant = kStepCached, int spw = 1,
+                       bool c16 = false);
 hipError_t launch_gather_nodes(hipStream_t s, const double *in, double *out, const int *perm, int n);
 // setup_rows for `count` QPs of a device QpDev table (count <= 65535)
 hipError_t launch_setup_rows_batch(hipStream_t s, const QpDev *qps, int count, int max_nodes);
@@ -101,6 +103,7 @@ hipError_t launch_dot(hipStream_t s, const double *a, const double *b, size_t n,
 // part (if non-null) must hold one double per 256 nodes; result receives the reduction
 hipError_t launch_projector(hipStream_t s, const ProjArgs &a, double *result);
 hipError_t launch_to_f32(hipStream_t s, const double *in, float *out, size_t n);
+hipError_t launch_to_u16(hipStream_t s, const int *in, unsigned short *out, size_t n);
 hipError_t launch_copy_stream(hipStream_t s, const void *in, void *out, size_t bytes);
 
 // Builds the SELL-64 device matrix from a node-order block CSR (blocks may be
@@ -128,12 +131,13 @@ struct BatchTables {
   int states_cap = 0;
   int ensure_states(int n_qps);
   // key: wavefronts per workgroup, launches, grid, mixed, kernel variant,
-  // slices per wavefront
-  std::map<std::tuple<int, int, int, int, int, int>, hipGraphExec_t> graphs;
-  std::map<std::tuple<int, int, int, int, int, int>, hipGraph_t> graph_defs;
+  // slices per wavefront, 16-bit columns
+  std::map<std::tuple<int, int, int, int, int, int, int>, hipGraphExec_t> graphs;
+  std::map<std::tuple<int, int, int, int, int, int, int>, hipGraph_t> graph_defs;
   int init(mprgp_ctx *c);
   int ensure(int n_qps, int n_slots);
-  int graph(hipStream_t s, int ts, int steps, int grid, bool mixed, int variant, hipGraphExec_t *out, int spw = 1);
+  int graph(hipStream_t s, int ts, int steps, int grid, bool mixed, int variant, hipGraphExec_t *out, int spw = 1,
+            bool c16 = false);
   void drop_graphs();
   void destroy();
 };
@@ -143,6 +147,13 @@ int check_batch_qp(const mprgp_ctx *ctx, const mprgp_qp *q, const mprgp_params &
 int pick_ts(const mprgp_params &p, long total_slices);
 // slices each wavefront streams per launch (params.slices_per_wave: 1 or 2)
 inline int slices_per_wave_of(const mprgp_params &p) { return p.slices_per_wave == 2 ? 2 : 1; }
+// Whether a launch table's SpMVs read 16-bit column indices: every QP's
+// matrix has them (all_have) and one slice per wavefront.  MPRGP_INDEX32=1
+// forces the 32-bit ones (the two give the same results, bit for bit).
+inline bool use_col16(bool all_have, int spw) {
+  const char *e = getenv("MPRGP_INDEX32");
+  return all_have && spw == 1 && !(e && e[0] && e[0] != '0');
+}
 void append_tiles(int slot, const mprgp_qp *q, int span, std::vector<Tile> &tiles);
 int tiles_for(int n_slices, int span);
 int round_grid(int n_tiles);
@@ -218,6 +229,9 @@ struct mprgp_matrix {
   std::vector<int> sl_off_h;
   int *d_sl_off = nullptr;
   int *d_col = nullptr;
+  // the same columns in 16 bits, for matrices of at most 65536 nodes (the
+  // step kernel's SpMVs read these when every QP of a launch has them)
+  unsigned short *d_col16 = nullptr;
   double *d_val = nullptr;
   double *d_dinv = nullptr;
   int *d_perm = nullptr;
@@ -232,7 +246,8 @@ struct mprgp_matrix {
 // Bytes of A one SpMV of the inner solve streams (values and column indices
 // of every SELL slot).
 inline double matrix_stream_bytes(const mprgp_matrix *A, bool mixed) {
-  return (double)A->slots * (9.0 * (mixed ? sizeof(float) : sizeof(double)) + sizeof(int));
+  return (double)A->slots *
+         (9.0 * (mixed ? sizeof(float) : sizeof(double)) + (A->d_col16 ? sizeof(unsigned short) : sizeof(int)));
 }
 
 struct mprgp_qp {

@@ -166,8 +166,10 @@ __device__ __forceinline__ V3 precond_of(const Vecs<T> &v, bool on, const Row &R
 // than the last-level cache, streaming them past it keeps the gathered
 // vectors cached (measured on 1.8 GB of FP64 matrices: batched SpMV 4.70 ->
 // 4.97 TB/s).  Matrices that fit stay cached from launch to launch instead.
-template <int NV, int U = (NV == 1 ? 3 : 1), bool NT = false, class T, class G>
-__device__ __forceinline__ void spmv_row(const int *sl_off, const int *col, const T *val, int s, int lane,
+// C: the column index type, 32-bit or 16-bit (2 of the 40 bytes an FP32
+// block streams: on the 256-QP batch, 3.1% shorter launches).
+template <int NV, int U = (NV == 1 ? 3 : 1), bool NT = false, class C, class T, class G>
+__device__ __forceinline__ void spmv_row(const int *sl_off, const C *col, const T *val, int s, int lane,
                                          G gather, V3 (&acc)[NV]) {
 #pragma unroll
   for (int t = 0; t < NV; ++t) acc[t] = V3{0.0, 0.0, 0.0};
@@ -181,7 +183,7 @@ __device__ __forceinline__ void spmv_row(const int *sl_off, const int *col, cons
     T a[U][9];  // kept in the storage precision until its multiply
     V3 xs[U][NV];
 #pragma unroll
-    for (int u = 0; u < U; ++u) j[u] = ldm<NT>(cp + (k + u) * kSlice);
+    for (int u = 0; u < U; ++u) j[u] = (int)ldm<NT>(cp + (k + u) * kSlice);
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -204,7 +206,7 @@ __device__ __forceinline__ void spmv_row(const int *sl_off, const int *col, cons
       }
   }
   for (; k < W; ++k) {
-    const int j = ldm<NT>(cp + k * kSlice);
+    const int j = (int)ldm<NT>(cp + k * kSlice);
     T a[9];
 #pragma unroll
     for (int e = 0; e < 9; ++e) a[e] = ldm<NT>(vp + (size_t)k * (9 * kSlice) + e * kSlice);
@@ -642,7 +644,17 @@ __device__ __forceinline__ void loop_head_row(const Vecs<T> &v, const Row &R, bo
   P[2] += dot3(sub(x, px), ph);
 }
 
-template <class T, bool MX, int UB, bool NT>
+// The column indices the SpMVs read: 16-bit (C16, matrices of at most
+// 65536 nodes) or 32-bit; the same columns either way.
+template <bool C16>
+__device__ __forceinline__ auto cols(const QpDev &q) {
+  if constexpr (C16)
+    return q.col16;
+  else
+    return q.col;
+}
+
+template <class T, bool MX, int UB, bool NT, bool C16>
 __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int s, int lane, int r, int op,
                                          const QpState &st, double (&P)[kNumPart]) {
   const bool mixed = MX && st.mixed;
@@ -668,7 +680,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       // g = A x - b ; DECIDE_FACE(x) ; phi ; z ; p = z   (MPRGPSolver.h:148-156, 263-268, 210-215)
       V3 acc[1];
       const double *x = xc;
-      spmv_row<1, UB, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
+      spmv_row<1, UB, NT>(q.sl_off, cols<C16>(q), v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
       V3 g = sub(acc[0], ld3(v.b, r));
       const V3 xr = ld3(xc, r);  // in flight with the row flag
       Row R = ldrow(q, v, r);
@@ -700,7 +712,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
     case OP_CL_AP: {
       V3 acc[1];
       const T *p = v.p;
-      spmv_row<1, UB, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(p, j); }, acc);
+      spmv_row<1, UB, NT>(q.sl_off, cols<C16>(q), v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(p, j); }, acc);
       st3(v.ap, r, acc[0]);
       V3 pr = ld3(v.p, r);
       P[0] += dot3(pr, acc[0]);
@@ -764,7 +776,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
         const T *pp = v.p;
         const double alpha = st.alpha;
         V3 acc[1];
-        spmv_row<1, 2, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) {
+        spmv_row<1, 2, NT>(q.sl_off, cols<C16>(q), v.val, s, lane, [&](int j, V3 (&xs)[1]) {
           const V3 yj = axmy(ld3(xx, j), alpha, ld3(pp, j));
           const V3 nj = ld3(v.cn, j);
           const double vj = gmem(v.cc)[j] - dot3(nj, yj);
@@ -807,7 +819,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       const T *zz = v.z, *pp = v.p;
       const double beta = st.beta_cg;
       V3 acc[1];
-      spmv_row<1, UB, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) {
+      spmv_row<1, UB, NT>(q.sl_off, cols<C16>(q), v.val, s, lane, [&](int j, V3 (&xs)[1]) {
         xs[0] = round_to<T>(axmy(ld3(zz, j), beta, ld3(pp, j)));
       }, acc);
       const V3 pn = round_to<T>(axmy(ld3(v.z, r), beta, ld3(v.p, r)));
@@ -823,7 +835,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       if (st.cg_entry) {
         const double *x = xc;
         V3 acc[2];
-        spmv_row<2, 1, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[2]) {
+        spmv_row<2, 1, NT>(q.sl_off, cols<C16>(q), v.val, s, lane, [&](int j, V3 (&xs)[2]) {
           xs[0] = ld3(py, j);
           xs[1] = ld3(x, j);
         }, acc);
@@ -834,7 +846,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
         P[3] += dot3(xr, br);
       } else {
         V3 acc[1];
-        spmv_row<1, UB, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(py, j); }, acc);
+        spmv_row<1, UB, NT>(q.sl_off, cols<C16>(q), v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(py, j); }, acc);
         P[0] += dot3(pyr, acc[0]);
         P[1] += dot3(pyr, br);
       }
@@ -849,7 +861,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       // the other buffer, current only once the step is accepted.
       const double *py = v.py;
       V3 acc[1];
-      spmv_row<1, UB, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(py, j); }, acc);
+      spmv_row<1, UB, NT>(q.sl_off, cols<C16>(q), v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(py, j); }, acc);
       const V3 pyr = ld3(v.py, r);
       P[0] += dot3(pyr, acc[0]);
       P[1] += dot3(pyr, ld3(v.b, r));
@@ -910,7 +922,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       // to py because neighbours still gather y in this launch.
       V3 acc[1];
       const double *y = yc;
-      spmv_row<1, UB, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(y, j); }, acc);
+      spmv_row<1, UB, NT>(q.sl_off, cols<C16>(q), v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(y, j); }, acc);
       V3 g = sub(acc[0], ld3(v.b, r));
       Row R = ldrow(q, v, r);
       V3 yr = ld3(yc, r);
@@ -929,7 +941,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
       V3 acc[2];
       const double *x = xc;
       const T *be = v.beta;
-      spmv_row<2, 1, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[2]) {
+      spmv_row<2, 1, NT>(q.sl_off, cols<C16>(q), v.val, s, lane, [&](int j, V3 (&xs)[2]) {
         xs[0] = ld3(x, j);
         xs[1] = ld3(be, j);
       }, acc);
@@ -944,7 +956,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
     case OP_CL_PROP1: {
       V3 acc[1];
       const T *be = v.beta;
-      spmv_row<1, UB, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(be, j); }, acc);
+      spmv_row<1, UB, NT>(q.sl_off, cols<C16>(q), v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(be, j); }, acc);
       st3(v.ap, r, acc[0]);
       V3 d = ld3(v.beta, r);
       P[0] += dot3(d, acc[0]);
@@ -1011,7 +1023,7 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
 // Ops on FP64 vectors in both forms: projection of the warm start, the power
 // method (its SpMV reads the matrix of the QP's form), and the outer
 // refinement of the mixed form.
-template <bool MX, int UB, bool NT>
+template <bool MX, int UB, bool NT, bool C16>
 __device__ __forceinline__ void row_work_fp64(const QpDev &q, int s, int lane, int r, int op,
                                               const QpState &st, double (&P)[kNumPart]) {
   const Vecs<double> &v = q.d;
@@ -1048,9 +1060,9 @@ __device__ __forceinline__ void row_work_fp64(const QpDev &q, int s, int lane, i
       V3 acc[1];
       auto gather = [&](int j, V3 (&xs)[1]) { xs[0] = ld3(in, j); };
       if (MX && st.mixed)
-        spmv_row<1, UB, NT>(q.sl_off, q.col, q.f.val, s, lane, gather, acc);
+        spmv_row<1, UB, NT>(q.sl_off, cols<C16>(q), q.f.val, s, lane, gather, acc);
       else
-        spmv_row<1, UB, NT>(q.sl_off, q.col, v.val, s, lane, gather, acc);
+        spmv_row<1, UB, NT>(q.sl_off, cols<C16>(q), v.val, s, lane, gather, acc);
       const V3 w = k == 0 ? acc[0] : V3{acc[0].x / nu, acc[0].y / nu, acc[0].z / nu};
       st3(wk, r, w);
       P[0] += dot3(w, w);
@@ -1067,7 +1079,7 @@ __device__ __forceinline__ void row_work_fp64(const QpDev &q, int s, int lane, i
       // is set up in the FP32 storage with d = 0.
       V3 acc[1];
       const double *x = v.x;
-      spmv_row<1, UB, NT>(q.sl_off, q.col, v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
+      spmv_row<1, UB, NT>(q.sl_off, cols<C16>(q), v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
       V3 g = sub(acc[0], ld3(v.b, r));
       V3 xr = ld3(v.x, r);  // in flight with the row flag
       Row R = ldrow(q, v, r);
@@ -1104,17 +1116,17 @@ __device__ __forceinline__ bool fp64_op(int op) {
 }
 
 // One slice of a tile per wavefront.
-template <int TS, bool MX, int UB, bool NT>
+template <int TS, bool MX, int UB, bool NT, bool C16>
 __device__ __forceinline__ void slice_work(const QpDev &q, int s, int lane, int op, const QpState &st,
                                            double (&P)[kNumPart]) {
   const int r = s * kSlice + lane;
   if (r >= q.n_nodes) return;
   if (fp64_op(op)) {
-    row_work_fp64<MX, UB, NT>(q, s, lane, r, op, st, P);
+    row_work_fp64<MX, UB, NT, C16>(q, s, lane, r, op, st, P);
   } else if (MX && st.mixed) {
-    row_work<float, MX, UB, NT>(q, q.f, s, lane, r, op, st, P);
+    row_work<float, MX, UB, NT, C16>(q, q.f, s, lane, r, op, st, P);
   } else {
-    row_work<double, MX, UB, NT>(q, q.d, s, lane, r, op, st, P);
+    row_work<double, MX, UB, NT, C16>(q, q.d, s, lane, r, op, st, P);
   }
 }
 
@@ -1124,16 +1136,16 @@ __device__ __forceinline__ void slice_work(const QpDev &q, int s, int lane, int 
 // lane's partials in that order.  SPW is 1 or 2; the second slice is a
 // second inlined call rather than a loop, whose body the register allocator
 // spilled (110 VGPRs against 3).
-template <int TS, bool MX, int UB, bool NT, int SPW>
+template <int TS, bool MX, int UB, bool NT, int SPW, bool C16>
 __device__ void tile_work(const QpDev &q, const Tile &t, int op, const QpState &st, double (&P)[kNumPart]) {
   const int wave = threadIdx.x / kSlice;
   const int lane = threadIdx.x % kSlice;
   if constexpr (SPW == 1) {
-    if (wave < t.slices) slice_work<TS, MX, UB, NT>(q, t.slice0 + wave, lane, op, st, P);
+    if (wave < t.slices) slice_work<TS, MX, UB, NT, C16>(q, t.slice0 + wave, lane, op, st, P);
   } else {
     static_assert(SPW == 2, "one or two slices per wavefront");
-    if (wave < t.slices) slice_work<TS, MX, UB, NT>(q, t.slice0 + wave, lane, op, st, P);
-    if (wave + TS < t.slices) slice_work<TS, MX, UB, NT>(q, t.slice0 + wave + TS, lane, op, st, P);
+    if (wave < t.slices) slice_work<TS, MX, UB, NT, C16>(q, t.slice0 + wave, lane, op, st, P);
+    if (wave + TS < t.slices) slice_work<TS, MX, UB, NT, C16>(q, t.slice0 + wave + TS, lane, op, st, P);
   }
 }
 
@@ -1187,8 +1199,9 @@ constexpr int kUnroll = MPRGP_STEP_UNROLL;
 constexpr int kUnrollTail = 6;
 
 // MX: the batch holds QPs of the mixed form (their inner ops run on the FP32
-// copies); FP64 QPs of the same batch take the FP64 path.
-template <int TS, bool MX, int UB, bool NT, int SPW>
+// copies); FP64 QPs of the same batch take the FP64 path.  C16: every QP of
+// the batch has 16-bit column indices (QpDev::col16).
+template <int TS, bool MX, int UB, bool NT, int SPW, bool C16>
 __global__ void __launch_bounds__(TS * kSlice, UB == kUnroll ? MPRGP_STEP_MIN_WAVES : 2)
     mprgp_step(const QpDev *__restrict__ qps,
                                                            const Tile *__restrict__ tiles,
@@ -1247,7 +1260,7 @@ __global__ void __launch_bounds__(TS * kSlice, UB == kUnroll ? MPRGP_STEP_MIN_WA
 #pragma unroll
     for (int j = 0; j < kNumPart; ++j) P[j] = 0.0;
     if (has_min) P[kMinSlot] = kDblMax;
-    tile_work<TS, MX, UB, NT, SPW>(q, t, op, st, P);
+    tile_work<TS, MX, UB, NT, SPW, C16>(q, t, op, st, P);
     // wave then block reduction in a fixed order, over the op's slots only
     const int ns = has_min ? kMinSlot + 1 : sum_slots(op);
     if (ns > 0) {
@@ -1546,6 +1559,11 @@ __global__ void __launch_bounds__(256) power_start(double *__restrict__ out, siz
 }
 
 // FP32 copies for the mixed form.
+__global__ void to_u16(const int *__restrict__ in, unsigned short *__restrict__ out, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = (unsigned short)in[i];
+}
+
 __global__ void to_f32(const double *__restrict__ in, float *__restrict__ out, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -1593,41 +1611,51 @@ __global__ void __launch_bounds__(256) spmv_plain(const int *__restrict__ sl_off
 
 // Host-callable launch wrappers ------------------------------------------
 
-template <bool MX, int UB, bool NT, int SPW>
+template <bool MX, int UB, bool NT, int SPW, bool C16>
 static hipError_t launch_step_t(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
                                 int *running, int parity) {
   switch (ts) {
-    case 1: mprgp_step<1, MX, UB, NT, SPW><<<grid, 1 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
-    case 2: mprgp_step<2, MX, UB, NT, SPW><<<grid, 2 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
-    case 4: mprgp_step<4, MX, UB, NT, SPW><<<grid, 4 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
-    case 8: mprgp_step<8, MX, UB, NT, SPW><<<grid, 8 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
+    case 1: mprgp_step<1, MX, UB, NT, SPW, C16><<<grid, 1 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
+    case 2: mprgp_step<2, MX, UB, NT, SPW, C16><<<grid, 2 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
+    case 4: mprgp_step<4, MX, UB, NT, SPW, C16><<<grid, 4 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
+    case 8: mprgp_step<8, MX, UB, NT, SPW, C16><<<grid, 8 * kSlice, 0, s>>>(qps, tiles, running, parity); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-template <int SPW>
+template <int SPW, bool C16>
 static hipError_t launch_step_v(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
                                 int *running, int parity, bool mixed, int variant) {
   switch (variant) {
     case kStepCached:
-      return mixed ? launch_step_t<true, kUnroll, false, SPW>(ts, grid, s, qps, tiles, running, parity)
-                   : launch_step_t<false, kUnroll, false, SPW>(ts, grid, s, qps, tiles, running, parity);
+      return mixed ? launch_step_t<true, kUnroll, false, SPW, C16>(ts, grid, s, qps, tiles, running, parity)
+                   : launch_step_t<false, kUnroll, false, SPW, C16>(ts, grid, s, qps, tiles, running, parity);
     case kStepStream:
-      return mixed ? launch_step_t<true, kUnroll, true, SPW>(ts, grid, s, qps, tiles, running, parity)
-                   : launch_step_t<false, kUnroll, true, SPW>(ts, grid, s, qps, tiles, running, parity);
+      return mixed ? launch_step_t<true, kUnroll, true, SPW, C16>(ts, grid, s, qps, tiles, running, parity)
+                   : launch_step_t<false, kUnroll, true, SPW, C16>(ts, grid, s, qps, tiles, running, parity);
     case kStepTail:
-      return mixed ? launch_step_t<true, kUnrollTail, false, SPW>(ts, grid, s, qps, tiles, running, parity)
-                   : launch_step_t<false, kUnrollTail, false, SPW>(ts, grid, s, qps, tiles, running, parity);
+      if constexpr (C16) {
+        return hipErrorInvalidValue;  // the tail form reads 32-bit columns (launch_step)
+      } else {
+        return mixed ? launch_step_t<true, kUnrollTail, false, SPW, C16>(ts, grid, s, qps, tiles, running, parity)
+                     : launch_step_t<false, kUnrollTail, false, SPW, C16>(ts, grid, s, qps, tiles, running, parity);
+      }
     default: return hipErrorInvalidValue;
   }
 }
 
+// The 16-bit columns pay where a launch streams its matrices (2 of the 40
+// bytes of an FP32 block); the tail form, bound by a lone row's chain of
+// load rounds, keeps the 32-bit ones (the plain one-QP loop ran 2.4% slower
+// with 16-bit ones).
 hipError_t launch_step(int ts, int grid, hipStream_t s, const QpDev *qps, const Tile *tiles,
-                       int *running, int parity, bool mixed, int variant, int spw) {
-  if (spw == 2) return launch_step_v<2>(ts, grid, s, qps, tiles, running, parity, mixed, variant);
+                       int *running, int parity, bool mixed, int variant, int spw, bool c16) {
+  if (spw == 2) return launch_step_v<2, false>(ts, grid, s, qps, tiles, running, parity, mixed, variant);
   if (spw != 1) return hipErrorInvalidValue;
-  return launch_step_v<1>(ts, grid, s, qps, tiles, running, parity, mixed, variant);
+  return c16 && variant != kStepTail
+             ? launch_step_v<1, true>(ts, grid, s, qps, tiles, running, parity, mixed, variant)
+             : launch_step_v<1, false>(ts, grid, s, qps, tiles, running, parity, mixed, variant);
 }
 
 hipError_t launch_setup_rows_batch(hipStream_t s, const QpDev *qps, int count, int max_nodes) {
@@ -1727,6 +1755,12 @@ hipError_t launch_to_f32(hipStream_t s, const double *in, float *out, size_t n) 
   if (n == 0) return hipSuccess;
   size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
   to_f32<<<(unsigned)blocks, 256, 0, s>>>(in, out, n);
+  return hipGetLastError();
+}
+hipError_t launch_to_u16(hipStream_t s, const int *in, unsigned short *out, size_t n) {
+  if (n == 0) return hipSuccess;
+  size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+  to_u16<<<(unsigned)blocks, 256, 0, s>>>(in, out, n);
   return hipGetLastError();
 }
 hipError_t launch_copy_stream(hipStream_t s, const void *in, void *out, size_t bytes) {

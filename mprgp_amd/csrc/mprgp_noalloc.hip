@@ -117,6 +117,7 @@ extern "C" int mprgp_qp_solve_noalloc(mprgp_qp *q, const mprgp_params *params, m
   const int ts = pick_ts(p, q->A->n_slices);
   const int spw = slices_per_wave_of(p);
   const int span = ts * spw;  // as mprgp_qp_solve
+  const bool c16 = use_col16(q->A->d_col16 != nullptr, spw);
   const int n_tiles = tiles_for(q->A->n_slices, span);
   const int grid = round_grid(n_tiles);
   if (grid > kNoAllocMaxTiles)
@@ -166,7 +167,7 @@ extern "C" int mprgp_qp_solve_noalloc(mprgp_qp *q, const mprgp_params *params, m
   long launched = 0;
   for (;;) {
     for (int i = 0; i < steps; ++i)
-      MPRGP_HIP(launch_step(ts, grid, s, qps, tiles, running, i & 1, mixed, variant, spw));
+      MPRGP_HIP(launch_step(ts, grid, s, qps, tiles, running, i & 1, mixed, variant, spw, c16));
     launched += steps;
     MPRGP_HIP(hipMemcpyAsync(ctx->h_running, running, sizeof(int), hipMemcpyDeviceToHost, s));
     MPRGP_HIP(hipStreamSynchronize(s));

@@ -38,8 +38,9 @@ class DeviceQP:
 
 
 # Streams each op touches, every stream counted once (ideal traffic): "M" the
-# matrix of the op's precision form and "M64" the FP64 matrix (values, int32
-# column indices, slice offsets; a fused two-vector SpMV still reads it once);
+# matrix of the op's precision form and "M64" the FP64 matrix (values, column
+# indices of `idx` bytes, slice offsets; a fused two-vector SpMV still reads
+# it once);
 # vector names stand for 3 values per node (gathered SpMV inputs count once);
 # "rows" is the per-node flag byte plus the 6 FP64 words of each constraint
 # row (normal, offset, |n|^2, tangent-Jacobi factor); "cc" one FP64 offset per
@@ -81,15 +82,24 @@ MIXED_F32 = {"M", "b", "g", "phi", "beta", "p", "z", "ap"}
 # mixed form's f(x)).
 
 
-def spmv_bytes(N, nb, s=8):
-    """Matrix once, input vector once, output vector once (index bytes int32)."""
-    return nb * (9 * s + 4) + 4 * (N + 1) + 2 * 3 * N * s
+def index_bytes(n_nodes, slices_per_wave=1):
+    """Bytes of one column index in the algorithmic count: 16-bit for
+    matrices of at most 65536 nodes at one slice per wavefront (the step
+    kernel reads those when every QP of a launch qualifies; its tail form, for
+    launches of few tiles, reads the 32-bit copy, and those 2 extra bytes
+    count as traffic, not as algorithmic bytes), else 32-bit."""
+    return 2 if n_nodes <= 65536 and slices_per_wave == 1 else 4
 
 
-def op_bytes(op, N, nb, m, mixed=False, precond=True):
+def spmv_bytes(N, nb, s=8, idx=4):
+    """Matrix once, input vector once, output vector once."""
+    return nb * (9 * s + idx) + 4 * (N + 1) + 2 * 3 * N * s
+
+
+def op_bytes(op, N, nb, m, mixed=False, precond=True, idx=4):
     """Algorithmic HBM bytes of one device op on one QP (ideal traffic: every
     stream read or written once).  ``mixed``: the inner-solve ops read the
-    FP32 matrix and FP32 direction vectors."""
+    FP32 matrix and FP32 direction vectors; ``idx``: bytes per column index."""
     name = OPS[op]
     total = 0
     streams = _STREAMS.get(name, ())
@@ -102,7 +112,7 @@ def op_bytes(op, N, nb, m, mixed=False, precond=True):
             total += N
         elif st in ("M", "M64"):
             s = 4 if (mixed and st == "M") else 8
-            total += nb * (9 * s + 4) + 4 * (N + 1)
+            total += nb * (9 * s + idx) + 4 * (N + 1)
         elif st == "dinv" and not precond:
             continue
         else:
@@ -111,10 +121,10 @@ def op_bytes(op, N, nb, m, mixed=False, precond=True):
     return total
 
 
-def solve_bytes(counts, N, nb, m, mixed=False, precond=True):
+def solve_bytes(counts, N, nb, m, mixed=False, precond=True, idx=4):
     """Algorithmic bytes of a whole solve from its per-op launch counts."""
     total = 0
     for op, k in enumerate(counts[:len(OPS)]):
         if k:
-            total += int(k) * op_bytes(op, N, nb, m, mixed, precond)
+            total += int(k) * op_bytes(op, N, nb, m, mixed, precond, idx)
     return total

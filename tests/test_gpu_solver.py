@@ -358,6 +358,32 @@ def test_from_x0_equals_reset_then_solve(ctx, prec):
     assert e.value.code == M._lib.EBADINPUT and "from_x0" in M.last_error()
 
 
+@pytest.mark.parametrize("prec", [M.FP64, M.MIXED])
+def test_index_widths_agree_bitwise(ctx, prec, monkeypatch):
+    """The step kernel's SpMVs read 16-bit column indices for matrices of at
+    most 65536 nodes and 32-bit ones otherwise (or with MPRGP_INDEX32=1): the
+    same columns, so the same bits, in a batch and alone."""
+    scenes = [problems.config1(seed=s) for s in (31, 32)] + [problems.config1_flat(seed=33)]
+    qs = []
+    for sc in scenes:
+        A, b, node, normal, c, x0 = scene_qp(sc)
+        qs.append((make_qp(ctx, A, b, node, normal, c, x0), 2.0 / oracle.spec_rad(A, 1e-10)[0]))
+    per = [M.params(tol=1e-8, max_it=3000, alpha_bar=ab, precision=prec, project_first=1, from_x0=1)
+           for _, ab in qs]
+    out = {}
+    for wide in ("0", "1"):
+        monkeypatch.setenv("MPRGP_INDEX32", wide)
+        rc, st = M.solve_batch(ctx, [q for q, _ in qs], per_qp=per)
+        assert rc == 0
+        res = [(q.get_x(), s["iterations"], s["residual"]) for (q, _), s in zip(qs, st)]
+        rc, s1 = qs[2][0].solve(per[2])
+        res.append((qs[2][0].get_x(), s1["iterations"], s1["residual"]))
+        out[wide] = res
+    for (x16, i16, r16), (x32, i32, r32) in zip(out["0"], out["1"]):
+        np.testing.assert_array_equal(x16, x32)
+        assert i16 == i32 and r16 == r32
+
+
 def test_config2_full_size_parity(ctx):
     sc = problems.config2()
     A, b, node, normal, c, x0 = scene_qp(sc)

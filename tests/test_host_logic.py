@@ -42,6 +42,10 @@ def test_byte_accounting():
     # no preconditioner: no D^-1 stream
     assert (workloads.op_bytes(op("CG_A"), N, nb, m) - workloads.op_bytes(op("CG_A"), N, nb, m, precond=False)
             == v64)
+    # 16-bit column indices: 2 bytes less per block
+    assert workloads.op_bytes(op("CG_AP"), N, nb, m, True, idx=2) == mat32 - 2 * nb + 4 * v32
+    assert workloads.index_bytes(65536) == 2 and workloads.index_bytes(65537) == 4
+    assert workloads.index_bytes(100, slices_per_wave=2) == 4
     counts = np.zeros(32, np.int32)
     counts[op("CG_AP")] = 3
     counts[op("TOP")] = 2
