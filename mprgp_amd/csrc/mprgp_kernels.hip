@@ -1169,12 +1169,45 @@ __device__ __forceinline__ void sum_part(const double *pp, int nt, int lane, dou
     for (int j = kGroup; j < kNumPart; ++j) acc[j] = 0.0;
     mn = fmin(lane < nt ? a[0][kMinSlot] : kDblMax, lane + kSlice < nt ? a[1][kMinSlot] : kDblMax);
   }
-  for (int k = lane + 2 * kSlice; k < nt; k += kSlice) {  // QPs of more than 128 tiles
+  // QPs of more than 128 tiles: rounds of four columns per lane, every load
+  // of a round issued before its sums, then one last round of the columns
+  // left.  Which columns a round holds is the same on every lane (a column of
+  // 64 tiles exists if its first tile does); in the last round lanes past the
+  // last tile load that tile and do not add it.  Each lane adds its columns
+  // in order, as one column at a time would.
+  int k0 = 2 * kSlice;
+  for (; k0 + 4 * kSlice <= nt; k0 += 4 * kSlice) {
+    double v[4][kGroup];
 #pragma unroll
-    for (int j = 0; j < kGroup; ++j) {
-      const double v = gmem(pp)[(size_t)(J0 + j) * nt + k];
-      acc[J0 + j] += v;
-      if (J0 == 0 && j == kMinSlot) mn = fmin(mn, v);
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int j = 0; j < kGroup; ++j) v[u][j] = gmem(pp)[(size_t)(J0 + j) * nt + k0 + u * kSlice + lane];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int j = 0; j < kGroup; ++j) {
+        acc[J0 + j] += v[u][j];
+        if (J0 == 0 && j == kMinSlot) mn = fmin(mn, v[u][j]);
+      }
+  }
+  if (k0 < nt) {
+    double v[4][kGroup];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (k0 + u * kSlice >= nt) break;
+      const int k = min(k0 + u * kSlice + lane, nt - 1);
+#pragma unroll
+      for (int j = 0; j < kGroup; ++j) v[u][j] = gmem(pp)[(size_t)(J0 + j) * nt + k];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (k0 + u * kSlice >= nt) break;
+      const bool in = k0 + u * kSlice + lane < nt;
+#pragma unroll
+      for (int j = 0; j < kGroup; ++j) {
+        acc[J0 + j] = in ? acc[J0 + j] + v[u][j] : acc[J0 + j];
+        if (J0 == 0 && j == kMinSlot) mn = in ? fmin(mn, v[u][j]) : mn;
+      }
     }
   }
 }
