@@ -1191,10 +1191,13 @@ int BatchTables::graph(hipStream_t s, int ts, int steps, int grid, bool mixed, i
 
 // Wavefronts per workgroup: small batches use narrow workgroups so enough of
 // them exist to fill the chip.  A QP's partial sums are grouped per tile, so
-// its result is bit-identical across batches that use the same value.
+// its result is bit-identical across batches that use the same value.  From
+// 512 slices on, 2 wavefronts: a lone QP's launch then re-reduces half as
+// many tiles' partial sums (config 2, 696 slices: 14.0 against 15.0 us per
+// power-method launch; a 224-slice QP: 9.7 against 8.9 us).
 int pick_ts(const mprgp_params &p, long total_slices) {
   if (p.tiles == 1 || p.tiles == 2 || p.tiles == 4 || p.tiles == 8) return p.tiles;
-  if (total_slices < 2048) return 1;
+  if (total_slices < 512) return 1;
   if (total_slices < 8192) return 2;
   return 4;
 }
