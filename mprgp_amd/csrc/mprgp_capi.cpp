@@ -7,7 +7,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <climits>
+#include <memory>
 #include <numeric>
+#include <thread>
 
 #include "mprgp_host.h"
 #include "mprgp_qpfile.h"
@@ -28,6 +31,50 @@ double power_start_norm(size_t n) {
 }  // namespace mprgp
 
 using namespace mprgp;
+
+namespace {
+
+constexpr int kHostThreads = 16;
+
+// Runs f(t, i0, i1) over [0, n) in contiguous ranges, range t on host thread
+// t (range 0 on the caller's): up to kHostThreads, and none with fewer than
+// `grain` items.  An exception in any range (or a thread that cannot start,
+// whose range then runs on the caller's thread) surfaces as std::bad_alloc
+// once every thread has joined.
+template <class F>
+void host_parallel(int n, int grain, F &&f) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  int nt = (int)std::min(hw ? hw : 1u, (unsigned)kHostThreads);
+  nt = std::max(1, std::min(nt, n / std::max(grain, 1)));
+  auto bounds = [&](int t, int &i0, int &i1) {
+    i0 = (int)((long long)n * t / nt);
+    i1 = (int)((long long)n * (t + 1) / nt);
+  };
+  char failed[kHostThreads] = {0};
+  auto run = [&](int t) {
+    int i0, i1;
+    bounds(t, i0, i1);
+    try {
+      f(t, i0, i1);
+    } catch (...) {
+      failed[t] = 1;
+    }
+  };
+  std::vector<std::thread> th;
+  int started = 1;
+  try {
+    th.reserve(nt);
+    for (; started < nt; ++started) th.emplace_back(run, started);
+  } catch (...) {
+  }
+  run(0);
+  for (int t = started; t < nt; ++t) run(t);
+  for (auto &x : th) x.join();
+  for (int t = 0; t < nt; ++t)
+    if (failed[t]) throw std::bad_alloc();
+}
+
+}  // namespace
 
 int mprgp_ctx::alloc(void **p, size_t bytes) {
   *p = nullptr;
@@ -169,21 +216,57 @@ int mprgp::build_matrix(mprgp_ctx *ctx, int n_nodes, const int *rowptr, const in
   *out = nullptr;
   const long long nb = rowptr[n_nodes];
   if (rowptr[0] != 0 || nb <= 0) return fail(MPRGP_EBADINPUT, "rowptr must start at 0 and be non-empty");
+  // Checked in the order a row-by-row pass would meet the faults: the first
+  // row whose rowptr decreases; before it, the first row with a column out of
+  // range or no diagonal block; then the first non-finite value.
+  int bad_ptr = n_nodes;
   std::vector<int> len(n_nodes);
   for (int i = 0; i < n_nodes; ++i) {
-    if (rowptr[i + 1] < rowptr[i]) return fail(MPRGP_EBADINPUT, "rowptr not monotone at row %d", i);
-    len[i] = rowptr[i + 1] - rowptr[i];
-    bool diag = false;
-    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
-      if (colidx[k] < 0 || colidx[k] >= n_nodes)
-        return fail(MPRGP_EBADINPUT, "column index %d out of range in row %d", colidx[k], i);
-      diag |= colidx[k] == i;
+    if (rowptr[i + 1] < rowptr[i]) {
+      bad_ptr = i;
+      break;
     }
-    if (!diag) return fail(MPRGP_EBADINPUT, "row %d has no diagonal block", i);
+    len[i] = rowptr[i + 1] - rowptr[i];
   }
-  if (blocks)
-    for (long long k = 0; k < 9 * nb; ++k)
-      if (!std::isfinite(blocks[k])) return fail(MPRGP_EBADINPUT, "non-finite matrix value at %lld", k);
+  int bad_row[kHostThreads];
+  std::fill(bad_row, bad_row + kHostThreads, INT_MAX);
+  host_parallel(bad_ptr, 4096, [&](int t, int i0, int i1) {
+    for (int i = i0; i < i1; ++i) {
+      bool ok = false;
+      for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+        if (colidx[k] < 0 || colidx[k] >= n_nodes) {
+          ok = false;
+          break;
+        }
+        ok |= colidx[k] == i;
+      }
+      if (!ok) {
+        bad_row[t] = i;
+        return;
+      }
+    }
+  });
+  const int first_bad = *std::min_element(bad_row, bad_row + kHostThreads);
+  if (first_bad != INT_MAX) {
+    for (int k = rowptr[first_bad]; k < rowptr[first_bad + 1]; ++k)
+      if (colidx[k] < 0 || colidx[k] >= n_nodes)
+        return fail(MPRGP_EBADINPUT, "column index %d out of range in row %d", colidx[k], first_bad);
+    return fail(MPRGP_EBADINPUT, "row %d has no diagonal block", first_bad);
+  }
+  if (bad_ptr < n_nodes) return fail(MPRGP_EBADINPUT, "rowptr not monotone at row %d", bad_ptr);
+  if (blocks) {
+    long long bad_v[kHostThreads];
+    std::fill(bad_v, bad_v + kHostThreads, LLONG_MAX);
+    host_parallel(n_nodes, 4096, [&](int t, int i0, int i1) {
+      for (long long k = 9LL * rowptr[i0]; k < 9LL * rowptr[i1]; ++k)
+        if (!std::isfinite(blocks[k])) {
+          bad_v[t] = k;
+          return;
+        }
+    });
+    const long long k = *std::min_element(bad_v, bad_v + kHostThreads);
+    if (k != LLONG_MAX) return fail(MPRGP_EBADINPUT, "non-finite matrix value at %lld", k);
+  }
   mprgp_matrix *A = new mprgp_matrix();
   A->ctx = ctx;
   A->n_nodes = n_nodes;
@@ -223,31 +306,47 @@ int mprgp::build_matrix(mprgp_ctx *ctx, int n_nodes, const int *rowptr, const in
   }
   A->slots = slots;
   // padding slots point at a valid node (the row itself, or the last node for
-  // lanes past the end) with zero values, so the SpMV needs no branch
-  std::vector<int> col(slots);
-  for (int s = 0; s < A->n_slices; ++s)
-    for (long long k = A->sl_off_h[s]; k < A->sl_off_h[s + 1]; ++k) {
-      int i = s * kSlice + (int)((k - A->sl_off_h[s]) % kSlice);
-      col[k] = i < n_nodes ? i : n_nodes - 1;
-    }
-  std::vector<double> val((size_t)slots * 9, 0.0);
-  A->blk_slot.resize(nb);
-  for (int s = 0; s < A->n_slices; ++s) {
-    const long long b0 = A->sl_off_h[s];
-    for (int l = 0; l < kSlice; ++l) {
-      int i = s * kSlice + l;
-      if (i >= n_nodes) continue;
-      int o = A->perm[i];
-      for (int k = 0; k < len[o]; ++k) {
-        long long src = rowptr[o] + k;
-        long long slot = b0 + (long long)k * kSlice + l;
-        col[slot] = A->iperm[colidx[src]];
-        A->blk_slot[src] = slot;
-        long long vbase = b0 * 9 + (long long)k * 9 * kSlice + l;
-        if (blocks)
-          for (int e = 0; e < 9; ++e) val[vbase + (long long)e * kSlice] = blocks[9 * src + e];
+  // lanes past the end) with zero values, so the SpMV needs no branch; the
+  // slices fill in parallel (each writes its own slots)
+  std::unique_ptr<int[]> col;
+  std::unique_ptr<double[]> val;
+  try {
+    col.reset(new int[slots]);
+    val.reset(new double[(size_t)slots * 9]);
+    A->blk_slot.resize(nb);
+  } catch (...) {
+    delete A;
+    throw;
+  }
+  auto fill = [&](int, int s0, int s1) {
+    for (int s = s0; s < s1; ++s) {
+      const long long b0 = A->sl_off_h[s], b1 = A->sl_off_h[s + 1];
+      std::memset(val.get() + b0 * 9, 0, sizeof(double) * 9 * (size_t)(b1 - b0));
+      for (long long k = b0; k < b1; ++k) {
+        const int i = s * kSlice + (int)((k - b0) % kSlice);
+        col[k] = i < n_nodes ? i : n_nodes - 1;
+      }
+      for (int l = 0; l < kSlice; ++l) {
+        const int i = s * kSlice + l;
+        if (i >= n_nodes) continue;
+        const int o = A->perm[i];
+        for (int k = 0; k < len[o]; ++k) {
+          const long long src = rowptr[o] + k;
+          const long long slot = b0 + (long long)k * kSlice + l;
+          col[slot] = A->iperm[colidx[src]];
+          A->blk_slot[src] = slot;
+          const long long vbase = b0 * 9 + (long long)k * 9 * kSlice + l;
+          if (blocks)
+            for (int e = 0; e < 9; ++e) val[vbase + (long long)e * kSlice] = blocks[9 * src + e];
+        }
       }
     }
+  };
+  try {
+    host_parallel(A->n_slices, 64, fill);
+  } catch (...) {
+    delete A;
+    throw;
   }
   size_t b_off = sizeof(int) * (A->n_slices + 1), b_col = sizeof(int) * slots,
          b_val = sizeof(double) * slots * 9, b_d = sizeof(double) * 3 * n_nodes, b_p = sizeof(int) * n_nodes;
@@ -267,9 +366,9 @@ int mprgp::build_matrix(mprgp_ctx *ctx, int n_nodes, const int *rowptr, const in
   }
   hipStream_t s = ctx->stream;
   hipError_t e = hipMemcpyAsync(A->d_sl_off, A->sl_off_h.data(), b_off, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(A->d_col, col.data(), b_col, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(A->d_col, col.get(), b_col, hipMemcpyHostToDevice, s);
   if (e == hipSuccess && c16) e = launch_to_u16(s, A->d_col, A->d_col16, (size_t)slots);
-  if (e == hipSuccess) e = hipMemcpyAsync(A->d_val, val.data(), b_val, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(A->d_val, val.get(), b_val, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipMemcpyAsync(A->d_perm, A->perm.data(), b_p, hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = launch_setup_diag(s, A->d_sl_off, A->d_col, A->d_val, A->d_dinv, n_nodes);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -297,37 +396,72 @@ extern "C" int mprgp_matrix_create_csr(mprgp_ctx *ctx, int n, const int *ptr, co
   if (!ctx || !ptr || !idx || !val || !out) return fail(MPRGP_EBADINPUT, "NULL argument");
   if (n <= 0 || n % 3) return fail(MPRGP_EBADINPUT, "n = %d is not a positive multiple of 3", n);
   const int N = n / 3;
+  // Checked in the order a row-by-row pass would meet the faults: the first
+  // scalar row whose ptr decreases, and before it the first column index out
+  // of range.
+  int bad_row = n;
+  for (int row = 0; row < n; ++row)
+    if (ptr[row + 1] < ptr[row]) {
+      bad_row = row;
+      break;
+    }
+  long long bad_k[kHostThreads];
+  std::fill(bad_k, bad_k + kHostThreads, LLONG_MAX);
+  host_parallel(bad_row, 8192, [&](int t, int r0, int r1) {
+    for (int row = r0; row < r1; ++row)
+      for (int k = ptr[row]; k < ptr[row + 1]; ++k)
+        if (idx[k] < 0 || idx[k] >= n) {
+          bad_k[t] = k;
+          return;
+        }
+  });
+  const long long first_bad = *std::min_element(bad_k, bad_k + kHostThreads);
+  if (first_bad != LLONG_MAX) return fail(MPRGP_EBADINPUT, "index %d out of range", idx[first_bad]);
+  if (bad_row < n) return fail(MPRGP_EBADINPUT, "ptr not monotone at %d", bad_row);
+  // node-row blocks: the distinct block columns of the 3 scalar rows plus the
+  // (possibly zero) diagonal block, ascending; scalars summed into them in
+  // CSR order.  Counted first, then filled, node rows in parallel.
   std::vector<int> rowptr(N + 1, 0);
-  std::vector<int> colidx;
-  std::vector<double> blocks;
-  std::vector<int> mark(N, -1), slot(N, -1);
-  for (int a = 0; a < N; ++a) {
-    std::vector<int> cols;
-    for (int r = 0; r < 3; ++r) {
-      int row = 3 * a + r;
-      if (ptr[row + 1] < ptr[row]) return fail(MPRGP_EBADINPUT, "ptr not monotone at %d", row);
-      for (int k = ptr[row]; k < ptr[row + 1]; ++k) {
-        if (idx[k] < 0 || idx[k] >= n) return fail(MPRGP_EBADINPUT, "index %d out of range", idx[k]);
-        int bcol = idx[k] / 3;
-        if (mark[bcol] != a) {
-          mark[bcol] = a;
-          cols.push_back(bcol);
+  host_parallel(N, 2048, [&](int, int a0, int a1) {
+    std::vector<int> mark(N, -1);
+    for (int a = a0; a < a1; ++a) {
+      int c = 0;
+      for (int k = ptr[3 * a]; k < ptr[3 * a + 3]; ++k) {
+        const int b = idx[k] / 3;
+        if (mark[b] != a) {
+          mark[b] = a;
+          ++c;
         }
       }
+      rowptr[a + 1] = c + (mark[a] != a ? 1 : 0);
     }
-    if (mark[a] != a) cols.push_back(a);  // keep a (zero) diagonal block
-    std::sort(cols.begin(), cols.end());
-    size_t base = colidx.size();
-    for (size_t k = 0; k < cols.size(); ++k) slot[cols[k]] = (int)(base + k);
-    colidx.insert(colidx.end(), cols.begin(), cols.end());
-    blocks.resize(colidx.size() * 9, 0.0);
-    for (int r = 0; r < 3; ++r) {
-      int row = 3 * a + r;
-      for (int k = ptr[row]; k < ptr[row + 1]; ++k)
-        blocks[(size_t)slot[idx[k] / 3] * 9 + r * 3 + idx[k] % 3] += val[k];
+  });
+  for (int a = 0; a < N; ++a) rowptr[a + 1] += rowptr[a];
+  std::vector<int> colidx(rowptr[N]);
+  std::vector<double> blocks((size_t)rowptr[N] * 9, 0.0);
+  host_parallel(N, 2048, [&](int, int a0, int a1) {
+    std::vector<int> mark(N, -1), slot(N, -1), cols;
+    for (int a = a0; a < a1; ++a) {
+      cols.clear();
+      for (int k = ptr[3 * a]; k < ptr[3 * a + 3]; ++k) {
+        const int b = idx[k] / 3;
+        if (mark[b] != a) {
+          mark[b] = a;
+          cols.push_back(b);
+        }
+      }
+      if (mark[a] != a) cols.push_back(a);  // keep a (zero) diagonal block
+      std::sort(cols.begin(), cols.end());
+      const int base = rowptr[a];
+      for (size_t k = 0; k < cols.size(); ++k) {
+        slot[cols[k]] = base + (int)k;
+        colidx[base + k] = cols[k];
+      }
+      for (int r = 0; r < 3; ++r)
+        for (int k = ptr[3 * a + r]; k < ptr[3 * a + r + 1]; ++k)
+          blocks[(size_t)slot[idx[k] / 3] * 9 + r * 3 + idx[k] % 3] += val[k];
     }
-    rowptr[a + 1] = (int)colidx.size();
-  }
+  });
   MPRGP_HIP(hipSetDevice(ctx->device));
   std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
   return build_matrix(ctx, N, rowptr.data(), colidx.data(), blocks.data(), out);

@@ -404,3 +404,76 @@ def test_copy_probe_rate_is_plausible(ctx):
     gbps = ctx.copy_probe(1 << 28, 4)
     print(f"copy probe {gbps:.0f} GB/s")
     assert 1600.0 < gbps < 8000.0
+
+
+def _create_csr(ctx, n, ptr, idx, val):
+    import ctypes as C
+    ptr, idx, val = M._i32(ptr), M._i32(idx), M._f64(val)
+    h = C.c_void_p()
+    rc = M._lib.load().mprgp_matrix_create_csr(ctx.h, int(n), M._p(ptr), M._p(idx), M._p(val), C.byref(h))
+    if rc == 0:
+        M._lib.load().mprgp_matrix_destroy(h)
+    return rc, M.last_error()
+
+
+def _create_bsr3(ctx, n_nodes, rp, ci, bl):
+    import ctypes as C
+    rp, ci, bl = M._i32(rp), M._i32(ci), M._f64(bl)
+    h = C.c_void_p()
+    rc = M._lib.load().mprgp_matrix_create_bsr3(ctx.h, int(n_nodes), M._p(rp), M._p(ci), M._p(bl), C.byref(h))
+    if rc == 0:
+        M._lib.load().mprgp_matrix_destroy(h)
+    return rc, M.last_error()
+
+
+def test_malformed_matrices_are_rejected(ctx):
+    """Matrix uploads check their arrays on several host threads and report
+    the fault a row-by-row pass meets first: on config 2's 133,623-row matrix
+    (16 ranges), a bad index late in the rows, or two faults in different
+    ranges, give the message of the earlier one."""
+    sc = problems.config2()
+    A = scene_qp(sc)[0].tocsr()
+    A.sort_indices()
+    n = A.shape[0]
+    ptr, idx, val = A.indptr.copy(), A.indices.copy(), A.data.copy()
+    assert _create_csr(ctx, n, ptr, idx, val)[0] == 0
+    late, early = ptr[120000] + 1, ptr[30000] + 2
+    bad = idx.copy()
+    bad[late] = n + 7
+    rc, msg = _create_csr(ctx, n, ptr, bad, val)
+    assert rc == M._lib.EBADINPUT and f"index {n + 7} out of range" in msg
+    bad[early] = -3
+    rc, msg = _create_csr(ctx, n, ptr, bad, val)
+    assert rc == M._lib.EBADINPUT and "index -3 out of range" in msg
+    # a decreasing ptr after an out-of-range index: the index; before it: the ptr
+    p2 = ptr.copy()
+    p2[90001] = p2[90000] - 1
+    rc, msg = _create_csr(ctx, n, p2, bad, val)
+    assert rc == M._lib.EBADINPUT and "index -3 out of range" in msg
+    rc, msg = _create_csr(ctx, n, p2, idx, val)
+    assert rc == M._lib.EBADINPUT and "ptr not monotone at 90000" in msg
+    # block CSR: rows without their diagonal block, columns out of range,
+    # non-finite values, a decreasing rowptr
+    B = A.tobsr(blocksize=(3, 3))
+    rp, ci, bl = B.indptr.copy(), B.indices.copy(), B.data.reshape(-1).copy()
+    N = sc.n_nodes
+    assert _create_bsr3(ctx, N, rp, ci, bl)[0] == 0
+    c2 = ci.copy()
+    row = 40000
+    d = rp[row] + int(np.where(ci[rp[row]:rp[row + 1]] == row)[0][0])
+    c2[d] = row + 1 if row + 1 not in ci[rp[row]:rp[row + 1]] else row - 1
+    c3 = c2.copy()
+    c3[rp[10000]] = N
+    rc, msg = _create_bsr3(ctx, N, rp, c2, bl)
+    assert rc == M._lib.EBADINPUT and f"row {row} has no diagonal block" in msg
+    rc, msg = _create_bsr3(ctx, N, rp, c3, bl)
+    assert rc == M._lib.EBADINPUT and f"column index {N} out of range in row 10000" in msg
+    b2 = bl.copy()
+    b2[9 * rp[35000] + 4] = np.nan
+    b2[9 * rp[5000] + 1] = np.inf
+    rc, msg = _create_bsr3(ctx, N, rp, ci, b2)
+    assert rc == M._lib.EBADINPUT and f"non-finite matrix value at {9 * rp[5000] + 1}" in msg
+    r2 = rp.copy()
+    r2[20001] = r2[20000] - 1
+    rc, msg = _create_bsr3(ctx, N, r2, ci, bl)
+    assert rc == M._lib.EBADINPUT and "rowptr not monotone at row 20000" in msg
