@@ -122,63 +122,66 @@ __global__ void __launch_bounds__(256) ica_rows(const int *sl_off, const int *co
   }
 }
 
+// The lower blocks' values in sweep order, 9 per block (row-major), gathered
+// once per call from their SELL slots: a slot's 9 values, for slice base b0
+// and block k with slot = b0 + 64 k + l, sit at 9 (b0 + 64 k) + l + 64 e.
+__global__ void __launch_bounds__(256) ica_lower_vals(const double *val, const int *lo_slot, int n_lo,
+                                                      double *lo_val) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_lo) return;
+  const long long slot = lo_slot[e];
+  const double *v = val + (9 * slot - 8 * (slot % kSlice));
+#pragma unroll
+  for (int k = 0; k < 9; ++k) lo_val[9 * (size_t)e + k] = v[k * kSlice];
+}
+
 // The forward block Gauss-Seidel sweep (ICASolver.h:394-404) in one
-// workgroup, level by level.  A level's lower blocks (each node's blocks on
-// nodes before it, already updated in earlier levels) are listed in sweep
-// order: lo_ptr[t] .. lo_ptr[t + 1] for the level-ordered node t, with each
-// block's SELL slot and column.  Each level runs in two phases:
-//   A. every lower block of the level, one per thread: its nine products
-//      v_ab * x_b, each rounded once, into prod;
-//   B. every node of the level, one per thread: rhs minus its blocks'
-//      products in block order, times Da^-1.
-// Each node's rows thus see the same rounded products subtracted in the same
-// order as the sequential sweep (r_a -= v_ab x_b over blocks, b = 0, 1, 2),
-// while the loads of a level are all issued at once instead of one block at
-// a time.  The barriers order phase A's products and phase B's x writes
-// before their readers; all waves of the workgroup share the CU's caches.
-__global__ void __launch_bounds__(1024) ica_sweep(const double *val, const int *lo_slot, const int *lo_col,
-                                                  const int *lo_ptr, const double *dinv, const double *rhs,
-                                                  double *x, double *prod, const int *lvl_ptr,
-                                                  const int *lvl_node, int n_levels) {
+// workgroup, level by level, one thread per node of a level.  A level's
+// nodes depend only on nodes of earlier levels, whose x the barrier after
+// each level publishes.  A node's lower blocks (its blocks on nodes before
+// it) are listed in sweep order, lo_ptr[t] .. lo_ptr[t + 1] for the
+// level-ordered node t, with their values copied in that order (lo_val) and
+// their columns; the thread loads them kChunk at a time, every load of a
+// chunk issued before the chunk's products, and subtracts v_ab x_b (each
+// product rounded once: no contraction in this file) from rhs in block
+// order, b = 0, 1, 2, as the sequential sweep does; then x = Da^-1 r.
+constexpr int kChunk = 6;
+__global__ void __launch_bounds__(512) ica_sweep(const double *lo_val, const int *lo_col, const int *lo_ptr,
+                                                 const double *dinv, const double *rhs, double *x,
+                                                 const int *lvl_ptr, const int *lvl_node, int n_levels) {
   for (int lv = 0; lv < n_levels; ++lv) {
-    const int t0 = lvl_ptr[lv], t1 = lvl_ptr[lv + 1];
-    const int e1 = lo_ptr[t1];
-    for (int e = lo_ptr[t0] + (int)threadIdx.x; e < e1; e += (int)blockDim.x) {
-      const long long slot = lo_slot[e];
-      // a slot's 9 values: slice base b0 and block k give slot = b0 + 64 k + l
-      // and values at 9 (b0 + 64 k) + l + 64 e
-      const double *v = val + (9 * slot - 8 * (slot % kSlice));
-      const int c = lo_col[e];
-      const double x0 = x[3 * (size_t)c], x1 = x[3 * (size_t)c + 1], x2 = x[3 * (size_t)c + 2];
-      double *pr = prod + 9 * (size_t)e;
-      pr[0] = v[0 * kSlice] * x0;
-      pr[1] = v[1 * kSlice] * x1;
-      pr[2] = v[2 * kSlice] * x2;
-      pr[3] = v[3 * kSlice] * x0;
-      pr[4] = v[4 * kSlice] * x1;
-      pr[5] = v[5 * kSlice] * x2;
-      pr[6] = v[6 * kSlice] * x0;
-      pr[7] = v[7 * kSlice] * x1;
-      pr[8] = v[8 * kSlice] * x2;
-    }
-    __syncthreads();
-    for (int t = t0 + (int)threadIdx.x; t < t1; t += (int)blockDim.x) {
-      const int i = lvl_node[t];
+    const int t1 = lvl_ptr[lv + 1];
+    for (int t = lvl_ptr[lv] + (int)threadIdx.x; t < t1; t += (int)blockDim.x) {
+      const int i = lvl_node[t], k0 = lo_ptr[t], k1 = lo_ptr[t + 1];
       const size_t r = 3 * (size_t)i;
       double r0 = rhs[r], r1 = rhs[r + 1], r2 = rhs[r + 2];
-      for (int e = lo_ptr[t]; e < lo_ptr[t + 1]; ++e) {
-        const double *pr = prod + 9 * (size_t)e;
-        r0 -= pr[0];
-        r0 -= pr[1];
-        r0 -= pr[2];
-        r1 -= pr[3];
-        r1 -= pr[4];
-        r1 -= pr[5];
-        r2 -= pr[6];
-        r2 -= pr[7];
-        r2 -= pr[8];
-      }
       const double *D = dinv + 9 * (size_t)i;
+      for (int e0 = k0; e0 < k1; e0 += kChunk) {
+        double v[kChunk][9], xb[kChunk][3];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+          const int e = min(e0 + u, k1 - 1);  // past the node's last block: its last again, not used
+          const int c = lo_col[e];
+#pragma unroll
+          for (int k = 0; k < 9; ++k) v[u][k] = lo_val[9 * (size_t)e + k];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) xb[u][k] = x[3 * (size_t)c + k];
+        }
+        asm volatile("" ::: "memory");  // the chunk's loads stay ahead of its products
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+          if (e0 + u >= k1) break;
+          r0 -= v[u][0] * xb[u][0];
+          r0 -= v[u][1] * xb[u][1];
+          r0 -= v[u][2] * xb[u][2];
+          r1 -= v[u][3] * xb[u][0];
+          r1 -= v[u][4] * xb[u][1];
+          r1 -= v[u][5] * xb[u][2];
+          r2 -= v[u][6] * xb[u][0];
+          r2 -= v[u][7] * xb[u][1];
+          r2 -= v[u][8] * xb[u][2];
+        }
+      }
       x[r] = D[0] * r0 + D[1] * r1 + D[2] * r2;
       x[r + 1] = D[3] * r0 + D[4] * r1 + D[5] * r2;
       x[r + 2] = D[6] * r0 + D[7] * r1 + D[8] * r2;
@@ -187,7 +190,7 @@ __global__ void __launch_bounds__(1024) ica_sweep(const double *val, const int *
   }
 }
 
-constexpr int kSweepThreads = 1024;
+constexpr int kSweepThreads = 512;
 
 // ---------------------------------------------------------------- host side
 
@@ -396,7 +399,7 @@ extern "C" int mprgp_ica_solve(mprgp_qp *q, int m, const int *j_ptr, const int *
   double *d_res = (double *)carve(sizeof(double) * 8);
   int *d_lvl_ptr = (int *)carve(sizeof(int) * (n_levels + 1));
   int *d_lvl_node = (int *)carve(sizeof(int) * (size_t)nn);
-  double *d_prod = (double *)carve(sizeof(double) * 9 * n_lo);
+  double *d_lo_val = (double *)carve(sizeof(double) * 9 * n_lo);
   int *d_lo_ptr = (int *)carve(sizeof(int) * ((size_t)nn + 1));
   int *d_lo_slot = (int *)carve(sizeof(int) * n_lo);
   int *d_lo_col = (int *)carve(sizeof(int) * n_lo);
@@ -422,6 +425,10 @@ extern "C" int mprgp_ica_solve(mprgp_qp *q, int m, const int *j_ptr, const int *
   const int blocks = (nn + 255) / 256;
   if (nn > 0) {
     ica_dinv<<<blocks, 256, 0, s>>>(A->d_sl_off, A->d_col, A->d_val, nn, d_dinv);
+    MPRGP_HIP(hipGetLastError());
+  }
+  if (n_lo > 0) {
+    ica_lower_vals<<<(unsigned)((n_lo + 255) / 256), 256, 0, s>>>(A->d_val, d_lo_slot, (int)n_lo, d_lo_val);
     MPRGP_HIP(hipGetLastError());
   }
   // B = J Da^-1 J^T on the host, from the inverse blocks in user order
@@ -465,8 +472,8 @@ extern "C" int mprgp_ica_solve(mprgp_qp *q, int m, const int *j_ptr, const int *
       ica_rows<<<blocks, 256, 0, s>>>(A->d_sl_off, A->d_col, A->d_val, A->d_perm, x_c, q->b, jtl, d_dinv, rh, nn,
                                       kUpper);
       MPRGP_HIP(hipGetLastError());
-      ica_sweep<<<1, kSweepThreads, 0, s>>>(A->d_val, d_lo_slot, d_lo_col, d_lo_ptr, d_dinv, rh, q->x, d_prod,
-                                            d_lvl_ptr, d_lvl_node, n_levels);
+      ica_sweep<<<1, kSweepThreads, 0, s>>>(d_lo_val, d_lo_col, d_lo_ptr, d_dinv, rh, q->x, d_lvl_ptr,
+                                            d_lvl_node, n_levels);
       MPRGP_HIP(hipGetLastError());
       ica_rows<<<blocks, 256, 0, s>>>(A->d_sl_off, A->d_col, A->d_val, A->d_perm, x_c, q->b, jtl, d_dinv, e, nn,
                                       kResidual);
