@@ -167,8 +167,8 @@ def measure_traffic(args, tiles, what="solve"):
 
 
 def host_info():
-    """The host's CPU model and core counts (BASELINE's CPU-baseline plan asks
-    for them in every results file): /proc/cpuinfo, os, lscpu."""
+    """The host's CPU model and core counts, reported with the CPU baseline:
+    /proc/cpuinfo, os, lscpu."""
     info = {"cpu_model": None, "host_logical_cpus": os.cpu_count(),
             "usable_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
             "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
