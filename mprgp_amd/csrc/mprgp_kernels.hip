@@ -678,9 +678,14 @@ __device__ __forceinline__ void row_work(const QpDev &q, const Vecs<T> &vin, int
     case OP_EXP3:
     case OP_CL_EXP2: {
       // g = A x - b ; DECIDE_FACE(x) ; phi ; z ; p = z   (MPRGPSolver.h:148-156, 263-268, 210-215)
-      V3 acc[1];
+      // The mixed form's INIT starts a correction QP from d = 0, which
+      // R_START wrote: A d is then +0 in every entry (finite values times
+      // zero, summed from +0), so the SpMV is skipped with the same bits.
+      V3 acc[1] = {V3{0.0, 0.0, 0.0}};
       const double *x = xc;
-      spmv_row<1, UB, NT>(q.sl_off, cols<C16>(q), v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); }, acc);
+      if (!(mixed && op == OP_INIT))
+        spmv_row<1, UB, NT>(q.sl_off, cols<C16>(q), v.val, s, lane, [&](int j, V3 (&xs)[1]) { xs[0] = ld3(x, j); },
+                            acc);
       V3 g = sub(acc[0], ld3(v.b, r));
       const V3 xr = ld3(xc, r);  // in flight with the row flag
       Row R = ldrow(q, v, r);

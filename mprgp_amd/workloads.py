@@ -111,6 +111,8 @@ def op_bytes(op, N, nb, m, mixed=False, precond=True, idx=4):
         elif st == "face":
             total += N
         elif st in ("M", "M64"):
+            if mixed and name == "INIT":
+                continue  # the correction QP starts from d = 0: no product with A
             s = 4 if (mixed and st == "M") else 8
             total += nb * (9 * s + idx) + 4 * (N + 1)
         elif st == "dinv" and not precond:

@@ -42,6 +42,8 @@ def test_byte_accounting():
     # no preconditioner: no D^-1 stream
     assert (workloads.op_bytes(op("CG_A"), N, nb, m) - workloads.op_bytes(op("CG_A"), N, nb, m, precond=False)
             == v64)
+    # the mixed form's INIT starts from d = 0 and reads no matrix
+    assert workloads.op_bytes(op("INIT"), N, nb, m) - workloads.op_bytes(op("INIT"), N, nb, m, True) > mat32
     # 16-bit column indices: 2 bytes less per block
     assert workloads.op_bytes(op("CG_AP"), N, nb, m, True, idx=2) == mat32 - 2 * nb + 4 * v32
     assert workloads.index_bytes(65536) == 2 and workloads.index_bytes(65537) == 4
