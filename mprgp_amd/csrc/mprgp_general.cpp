@@ -32,11 +32,8 @@ namespace {
 
 constexpr int kMaxGeneralRows = 2048;  // J J^T is kept dense on the host
 
-// lam = argmin 1/2 l'Gl - h'l subject to l >= 0, G (k x k, row-major) SPD.
-// Primal active set on the free set F: free the bound variable with the
-// largest positive residual (h - G l)_i, solve G_FF s = h_F by Cholesky,
-// and while some s_i <= 0 step towards s up to the first zero and bound it.
-// Finite for SPD G; false if a free set's G_FF is not positive definite.
+// Dense Cholesky of a k x k row-major SPD block in place (lower factor);
+// false if a pivot is not positive.
 bool chol_factor(std::vector<double> &L, int k) {
   for (int j = 0; j < k; ++j) {
     double d = L[(size_t)j * k + j];
@@ -66,15 +63,26 @@ void chol_solve(const std::vector<double> &L, int k, double *b) {
   }
 }
 
-bool qp_nonneg(int k, const std::vector<double> &G, const std::vector<double> &h, std::vector<double> &lam) {
+// lam = argmin 1/2 l'Gl - h'l subject to l >= 0, G (k x k, row-major) SPD.
+// Primal active set on the free set F: free the bound variable with the
+// largest positive residual (h - G l)_i, solve G_FF s = h_F by Cholesky,
+// and while some s_i <= 0 step towards s up to the first zero and bound it.
+// Finite for SPD G; false if a free set's G_FF is not positive definite.
+// tol: the residual a bound variable needs to be freed (qp_tol below).
+// The freeing threshold: 1e-14 max |h| (1e-14 if h = 0).
+double qp_tol(const double *h, int k) {
+  double hs = 0.0;
+  for (int i = 0; i < k; ++i) hs = std::max(hs, std::fabs(h[i]));
+  return 1e-14 * (hs > 0.0 ? hs : 1.0);
+}
+
+bool qp_nonneg(int k, const std::vector<double> &G, const std::vector<double> &h, std::vector<double> &lam,
+               double tol) {
   lam.assign(k, 0.0);
   if (k == 0) return true;
   std::vector<char> free_(k, 0);
   std::vector<int> idx;
   std::vector<double> L, s;
-  double hs = 0.0;
-  for (double v : h) hs = std::max(hs, std::fabs(v));
-  const double tol = 1e-14 * (hs > 0.0 ? hs : 1.0);
   for (int outer = 0; outer < 10 * k + 10; ++outer) {
     int best = -1;
     double wbest = tol;
@@ -149,9 +157,52 @@ struct General {
   std::vector<double> c, JJt, hm, hres;
   std::vector<char> face;
   std::vector<int> act;
-  std::vector<double> La;  // Cholesky factor of J_a J_a^T
+  // J J^T is block diagonal over the connected components of its pattern
+  // (rows that share a node): the projection's QP and the active rows'
+  // factor split into one small problem per component.  comps: each
+  // component's rows, ascending; ablk/aL: the active rows of each component
+  // with nonempty ones, and the Cholesky factor of their J_a J_a^T block.
+  std::vector<std::vector<int>> comps, ablk;
+  std::vector<std::vector<double>> aL;
   bool qp_fail = false;
   int y_size = 0;
+
+  void build_components() {
+    std::vector<int> parent(m);
+    for (int i = 0; i < m; ++i) parent[i] = i;
+    auto find = [&](int i) {
+      while (parent[i] != i) i = parent[i] = parent[parent[i]];
+      return i;
+    };
+    for (int i = 0; i < m; ++i)
+      for (int j = i + 1; j < m; ++j)
+        if (JJt[(size_t)i * m + j] != 0.0 || JJt[(size_t)j * m + i] != 0.0) {
+          const int a = find(i), b2 = find(j);
+          if (a != b2) parent[std::max(a, b2)] = std::min(a, b2);
+        }
+    std::vector<int> slot(m, -1);
+    comps.clear();
+    for (int i = 0; i < m; ++i) {
+      const int r = find(i);
+      if (slot[r] < 0) {
+        slot[r] = (int)comps.size();
+        comps.emplace_back();
+      }
+      comps[slot[r]].push_back(i);
+    }
+  }
+  // l[rows] = argmin 1/2 l'Gl - h'l, l >= 0, over J J^T's block of `rows`
+  // (h indexed by row), freeing threshold tol
+  void qp_rows(const std::vector<int> &rows, const double *h, double tol, std::vector<double> &l) {
+    const int k = (int)rows.size();
+    std::vector<double> Gk((size_t)k * k), hk(k), lk;
+    for (int a = 0; a < k; ++a) {
+      hk[a] = h[rows[a]];
+      for (int bb = 0; bb < k; ++bb) Gk[(size_t)a * k + bb] = JJt[(size_t)rows[a] * m + rows[bb]];
+    }
+    if (!qp_nonneg(k, Gk, hk, lk, tol)) qp_fail = true;
+    for (int a = 0; a < k; ++a) l[rows[a]] = lk.empty() ? 0.0 : lk[a];
+  }
 
   ~General() {
     if (blk) ctx->release(blk, blk_bytes);
@@ -204,9 +255,10 @@ struct General {
       return MPRGP_OK;
     }
     if (int rc = jmul(v)) return rc;
-    std::vector<double> h(m), l;
+    std::vector<double> h(m), l(m, 0.0);
     for (int i = 0; i < m; ++i) h[i] = c[i] - hm[i];
-    if (!qp_nonneg(m, JJt, h, l)) qp_fail = true;
+    const double tol = qp_tol(h.data(), m);
+    for (const auto &rows : comps) qp_rows(rows, h.data(), tol, l);
     if (int rc = jtmul(l, tl)) return rc;
     MPRGP_HIP(launch_axpby(s, n, 1.0, v, 1.0, tl, out));
     return MPRGP_OK;
@@ -214,17 +266,27 @@ struct General {
   // DECIDE_FACE + prepareActiveConMatrix (:485-520)
   int decide_face(const double *v) {
     act.clear();
+    ablk.clear();
+    aL.clear();
     if (m == 0) return MPRGP_OK;
     if (int rc = jmul(v)) return rc;
     for (int i = 0; i < m; ++i) {
       face[i] = std::fabs(hm[i] - c[i]) < kEps ? 1 : 0;
       if (face[i]) act.push_back(i);
     }
-    const int k = (int)act.size();
-    La.assign((size_t)k * k, 0.0);
-    for (int a = 0; a < k; ++a)
-      for (int bb = 0; bb < k; ++bb) La[(size_t)a * k + bb] = JJt[(size_t)act[a] * m + act[bb]];
-    if (!chol_factor(La, k)) qp_fail = true;
+    for (const auto &rows : comps) {
+      std::vector<int> a;
+      for (int i : rows)
+        if (face[i]) a.push_back(i);
+      if (a.empty()) continue;
+      const int k = (int)a.size();
+      std::vector<double> L((size_t)k * k);
+      for (int x0 = 0; x0 < k; ++x0)
+        for (int x1 = 0; x1 < k; ++x1) L[(size_t)x0 * k + x1] = JJt[(size_t)a[x0] * m + a[x1]];
+      if (!chol_factor(L, k)) qp_fail = true;
+      ablk.push_back(std::move(a));
+      aL.push_back(std::move(L));
+    }
     return MPRGP_OK;
   }
   int feasible(const double *v, bool *ok) {  // isFeasible (:298-310)
@@ -242,11 +304,15 @@ struct General {
       return MPRGP_OK;
     }
     if (int rc = jmul(g)) return rc;
-    const int k = (int)act.size();
-    std::vector<double> la(k), l(m, 0.0);
-    for (int a = 0; a < k; ++a) la[a] = -hm[act[a]];
-    chol_solve(La, k, la.data());
-    for (int a = 0; a < k; ++a) l[act[a]] = la[a];
+    std::vector<double> l(m, 0.0);
+    for (size_t blk_i = 0; blk_i < ablk.size(); ++blk_i) {
+      const std::vector<int> &a = ablk[blk_i];
+      const int k = (int)a.size();
+      std::vector<double> la(k);
+      for (int x0 = 0; x0 < k; ++x0) la[x0] = -hm[a[x0]];
+      chol_solve(aL[blk_i], k, la.data());
+      for (int x0 = 0; x0 < k; ++x0) l[a[x0]] = la[x0];
+    }
     if (int rc = jtmul(l, tl)) return rc;
     MPRGP_HIP(launch_axpby(s, n, 1.0, g, 1.0, tl, phi));
     return MPRGP_OK;
@@ -256,14 +322,11 @@ struct General {
     MPRGP_HIP(launch_axpby(s, n, 1.0, g, -1.0, phi, beta));
     if (act.empty()) return MPRGP_OK;
     if (int rc = jmul(beta)) return rc;
-    const int k = (int)act.size();
-    std::vector<double> G((size_t)k * k), h(k), la, l(m, 0.0);
-    for (int a = 0; a < k; ++a) {
-      h[a] = hm[act[a]];
-      for (int bb = 0; bb < k; ++bb) G[(size_t)a * k + bb] = JJt[(size_t)act[a] * m + act[bb]];
-    }
-    if (!qp_nonneg(k, G, h, la)) qp_fail = true;
-    for (int a = 0; a < k; ++a) l[act[a]] = la[a];
+    // the threshold from every active row's h, as one QP over them would
+    std::vector<double> ha(act.size()), l(m, 0.0);
+    for (size_t a = 0; a < act.size(); ++a) ha[a] = hm[act[a]];
+    const double tol = qp_tol(ha.data(), (int)ha.size());
+    for (const auto &a : ablk) qp_rows(a, hm.data(), tol, l);
     if (int rc = jtmul(l, tl)) return rc;
     MPRGP_HIP(launch_axpby(s, n, 1.0, beta, -1.0, tl, beta));
     return MPRGP_OK;
@@ -376,6 +439,7 @@ extern "C" int mprgp_general_solve(mprgp_qp *q, int m, const int *j_ptr, const i
   }
   G.hm.assign(std::max(m, 1), 0.0);
   G.face.assign(m, 0);
+  G.build_components();
   // device block: J, J^T, J v, lambda, dot partials and results
   const size_t parts = ((size_t)n + 255) / 256 + 1;
   const size_t bytes = sizeof(int) * ((size_t)m + 1 + (size_t)nnz + (size_t)n + 1 + (size_t)nnz) +
