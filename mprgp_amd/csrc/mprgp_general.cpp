@@ -204,7 +204,13 @@ struct General {
     for (int a = 0; a < k; ++a) l[rows[a]] = lk.empty() ? 0.0 : lk[a];
   }
 
+  double *h_lam = nullptr;  // pinned staging of the multipliers J^T l multiplies
+
   ~General() {
+    if (h_lam) {
+      (void)hipStreamSynchronize(s);  // an upload from it may still be queued
+      (void)hipHostFree(h_lam);
+    }
     if (blk) ctx->release(blk, blk_bytes);
   }
 
@@ -215,10 +221,12 @@ struct General {
     MPRGP_HIP(hipStreamSynchronize(s));
     return MPRGP_OK;
   }
-  // out = J^T l for host l (the upload completes before l can go away)
+  // out = J^T l for host l, uploaded from pinned staging with no wait: every
+  // caller reads J v back (jmul, which waits for the stream) before it
+  // fills the staging again, so the previous upload has completed by then
   int jtmul(const std::vector<double> &l, double *out) {
-    MPRGP_HIP(hipMemcpyAsync(lam, l.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
-    MPRGP_HIP(hipStreamSynchronize(s));
+    std::copy(l.begin(), l.end(), h_lam);
+    MPRGP_HIP(hipMemcpyAsync(lam, h_lam, sizeof(double) * m, hipMemcpyHostToDevice, s));
     MPRGP_HIP(launch_csr_mul(s, n, tp, tc, tv, lam, out));
     return MPRGP_OK;
   }
@@ -446,6 +454,10 @@ extern "C" int mprgp_general_solve(mprgp_qp *q, int m, const int *j_ptr, const i
                        sizeof(double) * (2 * (size_t)nnz + 2 * (size_t)m + parts + 16) + 1024;
   if (int rc = ctx->alloc(&G.blk, bytes)) return rc;
   G.blk_bytes = bytes;
+  if (hipHostMalloc((void **)&G.h_lam, sizeof(double) * ((size_t)m + 1), hipHostMallocDefault) != hipSuccess) {
+    G.h_lam = nullptr;
+    return fail(MPRGP_ENOMEM, "pinned staging of %d multipliers", m);
+  }
   {
     char *b8 = (char *)G.blk;
     auto carve = [&](size_t sz) {
