@@ -148,13 +148,13 @@ struct General {
   void *blk = nullptr;
   size_t blk_bytes = 0;
   int *jp = nullptr, *jc = nullptr, *tp = nullptr, *tc = nullptr;
-  double *jv = nullptr, *tv = nullptr, *jx = nullptr, *lam = nullptr, *part = nullptr, *res = nullptr;
-  // tl holds J^T lambda inside project/compute_phi/compute_beta only, so a
-  // caller may pass any of the other vectors (t included) as their input
+  double *jv = nullptr, *tv = nullptr, *lam = nullptr, *part = nullptr, *res = nullptr;
+  // tl holds J^T lambda inside jt_apply only, so a caller may pass any of
+  // the other vectors (t included) as its input
   double *x, *g, *p, *z, *phi, *beta, *y, *gp, *t, *t2, *t3, *tl, *pv, *pw;
   const double *b = nullptr, *dinv = nullptr;
   // host
-  std::vector<double> c, JJt, hm, hres;
+  std::vector<double> c, JJt;
   std::vector<char> face;
   std::vector<int> act;
   // J J^T is block diagonal over the connected components of its pattern
@@ -204,7 +204,19 @@ struct General {
     for (int a = 0; a < k; ++a) l[rows[a]] = lk.empty() ? 0.0 : lk[a];
   }
 
-  double *h_lam = nullptr;  // pinned staging of the multipliers J^T l multiplies
+  // Pinned host memory: the multipliers' staging (two slots of m + 1) and
+  // the copy of the device readback area.  Every fill of the staging
+  // follows a readback that waited for the stream after the previous upload
+  // was queued, so no upload still reads the slots it overwrites.
+  double *h_lam = nullptr, *h_rb = nullptr;
+  // Device readback area: three J v slots of m + 1, then 16 dot results.
+  double *rb = nullptr;
+  double *jslot(int k) const { return rb + (size_t)k * (m + 1); }
+  const double *hj(int k) const { return h_rb + (size_t)k * (m + 1); }
+  const double *hd() const { return h_rb + 3 * ((size_t)m + 1); }
+  // host copies of J x and J y, valid while the vector is unchanged
+  std::vector<double> jx_h, jy_h;
+  bool jx_ok = false, jy_ok = false;
 
   ~General() {
     if (h_lam) {
@@ -214,28 +226,50 @@ struct General {
     if (blk) ctx->release(blk, blk_bytes);
   }
 
-  // J v to the host
-  int jmul(const double *v) {
-    MPRGP_HIP(launch_csr_mul(s, m, jp, jc, jv, v, jx));
-    MPRGP_HIP(hipMemcpyAsync(hm.data(), jx, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+  // queue J v into readback slot k
+  int jlaunch(const double *v, int k) {
+    MPRGP_HIP(launch_csr_mul(s, m, jp, jc, jv, v, jslot(k)));
+    return MPRGP_OK;
+  }
+  // queue a . b into dot result i
+  int dlaunch(const double *a, const double *b2, int i) {
+    MPRGP_HIP(launch_dot(s, a, b2, (size_t)n, part, res + i));
+    return MPRGP_OK;
+  }
+  // J v slots [0, slots) and dot results [0, k) to the host, one wait
+  int readback(int slots, int k) {
+    if (slots > 0)
+      MPRGP_HIP(hipMemcpyAsync(h_rb, rb, sizeof(double) * (size_t)slots * (m + 1), hipMemcpyDeviceToHost, s));
+    if (k > 0)
+      MPRGP_HIP(hipMemcpyAsync(h_rb + 3 * ((size_t)m + 1), res, sizeof(double) * k, hipMemcpyDeviceToHost, s));
     MPRGP_HIP(hipStreamSynchronize(s));
     return MPRGP_OK;
   }
-  // out = J^T l for host l, uploaded from pinned staging with no wait: every
-  // caller reads J v back (jmul, which waits for the stream) before it
-  // fills the staging again, so the previous upload has completed by then
-  int jtmul(const std::vector<double> &l, double *out) {
-    std::copy(l.begin(), l.end(), h_lam);
-    MPRGP_HIP(hipMemcpyAsync(lam, h_lam, sizeof(double) * m, hipMemcpyHostToDevice, s));
-    MPRGP_HIP(launch_csr_mul(s, n, tp, tc, tv, lam, out));
+  // J v to the host (slot 0)
+  int jmul(const double *v) {
+    if (int rc = jlaunch(v, 0)) return rc;
+    return readback(1, 0);
+  }
+  // multipliers l into staging slot k
+  void stage(int k, const std::vector<double> &l) { std::copy(l.begin(), l.end(), h_lam + (size_t)k * (m + 1)); }
+  // upload staging slots [0, k), no wait of its own
+  int upload(int k) {
+    MPRGP_HIP(hipMemcpyAsync(lam, h_lam, sizeof(double) * (size_t)k * (m + 1), hipMemcpyHostToDevice, s));
+    return MPRGP_OK;
+  }
+  // out = v + sign J^T l for the multipliers uploaded into slot k
+  int jt_apply(int k, const double *v, double sign, double *out) {
+    MPRGP_HIP(launch_csr_mul(s, n, tp, tc, tv, lam + (size_t)k * (m + 1), tl));
+    MPRGP_HIP(launch_axpby(s, n, 1.0, v, sign, tl, out));
     return MPRGP_OK;
   }
   // dot products of the pairs, all read back with one synchronisation
   int dots(std::initializer_list<std::pair<const double *, const double *>> pairs, double *out) {
     int i = 0;
-    for (const auto &pr : pairs) MPRGP_HIP(launch_dot(s, pr.first, pr.second, (size_t)n, part, res + i++));
-    MPRGP_HIP(hipMemcpyAsync(out, res, sizeof(double) * i, hipMemcpyDeviceToHost, s));
-    MPRGP_HIP(hipStreamSynchronize(s));
+    for (const auto &pr : pairs)
+      if (int rc = dlaunch(pr.first, pr.second, i++)) return rc;
+    if (int rc = readback(0, i)) return rc;
+    std::copy(hd(), hd() + i, out);
     return MPRGP_OK;
   }
   int spmv(const double *v, double *out) {
@@ -248,38 +282,61 @@ struct General {
     MPRGP_HIP(launch_axpby(s, n, 1.0, g, -1.0, b, g));
     return MPRGP_OK;
   }
-  int func(const double *v, double *f) {  // 1/2 v'Av - v'b (FixedSparseMatrix::funcValue)
+  // queue 1/2 v'Av and v'b (FixedSparseMatrix::funcValue) into dot results
+  // i, i + 1: f = 0.5 d[i] - d[i + 1]
+  int func_launch(const double *v, int i) {
     if (int rc = spmv(v, t3)) return rc;
-    double d[2];
-    if (int rc = dots({{v, t3}, {v, b}}, d)) return rc;
-    *f = 0.5 * d[0] - d[1];
-    return MPRGP_OK;
+    if (int rc = dlaunch(v, t3, i)) return rc;
+    return dlaunch(v, b, i + 1);
   }
 
-  // GeneralConProjector::project (:412-431)
-  int project(const double *v, double *out) {
+  // GeneralConProjector::project (:412-431), the multipliers: one QP per
+  // component of J J^T, h = c - J v, with the freeing threshold of the
+  // whole QP
+  void proj_lam(const double *jv_h, std::vector<double> &l) {
+    std::vector<double> h(m);
+    for (int i = 0; i < m; ++i) h[i] = c[i] - jv_h[i];
+    const double tol = qp_tol(h.data(), m);
+    l.assign(m, 0.0);
+    for (const auto &rows : comps) qp_rows(rows, h.data(), tol, l);
+  }
+  // out = P(v); jv_h: J v on the host when already known
+  int project(const double *v, double *out, const double *jv_h = nullptr) {
     if (m == 0) {
       if (out != v) MPRGP_HIP(hipMemcpyAsync(out, v, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
       return MPRGP_OK;
     }
-    if (int rc = jmul(v)) return rc;
-    std::vector<double> h(m), l(m, 0.0);
-    for (int i = 0; i < m; ++i) h[i] = c[i] - hm[i];
-    const double tol = qp_tol(h.data(), m);
-    for (const auto &rows : comps) qp_rows(rows, h.data(), tol, l);
-    if (int rc = jtmul(l, tl)) return rc;
-    MPRGP_HIP(launch_axpby(s, n, 1.0, v, 1.0, tl, out));
-    return MPRGP_OK;
+    if (!jv_h) {
+      if (int rc = jmul(v)) return rc;
+      jv_h = hj(0);
+    }
+    std::vector<double> l;
+    proj_lam(jv_h, l);
+    stage(0, l);
+    if (int rc = upload(1)) return rc;
+    return jt_apply(0, v, 1.0, out);
   }
-  // DECIDE_FACE + prepareActiveConMatrix (:485-520)
-  int decide_face(const double *v) {
+  // DECIDE_FACE + prepareActiveConMatrix (:485-520) at v, then PHI
+  // (:432-451) at the current g: J v and J g come back together.  With v = x
+  // the host keeps J x.
+  int face_phi(const double *v) {
     act.clear();
     ablk.clear();
     aL.clear();
-    if (m == 0) return MPRGP_OK;
-    if (int rc = jmul(v)) return rc;
+    if (m == 0) {
+      MPRGP_HIP(hipMemcpyAsync(phi, g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+      return MPRGP_OK;
+    }
+    if (int rc = jlaunch(v, 0)) return rc;
+    if (int rc = jlaunch(g, 1)) return rc;
+    if (int rc = readback(2, 0)) return rc;
+    const double *jv_h = hj(0), *jg = hj(1);
+    if (v == x) {
+      jx_h.assign(jv_h, jv_h + m);
+      jx_ok = true;
+    }
     for (int i = 0; i < m; ++i) {
-      face[i] = std::fabs(hm[i] - c[i]) < kEps ? 1 : 0;
+      face[i] = std::fabs(jv_h[i] - c[i]) < kEps ? 1 : 0;
       if (face[i]) act.push_back(i);
     }
     for (const auto &rows : comps) {
@@ -295,60 +352,112 @@ struct General {
       ablk.push_back(std::move(a));
       aL.push_back(std::move(L));
     }
-    return MPRGP_OK;
+    if (act.empty()) {
+      MPRGP_HIP(hipMemcpyAsync(phi, g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+      return MPRGP_OK;
+    }
+    return phi_from(jg);
   }
-  int feasible(const double *v, bool *ok) {  // isFeasible (:298-310)
-    *ok = true;
-    if (m == 0) return MPRGP_OK;
-    if (int rc = jmul(v)) return rc;
-    for (int i = 0; i < m; ++i)
-      if (hm[i] < c[i] - kEps) *ok = false;
-    return MPRGP_OK;
+  // PHI (:432-451) from J g on the host: phi = g + J_a^T l, l solving
+  // (J_a J_a^T) l = -J_a g per component
+  int phi_from(const double *jg) {
+    std::vector<double> l(m, 0.0);
+    for (size_t blk_i = 0; blk_i < ablk.size(); ++blk_i) {
+      const std::vector<int> &a = ablk[blk_i];
+      const int k = (int)a.size();
+      std::vector<double> la(k);
+      for (int x0 = 0; x0 < k; ++x0) la[x0] = -jg[a[x0]];
+      chol_solve(aL[blk_i], k, la.data());
+      for (int x0 = 0; x0 < k; ++x0) l[a[x0]] = la[x0];
+    }
+    stage(0, l);
+    if (int rc = upload(1)) return rc;
+    return jt_apply(0, g, 1.0, phi);
   }
-  // PHI (:432-451)
+  // PHI at the current g and face
   int compute_phi() {
     if (act.empty()) {
       MPRGP_HIP(hipMemcpyAsync(phi, g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
       return MPRGP_OK;
     }
     if (int rc = jmul(g)) return rc;
-    std::vector<double> l(m, 0.0);
-    for (size_t blk_i = 0; blk_i < ablk.size(); ++blk_i) {
-      const std::vector<int> &a = ablk[blk_i];
-      const int k = (int)a.size();
-      std::vector<double> la(k);
-      for (int x0 = 0; x0 < k; ++x0) la[x0] = -hm[a[x0]];
-      chol_solve(aL[blk_i], k, la.data());
-      for (int x0 = 0; x0 < k; ++x0) l[a[x0]] = la[x0];
-    }
-    if (int rc = jtmul(l, tl)) return rc;
-    MPRGP_HIP(launch_axpby(s, n, 1.0, g, 1.0, tl, phi));
-    return MPRGP_OK;
+    return phi_from(hj(0));
   }
-  // BETA (:452-473)
-  int compute_beta() {
+  bool feasible_h(const double *jv_h) const {  // isFeasible (:298-310) from J v
+    for (int i = 0; i < m; ++i)
+      if (jv_h[i] < c[i] - kEps) return false;
+    return true;
+  }
+
+  // computeGradients without PHI (:163-173) and proportional (:174-181) at
+  // v, with phi current.  First half: beta0 = g - phi and t2 = v - ab phi,
+  // J beta0 (slot 0, for BETA :452-473) and J t2 (slot 1, for PHITPHI's
+  // projection :474-484) queued, and J y (slot 2) when want_jy; the caller
+  // adds its own dot products from result 2 on and reads everything back
+  // with head_wait.
+  int head_launch(const double *v, double ab, bool want_jy) {
     MPRGP_HIP(launch_axpby(s, n, 1.0, g, -1.0, phi, beta));
-    if (act.empty()) return MPRGP_OK;
-    if (int rc = jmul(beta)) return rc;
-    // the threshold from every active row's h, as one QP over them would
-    std::vector<double> ha(act.size()), l(m, 0.0);
-    for (size_t a = 0; a < act.size(); ++a) ha[a] = hm[act[a]];
-    const double tol = qp_tol(ha.data(), (int)ha.size());
-    for (const auto &a : ablk) qp_rows(a, hm.data(), tol, l);
-    if (int rc = jtmul(l, tl)) return rc;
-    MPRGP_HIP(launch_axpby(s, n, 1.0, beta, -1.0, tl, beta));
+    MPRGP_HIP(launch_axpby(s, n, 1.0, v, -ab, phi, t2));
+    head_slots = 0;
+    if (m == 0) return MPRGP_OK;
+    if (!act.empty())
+      if (int rc = jlaunch(beta, 0)) return rc;
+    if (int rc = jlaunch(t2, 1)) return rc;
+    head_slots = 2;
+    if (want_jy) {
+      if (int rc = jlaunch(y, 2)) return rc;
+      head_slots = 3;
+    }
     return MPRGP_OK;
   }
-  // PHITPHI (:474-484) into *r
-  int phitphi(const double *v, double ab, double *r) {
-    MPRGP_HIP(launch_axpby(s, n, 1.0, v, -ab, phi, t2));
-    if (int rc = project(t2, t2)) return rc;
-    MPRGP_HIP(launch_axpby(s, n, 1.0, v, -1.0, t2, t2));
-    double d;
-    if (int rc = dots({{t2, phi}}, &d)) return rc;
-    const double v2 = d * (1.0 / ab);
-    *r = v2 > 0.0 ? v2 : 0.0;
+  int head_slots = 0;
+  int head_wait(int k) {
+    if (int rc = readback(head_slots, k)) return rc;
+    if (head_slots == 3) {
+      jy_h.assign(hj(2), hj(2) + m);
+      jy_ok = true;
+    }
     return MPRGP_OK;
+  }
+  // Second half, after head_wait: the QPs of BETA (active rows, h = J beta)
+  // and of the projection of t2 on the host, both sets of multipliers
+  // uploaded with one copy, beta = beta0 - J^T l_b, t2 = v - P(t2),
+  // gp = phi + beta, and the sums gp.gp, beta.beta, t2.phi queued into dot
+  // results 0-2.  A failure of the projection's QP is kept in spec_fail:
+  // it counts only if proportional is evaluated.
+  bool spec_fail = false;
+  int head_finish(const double *v) {
+    if (m > 0) {
+      std::vector<double> lb(m, 0.0), lp;
+      if (!act.empty()) {
+        std::vector<double> ha(act.size());
+        for (size_t a = 0; a < act.size(); ++a) ha[a] = hj(0)[act[a]];
+        const double tol = qp_tol(ha.data(), (int)ha.size());
+        for (const auto &a : ablk) qp_rows(a, hj(0), tol, lb);
+      }
+      const bool failed = qp_fail;
+      proj_lam(hj(1), lp);
+      spec_fail = qp_fail && !failed;
+      qp_fail = failed;
+      stage(0, lb);
+      stage(1, lp);
+      if (int rc = upload(2)) return rc;
+      if (!act.empty())
+        if (int rc = jt_apply(0, beta, -1.0, beta)) return rc;
+      if (int rc = jt_apply(1, t2, 1.0, t2)) return rc;
+    }
+    MPRGP_HIP(launch_axpby(s, n, 1.0, phi, 1.0, beta, gp));
+    MPRGP_HIP(launch_axpby(s, n, 1.0, v, -1.0, t2, t2));
+    if (int rc = dlaunch(gp, gp, 0)) return rc;
+    if (int rc = dlaunch(beta, beta, 1)) return rc;
+    return dlaunch(t2, phi, 2);
+  }
+  // residual and proportional from dot results 0-2 (gamma = 1)
+  void head_result(double ab, double *residual, int *prop) const {
+    *residual = std::sqrt(hd()[0]);
+    const double v2 = hd()[2] * (1.0 / ab);
+    const double pp = v2 > 0.0 ? v2 : 0.0;
+    *prop = hd()[1] <= pp;
   }
   int precond(bool on) {  // DiagonalInFacePreconSolver::solve (MPRGPPrecondition.h:53-64)
     if (on) MPRGP_HIP(launch_vmul(s, n, phi, dinv, z));
@@ -445,19 +554,20 @@ extern "C" int mprgp_general_solve(mprgp_qp *q, int m, const int *j_ptr, const i
       for (int k = j_ptr[i]; k < j_ptr[i + 1]; ++k) row[jc[k]] = 0.0;
     }
   }
-  G.hm.assign(std::max(m, 1), 0.0);
   G.face.assign(m, 0);
   G.build_components();
   // device block: J, J^T, J v, lambda, dot partials and results
   const size_t parts = ((size_t)n + 255) / 256 + 1;
   const size_t bytes = sizeof(int) * ((size_t)m + 1 + (size_t)nnz + (size_t)n + 1 + (size_t)nnz) +
-                       sizeof(double) * (2 * (size_t)nnz + 2 * (size_t)m + parts + 16) + 1024;
+                       sizeof(double) * (2 * (size_t)nnz + 5 * ((size_t)m + 1) + parts + 16) + 1024;
   if (int rc = ctx->alloc(&G.blk, bytes)) return rc;
   G.blk_bytes = bytes;
-  if (hipHostMalloc((void **)&G.h_lam, sizeof(double) * ((size_t)m + 1), hipHostMallocDefault) != hipSuccess) {
+  if (hipHostMalloc((void **)&G.h_lam, sizeof(double) * (5 * ((size_t)m + 1) + 16), hipHostMallocDefault) !=
+      hipSuccess) {
     G.h_lam = nullptr;
     return fail(MPRGP_ENOMEM, "pinned staging of %d multipliers", m);
   }
+  G.h_rb = G.h_lam + 2 * ((size_t)m + 1);
   {
     char *b8 = (char *)G.blk;
     auto carve = [&](size_t sz) {
@@ -467,10 +577,10 @@ extern "C" int mprgp_general_solve(mprgp_qp *q, int m, const int *j_ptr, const i
     };
     G.jv = (double *)carve(sizeof(double) * nnz);
     G.tv = (double *)carve(sizeof(double) * nnz);
-    G.jx = (double *)carve(sizeof(double) * (m + 1));
-    G.lam = (double *)carve(sizeof(double) * (m + 1));
+    G.rb = (double *)carve(sizeof(double) * (3 * ((size_t)m + 1) + 16));
+    G.res = G.rb + 3 * ((size_t)m + 1);
+    G.lam = (double *)carve(sizeof(double) * 2 * ((size_t)m + 1));
     G.part = (double *)carve(sizeof(double) * parts);
-    G.res = (double *)carve(sizeof(double) * 16);
     G.jp = (int *)carve(sizeof(int) * (m + 1));
     G.jc = (int *)carve(sizeof(int) * nnz);
     G.tp = (int *)carve(sizeof(int) * (n + 1));
@@ -541,83 +651,96 @@ extern "C" int mprgp_general_solve(mprgp_qp *q, int m, const int *j_ptr, const i
     lmax = nout;
     alpha_bar = 2.0 / nout;
   }
-  // MPRGPMonotonic::solve (:412-454)
+  // MPRGPMonotonic::solve (:412-454).  Every kernel takes the inputs the
+  // reference's statement order gives it; decisions that need device values
+  // read them back in groups (J v of several vectors, several dot products),
+  // and a value that cannot change between two uses is read once: phi at
+  // the loop head is the one PHI formed at the end of the previous
+  // iteration (same g, same face), J x and J y are kept while x and y are
+  // unchanged, and the loop head also forms the CG entry's A p and sums,
+  // which no other branch reads.
   double residual = 0.0;
   int iteration = 0, num_cg = 0, num_exp = 0, num_prop = 0;
-  auto gradients = [&](bool comp_phi, double *r) -> int {  // computeGradients (:163-173)
-    if (comp_phi)
-      if (int rc = G.compute_phi()) return rc;
-    if (int rc = G.compute_beta()) return rc;
-    MPRGP_HIP(launch_axpby(s, n, 1.0, G.phi, 1.0, G.beta, G.gp));
-    double d;
-    if (int rc = G.dots({{G.gp, G.gp}}, &d)) return rc;
-    *r = std::sqrt(d);
-    return MPRGP_OK;
-  };
-  auto proportional = [&](const double *v, int *prop) -> int {  // :174-181
-    double bb, pp;
-    if (int rc = G.dots({{G.beta, G.beta}}, &bb)) return rc;
-    if (int rc = G.phitphi(v, alpha_bar, &pp)) return rc;
-    *prop = bb <= pp;  // gamma = 1
-    return MPRGP_OK;
-  };
   // initialize (:148-162)
   if (int rc = G.grad(G.x)) return rc;
-  if (int rc = G.decide_face(G.x)) return rc;
-  if (int rc = G.compute_phi()) return rc;
+  if (int rc = G.face_phi(G.x)) return rc;
   if (int rc = G.precond(pre)) return rc;
   MPRGP_HIP(hipMemcpyAsync(G.p, G.z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+  double *AP = G.gp;  // A p shares gp's buffer: gp is read only by its own sum
   for (iteration = 0; iteration < p.max_it;) {
-    if (int rc = gradients(true, &residual)) return rc;
-    if (residual <= tol) break;
+    // computeGradients (:163-173) with phi current, then proportional at x
+    if (int rc = G.head_launch(G.x, alpha_bar, m > 0 && G.y_size > 0)) return rc;
+    if (int rc = G.head_wait(0)) return rc;
+    if (int rc = G.head_finish(G.x)) return rc;
+    // the CG entry's A p and its sums p'Ap, z'g (:274-276), formed after gp's sum
+    if (int rc = G.spmv(G.p, AP)) return rc;
+    if (int rc = G.dlaunch(G.p, AP, 3)) return rc;
+    if (int rc = G.dlaunch(G.z, G.g, 4)) return rc;
+    if (int rc = G.readback(0, 5)) return rc;
     int prop = 0;
-    if (int rc = proportional(G.x, &prop)) return rc;
-    if (prop) {  // CGMonotonicStep (:272-316); AP in gp
-      double *AP = G.gp;
-      if (int rc = G.spmv(G.p, AP)) return rc;
-      double d[2];
-      if (int rc = G.dots({{G.p, AP}, {G.z, G.g}}, d)) return rc;
-      double alpha = d[1] / d[0];
+    G.head_result(alpha_bar, &residual, &prop);
+    if (residual <= tol) break;
+    if (G.spec_fail) G.qp_fail = true;
+    const bool jy_top = G.jy_ok;  // J y of the loop head: y unchanged unless CG steps run
+    if (prop) {  // CGMonotonicStep (:272-316)
+      G.jx_ok = G.jy_ok = false;
+      double alpha = G.hd()[4] / G.hd()[3];
       MPRGP_HIP(launch_axpby(s, n, 1.0, G.x, -alpha, G.p, G.y));
       G.y_size = n;
-      double fy, fx;
       if (int rc = G.project(G.y, G.t)) return rc;
-      if (int rc = G.func(G.t, &fy)) return rc;
-      if (int rc = G.func(G.x, &fx)) return rc;
+      if (int rc = G.func_launch(G.t, 0)) return rc;
+      if (int rc = G.func_launch(G.x, 2)) return rc;
+      if (int rc = G.readback(0, 4)) return rc;
+      double fy = 0.5 * G.hd()[0] - G.hd()[1], fx = 0.5 * G.hd()[2] - G.hd()[3];
       prop = 1;
       while (prop && residual > tol && fy <= fx && iteration < p.max_it) {
         MPRGP_HIP(hipMemcpyAsync(G.x, G.y, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         MPRGP_HIP(launch_axpby(s, n, 1.0, G.g, -alpha, AP, G.g));
         if (int rc = G.compute_phi()) return rc;
         if (int rc = G.precond(pre)) return rc;
-        double e[2];
-        if (int rc = G.dots({{G.z, AP}, {G.p, AP}}, e)) return rc;
-        const double beta = e[0] / e[1];
+        // z'AP and p'AP (beta) come back with the loop head's J products
+        if (int rc = G.head_launch(G.x, alpha_bar, false)) return rc;
+        if (int rc = G.dlaunch(G.z, AP, 3)) return rc;
+        if (int rc = G.dlaunch(G.p, AP, 4)) return rc;
+        if (int rc = G.head_wait(5)) return rc;
+        const double beta = G.hd()[3] / G.hd()[4];
         MPRGP_HIP(launch_axpby(s, n, 1.0, G.z, -beta, G.p, G.p));
-        if (int rc = gradients(false, &residual)) return rc;  // overwrites gp = AP
-        if (int rc = proportional(G.x, &prop)) return rc;
+        if (int rc = G.head_finish(G.x)) return rc;  // gp's sum read before A p overwrites it
         if (int rc = G.spmv(G.p, AP)) return rc;
-        if (int rc = G.dots({{G.p, AP}, {G.z, G.g}}, d)) return rc;
-        alpha = d[1] / d[0];
+        if (int rc = G.dlaunch(G.p, AP, 3)) return rc;
+        if (int rc = G.dlaunch(G.z, G.g, 4)) return rc;
+        if (int rc = G.readback(0, 5)) return rc;
+        G.head_result(alpha_bar, &residual, &prop);
+        if (G.spec_fail) G.qp_fail = true;
+        alpha = G.hd()[4] / G.hd()[3];
         MPRGP_HIP(launch_axpby(s, n, 1.0, G.x, -alpha, G.p, G.y));
         fx = fy;
         if (int rc = G.project(G.y, G.t)) return rc;
-        if (int rc = G.func(G.t, &fy)) return rc;
+        if (int rc = G.func_launch(G.t, 0)) return rc;
+        if (int rc = G.readback(0, 2)) return rc;
+        fy = 0.5 * G.hd()[0] - G.hd()[1];
         num_cg++;
         iteration++;
       }
     }
     if (residual <= tol) break;
     bool feas = false;
-    if (!prop && G.y_size > 0)
-      if (int rc = G.feasible(G.y, &feas)) return rc;
+    if (!prop && G.y_size > 0) {  // isFeasible(y) (:431)
+      if (m == 0) {
+        feas = true;
+      } else if (jy_top && G.jy_ok) {
+        feas = G.feasible_h(G.jy_h.data());
+      } else {
+        if (int rc = G.jmul(G.y)) return rc;
+        feas = G.feasible_h(G.hj(0));
+      }
+    }
     if (prop || G.y_size <= 0 || !feas) {  // ExpMonotonicStep (:250-271)
       num_exp++;
-      if (int rc = G.project(G.x, G.y)) return rc;
+      if (int rc = G.project(G.x, G.y, G.jx_ok ? G.jx_h.data() : nullptr)) return rc;
       G.y_size = n;
       if (int rc = G.grad(G.y)) return rc;
-      if (int rc = G.decide_face(G.y)) return rc;
-      if (int rc = G.compute_phi()) return rc;
+      if (int rc = G.face_phi(G.y)) return rc;
       MPRGP_HIP(launch_axpby(s, n, 1.0, G.y, -alpha_bar, G.phi, G.y));
       if (int rc = G.project(G.y, G.x)) return rc;
       if (int rc = G.grad(G.x)) return rc;
@@ -633,13 +756,14 @@ extern "C" int mprgp_general_solve(mprgp_qp *q, int m, const int *j_ptr, const i
       if (int rc = G.project(G.t, G.x)) return rc;
       MPRGP_HIP(launch_axpby(s, n, 1.0, G.g, -alpha, AD, G.g));
     }
-    if (int rc = G.decide_face(G.x)) return rc;
-    if (int rc = G.compute_phi()) return rc;
+    G.jx_ok = G.jy_ok = false;
+    if (int rc = G.face_phi(G.x)) return rc;
     if (int rc = G.precond(pre)) return rc;
     MPRGP_HIP(hipMemcpyAsync(G.p, G.z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     iteration++;
   }
-  if (int rc = G.project(G.x, G.t)) return rc;  // the final projection (:442-443)
+  // the final projection (:442-443)
+  if (int rc = G.project(G.x, G.t, G.jx_ok ? G.jx_h.data() : nullptr)) return rc;
   MPRGP_HIP(hipMemcpyAsync(G.x, G.t, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   MPRGP_HIP(hipEventRecord(t1, s));
   MPRGP_HIP(hipStreamSynchronize(s));
