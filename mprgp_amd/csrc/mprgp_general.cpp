@@ -231,9 +231,18 @@ struct General {
     MPRGP_HIP(launch_csr_mul(s, m, jp, jc, jv, v, jslot(k)));
     return MPRGP_OK;
   }
-  // queue a . b into dot result i
-  int dlaunch(const double *a, const double *b2, int i) {
-    MPRGP_HIP(launch_dot(s, a, b2, (size_t)n, part, res + i));
+  // queue the dot products of the pairs into results first, first + 1, ...
+  // (one kernel pair)
+  int dgroup(std::initializer_list<std::pair<const double *, const double *>> pairs, int first) {
+    if (pairs.size() > (size_t)kMaxDots) return fail(MPRGP_EHIP, "%zu dot products in one group", pairs.size());
+    const double *a[kMaxDots], *b2[kMaxDots];
+    int k = 0;
+    for (const auto &pr : pairs) {
+      a[k] = pr.first;
+      b2[k] = pr.second;
+      ++k;
+    }
+    MPRGP_HIP(launch_dots(s, k, a, b2, (size_t)n, part, res + first));
     return MPRGP_OK;
   }
   // J v slots [0, slots) and dot results [0, k) to the host, one wait
@@ -265,11 +274,10 @@ struct General {
   }
   // dot products of the pairs, all read back with one synchronisation
   int dots(std::initializer_list<std::pair<const double *, const double *>> pairs, double *out) {
-    int i = 0;
-    for (const auto &pr : pairs)
-      if (int rc = dlaunch(pr.first, pr.second, i++)) return rc;
-    if (int rc = readback(0, i)) return rc;
-    std::copy(hd(), hd() + i, out);
+    const int k = (int)pairs.size();
+    if (int rc = dgroup(pairs, 0)) return rc;
+    if (int rc = readback(0, k)) return rc;
+    std::copy(hd(), hd() + k, out);
     return MPRGP_OK;
   }
   int spmv(const double *v, double *out) {
@@ -286,8 +294,7 @@ struct General {
   // i, i + 1: f = 0.5 d[i] - d[i + 1]
   int func_launch(const double *v, int i) {
     if (int rc = spmv(v, t3)) return rc;
-    if (int rc = dlaunch(v, t3, i)) return rc;
-    return dlaunch(v, b, i + 1);
+    return dgroup({{v, t3}, {v, b}}, i);
   }
 
   // GeneralConProjector::project (:412-431), the multipliers: one QP per
@@ -448,9 +455,7 @@ struct General {
     }
     MPRGP_HIP(launch_axpby(s, n, 1.0, phi, 1.0, beta, gp));
     MPRGP_HIP(launch_axpby(s, n, 1.0, v, -1.0, t2, t2));
-    if (int rc = dlaunch(gp, gp, 0)) return rc;
-    if (int rc = dlaunch(beta, beta, 1)) return rc;
-    return dlaunch(t2, phi, 2);
+    return dgroup({{gp, gp}, {beta, beta}, {t2, phi}}, 0);
   }
   // residual and proportional from dot results 0-2 (gamma = 1)
   void head_result(double ab, double *residual, int *prop) const {
@@ -559,7 +564,7 @@ extern "C" int mprgp_general_solve(mprgp_qp *q, int m, const int *j_ptr, const i
   // device block: J, J^T, J v, lambda, dot partials and results
   const size_t parts = ((size_t)n + 255) / 256 + 1;
   const size_t bytes = sizeof(int) * ((size_t)m + 1 + (size_t)nnz + (size_t)n + 1 + (size_t)nnz) +
-                       sizeof(double) * (2 * (size_t)nnz + 5 * ((size_t)m + 1) + parts + 16) + 1024;
+                       sizeof(double) * (2 * (size_t)nnz + 5 * ((size_t)m + 1) + kMaxDots * parts + 16) + 1024;
   if (int rc = ctx->alloc(&G.blk, bytes)) return rc;
   G.blk_bytes = bytes;
   if (hipHostMalloc((void **)&G.h_lam, sizeof(double) * (5 * ((size_t)m + 1) + 16), hipHostMallocDefault) !=
@@ -580,7 +585,7 @@ extern "C" int mprgp_general_solve(mprgp_qp *q, int m, const int *j_ptr, const i
     G.rb = (double *)carve(sizeof(double) * (3 * ((size_t)m + 1) + 16));
     G.res = G.rb + 3 * ((size_t)m + 1);
     G.lam = (double *)carve(sizeof(double) * 2 * ((size_t)m + 1));
-    G.part = (double *)carve(sizeof(double) * parts);
+    G.part = (double *)carve(sizeof(double) * kMaxDots * parts);
     G.jp = (int *)carve(sizeof(int) * (m + 1));
     G.jc = (int *)carve(sizeof(int) * nnz);
     G.tp = (int *)carve(sizeof(int) * (n + 1));
@@ -674,8 +679,7 @@ extern "C" int mprgp_general_solve(mprgp_qp *q, int m, const int *j_ptr, const i
     if (int rc = G.head_finish(G.x)) return rc;
     // the CG entry's A p and its sums p'Ap, z'g (:274-276), formed after gp's sum
     if (int rc = G.spmv(G.p, AP)) return rc;
-    if (int rc = G.dlaunch(G.p, AP, 3)) return rc;
-    if (int rc = G.dlaunch(G.z, G.g, 4)) return rc;
+    if (int rc = G.dgroup({{G.p, AP}, {G.z, G.g}}, 3)) return rc;
     if (int rc = G.readback(0, 5)) return rc;
     int prop = 0;
     G.head_result(alpha_bar, &residual, &prop);
@@ -700,15 +704,13 @@ extern "C" int mprgp_general_solve(mprgp_qp *q, int m, const int *j_ptr, const i
         if (int rc = G.precond(pre)) return rc;
         // z'AP and p'AP (beta) come back with the loop head's J products
         if (int rc = G.head_launch(G.x, alpha_bar, false)) return rc;
-        if (int rc = G.dlaunch(G.z, AP, 3)) return rc;
-        if (int rc = G.dlaunch(G.p, AP, 4)) return rc;
+        if (int rc = G.dgroup({{G.z, AP}, {G.p, AP}}, 3)) return rc;
         if (int rc = G.head_wait(5)) return rc;
         const double beta = G.hd()[3] / G.hd()[4];
         MPRGP_HIP(launch_axpby(s, n, 1.0, G.z, -beta, G.p, G.p));
         if (int rc = G.head_finish(G.x)) return rc;  // gp's sum read before A p overwrites it
         if (int rc = G.spmv(G.p, AP)) return rc;
-        if (int rc = G.dlaunch(G.p, AP, 3)) return rc;
-        if (int rc = G.dlaunch(G.z, G.g, 4)) return rc;
+        if (int rc = G.dgroup({{G.p, AP}, {G.z, G.g}}, 3)) return rc;
         if (int rc = G.readback(0, 5)) return rc;
         G.head_result(alpha_bar, &residual, &prop);
         if (G.spec_fail) G.qp_fail = true;

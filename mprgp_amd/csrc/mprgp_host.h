@@ -100,6 +100,11 @@ hipError_t launch_get_diag(hipStream_t s, const int *sl_off, const int *col, con
                            int n);
 // a.b into out[0] (device); part holds one double per 256 entries
 hipError_t launch_dot(hipStream_t s, const double *a, const double *b, size_t n, double *part, double *out);
+// k <= kMaxDots dot products a[j].b[j] into out[0..k) with one kernel pair;
+// each has the bits launch_dot gives it.  part holds k doubles per 256 entries
+constexpr int kMaxDots = 8;
+hipError_t launch_dots(hipStream_t s, int k, const double *const *a, const double *const *b, size_t n,
+                       double *part, double *out);
 // part (if non-null) must hold one double per 256 nodes; result receives the reduction
 hipError_t launch_projector(hipStream_t s, const ProjArgs &a, double *result);
 hipError_t launch_to_f32(hipStream_t s, const double *in, float *out, size_t n);

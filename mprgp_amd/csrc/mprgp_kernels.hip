@@ -1408,6 +1408,26 @@ __global__ void __launch_bounds__(256) dot_parts(const double *__restrict__ a, c
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// Several dot products at once: pair blockIdx.y, its partials at
+// part[blockIdx.y * gridDim.x ...], each formed as dot_parts forms it.
+struct DotPairs {
+  const double *a[kMaxDots], *b[kMaxDots];
+};
+__global__ void __launch_bounds__(256) dot_parts_n(DotPairs pr, size_t n, double *__restrict__ part) {
+  __shared__ double red[256];
+  const int j = blockIdx.y;
+  const double *__restrict__ a = pr.a[j];
+  const double *__restrict__ b = pr.b[j];
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  red[threadIdx.x] = i < n ? a[i] * b[i] : 0.0;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(size_t)j * gridDim.x + blockIdx.x] = red[0];
+}
+
 // cs = 1/((n o dinv).n) per row (MPRGPPrecondition.h:81-88).
 __device__ __forceinline__ double row_factor(const double *dinv, const double *cn, unsigned char cf, int r) {
   if (!cf) return 0.0;
@@ -1562,6 +1582,22 @@ __global__ void __launch_bounds__(256) reduce_parts(const double *__restrict__ p
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// reduce_parts (sums) of k partial arrays of n, one workgroup each.
+__global__ void __launch_bounds__(256) reduce_parts_n(const double *__restrict__ part, int n,
+                                                      double *__restrict__ out) {
+  __shared__ double red[256];
+  const double *__restrict__ p = part + (size_t)blockIdx.x * n;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) acc = acc + p[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
 // Coupled-row solve (mprgp_general.cpp): CSR products one thread per row,
@@ -1753,6 +1789,21 @@ hipError_t launch_dot(hipStream_t s, const double *a, const double *b, size_t n,
   if (blocks > 0x7fffffff) return hipErrorInvalidValue;
   dot_parts<<<(unsigned)blocks, 256, 0, s>>>(a, b, n, part);
   reduce_parts<<<1, 256, 0, s>>>(part, (int)blocks, 0, out);
+  return hipGetLastError();
+}
+hipError_t launch_dots(hipStream_t s, int k, const double *const *a, const double *const *b, size_t n,
+                       double *part, double *out) {
+  if (k <= 0 || n == 0) return hipSuccess;
+  if (k > kMaxDots) return hipErrorInvalidValue;
+  const size_t blocks = (n + 255) / 256;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  DotPairs pr{};
+  for (int j = 0; j < k; ++j) {
+    pr.a[j] = a[j];
+    pr.b[j] = b[j];
+  }
+  dot_parts_n<<<dim3((unsigned)blocks, (unsigned)k), 256, 0, s>>>(pr, n, part);
+  reduce_parts_n<<<(unsigned)k, 256, 0, s>>>(part, (int)blocks, out);
   return hipGetLastError();
 }
 hipError_t launch_projector(hipStream_t s, const ProjArgs &a, double *result) {

@@ -86,6 +86,21 @@ def test_decoupled_rows_reach_the_decoupled_minimiser(ctx):
     assert np.linalg.norm(q.get_x() - xd) <= 1e-8 * np.linalg.norm(xd)
 
 
+def test_no_rows_is_unconstrained(ctx):
+    """With no rows every projection, face and BETA is empty: the solve is
+    preconditioned CG with the monotone test and reaches A^-1 b."""
+    A, b, J, c, x0 = tiny_coupled_qp(0)
+    J0 = sp.csr_matrix((0, A.shape[0]))
+    code, xr, sr = oracle.general_solve(A, b, J0, np.zeros(0), x0, tol=1e-11, max_it=3000)
+    q = qp_of(ctx, A, b, x0)
+    rc, st = M.general_solve(q, J0, np.zeros(0), tol=1e-11, max_it=3000, project_first=1)
+    assert code == 0 and rc == 0, st
+    xs = np.linalg.solve(A.toarray(), b)
+    assert np.abs(q.get_x() - xs).max() <= 1e-9 * np.abs(xs).max()
+    assert np.abs(q.get_x() - xr).max() <= 1e-9 * np.abs(xs).max()
+    assert st["active"] == 0 and st["residual"] <= 1e-11
+
+
 def test_coupled_solve_errors(ctx):
     A, b, J, c, x0 = tiny_coupled_qp(0)
     q = qp_of(ctx, A, b, x0)
