@@ -438,7 +438,8 @@ extern "C" int mprgp_matrix_create_csr(mprgp_ctx *ctx, int n, const int *ptr, co
   });
   for (int a = 0; a < N; ++a) rowptr[a + 1] += rowptr[a];
   std::vector<int> colidx(rowptr[N]);
-  std::vector<double> blocks((size_t)rowptr[N] * 9, 0.0);
+  // zeroed row by row in the parallel fill (first touch on the filling thread)
+  std::unique_ptr<double[]> blocks(new double[(size_t)rowptr[N] * 9]);
   host_parallel(N, 2048, [&](int, int a0, int a1) {
     std::vector<int> mark(N, -1), slot(N, -1), cols;
     for (int a = a0; a < a1; ++a) {
@@ -453,6 +454,7 @@ extern "C" int mprgp_matrix_create_csr(mprgp_ctx *ctx, int n, const int *ptr, co
       if (mark[a] != a) cols.push_back(a);  // keep a (zero) diagonal block
       std::sort(cols.begin(), cols.end());
       const int base = rowptr[a];
+      std::memset(blocks.get() + (size_t)base * 9, 0, sizeof(double) * 9 * cols.size());
       for (size_t k = 0; k < cols.size(); ++k) {
         slot[cols[k]] = base + (int)k;
         colidx[base + k] = cols[k];
@@ -464,7 +466,7 @@ extern "C" int mprgp_matrix_create_csr(mprgp_ctx *ctx, int n, const int *ptr, co
   });
   MPRGP_HIP(hipSetDevice(ctx->device));
   std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
-  return build_matrix(ctx, N, rowptr.data(), colidx.data(), blocks.data(), out);
+  return build_matrix(ctx, N, rowptr.data(), colidx.data(), blocks.get(), out);
   MPRGP_CATCH
 }
 
