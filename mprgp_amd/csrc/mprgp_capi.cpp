@@ -476,25 +476,42 @@ extern "C" int mprgp_matrix_update_bsr3(mprgp_matrix *A, const double *blocks) {
   mprgp_ctx *ctx = A->ctx;
   MPRGP_HIP(hipSetDevice(ctx->device));
   std::lock_guard<std::recursive_mutex> lk_(ctx->mu);
-  std::vector<double> val((size_t)A->slots * 9, 0.0);
-  for (long long b = 0; b < A->nb; ++b) {
-    long long slot = A->blk_slot[b];
-    // slot = b0 + k*64 + l with b0 a multiple of 64: value base = b0*9 + k*576 + l
-    long long l = slot % kSlice;
-    long long b0k = slot - l;  // b0 + 64k
-    // find slice start b0 <= slot
-    int s = (int)(std::upper_bound(A->sl_off_h.begin(), A->sl_off_h.end(), (int)slot) - A->sl_off_h.begin()) - 1;
-    long long b0 = A->sl_off_h[s];
-    long long k = (b0k - b0) / kSlice;
-    long long vbase = b0 * 9 + k * 9 * kSlice + l;
-    for (int e = 0; e < 9; ++e) {
-      double v = blocks[9 * b + e];
-      if (!std::isfinite(v)) return fail(MPRGP_EBADINPUT, "non-finite matrix value");
-      val[vbase + (long long)e * kSlice] = v;
+  // the first non-finite value in block order; nothing changes if there is one
+  const int nb = (int)A->nb;
+  long long bad_v[kHostThreads];
+  std::fill(bad_v, bad_v + kHostThreads, LLONG_MAX);
+  host_parallel(nb, 16384, [&](int t, int b0, int b1) {
+    for (long long k = 9LL * b0; k < 9LL * b1; ++k)
+      if (!std::isfinite(blocks[k])) {
+        bad_v[t] = k;
+        return;
+      }
+  });
+  const long long bad = *std::min_element(bad_v, bad_v + kHostThreads);
+  if (bad != LLONG_MAX) return fail(MPRGP_EBADINPUT, "non-finite matrix value at %lld", bad);
+  // SELL values: every slice zeroes its slots (the padding stays zero), then
+  // each block goes to its slot, both on the host threads
+  std::unique_ptr<double[]> val(new double[(size_t)A->slots * 9]);
+  host_parallel(A->n_slices, 64, [&](int, int s0, int s1) {
+    const long long a0 = A->sl_off_h[s0], a1 = A->sl_off_h[s1];
+    std::memset(val.get() + a0 * 9, 0, sizeof(double) * 9 * (size_t)(a1 - a0));
+  });
+  host_parallel(nb, 16384, [&](int, int b0, int b1) {
+    for (int b = b0; b < b1; ++b) {
+      const long long slot = A->blk_slot[b];
+      // slot = s0 + 64 k + l for the start s0 of its slice (a multiple of
+      // 64): the values sit at 9 s0 + 576 k + l + 64 e
+      const long long l = slot % kSlice;
+      const int s = (int)(std::upper_bound(A->sl_off_h.begin(), A->sl_off_h.end(), (int)slot) -
+                          A->sl_off_h.begin()) - 1;
+      const long long s0 = A->sl_off_h[s];
+      const long long k = (slot - l - s0) / kSlice;
+      const long long vbase = s0 * 9 + k * 9 * kSlice + l;
+      for (int e = 0; e < 9; ++e) val[vbase + (long long)e * kSlice] = blocks[9LL * b + e];
     }
-  }
+  });
   hipStream_t s = ctx->stream;
-  MPRGP_HIP(hipMemcpyAsync(A->d_val, val.data(), val.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  MPRGP_HIP(hipMemcpyAsync(A->d_val, val.get(), sizeof(double) * 9 * (size_t)A->slots, hipMemcpyHostToDevice, s));
   MPRGP_HIP(launch_setup_diag(s, A->d_sl_off, A->d_col, A->d_val, A->d_dinv, A->n_nodes));
   MPRGP_HIP(hipStreamSynchronize(s));
   A->version++;
