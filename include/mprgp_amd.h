@@ -155,7 +155,10 @@ int mprgp_matrix_create_csr(mprgp_ctx *ctx, int n, const int *ptr, const int *id
  * mprgp_matrix_info).  What the simulator's per-frame QP dump needs
  * (saveQP, MprgpFemSolver.h:129-133, with mprgp_write_qp). */
 int mprgp_matrix_get_bsr3(mprgp_matrix *A, int *rowptr, int *colidx, double *blocks);
-/* new values, same pattern and block order as at creation */
+/* new values, same pattern and block order as at creation (for a matrix made
+ * from scalar CSR: the block order mprgp_matrix_get_bsr3 returns).  A
+ * non-finite value gives MPRGP_EBADINPUT, naming the first, and leaves the
+ * values unchanged. */
 int mprgp_matrix_update_bsr3(mprgp_matrix *A, const double *blocks);
 int mprgp_matrix_info(const mprgp_matrix *A, int *n_nodes, long long *n_blocks, long long *bytes);
 /* y = A x on host vectors (node order), FixedSparseMatrix::multiply (:469-473) */
