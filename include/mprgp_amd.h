@@ -260,8 +260,9 @@ int mprgp_solve_batch(mprgp_ctx *ctx, int count, mprgp_qp *const *qps, const mpr
  * (MPRGPProjection.h:400-520): rows J x >= c that may span several nodes,
  * such as the self-contact rows of LinearConCollider.cpp:186-217 (a vertex
  * against a triangle, 12 nonzeros); J is m x 3*nodes CSR in user DoF order,
- * m <= 2048, with J J^T nonsingular on every set of rows a projection
- * frees.  Solves the QP's matrix, right-hand side and x (projected first when
+ * m <= 2^20, at most 2048 rows in one connected component of J J^T's
+ * pattern (rows linked through shared nodes), with J J^T nonsingular on
+ * every set of rows a projection frees.  Solves the QP's matrix, right-hand side and x (projected first when
  * params->project_first; the QP's own per-node rows are not used); FP64,
  * monotone variant; params->preconditioned selects the reference's
  * DiagonalInFacePreconSolver (z = D^-1 phi).  alpha_bar <= 0: the power

@@ -30,7 +30,8 @@ using namespace mprgp;
 
 namespace {
 
-constexpr int kMaxGeneralRows = 2048;  // J J^T is kept dense on the host
+constexpr int kMaxGeneralRows = 1 << 20;
+constexpr int kMaxComponentRows = 2048;  // a component's block of J J^T is dense on the host
 
 // Dense Cholesky of a k x k row-major SPD block in place (lower factor);
 // false if a pivot is not positive.
@@ -154,51 +155,103 @@ struct General {
   double *x, *g, *p, *z, *phi, *beta, *y, *gp, *t, *t2, *t3, *tl, *pv, *pw;
   const double *b = nullptr, *dinv = nullptr;
   // host
-  std::vector<double> c, JJt;
+  std::vector<double> c;
   std::vector<char> face;
   std::vector<int> act;
   // J J^T is block diagonal over the connected components of its pattern
   // (rows that share a node): the projection's QP and the active rows'
   // factor split into one small problem per component.  comps: each
-  // component's rows, ascending; ablk/aL: the active rows of each component
-  // with nonempty ones, and the Cholesky factor of their J_a J_a^T block.
+  // component's rows, ascending; comp_of/local_of: a row's component and
+  // place in it; gc: each component's block of J J^T, dense, row-major;
+  // ablk/aL: the active rows of each component with nonempty ones, and the
+  // Cholesky factor of their J_a J_a^T block.
   std::vector<std::vector<int>> comps, ablk;
-  std::vector<std::vector<double>> aL;
+  std::vector<int> comp_of, local_of;
+  std::vector<std::vector<double>> gc, aL;
+  double jjt(int i, int j) const {
+    const int c0 = comp_of[i];
+    if (comp_of[j] != c0) return 0.0;
+    return gc[c0][(size_t)local_of[i] * comps[c0].size() + local_of[j]];
+  }
   bool qp_fail = false;
   int y_size = 0;
 
-  void build_components() {
-    std::vector<int> parent(m);
+  // J J^T on the pairs of rows that share a column (J^T's pattern: column
+  // -> rows), each entry formed as a dense pass forms it: row i scattered
+  // into a dense vector, row j's terms against it in stored order.  Rows are
+  // joined where an entry is nonzero (other pairs are exactly 0), and each
+  // connected component keeps its dense block.
+  int build_jjt(const int *j_ptr, const double *j_val, const std::vector<int> &jc, const std::vector<int> &tptr,
+                const std::vector<int> &tc) {
+    std::vector<std::vector<std::pair<int, double>>> nz(m);
+    std::vector<double> row(n, 0.0);
+    std::vector<int> mark(m, -1), cand, parent(m);
     for (int i = 0; i < m; ++i) parent[i] = i;
     auto find = [&](int i) {
       while (parent[i] != i) i = parent[i] = parent[parent[i]];
       return i;
     };
-    for (int i = 0; i < m; ++i)
-      for (int j = i + 1; j < m; ++j)
-        if (JJt[(size_t)i * m + j] != 0.0 || JJt[(size_t)j * m + i] != 0.0) {
+    for (int i = 0; i < m; ++i) {
+      cand.clear();
+      for (int k = j_ptr[i]; k < j_ptr[i + 1]; ++k) {
+        row[jc[k]] += j_val[k];
+        for (int t = tptr[jc[k]]; t < tptr[jc[k] + 1]; ++t)
+          if (mark[tc[t]] != i) {
+            mark[tc[t]] = i;
+            cand.push_back(tc[t]);
+          }
+      }
+      std::sort(cand.begin(), cand.end());
+      for (int j : cand) {
+        double v = 0.0;
+        for (int k = j_ptr[j]; k < j_ptr[j + 1]; ++k) v += j_val[k] * row[jc[k]];
+        nz[i].emplace_back(j, v);
+        if (v != 0.0 && j != i) {
           const int a = find(i), b2 = find(j);
           if (a != b2) parent[std::max(a, b2)] = std::min(a, b2);
         }
+      }
+      for (int k = j_ptr[i]; k < j_ptr[i + 1]; ++k) row[jc[k]] = 0.0;
+    }
     std::vector<int> slot(m, -1);
     comps.clear();
+    comp_of.assign(m, 0);
+    local_of.assign(m, 0);
     for (int i = 0; i < m; ++i) {
       const int r = find(i);
       if (slot[r] < 0) {
         slot[r] = (int)comps.size();
         comps.emplace_back();
       }
+      comp_of[i] = slot[r];
+      local_of[i] = (int)comps[slot[r]].size();
       comps[slot[r]].push_back(i);
     }
+    gc.assign(comps.size(), {});
+    for (size_t c0 = 0; c0 < comps.size(); ++c0) {
+      const size_t k = comps[c0].size();
+      if (k > (size_t)kMaxComponentRows)
+        return fail(MPRGP_EBADINPUT, "%zu rows coupled through shared nodes; at most %d in one component", k,
+                    kMaxComponentRows);
+      gc[c0].assign(k * k, 0.0);
+      for (int i : comps[c0])
+        for (const auto &e : nz[i])
+          if (comp_of[e.first] == (int)c0) gc[c0][(size_t)local_of[i] * k + local_of[e.first]] = e.second;
+    }
+    return MPRGP_OK;
   }
   // l[rows] = argmin 1/2 l'Gl - h'l, l >= 0, over J J^T's block of `rows`
   // (h indexed by row), freeing threshold tol
   void qp_rows(const std::vector<int> &rows, const double *h, double tol, std::vector<double> &l) {
     const int k = (int)rows.size();
-    std::vector<double> Gk((size_t)k * k), hk(k), lk;
-    for (int a = 0; a < k; ++a) {
-      hk[a] = h[rows[a]];
-      for (int bb = 0; bb < k; ++bb) Gk[(size_t)a * k + bb] = JJt[(size_t)rows[a] * m + rows[bb]];
+    std::vector<double> Gk, hk(k), lk;
+    for (int a = 0; a < k; ++a) hk[a] = h[rows[a]];
+    if (k == (int)comps[comp_of[rows[0]]].size()) {  // a whole component: its block
+      Gk = gc[comp_of[rows[0]]];
+    } else {
+      Gk.resize((size_t)k * k);
+      for (int a = 0; a < k; ++a)
+        for (int bb = 0; bb < k; ++bb) Gk[(size_t)a * k + bb] = jjt(rows[a], rows[bb]);
     }
     if (!qp_nonneg(k, Gk, hk, lk, tol)) qp_fail = true;
     for (int a = 0; a < k; ++a) l[rows[a]] = lk.empty() ? 0.0 : lk[a];
@@ -354,7 +407,7 @@ struct General {
       const int k = (int)a.size();
       std::vector<double> L((size_t)k * k);
       for (int x0 = 0; x0 < k; ++x0)
-        for (int x1 = 0; x1 < k; ++x1) L[(size_t)x0 * k + x1] = JJt[(size_t)a[x0] * m + a[x1]];
+        for (int x1 = 0; x1 < k; ++x1) L[(size_t)x0 * k + x1] = jjt(a[x0], a[x1]);
       if (!chol_factor(L, k)) qp_fail = true;
       ablk.push_back(std::move(a));
       aL.push_back(std::move(L));
@@ -544,23 +597,9 @@ extern "C" int mprgp_general_solve(mprgp_qp *q, int m, const int *j_ptr, const i
   const int nnz = m > 0 ? j_ptr[m] : 0;
   const std::vector<int> &jc = R.jc, &tptr = R.tptr, &tc = R.tc;
   const std::vector<double> &tv = R.tv;
-  // J J^T (dense, host) by a column-dense pass over each row
   G.c.assign(c, c + m);
-  G.JJt.assign((size_t)m * m, 0.0);
-  {
-    std::vector<double> row(n, 0.0);
-    for (int i = 0; i < m; ++i) {
-      for (int k = j_ptr[i]; k < j_ptr[i + 1]; ++k) row[jc[k]] += j_val[k];
-      for (int j = 0; j < m; ++j) {
-        double v = 0.0;
-        for (int k = j_ptr[j]; k < j_ptr[j + 1]; ++k) v += j_val[k] * row[jc[k]];
-        G.JJt[(size_t)i * m + j] = v;
-      }
-      for (int k = j_ptr[i]; k < j_ptr[i + 1]; ++k) row[jc[k]] = 0.0;
-    }
-  }
+  if (int rc = G.build_jjt(j_ptr, j_val, jc, tptr, tc)) return rc;
   G.face.assign(m, 0);
-  G.build_components();
   // device block: J, J^T, J v, lambda, dot partials and results
   const size_t parts = ((size_t)n + 255) / 256 + 1;
   const size_t bytes = sizeof(int) * ((size_t)m + 1 + (size_t)nnz + (size_t)n + 1 + (size_t)nnz) +

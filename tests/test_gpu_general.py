@@ -86,6 +86,41 @@ def test_decoupled_rows_reach_the_decoupled_minimiser(ctx):
     assert np.linalg.norm(q.get_x() - xd) <= 1e-8 * np.linalg.norm(xd)
 
 
+def test_more_rows_than_one_dense_block(ctx):
+    """J J^T is kept per connected component of its pattern, so the row count
+    has no dense limit: 3,321 per-node rows (a 41x9x9-node beam) reach the
+    decoupled solve's minimiser, and with 12 coupled rows on top the solve
+    converges to a feasible point whose objective is not below the
+    minimum without them (more rows, smaller feasible set).  At tol 1e-9 the
+    coupled case stalls above its minimum, the reference's behaviour on
+    overlapping rows (test_general_solve_can_stall_on_overlapping_rows)."""
+    sc = problems.beam_scene("beam40x8x8", 40, 8, 8, seed=5, tol=1e-8)
+    A, b, node, normal, c, x0 = scene_qp(sc)
+    assert len(node) > 2048
+    J = problems.rows_to_J(node, normal, sc.n_nodes)
+    q = qp_of(ctx, A, b, x0)
+    q.set_rows(node, normal, c)
+    rc, sd = q.solve(tol=1e-10, max_it=20000, project_first=1)
+    xd = q.get_x()
+    ab = sd["alpha_bar"]
+    q.reset_x()
+    rc2, sg = M.general_solve(q, J, c, tol=1e-10, max_it=20000, alpha_bar=ab, project_first=1)
+    print(f"{len(node)} rows: decoupled {sd['iterations']} it, coupled path {sg['iterations']} it, "
+          f"{sg['elapsed_ms']:.1f} ms", flush=True)
+    assert rc == rc2 == 0
+    assert np.linalg.norm(q.get_x() - xd) <= 1e-8 * np.linalg.norm(xd)
+    Jc, cc = problems.self_contact_rows(sc, np.random.default_rng(2), 12)
+    J2, c2 = sp.vstack([J, Jc]).tocsr(), np.concatenate([c, cc])
+    q.reset_x()
+    rc3, s3 = M.general_solve(q, J2, c2, tol=1e-6, max_it=20000, alpha_bar=ab, project_first=1)
+    print(f"with 12 coupled rows: {s3['iterations']} it, code {rc3}", flush=True)
+    x3 = q.get_x()
+    assert rc3 == 0 and s3["residual"] <= 1e-6
+    assert (J2 @ x3 - c2).min() >= -1e-9
+    f = lambda x: 0.5 * x @ (A @ x) - b @ x
+    assert f(x3) >= f(xd) - 1e-10 * abs(f(xd))
+
+
 def test_no_rows_is_unconstrained(ctx):
     """With no rows every projection, face and BETA is empty: the solve is
     preconditioned CG with the monotone test and reaches A^-1 b."""
