@@ -32,6 +32,7 @@ namespace {
 
 constexpr int kMaxGeneralRows = 1 << 20;
 constexpr int kMaxComponentRows = 2048;  // a component's block of J J^T is dense on the host
+constexpr size_t kMaxCoupledPairs = (size_t)1 << 26;  // J J^T entries formed (16 bytes each on the host)
 
 // Dense Cholesky of a k x k row-major SPD block in place (lower factor);
 // false if a pivot is not positive.
@@ -186,6 +187,7 @@ struct General {
     std::vector<std::vector<std::pair<int, double>>> nz(m);
     std::vector<double> row(n, 0.0);
     std::vector<int> mark(m, -1), cand, parent(m);
+    size_t pairs = 0;
     for (int i = 0; i < m; ++i) parent[i] = i;
     auto find = [&](int i) {
       while (parent[i] != i) i = parent[i] = parent[parent[i]];
@@ -201,6 +203,14 @@ struct General {
             cand.push_back(tc[t]);
           }
       }
+      // rows that share a column with row i all join its component unless
+      // their products vanish; bound the work and memory up front
+      if (cand.size() > (size_t)kMaxComponentRows)
+        return fail(MPRGP_EBADINPUT, "row %d shares a column with %zu rows; at most %d in one component", i,
+                    cand.size(), kMaxComponentRows);
+      pairs += cand.size();
+      if (pairs > kMaxCoupledPairs)
+        return fail(MPRGP_EBADINPUT, "more than %zu pairs of rows share a column", kMaxCoupledPairs);
       std::sort(cand.begin(), cand.end());
       for (int j : cand) {
         double v = 0.0;

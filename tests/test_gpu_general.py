@@ -121,6 +121,20 @@ def test_more_rows_than_one_dense_block(ctx):
     assert f(x3) >= f(xd) - 1e-10 * abs(f(xd))
 
 
+def test_too_many_rows_on_one_node_are_rejected(ctx):
+    """More rows sharing a column than one component may hold (2048) are
+    rejected before J J^T is formed, with the row named."""
+    A, b, J, c, x0 = tiny_coupled_qp(0)
+    rng = np.random.default_rng(9)
+    m = 2100
+    nrm = rng.normal(size=(m, 3))
+    Jbig = sp.csr_matrix((nrm.ravel(), np.tile(np.arange(3), m), 3 * np.arange(m + 1)), shape=(m, A.shape[0]))
+    q = qp_of(ctx, A, b, x0)
+    with pytest.raises(M.MprgpError) as e:
+        M.general_solve(q, Jbig, Jbig @ x0 - 1.0, tol=1e-8, max_it=10, alpha_bar=0.01)
+    assert e.value.code == -2 and "shares a column with 2100 rows" in M.last_error()
+
+
 def test_no_rows_is_unconstrained(ctx):
     """With no rows every projection, face and BETA is empty: the solve is
     preconditioned CG with the monotone test and reaches A^-1 b."""
