@@ -317,6 +317,13 @@ struct General {
     MPRGP_HIP(hipStreamSynchronize(s));
     return MPRGP_OK;
   }
+  // J v[j] into readback slot[j], j < k, in one launch
+  int jlaunch_n(int k, const double *const *v, const int *slot) {
+    double *o[kMaxCsrVecs];
+    for (int j = 0; j < k; ++j) o[j] = jslot(slot[j]);
+    MPRGP_HIP(launch_csr_mul_n(s, m, jp, jc, jv, k, v, o));
+    return MPRGP_OK;
+  }
   // J v to the host (slot 0)
   int jmul(const double *v) {
     if (int rc = jlaunch(v, 0)) return rc;
@@ -397,8 +404,9 @@ struct General {
       MPRGP_HIP(hipMemcpyAsync(phi, g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
       return MPRGP_OK;
     }
-    if (int rc = jlaunch(v, 0)) return rc;
-    if (int rc = jlaunch(g, 1)) return rc;
+    const double *vs[2] = {v, g};
+    const int sl[2] = {0, 1};
+    if (int rc = jlaunch_n(2, vs, sl)) return rc;
     if (int rc = readback(2, 0)) return rc;
     const double *jv_h = hj(0), *jg = hj(1);
     if (v == x) {
@@ -470,15 +478,21 @@ struct General {
     MPRGP_HIP(launch_axpby(s, n, 1.0, v, -ab, phi, t2));
     head_slots = 0;
     if (m == 0) return MPRGP_OK;
-    if (!act.empty())
-      if (int rc = jlaunch(beta, 0)) return rc;
-    if (int rc = jlaunch(t2, 1)) return rc;
+    const double *vs[3];
+    int sl[3], k = 0;
+    if (!act.empty()) {
+      vs[k] = beta;
+      sl[k++] = 0;
+    }
+    vs[k] = t2;
+    sl[k++] = 1;
     head_slots = 2;
     if (want_jy) {
-      if (int rc = jlaunch(y, 2)) return rc;
+      vs[k] = y;
+      sl[k++] = 2;
       head_slots = 3;
     }
-    return MPRGP_OK;
+    return jlaunch_n(k, vs, sl);
   }
   int head_slots = 0;
   int head_wait(int k) {

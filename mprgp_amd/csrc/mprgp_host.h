@@ -72,6 +72,11 @@ int rows_to_internal(const mprgp_matrix *A, int m, const int *j_ptr, const int *
 hipError_t launch_csr_mul(hipStream_t s, int rows, const int *ptr, const int *col, const double *val,
                           const double *x, double *out);
 hipError_t launch_axpby(hipStream_t s, size_t n, double a, const double *x, double b, const double *y, double *out);
+// k <= kMaxCsrVecs products out[j] = C x[j] with one CSR matrix C in one
+// launch, each the bits launch_csr_mul gives
+constexpr int kMaxCsrVecs = 4;
+hipError_t launch_csr_mul_n(hipStream_t s, int rows, const int *ptr, const int *col, const double *val, int k,
+                            const double *const *x, double *const *out);
 hipError_t launch_vmul(hipStream_t s, size_t n, const double *x, const double *y, double *out);
 // out[i] = power_start_raw(i) / nr for i < n (out as is when nr = 0)
 hipError_t launch_power_start(hipStream_t s, double *out, size_t n, double nr);

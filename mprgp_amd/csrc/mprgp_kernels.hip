@@ -1611,6 +1611,21 @@ __global__ void __launch_bounds__(256) csr_mul(int rows, const int *__restrict__
   for (int k = ptr[i]; k < ptr[i + 1]; ++k) s += val[k] * x[col[k]];
   out[i] = s;
 }
+// Several products with one CSR matrix: vector blockIdx.y, each row formed
+// as csr_mul forms it.
+struct CsrVecs {
+  const double *x[kMaxCsrVecs];
+  double *out[kMaxCsrVecs];
+};
+__global__ void __launch_bounds__(256) csr_mul_n(int rows, const int *__restrict__ ptr, const int *__restrict__ col,
+                                                 const double *__restrict__ val, CsrVecs v) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= rows) return;
+  const double *__restrict__ x = v.x[blockIdx.y];
+  double s = 0.0;
+  for (int k = ptr[i]; k < ptr[i + 1]; ++k) s += val[k] * x[col[k]];
+  v.out[blockIdx.y][i] = s;
+}
 // out = a x + b y
 __global__ void __launch_bounds__(256) axpby(size_t n, double a, const double *__restrict__ x, double b,
                                              const double *__restrict__ y, double *__restrict__ out) {
@@ -1817,6 +1832,18 @@ hipError_t launch_csr_mul(hipStream_t s, int rows, const int *ptr, const int *co
                           const double *x, double *out) {
   if (rows <= 0) return hipSuccess;
   csr_mul<<<(rows + 255) / 256, 256, 0, s>>>(rows, ptr, col, val, x, out);
+  return hipGetLastError();
+}
+hipError_t launch_csr_mul_n(hipStream_t s, int rows, const int *ptr, const int *col, const double *val, int k,
+                            const double *const *x, double *const *out) {
+  if (rows <= 0 || k <= 0) return hipSuccess;
+  if (k > kMaxCsrVecs) return hipErrorInvalidValue;
+  CsrVecs v{};
+  for (int j = 0; j < k; ++j) {
+    v.x[j] = x[j];
+    v.out[j] = out[j];
+  }
+  csr_mul_n<<<dim3((rows + 255) / 256, (unsigned)k), 256, 0, s>>>(rows, ptr, col, val, v);
   return hipGetLastError();
 }
 hipError_t launch_axpby(hipStream_t s, size_t n, double a, const double *x, double b, const double *y, double *out) {
