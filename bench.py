@@ -286,6 +286,37 @@ def ica_leg(M, problems, workloads, ctx, tol=8e-5, max_it=5000):
     return out
 
 
+def coupled_leg(M, problems, workloads, ctx, tol=1e-9, max_it=3000, reps=3):
+    """The coupled-row solve (MPRGPGeneralCon<double>::solve,
+    MPRGPSolver.h:594-626, with GeneralConProjector): the flat-dropped
+    525-node beam with its 525 ground rows and 12 self-contact rows of 12
+    nonzeros (LinearConCollider.cpp:186-217), A and b assembled on the
+    device, alpha_bar from the device power method, x from the scene's
+    start, projected first.  Returns the timing and, for the CPU leg, the
+    same QP on the host."""
+    import scipy.sparse as sp
+    sc = problems.config1_flat()
+    dq = workloads.DeviceQP(ctx, sc)
+    node, normal, c = sc.rows()
+    Jc, cc = problems.self_contact_rows(sc, np.random.default_rng(2), 12)
+    J = sp.vstack([problems.rows_to_J(node, normal, sc.n_nodes), Jc]).tocsr()
+    c = np.concatenate([c, cc])
+    rc, st = dq.qp.solve(M.params(tol=tol, max_it=0, project_first=1))
+    ab = st["alpha_bar"]
+    ms, s = [], None
+    for _ in range(reps):
+        dq.qp.reset_x()
+        t0 = time.perf_counter()
+        rc, s = M.general_solve(dq.qp, J, c, tol=tol, max_it=max_it, alpha_bar=ab, project_first=1)
+        ms.append(1e3 * (time.perf_counter() - t0))
+    out = {"workload": f"beam 20x4x4 dropped flat, {3 * sc.n_nodes} DoF, {J.shape[0]} rows "
+                       f"({len(node)} ground, {Jc.shape[0]} self-contact), FP64, tol {tol:g}",
+           "gpu_ms": [round(t, 2) for t in ms], "device_ms": round(s["elapsed_ms"], 2), "code": rc,
+           "iterations": s["iterations"], "steps": [s["num_cg"], s["num_exp"], s["num_prop"]]}
+    host_qp = (dq.fem.matrix.to_csr(), dq.qp.get_rhs(), J, c, sc.x_n.ravel().copy(), ab, tol, max_it)
+    return out, host_qp
+
+
 def spmv_leg(M, workloads, ctx, qps, tiles, graph_steps):
     """Bandwidth of the batched SpMV (FixedSparseMatrix::multiply,
     MPRGPSolver.h:469-473) on the whole batch: every QP runs the power method
@@ -343,6 +374,7 @@ def main():
     ap.add_argument("--no-spmv", action="store_true", help="skip the batched SpMV bandwidth leg")
     ap.add_argument("--no-dropin", action="store_true", help="skip the one-shot drop-in leg")
     ap.add_argument("--no-ica", action="store_true", help="skip the ICA-versus-MPRGP comparison leg")
+    ap.add_argument("--no-coupled", action="store_true", help="skip the coupled-row solve leg")
     ap.add_argument("--pmc-child", choices=["copy", "solve", "spmv"], help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -488,6 +520,11 @@ def main():
         rel = np.array([np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300) for a, b in zip(x_head, x_plain)])
         err = {"vs_gpu_fp64_max": float(rel.max()), "vs_gpu_fp64_median": float(np.median(rel))}
 
+    coupled, coupled_host = None, None
+    if not args.no_coupled:
+        coupled, coupled_host = coupled_leg(M, problems, workloads, ctx)
+        log(f"coupled rows: {coupled}")
+
     cpu, cpu_all = None, None
     if not args.no_cpu:
         import oracle
@@ -579,6 +616,14 @@ def main():
                              f"{hinfo['host_logical_cpus']} logical CPUs)"}
         del sample
         log(f"cpu oracle, all cores: {cpu_all}")
+        if coupled_host is not None:
+            # the coupled solve's restatement on the same QP, one thread
+            A_, b_, J_, c_, x0_, ab_, tol_, it_ = coupled_host
+            s0 = time.perf_counter()
+            code_, _, so_ = oracle.general_solve(A_, b_, J_, c_, x0_, tol=tol_, max_it=it_, alpha_bar=ab_)
+            coupled["cpu_oracle"] = {"ms": round(1e3 * (time.perf_counter() - s0), 1), "code": code_,
+                                     "iterations": so_["iterations"], "cores": 1, "kind": "port"}
+            log(f"coupled rows, cpu oracle: {coupled['cpu_oracle']}")
 
     serve = None
     if not args.no_serving:
@@ -636,6 +681,7 @@ def main():
         "cpu_baseline_all_cores": cpu_all,
         "dropin_fp64_133k": dropin,
         "ica_vs_mprgp_133k": ica,
+        "coupled_rows_537": coupled,
     }
     print(json.dumps(out), flush=True)
 
