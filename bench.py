@@ -317,6 +317,39 @@ def coupled_leg(M, problems, workloads, ctx, tol=1e-9, max_it=3000, reps=3):
     return out, host_qp
 
 
+def step_leg(M, problems, ctx, tol=1e-6, max_it=1000, newton_max_it=10, newton_eps=1e-4, reps=3):
+    """One time step of MprgpFemSolver::advance (MprgpFemSolver.cpp:193-203,
+    with DecoupledMprgpFemSolver::forward's Newton loop, :514-537) on the
+    133,623-DoF beam, device-resident (mprgp_fem_advance): the planes'
+    collision at x_n, per Newton iteration the assembly at x1 and the MPRGP
+    solve, then the implicit Euler update.  alpha_bar comes from the device
+    power method of an untimed first step and is passed to every timed solve;
+    the CPU leg runs the oracle's step with the same alpha_bar."""
+    sc = problems.config2()
+    fem = M.Fem(ctx, sc.X0, sc.tets, sc.young, sc.poisson, sc.density)
+    q = fem.matrix.qp()
+    planes = problems.plane_array(sc.planes)
+    spar = fem.step_params(h=sc.h, gravity=sc.gravity, material=sc.material, planes=planes,
+                           newton_max_it=newton_max_it, newton_eps=newton_eps)
+    x_n, v_n = sc.x_n.ravel().copy(), sc.v_n.ravel().copy()
+    fem.set_state(x_n, v_n)
+    rc, st = fem.advance(q, spar, tol=tol, max_it=max_it)
+    ab = st["last"]["alpha_bar"]
+    ms = []
+    for _ in range(reps):
+        fem.set_state(x_n, v_n)
+        t0 = time.perf_counter()
+        rc, st = fem.advance(q, spar, tol=tol, max_it=max_it, alpha_bar=ab)
+        ms.append(1e3 * (time.perf_counter() - t0))
+    x_gpu, _ = fem.get_state()
+    out = {"workload": f"beam 100x20x20, {3 * sc.n_nodes} DoF, planes collided at x_n, Newton to "
+                       f"{newton_eps:g} (at most {newton_max_it}), FP64 solves to {tol:g}",
+           "gpu_ms": [round(t, 2) for t in ms], "code": rc, "rows": st["rows"],
+           "newton_iters": st["newton_iters"], "solve_iterations": st["iterations"]}
+    host = (sc, planes, x_n, v_n, ab, tol, max_it, newton_max_it, newton_eps, x_gpu)
+    return out, host
+
+
 def spmv_leg(M, workloads, ctx, qps, tiles, graph_steps):
     """Bandwidth of the batched SpMV (FixedSparseMatrix::multiply,
     MPRGPSolver.h:469-473) on the whole batch: every QP runs the power method
@@ -375,6 +408,7 @@ def main():
     ap.add_argument("--no-dropin", action="store_true", help="skip the one-shot drop-in leg")
     ap.add_argument("--no-ica", action="store_true", help="skip the ICA-versus-MPRGP comparison leg")
     ap.add_argument("--no-coupled", action="store_true", help="skip the coupled-row solve leg")
+    ap.add_argument("--no-step", action="store_true", help="skip the time-step leg")
     ap.add_argument("--pmc-child", choices=["copy", "solve", "spmv"], help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -520,6 +554,11 @@ def main():
         rel = np.array([np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300) for a, b in zip(x_head, x_plain)])
         err = {"vs_gpu_fp64_max": float(rel.max()), "vs_gpu_fp64_median": float(np.median(rel))}
 
+    step, step_host = None, None
+    if not args.no_step:
+        step, step_host = step_leg(M, problems, ctx)
+        log(f"time step: {step}")
+
     coupled, coupled_host = None, None
     if not args.no_coupled:
         coupled, coupled_host = coupled_leg(M, problems, workloads, ctx)
@@ -624,6 +663,21 @@ def main():
             coupled["cpu_oracle"] = {"ms": round(1e3 * (time.perf_counter() - s0), 1), "code": code_,
                                      "iterations": so_["iterations"], "cores": 1, "kind": "port"}
             log(f"coupled rows, cpu oracle: {coupled['cpu_oracle']}")
+        if step_host is not None:
+            # the oracle's time step (collision, Newton loop with assembly and
+            # solve, implicit Euler) from the same state, one thread
+            sc_, planes_, x_n_, v_n_, ab_, tol_, it_, nmax_, neps_, x_gpu_ = step_host
+            tets_, d_, vol_, mass_ = oracle.fem_setup(sc_.n_nodes, sc_.tets, sc_.X0)
+            s0 = time.perf_counter()
+            x_o, _, info_ = oracle.advance(sc_.n_nodes, tets_, d_, vol_, mass_, sc_.young, sc_.poisson,
+                                           sc_.density, x_n_, v_n_, sc_.gravity, sc_.h, sc_.material, planes_,
+                                           newton_max_it=nmax_, newton_eps=neps_, tol=tol_, max_it=it_,
+                                           alpha_bar=ab_)
+            step["cpu_oracle"] = {"ms": round(1e3 * (time.perf_counter() - s0), 1), "rows": info_["rows"],
+                                  "newton_iters": info_["newton_iters"], "solve_iterations": info_["iterations"],
+                                  "cores": 1, "kind": "port",
+                                  "x_rel_diff_vs_gpu": float(np.linalg.norm(x_gpu_ - x_o) / np.linalg.norm(x_o))}
+            log(f"time step, cpu oracle: {step['cpu_oracle']}")
 
     serve = None
     if not args.no_serving:
@@ -682,6 +736,7 @@ def main():
         "dropin_fp64_133k": dropin,
         "ica_vs_mprgp_133k": ica,
         "coupled_rows_537": coupled,
+        "time_step_133k": step,
     }
     print(json.dumps(out), flush=True)
 
