@@ -39,7 +39,9 @@
 
 namespace mprgp {
 
-// per tet: kc K_e blocks, then h^2 f_e, then K_e c_e (the damping term)
+// per tet, contiguous (tet-major, so a block's 9 values are one read for the
+// kernels that sum them): kc K_e blocks, then h^2 f_e, then K_e c_e (the
+// damping term)
 constexpr int kElemOut = 16 * 9 + 12 + 12;
 
 __device__ inline double det3(const double *M) {
@@ -258,19 +260,19 @@ __global__ void __launch_bounds__(128) fem_element(int T, const int4 *__restrict
       for (int r = 0; r < 3; ++r)
         for (int s = 0; s < 3; ++s) {
           double k = V * (lam * u[I][r] * u[J][s] + mu * (RRt[r * 3 + s] * bb + u[J][r] * u[I][s]));
-          if (!rhs_only) out[(size_t)(((I * 4 + J) * 9) + r * 3 + s) * T + t] = k * kc;
+          if (!rhs_only) out[(size_t)t * kElemOut + ((I * 4 + J) * 9) + r * 3 + s] = k * kc;
           if (damp) kd[r] += k * ce[J * 3 + s];
         }
     }
     if (damp)
-      for (int r = 0; r < 3; ++r) out[(size_t)(156 + I * 3 + r) * T + t] = kd[r];
+      for (int r = 0; r < 3; ++r) out[(size_t)t * kElemOut + 156 + I * 3 + r] = kd[r];
   }
   // h^2 f_e
   for (int I = 0; I < 4; ++I)
     for (int r = 0; r < 3; ++r) {
       double s = 0.0;
       for (int c = 0; c < 3; ++c) s += RP[r * 3 + c] * B[I][c];
-      out[(size_t)(144 + I * 3 + r) * T + t] = -(V * s) * h2;
+      out[(size_t)t * kElemOut + 144 + I * 3 + r] = -(V * s) * h2;
     }
 }
 
@@ -288,7 +290,8 @@ __global__ void fem_blocks(long long nb, int T, const int *__restrict__ ptr, con
   for (int k = ptr[b]; k < ptr[b + 1]; ++k) {
     const int code = ctr[k];
     const int t = code >> 4, IJ = code & 15;
-    for (int e = 0; e < 9; ++e) acc[e] += elem[(size_t)(IJ * 9 + e) * T + t];
+    const double *eb = elem + (size_t)t * kElemOut + IJ * 9;  // the block's 9 values, contiguous
+    for (int e = 0; e < 9; ++e) acc[e] += eb[e];
   }
   const int dn = diag_node[b];
   if (dn >= 0) {
@@ -318,14 +321,14 @@ __global__ void fem_rhs(int N, int T, const int *__restrict__ ptr, const int *__
   for (int k = ptr[a]; k < ptr[a + 1]; ++k) {
     const int code = ctr[k];
     const int t = code >> 2, I = code & 3;
-    for (int r = 0; r < 3; ++r) s[r] += elem[(size_t)(144 + I * 3 + r) * T + t];
+    for (int r = 0; r < 3; ++r) s[r] += elem[(size_t)t * kElemOut + 144 + I * 3 + r];
   }
   if (damp) {
     double kd[3] = {0.0, 0.0, 0.0};
     for (int k = ptr[a]; k < ptr[a + 1]; ++k) {
       const int code = ctr[k];
       const int t = code >> 2, I = code & 3;
-      for (int r = 0; r < 3; ++r) kd[r] += elem[(size_t)(156 + I * 3 + r) * T + t];
+      for (int r = 0; r < 3; ++r) kd[r] += elem[(size_t)t * kElemOut + 156 + I * 3 + r];
     }
     for (int r = 0; r < 3; ++r) s[r] += kd[r] * alpha_m;
   }
