@@ -345,20 +345,42 @@ __global__ void fem_rhs(int N, int T, const int *__restrict__ ptr, const int *__
   for (int r = 0; r < 3; ++r) b[3 * (size_t)a + r] = s[r];
 }
 
-// max_i |a_i - b_i| (updatePos's tol_err, MprgpFemSolver.cpp:368-374): one
-// workgroup, a fixed reduction order, so the result is reproducible.
-__global__ void __launch_bounds__(1024) max_abs_diff(const double *__restrict__ a, const double *__restrict__ b,
-                                                     size_t n, double *__restrict__ out) {
-  __shared__ double red[1024];
+// max_i |a_i - b_i| (updatePos's tol_err, MprgpFemSolver.cpp:368-374) in
+// two kernels: each workgroup's maximum over a strided share, then the
+// maximum of those.  fmax is exact and ignores NaN in any order, so the
+// result does not depend on the split.
+constexpr int kMaxParts = 256;
+__global__ void __launch_bounds__(256) max_abs_diff_parts(const double *__restrict__ a,
+                                                          const double *__restrict__ b, size_t n,
+                                                          double *__restrict__ part) {
+  __shared__ double red[256];
   double m = 0.0;
-  for (size_t i = threadIdx.x; i < n; i += 1024) m = fmax(m, fabs(a[i] - b[i]));
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+    m = fmax(m, fabs(a[i] - b[i]));
   red[threadIdx.x] = m;
   __syncthreads();
-  for (int w = 512; w > 0; w >>= 1) {
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+__global__ void __launch_bounds__(256) max_parts(const double *__restrict__ part, int np, double *__restrict__ out) {
+  __shared__ double red[256];
+  red[threadIdx.x] = (int)threadIdx.x < np ? part[threadIdx.x] : 0.0;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = red[0];
+}
+// out[0] = max |a - b|, out[1 .. kMaxParts] the partial maxima
+hipError_t launch_max_abs_diff(hipStream_t s, const double *a, const double *b, size_t n, double *out) {
+  const int np = (int)std::min<size_t>(kMaxParts, std::max<size_t>(1, (n + 4095) / 4096));
+  max_abs_diff_parts<<<np, 256, 0, s>>>(a, b, n, out + 1);
+  max_parts<<<1, 256, 0, s>>>(out + 1, np, out);
+  return hipGetLastError();
 }
 
 // updateMesh (MprgpFemSolver.cpp:377-390) with initVel's PHI and PSI
@@ -788,9 +810,8 @@ extern "C" int mprgp_fem_end_step(mprgp_fem *f, mprgp_qp *qp, const mprgp_step_p
   // updatePos: x2 = the solve's x (node order), tol_err = max |x2 - x1|
   MPRGP_HIP(launch_scatter_nodes(s, qp->x, f->x2, f->A->d_perm, N));
   if (delta) {
-    if (int rc = ctx->scratch_reserve(256)) return rc;
-    max_abs_diff<<<1, 1024, 0, s>>>(f->x2, f->x1, n, (double *)ctx->scratch);
-    MPRGP_HIP(hipGetLastError());
+    if (int rc = ctx->scratch_reserve(sizeof(double) * (kMaxParts + 1))) return rc;
+    MPRGP_HIP(launch_max_abs_diff(s, f->x2, f->x1, n, (double *)ctx->scratch));
     MPRGP_HIP(hipMemcpyAsync(delta, ctx->scratch, sizeof(double), hipMemcpyDeviceToHost, s));
   }
   // updateMesh: velocities from x2, then x_n = x2
@@ -835,9 +856,8 @@ extern "C" int mprgp_fem_advance(mprgp_fem *f, mprgp_qp *qp, const mprgp_step_pa
     st.iterations += ss.iterations;
     st.last = ss;
     MPRGP_HIP(launch_scatter_nodes(s, qp->x, f->x2, f->A->d_perm, N));
-    if (int rc = ctx->scratch_reserve(256)) return rc;
-    max_abs_diff<<<1, 1024, 0, s>>>(f->x2, f->x1, n, (double *)ctx->scratch);
-    MPRGP_HIP(hipGetLastError());
+    if (int rc = ctx->scratch_reserve(sizeof(double) * (kMaxParts + 1))) return rc;
+    MPRGP_HIP(launch_max_abs_diff(s, f->x2, f->x1, n, (double *)ctx->scratch));
     MPRGP_HIP(hipMemcpyAsync(&st.delta, ctx->scratch, sizeof(double), hipMemcpyDeviceToHost, s));
     MPRGP_HIP(hipStreamSynchronize(s));
     std::swap(f->x1, f->x2);  // x1 = the new iterate
